@@ -1,0 +1,191 @@
+"""Benchmark of the alignment core on one MI355X.
+
+One "step" = one batched banded Gotoh + traceback pass (band offset = half =
+b, run_dispatch(Some((b,1,1,b))) semantics of crates/map/src/extend.rs:293)
+over the uneven-length workload: 2,000,000 read-window pairs, reads
+U{50..250}, band U{16,32,48,64}, window = read + 2*band, 1% substitutions
+with A->G weighted 3x, 0.2% indels of 1-5 (seed 0xE5A00003).  Inputs are
+resident in HBM before the timed region.  value = GCUPS over the in-band
+cell census sum_pairs L*(2b+1).
+
+Reported beside it (same JSON line): the plain one-launch-per-call form on the
+10,000-pair 100/164-nt workload, the batched form on that workload, the
+batched EA-Myers filter on 1,000,000 150/214-nt pairs, the integer-VALU
+roofline of the Gotoh kernel, and the CPU restatement of the reference path
+timed on this host's cores on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X integer VALU peak: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+GOTOH_OPS_PER_CELL = 18      # E 3, F 3, subst 1, diag 1, H 3, dirs 2, hdir 3, pack 1, best 1
+MYERS_OPS_PER_WORD_COL = 31  # per 64-bit word and text column, +3 per column
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def band_cells(hb) -> int:
+    m = hb.read_len.astype(np.int64)
+    n = hb.win_len.astype(np.int64)
+    off = hb.offset.astype(np.int64)
+    half = np.maximum(hb.half.astype(np.int64), 0)
+    interior = (off - half >= 0) & (off + m + half <= n)
+    total = int(np.sum(m[interior] * (2 * half[interior] + 1)))
+    from esperanto_amd.batch import band_cells as bc
+    for k in np.nonzero(~interior)[0]:
+        total += bc(int(m[k]), int(n[k]), int(off[k]), int(half[k]))
+    return total
+
+
+def timed_steps(fn_launch, db, steps: int, warmup: int):
+    for _ in range(warmup):
+        fn_launch()
+        db.sync()
+    db.sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn_launch()
+        db.sync()
+        kms.append(db.last_kernel_ms())
+    db.sync()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, float(np.mean(kms))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pairs", type=int, default=2_000_000)
+    ap.add_argument("--myers-pairs", type=int, default=1_000_000)
+    ap.add_argument("--plain-calls", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested); pairs shard independently across GPUs, "
+              f"see the sharding design notes.", file=sys.stderr)
+        return 2
+
+    from esperanto_amd.batch import DeviceBatch, workload_c1, workload_c2, workload_c3
+    from esperanto_amd.extend import ExtendBuffer, band as plain_band
+    from esperanto_amd.records import ExtendParams
+
+    params = ExtendParams()
+    buf = ExtendBuffer(0)
+
+    log(f"generating {args.pairs} uneven pairs")
+    hb = workload_c3(args.pairs)
+    cells = band_cells(hb)
+    log(f"cells={cells} uploading")
+    db = DeviceBatch(hb, buf)
+    wall, kms = timed_steps(lambda: db.launch_gotoh(params, 0), db, args.steps, args.warmup)
+    gcups = cells / wall / 1e9
+    kernel_gcups = cells / (kms / 1e3) / 1e9
+    achieved_tops = cells * GOTOH_OPS_PER_CELL / (kms / 1e3) / 1e12
+    log(f"gotoh batched: {wall*1e3:.2f} ms/step, kernel {kms:.2f} ms, {gcups:.1f} GCUPS")
+    pairs_per_s = hb.npairs / wall
+    db.free()
+
+    # C1 batched and plain forms.
+    c1 = workload_c1(10_000)
+    d1 = DeviceBatch(c1, buf)
+    w1, k1 = timed_steps(lambda: d1.launch_gotoh(params, 0), d1, max(args.steps, 3), 1)
+    c1_batched_pps = c1.npairs / w1
+    d1.free()
+    ncalls = min(args.plain_calls, c1.npairs)
+    pairs = [c1.pair(k) for k in range(ncalls)]
+    plain_band(pairs[0][0], pairs[0][1], params, (32, 32), buf)
+    t0 = time.perf_counter()
+    for r, w in pairs:
+        plain_band(r, w, params, (32, 32), buf)
+    plain_pps = ncalls / (time.perf_counter() - t0)
+    log(f"c1: batched {c1_batched_pps:.0f} pairs/s, plain {plain_pps:.0f} pairs/s")
+
+    # Myers batched.
+    c2 = workload_c2(args.myers_pairs)
+    d2 = DeviceBatch(c2, buf)
+    w2, k2 = timed_steps(lambda: d2.launch_myers(False), d2, max(args.steps, 3), 1)
+    myers_pps = c2.npairs / w2
+    words = 3
+    myers_ops = c2.npairs * 214 * (MYERS_OPS_PER_WORD_COL * words + 3)
+    myers_tops = myers_ops / (k2 / 1e3) / 1e12
+    log(f"myers: {myers_pps/1e6:.1f} M pairs/s (kernel {k2:.2f} ms)")
+    d2.free()
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        probe = workload_c3(2000)
+        t0 = time.perf_counter()
+        oracle.band_batch(probe, params, 0, nthreads=threads)
+        dt = time.perf_counter() - t0
+        per_pair = dt / probe.npairs
+        nsample = int(min(args.pairs, max(2000, args.cpu_seconds / max(per_pair, 1e-9))))
+        sample = workload_c3(nsample)
+        scells = band_cells(sample)
+        log(f"cpu baseline: {nsample} pairs on {threads} threads")
+        t0 = time.perf_counter()
+        oracle.band_batch(sample, params, 0, nthreads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(scells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads,
+               "kind": "port",
+               "sample": f"first {nsample} pairs of the same uneven workload (seed 0xE5A00003), "
+                         f"scalar C restatement of run_dispatch/run_band/finish_extension, "
+                         f"{nsample / dt:.0f} pairs/s"}
+
+    out = {
+        "metric": "Gotoh GCUPS + Myers pairs/s, batched vs plain on 1 MI355X; p99 latency",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "batched uneven banded Gotoh+traceback: 2,000,000 pairs, reads "
+                               "U{50..250}, band U{16,32,48,64}, window=read+2*band",
+                   "pairs": hb.npairs, "cells": cells, "kernel": "k_gotoh_strip",
+                   "pairs_per_s": round(pairs_per_s),
+                   "kernel_gcups": round(kernel_gcups, 2),
+                   "c1_batched_pairs_per_s": round(c1_batched_pps),
+                   "c1_plain_pairs_per_s": round(plain_pps),
+                   "batched_over_plain": round(c1_batched_pps / plain_pps, 1),
+                   "myers_pairs_per_s": round(myers_pps),
+                   "myers_kernel_ms": round(k2, 3),
+                   "myers_valu_tops": round(myers_tops, 2)},
+        "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
+                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+                     "frac": round(achieved_tops / VALU_PEAK_TOPS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    buf.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

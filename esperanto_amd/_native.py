@@ -1,0 +1,94 @@
+"""ctypes binding of the in-tree HIP library esperanto_amd/lib/libesp_align.so.
+
+There is no CPU fallback: if the library is missing, or no GPU is visible,
+every entry point raises.  Symbol list = include/esp_align.h.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .records import CExt, CParams, CTouch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libesp_align.so")
+
+ESP_OK, ESP_E_INVAL, ESP_E_CAP, ESP_E_DEVICE, ESP_E_NOMEM, ESP_E_BUSY = range(6)
+_STATUS = {1: "invalid argument", 2: "capacity", 3: "device error", 4: "out of memory",
+           5: "busy"}
+
+EXPORTS = [
+    "esp_create", "esp_destroy", "esp_last_error", "esp_sync", "esp_extend", "esp_extend_hint",
+    "esp_band", "esp_myers_best_end", "esp_myers_best_start", "esp_cigar_capacity",
+    "esp_gotoh_batch", "esp_myers_batch", "esp_last_kernel_ms", "esp_device_alloc",
+    "esp_device_free", "esp_copy_to_device", "esp_copy_to_host", "esp_stitch_cigar",
+]
+
+
+class EspError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"esp status {code} ({_STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class CBatch(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_size_t), ("reads", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
+                ("wins", ctypes.c_void_p), ("win_off", ctypes.c_void_p),
+                ("win_len", ctypes.c_void_p), ("offset", ctypes.c_void_p),
+                ("half", ctypes.c_void_p)]
+
+
+class CBatchOut(ctypes.Structure):
+    _fields_ = [("ext", ctypes.c_void_p), ("touch", ctypes.c_void_p), ("cigar", ctypes.c_void_p),
+                ("cig_off", ctypes.c_void_p), ("cig_n", ctypes.c_void_p), ("aux", ctypes.c_void_p)]
+
+
+class CHostMeta(ctypes.Structure):
+    _fields_ = [("read_len", ctypes.c_void_p), ("win_len", ctypes.c_void_p),
+                ("offset", ctypes.c_void_p), ("half", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH}; run python -m esperanto_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp, sz, u8p, u32p = ctypes.c_void_p, ctypes.c_size_t, P(ctypes.c_uint8), P(ctypes.c_uint32)
+    L.esp_create.argtypes = [ctypes.c_int, P(vp)]
+    L.esp_destroy.argtypes = [vp]
+    L.esp_last_error.argtypes = [vp]
+    L.esp_last_error.restype = ctypes.c_char_p
+    L.esp_sync.argtypes = [vp]
+    L.esp_extend.argtypes = [vp, u8p, sz, u8p, sz, P(CParams), P(CExt), u32p, sz, P(sz)]
+    L.esp_extend_hint.argtypes = [vp, u8p, sz, u8p, sz, P(CParams), ctypes.c_int64, P(CExt), u32p,
+                                  sz, P(sz)]
+    L.esp_band.argtypes = [vp, u8p, sz, u8p, sz, P(CParams), ctypes.c_int, ctypes.c_int64,
+                           ctypes.c_int64, P(CExt), P(CTouch), u32p, sz, P(sz)]
+    L.esp_myers_best_end.argtypes = [vp, u8p, sz, u8p, sz, P(ctypes.c_int32), P(sz)]
+    L.esp_myers_best_start.argtypes = [vp, u8p, sz, u8p, sz, P(ctypes.c_int32), P(sz)]
+    L.esp_cigar_capacity.argtypes = [sz]
+    L.esp_cigar_capacity.restype = sz
+    L.esp_gotoh_batch.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), ctypes.c_int,
+                                  P(CBatchOut)]
+    L.esp_myers_batch.argtypes = [vp, P(CBatch), P(CHostMeta), vp, vp, vp]
+    L.esp_last_kernel_ms.argtypes = [vp, P(ctypes.c_double)]
+    L.esp_device_alloc.argtypes = [vp, sz, P(vp)]
+    L.esp_device_free.argtypes = [vp, vp]
+    L.esp_copy_to_device.argtypes = [vp, vp, vp, sz]
+    L.esp_copy_to_host.argtypes = [vp, vp, vp, sz]
+    L.esp_stitch_cigar.argtypes = [u32p, sz, ctypes.c_uint32, u32p, sz, u32p, sz, u32p, sz, P(sz)]
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != ESP_OK:
+        msg = lib().esp_last_error(ctx).decode() if ctx else ""
+        raise EspError(rc, msg)

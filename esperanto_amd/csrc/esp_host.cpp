@@ -1,0 +1,606 @@
+// esp_host.cpp -- host runtime behind the C ABI (include/esp_align.h).
+//
+// A context owns one HIP stream (default priority), grow-only device scratch
+// and pinned staging for the single-pair calls.  Batched calls are
+// asynchronous on the context stream; single-pair calls stage the pair,
+// launch one kernel and copy the result back (the plain form).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/esp_align.h"
+#include "esp_kernels.hpp"
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t grow = std::max(need, (size_t)4096);
+        hipError_t e = hipMalloc(&p, grow);
+        if (e == hipSuccess) bytes = grow;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t grow = std::max(need, (size_t)4096);
+        hipError_t e = hipHostMalloc(&p, grow, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = grow;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+constexpr size_t kTraceBudget = (size_t)8 << 30;  // bytes of trace scratch per launch
+constexpr size_t kRowBudget = (size_t)2 << 30;
+
+}  // namespace
+
+struct esp_ctx {
+    int device = 0;
+    int ncu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    DevBuf counter, rows, trace, stage, stage_out;
+    HostBuf hstage, hout;
+    double last_ms = 0.0;
+    int timed = 0;
+};
+
+namespace {
+
+int fail(esp_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(esp_ctx *c, hipError_t e, const char *where) {
+    return fail(c, ESP_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+// Scoring envelope (include/esp_align.h).  Inside it the reference's i16 and
+// i32 accumulators agree and the int32 device datapath cannot overflow.
+int check_params(esp_ctx *c, const esp_params *p) {
+    if (!p) return fail(c, ESP_E_INVAL, "params is NULL");
+    if (p->gap_open < 0 || p->gap_ext < 0 || p->gap_open + p->gap_ext > 16384)
+        return fail(c, ESP_E_INVAL, "gap penalties outside [0, 16384]");
+    if (p->match_score < -1024 || p->match_score > 1024)
+        return fail(c, ESP_E_INVAL, "match_score outside [-1024, 1024]");
+    if (p->mismatch < -16384 || p->mismatch > 16384)
+        return fail(c, ESP_E_INVAL, "mismatch outside [-16384, 16384]");
+    if (p->edit_dir < 0 || p->edit_dir > 2) return fail(c, ESP_E_INVAL, "edit_dir not in {0,1,2}");
+    return ESP_OK;
+}
+
+espd::GParams dev_params(const esp_params *p) {
+    espd::GParams g;
+    g.match = p->match_score;
+    g.mismatch = p->mismatch;
+    g.oe = -(p->gap_open + p->gap_ext);
+    g.ext = -p->gap_ext;
+    g.edit = p->edit_dir;
+    return g;
+}
+
+constexpr uint32_t kMaxRead = 16000;
+constexpr uint64_t kMaxWin = (uint64_t)1 << 24;
+
+int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
+    if (m > kMaxRead) return fail(c, ESP_E_INVAL, "read longer than 16000");
+    if (n > kMaxWin) return fail(c, ESP_E_INVAL, "window longer than 1<<24");
+    return ESP_OK;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Launch geometry of the persistent Gotoh grid for a batch whose largest
+// pair has maxm rows and maxw trace words.
+int gotoh_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t maxm, uint64_t maxw) {
+    const int per_cu = espk::gotoh_strip_blocks_per_cu(a.P.edit);
+    uint64_t lanes = (uint64_t)c->ncu * per_cu * espk::GOTOH_BLOCK;
+    const uint64_t need_pairs = align_up(a.npairs, 64);
+    lanes = std::min<uint64_t>(lanes, need_pairs);
+    const uint64_t tw = std::max<uint64_t>(maxw, 1), rw = 5 * std::max<uint64_t>(maxm, 1);
+    lanes = std::min<uint64_t>(lanes, kTraceBudget / (tw * 4));
+    lanes = std::min<uint64_t>(lanes, kRowBudget / (rw * 4));
+    lanes = std::max<uint64_t>(lanes / 64 * 64, 64);
+    a.nl = (uint32_t)lanes;
+    a.maxm = (uint32_t)std::max<uint64_t>(maxm, 1);
+    a.maxw = (uint32_t)tw;
+    HIP_TRY(c, c->counter.ensure(256));
+    HIP_TRY(c, c->rows.ensure(rw * lanes * 4));
+    HIP_TRY(c, c->trace.ensure(tw * lanes * 4));
+    a.counter = (uint32_t *)c->counter.p;
+    a.rows = (int32_t *)c->rows.p;
+    a.trace = (uint32_t *)c->trace.p;
+    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
+    if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, espk::launch_gotoh_strip(a, (int)(lanes / espk::GOTOH_BLOCK), c->stream));
+    if (c->timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
+// Single-pair staging layout (host pinned -> device), 16-byte aligned fields.
+struct PairStage {
+    size_t o_read, o_win, o_meta, total;
+};
+
+PairStage pair_layout(size_t m, size_t n) {
+    PairStage s;
+    s.o_read = 0;
+    s.o_win = align_up(m, 16);
+    s.o_meta = s.o_win + align_up(n, 16);
+    // meta: read_off, win_off (u64), read_len, win_len (u32, padded), offset,
+    // half (i64), cig_off[2] (u64)
+    s.total = s.o_meta + 8 * 8;
+    return s;
+}
+
+struct PairOutLayout {
+    size_t o_ext, o_touch, o_n, o_aux, o_cig, total;
+};
+
+PairOutLayout out_layout(size_t cap) {
+    PairOutLayout o;
+    o.o_ext = 0;
+    o.o_touch = 16;
+    o.o_n = 48;
+    o.o_aux = 52;
+    o.o_cig = 64;
+    o.total = o.o_cig + cap * 4;
+    return o;
+}
+
+int run_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+               const esp_params *p, int mode, int64_t offset, int64_t half, esp_ext *out,
+               esp_touch *touch, uint32_t *cigar, size_t cap, size_t *n_ops, int32_t *aux_out) {
+    if (!c) return ESP_E_INVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if ((m && !read) || (n && !win) || !out || !n_ops) return fail(c, ESP_E_INVAL, "NULL argument");
+    rc = check_lengths(c, m, n);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t slot_cap = esp_cigar_capacity(m);
+    PairStage L = pair_layout(m, n);
+    PairOutLayout O = out_layout(slot_cap);
+    HIP_TRY(c, c->hstage.ensure(L.total));
+    HIP_TRY(c, c->stage.ensure(L.total));
+    HIP_TRY(c, c->hout.ensure(O.total));
+    HIP_TRY(c, c->stage_out.ensure(O.total));
+    uint8_t *h = (uint8_t *)c->hstage.p;
+    if (m) std::memcpy(h + L.o_read, read, m);
+    if (n) std::memcpy(h + L.o_win, win, n);
+    uint64_t *meta = (uint64_t *)(h + L.o_meta);
+    meta[0] = 0;  // read_off
+    meta[1] = 0;  // win_off
+    uint32_t *lens = (uint32_t *)(meta + 2);
+    lens[0] = (uint32_t)m;
+    lens[1] = (uint32_t)n;
+    int64_t *oh = (int64_t *)(meta + 3);
+    oh[0] = offset;
+    oh[1] = half;
+    uint64_t *co = meta + 5;
+    co[0] = 0;
+    co[1] = slot_cap;
+    HIP_TRY(c, hipMemcpyAsync(c->stage.p, h, L.total, hipMemcpyHostToDevice, c->stream));
+    uint8_t *d = (uint8_t *)c->stage.p;
+    uint8_t *dout = (uint8_t *)c->stage_out.p;
+    espk::GotohArgs a{};
+    a.npairs = 1;
+    a.order = nullptr;
+    a.reads = d + L.o_read;
+    a.wins = d + L.o_win;
+    a.read_off = (const uint64_t *)(d + L.o_meta);
+    a.win_off = (const uint64_t *)(d + L.o_meta) + 1;
+    a.read_len = (const uint32_t *)(d + L.o_meta + 16);
+    a.win_len = (const uint32_t *)(d + L.o_meta + 20);
+    a.offset = (const int64_t *)(d + L.o_meta + 24);
+    a.half = (const int64_t *)(d + L.o_meta + 32);
+    a.cig_off = (const uint64_t *)(d + L.o_meta + 40);
+    a.mode = mode;
+    a.P = dev_params(p);
+    a.ext = (esp_ext *)(dout + O.o_ext);
+    a.touch = (esp_touch *)(dout + O.o_touch);
+    a.cig_n = (uint32_t *)(dout + O.o_n);
+    a.aux = (int32_t *)(dout + O.o_aux);
+    a.cigar = (uint32_t *)(dout + O.o_cig);
+    HIP_TRY(c, c->counter.ensure(256));
+    a.counter = (uint32_t *)c->counter.p;
+    const uint64_t maxw = espk::strip_trace_words(m, n);
+    HIP_TRY(c, c->rows.ensure(5 * std::max<size_t>(m, 1) * 64 * 4));
+    HIP_TRY(c, c->trace.ensure(std::max<uint64_t>(maxw, 1) * 64 * 4));
+    a.rows = (int32_t *)c->rows.p;
+    a.trace = (uint32_t *)c->trace.p;
+    a.maxm = (uint32_t)std::max<size_t>(m, 1);
+    a.maxw = (uint32_t)std::max<uint64_t>(maxw, 1);
+    a.nl = 64;
+    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
+    HIP_TRY(c, espk::launch_gotoh_strip(a, 1, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->hout.p, c->stage_out.p, O.total, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint8_t *ho = (const uint8_t *)c->hout.p;
+    std::memcpy(out, ho + O.o_ext, sizeof(esp_ext));
+    if (touch) std::memcpy(touch, ho + O.o_touch, sizeof(esp_touch));
+    if (aux_out) std::memcpy(aux_out, ho + O.o_aux, sizeof(int32_t));
+    uint32_t nops;
+    std::memcpy(&nops, ho + O.o_n, 4);
+    *n_ops = nops;
+    if (nops > cap || (nops && !cigar)) return fail(c, ESP_E_CAP, "CIGAR capacity too small");
+    if (nops) std::memcpy(cigar, ho + O.o_cig, nops * 4);
+    return ESP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t esp_cigar_capacity(size_t m) { return 2 * m + 5; }
+
+int esp_create(int device, esp_ctx **out) {
+    if (!out) return ESP_E_INVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return ESP_E_DEVICE;
+    esp_ctx *c = new esp_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return ESP_E_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return ESP_E_DEVICE;
+    }
+    c->ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return ESP_E_DEVICE;
+    }
+    *out = c;
+    return ESP_OK;
+}
+
+int esp_destroy(esp_ctx *c) {
+    if (!c) return ESP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->counter.release();
+    c->rows.release();
+    c->trace.release();
+    c->stage.release();
+    c->stage_out.release();
+    c->hstage.release();
+    c->hout.release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ESP_OK;
+}
+
+const char *esp_last_error(const esp_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int esp_sync(esp_ctx *c) {
+    if (!c) return ESP_E_INVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ESP_OK;
+}
+
+int esp_extend(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+               const esp_params *p, esp_ext *out, uint32_t *cigar, size_t cap, size_t *n_ops) {
+    return run_single(c, read, m, win, n, p, ESP_MODE_FULL, 0, 0, out, nullptr, cigar, cap, n_ops,
+                      nullptr);
+}
+
+int esp_extend_hint(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+                    const esp_params *p, int64_t hint_offset, esp_ext *out, uint32_t *cigar,
+                    size_t cap, size_t *n_ops) {
+    return run_single(c, read, m, win, n, p, ESP_MODE_HINT, hint_offset, 0, out, nullptr, cigar,
+                      cap, n_ops, nullptr);
+}
+
+int esp_band(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+             const esp_params *p, int has_band, int64_t offset, int64_t half, esp_ext *out,
+             esp_touch *touch, uint32_t *cigar, size_t cap, size_t *n_ops) {
+    return run_single(c, read, m, win, n, p, has_band ? ESP_MODE_BAND : ESP_MODE_FULL, offset,
+                      half, out, touch, cigar, cap, n_ops, nullptr);
+}
+
+int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
+                    const esp_params *p, int mode, esp_batch_out *o) {
+    if (!c) return ESP_E_INVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!b || !o || !meta || !meta->read_len || !meta->win_len)
+        return fail(c, ESP_E_INVAL, "batch, output and host lengths are required");
+    if (mode < ESP_MODE_BAND || mode > ESP_MODE_HINT) return fail(c, ESP_E_INVAL, "bad mode");
+    if (b->npairs == 0) return ESP_OK;
+    if (b->npairs > 0xFFFFFF00ull) return fail(c, ESP_E_INVAL, "too many pairs");
+    if ((mode != ESP_MODE_FULL && !b->offset) || (mode == ESP_MODE_BAND && !b->half))
+        return fail(c, ESP_E_INVAL, "offset/half arrays required for this mode");
+    uint64_t maxm = 0, maxw = 0;
+    for (size_t k = 0; k < b->npairs; k++) {
+        const uint64_t m = meta->read_len[k], n = meta->win_len[k];
+        rc = check_lengths(c, m, n);
+        if (rc) return rc;
+        maxm = std::max(maxm, m);
+        maxw = std::max(maxw, espk::strip_trace_words(m, n));
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    espk::GotohArgs a{};
+    a.npairs = (uint32_t)b->npairs;
+    a.order = nullptr;
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.wins = b->wins;
+    a.win_off = b->win_off;
+    a.win_len = b->win_len;
+    a.offset = b->offset;
+    a.half = b->half;
+    a.mode = mode;
+    a.P = dev_params(p);
+    a.ext = o->ext;
+    a.touch = o->touch;
+    a.cigar = o->cigar;
+    a.cig_off = o->cig_off;
+    a.cig_n = o->cig_n;
+    a.aux = o->aux;
+    c->timed = 1;
+    rc = gotoh_launch(c, a, maxm, maxw);
+    c->timed = 0;
+    return rc;
+}
+
+int esp_last_kernel_ms(esp_ctx *c, double *ms) {
+    if (!c || !ms) return ESP_E_INVAL;
+    float f = 0.f;
+    HIP_TRY(c, hipEventSynchronize(c->ev1));
+    HIP_TRY(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return ESP_OK;
+}
+
+int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, int32_t *dist,
+                    uint64_t *end, uint64_t *start) {
+    if (!c) return ESP_E_INVAL;
+    if (!b || !meta || !meta->read_len || !dist || !end)
+        return fail(c, ESP_E_INVAL, "batch, host lengths and outputs are required");
+    if (b->npairs == 0) return ESP_OK;
+    uint32_t maxm = 0;
+    for (size_t k = 0; k < b->npairs; k++) {
+        const uint32_t m = meta->read_len[k];
+        if (m <= 256) maxm = std::max(maxm, m);
+        if (meta->win_len && meta->win_len[k] > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
+    }
+    const int words = std::max(1, (int)((maxm + 63) / 64));
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->counter.ensure(256));
+    espk::MyersArgs a{};
+    a.npairs = (uint32_t)b->npairs;
+    a.order = nullptr;
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.texts = b->wins;
+    a.text_off = b->win_off;
+    a.text_len = b->win_len;
+    a.dist = dist;
+    a.end = end;
+    a.start = start;
+    a.counter = (uint32_t *)c->counter.p;
+    const int per_cu = espk::myers_blocks_per_cu(words);
+    uint64_t blocks = (uint64_t)c->ncu * per_cu;
+    blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
+    blocks = std::max<uint64_t>(blocks, 1);
+    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, espk::launch_myers(a, words, (int)blocks, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
+static int myers_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *text, size_t n,
+                        int32_t *dist, size_t *end, size_t *start) {
+    if (!c) return ESP_E_INVAL;
+    if ((m && !read) || (n && !text) || !dist) return fail(c, ESP_E_INVAL, "NULL argument");
+    if (n > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
+    HIP_TRY(c, hipSetDevice(c->device));
+    PairStage L = pair_layout(m, n);
+    HIP_TRY(c, c->hstage.ensure(L.total));
+    HIP_TRY(c, c->stage.ensure(L.total));
+    HIP_TRY(c, c->hout.ensure(64));
+    HIP_TRY(c, c->stage_out.ensure(64));
+    uint8_t *h = (uint8_t *)c->hstage.p;
+    if (m) std::memcpy(h + L.o_read, read, m);
+    if (n) std::memcpy(h + L.o_win, text, n);
+    uint64_t *meta = (uint64_t *)(h + L.o_meta);
+    meta[0] = 0;
+    meta[1] = 0;
+    uint32_t *lens = (uint32_t *)(meta + 2);
+    lens[0] = (uint32_t)m;
+    lens[1] = (uint32_t)n;
+    HIP_TRY(c, hipMemcpyAsync(c->stage.p, h, L.total, hipMemcpyHostToDevice, c->stream));
+    uint8_t *d = (uint8_t *)c->stage.p;
+    uint8_t *dout = (uint8_t *)c->stage_out.p;
+    HIP_TRY(c, c->counter.ensure(256));
+    espk::MyersArgs a{};
+    a.npairs = 1;
+    a.reads = d + L.o_read;
+    a.texts = d + L.o_win;
+    a.read_off = (const uint64_t *)(d + L.o_meta);
+    a.text_off = (const uint64_t *)(d + L.o_meta) + 1;
+    a.read_len = (const uint32_t *)(d + L.o_meta + 16);
+    a.text_len = (const uint32_t *)(d + L.o_meta + 20);
+    a.dist = (int32_t *)dout;
+    a.end = (uint64_t *)(dout + 8);
+    a.start = start ? (uint64_t *)(dout + 16) : nullptr;
+    a.counter = (uint32_t *)c->counter.p;
+    const int words = m > 256 ? 1 : std::max(1, (int)((m + 63) / 64));
+    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
+    HIP_TRY(c, espk::launch_myers(a, words, 1, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->hout.p, dout, 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint8_t *ho = (const uint8_t *)c->hout.p;
+    std::memcpy(dist, ho, 4);
+    uint64_t e, s;
+    std::memcpy(&e, ho + 8, 8);
+    std::memcpy(&s, ho + 16, 8);
+    if (end) *end = (size_t)e;
+    if (start) *start = (size_t)s;
+    return ESP_OK;
+}
+
+int esp_myers_best_end(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *text, size_t n,
+                       int32_t *dist, size_t *end) {
+    if (!end) return fail(c, ESP_E_INVAL, "NULL argument");
+    return myers_single(c, read, m, text, n, dist, end, nullptr);
+}
+
+int esp_myers_best_start(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *text,
+                         size_t n, int32_t *dist, size_t *start) {
+    if (!start) return fail(c, ESP_E_INVAL, "NULL argument");
+    return myers_single(c, read, m, text, n, dist, nullptr, start);
+}
+
+int esp_device_alloc(esp_ctx *c, size_t bytes, void **out) {
+    if (!c || !out) return ESP_E_INVAL;
+    *out = nullptr;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 1));
+    if (e != hipSuccess) return fail(c, ESP_E_NOMEM, hipGetErrorString(e));
+    return ESP_OK;
+}
+
+int esp_device_free(esp_ctx *c, void *p) {
+    if (!c) return ESP_E_INVAL;
+    if (!p) return ESP_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipFree(p));
+    return ESP_OK;
+}
+
+int esp_copy_to_device(esp_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return ESP_E_INVAL;
+    if (!bytes) return ESP_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ESP_OK;
+}
+
+int esp_copy_to_host(esp_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return ESP_E_INVAL;
+    if (!bytes) return ESP_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ESP_OK;
+}
+
+// stitch_cigar (splice.rs:435-494): Match/Del split at exon ends, RefSkip of
+// the junction length inserted at each crossing, Ins/SoftClip/RefSkip pass
+// through; runs of one kind merge as push_op does (extend.rs:254-282).
+int esp_stitch_cigar(const uint32_t *cigar, size_t n_ops, uint32_t p_start,
+                     const uint32_t *bounds, size_t n_exons, const uint32_t *skips,
+                     size_t n_junctions, uint32_t *out, size_t cap, size_t *n_out) {
+    if ((n_ops && !cigar) || !n_out || (n_exons && !bounds) || (n_junctions && !skips))
+        return ESP_E_INVAL;
+    std::vector<uint32_t> v;
+    v.reserve(n_ops + n_exons + 2);
+    auto push = [&](uint32_t kind, uint64_t len) {
+        if (!v.empty() && (v.back() & 0xFu) == kind) {
+            v.back() = ((uint32_t)((v.back() >> 4) + len) << 4) | kind;
+        } else {
+            v.push_back(((uint32_t)len << 4) | kind);
+        }
+    };
+    if (n_exons == 0) {
+        v.assign(cigar, cigar + n_ops);
+    } else {
+        auto exon_end = [&](size_t ei) -> uint64_t {
+            return (uint64_t)bounds[3 * ei] + (uint64_t)(bounds[3 * ei + 2] - bounds[3 * ei + 1]);
+        };
+        uint64_t p = p_start;
+        size_t ei = 0;
+        while (ei + 1 < n_exons && p >= exon_end(ei)) ei++;
+        for (size_t k = 0; k < n_ops; k++) {
+            const uint32_t kind = cigar[k] & 0xFu;
+            const uint32_t len = cigar[k] >> 4;
+            if (kind != espd::OP_M && kind != espd::OP_D) {
+                push(kind, len);
+                continue;
+            }
+            uint64_t remaining = len;
+            while (remaining > 0) {
+                if (ei >= n_exons) {
+                    push(kind, remaining);
+                    p += remaining;
+                    break;
+                }
+                if (p < bounds[3 * ei]) p = bounds[3 * ei];
+                const uint64_t ee = exon_end(ei);
+                const uint64_t in_exon = ee > p ? ee - p : 0;
+                const uint64_t take = std::min(remaining, in_exon);
+                if (take > 0) push(kind, take);
+                p += take;
+                remaining -= take;
+                if (remaining > 0) {
+                    if (ei + 1 >= n_exons) {
+                        push(kind, remaining);
+                        p += remaining;
+                        break;
+                    }
+                    if (ei < n_junctions) push(espd::OP_N, skips[ei]);
+                    ei++;
+                }
+            }
+        }
+    }
+    *n_out = v.size();
+    if (v.size() > cap || (!v.empty() && !out)) return ESP_E_CAP;
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 4);
+    return ESP_OK;
+}
+
+}  // extern "C"

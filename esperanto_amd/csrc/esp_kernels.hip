@@ -1,0 +1,586 @@
+// esp_kernels.hip -- hand-written gfx950 kernels of the alignment core.
+//
+// Gotoh (general form): one lane owns one read-window pair and sweeps the
+// DP in vertical strips of STRIP_T columns.  The strip's previous-row H/E and
+// window codes live in VGPRs (static indices after unrolling), so a cell
+// costs only VALU work; the strip-to-strip hand-off (H and F of the strip's
+// last column per row) and the per-row maximum go through lane-interleaved
+// global scratch, which makes every such access a coalesced 256-byte wave
+// transaction.  Cells outside a row's band window take the reference's
+// margin values (H = 0, E = F = NEG), which is exactly what the reference
+// reads out of window (extend.rs:718-759), so banded, clamped and
+// full-matrix calls share one code path.
+//
+// The grid is persistent: each wave pops 64 pairs from a device work queue
+// until it is empty, so every wave reaches the exit.
+#include "esp_kernels.hpp"
+
+using namespace espd;
+
+namespace espk {
+
+namespace {
+
+struct PairView {
+    const uint8_t *read;
+    const uint8_t *win;
+    int32_t m, n, off, half;
+};
+
+__device__ __forceinline__ int32_t clamp_off(int64_t off, int32_t m, int32_t n) {
+    int64_t lo = -(int64_t)m - 1, hi = n;
+    return (int32_t)(off < lo ? lo : (off > hi ? hi : off));
+}
+__device__ __forceinline__ int32_t clamp_half(int64_t half, int32_t n) {
+    return (int32_t)(half < 0 ? 0 : (half > n ? n : half));
+}
+
+struct FillResult {
+    int32_t best, bi, bj;
+    int32_t touched, trow, tleft, tclamped;
+};
+
+// Lane-interleaved scratch accessors.
+struct Slot {
+    int32_t *rows;
+    uint32_t *trace;
+    uint32_t maxm, nl, slot;
+    __device__ __forceinline__ int32_t *row(int arr, int32_t i) const {
+        return rows + ((size_t)arr * maxm + (size_t)(i - 1)) * nl + slot;
+    }
+    __device__ __forceinline__ uint32_t *tword(size_t w) const { return trace + w * nl + slot; }
+};
+
+enum { R_BNDH = 0, R_BNDF = 1, R_MAX = 2, R_HL = 3, R_HR = 4 };
+
+// Fills the DP of one pair strip by strip.  Returns the best cell (first
+// row-major strict maximum, extend.rs:789-792, :823-827) and the first touch
+// (extend.rs:803-821).
+template <int T, bool EDIT>
+__device__ FillResult fill_strips(const PairView &pv, const GParams &P, const Slot &S) {
+    const int32_t m = pv.m, n = pv.n, off = pv.off, half = pv.half;
+    for (int32_t i = 1; i <= m; i++) {
+        *S.row(R_BNDH, i) = 0;
+        *S.row(R_BNDF, i) = NEG;
+        *S.row(R_MAX, i) = -1;
+    }
+    int32_t bestv = 0, besti = 0, bestj = 0;
+    const int32_t nstrips = (n + T - 1) / T;
+    int32_t ia = 1;
+    for (int32_t s = 0; s < nstrips; s++) {
+        const int32_t J0 = 1 + s * T, J1 = J0 + T - 1;
+        int32_t lo, hi;
+        // First row reaching this strip (hi non-decreasing in i).
+        while (ia <= m) {
+            row_window(off, half, n, ia, lo, hi);
+            if (hi >= J0) break;
+            ia++;
+        }
+        if (ia > m) break;
+        int32_t fc[T], Hr[T], Er[T];
+#pragma unroll
+        for (int c = 0; c < T; c++) {
+            int32_t j = J0 + c;
+            fc[c] = j <= n ? win_code(pv.win[j - 1]) : 7;
+            Hr[c] = 0;
+            Er[c] = NEG;
+        }
+        // H(ia-1, J0-1) inside row ia-1's window, else the margin 0.
+        int32_t Hd = 0;
+        if (ia > 1) {
+            int32_t plo, phi;
+            row_window(off, half, n, ia - 1, plo, phi);
+            if (plo <= J0 - 1 && J0 - 1 <= phi) Hd = *S.row(R_BNDH, ia - 1);
+        }
+        for (int32_t i = ia; i <= m; i++) {
+            row_window(off, half, n, i, lo, hi);
+            if (lo > J1) break;
+            const int32_t rc = read_code(pv.read[i - 1]);
+            const int32_t partner = EDIT ? exempt_partner(rc, P.edit) : 6;
+            const bool lin = lo <= J0 - 1 && J0 - 1 <= hi;
+            const int32_t Hl = lin ? *S.row(R_BNDH, i) : 0;
+            const int32_t Fl = lin ? *S.row(R_BNDF, i) : NEG;
+            const int32_t clo = lo - J0, chi = hi - J0;
+            int32_t hdiag = Hd, hleft = Hl, fleft = Fl;
+            int32_t rmax = -1, rarg = 0, hlo = -1, hhi = -1;
+            uint32_t word = 0;
+            const size_t wbase = ((size_t)s * m + (size_t)(i - 1)) * (T / 8);
+#pragma unroll
+            for (int c = 0; c < T; c++) {
+                const int32_t hup = Hr[c], eup = Er[c];
+                const int32_t eo = hup + P.oe, ee = eup + P.ext;
+                const int32_t E = max(eo, ee);
+                const uint32_t edir = ee > eo;
+                const int32_t fo = hleft + P.oe, fe = fleft + P.ext;
+                const int32_t F = max(fo, fe);
+                const uint32_t fdir = fe > fo;
+                int32_t sc = rc == fc[c] ? P.match : P.mismatch;
+                if (EDIT) sc = fc[c] == partner ? 0 : sc;
+                const int32_t D = hdiag + sc;
+                const int32_t H = max(max(D, E), max(F, 0));
+                const uint32_t hd = H == 0 ? HD_START : (D == H ? HD_DIAG : (E == H ? HD_E : HD_F));
+                const bool valid = c >= clo && c <= chi;
+                const int32_t Hv = valid ? H : 0;
+                const int32_t Ev = valid ? E : NEG;
+                const int32_t Fv = valid ? F : NEG;
+                hdiag = hup;
+                Hr[c] = Hv;
+                Er[c] = Ev;
+                hleft = Hv;
+                fleft = Fv;
+                word |= (hd | (edir << 2) | (fdir << 3)) << (4 * (c & 7));
+                if ((c & 7) == 7) {
+                    *S.tword(wbase + (c >> 3)) = word;
+                    word = 0;
+                }
+                if (Hv > rmax) {
+                    rmax = Hv;
+                    rarg = c;
+                }
+                hlo = c == clo ? Hv : hlo;
+                hhi = c == chi ? Hv : hhi;
+            }
+            // Hand-off to the next strip: H and F of column J1 (margins when
+            // J1 lies outside the row window).
+            *S.row(R_BNDH, i) = hleft;
+            *S.row(R_BNDF, i) = fleft;
+            Hd = Hl;
+            int32_t *rm = S.row(R_MAX, i);
+            if (rmax > *rm) *rm = rmax;
+            if (clo >= 0 && clo < T) *S.row(R_HL, i) = hlo;
+            if (chi >= 0 && chi < T) *S.row(R_HR, i) = hhi;
+            if (rmax > bestv || (rmax == bestv && rmax > 0 && i < besti)) {
+                bestv = rmax;
+                besti = i;
+                bestj = J0 + rarg;
+            }
+        }
+    }
+    FillResult r;
+    r.best = bestv;
+    r.bi = besti;
+    r.bj = bestj;
+    r.touched = 0;
+    r.trow = 0;
+    r.tleft = 0;
+    r.tclamped = 0;
+    for (int32_t i = 1; i <= m; i++) {
+        int32_t lo, hi;
+        row_window(off, half, n, i, lo, hi);
+        const int32_t rm = *S.row(R_MAX, i);
+        if (*S.row(R_HL, i) == rm) {
+            r.touched = 1;
+            r.trow = i;
+            r.tleft = 1;
+            r.tclamped = lo <= 1;
+            break;
+        }
+        if (hi < n && *S.row(R_HR, i) == rm) {
+            r.touched = 1;
+            r.trow = i;
+            r.tleft = 0;
+            r.tclamped = 0;
+            break;
+        }
+    }
+    return r;
+}
+
+struct TbResult {
+    uint32_t read_start, ref_start, nm, ni, nd, nrev;
+};
+
+// State-machine traceback (extend.rs:884-930) over the strip trace layout.
+// Runs are written in walk order to slot[cap-1], slot[cap-2], ...
+template <int T>
+__device__ TbResult traceback_strips(const PairView &pv, const Slot &S, int32_t bi, int32_t bj,
+                                     uint32_t *slot, uint32_t cap) {
+    const int32_t m = pv.m, n = pv.n;
+    int32_t i = bi, j = bj;
+    uint32_t state = 0;  // 0 H, 1 E, 2 F
+    uint32_t cur_op = 0xFF, cur_len = 0, nrev = 0, nm = 0, ni = 0, nd = 0;
+    auto emit = [&](uint32_t op) {
+        if (op == cur_op) {
+            cur_len++;
+        } else {
+            if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
+            cur_op = op;
+            cur_len = 1;
+        }
+    };
+    while (i > 0 && j > 0) {
+        int32_t lo, hi;
+        row_window(pv.off, pv.half, n, i, lo, hi);
+        if (j < lo || j > hi) break;
+        const int32_t s = (j - 1) / T, c = (j - 1) % T;
+        const size_t w = ((size_t)s * m + (size_t)(i - 1)) * (T / 8) + (c >> 3);
+        const uint32_t nib = (*S.tword(w) >> (4 * (c & 7))) & 0xFu;
+        if (state == 0) {
+            const uint32_t hd = nib & 3u;
+            if (hd == HD_DIAG) {
+                emit(OP_M);
+                nm++;
+                i--;
+                j--;
+            } else if (hd == HD_E) {
+                emit(OP_I);
+                ni++;
+                state = (nib >> 2) & 1u ? 1u : 0u;
+                i--;
+            } else if (hd == HD_F) {
+                emit(OP_D);
+                nd++;
+                state = (nib >> 3) & 1u ? 2u : 0u;
+                j--;
+            } else {
+                break;
+            }
+        } else if (state == 1) {
+            emit(OP_I);
+            ni++;
+            if (((nib >> 2) & 1u) == 0) state = 0;
+            i--;
+        } else {
+            emit(OP_D);
+            nd++;
+            if (((nib >> 3) & 1u) == 0) state = 0;
+            j--;
+        }
+    }
+    if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
+    TbResult t;
+    t.read_start = (uint32_t)max(i, 0);
+    t.ref_start = (uint32_t)max(j, 0);
+    t.nm = nm;
+    t.ni = ni;
+    t.nd = nd;
+    t.nrev = nrev;
+    return t;
+}
+
+// End trimming and score rebuild (extend.rs:932-990) reduce to the identity
+// here: with gap_open, gap_ext >= 0 every prefix of the walked path ends on
+// a cell whose H (or gap value) is positive, and every proper prefix stays
+// below the first row-major maximum, so neither trim fires and the score is
+// the best cell's H.  Neutral flanks and soft clips follow extend.rs:991-1059.
+__device__ void finish_pair(const PairView &pv, const GParams &P, int32_t best,
+                            const TbResult &tb, uint32_t *slot, uint32_t cap, esp_ext *ext,
+                            uint32_t *n_ops) {
+    uint32_t read_start = tb.read_start, ref_start = tb.ref_start;
+    uint32_t read_end = read_start + tb.nm + tb.ni;
+    uint32_t nl = 0, nr = 0;
+    if (best > 0) {
+        int32_t delta = 0;
+        const uint32_t lim = min(read_start, ref_start);
+        for (uint32_t k = 1; k <= lim; k++) {
+            const int32_t q = read_code(pv.read[read_start - k]);
+            const int32_t r = win_code(pv.win[ref_start - k]);
+            if (!neutral_column(q, r)) break;
+            delta += lut_score(P, q, r);
+            if (delta == 0) nl = k;
+        }
+        const uint32_t ref_end = ref_start + tb.nm + tb.nd;
+        const uint32_t avail = min((uint32_t)pv.m - read_end, (uint32_t)pv.n - ref_end);
+        delta = 0;
+        for (uint32_t k = 0; k < avail; k++) {
+            const int32_t q = read_code(pv.read[read_end + k]);
+            const int32_t r = win_code(pv.win[ref_end + k]);
+            if (!neutral_column(q, r)) break;
+            delta += lut_score(P, q, r);
+            if (delta == 0) nr = k + 1;
+        }
+        read_start -= nl;
+        ref_start -= nl;
+        read_end += nr;
+    }
+    *n_ops = assemble_cigar(slot, cap, tb.nrev, (uint32_t)pv.m, read_start, nl, read_end, nr);
+    ext->read_start = read_start;
+    ext->read_end = read_end;
+    ext->ref_start = ref_start;
+    ext->score = best;
+}
+
+__device__ __forceinline__ uint32_t wave_pop(uint32_t *counter, uint32_t lane) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0);
+}
+
+template <int T, bool EDIT>
+__device__ void gotoh_one(const GotohArgs &a, uint32_t p, const Slot &S) {
+    const uint32_t m = a.read_len[p], n = a.win_len[p];
+    PairView pv;
+    pv.read = a.reads + a.read_off[p];
+    pv.win = a.wins + a.win_off[p];
+    pv.m = (int32_t)m;
+    pv.n = (int32_t)n;
+    const uint64_t c0 = a.cig_off[p];
+    const uint32_t cap = (uint32_t)(a.cig_off[p + 1] - c0);
+    uint32_t *slot = a.cigar + c0;
+    esp_touch tr;
+    tr.touched = 0;
+    tr.row = 0;
+    tr.left = 0;
+    tr.clamped = 0;
+    int32_t path = 0;
+    if (m == 0 || n == 0) {
+        // empty_extension (extend.rs:430-442)
+        esp_ext e;
+        e.read_start = e.read_end = e.ref_start = 0;
+        e.score = 0;
+        uint32_t k = 0;
+        if (m > 0) slot[k++] = (m << 4) | OP_S;
+        a.ext[p] = e;
+        a.cig_n[p] = k;
+        if (a.touch) a.touch[p] = tr;
+        if (a.aux) a.aux[p] = 2;
+        return;
+    }
+    bool band;
+    if (a.mode == ESP_MODE_FULL) {
+        band = false;
+    } else if (a.mode == ESP_MODE_HINT) {
+        band = 30 < (int64_t)n;  // INITIAL_BAND >= n: full matrix directly
+        path = band ? 0 : 3;
+    } else {
+        band = true;
+    }
+    pv.off = band ? clamp_off(a.offset[p], pv.m, pv.n) : 0;
+    pv.half = band ? clamp_half(a.mode == ESP_MODE_HINT ? 30 : a.half[p], pv.n) : pv.n;
+    FillResult fr = fill_strips<T, EDIT>(pv, a.P, S);
+    if (a.mode == ESP_MODE_HINT && band) {
+        const int32_t perfect = pv.m * a.P.match;
+        if (fr.touched || fr.best * 20 < perfect * 17) {
+            pv.off = 0;
+            pv.half = pv.n;
+            fr = fill_strips<T, EDIT>(pv, a.P, S);
+            path = 1;
+        }
+    }
+    TbResult tb = traceback_strips<T>(pv, S, fr.bi, fr.bj, slot, cap);
+    esp_ext e;
+    uint32_t nops;
+    finish_pair(pv, a.P, fr.best, tb, slot, cap, &e, &nops);
+    a.ext[p] = e;
+    a.cig_n[p] = nops;
+    if (a.touch) {
+        tr.touched = fr.touched;
+        tr.row = fr.trow;
+        tr.left = fr.tleft;
+        tr.clamped = fr.tclamped;
+        a.touch[p] = tr;
+    }
+    if (a.aux) a.aux[p] = path;
+}
+
+template <int T, bool EDIT>
+__global__ __launch_bounds__(GOTOH_BLOCK) void k_gotoh_strip(GotohArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    Slot S;
+    S.rows = a.rows;
+    S.trace = a.trace;
+    S.maxm = a.maxm;
+    S.nl = a.nl;
+    S.slot = blockIdx.x * blockDim.x + threadIdx.x;
+    for (;;) {
+        const uint32_t base = wave_pop(a.counter, lane);
+        if (base >= a.npairs) break;
+        const uint32_t idx = base + lane;
+        if (idx < a.npairs) {
+            const uint32_t p = a.order ? a.order[idx] : idx;
+            gotoh_one<T, EDIT>(a, p, S);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- Myers
+//
+// EA-Myers (myers_ea.rs) with W 64-bit words per lane and an edlib-form
+// carry between words.  The pattern-equality table of each lane lives in
+// LDS as peq[code][word][lane] (codes A, C, G, T and "other", which matches
+// nothing), so a text column costs W LDS reads and ~31 VALU ops per word.
+
+struct Peq {
+    uint64_t *base;  // [5][W][64] for this wave
+    uint32_t lane;
+    int W;
+    __device__ __forceinline__ uint64_t &at(int code, int w) const {
+        return base[((size_t)code * W + w) * 64 + lane];
+    }
+};
+
+// ea_matches(read, ref) (myers_ea.rs:28-38): read A={A}, C={C,T}, G={G,A}, T={T}.
+template <int W>
+__device__ void build_peq(const Peq &pq, const uint8_t *read, int32_t m, bool reverse) {
+    uint64_t a[W], c[W], g[W], t[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) a[w] = c[w] = g[w] = t[w] = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        for (int b = 0; b < 64; b++) {
+            const int32_t i = w * 64 + b;
+            if (i >= m) break;
+            const int32_t rc = read_code(read[reverse ? (m - 1 - i) : i]);
+            const uint64_t bit = 1ull << b;
+            if (rc == 0) { a[w] |= bit; }
+            if (rc == 1) { c[w] |= bit; t[w] |= bit; }
+            if (rc == 2) { g[w] |= bit; a[w] |= bit; }
+            if (rc == 3) { t[w] |= bit; }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        pq.at(0, w) = a[w];
+        pq.at(1, w) = c[w];
+        pq.at(2, w) = g[w];
+        pq.at(3, w) = t[w];
+        pq.at(4, w) = 0;
+    }
+}
+
+// Column scan; returns (best, best_at) with the strict-< earliest-end rule
+// of infix_best_end (myers_ea.rs:141-156).
+template <int W>
+__device__ void myers_scan(const Peq &pq, int32_t m, const uint8_t *text, int32_t n, bool reverse,
+                           int32_t &best_out, uint32_t &at_out) {
+    uint64_t pv[W], mv[W];
+    const int lastw = (m - 1) >> 6;
+    const int lastb = (m - 1) & 63;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        pv[w] = w < lastw ? ~0ull : (w == lastw ? (lastb == 63 ? ~0ull : ((1ull << (lastb + 1)) - 1)) : 0);
+        mv[w] = 0;
+    }
+    int32_t score = m, best = m;
+    uint32_t best_at = 0;
+    for (int32_t j = 0; j < n; j++) {
+        const int32_t code = read_code(text[reverse ? (n - 1 - j) : j]);
+        int32_t hin = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (w <= lastw) {
+                const uint64_t mask = w < lastw ? ~0ull : (lastb == 63 ? ~0ull : ((1ull << (lastb + 1)) - 1));
+                const uint64_t high = w < lastw ? (1ull << 63) : (1ull << lastb);
+                const uint64_t eq_in = pq.at(code, w);
+                const uint64_t hneg = hin < 0 ? 1ull : 0ull;
+                const uint64_t hpos = hin > 0 ? 1ull : 0ull;
+                const uint64_t xv = eq_in | mv[w];
+                const uint64_t eq = eq_in | hneg;
+                const uint64_t xh = (((eq & pv[w]) + pv[w]) ^ pv[w]) | eq;
+                const uint64_t ph = mv[w] | ~(xh | pv[w]);
+                const uint64_t mh = pv[w] & xh;
+                const int32_t hout = (ph & high) ? 1 : ((mh & high) ? -1 : 0);
+                const uint64_t phs = (ph << 1) | hpos;
+                const uint64_t mhs = (mh << 1) | hneg;
+                pv[w] = (mhs | ~(xv | phs)) & mask;
+                mv[w] = (phs & xv) & mask;
+                hin = hout;
+            }
+        }
+        score += hin;
+        if (score < best) {
+            best = score;
+            best_at = (uint32_t)(j + 1);
+        }
+    }
+    best_out = best;
+    at_out = best_at;
+}
+
+template <int W>
+__global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    Peq pq;
+    pq.base = lds + (size_t)wave * 5 * W * 64;
+    pq.lane = lane;
+    pq.W = W;
+    for (;;) {
+        const uint32_t base = wave_pop(a.counter, lane);
+        if (base >= a.npairs) break;
+        const uint32_t idx = base + lane;
+        if (idx < a.npairs) {
+            const uint32_t p = a.order ? a.order[idx] : idx;
+            const int32_t m = (int32_t)a.read_len[p], n = (int32_t)a.text_len[p];
+            const uint8_t *read = a.reads + a.read_off[p];
+            const uint8_t *text = a.texts + a.text_off[p];
+            if (m == 0 || m > 256) {
+                // long::infix_best_end guards (myers_ea.rs:275-280)
+                a.dist[p] = m == 0 ? 0 : m;
+                a.end[p] = 0;
+                if (a.start) a.start[p] = 0;
+            } else {
+                int32_t d;
+                uint32_t e;
+                build_peq<W>(pq, read, m, false);
+                myers_scan<W>(pq, m, text, n, false, d, e);
+                a.dist[p] = d;
+                a.end[p] = e;
+                if (a.start) {
+                    // infix_best_start: best end of the doubly reversed pair
+                    // (myers_ea.rs:166-185); start = n - rev_end.
+                    build_peq<W>(pq, read, m, true);
+                    myers_scan<W>(pq, m, text, n, true, d, e);
+                    a.start[p] = (uint64_t)(n - (int32_t)e);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s) {
+    if (a.P.edit) {
+        hipLaunchKernelGGL((k_gotoh_strip<STRIP_T, true>), dim3(blocks), dim3(GOTOH_BLOCK), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_gotoh_strip<STRIP_T, false>), dim3(blocks), dim3(GOTOH_BLOCK), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s) {
+    const size_t lds = (size_t)(MYERS_BLOCK / 64) * 5 * words * 64 * sizeof(uint64_t);
+    switch (words) {
+    case 1: hipLaunchKernelGGL(k_myers<1>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 2: hipLaunchKernelGGL(k_myers<2>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 3: hipLaunchKernelGGL(k_myers<3>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_myers<4>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+int gotoh_strip_blocks_per_cu(int edit) {
+    int nb = 0;
+    if (edit)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_strip<STRIP_T, true>, GOTOH_BLOCK, 0);
+    else
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_strip<STRIP_T, false>, GOTOH_BLOCK, 0);
+    return nb > 0 ? nb : 1;
+}
+
+int myers_blocks_per_cu(int words) {
+    int nb = 0;
+    const size_t lds = (size_t)(MYERS_BLOCK / 64) * 5 * words * 64 * sizeof(uint64_t);
+    switch (words) {
+    case 1: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<1>, MYERS_BLOCK, lds); break;
+    case 2: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<2>, MYERS_BLOCK, lds); break;
+    case 3: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<3>, MYERS_BLOCK, lds); break;
+    default: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<4>, MYERS_BLOCK, lds); break;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace espk

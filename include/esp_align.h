@@ -1,0 +1,181 @@
+/*
+ * esp_align.h -- C ABI of the MI355X alignment core.
+ *
+ * Drop-in boundary for the reference's crates/map alignment core.  Every
+ * entry point is extern "C", takes plain pointers and sizes, never throws and
+ * returns an esp_status.  CIGARs are BAM u32 words (len << 4 | op) with
+ * op M=0 I=1 D=2 N=3 S=4 (reference crates/bamio/src/encode.rs:169-177).
+ *
+ * Reference interfaces replaced (file:line in intell-biol-lab/esperanto):
+ *   esp_extend             crates/map/src/extend.rs:347   extend()
+ *   esp_extend_hint        crates/map/src/extend.rs:392   extend_hint()
+ *   esp_band               crates/map/src/extend.rs:293   run_dispatch(.., Some((offset,1,1,half)))
+ *   esp_myers_*            crates/map/src/myers_ea.rs:118-185, :249-310
+ *   esp_stitch_cigar       crates/map/src/splice.rs:435   stitch_cigar()
+ *   esp_ctx                crates/map/src/extend.rs:231   ExtendBuffer (per-thread scratch)
+ *   esp_*_batch            crates/map/src/pipeline.rs:393-404 (rayon par_iter over reads)
+ *
+ * Sequences are ASCII bytes exactly as the reference receives them
+ * (uppercase/lowercase ACGT, N and any other byte scoring as the reference's
+ * base_code, extend.rs:151-159).
+ *
+ * Scoring envelope accepted by the device path (ESP_E_INVAL otherwise):
+ *   gap_open >= 0, gap_ext >= 0, gap_open + gap_ext <= 16384,
+ *   |match_score| <= 1024, |mismatch| <= 16384, read length <= 16000,
+ *   window length <= 1 << 24.  Inside it the i16 and i32 reference kernels
+ *   agree, so results equal the reference's for every byte pattern.
+ */
+#ifndef ESP_ALIGN_H
+#define ESP_ALIGN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ESP_OK = 0,
+    ESP_E_INVAL = 1,   /* invalid argument / scoring outside the envelope */
+    ESP_E_CAP = 2,     /* caller buffer too small; required size reported */
+    ESP_E_DEVICE = 3,  /* HIP runtime error */
+    ESP_E_NOMEM = 4,   /* device allocation failed / admission refused */
+    ESP_E_BUSY = 5     /* server queue full */
+} esp_status;
+
+/* ExtendParams, crates/map/src/extend.rs:96-122.  edit_dir: 0 Off, 1 Ag, 2 Tc. */
+typedef struct {
+    int32_t match_score;
+    int32_t mismatch;
+    int32_t gap_open;
+    int32_t gap_ext;
+    int32_t edit_dir;
+} esp_params;
+
+/* Extension minus the CIGAR vector, crates/map/src/extend.rs:202-214. */
+typedef struct {
+    uint32_t read_start;
+    uint32_t read_end;
+    uint32_t ref_start;
+    int32_t score;
+} esp_ext;
+
+/* TouchRec, crates/map/src/extend.rs:220-229. */
+typedef struct {
+    int32_t touched;
+    int64_t row;
+    int32_t left;
+    int32_t clamped;
+} esp_touch;
+
+/* Context: one HIP stream plus device scratch; replaces ExtendBuffer.  Not
+ * thread-safe; use one context per host thread. */
+typedef struct esp_ctx esp_ctx;
+
+int esp_create(int device, esp_ctx **out);
+int esp_destroy(esp_ctx *ctx);
+const char *esp_last_error(const esp_ctx *ctx);
+int esp_sync(esp_ctx *ctx);
+
+/* ---- single-pair ("plain") forms: one kernel launch per call ----------- */
+
+/* extend(): full-matrix local extension.  cap = CIGAR capacity in ops; on
+ * ESP_E_CAP *n_ops holds the required count. */
+int esp_extend(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+               const esp_params *p, esp_ext *out, uint32_t *cigar, size_t cap, size_t *n_ops);
+
+/* extend_hint(): half-band 30 around hint_offset, escalating to the full
+ * matrix when a row maximum touches the band edge or score*20 < 17*perfect. */
+int esp_extend_hint(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+                    const esp_params *p, int64_t hint_offset, esp_ext *out, uint32_t *cigar,
+                    size_t cap, size_t *n_ops);
+
+/* run_dispatch(.., Some((offset,1,1,half))) when has_band != 0, else
+ * run_dispatch(.., None); also returns the TouchRec. */
+int esp_band(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *win, size_t n,
+             const esp_params *p, int has_band, int64_t offset, int64_t half, esp_ext *out,
+             esp_touch *touch, uint32_t *cigar, size_t cap, size_t *n_ops);
+
+/* myers_ea::long::infix_best_end / infix_best_start (m == 0 -> (0,0),
+ * m > 256 -> (m,0)); the short-module functions are the same for m <= 128. */
+int esp_myers_best_end(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *text,
+                       size_t n, int32_t *dist, size_t *end);
+int esp_myers_best_start(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *text,
+                         size_t n, int32_t *dist, size_t *start);
+
+/* ---- batched forms ------------------------------------------------------ */
+
+/* Pairs as concatenated bytes.  All pointers are DEVICE pointers for the
+ * *_batch calls (inputs resident in HBM).  offset/half: per pair (band mode)
+ * or the hint offset (extend_hint mode); half may be NULL otherwise. */
+typedef struct {
+    size_t npairs;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint8_t *wins;
+    const uint64_t *win_off;
+    const uint32_t *win_len;
+    const int64_t *offset;
+    const int64_t *half;
+} esp_batch;
+
+/* Output arena (device pointers).  cig_off has npairs + 1 entries; pair k
+ * owns cigar[cig_off[k] .. cig_off[k+1]) and needs at least 2*m + 5 words
+ * (esp_cigar_capacity).  touch and aux may be NULL. */
+typedef struct {
+    esp_ext *ext;
+    esp_touch *touch;
+    uint32_t *cigar;
+    const uint64_t *cig_off;
+    uint32_t *cig_n;
+    int32_t *aux; /* extend_hint: 0 band kept, 1 escalated, 2 empty, 3 band covers window */
+} esp_batch_out;
+
+typedef enum { ESP_MODE_BAND = 0, ESP_MODE_FULL = 1, ESP_MODE_HINT = 2 } esp_mode;
+
+size_t esp_cigar_capacity(size_t m);
+
+/* Asynchronous on the context stream; esp_sync() waits.  The host copies
+ * of read_len / win_len / offset / half (host_meta, may be NULL) let the
+ * planner bucket pairs without a device round trip. */
+typedef struct {
+    const uint32_t *read_len;
+    const uint32_t *win_len;
+    const int64_t *offset;
+    const int64_t *half;
+} esp_host_meta;
+
+int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
+                    const esp_params *p, int mode, esp_batch_out *out);
+
+/* long::infix_best_end (+ infix_best_start when start != NULL) per pair,
+ * text = window. */
+int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
+                    int32_t *dist, uint64_t *end, uint64_t *start);
+
+/* Average device duration (ms) of the kernels of the last batch call,
+ * measured with HIP events on the context stream. */
+int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
+
+/* Device memory helpers for callers without an allocator of their own.
+ * Copies are synchronous with respect to the context stream. */
+int esp_device_alloc(esp_ctx *ctx, size_t bytes, void **out);
+int esp_device_free(esp_ctx *ctx, void *p);
+int esp_copy_to_device(esp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int esp_copy_to_host(esp_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- splice / mate-rescue glue (host side, no device work) ------------- */
+
+/* stitch_cigar, splice.rs:435-494.  bounds: 3 u32 per exon (pseudo_start,
+ * ref_start, ref_end); skips[k] = junction k length (end - start). */
+int esp_stitch_cigar(const uint32_t *cigar, size_t n_ops, uint32_t p_start,
+                     const uint32_t *bounds, size_t n_exons, const uint32_t *skips,
+                     size_t n_junctions, uint32_t *out, size_t cap, size_t *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESP_ALIGN_H */

@@ -1,0 +1,892 @@
+/*
+ * esp_oracle.c -- CPU restatement of the reference alignment core, used ONLY
+ * as test infrastructure: the checker in tests/, the smoke check and the
+ * timed CPU baseline leg of bench.py.  Nothing in the product library
+ * (esperanto_amd/) links or calls this file.
+ *
+ * It restates, function by function, the reference's
+ *   crates/map/src/extend.rs    (scalar run_band<i16|i32>, finish_extension,
+ *                                run_dispatch, extend, extend_hint, push_op)
+ *   crates/map/src/myers_ea.rs  (build_peq, calculate_block, scan, infix*,
+ *                                long::*)
+ *   crates/map/src/splice.rs    (stitch_cigar)
+ *   crates/map/src/align.rs     (mate_window_placement selection rule)
+ * plus two independent textbook checkers (local affine-gap DP max score over
+ * a row-window set, semi-global EA edit distance) used to pin the
+ * restatement where the reference has no literal expected outputs.
+ *
+ * The reference's AVX2 kernels (extend.rs:1094-2026) are specified to be
+ * cell-for-cell identical to run_band::<i16> (their own tests,
+ * extend.rs:2578-2913), so run_dispatch here routes every pure input to the
+ * scalar i16 restatement; the results are the same by that contract.
+ *
+ * Result layout: CIGAR ops as BAM u32 (len << 4 | op), op M=0 I=1 D=2 N=3
+ * S=4 (bamio/src/encode.rs:169-177).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+#define OP_M 0u
+#define OP_I 1u
+#define OP_D 2u
+#define OP_N 3u
+#define OP_S 4u
+
+typedef struct {
+    int32_t match_score; /* extend.rs:98-122 */
+    int32_t mismatch;
+    int32_t gap_open;
+    int32_t gap_ext;
+    int32_t edit_dir; /* 0 Off, 1 Ag, 2 Tc (extend.rs:43-50) */
+} espo_params;
+
+typedef struct {
+    uint32_t read_start, read_end, ref_start;
+    int32_t score;
+} espo_ext;
+
+typedef struct {
+    int32_t touched;
+    int64_t row;
+    int32_t left;
+    int32_t clamped;
+} espo_touch;
+
+/* ---------------------------------------------------------------- codes */
+
+/* extend.rs:151-159 */
+static inline int base_code(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+    }
+}
+
+/* extend.rs:82-93: Some(true) = ref A/read G, Some(false) = ref T/read C. */
+static inline int exempt_dir(uint8_t ref_b, uint8_t read_b) {
+    if (ref_b == 'A' && read_b == 'G') return 1;
+    if (ref_b == 'T' && read_b == 'C') return 0;
+    return -1;
+}
+
+/* extend.rs:68-73 */
+static inline int edit_exempt(int dir, uint8_t ref_b, uint8_t read_b) {
+    int e = exempt_dir(ref_b, read_b);
+    return (dir == 1 && e == 1) || (dir == 2 && e == 0);
+}
+
+typedef struct {
+    int16_t lut[25]; /* [read_code*5 + ref_code], extend.rs:161-186 */
+    int32_t open_ext, ext; /* extend.rs:450-459 */
+} core_params;
+
+static void core_new(const espo_params *p, core_params *c) {
+    int16_t m = (int16_t)p->match_score, mm = (int16_t)p->mismatch;
+    for (int k = 0; k < 25; k++) c->lut[k] = mm;
+    for (int k = 0; k < 4; k++) c->lut[k * 5 + k] = m;
+    if (edit_exempt(p->edit_dir, 'A', 'G')) c->lut[10] = 0;
+    if (edit_exempt(p->edit_dir, 'T', 'C')) c->lut[8] = 0;
+    c->open_ext = -(p->gap_open + p->gap_ext);
+    c->ext = -p->gap_ext;
+}
+
+/* ---------------------------------------------------------------- CIGAR */
+
+typedef struct {
+    uint32_t *ops;
+    size_t n, cap;
+    int overflow;
+} opvec;
+
+static inline uint32_t op_kind(uint32_t op) { return op & 0xfu; }
+static inline uint32_t op_len(uint32_t op) { return op >> 4; }
+
+/* push_op: extend.rs:254-282 (merge with the previous op of the same kind) */
+static void push_op(opvec *v, uint32_t kind, uint32_t len) {
+    if (v->n > 0 && op_kind(v->ops[v->n - 1]) == kind) {
+        v->ops[v->n - 1] = ((op_len(v->ops[v->n - 1]) + len) << 4) | kind;
+        return;
+    }
+    if (v->n >= v->cap) {
+        v->overflow = 1;
+        v->n++;
+        return;
+    }
+    v->ops[v->n++] = (len << 4) | kind;
+}
+
+/* ---------------------------------------------------------------- Acc */
+
+#define NEG16 (-30000)
+#define NEG32 (-1000000000)
+
+/* i16 Acc (extend.rs:473-501): exact-equality sentinel, wrapping add, clamp.
+ * i32 Acc (extend.rs:503-528).  Values are carried in int32. */
+typedef struct {
+    int is16;
+    int32_t NEG;
+} acc_t;
+
+static inline int32_t acc_from(const acc_t *a, int32_t v) {
+    return a->is16 ? (int32_t)(int16_t)v : v;
+}
+static inline int32_t acc_add(const acc_t *a, int32_t x, int32_t y) {
+    if (x == a->NEG || y == a->NEG) return a->NEG;
+    if (a->is16) {
+        int32_t s = (int32_t)(int16_t)(uint16_t)((uint32_t)x + (uint32_t)y);
+        return s > a->NEG ? s : a->NEG;
+    }
+    return (int32_t)((uint32_t)x + (uint32_t)y);
+}
+
+/* ---------------------------------------------------------------- run_band */
+
+#define H_DIAG 0
+#define H_FROM_E 1
+#define H_FROM_F 2
+#define H_START 3
+
+typedef struct {
+    int64_t lo;
+    size_t off, len;
+} row_trace;
+
+typedef struct {
+    uint8_t op;   /* OP_M / OP_I / OP_D */
+    int32_t delta;
+} step_t;
+
+/* finish_extension: extend.rs:864-1068. */
+static void finish_extension(int64_t m, const core_params *core, const row_trace *rows,
+                             const uint8_t *rows_flat, const int *read_codes,
+                             const int *ref_codes, int64_t n_ref, int64_t best_i,
+                             int64_t best_j, espo_ext *out, opvec *cig) {
+    size_t steps_cap = (size_t)(best_i + best_j + 4);
+    step_t *steps = (step_t *)malloc(steps_cap * sizeof(step_t));
+    size_t ns = 0;
+    int state = 0; /* 0 H, 1 E, 2 F */
+    int64_t i = best_i, j = best_j;
+    for (;;) {
+        if (i <= 0 || j <= 0) break;
+        const row_trace *row = &rows[i];
+        if (j < row->lo || j >= row->lo + (int64_t)row->len) break;
+        uint8_t cell = rows_flat[row->off + (size_t)(j - row->lo)];
+        if (state == 0) {
+            int hd = cell & 3;
+            if (hd == H_DIAG) {
+                int32_t s = core->lut[read_codes[i - 1] * 5 + ref_codes[j - 1]];
+                steps[ns].op = OP_M; steps[ns].delta = s; ns++;
+                i--; j--;
+            } else if (hd == H_FROM_E) {
+                steps[ns].op = OP_I; steps[ns].delta = core->open_ext; ns++;
+                state = ((cell >> 2) & 1) ? 1 : 0;
+                i--;
+            } else if (hd == H_FROM_F) {
+                steps[ns].op = OP_D; steps[ns].delta = core->open_ext; ns++;
+                state = ((cell >> 4) & 1) ? 2 : 0;
+                j--;
+            } else {
+                break;
+            }
+        } else if (state == 1) {
+            steps[ns].op = OP_I; steps[ns].delta = core->ext; ns++;
+            if (((cell >> 2) & 1) == 0) state = 0;
+            i--;
+        } else {
+            steps[ns].op = OP_D; steps[ns].delta = core->ext; ns++;
+            if (((cell >> 4) & 1) == 0) state = 0;
+            j--;
+        }
+    }
+    /* steps.reverse() */
+    for (size_t a = 0, b = ns ? ns - 1 : 0; a < b; a++, b--) {
+        step_t t = steps[a]; steps[a] = steps[b]; steps[b] = t;
+    }
+    /* End trimming (extend.rs:932-960). */
+    size_t lead = 0, trail = 0;
+    {
+        int32_t acc = 0;
+        for (size_t k = 0; k < ns; k++) {
+            acc += steps[k].delta;
+            if (acc <= 0) lead = k + 1;
+        }
+    }
+    {
+        int32_t acc = 0;
+        for (size_t kk = ns; kk > 0; kk--) {
+            size_t k = kk - 1;
+            acc += steps[k].delta;
+            if (acc <= 0) trail = ns - k;
+        }
+    }
+    if (ns > 0 && lead + trail >= ns) {
+        lead = ns / 2;
+        trail = ns - lead - 1;
+    }
+    uint32_t read_start = (uint32_t)(i > 0 ? i : 0);
+    uint32_t ref_start = (uint32_t)(j > 0 ? j : 0);
+    for (size_t k = 0; k < lead; k++) {
+        if (steps[k].op == OP_M || steps[k].op == OP_I) read_start++;
+        if (steps[k].op == OP_M || steps[k].op == OP_D) ref_start++;
+    }
+    /* Rebuild (extend.rs:974-990). */
+    int32_t score = 0;
+    uint32_t *tmp_ops = (uint32_t *)malloc((ns + 4) * sizeof(uint32_t));
+    opvec ops = {tmp_ops, 0, ns + 4, 0};
+    uint32_t rm = 0, used_ref = 0;
+    for (size_t k = lead; k < ns - trail; k++) {
+        score += steps[k].delta;
+        push_op(&ops, steps[k].op, 1);
+        if (steps[k].op == OP_M || steps[k].op == OP_I) rm++;
+        if (steps[k].op == OP_M || steps[k].op == OP_D) used_ref++;
+    }
+    uint32_t read_end = read_start + rm;
+    uint32_t neutral_left = 0, neutral_right = 0;
+    if (score > 0) {
+        /* Neutral flank scan (extend.rs:991-1049). */
+        static const uint8_t ACGT[4] = {'A', 'C', 'G', 'T'};
+#define NEUTRAL(q, r) ((q) < 4 && (r) < 4 && ((q) == (r) || exempt_dir(ACGT[r], ACGT[q]) >= 0))
+        int32_t delta = 0;
+        uint32_t lim = read_start < ref_start ? read_start : ref_start;
+        for (uint32_t nn = 1; nn <= lim; nn++) {
+            int q = read_codes[read_start - nn];
+            int r = ref_codes[ref_start - nn];
+            if (!NEUTRAL(q, r)) break;
+            delta += core->lut[q * 5 + r];
+            if (delta == 0) neutral_left = nn;
+        }
+        uint32_t ref_end = ref_start + used_ref;
+        delta = 0;
+        uint32_t av1 = (uint32_t)m - read_end, av2 = (uint32_t)n_ref - ref_end;
+        uint32_t available = av1 < av2 ? av1 : av2;
+        for (uint32_t nn = 0; nn < available; nn++) {
+            int q = read_codes[read_end + nn];
+            int r = ref_codes[ref_end + nn];
+            if (!NEUTRAL(q, r)) break;
+            delta += core->lut[q * 5 + r];
+            if (delta == 0) neutral_right = nn + 1;
+        }
+#undef NEUTRAL
+        read_start -= neutral_left;
+        ref_start -= neutral_left;
+        read_end += neutral_right;
+    }
+    /* CIGAR assembly (extend.rs:1036-1059). */
+    if (read_start > 0) push_op(cig, OP_S, read_start);
+    if (neutral_left > 0) push_op(cig, OP_M, neutral_left);
+    for (size_t k = 0; k < ops.n; k++) push_op(cig, op_kind(ops.ops[k]), op_len(ops.ops[k]));
+    if (neutral_right > 0) push_op(cig, OP_M, neutral_right);
+    if (read_end < (uint32_t)m) push_op(cig, OP_S, (uint32_t)m - read_end);
+    out->read_start = read_start;
+    out->read_end = read_end;
+    out->ref_start = ref_start;
+    out->score = score;
+    free(tmp_ops);
+    free(steps);
+}
+
+static void empty_extension(uint32_t m, espo_ext *out, opvec *cig) {
+    out->read_start = out->read_end = out->ref_start = 0;
+    out->score = 0;
+    if (m > 0) push_op(cig, OP_S, m);
+}
+
+/* run_band<T>: extend.rs:606-851.  is16 selects the i16 Acc. */
+static void run_band(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t n,
+                     const core_params *core, int is16, int has_band, int64_t b_offset,
+                     int64_t b_half, espo_ext *out, opvec *cig, espo_touch *touch) {
+    memset(touch, 0, sizeof(*touch));
+    if (m == 0 || n == 0) {
+        empty_extension((uint32_t)m, out, cig);
+        return;
+    }
+    acc_t A = {is16, is16 ? NEG16 : NEG32};
+    int64_t offset, half;
+    if (has_band) {
+        offset = b_offset;
+        half = b_half > 0 ? b_half : 0;
+    } else {
+        offset = 0;
+        half = n;
+    }
+    size_t W = (size_t)n + 2; /* every stored row is at most n + 1 wide */
+    int32_t *h_prev = (int32_t *)malloc(W * sizeof(int32_t));
+    int32_t *h_cur = (int32_t *)malloc(W * sizeof(int32_t));
+    int32_t *e_prev = (int32_t *)malloc(W * sizeof(int32_t));
+    int32_t *e_cur = (int32_t *)malloc(W * sizeof(int32_t));
+    int32_t *f_cur = (int32_t *)malloc(W * sizeof(int32_t));
+    for (size_t k = 0; k < W; k++) h_prev[k] = h_cur[k] = e_prev[k] = e_cur[k] = f_cur[k] = A.NEG;
+    row_trace *rows = (row_trace *)malloc((size_t)(m + 1) * sizeof(row_trace));
+    size_t flat_cap = 1024, flat_n = 0;
+    uint8_t *flat = (uint8_t *)malloc(flat_cap);
+    int *rc_codes = (int *)malloc((size_t)m * sizeof(int));
+    int *rf_codes = (int *)malloc((size_t)n * sizeof(int));
+    for (int64_t k = 0; k < m; k++) rc_codes[k] = base_code(read[k]);
+    for (int64_t k = 0; k < n; k++) rf_codes[k] = base_code(rf[k]);
+
+#define FLAT_GROW(extra)                                              \
+    do {                                                              \
+        while (flat_n + (extra) > flat_cap) flat_cap *= 2;            \
+        flat = (uint8_t *)realloc(flat, flat_cap);                    \
+    } while (0)
+
+    int64_t center0 = offset;
+    int64_t lo0 = center0 - half > 0 ? center0 - half : 0;
+    int64_t hi0 = center0 + half < n ? center0 + half : n;
+    if (lo0 <= hi0) {
+        size_t len = (size_t)(hi0 - lo0 + 1);
+        FLAT_GROW(len);
+        memset(flat + flat_n, H_START, len);
+        for (size_t c = 0; c < len; c++) h_prev[c] = acc_from(&A, 0);
+        rows[0].lo = lo0; rows[0].off = flat_n; rows[0].len = len;
+        flat_n += len;
+    } else {
+        rows[0].lo = lo0; rows[0].off = flat_n; rows[0].len = 0;
+    }
+    int32_t open_ext = acc_from(&A, core->open_ext);
+    int32_t ext_p = acc_from(&A, core->ext);
+    int32_t zero = 0;
+    int32_t best_score = zero;
+    int64_t best_i = 0, best_j = 0;
+    int64_t plo = lo0, phi = hi0;
+    for (int64_t i = 1; i <= m; i++) {
+        int64_t center = offset + i;
+        int64_t c = center < 1 ? 1 : (center > n ? n : center);
+        int64_t lo = c - half > 1 ? c - half : 1;
+        int64_t hi = c + half < n ? c + half : n;
+        if (lo > hi) {
+            rows[i].lo = lo; rows[i].off = flat_n; rows[i].len = 0;
+            plo = lo; phi = hi;
+            continue;
+        }
+        size_t len = (size_t)(hi - lo + 1);
+        FLAT_GROW(len);
+        size_t off = flat_n;
+        memset(flat + off, 0, len);
+        flat_n += len;
+        int rc = rc_codes[i - 1];
+        int32_t row_max = A.NEG;
+        int64_t row_max_cell = 0;
+        for (int64_t jj = lo; jj <= hi; jj++) {
+            size_t cidx = (size_t)(jj - lo);
+            int in_prev = jj >= plo && jj <= phi;
+            int32_t h_up = in_prev ? h_prev[jj - plo] : zero;
+            int32_t e_up = in_prev ? e_prev[jj - plo] : A.NEG;
+            int32_t e_open = acc_add(&A, h_up, open_ext);
+            int32_t e_ext = acc_add(&A, e_up, ext_p);
+            int32_t e_val; int edir;
+            if (e_ext > e_open) { e_val = e_ext; edir = 1; } else { e_val = e_open; edir = 0; }
+            int32_t h_left = jj > lo ? h_cur[cidx - 1] : zero;
+            int32_t f_left = jj > lo ? f_cur[cidx - 1] : A.NEG;
+            int32_t f_open = acc_add(&A, h_left, open_ext);
+            int32_t f_ext = acc_add(&A, f_left, ext_p);
+            int32_t f_val; int fdir;
+            if (f_ext > f_open) { f_val = f_ext; fdir = 1; } else { f_val = f_open; fdir = 0; }
+            int64_t hp = jj - 1;
+            int32_t h_diag = (hp >= plo && hp <= phi) ? h_prev[hp - plo] : zero;
+            int32_t s = core->lut[rc * 5 + rf_codes[jj - 1]];
+            int32_t diag = acc_add(&A, h_diag, acc_from(&A, s));
+            int32_t h_val = zero;
+            int hdir = H_START;
+            if (f_val != A.NEG && !(h_val > f_val)) { h_val = f_val; hdir = H_FROM_F; }
+            if (e_val != A.NEG && !(h_val > e_val)) { h_val = e_val; hdir = H_FROM_E; }
+            if (diag != A.NEG && !(h_val > diag)) { h_val = diag; hdir = H_DIAG; }
+            if (h_val == zero) hdir = H_START;
+            h_cur[cidx] = h_val;
+            e_cur[cidx] = e_val;
+            f_cur[cidx] = f_val;
+            flat[off + cidx] = (uint8_t)(hdir | (edir << 2) | (fdir << 4));
+            if (h_val > row_max) { row_max = h_val; row_max_cell = (int64_t)cidx; }
+        }
+        int left_strict = lo > 0;
+        int right_strict = hi < n;
+        if (!touch->touched) {
+            if (left_strict && h_cur[0] == row_max) {
+                touch->touched = 1; touch->row = i; touch->left = 1; touch->clamped = lo <= 1;
+            } else if (right_strict && h_cur[len - 1] == row_max) {
+                touch->touched = 1; touch->row = i; touch->left = 0; touch->clamped = hi >= n;
+            }
+        }
+        if (row_max > best_score) {
+            best_score = row_max;
+            best_i = i;
+            best_j = lo + row_max_cell;
+        }
+        rows[i].lo = lo; rows[i].off = off; rows[i].len = len;
+        int32_t *t;
+        t = h_prev; h_prev = h_cur; h_cur = t;
+        t = e_prev; e_prev = e_cur; e_cur = t;
+        plo = lo; phi = hi;
+    }
+#undef FLAT_GROW
+    finish_extension(m, core, rows, flat, rc_codes, rf_codes, n, best_i, best_j, out, cig);
+    free(h_prev); free(h_cur); free(e_prev); free(e_cur); free(f_cur);
+    free(rows); free(flat); free(rc_codes); free(rf_codes);
+}
+
+/* ---------------------------------------------------------------- API */
+
+static int pure_byte(uint8_t b) { return b == 'A' || b == 'C' || b == 'G' || b == 'T' || b == 'N'; }
+
+/* run_dispatch: extend.rs:293-344.  The AVX2 branches are cell-for-cell the
+ * i16 scalar kernel by the reference's own contract, so pure inputs run the
+ * i16 restatement and everything else the i32 one. */
+static void run_dispatch(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t n,
+                         const espo_params *p, int has_band, int64_t offset, int64_t half,
+                         espo_ext *out, opvec *cig, espo_touch *touch) {
+    core_params core;
+    core_new(p, &core);
+    int pure = 1;
+    for (int64_t k = 0; k < m && pure; k++) pure = pure_byte(read[k]);
+    for (int64_t k = 0; k < n && pure; k++) pure = pure_byte(rf[k]);
+    run_band(read, m, rf, n, &core, pure, has_band, offset, half, out, cig, touch);
+}
+
+#define INITIAL_BAND 30 /* extend.rs:126 */
+
+static int finish_call(opvec *cig, size_t *n_ops) {
+    *n_ops = cig->n;
+    return cig->overflow ? 2 : 0;
+}
+
+/* Internal run_band with explicit accumulator width (tests compare both). */
+int espo_run_band(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
+                  const espo_params *p, int is16, int has_band, int64_t offset, int64_t half,
+                  espo_ext *out, espo_touch *touch, uint32_t *cigar, size_t cap, size_t *n_ops) {
+    core_params core;
+    core_new(p, &core);
+    opvec cig = {cigar, 0, cap, 0};
+    run_band(read, (int64_t)m, rf, (int64_t)n, &core, is16, has_band, offset, half, out, &cig, touch);
+    return finish_call(&cig, n_ops);
+}
+
+/* run_dispatch(.., Some((offset,1,1,half))) or None when has_band == 0. */
+int espo_band(const uint8_t *read, size_t m, const uint8_t *rf, size_t n, const espo_params *p,
+              int has_band, int64_t offset, int64_t half, espo_ext *out, espo_touch *touch,
+              uint32_t *cigar, size_t cap, size_t *n_ops) {
+    opvec cig = {cigar, 0, cap, 0};
+    run_dispatch(read, (int64_t)m, rf, (int64_t)n, p, has_band, offset, half, out, &cig, touch);
+    return finish_call(&cig, n_ops);
+}
+
+/* extend: extend.rs:347-354 */
+int espo_extend(const uint8_t *read, size_t m, const uint8_t *rf, size_t n, const espo_params *p,
+                espo_ext *out, uint32_t *cigar, size_t cap, size_t *n_ops) {
+    espo_touch t;
+    return espo_band(read, m, rf, n, p, 0, 0, 0, out, &t, cigar, cap, n_ops);
+}
+
+/* extend_hint: extend.rs:392-428.  *escalated reports which attempt won
+ * (0 band accepted, 1 full matrix, 2 empty, 3 band covers window). */
+int espo_extend_hint(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
+                     const espo_params *p, int64_t hint_offset, espo_ext *out, uint32_t *cigar,
+                     size_t cap, size_t *n_ops, int32_t *escalated) {
+    opvec cig = {cigar, 0, cap, 0};
+    espo_touch t;
+    if (m == 0 || n == 0) {
+        empty_extension((uint32_t)m, out, &cig);
+        *escalated = 2;
+        return finish_call(&cig, n_ops);
+    }
+    if ((int64_t)INITIAL_BAND >= (int64_t)n) {
+        run_dispatch(read, (int64_t)m, rf, (int64_t)n, p, 0, 0, 0, out, &cig, &t);
+        *escalated = 3;
+        return finish_call(&cig, n_ops);
+    }
+    run_dispatch(read, (int64_t)m, rf, (int64_t)n, p, 1, hint_offset, INITIAL_BAND, out, &cig, &t);
+    int32_t perfect = (int32_t)m * p->match_score;
+    if (t.touched || out->score * 20 < perfect * 17) {
+        cig.n = 0;
+        cig.overflow = 0;
+        run_dispatch(read, (int64_t)m, rf, (int64_t)n, p, 0, 0, 0, out, &cig, &t);
+        *escalated = 1;
+        return finish_call(&cig, n_ops);
+    }
+    *escalated = 0;
+    return finish_call(&cig, n_ops);
+}
+
+/* ---------------------------------------------------------------- Myers */
+
+typedef unsigned __int128 u128;
+
+/* myers_ea.rs:27-38 */
+static int ea_matches(uint8_t read, uint8_t reference) {
+    uint8_t r = (read >= 'a' && read <= 'z') ? (uint8_t)(read - 32) : read;
+    uint8_t f = (reference >= 'a' && reference <= 'z') ? (uint8_t)(reference - 32) : reference;
+    switch (r) {
+    case 'A': return f == 'A';
+    case 'C': return f == 'C' || f == 'T';
+    case 'G': return f == 'G' || f == 'A';
+    case 'T': return f == 'T';
+    default: return 0;
+    }
+}
+
+/* myers_ea.rs:18-24 */
+static u128 mask_for(size_t m) { return m >= 128 ? ~(u128)0 : (((u128)1 << m) - 1); }
+
+/* myers_ea.rs:42-56 (block 0) and :194-213 (2-block) */
+static void build_peq_blocks(const uint8_t *read, size_t m, size_t nblk, u128 *peq /* nblk*256 */) {
+    memset(peq, 0, nblk * 256 * sizeof(u128));
+    static const uint8_t ACGT[4] = {'A', 'C', 'G', 'T'};
+    size_t lim = nblk * 128 < m ? nblk * 128 : m;
+    for (size_t i = 0; i < lim; i++)
+        for (int f = 0; f < 4; f++)
+            if (ea_matches(read[i], ACGT[f])) peq[(i / 128) * 256 + ACGT[f]] |= (u128)1 << (i % 128);
+    static const uint8_t lo[4] = {'a', 'c', 'g', 't'};
+    for (size_t b = 0; b < nblk; b++)
+        for (int f = 0; f < 4; f++) peq[b * 256 + lo[f]] = peq[b * 256 + ACGT[f]];
+}
+
+/* calculate_block: myers_ea.rs:64-91 */
+static void calculate_block(u128 pv, u128 mv, u128 eq_in, int hin, u128 mask, u128 high_bit,
+                            u128 *pv_out, u128 *mv_out, int *hout) {
+    u128 hin_is_neg = (u128)(hin < 0);
+    u128 ph_bit0 = (u128)(hin > 0);
+    u128 xv = eq_in | mv;
+    u128 eq = eq_in | hin_is_neg;
+    u128 xh = (((eq & pv) + pv) ^ pv) | eq;
+    u128 ph = mv | ~(xh | pv);
+    u128 mh = pv & xh;
+    *hout = (ph & high_bit) ? 1 : ((mh & high_bit) ? -1 : 0);
+    u128 ph_sh = (ph << 1) | ph_bit0;
+    u128 mh_sh = (mh << 1) | hin_is_neg;
+    *pv_out = (mh_sh | ~(xv | ph_sh)) & mask;
+    *mv_out = (ph_sh & xv) & mask;
+}
+
+/* scan (myers_ea.rs:93-114) / scan2 (:215-246), with the best/best_at update
+ * of infix_best_end (strict <, earliest end). */
+static void myers_scan(const uint8_t *read, size_t m, const uint8_t *text, size_t n, int32_t *dist,
+                       size_t *end) {
+    if (m <= 128) {
+        u128 peq[256];
+        build_peq_blocks(read, m, 1, peq);
+        u128 mask = mask_for(m);
+        u128 high = (u128)1 << (m - 1);
+        u128 vp = mask, vn = 0;
+        int32_t score = (int32_t)m, best = score;
+        size_t best_at = 0;
+        for (size_t j = 0; j < n; j++) {
+            int h;
+            calculate_block(vp, vn, peq[text[j]], 0, mask, high, &vp, &vn, &h);
+            score += h;
+            if (score < best) { best = score; best_at = j + 1; }
+        }
+        *dist = best;
+        *end = best_at;
+        return;
+    }
+    u128 *peq = (u128 *)malloc(2 * 256 * sizeof(u128));
+    build_peq_blocks(read, m, 2, peq);
+    size_t m1 = m - 128;
+    u128 mask1 = mask_for(m1);
+    u128 high0 = (u128)1 << 127;
+    u128 high1 = (u128)1 << (m1 - 1);
+    u128 vp0 = ~(u128)0, vn0 = 0, vp1 = mask1, vn1 = 0;
+    int32_t score = (int32_t)m, best = score;
+    size_t best_at = 0;
+    for (size_t j = 0; j < n; j++) {
+        int h0, h1;
+        u128 a, b, c, d;
+        calculate_block(vp0, vn0, peq[text[j]], 0, ~(u128)0, high0, &a, &b, &h0);
+        calculate_block(vp1, vn1, peq[256 + text[j]], h0, mask1, high1, &c, &d, &h1);
+        vp0 = a; vn0 = b; vp1 = c; vn1 = d;
+        score += h1;
+        if (score < best) { best = score; best_at = j + 1; }
+    }
+    free(peq);
+    *dist = best;
+    *end = best_at;
+}
+
+/* long::infix_best_end (myers_ea.rs:271-286): m == 0 -> (0,0), m > 256 -> (m,0). */
+void espo_myers_best_end(const uint8_t *read, size_t m, const uint8_t *text, size_t n,
+                         int32_t *dist, size_t *end) {
+    if (m == 0) { *dist = 0; *end = 0; return; }
+    if (m > 256) { *dist = (int32_t)m; *end = 0; return; }
+    myers_scan(read, m, text, n, dist, end);
+}
+
+/* long::infix_best_start_into (myers_ea.rs:294-310) */
+void espo_myers_best_start(const uint8_t *read, size_t m, const uint8_t *text, size_t n,
+                           int32_t *dist, size_t *start) {
+    if (m == 0) { *dist = 0; *start = 0; return; }
+    if (m > 256) { *dist = (int32_t)m; *start = 0; return; }
+    uint8_t *rr = (uint8_t *)malloc(m ? m : 1), *rt = (uint8_t *)malloc(n ? n : 1);
+    for (size_t k = 0; k < m; k++) rr[k] = read[m - 1 - k];
+    for (size_t k = 0; k < n; k++) rt[k] = text[n - 1 - k];
+    size_t rev_end;
+    myers_scan(rr, m, rt, n, dist, &rev_end);
+    *start = n - rev_end;
+    free(rr);
+    free(rt);
+}
+
+/* ---------------------------------------------------------------- splice */
+
+/* stitch_cigar: splice.rs:435-494.  bounds: 3 u32 per exon (pseudo_start,
+ * ref_start, ref_end); skips[k] = junctions[k].end - junctions[k].start. */
+int espo_stitch_cigar(const uint32_t *cigar, size_t n_ops, uint32_t p_start,
+                      const uint32_t *bounds, size_t n_exons, const uint32_t *skips,
+                      size_t n_junctions, uint32_t *out, size_t cap, size_t *n_out) {
+    opvec o = {out, 0, cap, 0};
+    if (n_exons == 0) {
+        for (size_t k = 0; k < n_ops; k++) {
+            if (o.n < cap) out[o.n] = cigar[k];
+            else o.overflow = 1;
+            o.n++;
+        }
+        *n_out = o.n;
+        return o.overflow ? 2 : 0;
+    }
+#define EXON_END(ei) ((uint64_t)bounds[3 * (ei)] + (uint64_t)(bounds[3 * (ei) + 2] - bounds[3 * (ei) + 1]))
+    uint64_t p = p_start;
+    size_t ei = 0;
+    while (ei + 1 < n_exons && p >= EXON_END(ei)) ei++;
+    for (size_t k = 0; k < n_ops; k++) {
+        uint32_t kind = op_kind(cigar[k]), len = op_len(cigar[k]);
+        if (kind == OP_I || kind == OP_S || kind == OP_N) {
+            push_op(&o, kind, len);
+            continue;
+        }
+        uint64_t remaining = len;
+        while (remaining > 0) {
+            if (ei >= n_exons) {
+                push_op(&o, kind, (uint32_t)remaining);
+                p += remaining;
+                break;
+            }
+            if (p < bounds[3 * ei]) p = bounds[3 * ei];
+            uint64_t ee = EXON_END(ei);
+            uint64_t rie = ee > p ? ee - p : 0;
+            uint64_t take = remaining < rie ? remaining : rie;
+            if (take > 0) push_op(&o, kind, (uint32_t)take);
+            p += take;
+            remaining -= take;
+            if (remaining > 0) {
+                if (ei + 1 >= n_exons) {
+                    push_op(&o, kind, (uint32_t)remaining);
+                    p += remaining;
+                    break;
+                }
+                if (ei < n_junctions) push_op(&o, OP_N, skips[ei]);
+                ei++;
+            }
+        }
+    }
+#undef EXON_END
+    *n_out = o.n;
+    return o.overflow ? 2 : 0;
+}
+
+/* ------------------------------------------------- independent checkers */
+
+/* Textbook local affine-gap DP (Gotoh 1982) over the cell set the band
+ * defines: row i >= 1 holds columns [lo_i, hi_i] with c = clamp(offset+i,
+ * 1, n), lo_i = max(c-half,1), hi_i = min(c+half,n).  A local alignment
+ * may begin at any cell, so H outside the cell set is 0 while the gap states
+ * E/F there are -inf.  Gap of length k costs open + k*ext.  Returns the
+ * maximum H.  Written from the recurrence, not from run_band. */
+int32_t espo_bf_local_affine(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
+                             const espo_params *p, int has_band, int64_t offset, int64_t half) {
+    if (m == 0 || n == 0) return 0;
+    const int64_t NINF = -((int64_t)1 << 40);
+    size_t W = n + 1;
+    int64_t *H = (int64_t *)malloc((m + 1) * W * sizeof(int64_t));
+    int64_t *E = (int64_t *)malloc((m + 1) * W * sizeof(int64_t));
+    int64_t *F = (int64_t *)malloc((m + 1) * W * sizeof(int64_t));
+    for (size_t k = 0; k < (m + 1) * W; k++) { H[k] = 0; E[k] = NINF; F[k] = NINF; }
+    int64_t go = p->gap_open, ge = p->gap_ext, best = 0;
+    if (!has_band) { offset = 0; half = (int64_t)n; }
+    if (half < 0) half = 0;
+    for (size_t i = 1; i <= m; i++) {
+        int64_t c = offset + (int64_t)i;
+        if (c < 1) c = 1;
+        if (c > (int64_t)n) c = (int64_t)n;
+        int64_t lo = c - half > 1 ? c - half : 1, hi = c + half < (int64_t)n ? c + half : (int64_t)n;
+        int rcode = base_code(read[i - 1]);
+        for (int64_t j = lo; j <= hi; j++) {
+            int fcode = base_code(rf[j - 1]);
+            int64_t s;
+            if (rcode < 4 && rcode == fcode) s = p->match_score;
+            else if (rcode < 4 && fcode < 4 && edit_exempt(p->edit_dir, "ACGT"[fcode], "ACGT"[rcode])) s = 0;
+            else s = p->mismatch;
+            size_t ij = i * W + (size_t)j, up = (i - 1) * W + (size_t)j, lf = ij - 1, dg = up - 1;
+            int64_t e = NINF, f = NINF;
+            e = H[up] - go - ge;
+            if (E[up] > NINF && E[up] - ge > e) e = E[up] - ge;
+            f = H[lf] - go - ge;
+            if (F[lf] > NINF && F[lf] - ge > f) f = F[lf] - ge;
+            int64_t h = 0;
+            if (H[dg] + s > h) h = H[dg] + s;
+            if (e > h) h = e;
+            if (f > h) h = f;
+            H[ij] = h; E[ij] = e; F[ij] = f;
+            if (h > best) best = h;
+        }
+    }
+    free(H); free(E); free(F);
+    return (int32_t)best;
+}
+
+/* Textbook semi-global unit-cost edit distance under EA equality
+ * (free text prefix and suffix): D[0][j] = 0, D[i][0] = i.  Returns
+ * (min_j D[m][j], earliest j attaining it, counting j = 0 with D = m). */
+void espo_bf_ea_edit(const uint8_t *read, size_t m, const uint8_t *text, size_t n, int32_t *dist,
+                     size_t *end) {
+    int32_t *prev = (int32_t *)malloc((m + 1) * sizeof(int32_t));
+    int32_t *cur = (int32_t *)malloc((m + 1) * sizeof(int32_t));
+    for (size_t i = 0; i <= m; i++) prev[i] = (int32_t)i; /* column j = 0 */
+    int32_t best = (int32_t)m;
+    size_t best_at = 0;
+    for (size_t j = 1; j <= n; j++) {
+        cur[0] = 0;
+        for (size_t i = 1; i <= m; i++) {
+            int32_t d = prev[i - 1] + (ea_matches(read[i - 1], text[j - 1]) ? 0 : 1);
+            int32_t a = prev[i] + 1, b = cur[i - 1] + 1;
+            if (a < d) d = a;
+            if (b < d) d = b;
+            cur[i] = d;
+        }
+        if (cur[m] < best) { best = cur[m]; best_at = j; }
+        int32_t *t = prev; prev = cur; cur = t;
+    }
+    free(prev); free(cur);
+    *dist = best;
+    *end = best_at;
+}
+
+/* ---------------------------------------------------------------- batches */
+
+/* Packed batch description shared by the batch helpers: sequences are
+ * concatenated bytes addressed by 64-bit offsets. */
+typedef struct {
+    const uint8_t *reads, *wins;
+    const uint64_t *read_off, *win_off;
+    const uint32_t *read_len, *win_len;
+    const int64_t *offset, *half; /* band mode */
+    const espo_params *p;
+    int mode; /* 0 band, 1 full, 2 extend_hint (offset = hint) */
+    size_t lo, hi;
+    espo_ext *out;
+    espo_touch *touch;
+    uint32_t *cigar;
+    const uint64_t *cig_off;
+    uint32_t *cig_n;
+    int32_t *aux;
+    int rc;
+} band_job;
+
+static void *band_worker(void *arg) {
+    band_job *J = (band_job *)arg;
+    for (size_t k = J->lo; k < J->hi; k++) {
+        const uint8_t *r = J->reads + J->read_off[k], *w = J->wins + J->win_off[k];
+        size_t cap = (size_t)(J->cig_off[k + 1] - J->cig_off[k]), nops = 0;
+        uint32_t *cg = J->cigar + J->cig_off[k];
+        espo_touch t = {0, 0, 0, 0};
+        int rc;
+        if (J->mode == 2) {
+            int32_t esc = 0;
+            rc = espo_extend_hint(r, J->read_len[k], w, J->win_len[k], J->p, J->offset[k],
+                                  &J->out[k], cg, cap, &nops, &esc);
+            if (J->aux) J->aux[k] = esc;
+        } else {
+            rc = espo_band(r, J->read_len[k], w, J->win_len[k], J->p, J->mode == 0,
+                           J->mode == 0 ? J->offset[k] : 0, J->mode == 0 ? J->half[k] : 0,
+                           &J->out[k], &t, cg, cap, &nops);
+        }
+        if (J->touch) J->touch[k] = t;
+        J->cig_n[k] = (uint32_t)nops;
+        if (rc) J->rc = rc;
+    }
+    return NULL;
+}
+
+/* Runs many pairs over nthreads POSIX threads (static chunks, one scratch
+ * set per call, mirroring the reference's per-worker ExtendBuffer). */
+int espo_band_batch(size_t npairs, const uint8_t *reads, const uint64_t *read_off,
+                    const uint32_t *read_len, const uint8_t *wins, const uint64_t *win_off,
+                    const uint32_t *win_len, const int64_t *offset, const int64_t *half,
+                    const espo_params *p, int mode, espo_ext *out, espo_touch *touch,
+                    uint32_t *cigar, const uint64_t *cig_off, uint32_t *cig_n, int32_t *aux,
+                    int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    band_job *jobs = (band_job *)calloc((size_t)nthreads, sizeof(band_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    size_t chunk = (npairs + (size_t)nthreads - 1) / (size_t)nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        band_job *J = &jobs[t];
+        J->reads = reads; J->wins = wins; J->read_off = read_off; J->win_off = win_off;
+        J->read_len = read_len; J->win_len = win_len; J->offset = offset; J->half = half;
+        J->p = p; J->mode = mode; J->out = out; J->touch = touch; J->cigar = cigar;
+        J->cig_off = cig_off; J->cig_n = cig_n; J->aux = aux;
+        J->lo = (size_t)t * chunk < npairs ? (size_t)t * chunk : npairs;
+        J->hi = J->lo + chunk < npairs ? J->lo + chunk : npairs;
+        pthread_create(&th[t], NULL, band_worker, J);
+    }
+    int rc = 0;
+    for (int t = 0; t < nthreads; t++) {
+        pthread_join(th[t], NULL);
+        if (jobs[t].rc) rc = jobs[t].rc;
+    }
+    free(jobs);
+    free(th);
+    return rc;
+}
+
+typedef struct {
+    const uint8_t *reads, *texts;
+    const uint64_t *read_off, *text_off;
+    const uint32_t *read_len, *text_len;
+    size_t lo, hi;
+    int32_t *dist;
+    uint64_t *end, *start;
+} myers_job;
+
+static void *myers_worker(void *arg) {
+    myers_job *J = (myers_job *)arg;
+    for (size_t k = J->lo; k < J->hi; k++) {
+        const uint8_t *r = J->reads + J->read_off[k], *t = J->texts + J->text_off[k];
+        size_t e = 0, s = 0;
+        int32_t d = 0, d2 = 0;
+        espo_myers_best_end(r, J->read_len[k], t, J->text_len[k], &d, &e);
+        J->dist[k] = d;
+        J->end[k] = e;
+        if (J->start) {
+            espo_myers_best_start(r, J->read_len[k], t, J->text_len[k], &d2, &s);
+            J->start[k] = s;
+        }
+    }
+    return NULL;
+}
+
+/* long::infix_best_end (+ infix_best_start when start != NULL) over a batch. */
+int espo_myers_batch(size_t npairs, const uint8_t *reads, const uint64_t *read_off,
+                     const uint32_t *read_len, const uint8_t *texts, const uint64_t *text_off,
+                     const uint32_t *text_len, int32_t *dist, uint64_t *end, uint64_t *start,
+                     int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    myers_job *jobs = (myers_job *)calloc((size_t)nthreads, sizeof(myers_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    size_t chunk = (npairs + (size_t)nthreads - 1) / (size_t)nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        myers_job *J = &jobs[t];
+        J->reads = reads; J->texts = texts; J->read_off = read_off; J->text_off = text_off;
+        J->read_len = read_len; J->text_len = text_len; J->dist = dist; J->end = end;
+        J->start = start;
+        J->lo = (size_t)t * chunk < npairs ? (size_t)t * chunk : npairs;
+        J->hi = J->lo + chunk < npairs ? J->lo + chunk : npairs;
+        pthread_create(&th[t], NULL, myers_worker, J);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(jobs);
+    free(th);
+    return 0;
+}
