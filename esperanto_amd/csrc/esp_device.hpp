@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/esp_align.h"
+
 namespace espd {
 
 constexpr int32_t NEG = -1000000000;  // i32 sentinel of the reference (extend.rs:504)
@@ -99,6 +101,73 @@ __device__ __forceinline__ uint32_t assemble_cigar(uint32_t *slot, uint32_t cap,
     if (neutral_right > 0) push(OP_M, neutral_right);
     if (read_end < m) push(OP_S, m - read_end);
     return k;
+}
+
+struct PairView {
+    const uint8_t *read;
+    const uint8_t *win;
+    int32_t m, n, off, half;
+};
+
+__device__ __forceinline__ int32_t clamp_off(int64_t off, int32_t m, int32_t n) {
+    int64_t lo = -(int64_t)m - 1, hi = n;
+    return (int32_t)(off < lo ? lo : (off > hi ? hi : off));
+}
+__device__ __forceinline__ int32_t clamp_half(int64_t half, int32_t n) {
+    return (int32_t)(half < 0 ? 0 : (half > n ? n : half));
+}
+
+struct TbResult {
+    uint32_t read_start, ref_start, nm, ni, nd, nrev;
+};
+
+// End trimming and score rebuild (extend.rs:932-990) reduce to the identity
+// here: with gap_open, gap_ext >= 0 every prefix of the walked path ends on
+// a cell whose H (or gap value) is positive, and every proper prefix stays
+// below the first row-major maximum, so neither trim fires and the score is
+// the best cell's H.  Neutral flanks and soft clips follow extend.rs:991-1059.
+static __device__ void finish_pair(const PairView &pv, const GParams &P, int32_t best,
+                            const TbResult &tb, uint32_t *slot, uint32_t cap, esp_ext *ext,
+                            uint32_t *n_ops) {
+    uint32_t read_start = tb.read_start, ref_start = tb.ref_start;
+    uint32_t read_end = read_start + tb.nm + tb.ni;
+    uint32_t nl = 0, nr = 0;
+    if (best > 0) {
+        int32_t delta = 0;
+        const uint32_t lim = min(read_start, ref_start);
+        for (uint32_t k = 1; k <= lim; k++) {
+            const int32_t q = read_code(pv.read[read_start - k]);
+            const int32_t r = win_code(pv.win[ref_start - k]);
+            if (!neutral_column(q, r)) break;
+            delta += lut_score(P, q, r);
+            if (delta == 0) nl = k;
+        }
+        const uint32_t ref_end = ref_start + tb.nm + tb.nd;
+        const uint32_t avail = min((uint32_t)pv.m - read_end, (uint32_t)pv.n - ref_end);
+        delta = 0;
+        for (uint32_t k = 0; k < avail; k++) {
+            const int32_t q = read_code(pv.read[read_end + k]);
+            const int32_t r = win_code(pv.win[ref_end + k]);
+            if (!neutral_column(q, r)) break;
+            delta += lut_score(P, q, r);
+            if (delta == 0) nr = k + 1;
+        }
+        read_start -= nl;
+        ref_start -= nl;
+        read_end += nr;
+    }
+    *n_ops = assemble_cigar(slot, cap, tb.nrev, (uint32_t)pv.m, read_start, nl, read_end, nr);
+    ext->read_start = read_start;
+    ext->read_end = read_end;
+    ext->ref_start = ref_start;
+    ext->score = best;
+}
+
+// Work-queue pop by lane 0 of the wave, broadcast to all lanes.
+__device__ __forceinline__ uint32_t wave_pop(uint32_t *counter, uint32_t lane, uint32_t step = 64u) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(counter, step);
+    return __shfl(base, 0);
 }
 
 }  // namespace espd

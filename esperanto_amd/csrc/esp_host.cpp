@@ -71,8 +71,8 @@ struct esp_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
-    DevBuf counter, rows, trace, stage, stage_out;
-    HostBuf hstage, hout;
+    DevBuf counter, rows, trace, stage, stage_out, plan, order;
+    HostBuf hstage, hout, hplan;
     double last_ms = 0.0;
     int timed = 0;
 };
@@ -129,13 +129,32 @@ int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Launch geometry of the persistent Gotoh grid for a batch whose largest
-// pair has maxm rows and maxw trace words.
-int gotoh_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t maxm, uint64_t maxw) {
+// Substitution rows of the band kernel: for read code rc = 0..4, the
+// 5-bit signed LUT score of window code f = 0..5 at bit 5*f (window code 4
+// never occurs; 5 is any non-ACGT byte).  Mirrors SubstLut (extend.rs:161-186).
+bool band_lut_fits(const esp_params *p) {
+    return p->match_score >= -16 && p->match_score <= 15 && p->mismatch >= -16 &&
+           p->mismatch <= 15;
+}
+
+void band_lut(const esp_params *p, uint32_t out[5]) {
+    for (int rc = 0; rc < 5; rc++) {
+        const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;
+        uint32_t w = 0;
+        for (int f = 0; f < 6; f++) {
+            int s = (rc < 4 && f == rc) ? p->match_score : (f == partner ? 0 : p->mismatch);
+            w |= ((uint32_t)s & 31u) << (5 * f);
+        }
+        out[rc] = w;
+    }
+}
+
+// Launch geometry of the persistent strip grid for `count` pairs whose
+// largest has maxm rows and maxw trace words.
+int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t maxm, uint64_t maxw) {
     const int per_cu = espk::gotoh_strip_blocks_per_cu(a.P.edit);
     uint64_t lanes = (uint64_t)c->ncu * per_cu * espk::GOTOH_BLOCK;
-    const uint64_t need_pairs = align_up(a.npairs, 64);
-    lanes = std::min<uint64_t>(lanes, need_pairs);
+    lanes = std::min<uint64_t>(lanes, align_up(count, 64));
     const uint64_t tw = std::max<uint64_t>(maxw, 1), rw = 5 * std::max<uint64_t>(maxm, 1);
     lanes = std::min<uint64_t>(lanes, kTraceBudget / (tw * 4));
     lanes = std::min<uint64_t>(lanes, kRowBudget / (rw * 4));
@@ -143,16 +162,11 @@ int gotoh_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t maxm, uint64_t maxw) {
     a.nl = (uint32_t)lanes;
     a.maxm = (uint32_t)std::max<uint64_t>(maxm, 1);
     a.maxw = (uint32_t)tw;
-    HIP_TRY(c, c->counter.ensure(256));
     HIP_TRY(c, c->rows.ensure(rw * lanes * 4));
     HIP_TRY(c, c->trace.ensure(tw * lanes * 4));
-    a.counter = (uint32_t *)c->counter.p;
     a.rows = (int32_t *)c->rows.p;
     a.trace = (uint32_t *)c->trace.p;
-    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
-    if (c->timed) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     HIP_TRY(c, espk::launch_gotoh_strip(a, (int)(lanes / espk::GOTOH_BLOCK), c->stream));
-    if (c->timed) HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
 }
 
@@ -304,6 +318,9 @@ int esp_destroy(esp_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->counter.release();
+    c->plan.release();
+    c->order.release();
+    c->hplan.release();
     c->rows.release();
     c->trace.release();
     c->stage.release();
@@ -347,28 +364,19 @@ int esp_band(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, size
 
 int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
                     const esp_params *p, int mode, esp_batch_out *o) {
+    (void)meta;  // the device planner reads the lengths itself
     if (!c) return ESP_E_INVAL;
     int rc = check_params(c, p);
     if (rc) return rc;
-    if (!b || !o || !meta || !meta->read_len || !meta->win_len)
-        return fail(c, ESP_E_INVAL, "batch, output and host lengths are required");
+    if (!b || !o) return fail(c, ESP_E_INVAL, "batch and output are required");
     if (mode < ESP_MODE_BAND || mode > ESP_MODE_HINT) return fail(c, ESP_E_INVAL, "bad mode");
     if (b->npairs == 0) return ESP_OK;
     if (b->npairs > 0xFFFFFF00ull) return fail(c, ESP_E_INVAL, "too many pairs");
     if ((mode != ESP_MODE_FULL && !b->offset) || (mode == ESP_MODE_BAND && !b->half))
         return fail(c, ESP_E_INVAL, "offset/half arrays required for this mode");
-    uint64_t maxm = 0, maxw = 0;
-    for (size_t k = 0; k < b->npairs; k++) {
-        const uint64_t m = meta->read_len[k], n = meta->win_len[k];
-        rc = check_lengths(c, m, n);
-        if (rc) return rc;
-        maxm = std::max(maxm, m);
-        maxw = std::max(maxw, espk::strip_trace_words(m, n));
-    }
     HIP_TRY(c, hipSetDevice(c->device));
     espk::GotohArgs a{};
     a.npairs = (uint32_t)b->npairs;
-    a.order = nullptr;
     a.reads = b->reads;
     a.read_off = b->read_off;
     a.read_len = b->read_len;
@@ -379,16 +387,65 @@ int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
     a.half = b->half;
     a.mode = mode;
     a.P = dev_params(p);
+    band_lut(p, a.lut);
     a.ext = o->ext;
     a.touch = o->touch;
     a.cigar = o->cigar;
     a.cig_off = o->cig_off;
     a.cig_n = o->cig_n;
     a.aux = o->aux;
-    c->timed = 1;
-    rc = gotoh_launch(c, a, maxm, maxw);
-    c->timed = 0;
-    return rc;
+
+    // 1. Device planner: classify, bucket by (class, read length), order.
+    HIP_TRY(c, c->plan.ensure(sizeof(espk::PlanDev)));
+    HIP_TRY(c, c->order.ensure(b->npairs * 4));
+    HIP_TRY(c, c->hplan.ensure(sizeof(espk::PlanDev)));
+    espk::PlanArgs pa{};
+    pa.npairs = a.npairs;
+    pa.read_len = b->read_len;
+    pa.win_len = b->win_len;
+    pa.offset = b->offset;
+    pa.half = b->half;
+    pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
+    pa.max_read = kMaxRead;
+    pa.max_win = (uint32_t)kMaxWin;
+    pa.plan = (espk::PlanDev *)c->plan.p;
+    pa.order = (uint32_t *)c->order.p;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanDev), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const espk::PlanDev *hp = (const espk::PlanDev *)c->hplan.p;
+    if (hp->bad) return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+    a.order = pa.order;
+    espk::PlanDev *dp = pa.plan;
+
+    // 2. Band kernel per populated half; 3. strip kernel for the rest.
+    for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+        if (hp->range[cls][1] == 0) continue;
+        const int half = espk::plan_class_half(cls);
+        const uint64_t W = 2 * (uint64_t)half + 1;
+        const uint64_t maxw = espk::band_trace_words(espk::BAND_SPAN - W, half);
+        const int per_cu = espk::gotoh_band_blocks_per_cu(half, 0);
+        uint64_t blocks = (uint64_t)c->ncu * per_cu;
+        blocks = std::min<uint64_t>(blocks, (hp->range[cls][1] + 63) / 64);
+        blocks = std::min<uint64_t>(blocks, kTraceBudget / (maxw * 64 * 4));
+        blocks = std::max<uint64_t>(blocks, 1);
+        HIP_TRY(c, c->trace.ensure(maxw * 64 * 4 * blocks));
+        a.trace = (uint32_t *)c->trace.p;
+        a.maxw = (uint32_t)maxw;
+        a.range = &dp->range[cls][0];
+        a.counter = &dp->counters[cls];
+        HIP_TRY(c, espk::launch_gotoh_band(a, half, (int)blocks, c->stream));
+    }
+    if (hp->range[espk::PLAN_STRIP][1]) {
+        a.range = &dp->range[espk::PLAN_STRIP][0];
+        a.counter = &dp->counters[espk::PLAN_STRIP];
+        rc = strip_launch(c, a, hp->range[espk::PLAN_STRIP][1], hp->strip_maxm, hp->strip_maxw);
+        if (rc) return rc;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
 }
 
 int esp_last_kernel_ms(esp_ctx *c, double *ms) {

@@ -15,25 +15,13 @@
 // until it is empty, so every wave reaches the exit.
 #include "esp_kernels.hpp"
 
+#include <algorithm>
+
 using namespace espd;
 
 namespace espk {
 
 namespace {
-
-struct PairView {
-    const uint8_t *read;
-    const uint8_t *win;
-    int32_t m, n, off, half;
-};
-
-__device__ __forceinline__ int32_t clamp_off(int64_t off, int32_t m, int32_t n) {
-    int64_t lo = -(int64_t)m - 1, hi = n;
-    return (int32_t)(off < lo ? lo : (off > hi ? hi : off));
-}
-__device__ __forceinline__ int32_t clamp_half(int64_t half, int32_t n) {
-    return (int32_t)(half < 0 ? 0 : (half > n ? n : half));
-}
 
 struct FillResult {
     int32_t best, bi, bj;
@@ -186,10 +174,6 @@ __device__ FillResult fill_strips(const PairView &pv, const GParams &P, const Sl
     return r;
 }
 
-struct TbResult {
-    uint32_t read_start, ref_start, nm, ni, nd, nrev;
-};
-
 // State-machine traceback (extend.rs:884-930) over the strip trace layout.
 // Runs are written in walk order to slot[cap-1], slot[cap-2], ...
 template <int T>
@@ -256,54 +240,6 @@ __device__ TbResult traceback_strips(const PairView &pv, const Slot &S, int32_t 
     t.nd = nd;
     t.nrev = nrev;
     return t;
-}
-
-// End trimming and score rebuild (extend.rs:932-990) reduce to the identity
-// here: with gap_open, gap_ext >= 0 every prefix of the walked path ends on
-// a cell whose H (or gap value) is positive, and every proper prefix stays
-// below the first row-major maximum, so neither trim fires and the score is
-// the best cell's H.  Neutral flanks and soft clips follow extend.rs:991-1059.
-__device__ void finish_pair(const PairView &pv, const GParams &P, int32_t best,
-                            const TbResult &tb, uint32_t *slot, uint32_t cap, esp_ext *ext,
-                            uint32_t *n_ops) {
-    uint32_t read_start = tb.read_start, ref_start = tb.ref_start;
-    uint32_t read_end = read_start + tb.nm + tb.ni;
-    uint32_t nl = 0, nr = 0;
-    if (best > 0) {
-        int32_t delta = 0;
-        const uint32_t lim = min(read_start, ref_start);
-        for (uint32_t k = 1; k <= lim; k++) {
-            const int32_t q = read_code(pv.read[read_start - k]);
-            const int32_t r = win_code(pv.win[ref_start - k]);
-            if (!neutral_column(q, r)) break;
-            delta += lut_score(P, q, r);
-            if (delta == 0) nl = k;
-        }
-        const uint32_t ref_end = ref_start + tb.nm + tb.nd;
-        const uint32_t avail = min((uint32_t)pv.m - read_end, (uint32_t)pv.n - ref_end);
-        delta = 0;
-        for (uint32_t k = 0; k < avail; k++) {
-            const int32_t q = read_code(pv.read[read_end + k]);
-            const int32_t r = win_code(pv.win[ref_end + k]);
-            if (!neutral_column(q, r)) break;
-            delta += lut_score(P, q, r);
-            if (delta == 0) nr = k + 1;
-        }
-        read_start -= nl;
-        ref_start -= nl;
-        read_end += nr;
-    }
-    *n_ops = assemble_cigar(slot, cap, tb.nrev, (uint32_t)pv.m, read_start, nl, read_end, nr);
-    ext->read_start = read_start;
-    ext->read_end = read_end;
-    ext->ref_start = ref_start;
-    ext->score = best;
-}
-
-__device__ __forceinline__ uint32_t wave_pop(uint32_t *counter, uint32_t lane) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(counter, 64u);
-    return __shfl(base, 0);
 }
 
 template <int T, bool EDIT>
@@ -382,15 +318,117 @@ __global__ __launch_bounds__(GOTOH_BLOCK) void k_gotoh_strip(GotohArgs a) {
     S.maxm = a.maxm;
     S.nl = a.nl;
     S.slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t first = a.range ? a.range[0] : 0u;
+    const uint32_t count = a.range ? a.range[1] : a.npairs;
     for (;;) {
         const uint32_t base = wave_pop(a.counter, lane);
-        if (base >= a.npairs) break;
+        if (base >= count) break;
         const uint32_t idx = base + lane;
-        if (idx < a.npairs) {
-            const uint32_t p = a.order ? a.order[idx] : idx;
+        if (idx < count) {
+            const uint32_t p = a.order ? a.order[first + idx] : idx;
             gotoh_one<T, EDIT>(a, p, S);
         }
     }
+}
+
+// ----------------------------------------------------------------- planner
+
+constexpr int kPlanHalves[5] = {16, 30, 32, 48, 64};
+
+__device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint32_t &cls) {
+    const uint32_t m = a.read_len[p], n = a.win_len[p];
+    cls = PLAN_STRIP;
+    if (a.band_ok && m >= 1) {
+        const int64_t off = a.offset[p], half = a.half[p];
+        int c = -1;
+#pragma unroll
+        for (int t = 0; t < 5; t++)
+            if (half == kPlanHalves[t]) c = t;
+        if (c >= 0 && off >= half && off + (int64_t)m + half <= (int64_t)n &&
+            (int64_t)m + 2 * half + 1 <= BAND_SPAN)
+            cls = (uint32_t)c;
+    }
+    return cls * PLAN_NBIN + (PLAN_NBIN - 1 - min(m, (uint32_t)PLAN_NBIN - 1));
+}
+
+__global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
+    __shared__ uint32_t h[PLAN_NCLASS * PLAN_NBIN];
+    for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x) h[t] = 0;
+    __syncthreads();
+    uint32_t smax = 0;
+    uint64_t wmax = 0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npairs; p += gridDim.x * blockDim.x) {
+        uint32_t cls;
+        const uint32_t b = plan_bin(a, p, cls);
+        atomicAdd(&h[b], 1u);
+        if (a.read_len[p] > a.max_read || a.win_len[p] > a.max_win) atomicAdd(&a.plan->bad, 1u);
+        if (cls == PLAN_STRIP) {
+            const uint32_t m = a.read_len[p];
+            smax = max(smax, m);
+            wmax = max(wmax, strip_trace_words(m, a.win_len[p]));
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x)
+        if (h[t]) atomicAdd(&a.plan->hist[t], h[t]);
+    atomicMax(&a.plan->strip_maxm, smax);
+    atomicMax(&a.plan->strip_maxw, (unsigned long long)wmax);
+}
+
+// Exclusive scan of the bins (one block), class ranges, cursors.
+__global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs a) {
+    __shared__ uint32_t part[1024];
+    constexpr int NB = PLAN_NCLASS * PLAN_NBIN;
+    constexpr int PER = (NB + 1023) / 1024;
+    const int t = threadIdx.x;
+    uint32_t local[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int b = t * PER + k;
+        local[k] = b < NB ? a.plan->hist[b] : 0u;
+        sum += local[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int b = t * PER + k;
+        if (b < NB) {
+            a.plan->cursor[b] = run;
+            if (b % PLAN_NBIN == 0) a.plan->range[b / PLAN_NBIN][0] = run;
+        }
+        run += local[k];
+    }
+    __syncthreads();
+    if (t < PLAN_NCLASS) {
+        uint32_t c = 0;
+        for (int b = 0; b < PLAN_NBIN; b++) c += a.plan->hist[t * PLAN_NBIN + b];
+        a.plan->range[t][1] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_plan_scatter(PlanArgs a) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npairs; p += gridDim.x * blockDim.x) {
+        uint32_t cls;
+        const uint32_t b = plan_bin(a, p, cls);
+        const uint32_t pos = atomicAdd(&a.plan->cursor[b], 1u);
+        a.order[pos] = p;
+    }
+}
+
+__global__ void k_zero_plan(PlanDev *d) {
+    uint32_t *w = (uint32_t *)d;
+    const size_t n = sizeof(PlanDev) / 4;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        w[i] = 0;
 }
 
 // ----------------------------------------------------------------- Myers
@@ -544,6 +582,17 @@ hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s) {
     }
     return hipGetLastError();
 }
+
+hipError_t launch_plan(const PlanArgs &a, int ncu, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_plan, dim3(8), dim3(256), 0, s, a.plan);
+    const int blocks = (int)std::min<uint64_t>((uint64_t)ncu * 4, ((uint64_t)a.npairs + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_plan_hist, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_plan_scatter, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+int plan_class_half(int c) { return c >= 0 && c < 5 ? kPlanHalves[c] : -1; }
 
 hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s) {
     const size_t lds = (size_t)(MYERS_BLOCK / 64) * 5 * words * 64 * sizeof(uint64_t);
