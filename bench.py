@@ -53,19 +53,33 @@ def band_cells(hb) -> int:
 
 
 def timed_steps(fn_launch, db, steps: int, warmup: int):
+    """Runs warmup + timed steps; returns (wall s/step, device ms/step, per-launch
+    timings of every timed step)."""
     for _ in range(warmup):
         fn_launch()
         db.sync()
     db.sync()
-    kms = []
+    kms, tims = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         fn_launch()
         db.sync()
         kms.append(db.last_kernel_ms())
+        tims.append(db.last_timings())
     db.sync()
     wall = (time.perf_counter() - t0) / steps
-    return wall, float(np.mean(kms))
+    return wall, float(np.mean(kms)), tims
+
+
+def read_traffic(path: str):
+    """HBM bytes per band-kernel launch from the committed PMC summary, if any."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def main() -> int:
@@ -78,6 +92,10 @@ def main() -> int:
     ap.add_argument("--plain-calls", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-gotoh", action="store_true",
+                    help="time only the batched Gotoh workload (profiling runs)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -97,18 +115,45 @@ def main() -> int:
     cells = band_cells(hb)
     log(f"cells={cells} uploading")
     db = DeviceBatch(hb, buf)
-    wall, kms = timed_steps(lambda: db.launch_gotoh(params, 0), db, args.steps, args.warmup)
+    wall, kms, tims = timed_steps(lambda: db.launch_gotoh(params, 0), db, args.steps, args.warmup)
     gcups = cells / wall / 1e9
     kernel_gcups = cells / (kms / 1e3) / 1e9
-    achieved_tops = cells * GOTOH_OPS_PER_CELL / (kms / 1e3) / 1e12
-    log(f"gotoh batched: {wall*1e3:.2f} ms/step, kernel {kms:.2f} ms, {gcups:.1f} GCUPS")
+    # Dominant kernel: the band kernel (one launch per band half per step).
+    band_ms = float(np.mean([sum(t[2] for t in st if t[0] == 1) for st in tims]))
+    plan_ms = float(np.mean([sum(t[2] for t in st if t[0] == 0) for st in tims]))
+    strip_ms = float(np.mean([sum(t[2] for t in st if t[0] == 2) for st in tims]))
+    per_half = {}
+    for st in tims:
+        for kind, half, ms in st:
+            if kind == 1:
+                per_half.setdefault(half, []).append(ms)
+    per_half = {int(h): round(float(np.mean(v)), 3) for h, v in sorted(per_half.items())}
+    achieved_tops = cells * GOTOH_OPS_PER_CELL / (band_ms / 1e3) / 1e12
+    log(f"gotoh batched: {wall*1e3:.2f} ms/step, device {kms:.2f} ms (plan {plan_ms:.3f}, "
+        f"band {band_ms:.2f} {per_half}, strip {strip_ms:.3f}), {gcups:.1f} GCUPS")
     pairs_per_s = hb.npairs / wall
+    if args.copy_bytes:
+        from esperanto_amd import _native
+        import ctypes
+        src = db.d["wins"]
+        nbytes = min(args.copy_bytes, src.nbytes)
+        from esperanto_amd.batch import DeviceArray
+        dst = DeviceArray(buf, nbytes)
+        _native.check(_native.lib().esp_device_copy(buf.handle, ctypes.c_void_p(dst.ptr),
+                                                    ctypes.c_void_p(src.ptr), nbytes), buf.handle)
+        log(f"copy calibration: {nbytes} bytes")
+        dst.free()
     db.free()
+    if args.only_gotoh:
+        print(json.dumps({"gcups": round(gcups, 2), "band_ms": round(band_ms, 3),
+                          "per_half_ms": per_half, "plan_ms": round(plan_ms, 3)}), flush=True)
+        buf.close()
+        return 0
 
     # C1 batched and plain forms.
     c1 = workload_c1(10_000)
     d1 = DeviceBatch(c1, buf)
-    w1, k1 = timed_steps(lambda: d1.launch_gotoh(params, 0), d1, max(args.steps, 3), 1)
+    w1, k1, _ = timed_steps(lambda: d1.launch_gotoh(params, 0), d1, max(args.steps, 3), 1)
     c1_batched_pps = c1.npairs / w1
     d1.free()
     ncalls = min(args.plain_calls, c1.npairs)
@@ -123,7 +168,7 @@ def main() -> int:
     # Myers batched.
     c2 = workload_c2(args.myers_pairs)
     d2 = DeviceBatch(c2, buf)
-    w2, k2 = timed_steps(lambda: d2.launch_myers(False), d2, max(args.steps, 3), 1)
+    w2, k2, _ = timed_steps(lambda: d2.launch_myers(False), d2, max(args.steps, 3), 1)
     myers_pps = c2.npairs / w2
     words = 3
     myers_ops = c2.npairs * 214 * (MYERS_OPS_PER_WORD_COL * words + 3)
@@ -168,8 +213,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "batched uneven banded Gotoh+traceback: 2,000,000 pairs, reads "
                                "U{50..250}, band U{16,32,48,64}, window=read+2*band",
-                   "pairs": hb.npairs, "cells": cells, "kernel": "k_gotoh_strip",
+                   "pairs": hb.npairs, "cells": cells, "kernel": "k_gotoh_band",
                    "pairs_per_s": round(pairs_per_s),
+                   "device_ms": round(kms, 3), "planner_ms": round(plan_ms, 3),
+                   "band_kernel_ms": round(band_ms, 3), "band_kernel_ms_per_half": per_half,
                    "kernel_gcups": round(kernel_gcups, 2),
                    "c1_batched_pairs_per_s": round(c1_batched_pps),
                    "c1_plain_pairs_per_s": round(plain_pps),
@@ -179,7 +226,9 @@ def main() -> int:
                    "myers_valu_tops": round(myers_tops, 2)},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-                     "frac": round(achieved_tops / VALU_PEAK_TOPS, 4), "traffic": None},
+                     "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),
+                     "traffic": read_traffic(os.path.join(ROOT, "profiles",
+                                                          "gotoh_band_uneven_2M_traffic.txt"))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -21,6 +21,7 @@ EXPORTS = [
     "esp_band", "esp_myers_best_end", "esp_myers_best_start", "esp_cigar_capacity",
     "esp_gotoh_batch", "esp_myers_batch", "esp_last_kernel_ms", "esp_device_alloc",
     "esp_device_free", "esp_copy_to_device", "esp_copy_to_host", "esp_stitch_cigar",
+    "esp_last_timings", "esp_device_copy",
 ]
 
 
@@ -41,6 +42,10 @@ class CBatch(ctypes.Structure):
 class CBatchOut(ctypes.Structure):
     _fields_ = [("ext", ctypes.c_void_p), ("touch", ctypes.c_void_p), ("cigar", ctypes.c_void_p),
                 ("cig_off", ctypes.c_void_p), ("cig_n", ctypes.c_void_p), ("aux", ctypes.c_void_p)]
+
+
+class CTiming(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("half", ctypes.c_int32), ("ms", ctypes.c_double)]
 
 
 class CHostMeta(ctypes.Structure):
@@ -79,6 +84,8 @@ def lib():
                                   P(CBatchOut)]
     L.esp_myers_batch.argtypes = [vp, P(CBatch), P(CHostMeta), vp, vp, vp]
     L.esp_last_kernel_ms.argtypes = [vp, P(ctypes.c_double)]
+    L.esp_last_timings.argtypes = [vp, P(CTiming), sz, P(sz)]
+    L.esp_device_copy.argtypes = [vp, vp, vp, sz]
     L.esp_device_alloc.argtypes = [vp, sz, P(vp)]
     L.esp_device_free.argtypes = [vp, vp]
     L.esp_copy_to_device.argtypes = [vp, vp, vp, sz]

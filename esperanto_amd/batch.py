@@ -245,6 +245,14 @@ class DeviceBatch:
                       self.buf.handle)
         return ms.value
 
+    def last_timings(self):
+        """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip, 3 Myers."""
+        arr = (_native.CTiming * 16)()
+        n = ctypes.c_size_t()
+        _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),
+                      self.buf.handle)
+        return [(arr[k].kind, arr[k].half, arr[k].ms) for k in range(min(n.value, 16))]
+
     # ---- results ----
     def gotoh_results(self):
         """(ext[n,4] int64, touch[n,4] int64, cigar words, cig_off, cig_n, aux)."""

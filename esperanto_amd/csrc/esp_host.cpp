@@ -71,10 +71,15 @@ struct esp_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
-    DevBuf counter, rows, trace, stage, stage_out, plan, order;
+    DevBuf counter, rows, trace, btrace, stage, stage_out, plan, order;
+    hipEvent_t ev_plan = nullptr;
+    int band_blocks[espk::PLAN_STRIP] = {};
     HostBuf hstage, hout, hplan;
-    double last_ms = 0.0;
-    int timed = 0;
+    // Per-launch timing of the last batch call.
+    static constexpr int kMaxEv = 16;
+    hipEvent_t evs[kMaxEv + 1] = {};
+    esp_timing tim[kMaxEv] = {};
+    int nev = 0;
 };
 
 namespace {
@@ -129,12 +134,30 @@ int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Launch timing: mark() before the first launch and after every launch of
+// a batch call; esp_last_timings() reports the differences.
+void tim_begin(esp_ctx *c) {
+    c->nev = 0;
+    (void)hipEventRecord(c->evs[0], c->stream);
+}
+void tim_mark(esp_ctx *c, int kind, int half) {
+    if (c->nev >= esp_ctx::kMaxEv) return;
+    c->tim[c->nev].kind = kind;
+    c->tim[c->nev].half = half;
+    c->nev++;
+    (void)hipEventRecord(c->evs[c->nev], c->stream);
+}
+
 // Substitution rows of the band kernel: for read code rc = 0..4, the
 // 5-bit signed LUT score of window code f = 0..5 at bit 5*f (window code 4
 // never occurs; 5 is any non-ACGT byte).  Mirrors SubstLut (extend.rs:161-186).
+// The band kernel's scores are 5-bit LUT fields, and its wide-band
+// specialisations keep U = H + open_ext and E + ext as int16: with reads of
+// at most band_span(64) bases and |score| <= 16, every such value lies in
+// (-20000, 16384) when gap_open + gap_ext <= 8000.
 bool band_lut_fits(const esp_params *p) {
     return p->match_score >= -16 && p->match_score <= 15 && p->mismatch >= -16 &&
-           p->mismatch <= 15;
+           p->mismatch <= 15 && p->gap_open + p->gap_ext <= 8000;
 }
 
 void band_lut(const esp_params *p, uint32_t out[5]) {
@@ -305,9 +328,16 @@ int esp_create(int device, esp_ctx **out) {
     }
     c->ncu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return ESP_E_DEVICE;
+    }
+    for (int k = 0; k <= esp_ctx::kMaxEv; k++) {
+        if (hipEventCreate(&c->evs[k]) != hipSuccess) {
+            delete c;
+            return ESP_E_DEVICE;
+        }
     }
     *out = c;
     return ESP_OK;
@@ -323,10 +353,14 @@ int esp_destroy(esp_ctx *c) {
     c->hplan.release();
     c->rows.release();
     c->trace.release();
+    c->btrace.release();
+    if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
     c->stage.release();
     c->stage_out.release();
     c->hstage.release();
     c->hout.release();
+    for (int k = 0; k <= esp_ctx::kMaxEv; k++)
+        if (c->evs[k]) (void)hipEventDestroy(c->evs[k]);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -398,7 +432,7 @@ int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
     // 1. Device planner: classify, bucket by (class, read length), order.
     HIP_TRY(c, c->plan.ensure(sizeof(espk::PlanDev)));
     HIP_TRY(c, c->order.ensure(b->npairs * 4));
-    HIP_TRY(c, c->hplan.ensure(sizeof(espk::PlanDev)));
+    HIP_TRY(c, c->hplan.ensure(sizeof(espk::PlanSummary)));
     espk::PlanArgs pa{};
     pa.npairs = a.npairs;
     pa.read_len = b->read_len;
@@ -411,38 +445,48 @@ int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
     pa.plan = (espk::PlanDev *)c->plan.p;
     pa.order = (uint32_t *)c->order.p;
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
     HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanDev), hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const espk::PlanDev *hp = (const espk::PlanDev *)c->hplan.p;
-    if (hp->bad) return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+    tim_mark(c, 0, 0);
+    // The plan summary travels back while the band kernels run: they read
+    // their class ranges from device memory and need nothing from the host.
+    HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanSummary),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_plan, c->stream));
     a.order = pa.order;
     espk::PlanDev *dp = pa.plan;
 
-    // 2. Band kernel per populated half; 3. strip kernel for the rest.
-    for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
-        if (hp->range[cls][1] == 0) continue;
-        const int half = espk::plan_class_half(cls);
-        const uint64_t W = 2 * (uint64_t)half + 1;
-        const uint64_t maxw = espk::band_trace_words(espk::BAND_SPAN - W, half);
-        const int per_cu = espk::gotoh_band_blocks_per_cu(half, 0);
-        uint64_t blocks = (uint64_t)c->ncu * per_cu;
-        blocks = std::min<uint64_t>(blocks, (hp->range[cls][1] + 63) / 64);
-        blocks = std::min<uint64_t>(blocks, kTraceBudget / (maxw * 64 * 4));
-        blocks = std::max<uint64_t>(blocks, 1);
-        HIP_TRY(c, c->trace.ensure(maxw * 64 * 4 * blocks));
-        a.trace = (uint32_t *)c->trace.p;
-        a.maxw = (uint32_t)maxw;
-        a.range = &dp->range[cls][0];
-        a.counter = &dp->counters[cls];
-        HIP_TRY(c, espk::launch_gotoh_band(a, half, (int)blocks, c->stream));
+    // 2. Band kernel per half specialisation (an empty class exits at once).
+    if (pa.band_ok) {
+        for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+            const int half = espk::plan_class_half(cls);
+            const uint64_t W = 2 * (uint64_t)half + 1;
+            const uint64_t maxw = espk::band_trace_words(espk::band_span(half) - W, half);
+            if (!c->band_blocks[cls]) c->band_blocks[cls] = espk::gotoh_band_blocks_per_cu(half);
+            uint64_t blocks = (uint64_t)c->ncu * c->band_blocks[cls];
+            blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
+            blocks = std::min<uint64_t>(blocks, kTraceBudget / (maxw * 64 * 4));
+            blocks = std::max<uint64_t>(blocks, 1);
+            HIP_TRY(c, c->btrace.ensure(maxw * 64 * 4 * blocks));
+            a.trace = (uint32_t *)c->btrace.p;
+            a.maxw = (uint32_t)maxw;
+            a.range = &dp->sum.range[cls][0];
+            a.counter = &dp->counters[cls];
+            HIP_TRY(c, espk::launch_gotoh_band(a, half, (int)blocks, c->stream));
+            tim_mark(c, 1, half);
+        }
     }
+
+    // 3. Strip kernel for everything else, sized from the plan summary.
+    HIP_TRY(c, hipEventSynchronize(c->ev_plan));
+    const espk::PlanSummary *hp = (const espk::PlanSummary *)c->hplan.p;
+    if (hp->bad) return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
     if (hp->range[espk::PLAN_STRIP][1]) {
-        a.range = &dp->range[espk::PLAN_STRIP][0];
+        a.range = &dp->sum.range[espk::PLAN_STRIP][0];
         a.counter = &dp->counters[espk::PLAN_STRIP];
         rc = strip_launch(c, a, hp->range[espk::PLAN_STRIP][1], hp->strip_maxm, hp->strip_maxw);
         if (rc) return rc;
+        tim_mark(c, 2, 0);
     }
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
@@ -454,6 +498,28 @@ int esp_last_kernel_ms(esp_ctx *c, double *ms) {
     HIP_TRY(c, hipEventSynchronize(c->ev1));
     HIP_TRY(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
     *ms = f;
+    return ESP_OK;
+}
+
+int esp_last_timings(esp_ctx *c, esp_timing *out, size_t cap, size_t *n) {
+    if (!c || !n) return ESP_E_INVAL;
+    *n = (size_t)c->nev;
+    if (c->nev == 0) return ESP_OK;
+    HIP_TRY(c, hipEventSynchronize(c->evs[c->nev]));
+    for (int k = 0; k < c->nev && (size_t)k < cap; k++) {
+        float f = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&f, c->evs[k], c->evs[k + 1]));
+        out[k] = c->tim[k];
+        out[k].ms = f;
+    }
+    return ESP_OK;
+}
+
+int esp_device_copy(esp_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return ESP_E_INVAL;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, espk::launch_copy(dst, src, bytes, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ESP_OK;
 }
 
@@ -491,7 +557,9 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     blocks = std::max<uint64_t>(blocks, 1);
     HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
     HIP_TRY(c, espk::launch_myers(a, words, (int)blocks, c->stream));
+    tim_mark(c, 3, 0);
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
 }

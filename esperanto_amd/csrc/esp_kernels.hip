@@ -345,7 +345,7 @@ __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint
         for (int t = 0; t < 5; t++)
             if (half == kPlanHalves[t]) c = t;
         if (c >= 0 && off >= half && off + (int64_t)m + half <= (int64_t)n &&
-            (int64_t)m + 2 * half + 1 <= BAND_SPAN)
+            (int64_t)m + 2 * half + 1 <= band_span((int)half))
             cls = (uint32_t)c;
     }
     return cls * PLAN_NBIN + (PLAN_NBIN - 1 - min(m, (uint32_t)PLAN_NBIN - 1));
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
         uint32_t cls;
         const uint32_t b = plan_bin(a, p, cls);
         atomicAdd(&h[b], 1u);
-        if (a.read_len[p] > a.max_read || a.win_len[p] > a.max_win) atomicAdd(&a.plan->bad, 1u);
+        if (a.read_len[p] > a.max_read || a.win_len[p] > a.max_win) atomicAdd(&a.plan->sum.bad, 1u);
         if (cls == PLAN_STRIP) {
             const uint32_t m = a.read_len[p];
             smax = max(smax, m);
@@ -371,8 +371,8 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
     __syncthreads();
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x)
         if (h[t]) atomicAdd(&a.plan->hist[t], h[t]);
-    atomicMax(&a.plan->strip_maxm, smax);
-    atomicMax(&a.plan->strip_maxw, (unsigned long long)wmax);
+    atomicMax(&a.plan->sum.strip_maxm, smax);
+    atomicMax(&a.plan->sum.strip_maxw, (unsigned long long)wmax);
 }
 
 // Exclusive scan of the bins (one block), class ranges, cursors.
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs a) {
         const int b = t * PER + k;
         if (b < NB) {
             a.plan->cursor[b] = run;
-            if (b % PLAN_NBIN == 0) a.plan->range[b / PLAN_NBIN][0] = run;
+            if (b % PLAN_NBIN == 0) a.plan->sum.range[b / PLAN_NBIN][0] = run;
         }
         run += local[k];
     }
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs a) {
     if (t < PLAN_NCLASS) {
         uint32_t c = 0;
         for (int b = 0; b < PLAN_NBIN; b++) c += a.plan->hist[t * PLAN_NBIN + b];
-        a.plan->range[t][1] = c;
+        a.plan->sum.range[t][1] = c;
     }
 }
 
@@ -572,7 +572,27 @@ __global__ void k_zero_u32(uint32_t *p, size_t n) {
     if (i < n) p[i] = 0;
 }
 
+__global__ __launch_bounds__(256) void k_copy16(uint4 *dst, const uint4 *src, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_copy1(uint8_t *dst, const uint8_t *src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    if (n16) hipLaunchKernelGGL(k_copy16, dim3(2048), dim3(256), 0, s, (uint4 *)dst, (const uint4 *)src, n16);
+    const size_t tail = bytes - n16 * 16;
+    if (tail)
+        hipLaunchKernelGGL(k_copy1, dim3(1), dim3(64), 0, s, (uint8_t *)dst + n16 * 16,
+                           (const uint8_t *)src + n16 * 16, tail);
+    return hipGetLastError();
+}
 
 hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s) {
     if (a.P.edit) {

@@ -64,14 +64,17 @@ struct MyersArgs {
 constexpr int PLAN_NCLASS = 6;  // halves 16, 30, 32, 48, 64, then the strip class
 constexpr int PLAN_STRIP = 5;
 constexpr int PLAN_NBIN = 512;
-struct PlanDev {
-    uint32_t hist[PLAN_NCLASS * PLAN_NBIN];
-    uint32_t cursor[PLAN_NCLASS * PLAN_NBIN];
+struct PlanSummary {
     uint32_t range[PLAN_NCLASS][2];  // {start, count} per class
-    uint32_t counters[PLAN_NCLASS + 2];
     uint32_t strip_maxm;
     uint32_t bad;                   // pairs outside the length limits
     unsigned long long strip_maxw;  // largest strip-layout trace of the strip class (words)
+};
+struct PlanDev {
+    PlanSummary sum;  // first: the host reads back only this part
+    uint32_t counters[PLAN_NCLASS + 2];
+    uint32_t hist[PLAN_NCLASS * PLAN_NBIN];
+    uint32_t cursor[PLAN_NCLASS * PLAN_NBIN];
 };
 struct PlanArgs {
     uint32_t npairs;
@@ -104,13 +107,20 @@ hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s);
 // [1, n]); half must be one of band_halves().  LDS holds the window codes of
 // the unit's 64 pairs: (max m + 2*half) * 64 bytes.
 constexpr int BAND_BLOCK = 64;
-constexpr int BAND_SPAN = 512;  // max m + 2*half handled by the band kernel
+// LDS rows of window codes per wave (64 bytes each): a pair fits when
+// m + 2*half + 1 <= span, i.e. reads up to 255 bases at half 64.  The spans
+// keep LDS at 20-24 KB per wave so that LDS does not cut occupancy below
+// what the register arrays allow.
+__host__ __device__ constexpr int band_span(int half) {
+    return half <= 32 ? 320 : (half <= 48 ? 352 : 384);
+}
 bool band_half_supported(int half);
 hipError_t launch_gotoh_band(const GotohArgs &a, int half, int blocks, hipStream_t s);
-int gotoh_band_blocks_per_cu(int half, uint32_t maxm);
+int gotoh_band_blocks_per_cu(int half);
 __host__ __device__ inline uint64_t band_trace_words(uint64_t m, uint64_t half) { return m * ((2 * half + 1 + 7) / 8); }
 hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 int gotoh_strip_blocks_per_cu(int edit);
 int myers_blocks_per_cu(int words);
 

@@ -155,9 +155,19 @@ int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
 int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     int32_t *dist, uint64_t *end, uint64_t *start);
 
-/* Average device duration (ms) of the kernels of the last batch call,
+/* Device time (ms) of the last batch call (planner plus every kernel),
  * measured with HIP events on the context stream. */
 int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
+
+/* Per-launch device times of the last batch call: kind (0 planner, 1 band
+ * kernel, 2 strip kernel, 3 Myers kernel, 4 host round trip between the
+ * planner and the alignment kernels), half (band kernel) and ms. */
+typedef struct {
+    int32_t kind;
+    int32_t half;
+    double ms;
+} esp_timing;
+int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
 
 /* Device memory helpers for callers without an allocator of their own.
  * Copies are synchronous with respect to the context stream. */
@@ -165,6 +175,9 @@ int esp_device_alloc(esp_ctx *ctx, size_t bytes, void **out);
 int esp_device_free(esp_ctx *ctx, void *p);
 int esp_copy_to_device(esp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int esp_copy_to_host(esp_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Plain device-to-device copy kernel (16-byte vectors); the reference byte
+ * count for calibrating HBM traffic counters. */
+int esp_device_copy(esp_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- splice / mate-rescue glue (host side, no device work) ------------- */
 
