@@ -431,142 +431,6 @@ __global__ void k_zero_plan(PlanDev *d) {
         w[i] = 0;
 }
 
-// ----------------------------------------------------------------- Myers
-//
-// EA-Myers (myers_ea.rs) with W 64-bit words per lane and an edlib-form
-// carry between words.  The pattern-equality table of each lane lives in
-// LDS as peq[code][word][lane] (codes A, C, G, T and "other", which matches
-// nothing), so a text column costs W LDS reads and ~31 VALU ops per word.
-
-struct Peq {
-    uint64_t *base;  // [5][W][64] for this wave
-    uint32_t lane;
-    int W;
-    __device__ __forceinline__ uint64_t &at(int code, int w) const {
-        return base[((size_t)code * W + w) * 64 + lane];
-    }
-};
-
-// ea_matches(read, ref) (myers_ea.rs:28-38): read A={A}, C={C,T}, G={G,A}, T={T}.
-template <int W>
-__device__ void build_peq(const Peq &pq, const uint8_t *read, int32_t m, bool reverse) {
-    uint64_t a[W], c[W], g[W], t[W];
-#pragma unroll
-    for (int w = 0; w < W; w++) a[w] = c[w] = g[w] = t[w] = 0;
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-        for (int b = 0; b < 64; b++) {
-            const int32_t i = w * 64 + b;
-            if (i >= m) break;
-            const int32_t rc = read_code(read[reverse ? (m - 1 - i) : i]);
-            const uint64_t bit = 1ull << b;
-            if (rc == 0) { a[w] |= bit; }
-            if (rc == 1) { c[w] |= bit; t[w] |= bit; }
-            if (rc == 2) { g[w] |= bit; a[w] |= bit; }
-            if (rc == 3) { t[w] |= bit; }
-        }
-    }
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-        pq.at(0, w) = a[w];
-        pq.at(1, w) = c[w];
-        pq.at(2, w) = g[w];
-        pq.at(3, w) = t[w];
-        pq.at(4, w) = 0;
-    }
-}
-
-// Column scan; returns (best, best_at) with the strict-< earliest-end rule
-// of infix_best_end (myers_ea.rs:141-156).
-template <int W>
-__device__ void myers_scan(const Peq &pq, int32_t m, const uint8_t *text, int32_t n, bool reverse,
-                           int32_t &best_out, uint32_t &at_out) {
-    uint64_t pv[W], mv[W];
-    const int lastw = (m - 1) >> 6;
-    const int lastb = (m - 1) & 63;
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-        pv[w] = w < lastw ? ~0ull : (w == lastw ? (lastb == 63 ? ~0ull : ((1ull << (lastb + 1)) - 1)) : 0);
-        mv[w] = 0;
-    }
-    int32_t score = m, best = m;
-    uint32_t best_at = 0;
-    for (int32_t j = 0; j < n; j++) {
-        const int32_t code = read_code(text[reverse ? (n - 1 - j) : j]);
-        int32_t hin = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            if (w <= lastw) {
-                const uint64_t mask = w < lastw ? ~0ull : (lastb == 63 ? ~0ull : ((1ull << (lastb + 1)) - 1));
-                const uint64_t high = w < lastw ? (1ull << 63) : (1ull << lastb);
-                const uint64_t eq_in = pq.at(code, w);
-                const uint64_t hneg = hin < 0 ? 1ull : 0ull;
-                const uint64_t hpos = hin > 0 ? 1ull : 0ull;
-                const uint64_t xv = eq_in | mv[w];
-                const uint64_t eq = eq_in | hneg;
-                const uint64_t xh = (((eq & pv[w]) + pv[w]) ^ pv[w]) | eq;
-                const uint64_t ph = mv[w] | ~(xh | pv[w]);
-                const uint64_t mh = pv[w] & xh;
-                const int32_t hout = (ph & high) ? 1 : ((mh & high) ? -1 : 0);
-                const uint64_t phs = (ph << 1) | hpos;
-                const uint64_t mhs = (mh << 1) | hneg;
-                pv[w] = (mhs | ~(xv | phs)) & mask;
-                mv[w] = (phs & xv) & mask;
-                hin = hout;
-            }
-        }
-        score += hin;
-        if (score < best) {
-            best = score;
-            best_at = (uint32_t)(j + 1);
-        }
-    }
-    best_out = best;
-    at_out = best_at;
-}
-
-template <int W>
-__global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
-    extern __shared__ uint64_t lds[];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
-    Peq pq;
-    pq.base = lds + (size_t)wave * 5 * W * 64;
-    pq.lane = lane;
-    pq.W = W;
-    for (;;) {
-        const uint32_t base = wave_pop(a.counter, lane);
-        if (base >= a.npairs) break;
-        const uint32_t idx = base + lane;
-        if (idx < a.npairs) {
-            const uint32_t p = a.order ? a.order[idx] : idx;
-            const int32_t m = (int32_t)a.read_len[p], n = (int32_t)a.text_len[p];
-            const uint8_t *read = a.reads + a.read_off[p];
-            const uint8_t *text = a.texts + a.text_off[p];
-            if (m == 0 || m > 256) {
-                // long::infix_best_end guards (myers_ea.rs:275-280)
-                a.dist[p] = m == 0 ? 0 : m;
-                a.end[p] = 0;
-                if (a.start) a.start[p] = 0;
-            } else {
-                int32_t d;
-                uint32_t e;
-                build_peq<W>(pq, read, m, false);
-                myers_scan<W>(pq, m, text, n, false, d, e);
-                a.dist[p] = d;
-                a.end[p] = e;
-                if (a.start) {
-                    // infix_best_start: best end of the doubly reversed pair
-                    // (myers_ea.rs:166-185); start = n - rev_end.
-                    build_peq<W>(pq, read, m, true);
-                    myers_scan<W>(pq, m, text, n, true, d, e);
-                    a.start[p] = (uint64_t)(n - (int32_t)e);
-                }
-            }
-        }
-    }
-}
-
 __global__ void k_zero_u32(uint32_t *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -614,17 +478,6 @@ hipError_t launch_plan(const PlanArgs &a, int ncu, hipStream_t s) {
 
 int plan_class_half(int c) { return c >= 0 && c < 5 ? kPlanHalves[c] : -1; }
 
-hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s) {
-    const size_t lds = (size_t)(MYERS_BLOCK / 64) * 5 * words * 64 * sizeof(uint64_t);
-    switch (words) {
-    case 1: hipLaunchKernelGGL(k_myers<1>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    case 2: hipLaunchKernelGGL(k_myers<2>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    case 3: hipLaunchKernelGGL(k_myers<3>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    default: hipLaunchKernelGGL(k_myers<4>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
@@ -637,18 +490,6 @@ int gotoh_strip_blocks_per_cu(int edit) {
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_strip<STRIP_T, true>, GOTOH_BLOCK, 0);
     else
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_strip<STRIP_T, false>, GOTOH_BLOCK, 0);
-    return nb > 0 ? nb : 1;
-}
-
-int myers_blocks_per_cu(int words) {
-    int nb = 0;
-    const size_t lds = (size_t)(MYERS_BLOCK / 64) * 5 * words * 64 * sizeof(uint64_t);
-    switch (words) {
-    case 1: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<1>, MYERS_BLOCK, lds); break;
-    case 2: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<2>, MYERS_BLOCK, lds); break;
-    case 3: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<3>, MYERS_BLOCK, lds); break;
-    default: hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<4>, MYERS_BLOCK, lds); break;
-    }
     return nb > 0 ? nb : 1;
 }
 

@@ -46,6 +46,7 @@ struct GotohArgs {
 struct MyersArgs {
     uint32_t npairs;
     const uint32_t *order;
+    const uint32_t *range;  // device {start, count} into order, or nullptr
     const uint8_t *reads;
     const uint64_t *read_off;
     const uint32_t *read_len;
