@@ -21,7 +21,8 @@ EXPORTS = [
     "esp_band", "esp_myers_best_end", "esp_myers_best_start", "esp_cigar_capacity",
     "esp_gotoh_batch", "esp_myers_batch", "esp_last_kernel_ms", "esp_device_alloc",
     "esp_device_free", "esp_copy_to_device", "esp_copy_to_host", "esp_stitch_cigar",
-    "esp_last_timings", "esp_device_copy",
+    "esp_last_timings", "esp_device_copy", "esp_ref_upload", "esp_ref_free",
+    "esp_ref_device_bytes", "esp_gather_batch", "esp_mate_rescue_batch", "esp_spliced_batch",
 ]
 
 
@@ -51,6 +52,22 @@ class CTiming(ctypes.Structure):
 class CHostMeta(ctypes.Structure):
     _fields_ = [("read_len", ctypes.c_void_p), ("win_len", ctypes.c_void_p),
                 ("offset", ctypes.c_void_p), ("half", ctypes.c_void_p)]
+
+
+class CMateBatch(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_size_t), ("reads", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
+                ("mate_contig", ctypes.c_void_p), ("mate_pos", ctypes.c_void_p),
+                ("max_read_len", ctypes.c_uint32)]
+
+
+class CSpliceBatch(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_size_t), ("reads", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
+                ("contig", ctypes.c_void_p), ("hint", ctypes.c_void_p),
+                ("exons", ctypes.c_void_p), ("exon_off", ctypes.c_void_p),
+                ("pseudo_off", ctypes.c_void_p), ("pseudo_bytes", ctypes.c_uint64),
+                ("n_exons", ctypes.c_size_t)]
 
 
 _lib = None
@@ -91,6 +108,12 @@ def lib():
     L.esp_copy_to_device.argtypes = [vp, vp, vp, sz]
     L.esp_copy_to_host.argtypes = [vp, vp, vp, sz]
     L.esp_stitch_cigar.argtypes = [u32p, sz, ctypes.c_uint32, u32p, sz, u32p, sz, u32p, sz, P(sz)]
+    L.esp_ref_upload.argtypes = [vp, sz, vp, vp, vp, vp, vp, P(vp)]
+    L.esp_ref_free.argtypes = [vp, vp]
+    L.esp_ref_device_bytes.argtypes = [vp, P(sz)]
+    L.esp_gather_batch.argtypes = [vp, vp, vp, sz, vp, vp]
+    L.esp_mate_rescue_batch.argtypes = [vp, vp, P(CMateBatch), P(CParams), vp, vp, vp, vp]
+    L.esp_spliced_batch.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp]
     _lib = L
     return L
 

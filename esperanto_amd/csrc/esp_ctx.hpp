@@ -1,0 +1,201 @@
+// esp_ctx.hpp -- host-side context shared by the C ABI translation units
+// (esp_host.cpp: extension and Myers calls; esp_window.cpp: resident
+// reference, window gather and the mate-rescue / spliced pipelines).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/esp_align.h"
+#include "esp_kernels.hpp"
+
+namespace esph {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t grow = std::max(need, (size_t)4096);
+        hipError_t e = hipMalloc(&p, grow);
+        if (e == hipSuccess) bytes = grow;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t grow = std::max(need, (size_t)4096);
+        hipError_t e = hipHostMalloc(&p, grow, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = grow;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+inline constexpr size_t kTraceBudget = (size_t)8 << 30;  // bytes of trace scratch per launch
+inline constexpr size_t kRowBudget = (size_t)2 << 30;
+
+}  // namespace esph
+
+struct esp_ctx {
+    int device = 0;
+    int ncu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    esph::DevBuf counter, rows, trace, btrace, stage, stage_out, plan, order;
+    // window pipelines: query arena, window / pseudo-reference arena,
+    // per-pair descriptors and Myers results
+    esph::DevBuf wquery, warena, wdesc, wmyers;
+    hipEvent_t ev_plan = nullptr;
+    int band_blocks[espk::PLAN_STRIP] = {};
+    esph::HostBuf hstage, hout, hplan;
+    // Per-launch timing of the last batch call.
+    static constexpr int kMaxEv = 16;
+    hipEvent_t evs[kMaxEv + 1] = {};
+    esp_timing tim[kMaxEv] = {};
+    int nev = 0;
+};
+
+namespace esph {
+
+inline int fail(esp_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+inline int hip_fail(esp_ctx *c, hipError_t e, const char *where) {
+    return fail(c, ESP_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+// Scoring envelope (include/esp_align.h).  Inside it the reference's i16 and
+// i32 accumulators agree and the int32 device datapath cannot overflow.
+inline int check_params(esp_ctx *c, const esp_params *p) {
+    if (!p) return fail(c, ESP_E_INVAL, "params is NULL");
+    if (p->gap_open < 0 || p->gap_ext < 0 || p->gap_open + p->gap_ext > 16384)
+        return fail(c, ESP_E_INVAL, "gap penalties outside [0, 16384]");
+    if (p->match_score < -1024 || p->match_score > 1024)
+        return fail(c, ESP_E_INVAL, "match_score outside [-1024, 1024]");
+    if (p->mismatch < -16384 || p->mismatch > 16384)
+        return fail(c, ESP_E_INVAL, "mismatch outside [-16384, 16384]");
+    if (p->edit_dir < 0 || p->edit_dir > 2) return fail(c, ESP_E_INVAL, "edit_dir not in {0,1,2}");
+    return ESP_OK;
+}
+
+inline espd::GParams dev_params(const esp_params *p) {
+    espd::GParams g;
+    g.match = p->match_score;
+    g.mismatch = p->mismatch;
+    g.oe = -(p->gap_open + p->gap_ext);
+    g.ext = -p->gap_ext;
+    g.edit = p->edit_dir;
+    return g;
+}
+
+inline constexpr uint32_t kMaxRead = 16000;
+inline constexpr uint64_t kMaxWin = (uint64_t)1 << 24;
+
+inline int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
+    if (m > kMaxRead) return fail(c, ESP_E_INVAL, "read longer than 16000");
+    if (n > kMaxWin) return fail(c, ESP_E_INVAL, "window longer than 1<<24");
+    return ESP_OK;
+}
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Launch timing: mark() before the first launch and after every launch of
+// a batch call; esp_last_timings() reports the differences.
+inline void tim_begin(esp_ctx *c) {
+    c->nev = 0;
+    (void)hipEventRecord(c->evs[0], c->stream);
+}
+inline void tim_mark(esp_ctx *c, int kind, int half) {
+    if (c->nev >= esp_ctx::kMaxEv) return;
+    c->tim[c->nev].kind = kind;
+    c->tim[c->nev].half = half;
+    c->nev++;
+    (void)hipEventRecord(c->evs[c->nev], c->stream);
+}
+
+// Substitution rows of the band kernel: for read code rc = 0..4, the
+// 5-bit signed LUT score of window code f = 0..5 at bit 5*f (window code 4
+// never occurs; 5 is any non-ACGT byte).  Mirrors SubstLut (extend.rs:161-186).
+// The band kernel's scores are 5-bit LUT fields, and its wide-band
+// specialisations keep U = H + open_ext and E + ext as int16: with reads of
+// at most band_span(64) bases and |score| <= 16, every such value lies in
+// (-20000, 16384) when gap_open + gap_ext <= 8000.
+inline bool band_lut_fits(const esp_params *p) {
+    return p->match_score >= -16 && p->match_score <= 15 && p->mismatch >= -16 &&
+           p->mismatch <= 15 && p->gap_open + p->gap_ext <= 8000;
+}
+
+inline void band_lut(const esp_params *p, uint32_t out[5]) {
+    for (int rc = 0; rc < 5; rc++) {
+        const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;
+        uint32_t w = 0;
+        for (int f = 0; f < 6; f++) {
+            int s = (rc < 4 && f == rc) ? p->match_score : (f == partner ? 0 : p->mismatch);
+            w |= ((uint32_t)s & 31u) << (5 * f);
+        }
+        out[rc] = w;
+    }
+}
+
+// Launch geometry of the persistent strip grid for `count` pairs whose
+// largest has maxm rows and maxw trace words.
+inline int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t maxm, uint64_t maxw) {
+    const int per_cu = espk::gotoh_strip_blocks_per_cu(a.P.edit);
+    uint64_t lanes = (uint64_t)c->ncu * per_cu * espk::GOTOH_BLOCK;
+    lanes = std::min<uint64_t>(lanes, align_up(count, 64));
+    const uint64_t tw = std::max<uint64_t>(maxw, 1), rw = 5 * std::max<uint64_t>(maxm, 1);
+    lanes = std::min<uint64_t>(lanes, kTraceBudget / (tw * 4));
+    lanes = std::min<uint64_t>(lanes, kRowBudget / (rw * 4));
+    lanes = std::max<uint64_t>(lanes / 64 * 64, 64);
+    a.nl = (uint32_t)lanes;
+    a.maxm = (uint32_t)std::max<uint64_t>(maxm, 1);
+    a.maxw = (uint32_t)tw;
+    HIP_TRY(c, c->rows.ensure(rw * lanes * 4));
+    HIP_TRY(c, c->trace.ensure(tw * lanes * 4));
+    a.rows = (int32_t *)c->rows.p;
+    a.trace = (uint32_t *)c->trace.p;
+    HIP_TRY(c, espk::launch_gotoh_strip(a, (int)(lanes / espk::GOTOH_BLOCK), c->stream));
+    return ESP_OK;
+}
+
+
+// Gotoh batch on the context stream without resetting the launch timings
+// (the pipelines of esp_window.cpp time their own stages around it).
+int gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, int mode,
+                     esp_batch_out *o);
+
+}  // namespace esph

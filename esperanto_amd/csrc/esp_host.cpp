@@ -13,185 +13,13 @@
 #include <vector>
 
 #include "../../include/esp_align.h"
+#include "esp_ctx.hpp"
 #include "esp_kernels.hpp"
+#include "esp_stitch.hpp"
 
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            bytes = 0;
-            if (e != hipSuccess) return e;
-        }
-        size_t grow = std::max(need, (size_t)4096);
-        hipError_t e = hipMalloc(&p, grow);
-        if (e == hipSuccess) bytes = grow;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-struct HostBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t grow = std::max(need, (size_t)4096);
-        hipError_t e = hipHostMalloc(&p, grow, hipHostMallocDefault);
-        if (e == hipSuccess) bytes = grow;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-constexpr size_t kTraceBudget = (size_t)8 << 30;  // bytes of trace scratch per launch
-constexpr size_t kRowBudget = (size_t)2 << 30;
-
-}  // namespace
-
-struct esp_ctx {
-    int device = 0;
-    int ncu = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    DevBuf counter, rows, trace, btrace, stage, stage_out, plan, order;
-    hipEvent_t ev_plan = nullptr;
-    int band_blocks[espk::PLAN_STRIP] = {};
-    HostBuf hstage, hout, hplan;
-    // Per-launch timing of the last batch call.
-    static constexpr int kMaxEv = 16;
-    hipEvent_t evs[kMaxEv + 1] = {};
-    esp_timing tim[kMaxEv] = {};
-    int nev = 0;
-};
-
-namespace {
-
-int fail(esp_ctx *c, int code, const std::string &msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-int hip_fail(esp_ctx *c, hipError_t e, const char *where) {
-    return fail(c, ESP_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(ctx, expr)                                   \
-    do {                                                     \
-        hipError_t _e = (expr);                              \
-        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
-    } while (0)
-
-// Scoring envelope (include/esp_align.h).  Inside it the reference's i16 and
-// i32 accumulators agree and the int32 device datapath cannot overflow.
-int check_params(esp_ctx *c, const esp_params *p) {
-    if (!p) return fail(c, ESP_E_INVAL, "params is NULL");
-    if (p->gap_open < 0 || p->gap_ext < 0 || p->gap_open + p->gap_ext > 16384)
-        return fail(c, ESP_E_INVAL, "gap penalties outside [0, 16384]");
-    if (p->match_score < -1024 || p->match_score > 1024)
-        return fail(c, ESP_E_INVAL, "match_score outside [-1024, 1024]");
-    if (p->mismatch < -16384 || p->mismatch > 16384)
-        return fail(c, ESP_E_INVAL, "mismatch outside [-16384, 16384]");
-    if (p->edit_dir < 0 || p->edit_dir > 2) return fail(c, ESP_E_INVAL, "edit_dir not in {0,1,2}");
-    return ESP_OK;
-}
-
-espd::GParams dev_params(const esp_params *p) {
-    espd::GParams g;
-    g.match = p->match_score;
-    g.mismatch = p->mismatch;
-    g.oe = -(p->gap_open + p->gap_ext);
-    g.ext = -p->gap_ext;
-    g.edit = p->edit_dir;
-    return g;
-}
-
-constexpr uint32_t kMaxRead = 16000;
-constexpr uint64_t kMaxWin = (uint64_t)1 << 24;
-
-int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
-    if (m > kMaxRead) return fail(c, ESP_E_INVAL, "read longer than 16000");
-    if (n > kMaxWin) return fail(c, ESP_E_INVAL, "window longer than 1<<24");
-    return ESP_OK;
-}
-
-size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
-// Launch timing: mark() before the first launch and after every launch of
-// a batch call; esp_last_timings() reports the differences.
-void tim_begin(esp_ctx *c) {
-    c->nev = 0;
-    (void)hipEventRecord(c->evs[0], c->stream);
-}
-void tim_mark(esp_ctx *c, int kind, int half) {
-    if (c->nev >= esp_ctx::kMaxEv) return;
-    c->tim[c->nev].kind = kind;
-    c->tim[c->nev].half = half;
-    c->nev++;
-    (void)hipEventRecord(c->evs[c->nev], c->stream);
-}
-
-// Substitution rows of the band kernel: for read code rc = 0..4, the
-// 5-bit signed LUT score of window code f = 0..5 at bit 5*f (window code 4
-// never occurs; 5 is any non-ACGT byte).  Mirrors SubstLut (extend.rs:161-186).
-// The band kernel's scores are 5-bit LUT fields, and its wide-band
-// specialisations keep U = H + open_ext and E + ext as int16: with reads of
-// at most band_span(64) bases and |score| <= 16, every such value lies in
-// (-20000, 16384) when gap_open + gap_ext <= 8000.
-bool band_lut_fits(const esp_params *p) {
-    return p->match_score >= -16 && p->match_score <= 15 && p->mismatch >= -16 &&
-           p->mismatch <= 15 && p->gap_open + p->gap_ext <= 8000;
-}
-
-void band_lut(const esp_params *p, uint32_t out[5]) {
-    for (int rc = 0; rc < 5; rc++) {
-        const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;
-        uint32_t w = 0;
-        for (int f = 0; f < 6; f++) {
-            int s = (rc < 4 && f == rc) ? p->match_score : (f == partner ? 0 : p->mismatch);
-            w |= ((uint32_t)s & 31u) << (5 * f);
-        }
-        out[rc] = w;
-    }
-}
-
-// Launch geometry of the persistent strip grid for `count` pairs whose
-// largest has maxm rows and maxw trace words.
-int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t maxm, uint64_t maxw) {
-    const int per_cu = espk::gotoh_strip_blocks_per_cu(a.P.edit);
-    uint64_t lanes = (uint64_t)c->ncu * per_cu * espk::GOTOH_BLOCK;
-    lanes = std::min<uint64_t>(lanes, align_up(count, 64));
-    const uint64_t tw = std::max<uint64_t>(maxw, 1), rw = 5 * std::max<uint64_t>(maxm, 1);
-    lanes = std::min<uint64_t>(lanes, kTraceBudget / (tw * 4));
-    lanes = std::min<uint64_t>(lanes, kRowBudget / (rw * 4));
-    lanes = std::max<uint64_t>(lanes / 64 * 64, 64);
-    a.nl = (uint32_t)lanes;
-    a.maxm = (uint32_t)std::max<uint64_t>(maxm, 1);
-    a.maxw = (uint32_t)tw;
-    HIP_TRY(c, c->rows.ensure(rw * lanes * 4));
-    HIP_TRY(c, c->trace.ensure(tw * lanes * 4));
-    a.rows = (int32_t *)c->rows.p;
-    a.trace = (uint32_t *)c->trace.p;
-    HIP_TRY(c, espk::launch_gotoh_strip(a, (int)(lanes / espk::GOTOH_BLOCK), c->stream));
-    return ESP_OK;
-}
+using namespace esph;
 
 // Single-pair staging layout (host pinned -> device), 16-byte aligned fields.
 struct PairStage {
@@ -306,6 +134,99 @@ int run_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, si
 
 }  // namespace
 
+// Planner, band kernels and strip kernel of one Gotoh batch (asynchronous).
+int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, int mode,
+                           esp_batch_out *o) {
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!b || !o) return fail(c, ESP_E_INVAL, "batch and output are required");
+    if (mode < ESP_MODE_BAND || mode > ESP_MODE_HINT) return fail(c, ESP_E_INVAL, "bad mode");
+    if (b->npairs == 0) return ESP_OK;
+    if (b->npairs > 0xFFFFFF00ull) return fail(c, ESP_E_INVAL, "too many pairs");
+    if ((mode != ESP_MODE_FULL && !b->offset) || (mode == ESP_MODE_BAND && !b->half))
+        return fail(c, ESP_E_INVAL, "offset/half arrays required for this mode");
+    HIP_TRY(c, hipSetDevice(c->device));
+    espk::GotohArgs a{};
+    a.npairs = (uint32_t)b->npairs;
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.wins = b->wins;
+    a.win_off = b->win_off;
+    a.win_len = b->win_len;
+    a.offset = b->offset;
+    a.half = b->half;
+    a.mode = mode;
+    a.P = dev_params(p);
+    band_lut(p, a.lut);
+    a.ext = o->ext;
+    a.touch = o->touch;
+    a.cigar = o->cigar;
+    a.cig_off = o->cig_off;
+    a.cig_n = o->cig_n;
+    a.aux = o->aux;
+
+    // 1. Device planner: classify, bucket by (class, read length), order.
+    HIP_TRY(c, c->plan.ensure(sizeof(espk::PlanDev)));
+    HIP_TRY(c, c->order.ensure(b->npairs * 4));
+    HIP_TRY(c, c->hplan.ensure(sizeof(espk::PlanSummary)));
+    espk::PlanArgs pa{};
+    pa.npairs = a.npairs;
+    pa.read_len = b->read_len;
+    pa.win_len = b->win_len;
+    pa.offset = b->offset;
+    pa.half = b->half;
+    pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
+    pa.max_read = kMaxRead;
+    pa.max_win = (uint32_t)kMaxWin;
+    pa.plan = (espk::PlanDev *)c->plan.p;
+    pa.order = (uint32_t *)c->order.p;
+    HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));
+    tim_mark(c, 0, 0);
+    // The plan summary travels back while the band kernels run: they read
+    // their class ranges from device memory and need nothing from the host.
+    HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanSummary),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_plan, c->stream));
+    a.order = pa.order;
+    espk::PlanDev *dp = pa.plan;
+
+    // 2. Band kernel per half specialisation (an empty class exits at once).
+    if (pa.band_ok) {
+        for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+            const int half = espk::plan_class_half(cls);
+            const uint64_t W = 2 * (uint64_t)half + 1;
+            const uint64_t maxw = espk::band_trace_words(espk::band_span(half) - W, half);
+            if (!c->band_blocks[cls]) c->band_blocks[cls] = espk::gotoh_band_blocks_per_cu(half);
+            uint64_t blocks = (uint64_t)c->ncu * c->band_blocks[cls];
+            blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
+            blocks = std::min<uint64_t>(blocks, kTraceBudget / (maxw * 64 * 4));
+            blocks = std::max<uint64_t>(blocks, 1);
+            HIP_TRY(c, c->btrace.ensure(maxw * 64 * 4 * blocks));
+            a.trace = (uint32_t *)c->btrace.p;
+            a.maxw = (uint32_t)maxw;
+            a.range = &dp->sum.range[cls][0];
+            a.counter = &dp->counters[cls];
+            HIP_TRY(c, espk::launch_gotoh_band(a, half, (int)blocks, c->stream));
+            tim_mark(c, 1, half);
+        }
+    }
+
+    // 3. Strip kernel for everything else, sized from the plan summary.
+    HIP_TRY(c, hipEventSynchronize(c->ev_plan));
+    const espk::PlanSummary *hp = (const espk::PlanSummary *)c->hplan.p;
+    if (hp->bad) return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+    if (hp->range[espk::PLAN_STRIP][1]) {
+        a.range = &dp->sum.range[espk::PLAN_STRIP][0];
+        a.counter = &dp->counters[espk::PLAN_STRIP];
+        rc = strip_launch(c, a, hp->range[espk::PLAN_STRIP][1], hp->strip_maxm, hp->strip_maxw);
+        if (rc) return rc;
+        tim_mark(c, 2, 0);
+    }
+    return ESP_OK;
+}
+
+
 extern "C" {
 
 size_t esp_cigar_capacity(size_t m) { return 2 * m + 5; }
@@ -357,6 +278,10 @@ int esp_destroy(esp_ctx *c) {
     if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
     c->stage.release();
     c->stage_out.release();
+    c->wquery.release();
+    c->warena.release();
+    c->wdesc.release();
+    c->wmyers.release();
     c->hstage.release();
     c->hout.release();
     for (int k = 0; k <= esp_ctx::kMaxEv; k++)
@@ -400,94 +325,11 @@ int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
                     const esp_params *p, int mode, esp_batch_out *o) {
     (void)meta;  // the device planner reads the lengths itself
     if (!c) return ESP_E_INVAL;
-    int rc = check_params(c, p);
-    if (rc) return rc;
-    if (!b || !o) return fail(c, ESP_E_INVAL, "batch and output are required");
-    if (mode < ESP_MODE_BAND || mode > ESP_MODE_HINT) return fail(c, ESP_E_INVAL, "bad mode");
-    if (b->npairs == 0) return ESP_OK;
-    if (b->npairs > 0xFFFFFF00ull) return fail(c, ESP_E_INVAL, "too many pairs");
-    if ((mode != ESP_MODE_FULL && !b->offset) || (mode == ESP_MODE_BAND && !b->half))
-        return fail(c, ESP_E_INVAL, "offset/half arrays required for this mode");
     HIP_TRY(c, hipSetDevice(c->device));
-    espk::GotohArgs a{};
-    a.npairs = (uint32_t)b->npairs;
-    a.reads = b->reads;
-    a.read_off = b->read_off;
-    a.read_len = b->read_len;
-    a.wins = b->wins;
-    a.win_off = b->win_off;
-    a.win_len = b->win_len;
-    a.offset = b->offset;
-    a.half = b->half;
-    a.mode = mode;
-    a.P = dev_params(p);
-    band_lut(p, a.lut);
-    a.ext = o->ext;
-    a.touch = o->touch;
-    a.cigar = o->cigar;
-    a.cig_off = o->cig_off;
-    a.cig_n = o->cig_n;
-    a.aux = o->aux;
-
-    // 1. Device planner: classify, bucket by (class, read length), order.
-    HIP_TRY(c, c->plan.ensure(sizeof(espk::PlanDev)));
-    HIP_TRY(c, c->order.ensure(b->npairs * 4));
-    HIP_TRY(c, c->hplan.ensure(sizeof(espk::PlanSummary)));
-    espk::PlanArgs pa{};
-    pa.npairs = a.npairs;
-    pa.read_len = b->read_len;
-    pa.win_len = b->win_len;
-    pa.offset = b->offset;
-    pa.half = b->half;
-    pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
-    pa.max_read = kMaxRead;
-    pa.max_win = (uint32_t)kMaxWin;
-    pa.plan = (espk::PlanDev *)c->plan.p;
-    pa.order = (uint32_t *)c->order.p;
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     tim_begin(c);
-    HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));
-    tim_mark(c, 0, 0);
-    // The plan summary travels back while the band kernels run: they read
-    // their class ranges from device memory and need nothing from the host.
-    HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanSummary),
-                              hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ev_plan, c->stream));
-    a.order = pa.order;
-    espk::PlanDev *dp = pa.plan;
-
-    // 2. Band kernel per half specialisation (an empty class exits at once).
-    if (pa.band_ok) {
-        for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
-            const int half = espk::plan_class_half(cls);
-            const uint64_t W = 2 * (uint64_t)half + 1;
-            const uint64_t maxw = espk::band_trace_words(espk::band_span(half) - W, half);
-            if (!c->band_blocks[cls]) c->band_blocks[cls] = espk::gotoh_band_blocks_per_cu(half);
-            uint64_t blocks = (uint64_t)c->ncu * c->band_blocks[cls];
-            blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
-            blocks = std::min<uint64_t>(blocks, kTraceBudget / (maxw * 64 * 4));
-            blocks = std::max<uint64_t>(blocks, 1);
-            HIP_TRY(c, c->btrace.ensure(maxw * 64 * 4 * blocks));
-            a.trace = (uint32_t *)c->btrace.p;
-            a.maxw = (uint32_t)maxw;
-            a.range = &dp->sum.range[cls][0];
-            a.counter = &dp->counters[cls];
-            HIP_TRY(c, espk::launch_gotoh_band(a, half, (int)blocks, c->stream));
-            tim_mark(c, 1, half);
-        }
-    }
-
-    // 3. Strip kernel for everything else, sized from the plan summary.
-    HIP_TRY(c, hipEventSynchronize(c->ev_plan));
-    const espk::PlanSummary *hp = (const espk::PlanSummary *)c->hplan.p;
-    if (hp->bad) return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
-    if (hp->range[espk::PLAN_STRIP][1]) {
-        a.range = &dp->sum.range[espk::PLAN_STRIP][0];
-        a.counter = &dp->counters[espk::PLAN_STRIP];
-        rc = strip_launch(c, a, hp->range[espk::PLAN_STRIP][1], hp->strip_maxm, hp->strip_maxw);
-        if (rc) return rc;
-        tim_mark(c, 2, 0);
-    }
+    const int rc = gotoh_batch_core(c, b, p, mode, o);
+    if (rc) return rc;
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
 }
@@ -526,7 +368,7 @@ int esp_device_copy(esp_ctx *c, void *dst, const void *src, size_t bytes) {
 int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, int32_t *dist,
                     uint64_t *end, uint64_t *start) {
     if (!c) return ESP_E_INVAL;
-    if (!b || !meta || !meta->read_len || !dist || !end)
+    if (!b || !meta || !meta->read_len || !dist)
         return fail(c, ESP_E_INVAL, "batch, host lengths and outputs are required");
     if (b->npairs == 0) return ESP_OK;
     uint32_t maxm = 0;
@@ -663,68 +505,21 @@ int esp_copy_to_host(esp_ctx *c, void *dst, const void *src, size_t bytes) {
     return ESP_OK;
 }
 
-// stitch_cigar (splice.rs:435-494): Match/Del split at exon ends, RefSkip of
-// the junction length inserted at each crossing, Ins/SoftClip/RefSkip pass
-// through; runs of one kind merge as push_op does (extend.rs:254-282).
+// stitch_cigar (splice.rs:435-494); the walk is shared with the spliced
+// pipeline kernel (esp_stitch.hpp).
 int esp_stitch_cigar(const uint32_t *cigar, size_t n_ops, uint32_t p_start,
                      const uint32_t *bounds, size_t n_exons, const uint32_t *skips,
                      size_t n_junctions, uint32_t *out, size_t cap, size_t *n_out) {
     if ((n_ops && !cigar) || !n_out || (n_exons && !bounds) || (n_junctions && !skips))
         return ESP_E_INVAL;
-    std::vector<uint32_t> v;
-    v.reserve(n_ops + n_exons + 2);
-    auto push = [&](uint32_t kind, uint64_t len) {
-        if (!v.empty() && (v.back() & 0xFu) == kind) {
-            v.back() = ((uint32_t)((v.back() >> 4) + len) << 4) | kind;
-        } else {
-            v.push_back(((uint32_t)len << 4) | kind);
-        }
-    };
-    if (n_exons == 0) {
-        v.assign(cigar, cigar + n_ops);
-    } else {
-        auto exon_end = [&](size_t ei) -> uint64_t {
-            return (uint64_t)bounds[3 * ei] + (uint64_t)(bounds[3 * ei + 2] - bounds[3 * ei + 1]);
-        };
-        uint64_t p = p_start;
-        size_t ei = 0;
-        while (ei + 1 < n_exons && p >= exon_end(ei)) ei++;
-        for (size_t k = 0; k < n_ops; k++) {
-            const uint32_t kind = cigar[k] & 0xFu;
-            const uint32_t len = cigar[k] >> 4;
-            if (kind != espd::OP_M && kind != espd::OP_D) {
-                push(kind, len);
-                continue;
-            }
-            uint64_t remaining = len;
-            while (remaining > 0) {
-                if (ei >= n_exons) {
-                    push(kind, remaining);
-                    p += remaining;
-                    break;
-                }
-                if (p < bounds[3 * ei]) p = bounds[3 * ei];
-                const uint64_t ee = exon_end(ei);
-                const uint64_t in_exon = ee > p ? ee - p : 0;
-                const uint64_t take = std::min(remaining, in_exon);
-                if (take > 0) push(kind, take);
-                p += take;
-                remaining -= take;
-                if (remaining > 0) {
-                    if (ei + 1 >= n_exons) {
-                        push(kind, remaining);
-                        p += remaining;
-                        break;
-                    }
-                    if (ei < n_junctions) push(espd::OP_N, skips[ei]);
-                    ei++;
-                }
-            }
-        }
-    }
-    *n_out = v.size();
-    if (v.size() > cap || (!v.empty() && !out)) return ESP_E_CAP;
-    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 4);
+    if (n_ops > 0xFFFFFFFFull || n_exons > 0xFFFFFFFFull || n_junctions > 0xFFFFFFFFull)
+        return ESP_E_INVAL;
+    const uint32_t cap32 = (uint32_t)std::min<size_t>(cap, 0xFFFFFFFFull);
+    const uint32_t n = esps::stitch_walk(cigar, (uint32_t)n_ops, p_start, bounds,
+                                         (uint32_t)n_exons, skips, (uint32_t)n_junctions,
+                                         out ? out : nullptr, out ? cap32 : 0u);
+    *n_out = n;
+    if (n > cap || (n && !out)) return ESP_E_CAP;
     return ESP_OK;
 }
 

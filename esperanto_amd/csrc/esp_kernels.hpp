@@ -54,8 +54,8 @@ struct MyersArgs {
     const uint64_t *text_off;
     const uint32_t *text_len;
     int32_t *dist;
-    uint64_t *end;
-    uint64_t *start;  // may be nullptr
+    uint64_t *end;    // nullptr: no forward scan unless start is nullptr too (infix)
+    uint64_t *start;  // nullptr: no reversed scan
     uint32_t *counter;
 };
 
@@ -124,5 +124,82 @@ hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 int gotoh_strip_blocks_per_cu(int edit);
 int myers_blocks_per_cu(int words);
+
+// ---- resident reference and window pipelines (esp_window.hip) ----------
+
+// Device view of the resident 2-bit reference.  Contig k's packed bytes
+// start at packed + pk_off[k] and are followed by at least 8 zero bytes, so
+// a 5-byte window read at any in-range base stays inside the allocation.
+struct RefView {
+    const uint8_t *packed;
+    const uint64_t *pk_off;
+    const uint32_t *len;
+    const uint32_t *iv;      // 2 u32 per N interval
+    const uint64_t *iv_off;  // ncontig + 1
+    uint32_t ncontig;
+};
+
+struct GatherArgs {
+    RefView R;
+    const esp_gather_job *jobs;
+    uint32_t njobs;
+    uint8_t *dst;
+    int32_t *status;
+};
+
+struct MateArgs {
+    uint32_t npairs;
+    RefView R;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *mate_contig;
+    const uint32_t *mate_pos;
+    uint32_t max_read;
+    uint32_t qstride;  // query slot per pair: read at 0, reverse complement at qstride / 2
+    uint64_t wstride;  // window slot per pair
+    uint8_t *qarena;
+    uint8_t *warena;
+    // per-pair scratch
+    uint64_t *q_off_plus, *q_off_minus, *w_off, *s_plus, *s_minus, *g_read_off;
+    int64_t *g_offset;
+    uint32_t *q_len, *w_len, *wstart, *g_read_len;
+    int32_t *state, *d_plus, *d_minus, *sel, *dist;
+    esp_ext *ext;
+    // outputs
+    esp_mate_rec *out;
+    uint32_t *cig_n;
+};
+
+struct SpliceArgs {
+    uint32_t npairs;
+    RefView R;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *exons;
+    const uint64_t *exon_off;
+    const uint64_t *pseudo_off;
+    uint64_t n_exons;       // exon_off[npairs]: exon indices must stay below it
+    uint64_t pseudo_bytes;  // pseudo_off[npairs]: the arena size
+    uint8_t *parena;
+    uint32_t *bounds;  // 3 per exon: pseudo_start, ref_start, ref_end
+    int32_t *state;
+    uint32_t *g_read_len;
+    uint32_t *p_len;
+    const esp_ext *ext;
+    const uint32_t *cigar;
+    const uint64_t *cig_off;
+    const uint32_t *cig_n;
+    uint32_t *stitched;
+    const uint64_t *stitch_off;
+    esp_splice_rec *rec;
+};
+
+hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
+hipError_t launch_mate_prep(const MateArgs &a, hipStream_t s);
+hipError_t launch_mate_select(const MateArgs &a, hipStream_t s);
+hipError_t launch_mate_finish(const MateArgs &a, hipStream_t s);
+hipError_t launch_splice_prep(const SpliceArgs &a, hipStream_t s);
+hipError_t launch_splice_finish(const SpliceArgs &a, hipStream_t s);
 
 }  // namespace espk

@@ -164,20 +164,26 @@ __global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
             if (m == 0 || m > 256) {
                 // long::infix_best_end guards (myers_ea.rs:275-280)
                 a.dist[p] = m == 0 ? 0 : m;
-                a.end[p] = 0;
+                if (a.end) a.end[p] = 0;
                 if (a.start) a.start[p] = 0;
             } else {
                 int32_t d;
                 uint32_t e;
-                build_peq<W>(peq, chunk, read, m, false);
-                myers_scan<W>(peq, chunk, m, text, n, false, d, e);
-                a.dist[p] = d;
-                a.end[p] = e;
+                // Forward scan for the end (and for infix, which asks for
+                // neither position); the distance of the reversed pair is
+                // the same minimum, so a start-only call scans once.
+                if (a.end || !a.start) {
+                    build_peq<W>(peq, chunk, read, m, false);
+                    myers_scan<W>(peq, chunk, m, text, n, false, d, e);
+                    a.dist[p] = d;
+                    if (a.end) a.end[p] = e;
+                }
                 if (a.start) {
                     // infix_best_start: best end of the doubly reversed pair
                     // (myers_ea.rs:166-185); start = n - rev_end.
                     build_peq<W>(peq, chunk, read, m, true);
                     myers_scan<W>(peq, chunk, m, text, n, true, d, e);
+                    a.dist[p] = d;
                     a.start[p] = (uint64_t)(n - (int32_t)e);
                 }
             }

@@ -1,0 +1,329 @@
+// esp_window.hip -- window gather from the resident 2-bit reference and the
+// per-pair stages of the mate-rescue and spliced window pipelines.
+//
+// Gather: one wave per window; each lane decodes 16 consecutive bases per
+// pass from a 5-byte big-endian window of the packed contig (base at pos in
+// byte pos/4 at shift 6 - 2*(pos%4), fasta.rs:1-6), masks N intervals found
+// by one binary search per 16-base run (the reference searches per base,
+// fasta.rs:119-135; the intervals are sorted and disjoint, so the masks are
+// the same), and stores the run as 4 dwords when aligned.
+#include "esp_kernels.hpp"
+#include "esp_stitch.hpp"
+
+using namespace espd;
+
+namespace espk {
+
+namespace {
+
+constexpr int WIN_BLOCK = 256;  // 4 waves, one window (or pair) each
+constexpr uint32_t ACGT_ASCII = 0x54474341u;  // 'A' 'C' 'G' 'T' by code
+
+// Writes cnt (<= 16) bytes held little-endian in w[4] to d.
+__device__ __forceinline__ void store_run(uint8_t *d, const uint32_t w[4], uint32_t cnt) {
+    if (cnt == 16 && ((uintptr_t)d & 3u) == 0) {
+        uint32_t *d4 = (uint32_t *)d;
+#pragma unroll
+        for (int q = 0; q < 4; q++) d4[q] = w[q];
+    } else {
+#pragma unroll
+        for (uint32_t t = 0; t < 16; t++)
+            if (t < cnt) d[t] = (uint8_t)(w[t >> 2] >> (8 * (t & 3)));
+    }
+}
+
+// Bit mask over [lo, lo + cnt) of the positions an N interval covers.
+__device__ uint32_t n_mask(const RefView &R, uint32_t ctg, uint32_t lo, uint32_t cnt) {
+    uint64_t a = R.iv_off[ctg], b = R.iv_off[ctg + 1];
+    const uint64_t hi = b;
+    if (a == b) return 0u;
+    while (a < b) {  // first interval ending after lo
+        const uint64_t mid = (a + b) >> 1;
+        if (R.iv[2 * mid + 1] > lo) b = mid;
+        else a = mid + 1;
+    }
+    uint32_t mask = 0;
+    for (uint64_t j = a; j < hi && R.iv[2 * j] < lo + cnt; j++) {
+        const uint32_t s = max(R.iv[2 * j], lo) - lo;
+        const uint32_t e = min(R.iv[2 * j + 1], lo + cnt) - lo;
+        mask |= ((1u << e) - 1u) & ~((1u << s) - 1u);
+    }
+    return mask;
+}
+
+// Wave-cooperative decode of contig[start, end) to dst, reverse-complemented
+// when rc (output byte k is the complement of base end - 1 - k; N stays N).
+__device__ void wave_decode(const RefView &R, uint32_t ctg, uint32_t start, uint32_t end, bool rc,
+                            uint8_t *dst, uint32_t lane) {
+    const uint8_t *pk = R.packed + R.pk_off[ctg];
+    const uint32_t L = end - start;
+    for (uint32_t k0 = lane * 16; k0 < L; k0 += 1024) {
+        const uint32_t cnt = min(16u, L - k0);
+        const uint32_t lo = rc ? end - k0 - cnt : start + k0;
+        const uint32_t b0 = lo >> 2;
+        uint64_t win = 0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) win = (win << 8) | pk[b0 + q];
+        const uint32_t nm = n_mask(R, ctg, lo, cnt);
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (uint32_t t = 0; t < 16; t++) {
+            if (t < cnt) {
+                const uint32_t r = rc ? cnt - 1 - t : t;   // run position of output byte t
+                const uint32_t rel = (lo & 3u) + r;        // base index from byte b0
+                uint32_t code = (uint32_t)(win >> (38 - 2 * rel)) & 3u;
+                if (rc) code ^= 3u;                        // A<->T, C<->G
+                uint32_t ch = (ACGT_ASCII >> (8 * code)) & 0xFFu;
+                if ((nm >> r) & 1u) ch = 'N';
+                w[t >> 2] |= ch << (8 * (t & 3));
+            }
+        }
+        store_run(dst + k0, w, cnt);
+    }
+}
+
+// Complement of one raw read byte through Base::from_ascii (fasta.rs:30-39):
+// A/a -> T, C/c -> G, G/g -> C, T/t -> A, anything else -> N.
+__device__ __forceinline__ uint32_t comp_ascii(uint32_t b) {
+    const uint32_t u = b & 0xDFu;
+    return u == 'A' ? 'T' : u == 'C' ? 'G' : u == 'G' ? 'C' : u == 'T' ? 'A' : 'N';
+}
+
+// Wave-cooperative copy of the read and of its reverse complement
+// (revcomp_into, align.rs:2525-2533).
+__device__ void wave_read_pair(const uint8_t *src, uint32_t m, uint8_t *fwd, uint8_t *rcd,
+                               uint32_t lane) {
+    for (uint32_t k0 = lane * 16; k0 < m; k0 += 1024) {
+        const uint32_t cnt = min(16u, m - k0);
+        uint32_t wf[4] = {0u, 0u, 0u, 0u}, wr[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (uint32_t t = 0; t < 16; t++) {
+            if (t < cnt) {
+                wf[t >> 2] |= (uint32_t)src[k0 + t] << (8 * (t & 3));
+                wr[t >> 2] |= comp_ascii(src[m - 1 - (k0 + t)]) << (8 * (t & 3));
+            }
+        }
+        store_run(fwd + k0, wf, cnt);
+        store_run(rcd + k0, wr, cnt);
+    }
+}
+
+__global__ __launch_bounds__(WIN_BLOCK) void k_gather(GatherArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (WIN_BLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= a.njobs) return;
+    const esp_gather_job j = a.jobs[k];
+    const bool ok = j.contig < a.R.ncontig && j.start <= j.end && j.end <= a.R.len[j.contig];
+    if (lane == 0 && a.status) a.status[k] = ok ? 0 : (int32_t)ESP_E_INVAL;
+    if (!ok) return;
+    wave_decode(a.R, j.contig, j.start, j.end, (j.flags & 1u) != 0, a.dst + j.dst_off, lane);
+}
+
+// Mate rescue, stage 1 (align.rs:857-874): geometry, window decode, read and
+// reverse-complement slots, Myers descriptors.
+__global__ __launch_bounds__(WIN_BLOCK) void k_mate_prep(MateArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (WIN_BLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= a.npairs) return;
+    const uint32_t m = a.read_len[k];
+    const uint32_t ctg = a.mate_contig[k];
+    int32_t state = 1;
+    uint32_t ws = 0, wl = 0;
+    if (m == 0 || m > 256) {
+        state = 0;
+    } else if (m > a.max_read || ctg >= a.R.ncontig) {
+        state = -1;
+    } else {
+        const uint32_t mp = a.mate_pos[k];
+        const uint32_t span = 750u + m;  // MATE_WINDOW_INSERT * 3/2 + read_len
+        const uint32_t wstart = mp > span ? mp - span : 0u;
+        const uint64_t sat = min((uint64_t)mp + span + m, (uint64_t)0xFFFFFFFFu);
+        const uint32_t wend = (uint32_t)min(sat, (uint64_t)a.R.len[ctg]);
+        if (wend <= wstart || wend - wstart < m) {
+            state = 0;
+        } else {
+            ws = wstart;
+            wl = wend - wstart;
+        }
+    }
+    const uint64_t qo = (uint64_t)k * a.qstride;
+    if (lane == 0) {
+        a.state[k] = state;
+        a.wstart[k] = ws;
+        a.w_len[k] = wl;
+        a.w_off[k] = (uint64_t)k * a.wstride;
+        a.q_off_plus[k] = qo;
+        a.q_off_minus[k] = qo + a.qstride / 2;
+        a.q_len[k] = state == 1 ? m : 0u;
+    }
+    if (state != 1) return;
+    wave_decode(a.R, ctg, ws, ws + wl, false, a.warena + (uint64_t)k * a.wstride, lane);
+    wave_read_pair(a.reads + a.read_off[k], m, a.qarena + qo, a.qarena + qo + a.qstride / 2, lane);
+}
+
+// Mate rescue, stage 2 (align.rs:875-902): threshold max(m/3, 10), the
+// start + m <= window guard, Plus first and Minus only when strictly closer;
+// descriptors of the extend_hint batch (read_len 0 when nothing qualifies).
+__global__ __launch_bounds__(WIN_BLOCK) void k_mate_select(MateArgs a) {
+    const uint32_t k = blockIdx.x * WIN_BLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    int32_t sel = 0, d = 0;
+    uint64_t start = 0;
+    if (a.state[k] == 1) {
+        const uint32_t m = a.read_len[k];
+        const int32_t thr = max((int32_t)m / 3, 10);
+        const uint64_t wl = a.w_len[k];
+        if (a.d_plus[k] <= thr && a.s_plus[k] + m <= wl) {
+            sel = 1;
+            d = a.d_plus[k];
+            start = a.s_plus[k];
+        }
+        if (a.d_minus[k] <= thr && a.s_minus[k] + m <= wl && (sel == 0 || a.d_minus[k] < d)) {
+            sel = 2;
+            d = a.d_minus[k];
+            start = a.s_minus[k];
+        }
+    }
+    a.sel[k] = sel;
+    a.dist[k] = d;
+    a.g_read_off[k] = sel == 2 ? a.q_off_minus[k] : a.q_off_plus[k];
+    a.g_read_len[k] = sel ? a.read_len[k] : 0u;
+    a.g_offset[k] = (int64_t)start;
+}
+
+// Mate rescue, stage 3 (align.rs:903-920): placement when the score is positive.
+__global__ __launch_bounds__(WIN_BLOCK) void k_mate_finish(MateArgs a) {
+    const uint32_t k = blockIdx.x * WIN_BLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    const int32_t state = a.state[k], sel = a.sel[k];
+    esp_mate_rec r;
+    r.strand = sel == 2 ? 1 : 0;
+    r.dist = a.dist[k];
+    r.wstart = a.wstart[k];
+    r.wlen = a.w_len[k];
+    if (sel) {
+        const esp_ext e = a.ext[k];
+        r.placed = e.score > 0 ? 1 : 0;
+        r.pos = r.wstart + e.ref_start;
+        r.score = e.score;
+        r.read_start = e.read_start;
+        r.read_end = e.read_end;
+    } else {
+        r.placed = state < 0 ? -1 : 0;
+        r.pos = 0;
+        r.score = 0;
+        r.read_start = 0;
+        r.read_end = 0;
+    }
+    a.out[k] = r;
+    if (r.placed != 1) a.cig_n[k] = 0;
+}
+
+// Spliced pipeline, stage 1 (splice.rs:589-608, intron_chain.rs:497-505):
+// exon checks, exon bounds and the concatenated pseudo-reference.
+__global__ __launch_bounds__(WIN_BLOCK) void k_splice_prep(SpliceArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (WIN_BLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= a.npairs) return;
+    const uint32_t ctg = a.contig[k];
+    const uint64_t e0 = a.exon_off[k], e1 = a.exon_off[k + 1];
+    int32_t state = 1;
+    uint64_t total = 0;
+    if (ctg >= a.R.ncontig || e1 <= e0 || e1 > a.n_exons ||
+        a.pseudo_off[k + 1] > a.pseudo_bytes || a.pseudo_off[k + 1] < a.pseudo_off[k]) {
+        state = -1;
+    } else {
+        const uint32_t clen = a.R.len[ctg];
+        bool rejected = false;
+        for (uint64_t e = e0; e < e1; e++) {
+            const uint32_t rs = a.exons[2 * e], re = a.exons[2 * e + 1];
+            if (re <= rs) rejected = true;
+            else if (re > clen) state = -1;
+            else total += re - rs;
+        }
+        if (state == 1 && rejected) state = 0;
+        if (state == 1 && a.pseudo_off[k + 1] - a.pseudo_off[k] != total) state = -1;
+    }
+    if (lane == 0) {
+        a.state[k] = state;
+        a.p_len[k] = state == 1 ? (uint32_t)total : 0u;
+        a.g_read_len[k] = state == 1 ? a.read_len[k] : 0u;
+    }
+    if (state != 1) return;
+    uint8_t *dst = a.parena + a.pseudo_off[k];
+    uint32_t ps = 0;
+    for (uint64_t e = e0; e < e1; e++) {
+        const uint32_t rs = a.exons[2 * e], re = a.exons[2 * e + 1];
+        if (lane == 0) {
+            a.bounds[3 * e] = ps;
+            a.bounds[3 * e + 1] = rs;
+            a.bounds[3 * e + 2] = re;
+        }
+        wave_decode(a.R, ctg, rs, re, false, dst + ps, lane);
+        ps += re - rs;
+    }
+}
+
+// Spliced pipeline, stage 2 (splice.rs:651-663): stitch the pseudo-reference
+// CIGAR into genomic coordinates and map the start back.
+__global__ __launch_bounds__(WIN_BLOCK) void k_splice_finish(SpliceArgs a) {
+    const uint32_t k = blockIdx.x * WIN_BLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    esp_splice_rec r;
+    r.status = a.state[k];
+    r.pseudo_len = a.p_len[k];
+    r.genomic_start = 0;
+    r.n_stitched = 0;
+    if (r.status == 1) {
+        const uint64_t e0 = a.exon_off[k];
+        const uint32_t nex = (uint32_t)(a.exon_off[k + 1] - e0);
+        const esp_ext e = a.ext[k];
+        const uint32_t *bounds = a.bounds + 3 * e0;
+        const uint64_t so = a.stitch_off[k];
+        const uint32_t cap = (uint32_t)min(a.stitch_off[k + 1] - so, (uint64_t)0xFFFFFFFFu);
+        const uint32_t n = esps::stitch_walk(a.cigar + a.cig_off[k], a.cig_n[k], e.ref_start,
+                                             bounds, nex, nullptr, nex - 1, a.stitched + so, cap);
+        r.n_stitched = n;
+        if (n > cap) r.status = -2;
+        r.genomic_start = esps::genomic_start(e.ref_start, bounds, nex);
+    }
+    a.rec[k] = r;
+}
+
+uint32_t wave_blocks(uint32_t n) { return (n + WIN_BLOCK / 64 - 1) / (WIN_BLOCK / 64); }
+uint32_t lane_blocks(uint32_t n) { return (n + WIN_BLOCK - 1) / WIN_BLOCK; }
+
+}  // namespace
+
+hipError_t launch_gather(const GatherArgs &a, hipStream_t s) {
+    if (a.njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(wave_blocks(a.njobs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mate_prep(const MateArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_mate_prep, dim3(wave_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mate_select(const MateArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_mate_select, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mate_finish(const MateArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_mate_finish, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_splice_prep(const SpliceArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_splice_prep, dim3(wave_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_splice_finish(const SpliceArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_splice_finish, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace espk

@@ -1,0 +1,70 @@
+"""Seeded synthetic inputs for the spliced and mate-rescue window pipelines.
+
+splice_rescue_workload: one uniform-ACGT contig (default 64 Mb); per read
+pair, R1 spans an intron: split u ~ U{1..m-1}, intron length log-uniform in
+[60, 50000] with GT at [s, s+2) and AG at [e-2, e) planted in the contig, R1
+= contig[s-u, s) ++ contig[e, e+m-u).  Its spliced request is the two 300-nt
+anchor windows [(s-300, s), (e, e+300)] with hint 300-u.  R2 is the reverse
+complement of contig[p2, p2+m) at fragment length I ~ N(300, 50) (clipped
+to [m+u, 600]), p2 = e + I - m - u; its mate-rescue request is (R2, contig
+0, e).  Both reads carry 1% substitutions.  All positions come from numpy's
+PCG64 stream seeded with `seed`.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .reference import Reference
+from .rescue import MateRequests
+from .splice import SpliceRequests
+
+_COMP = np.zeros(256, dtype=np.uint8)
+for _a, _b in zip(b"ACGT", b"TGCA"):
+    _COMP[_a] = _b
+
+
+def _substitute(rng, reads: np.ndarray, rate: float) -> None:
+    hit = rng.random(reads.shape) < rate
+    alt = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, int(hit.sum()))]
+    reads[hit] = alt
+
+
+def splice_rescue_workload(npairs: int, ref_len: int = 64_000_000, read_len: int = 150,
+                           flank: int = 300, seed: int = 0xE5A00004, sub_rate: float = 0.01):
+    """Returns (Reference, SpliceRequests, MateRequests, truth) where truth holds
+    the planted s, e, u and R2 position per pair."""
+    rng = np.random.default_rng(seed)
+    m = read_len
+    ref = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, ref_len)].copy()
+    u = rng.integers(1, m, npairs)
+    intron = np.exp(rng.uniform(np.log(60), np.log(50_000), npairs)).astype(np.int64)
+    margin = flank + 1000
+    s = rng.integers(margin, ref_len - intron - margin - 1000)
+    e = s + intron
+    frag = np.clip(np.rint(rng.normal(300, 50, npairs)).astype(np.int64), m + u, 600)
+    p2 = e + frag - m - u
+    ref[s], ref[s + 1] = ord("G"), ord("T")
+    ref[e - 2], ref[e - 1] = ord("A"), ord("G")
+    t = np.arange(m, dtype=np.int64)[None, :]
+    r1 = np.empty((npairs, m), dtype=np.uint8)
+    r2 = np.empty((npairs, m), dtype=np.uint8)
+    for lo in range(0, npairs, 65536):
+        hi = min(npairs, lo + 65536)
+        uu = u[lo:hi, None]
+        idx = np.where(t < uu, s[lo:hi, None] - uu + t, e[lo:hi, None] + t - uu)
+        r1[lo:hi] = ref[idx]
+        fw = ref[p2[lo:hi, None] + t]
+        r2[lo:hi] = _COMP[fw[:, ::-1]]
+    _substitute(rng, r1, sub_rate)
+    _substitute(rng, r2, sub_rate)
+    reference = Reference.from_sequences([ref.tobytes()])
+    rl = np.full(npairs, m, dtype=np.uint32)
+    ro = (np.arange(npairs, dtype=np.uint64) * m)
+    exons = np.stack([s - flank, s, e, e + flank], axis=1).astype(np.uint32).reshape(-1, 2)
+    eo = (np.arange(npairs + 1, dtype=np.uint64) * 2)
+    splice = SpliceRequests(r1.reshape(-1), ro, rl, np.zeros(npairs, dtype=np.uint32),
+                            (flank - u).astype(np.int64), exons, eo)
+    mate = MateRequests(r2.reshape(-1), ro.copy(), rl.copy(), np.zeros(npairs, dtype=np.uint32),
+                        e.astype(np.uint32))
+    truth = {"s": s, "e": e, "u": u, "p2": p2}
+    return reference, splice, mate, truth

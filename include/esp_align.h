@@ -150,8 +150,11 @@ typedef struct {
 int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     const esp_params *p, int mode, esp_batch_out *out);
 
-/* long::infix_best_end (+ infix_best_start when start != NULL) per pair,
- * text = window. */
+/* Per pair, text = window (myers_ea.rs:249-310):
+ *   end != NULL            long::infix_best_end    -> dist, end
+ *   start != NULL          long::infix_best_start  -> dist, start
+ *   both                   both (dist is the same minimum)
+ *   neither                long::infix             -> dist only */
 int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     int32_t *dist, uint64_t *end, uint64_t *start);
 
@@ -160,8 +163,8 @@ int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
 int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
- * kernel, 2 strip kernel, 3 Myers kernel, 4 host round trip between the
- * planner and the alignment kernels), half (band kernel) and ms. */
+ * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
+ * 6 selection / stitching), half (band kernel) and ms. */
 typedef struct {
     int32_t kind;
     int32_t half;
@@ -186,6 +189,112 @@ int esp_device_copy(esp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int esp_stitch_cigar(const uint32_t *cigar, size_t n_ops, uint32_t p_start,
                      const uint32_t *bounds, size_t n_exons, const uint32_t *skips,
                      size_t n_junctions, uint32_t *out, size_t cap, size_t *n_out);
+
+/* ---- resident reference and window pipelines ---------------------------- */
+
+/* 2-bit packed reference in the layout of crates/map/src/fasta.rs:1-135 /
+ * :291-310 (Contig{len, packed, n_intervals}).  Contig k has len[k] bases;
+ * its ceil(len/4) packed bytes start at packed + packed_off[k] (base at pos
+ * in byte pos/4 at shift 6 - 2*(pos%4)); its merged, sorted, half-open N
+ * intervals are (iv[2j], iv[2j+1]) for j in [iv_off[k], iv_off[k+1]).
+ * Host pointers: the reference is copied to the device once and stays
+ * resident until esp_ref_free (the Reference held by the aligner). */
+typedef struct esp_ref esp_ref;
+int esp_ref_upload(esp_ctx *ctx, size_t n_contigs, const uint32_t *len, const uint8_t *packed,
+                   const uint64_t *packed_off, const uint32_t *iv, const uint64_t *iv_off,
+                   esp_ref **out);
+int esp_ref_free(esp_ctx *ctx, esp_ref *ref);
+/* Device bytes the resident reference holds. */
+int esp_ref_device_bytes(const esp_ref *ref, size_t *bytes);
+
+/* Window gather.  Job k writes contig[start, end) as ASCII (Contig::decode_into,
+ * fasta.rs:77-135: A C G T, N over N intervals), reverse-complemented when
+ * flags & 1 (revcomp_into, align.rs:2525-2533), to dst + dst_off.  Jobs and
+ * dst are device pointers.  A job outside its contig writes nothing and
+ * sets status[k] = ESP_E_INVAL (0 otherwise; status may be NULL). */
+typedef struct {
+    uint32_t contig;
+    uint32_t start;
+    uint32_t end;
+    uint32_t flags;
+    uint64_t dst_off;
+} esp_gather_job;
+int esp_gather_batch(esp_ctx *ctx, const esp_ref *ref, const esp_gather_job *jobs, size_t njobs,
+                     uint8_t *dst, int32_t *status);
+
+/* Mate-window rescue, mate_window_placement (crates/map/src/align.rs:851-921),
+ * per pair: window [mate_pos - span, mate_pos + span + m) clipped to the
+ * contig with span = 750 + m (MATE_WINDOW_INSERT = 500, align.rs:181); the
+ * EA-Myers best start of the read and of its reverse complement over the
+ * window, kept when dist <= max(m/3, 10) and start + m <= window length,
+ * Plus unless Minus is strictly closer; extend_hint at that start; placed
+ * when the score is positive.  Device pointers; max_read_len (host) bounds
+ * read_len over the batch and sizes the scratch (at most 256: longer reads
+ * never place). */
+typedef struct {
+    size_t npairs;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *mate_contig;
+    const uint32_t *mate_pos;
+    uint32_t max_read_len;
+} esp_mate_batch;
+typedef struct {
+    int32_t placed;      /* 1 placed, 0 no placement, -1 invalid input (contig
+                            index out of range or read longer than max_read_len) */
+    int32_t strand;      /* 0 Plus, 1 Minus */
+    uint32_t pos;        /* window start + extension ref_start */
+    int32_t score;       /* extension score */
+    int32_t dist;        /* EA-Myers distance of the chosen orientation */
+    uint32_t wstart;     /* window [wstart, wstart + wlen) */
+    uint32_t wlen;
+    uint32_t read_start; /* extension read span */
+    uint32_t read_end;
+} esp_mate_rec;
+/* cigar/cig_off/cig_n: the extension CIGAR arena (device; pair k needs
+ * esp_cigar_capacity(read_len[k]) words); cig_n = 0 when not placed. */
+int esp_mate_rescue_batch(esp_ctx *ctx, const esp_ref *ref, const esp_mate_batch *b,
+                          const esp_params *p, esp_mate_rec *out, uint32_t *cigar,
+                          const uint64_t *cig_off, uint32_t *cig_n);
+
+/* Spliced window DP: the extension and stitching of align_spliced
+ * (splice.rs:625-664, after prepare_spliced) and of the intron-chain fast
+ * path (intron_chain.rs:497-536).  Pair k's pseudo-reference concatenates
+ * contig[k][rs, re) over its exons (exons[2e], exons[2e+1]) for e in
+ * [exon_off[k], exon_off[k+1]); extend_hint(read, pseudo, hint[k]); the CIGAR
+ * is stitched with a RefSkip of rs[e+1] - re[e] at each seam; the genomic
+ * start is SplicedAlignment::genomic_start.  pseudo_off (npairs + 1) holds
+ * the prefix sums of the pairs' exon lengths.  Device pointers except
+ * pseudo_bytes and n_exons (host copies of the last prefix entries). */
+typedef struct {
+    size_t npairs;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const int64_t *hint;
+    const uint32_t *exons;
+    const uint64_t *exon_off;
+    const uint64_t *pseudo_off;
+    uint64_t pseudo_bytes;  /* host: pseudo_off[npairs] */
+    size_t n_exons;         /* host: exon_off[npairs] */
+} esp_splice_batch;
+typedef struct {
+    int32_t status;         /* 1 aligned, 0 rejected (an exon with re <= rs),
+                               -1 invalid (contig or exon out of range, or
+                               pseudo_off inconsistent), -2 stitched CIGAR
+                               capacity too small (n_stitched = required) */
+    uint32_t genomic_start;
+    uint32_t pseudo_len;
+    uint32_t n_stitched;
+} esp_splice_rec;
+/* ext_out: extension records and the pseudo-reference CIGAR arena (as for
+ * esp_gotoh_batch in ESP_MODE_HINT); stitched/stitch_off: the genomic CIGAR
+ * arena (pair k needs esp_cigar_capacity(m) + 2 * n_exons words). */
+int esp_spliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_splice_batch *b,
+                      const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
+                      const uint64_t *stitch_off, esp_splice_rec *rec);
 
 #ifdef __cplusplus
 }

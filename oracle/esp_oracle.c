@@ -9,8 +9,10 @@
  *                                run_dispatch, extend, extend_hint, push_op)
  *   crates/map/src/myers_ea.rs  (build_peq, calculate_block, scan, infix*,
  *                                long::*)
- *   crates/map/src/splice.rs    (stitch_cigar)
- *   crates/map/src/align.rs     (mate_window_placement selection rule)
+ *   crates/map/src/splice.rs    (stitch_cigar, align_spliced after
+ *                                prepare_spliced, SplicedAlignment::genomic_start)
+ *   crates/map/src/fasta.rs     (Contig::decode_into / base / is_n)
+ *   crates/map/src/align.rs     (mate_window_placement, revcomp_into)
  * plus two independent textbook checkers (local affine-gap DP max score over
  * a row-window set, semi-global EA edit distance) used to pin the
  * restatement where the reference has no literal expected outputs.
@@ -684,6 +686,305 @@ int espo_stitch_cigar(const uint32_t *cigar, size_t n_ops, uint32_t p_start,
 #undef EXON_END
     *n_out = o.n;
     return o.overflow ? 2 : 0;
+}
+
+/* ------------------------------------------------------ reference windows */
+
+/* Resident reference in the layout of fasta.rs:62-71: contig k has len[k]
+ * bases, packed bytes at packed + packed_off[k] (base at pos in byte pos/4
+ * at shift 6 - 2*(pos%4)), N intervals (iv[2j], iv[2j+1]) for
+ * j in [iv_off[k], iv_off[k+1]). */
+typedef struct {
+    size_t ncontig;
+    const uint32_t *len;
+    const uint8_t *packed;
+    const uint64_t *packed_off;
+    const uint32_t *iv;
+    const uint64_t *iv_off;
+} espo_ref;
+
+/* Contig::is_n: fasta.rs:119-135 (binary search per base). */
+static int contig_is_n(const espo_ref *R, size_t k, uint32_t pos) {
+    size_t lo = 0, hi = (size_t)(R->iv_off[k + 1] - R->iv_off[k]);
+    const uint32_t *iv = R->iv + 2 * R->iv_off[k];
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (pos < iv[2 * mid]) hi = mid;
+        else if (pos >= iv[2 * mid + 1]) lo = mid + 1;
+        else return 1;
+    }
+    return 0;
+}
+
+/* Contig::base + Base::to_ascii: fasta.rs:105-117, :138-146. */
+static uint8_t contig_base(const espo_ref *R, size_t k, uint32_t pos) {
+    if (contig_is_n(R, k, pos)) return 'N';
+    uint8_t byte = R->packed[R->packed_off[k] + (pos >> 2)];
+    unsigned shift = 6 - 2 * (pos & 3);
+    static const uint8_t ascii[4] = {'A', 'C', 'G', 'T'};
+    return ascii[(byte >> shift) & 3];
+}
+
+/* Base::from_ascii(b).complement().to_ascii(): fasta.rs:30-39, :147-156. */
+static uint8_t comp_ascii(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 'T';
+    case 'C': case 'c': return 'G';
+    case 'G': case 'g': return 'C';
+    case 'T': case 't': return 'A';
+    default: return 'N';
+    }
+}
+
+/* Contig::decode_into (fasta.rs:77-82), or revcomp_into of it
+ * (align.rs:2525-2533) when rc != 0.  Returns 1 when out of range. */
+int espo_decode(const espo_ref *R, uint32_t contig, uint32_t start, uint32_t end, int rc,
+                uint8_t *out) {
+    if (contig >= R->ncontig || start > end || end > R->len[contig]) return 1;
+    uint32_t L = end - start;
+    for (uint32_t k = 0; k < L; k++)
+        out[k] = rc ? comp_ascii(contig_base(R, contig, end - 1 - k)) : contig_base(R, contig, start + k);
+    return 0;
+}
+
+/* revcomp_into: align.rs:2525-2533. */
+void espo_revcomp(const uint8_t *seq, size_t m, uint8_t *out) {
+    for (size_t k = 0; k < m; k++) out[k] = comp_ascii(seq[m - 1 - k]);
+}
+
+typedef struct {
+    int32_t placed, strand;
+    uint32_t pos;
+    int32_t score, dist;
+    uint32_t wstart, wlen, read_start, read_end;
+} espo_mate_rec;
+
+/* mate_window_placement: align.rs:851-921.  placed = -1 for a contig index
+ * out of range (the reference indexes its contig table and panics).  The
+ * m <= 128 route (short::infix_best_start_into) equals the long one. */
+int espo_mate_window(const espo_ref *R, const uint8_t *read, size_t m, uint32_t contig,
+                     uint32_t mate_pos, const espo_params *p, espo_mate_rec *rec, uint32_t *cigar,
+                     size_t cap, size_t *n_ops) {
+    memset(rec, 0, sizeof(*rec));
+    *n_ops = 0;
+    if (m == 0 || m > 256) return 0;
+    if (contig >= R->ncontig) {
+        rec->placed = -1;
+        return 0;
+    }
+    uint32_t read_len = (uint32_t)m;
+    uint32_t clen = R->len[contig];
+    uint32_t span = 500u + 500u / 2u + read_len; /* MATE_WINDOW_INSERT = 500 (align.rs:181) */
+    uint32_t wstart = mate_pos > span ? mate_pos - span : 0u;
+    uint64_t we = (uint64_t)mate_pos + span + read_len;
+    if (we > 0xFFFFFFFFull) we = 0xFFFFFFFFull;
+    uint32_t wend = (uint32_t)(we < clen ? we : clen);
+    if (wend <= wstart || (size_t)(wend - wstart) < m) return 0;
+    size_t n = wend - wstart;
+    uint8_t *win = (uint8_t *)malloc(n), *rc = (uint8_t *)malloc(m);
+    espo_decode(R, contig, wstart, wend, 0, win);
+    espo_revcomp(read, m, rc);
+    rec->wstart = wstart;
+    rec->wlen = (uint32_t)n;
+    int32_t threshold = (int32_t)read_len / 3;
+    if (threshold < 10) threshold = 10;
+    int best = 0; /* 0 none, 1 Plus, 2 Minus */
+    int32_t best_d = 0;
+    size_t best_start = 0;
+    for (int strand = 1; strand <= 2; strand++) {
+        const uint8_t *q = strand == 1 ? read : rc;
+        int32_t d;
+        size_t start;
+        espo_myers_best_start(q, m, win, n, &d, &start);
+        if (d > threshold || start + m > n) continue;
+        if (best == 0 || d < best_d) {
+            best = strand;
+            best_d = d;
+            best_start = start;
+        }
+    }
+    int rc_code = 0;
+    if (best) {
+        espo_ext e;
+        int32_t esc;
+        size_t nops = 0;
+        rc_code = espo_extend_hint(best == 1 ? read : rc, m, win, n, p, (int64_t)best_start, &e,
+                                   cigar, cap, &nops, &esc);
+        rec->strand = best == 2;
+        rec->dist = best_d;
+        rec->pos = wstart + e.ref_start;
+        rec->score = e.score;
+        rec->read_start = e.read_start;
+        rec->read_end = e.read_end;
+        rec->placed = e.score > 0;
+        *n_ops = rec->placed ? nops : 0;
+    }
+    free(win);
+    free(rc);
+    return rc_code;
+}
+
+typedef struct {
+    int32_t status;
+    uint32_t genomic_start, pseudo_len, n_stitched;
+} espo_splice_rec;
+
+/* SplicedAlignment::genomic_start: splice.rs:414-425. */
+static uint32_t genomic_start_of(uint32_t p, const uint32_t *bounds, size_t n_exons) {
+    for (size_t e = 0; e < n_exons; e++) {
+        uint32_t ps = bounds[3 * e], rs = bounds[3 * e + 1], re = bounds[3 * e + 2];
+        if (p >= ps && p < ps + (re - rs)) return rs + (p - ps);
+    }
+    uint32_t ps = bounds[3 * (n_exons - 1)], rs = bounds[3 * (n_exons - 1) + 1];
+    return rs + (p > ps ? p - ps : 0u);
+}
+
+/* Spliced window DP: the pseudo-reference concatenation of prepare_spliced
+ * (splice.rs:589-608) / intron_chain.rs:497-505 over the given exons, then
+ * align_spliced's extend_hint + stitch_cigar (splice.rs:625-664) with the
+ * junction between adjacent exons.  status: 1 aligned, 0 an exon with
+ * re <= rs (prepare_spliced returns None), -1 no exons or an exon outside
+ * the contig, -2 stitched CIGAR capacity too small. */
+int espo_spliced(const espo_ref *R, const uint8_t *read, size_t m, uint32_t contig,
+                 const uint32_t *exons, size_t n_exons, int64_t hint, const espo_params *p,
+                 espo_ext *ext, uint32_t *cigar, size_t cap, size_t *n_ops, uint32_t *stitched,
+                 size_t scap, espo_splice_rec *rec) {
+    memset(rec, 0, sizeof(*rec));
+    memset(ext, 0, sizeof(*ext));
+    *n_ops = 0;
+    if (contig >= R->ncontig || n_exons == 0) {
+        rec->status = -1;
+        return 0;
+    }
+    uint64_t total = 0;
+    int rejected = 0;
+    for (size_t e = 0; e < n_exons; e++) {
+        uint32_t rs = exons[2 * e], re = exons[2 * e + 1];
+        if (re <= rs) rejected = 1;
+        else if (re > R->len[contig]) { rec->status = -1; return 0; }
+        else total += re - rs;
+    }
+    if (rejected) return 0;
+    uint8_t *pseudo = (uint8_t *)malloc(total ? total : 1);
+    uint32_t *bounds = (uint32_t *)malloc(3 * n_exons * sizeof(uint32_t));
+    uint32_t *skips = (uint32_t *)malloc(n_exons * sizeof(uint32_t));
+    uint32_t ps = 0;
+    for (size_t e = 0; e < n_exons; e++) {
+        uint32_t rs = exons[2 * e], re = exons[2 * e + 1];
+        bounds[3 * e] = ps;
+        bounds[3 * e + 1] = rs;
+        bounds[3 * e + 2] = re;
+        espo_decode(R, contig, rs, re, 0, pseudo + ps);
+        ps += re - rs;
+        if (e + 1 < n_exons) skips[e] = exons[2 * (e + 1)] - re;
+    }
+    int32_t esc;
+    int rc = espo_extend_hint(read, m, pseudo, total, p, hint, ext, cigar, cap, n_ops, &esc);
+    size_t ns = 0;
+    int src = espo_stitch_cigar(cigar, *n_ops, ext->ref_start, bounds, n_exons, skips,
+                                n_exons - 1, stitched, scap, &ns);
+    rec->status = src ? -2 : 1;
+    rec->n_stitched = (uint32_t)ns;
+    rec->pseudo_len = (uint32_t)total;
+    rec->genomic_start = genomic_start_of(ext->ref_start, bounds, n_exons);
+    free(pseudo);
+    free(bounds);
+    free(skips);
+    return rc;
+}
+
+typedef struct {
+    const espo_ref *R;
+    const espo_params *p;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *mate_pos;
+    const int64_t *hint;
+    const uint32_t *exons;
+    const uint64_t *exon_off;
+    espo_mate_rec *mrec;
+    espo_splice_rec *srec;
+    espo_ext *ext;
+    uint32_t *cigar;
+    const uint64_t *cig_off;
+    uint32_t *cig_n;
+    uint32_t *stitched;
+    const uint64_t *stitch_off;
+    size_t lo, hi;
+} window_job;
+
+static void *mate_worker(void *arg) {
+    window_job *J = (window_job *)arg;
+    for (size_t k = J->lo; k < J->hi; k++) {
+        size_t nops = 0;
+        espo_mate_window(J->R, J->reads + J->read_off[k], J->read_len[k], J->contig[k],
+                         J->mate_pos[k], J->p, &J->mrec[k], J->cigar + J->cig_off[k],
+                         (size_t)(J->cig_off[k + 1] - J->cig_off[k]), &nops);
+        J->cig_n[k] = (uint32_t)nops;
+    }
+    return NULL;
+}
+
+static void *splice_worker(void *arg) {
+    window_job *J = (window_job *)arg;
+    for (size_t k = J->lo; k < J->hi; k++) {
+        size_t nops = 0;
+        espo_spliced(J->R, J->reads + J->read_off[k], J->read_len[k], J->contig[k],
+                     J->exons + 2 * J->exon_off[k], (size_t)(J->exon_off[k + 1] - J->exon_off[k]),
+                     J->hint[k], J->p, &J->ext[k], J->cigar + J->cig_off[k],
+                     (size_t)(J->cig_off[k + 1] - J->cig_off[k]), &nops,
+                     J->stitched + J->stitch_off[k],
+                     (size_t)(J->stitch_off[k + 1] - J->stitch_off[k]), &J->srec[k]);
+        J->cig_n[k] = (uint32_t)nops;
+    }
+    return NULL;
+}
+
+static void run_window_jobs(window_job *tmpl, size_t npairs, int nthreads, void *(*fn)(void *)) {
+    if (nthreads < 1) nthreads = 1;
+    window_job *jobs = (window_job *)calloc((size_t)nthreads, sizeof(window_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    size_t chunk = (npairs + (size_t)nthreads - 1) / (size_t)nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t] = *tmpl;
+        jobs[t].lo = (size_t)t * chunk < npairs ? (size_t)t * chunk : npairs;
+        jobs[t].hi = jobs[t].lo + chunk < npairs ? jobs[t].lo + chunk : npairs;
+        pthread_create(&th[t], NULL, fn, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(jobs);
+    free(th);
+}
+
+/* mate_window_placement over a batch (nthreads POSIX threads). */
+void espo_mate_batch(const espo_ref *R, size_t npairs, const uint8_t *reads,
+                     const uint64_t *read_off, const uint32_t *read_len, const uint32_t *contig,
+                     const uint32_t *mate_pos, const espo_params *p, espo_mate_rec *rec,
+                     uint32_t *cigar, const uint64_t *cig_off, uint32_t *cig_n, int nthreads) {
+    window_job J;
+    memset(&J, 0, sizeof(J));
+    J.R = R; J.p = p; J.reads = reads; J.read_off = read_off; J.read_len = read_len;
+    J.contig = contig; J.mate_pos = mate_pos; J.mrec = rec; J.cigar = cigar; J.cig_off = cig_off;
+    J.cig_n = cig_n;
+    run_window_jobs(&J, npairs, nthreads, mate_worker);
+}
+
+/* Spliced window DP over a batch (nthreads POSIX threads). */
+void espo_spliced_batch(const espo_ref *R, size_t npairs, const uint8_t *reads,
+                        const uint64_t *read_off, const uint32_t *read_len, const uint32_t *contig,
+                        const int64_t *hint, const uint32_t *exons, const uint64_t *exon_off,
+                        const espo_params *p, espo_ext *ext, uint32_t *cigar,
+                        const uint64_t *cig_off, uint32_t *cig_n, uint32_t *stitched,
+                        const uint64_t *stitch_off, espo_splice_rec *rec, int nthreads) {
+    window_job J;
+    memset(&J, 0, sizeof(J));
+    J.R = R; J.p = p; J.reads = reads; J.read_off = read_off; J.read_len = read_len;
+    J.contig = contig; J.hint = hint; J.exons = exons; J.exon_off = exon_off; J.ext = ext;
+    J.cigar = cigar; J.cig_off = cig_off; J.cig_n = cig_n; J.stitched = stitched;
+    J.stitch_off = stitch_off; J.srec = rec;
+    run_window_jobs(&J, npairs, nthreads, splice_worker);
 }
 
 /* ------------------------------------------------- independent checkers */

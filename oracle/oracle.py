@@ -21,6 +21,13 @@ LIB_PATH = os.path.join(_HERE, "lib", "libesp_oracle.so")
 _lib = None
 
 
+class CRef(ctypes.Structure):
+    """espo_ref: the resident-reference layout of fasta.rs:62-71, flattened."""
+    _fields_ = [("ncontig", ctypes.c_size_t), ("len", ctypes.c_void_p), ("packed", ctypes.c_void_p),
+                ("packed_off", ctypes.c_void_p), ("iv", ctypes.c_void_p),
+                ("iv_off", ctypes.c_void_p)]
+
+
 def build() -> None:
     import subprocess
     subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
@@ -57,6 +64,18 @@ def lib():
                                       u32p, i32p, ctypes.c_int]
         L.espo_myers_batch.argtypes = [sz, u8p, u64p, u32p, u8p, u64p, u32p, i32p, u64p, u64p,
                                        ctypes.c_int]
+        vp = ctypes.c_void_p
+        L.espo_decode.argtypes = [P(CRef), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_int, u8p]
+        L.espo_revcomp.argtypes = [u8p, sz, u8p]
+        L.espo_mate_window.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, ctypes.c_uint32,
+                                       P(CParams), vp, u32p, sz, P(sz)]
+        L.espo_spliced.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, u32p, sz, ctypes.c_int64,
+                                   P(CParams), P(CExt), u32p, sz, P(sz), u32p, sz, vp]
+        L.espo_mate_batch.argtypes = [P(CRef), sz, vp, vp, vp, vp, vp, P(CParams), vp, vp, vp, vp,
+                                      ctypes.c_int]
+        L.espo_spliced_batch.argtypes = [P(CRef), sz, vp, vp, vp, vp, vp, vp, vp, P(CParams), vp,
+                                         vp, vp, vp, vp, vp, vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -224,3 +243,117 @@ def myers_batch(batch, with_start: bool = True, nthreads: int = 1):
                            _ptr(dist, ctypes.c_int32), _ptr(end, ctypes.c_uint64),
                            _ptr(start, ctypes.c_uint64) if with_start else None, nthreads)
     return dist, end, start
+
+
+# ------------------------------------------------------------ reference windows
+
+class OracleRef:
+    """Keeps the numpy arrays of an esperanto_amd.reference.Reference alive
+    behind the espo_ref view."""
+
+    def __init__(self, ref):
+        self.arrs = [np.ascontiguousarray(ref.lens, dtype=np.uint32),
+                     np.ascontiguousarray(ref.packed, dtype=np.uint8) if ref.packed.size else np.zeros(1, np.uint8),
+                     np.ascontiguousarray(ref.packed_off, dtype=np.uint64),
+                     np.ascontiguousarray(ref.iv.reshape(-1), dtype=np.uint32) if ref.iv.size else np.zeros(2, np.uint32),
+                     np.ascontiguousarray(ref.iv_off, dtype=np.uint64)]
+        self.c = CRef(ref.ncontig, *(a.ctypes.data for a in self.arrs))
+
+
+def decode(oref: OracleRef, contig: int, start: int, end: int, rc: bool = False) -> Optional[bytes]:
+    """Contig::decode_into (fasta.rs:77-82), revcomp_into of it when rc; None out of range."""
+    n = max(end - start, 0)
+    out = (ctypes.c_uint8 * max(n, 1))()
+    bad = lib().espo_decode(ctypes.byref(oref.c), contig, start, end, int(rc), out)
+    return None if bad else bytes(out[:n])
+
+
+def revcomp(seq: bytes) -> bytes:
+    """align.rs:2525 revcomp_into"""
+    s, m = _buf(seq)
+    out = (ctypes.c_uint8 * max(m, 1))()
+    lib().espo_revcomp(s, m, out)
+    return bytes(out[:m])
+
+
+MATE_REC_DTYPE = np.dtype([("placed", "<i4"), ("strand", "<i4"), ("pos", "<u4"), ("score", "<i4"),
+                           ("dist", "<i4"), ("wstart", "<u4"), ("wlen", "<u4"),
+                           ("read_start", "<u4"), ("read_end", "<u4")])
+SPLICE_REC_DTYPE = np.dtype([("status", "<i4"), ("genomic_start", "<u4"), ("pseudo_len", "<u4"),
+                             ("n_stitched", "<u4")])
+
+
+def mate_window(oref: OracleRef, read: bytes, contig: int, mate_pos: int,
+                params: Optional[ExtendParams] = None):
+    """align.rs:851 mate_window_placement: (record, cigar ops)."""
+    r, m = _buf(read)
+    rec = np.zeros(1, dtype=MATE_REC_DTYPE)
+    cap = cigar_capacity(m)
+    cig = (ctypes.c_uint32 * cap)()
+    n = ctypes.c_size_t()
+    rc = lib().espo_mate_window(ctypes.byref(oref.c), r, m, contig, mate_pos,
+                                ctypes.byref(_params(params)), rec.ctypes.data, cig, cap,
+                                ctypes.byref(n))
+    assert rc == 0, rc
+    return rec[0], cigar_from_words(cig[: n.value])
+
+
+def spliced(oref: OracleRef, read: bytes, contig: int, exons: Sequence[Tuple[int, int]], hint: int,
+            params: Optional[ExtendParams] = None):
+    """Spliced window DP (splice.rs:589-664): (record, Extension, stitched ops)."""
+    r, m = _buf(read)
+    flat = [x for e in exons for x in e]
+    ex = (ctypes.c_uint32 * max(len(flat), 1))(*flat)
+    cap = cigar_capacity(m)
+    cig = (ctypes.c_uint32 * cap)()
+    scap = cap + 2 * len(exons)
+    st = (ctypes.c_uint32 * scap)()
+    e, n = CExt(), ctypes.c_size_t()
+    rec = np.zeros(1, dtype=SPLICE_REC_DTYPE)
+    rc = lib().espo_spliced(ctypes.byref(oref.c), r, m, contig, ex, len(exons), hint,
+                            ctypes.byref(_params(params)), ctypes.byref(e), cig, cap,
+                            ctypes.byref(n), st, scap, rec.ctypes.data)
+    assert rc == 0, rc
+    return rec[0], Extension.from_c(e, cig[: n.value]), cigar_from_words(st[: rec[0]["n_stitched"]])
+
+
+def mate_batch(oref: OracleRef, req, params: Optional[ExtendParams] = None, nthreads: int = 1):
+    """mate_window_placement over esperanto_amd.rescue.MateRequests:
+    (records, cigar words, cig_off, cig_n)."""
+    n = req.npairs
+    rec = np.zeros(max(n, 1), dtype=MATE_REC_DTYPE)
+    off = req.cigar_offsets()
+    cig = np.zeros(int(off[-1]) + 1, dtype=np.uint32)
+    cig_n = np.zeros(max(n, 1), dtype=np.uint32)
+    arrs = [np.ascontiguousarray(a) for a in (req.reads, req.read_off, req.read_len, req.contig,
+                                              req.mate_pos)]
+    lib().espo_mate_batch(ctypes.byref(oref.c), n, *(a.ctypes.data for a in arrs),
+                          ctypes.byref(_params(params)), rec.ctypes.data, cig.ctypes.data,
+                          off.ctypes.data, cig_n.ctypes.data, nthreads)
+    return rec[:n], cig, off, cig_n[:n]
+
+
+def spliced_batch(oref: OracleRef, req, params: Optional[ExtendParams] = None, nthreads: int = 1):
+    """Spliced window DP over esperanto_amd.splice.SpliceRequests:
+    (records, ext[n,4], cigar, cig_off, cig_n, stitched, stitch_off)."""
+    n = req.npairs
+    rec = np.zeros(max(n, 1), dtype=SPLICE_REC_DTYPE)
+    off = req.cigar_offsets()
+    soff = req.stitch_offsets()
+    cig = np.zeros(int(off[-1]) + 1, dtype=np.uint32)
+    st = np.zeros(int(soff[-1]) + 1, dtype=np.uint32)
+    cig_n = np.zeros(max(n, 1), dtype=np.uint32)
+    ext = np.zeros((max(n, 1), 4), dtype=np.uint32)
+    exons = np.ascontiguousarray(req.exons.reshape(-1), dtype=np.uint32)
+    if exons.size == 0:
+        exons = np.zeros(2, np.uint32)
+    arrs = [np.ascontiguousarray(a) for a in (req.reads, req.read_off, req.read_len, req.contig,
+                                              req.hint)]
+    lib().espo_spliced_batch(ctypes.byref(oref.c), n, *(a.ctypes.data for a in arrs),
+                             exons.ctypes.data, np.ascontiguousarray(req.exon_off).ctypes.data,
+                             ctypes.byref(_params(params)), ext.ctypes.data, cig.ctypes.data,
+                             off.ctypes.data, cig_n.ctypes.data, st.ctypes.data, soff.ctypes.data,
+                             rec.ctypes.data, nthreads)
+    e = ext[:n].astype(np.int64)
+    e[:, 3] = ext[:n, 3].view(np.int32)
+    return rec[:n], e, cig, off, cig_n[:n], st, soff
