@@ -82,6 +82,73 @@ def read_traffic(path: str):
     return None
 
 
+def bench_window(args, buf, params):
+    """Spliced window DP and mate-window rescue over the 2x150-nt pair workload
+    (R1 across a GT-AG intron of 60-50,000 nt: two 300-nt anchor windows; R2
+    rescued in the mate window, reference geometry 2*(750+m)+m nt)."""
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.rescue import DeviceMateBatch
+    from esperanto_amd.splice import DeviceSpliceBatch
+    from esperanto_amd.synth_window import splice_rescue_workload
+    n = args.window_pairs
+    log(f"splice/rescue workload: {n} pairs on a {args.window_ref_mb} Mb reference")
+    ref, sp, mt, _ = splice_rescue_workload(n, ref_len=args.window_ref_mb * 1_000_000)
+    dref = DeviceReference(ref, buf)
+    out = {"pairs": n, "reference_bases": int(ref.lens.sum()),
+           "reference_device_bytes": dref.device_bytes()}
+    for name, db in (("spliced", DeviceSpliceBatch(sp, dref, buf)),
+                     ("mate_rescue", DeviceMateBatch(mt, dref, buf))):
+        w, k, tims = timed_steps(lambda: db.launch(params), db, max(args.steps, 3), 1)
+        stages = {}
+        for st in tims:
+            for kind, half, ms in st:
+                key = {0: "plan", 1: f"band{half}", 2: "strip", 3: "myers", 5: "prep",
+                       6: "select_finish", 7: "escalated_full"}.get(kind, str(kind))
+                stages.setdefault(key, []).append(ms)
+        nsteps = len(tims)
+        stages = {k2: round(float(np.sum(v)) / nsteps, 3) for k2, v in stages.items()}
+        out[name] = {"pairs_per_s": round(n / w), "ms_per_step": round(w * 1e3, 3),
+                     "device_ms": round(k, 3), "stages_ms": stages}
+        log(f"{name}: {n / w / 1e6:.2f} M pairs/s, {w * 1e3:.2f} ms/step, stages {stages}")
+        if name == "mate_rescue":
+            rec = db.raw_results()[0]
+            out[name]["placed_fraction"] = round(float((rec["placed"] == 1).mean()), 4)
+        db.free()
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.OracleRef(ref)
+        nsample = min(n, args.window_cpu_pairs)
+        sub_sp = _subset_splice(sp, nsample)
+        sub_mt = _subset_mate(mt, nsample)
+        t0 = time.perf_counter()
+        oracle.spliced_batch(o, sub_sp, params, nthreads=threads)
+        t1 = time.perf_counter()
+        oracle.mate_batch(o, sub_mt, params, nthreads=threads)
+        t2 = time.perf_counter()
+        out["cpu_port"] = {"spliced_pairs_per_s": round(nsample / (t1 - t0)),
+                           "mate_rescue_pairs_per_s": round(nsample / (t2 - t1)),
+                           "cores": threads, "sample_pairs": nsample}
+        log(f"cpu port: {out['cpu_port']}")
+    dref.close()
+    return out
+
+
+def _subset_splice(sp, k):
+    from esperanto_amd.splice import SpliceRequests
+    e1 = int(sp.exon_off[k])
+    r1 = int(sp.read_off[k - 1] + sp.read_len[k - 1])
+    return SpliceRequests(sp.reads[:r1], sp.read_off[:k], sp.read_len[:k], sp.contig[:k],
+                          sp.hint[:k], sp.exons[:e1], sp.exon_off[:k + 1])
+
+
+def _subset_mate(mt, k):
+    from esperanto_amd.rescue import MateRequests
+    r1 = int(mt.read_off[k - 1] + mt.read_len[k - 1])
+    return MateRequests(mt.reads[:r1], mt.read_off[:k], mt.read_len[:k], mt.contig[:k],
+                        mt.mate_pos[:k])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +159,12 @@ def main() -> int:
     ap.add_argument("--plain-calls", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--window-pairs", type=int, default=500_000,
+                    help="read pairs of the splice + mate-rescue leg (0 skips it)")
+    ap.add_argument("--window-ref-mb", type=int, default=64)
+    ap.add_argument("--window-cpu-pairs", type=int, default=20_000)
+    ap.add_argument("--only-window", action="store_true",
+                    help="run only the splice + mate-rescue leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -109,6 +182,10 @@ def main() -> int:
 
     params = ExtendParams()
     buf = ExtendBuffer(0)
+    if args.only_window:
+        print(json.dumps(bench_window(args, buf, params)), flush=True)
+        buf.close()
+        return 0
 
     log(f"generating {args.pairs} uneven pairs")
     hb = workload_c3(args.pairs)
@@ -176,6 +253,11 @@ def main() -> int:
     log(f"myers: {myers_pps/1e6:.1f} M pairs/s (kernel {k2:.2f} ms)")
     d2.free()
 
+    # Splice + mate-rescue window pipelines on a 64 Mb reference.
+    win = None
+    if args.window_pairs:
+        win = bench_window(args, buf, params)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
@@ -223,7 +305,8 @@ def main() -> int:
                    "batched_over_plain": round(c1_batched_pps / plain_pps, 1),
                    "myers_pairs_per_s": round(myers_pps),
                    "myers_kernel_ms": round(k2, 3),
-                   "myers_valu_tops": round(myers_tops, 2)},
+                   "myers_valu_tops": round(myers_tops, 2),
+                   "splice_rescue": win},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
                      "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),

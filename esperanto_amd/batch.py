@@ -190,7 +190,29 @@ class DeviceArray:
             pass
 
 
-class DeviceBatch:
+class ContextTimings:
+    """Device times of the last batch call on self.buf's context."""
+
+    def sync(self) -> None:
+        _native.check(_native.lib().esp_sync(self.buf.handle), self.buf.handle)
+
+    def last_kernel_ms(self) -> float:
+        ms = ctypes.c_double()
+        _native.check(_native.lib().esp_last_kernel_ms(self.buf.handle, ctypes.byref(ms)),
+                      self.buf.handle)
+        return ms.value
+
+    def last_timings(self):
+        """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,
+        3 Myers, 5 window preparation, 6 selection / stitching."""
+        arr = (_native.CTiming * 16)()
+        n = ctypes.c_size_t()
+        _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),
+                      self.buf.handle)
+        return [(arr[k].kind, arr[k].half, arr[k].ms) for k in range(min(n.value, 16))]
+
+
+class DeviceBatch(ContextTimings):
     """A PairBatch resident in HBM plus its result arena."""
 
     def __init__(self, host: PairBatch, buf: Optional[ExtendBuffer] = None):
@@ -235,23 +257,6 @@ class DeviceBatch:
             self.buf.handle, ctypes.byref(self._cb), ctypes.byref(self._meta),
             ctypes.c_void_p(self.dist.ptr), ctypes.c_void_p(self.end.ptr),
             ctypes.c_void_p(self.start.ptr) if with_start else None), self.buf.handle)
-
-    def sync(self) -> None:
-        _native.check(_native.lib().esp_sync(self.buf.handle), self.buf.handle)
-
-    def last_kernel_ms(self) -> float:
-        ms = ctypes.c_double()
-        _native.check(_native.lib().esp_last_kernel_ms(self.buf.handle, ctypes.byref(ms)),
-                      self.buf.handle)
-        return ms.value
-
-    def last_timings(self):
-        """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip, 3 Myers."""
-        arr = (_native.CTiming * 16)()
-        n = ctypes.c_size_t()
-        _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),
-                      self.buf.handle)
-        return [(arr[k].kind, arr[k].half, arr[k].ms) for k in range(min(n.value, 16))]
 
     # ---- results ----
     def gotoh_results(self):

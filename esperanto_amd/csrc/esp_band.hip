@@ -329,7 +329,15 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                 }
             }
         }
-        if (active) {
+        // extend_hint (extend.rs:420-426): a band maximum on the band edge, or
+        // a score under 85% of perfect, sends the pair to the full matrix.
+        const bool escalate =
+            active && a.mode == ESP_MODE_HINT && (touched || best * 20 < (m * P.match) * 17);
+        if (escalate) {
+            a.esc_list[atomicAdd(&a.esc_range[1], 1u)] = p;
+            if (a.aux) a.aux[p] = 1;
+        }
+        if (active && !escalate) {
             const uint64_t c0 = a.cig_off[p];
             const uint32_t cap = (uint32_t)(a.cig_off[p + 1] - c0);
             uint32_t *cslot = a.cigar + c0;

@@ -168,7 +168,9 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
 
     // 1. Device planner: classify, bucket by (class, read length), order.
     HIP_TRY(c, c->plan.ensure(sizeof(espk::PlanDev)));
-    HIP_TRY(c, c->order.ensure(b->npairs * 4));
+    const bool hint_band = mode == ESP_MODE_HINT && band_lut_fits(p);
+    // order: the planner's visiting order, then (extend_hint) the escalation list
+    HIP_TRY(c, c->order.ensure(b->npairs * 4 * (hint_band ? 2 : 1)));
     HIP_TRY(c, c->hplan.ensure(sizeof(espk::PlanSummary)));
     espk::PlanArgs pa{};
     pa.npairs = a.npairs;
@@ -177,6 +179,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     pa.offset = b->offset;
     pa.half = b->half;
     pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
+    pa.hint_mode = hint_band;
     pa.max_read = kMaxRead;
     pa.max_win = (uint32_t)kMaxWin;
     pa.plan = (espk::PlanDev *)c->plan.p;
@@ -190,11 +193,15 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     HIP_TRY(c, hipEventRecord(c->ev_plan, c->stream));
     a.order = pa.order;
     espk::PlanDev *dp = pa.plan;
+    a.esc_range = dp->esc_range;
+    a.esc_list = pa.order + b->npairs;
 
-    // 2. Band kernel per half specialisation (an empty class exits at once).
-    if (pa.band_ok) {
+    // 2. Band kernel per half specialisation (an empty class exits at once);
+    //    extend_hint's band attempt uses only the half-30 class.
+    if (pa.band_ok || pa.hint_mode) {
         for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
             const int half = espk::plan_class_half(cls);
+            if (pa.hint_mode && half != 30) continue;
             const uint64_t W = 2 * (uint64_t)half + 1;
             const uint64_t maxw = espk::band_trace_words(espk::band_span(half) - W, half);
             if (!c->band_blocks[cls]) c->band_blocks[cls] = espk::gotoh_band_blocks_per_cu(half);
@@ -222,6 +229,19 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         rc = strip_launch(c, a, hp->range[espk::PLAN_STRIP][1], hp->strip_maxm, hp->strip_maxw);
         if (rc) return rc;
         tim_mark(c, 2, 0);
+    }
+    // 4. extend_hint attempt 2: the full matrix for every band pair that
+    //    escalated (its count is known only on the device; the grid drains
+    //    the list and exits).  aux already holds 1 for these pairs.
+    if (pa.hint_mode && hp->range[1][1]) {
+        espk::GotohArgs f = a;
+        f.mode = ESP_MODE_FULL;
+        f.aux = nullptr;
+        f.range = dp->esc_range;
+        f.counter = &dp->counters[espk::PLAN_NCLASS];
+        rc = strip_launch(c, f, hp->range[1][1], hp->esc_maxm, hp->esc_maxw);
+        if (rc) return rc;
+        tim_mark(c, 7, 0);
     }
     return ESP_OK;
 }

@@ -338,7 +338,13 @@ constexpr int kPlanHalves[5] = {16, 30, 32, 48, 64};
 __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint32_t &cls) {
     const uint32_t m = a.read_len[p], n = a.win_len[p];
     cls = PLAN_STRIP;
-    if (a.band_ok && m >= 1) {
+    if (a.hint_mode && m >= 1 && n > 30u) {
+        // extend_hint attempt 1 is the band (hint, 30) (extend.rs:408-419):
+        // interior ones run in the half-30 band kernel.
+        const int64_t off = a.offset[p];
+        if (off >= 30 && off + (int64_t)m + 30 <= (int64_t)n && (int64_t)m + 61 <= band_span(30))
+            cls = 1;
+    } else if (a.band_ok && m >= 1) {
         const int64_t off = a.offset[p], half = a.half[p];
         int c = -1;
 #pragma unroll
@@ -351,28 +357,62 @@ __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint
     return cls * PLAN_NBIN + (PLAN_NBIN - 1 - min(m, (uint32_t)PLAN_NBIN - 1));
 }
 
+// Wave-aggregated increment of counter[b]: lanes that share a bin add once
+// (one atomic per distinct bin in the wave, not per lane); returns the old
+// value plus this lane's rank among them.  Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t agg_inc(uint32_t *counter, uint32_t b, bool active) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t pending = __ballot(active);
+    uint32_t mine = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lb = __shfl(b, leader);
+        const uint64_t same = __ballot(active && b == lb) & pending;
+        uint32_t old = 0;
+        if ((int)lane == leader) old = atomicAdd(&counter[lb], (uint32_t)__popcll(same));
+        old = __shfl(old, leader);
+        if ((same >> lane) & 1ull) mine = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return mine;
+}
+
 __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
     __shared__ uint32_t h[PLAN_NCLASS * PLAN_NBIN];
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x) h[t] = 0;
     __syncthreads();
-    uint32_t smax = 0;
-    uint64_t wmax = 0;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npairs; p += gridDim.x * blockDim.x) {
-        uint32_t cls;
-        const uint32_t b = plan_bin(a, p, cls);
-        atomicAdd(&h[b], 1u);
-        if (a.read_len[p] > a.max_read || a.win_len[p] > a.max_win) atomicAdd(&a.plan->sum.bad, 1u);
-        if (cls == PLAN_STRIP) {
+    uint32_t smax = 0, emax = 0;
+    uint64_t wmax = 0, ewmax = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // The loop bound is uniform per wave, so agg_inc sees whole waves.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.npairs; base += stride) {
+        const uint32_t p = base + threadIdx.x;
+        const bool active = p < a.npairs;
+        uint32_t cls = PLAN_STRIP, b = 0;
+        if (active) {
+            b = plan_bin(a, p, cls);
+            if (a.read_len[p] > a.max_read || a.win_len[p] > a.max_win) atomicAdd(&a.plan->sum.bad, 1u);
             const uint32_t m = a.read_len[p];
-            smax = max(smax, m);
-            wmax = max(wmax, strip_trace_words(m, a.win_len[p]));
+            if (cls == PLAN_STRIP) {
+                smax = max(smax, m);
+                wmax = max(wmax, strip_trace_words(m, a.win_len[p]));
+            } else if (a.hint_mode) {
+                // a band attempt that escalates reruns on the full matrix
+                emax = max(emax, m);
+                ewmax = max(ewmax, strip_trace_words(m, a.win_len[p]));
+            }
         }
+        (void)agg_inc(h, b, active);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x)
         if (h[t]) atomicAdd(&a.plan->hist[t], h[t]);
     atomicMax(&a.plan->sum.strip_maxm, smax);
     atomicMax(&a.plan->sum.strip_maxw, (unsigned long long)wmax);
+    if (a.hint_mode) {
+        atomicMax(&a.plan->sum.esc_maxm, emax);
+        atomicMax(&a.plan->sum.esc_maxw, (unsigned long long)ewmax);
+    }
 }
 
 // Exclusive scan of the bins (one block), class ranges, cursors.
@@ -413,14 +453,18 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs a) {
         for (int b = 0; b < PLAN_NBIN; b++) c += a.plan->hist[t * PLAN_NBIN + b];
         a.plan->sum.range[t][1] = c;
     }
+    if (t == 0) a.plan->esc_range[0] = a.npairs;  // escalation list follows the order
 }
 
 __global__ __launch_bounds__(256) void k_plan_scatter(PlanArgs a) {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < a.npairs; p += gridDim.x * blockDim.x) {
-        uint32_t cls;
-        const uint32_t b = plan_bin(a, p, cls);
-        const uint32_t pos = atomicAdd(&a.plan->cursor[b], 1u);
-        a.order[pos] = p;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.npairs; base += stride) {
+        const uint32_t p = base + threadIdx.x;
+        const bool active = p < a.npairs;
+        uint32_t cls = PLAN_STRIP, b = 0;
+        if (active) b = plan_bin(a, p, cls);
+        const uint32_t pos = agg_inc(a.plan->cursor, b, active);
+        if (active) a.order[pos] = p;
     }
 }
 

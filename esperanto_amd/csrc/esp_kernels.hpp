@@ -41,6 +41,11 @@ struct GotohArgs {
     // Band kernel substitution rows: for read code 0..4, 5-bit signed scores
     // of window codes 0..5 packed at bit 5*code.
     uint32_t lut[5];
+    // extend_hint escalation (band kernel in ESP_MODE_HINT): pairs whose band
+    // attempt must be redone on the full matrix are appended to esc_list
+    // through the counter esc_range[1].
+    uint32_t *esc_range;
+    uint32_t *esc_list;
 };
 
 struct MyersArgs {
@@ -70,9 +75,13 @@ struct PlanSummary {
     uint32_t strip_maxm;
     uint32_t bad;                   // pairs outside the length limits
     unsigned long long strip_maxw;  // largest strip-layout trace of the strip class (words)
+    uint32_t esc_maxm;              // extend_hint band pairs: largest read and
+    uint32_t pad;
+    unsigned long long esc_maxw;    // largest full-matrix strip trace (words)
 };
 struct PlanDev {
     PlanSummary sum;  // first: the host reads back only this part
+    uint32_t esc_range[2];  // {start in order (= npairs), escalated count}
     uint32_t counters[PLAN_NCLASS + 2];
     uint32_t hist[PLAN_NCLASS * PLAN_NBIN];
     uint32_t cursor[PLAN_NCLASS * PLAN_NBIN];
@@ -83,7 +92,8 @@ struct PlanArgs {
     const uint32_t *win_len;
     const int64_t *offset;
     const int64_t *half;
-    int32_t band_ok;  // 1 when mode is band and the scoring fits the band kernel
+    int32_t band_ok;    // 1 when mode is band and the scoring fits the band kernel
+    int32_t hint_mode;  // 1 when mode is extend_hint and the scoring fits the band kernel
     uint32_t max_read;
     uint32_t max_win;
     PlanDev *plan;

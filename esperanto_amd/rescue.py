@@ -15,7 +15,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _native
-from .batch import DeviceArray
+from .batch import ContextTimings, DeviceArray
 from .extend import ExtendBuffer, default_buffer
 from .records import CigarOp, ExtendParams, cigar_capacity, cigar_from_words
 from .reference import DeviceReference
@@ -68,7 +68,7 @@ class MateRequests:
         return off
 
 
-class DeviceMateBatch:
+class DeviceMateBatch(ContextTimings):
     """MateRequests resident in HBM plus the result arena."""
 
     def __init__(self, req: MateRequests, dref: DeviceReference, buf: Optional[ExtendBuffer] = None):
@@ -99,9 +99,6 @@ class DeviceMateBatch:
             self.buf.handle, self.dref.handle, ctypes.byref(self._cb), ctypes.byref(p),
             ctypes.c_void_p(self.rec.ptr), ctypes.c_void_p(self.cigar.ptr),
             ctypes.c_void_p(self.cig_off.ptr), ctypes.c_void_p(self.cig_n.ptr)), self.buf.handle)
-
-    def sync(self) -> None:
-        _native.check(_native.lib().esp_sync(self.buf.handle), self.buf.handle)
 
     def raw_results(self):
         """(records, cigar words, cig_off, cig_n) as numpy arrays."""

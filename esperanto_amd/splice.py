@@ -17,7 +17,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native
-from .batch import DeviceArray
+from .batch import ContextTimings, DeviceArray
 from .extend import ExtendBuffer
 from .records import CigarOp, Extension, ExtendParams, cigar_from_words
 from .reference import DeviceReference
@@ -107,7 +107,7 @@ class SpliceRequests:
         return off
 
 
-class DeviceSpliceBatch:
+class DeviceSpliceBatch(ContextTimings):
     def __init__(self, req: SpliceRequests, dref: DeviceReference, buf: Optional[ExtendBuffer] = None):
         self.buf = buf or dref.buf
         self.req = req
@@ -147,9 +147,6 @@ class DeviceSpliceBatch:
             self.buf.handle, self.dref.handle, ctypes.byref(self._cb), ctypes.byref(p),
             ctypes.byref(self._out), ctypes.c_void_p(self.stitched.ptr),
             ctypes.c_void_p(self.stitch_off.ptr), ctypes.c_void_p(self.rec.ptr)), self.buf.handle)
-
-    def sync(self) -> None:
-        _native.check(_native.lib().esp_sync(self.buf.handle), self.buf.handle)
 
     def raw_results(self):
         """(records, ext[n,4] int64, cigar, cig_off, cig_n, stitched, stitch_off)."""

@@ -164,7 +164,8 @@ int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
  * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
- * 6 selection / stitching), half (band kernel) and ms. */
+ * 6 selection / stitching, 7 extend_hint full-matrix rerun of escalated band
+ * attempts), half (band kernel) and ms. */
 typedef struct {
     int32_t kind;
     int32_t half;

@@ -167,6 +167,51 @@ def test_config1_full_size(buf):
     db.free()
 
 
+def test_extend_hint_band_route_and_escalation(buf):
+    """extend_hint(offset 32) on 10,000 pairs of the 100/164-nt workload (band
+    attempts in the half-30 band kernel), plus pairs built to escalate: a
+    deletion or insertion of 35-60 bases shifts the diagonal past the band,
+    and heavy substitution drops the score under 85% of perfect."""
+    from esperanto_amd.batch import DeviceBatch, PairBatch, workload_c1
+    hb = workload_c1(10_000)
+    rng = np.random.default_rng(17)
+    extra = []
+    for _ in range(600):
+        w = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, 400)].tobytes()
+        kind = int(rng.integers(0, 3))
+        if kind == 0:  # deletion from the read: second half sits further right
+            d = int(rng.integers(35, 61))
+            r = w[60:120] + w[120 + d:180 + d]
+        elif kind == 1:  # insertion into the read
+            d = int(rng.integers(35, 61))
+            r = w[60:120] + bytes(rng.integers(65, 69, d).astype(np.uint8)) + w[120:150]
+        else:  # 20% substitutions
+            r = bytearray(w[60:180])
+            for q in rng.choice(120, 24, replace=False):
+                r[q] = b"ACGT"[int(rng.integers(0, 4))]
+            r = bytes(r)
+        extra.append((r, w, 60, 0))
+    for part, label in ((hb, "hint_c1"), (PairBatch.from_pairs(extra), "hint_escalating")):
+        say(f"{label} {part.npairs} pairs")
+        db = DeviceBatch(part, buf)
+        db.launch_gotoh(ExtendParams(), 2)
+        db.sync()
+        g = db.gotoh_results()
+        o = oracle.band_batch(part, ExtendParams(), 2, nthreads=8)
+        assert np.array_equal(g[0], o[0]), label
+        assert np.array_equal(g[4], o[4]), label
+        assert np.array_equal(g[5], o[5]), label
+        off = g[3]
+        for k in range(part.npairs):
+            a = int(off[k])
+            assert np.array_equal(g[2][a:a + g[4][k]], o[2][a:a + o[4][k]]), (label, k)
+        if label == "hint_escalating":
+            assert int((g[5] == 1).sum()) > 400
+        kinds = {t[0] for t in db.last_timings()}
+        assert 1 in kinds  # the band kernel ran the first attempts
+        db.free()
+
+
 def test_config3_sample(buf):
     """20,000 pairs of the uneven-length workload (bands 16..64)."""
     from esperanto_amd.batch import DeviceBatch, workload_c3
