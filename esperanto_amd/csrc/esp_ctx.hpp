@@ -58,6 +58,7 @@ struct HostBuf {
 
 inline constexpr size_t kTraceBudget = (size_t)8 << 30;  // bytes of trace scratch per launch
 inline constexpr size_t kRowBudget = (size_t)2 << 30;
+inline constexpr size_t kFullTraceBudget = (size_t)16 << 30;  // full-matrix kernel trace
 
 }  // namespace esph
 
@@ -67,7 +68,8 @@ struct esp_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
-    esph::DevBuf counter, rows, trace, btrace, stage, stage_out, plan, order;
+    esph::DevBuf counter, rows, trace, btrace, ftrace, stage, stage_out, plan, order;
+    int full_blocks[512] = {};  // occupancy of the full-matrix kernel per row tile
     // window pipelines: query arena, window / pseudo-reference arena,
     // per-pair descriptors and Myers results
     esph::DevBuf wquery, warena, wdesc, wmyers;
@@ -157,6 +159,21 @@ inline void tim_mark(esp_ctx *c, int kind, int half) {
 inline bool band_lut_fits(const esp_params *p) {
     return p->match_score >= -16 && p->match_score <= 15 && p->mismatch >= -16 &&
            p->mismatch <= 15 && p->gap_open + p->gap_ext <= 8000;
+}
+
+// Columns of the full-matrix kernel: for window code f = 0..5 the 5-bit
+// score of read code rc = 0..4 at bit 5*rc, and -16 at bit 25 for the rows
+// past the read (the transpose of band_lut).
+inline void full_lut(const esp_params *p, uint32_t out[6]) {
+    for (int f = 0; f < 6; f++) {
+        uint32_t w = (uint32_t)(-16 & 31) << 25;
+        for (int rc = 0; rc < 5; rc++) {
+            const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;
+            const int s = (rc < 4 && f == rc) ? p->match_score : (f == partner ? 0 : p->mismatch);
+            w |= ((uint32_t)s & 31u) << (5 * rc);
+        }
+        out[f] = w;
+    }
 }
 
 inline void band_lut(const esp_params *p, uint32_t out[5]) {

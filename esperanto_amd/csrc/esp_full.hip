@@ -1,0 +1,335 @@
+// esp_full.hip -- full-matrix Gotoh kernel for extend_hint's second attempt.
+//
+// extend_hint (extend.rs:392-428) reruns a band attempt on the full matrix
+// [1, m] x [1, n] when the band's maximum touched its edge or scored under
+// 85% of perfect; with the reference's own edge rule this is frequent (a
+// first-row maximum tied on the band edge already counts).  These windows are
+// wide (600-2000 columns for spliced and mate-rescue windows) and reads short
+// (at most 259 bases: the band attempt admitted them), so the matrix is swept
+// column by column with the whole read held in registers:
+//   * lane per pair; row i's state (U = H(i, j-1) + open_ext, Fx = F(i, j-1)
+//     + ext) lives in one VGPR as two int16 halves (read lengths and the
+//     band-kernel scoring envelope keep every value inside int16);
+//   * a column is MAXM unrolled rows top-down, the vertical carries (H + oe,
+//     E + ext of the cell above) in registers, so a cell is pure VALU work;
+//   * window codes stream through an LDS chunk per lane, read codes sit in
+//     LDS per row, and the substitution score is one v_bfe_i32 of a per-
+//     column LUT word indexed by the read code;
+//   * trace: the band kernel's 4 bits per cell, column-major, 8 rows a word.
+// Rows past m compute with a read code that scores -16 against everything,
+// so no such cell can beat or tie a real row's maximum at an earlier row.
+// The best cell is the first row-major strict maximum (extend.rs:751-772):
+// a column key (H << 16 | 0xFFFF - i) taken strictly across columns.
+#include "esp_kernels.hpp"
+
+using namespace espd;
+
+namespace espk {
+
+namespace {
+
+constexpr int FULL_CHUNK = 64;  // window columns staged per LDS chunk
+constexpr int32_t NEG16 = -20000;
+
+// Writes win_code(src[q]) to chunk[q * 64] for q in [0, len): head and
+// tail bytewise, the aligned body as dwords, eight loads in flight.
+__device__ void stage_win(const uint8_t *src, int32_t len, uint8_t *chunk) {
+    const int32_t head = min(len, (int32_t)((4u - ((uint32_t)(uintptr_t)src & 3u)) & 3u));
+    for (int32_t q = 0; q < head; q++) chunk[q * 64] = (uint8_t)win_code(src[q]);
+    const uint32_t *d = (const uint32_t *)(src + head);
+    const int32_t nd = (len - head) >> 2;
+    for (int32_t q = 0; q < nd; q += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) w[u] = q + u < nd ? d[q + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (q + u < nd) {
+                const int32_t r = head + 4 * (q + u);
+#pragma unroll
+                for (int b = 0; b < 4; b++) chunk[(r + b) * 64] = (uint8_t)win_code((w[u] >> (8 * b)) & 0xFFu);
+            }
+        }
+    }
+    for (int32_t q = head + 4 * nd; q < len; q++) chunk[q * 64] = (uint8_t)win_code(src[q]);
+}
+
+// Traceback over the column-major trace (extend.rs:884-930) with value
+// tracking; runs are written in walk order to slot[cap-1], slot[cap-2], ...
+// A diagonal cache holds the nibbles and window codes of cells (i-q, j-q),
+// q < 32: one batch of independent loads serves up to 32 diagonal steps.
+template <int WR>
+__device__ TbResult traceback_full(const PairView &pv, const GParams &P, const uint8_t *rcode,
+                                   const uint32_t *tr, int32_t best, int32_t bi, int32_t bj,
+                                   uint32_t *slot, uint32_t cap) {
+    int32_t i = bi, j = bj, V = best;
+    uint32_t state = 0;
+    uint32_t cur_op = 0xFF, cur_len = 0, nrev = 0, nm = 0, ni = 0, nd = 0;
+    auto emit = [&](uint32_t op) {
+        if (op == cur_op) {
+            cur_len++;
+        } else {
+            if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
+            cur_op = op;
+            cur_len = 1;
+        }
+    };
+    int32_t ci0 = -1, cdg = 0x7FFFFFFF;
+    uint64_t nlo = 0, nhi = 0, flo = 0, fhi = 0;
+    while (i > 0 && j > 0) {
+        if (j - i != cdg || i > ci0 || i <= ci0 - 32) {
+            uint32_t w[32], f[32];
+#pragma unroll
+            for (int q = 0; q < 32; q++) {
+                const bool ok = i - q >= 1 && j - q >= 1;
+                w[q] = ok ? tr[((size_t)(j - 1 - q) * WR + ((i - 1 - q) >> 3)) * 64] : 0u;
+                f[q] = ok ? pv.win[j - 1 - q] : 0u;
+            }
+            nlo = nhi = flo = fhi = 0;
+#pragma unroll
+            for (int q = 0; q < 32; q++) {
+                const uint32_t r = (uint32_t)(i - 1 - q) & 7u;
+                const uint64_t nib = (w[q] >> (4 * (7 - r))) & 0xFu;
+                const uint64_t fc = (uint64_t)win_code(f[q] & 0xFFu);
+                if (q < 16) {
+                    nlo |= nib << (4 * q);
+                    flo |= fc << (4 * q);
+                } else {
+                    nhi |= nib << (4 * (q - 16));
+                    fhi |= fc << (4 * (q - 16));
+                }
+            }
+            ci0 = i;
+            cdg = j - i;
+        }
+        const int32_t d = ci0 - i;
+        const uint32_t nib = (uint32_t)((d < 16 ? (nlo >> (4 * d)) : (nhi >> (4 * (d - 16)))) & 0xFu);
+        if (state == 0) {
+            if (V == 0) break;  // H == 0: the reference's H_START
+            if (!(nib & 8u)) {
+                const int32_t rc = rcode[(size_t)i * 64] / 5;
+                const uint32_t fc5 = (uint32_t)((d < 16 ? (flo >> (4 * d)) : (fhi >> (4 * (d - 16)))) & 0xFu);
+                V -= lut_score(P, rc, (int32_t)fc5);
+                emit(OP_M);
+                nm++;
+                i--;
+                j--;
+            } else if (!(nib & 4u)) {
+                emit(OP_I);
+                ni++;
+                if (nib & 2u) {
+                    state = 1;
+                    V -= P.ext;
+                } else {
+                    V -= P.oe;
+                }
+                i--;
+            } else {
+                emit(OP_D);
+                nd++;
+                if (nib & 1u) {
+                    state = 2;
+                    V -= P.ext;
+                } else {
+                    V -= P.oe;
+                }
+                j--;
+            }
+        } else if (state == 1) {
+            emit(OP_I);
+            ni++;
+            if (nib & 2u) {
+                V -= P.ext;
+            } else {
+                state = 0;
+                V -= P.oe;
+            }
+            i--;
+        } else {
+            emit(OP_D);
+            nd++;
+            if (nib & 1u) {
+                V -= P.ext;
+            } else {
+                state = 0;
+                V -= P.oe;
+            }
+            j--;
+        }
+    }
+    if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
+    TbResult r;
+    r.read_start = (uint32_t)max(i, 0);
+    r.ref_start = (uint32_t)max(j, 0);
+    r.nm = nm;
+    r.ni = ni;
+    r.nd = nd;
+    r.nrev = nrev;
+    return r;
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
+    static_assert(MAXM % 8 == 0, "rows come in groups of 8");
+    constexpr int WR = MAXM / 8;  // trace words per column
+    extern __shared__ uint8_t lds[];
+    // LDS: column LUTs (8 words), read codes x5 of rows 0..MAXM, window chunk
+    uint32_t *flut = (uint32_t *)lds;
+    const uint32_t lane = threadIdx.x;
+    uint8_t *rcode = lds + 32 + lane;                      // row i at rcode[i * 64]
+    uint8_t *chunk = lds + 32 + (MAXM + 1) * 64 + lane;    // column q at chunk[q * 64]
+    const GParams P = a.P;
+    const int32_t oe = P.oe, ext = P.ext, noe = -oe;
+    const uint32_t first = a.range[0], total = a.range[1];
+    const uint32_t nunits = (total + 63) / 64;
+    if (lane < 8) flut[lane] = a.flut[lane < 6 ? lane : 5];
+    for (int q = 0; q < FULL_CHUNK; q++) chunk[q * 64] = 0;
+    __syncthreads();
+    uint32_t *tr = a.trace + (size_t)blockIdx.x * a.maxw * 64 + lane;
+    for (;;) {
+        const uint32_t u = wave_pop(a.counter, lane, 1u);
+        if (u >= nunits) break;
+        const uint32_t ubegin = first + 64 * u;
+        const uint32_t ucount = min(64u, total - 64 * u);
+        const bool active = lane < ucount;
+        const uint32_t p = a.order[ubegin + (active ? lane : 0u)];
+        PairView pv;
+        pv.read = a.reads + a.read_off[p];
+        pv.win = a.wins + a.win_off[p];
+        pv.m = active ? (int32_t)a.read_len[p] : 0;
+        pv.n = active ? (int32_t)a.win_len[p] : 0;
+        pv.off = 0;
+        pv.half = pv.n;
+        const int32_t m = pv.m, n = pv.n;
+        // wave maxima (uniform loop bounds)
+        int32_t umaxm = m, umaxn = n;
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+            umaxm = max(umaxm, __shfl_xor(umaxm, sh));
+            umaxn = max(umaxn, __shfl_xor(umaxn, sh));
+        }
+        const int32_t ngroups = (umaxm + 7) >> 3;
+        // Read codes x5 into LDS rows 1..m; rows past m get code 5 (-16 vs all).
+        for (int32_t i = 1; i <= 8 * ngroups; i++)
+            rcode[(size_t)i * 64] = (uint8_t)(5 * (i <= m ? read_code(pv.read[i - 1]) : 5u));
+        uint32_t Pk[MAXM];
+        const uint32_t pinit = ((uint32_t)oe & 0xFFFFu) | ((uint32_t)NEG16 << 16);  // H = 0, F = NEG
+#pragma unroll
+        for (int r = 0; r < MAXM; r++) Pk[r] = pinit;
+        int32_t bestkey = 0xFFFF, bj = 0;
+        for (int32_t j0 = 0; j0 < umaxn; j0 += FULL_CHUNK) {
+            const int32_t len = max(0, min(FULL_CHUNK, n - j0));
+            stage_win(pv.win + j0, len, chunk);
+            const int32_t qend = min(FULL_CHUNK, umaxn - j0);
+            for (int32_t q = 0; q < qend; q++) {
+                const int32_t j = j0 + q + 1;
+                const uint32_t clut = flut[chunk[q * 64]];
+                // row 0: H(0, j) = 0, E(0, j) = NEG; the diagonal of row 1 is H(0, j-1) = 0
+                int32_t Uc = oe, Bc = NEG + ext, Ud = oe, ckey = 0;
+                uint32_t *tcol = tr + (size_t)(j - 1) * WR * 64;
+#pragma unroll
+                for (int g = 0; g < WR; g++) {
+                    if (g < ngroups) {
+                        uint32_t acc = 0;
+#pragma unroll
+                        for (int r = 0; r < 8; r++) {
+                            const int i = 8 * g + r + 1;
+                            const uint32_t pk = Pk[i - 1];
+                            const int32_t Ul = (int32_t)(int16_t)(pk & 0xFFFFu);  // H(i, j-1) + oe
+                            const int32_t Fx = ((int32_t)pk) >> 16;                 // F(i, j-1) + ext
+                            const uint32_t rc5 = rcode[(size_t)i * 64];
+                            const int32_t sc = __builtin_amdgcn_sbfe((int32_t)clut, rc5, 5);
+                            int32_t D;
+                            asm("v_add3_u32 %0, %1, %2, %3" : "=v"(D) : "v"(Ud), "v"(sc), "s"(noe));
+                            const int32_t E = max(Uc, Bc);
+                            const int32_t F = max(Ul, Fx);
+                            const int32_t H = max(max(max(D, E), F), 0);
+                            const uint32_t x0 = (uint32_t)(D - H), x1 = (uint32_t)(E - H);
+                            const uint32_t x2 = (uint32_t)(Uc - Bc), x3 = (uint32_t)(Ul - Fx);
+                            asm("v_alignbit_b32 %0, %0, %1, 31\n\t"
+                                "v_alignbit_b32 %0, %0, %2, 31\n\t"
+                                "v_alignbit_b32 %0, %0, %3, 31\n\t"
+                                "v_alignbit_b32 %0, %0, %4, 31"
+                                : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3));
+                            const int32_t Un = H + oe;
+                            Ud = Ul;  // H(i, j-1) + oe is the next row's diagonal
+                            Uc = Un;
+                            Bc = E + ext;
+                            Pk[i - 1] = __builtin_amdgcn_perm((uint32_t)(F + ext), (uint32_t)Un, 0x05040100u);
+                            int32_t kk;
+                            asm("v_lshl_or_b32 %0, %1, 16, %2" : "=v"(kk) : "v"(H), "s"(0xFFFF - i));
+                            asm("v_max_i32 %0, %1, %2" : "=v"(ckey) : "v"(kk), "v"(ckey));
+                        }
+                        tcol[g * 64] = acc;
+                    }
+                }
+                if (j <= n && ckey > bestkey) {
+                    bestkey = ckey;
+                    bj = j;
+                }
+            }
+        }
+        if (active) {
+            int32_t best = bestkey >> 16, bi = 0xFFFF - (bestkey & 0xFFFF);
+            if (best <= 0) {
+                best = 0;
+                bi = 0;
+                bj = 0;
+            }
+            const uint64_t c0 = a.cig_off[p];
+            const uint32_t cap = (uint32_t)(a.cig_off[p + 1] - c0);
+            uint32_t *cslot = a.cigar + c0;
+            TbResult tb = traceback_full<WR>(pv, P, rcode, tr, best, bi, bj, cslot, cap);
+            esp_ext e;
+            uint32_t nops;
+            finish_pair(pv, P, best, tb, cslot, cap, &e, &nops);
+            a.ext[p] = e;
+            a.cig_n[p] = nops;
+        }
+    }
+}
+
+template <int MAXM>
+size_t full_lds() { return 32 + (size_t)(MAXM + 1) * 64 + (size_t)FULL_CHUNK * 64; }
+
+template <int MAXM>
+hipError_t launch_full_t(const GotohArgs &a, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_gotoh_full<MAXM>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
+    return hipGetLastError();
+}
+
+template <int MAXM>
+int full_blocks_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_full<MAXM>, 64, full_lds<MAXM>()) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace
+
+int full_rows(uint32_t maxm) {
+    return maxm <= 64 ? 64 : maxm <= 128 ? 128 : maxm <= 160 ? 160 : maxm <= 192 ? 192 : 264;
+}
+
+hipError_t launch_gotoh_full(const GotohArgs &a, int rows, int blocks, hipStream_t s) {
+    switch (rows) {
+    case 64: return launch_full_t<64>(a, blocks, s);
+    case 128: return launch_full_t<128>(a, blocks, s);
+    case 160: return launch_full_t<160>(a, blocks, s);
+    case 192: return launch_full_t<192>(a, blocks, s);
+    default: return launch_full_t<264>(a, blocks, s);
+    }
+}
+
+int gotoh_full_blocks_per_cu(int rows) {
+    switch (rows) {
+    case 64: return full_blocks_t<64>();
+    case 128: return full_blocks_t<128>();
+    case 160: return full_blocks_t<160>();
+    case 192: return full_blocks_t<192>();
+    default: return full_blocks_t<264>();
+    }
+}
+
+}  // namespace espk

@@ -159,6 +159,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     a.mode = mode;
     a.P = dev_params(p);
     band_lut(p, a.lut);
+    full_lut(p, a.flut);
     a.ext = o->ext;
     a.touch = o->touch;
     a.cigar = o->cigar;
@@ -237,11 +238,21 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         espk::GotohArgs f = a;
         f.mode = ESP_MODE_FULL;
         f.aux = nullptr;
+        f.touch = nullptr;  // extend_hint returns no TouchRec
         f.range = dp->esc_range;
         f.counter = &dp->counters[espk::PLAN_NCLASS];
-        rc = strip_launch(c, f, hp->range[1][1], hp->esc_maxm, hp->esc_maxw);
-        if (rc) return rc;
-        tim_mark(c, 7, 0);
+        const int rows = espk::full_rows(hp->esc_maxm);
+        const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
+        if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
+        uint64_t blocks = (uint64_t)c->ncu * c->full_blocks[rows & 511];
+        blocks = std::min<uint64_t>(blocks, ((uint64_t)hp->range[1][1] + 63) / 64);
+        blocks = std::min<uint64_t>(blocks, kFullTraceBudget / (maxw * 64 * 4));
+        blocks = std::max<uint64_t>(blocks, 1);
+        HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * blocks));
+        f.trace = (uint32_t *)c->ftrace.p;
+        f.maxw = (uint32_t)maxw;
+        HIP_TRY(c, espk::launch_gotoh_full(f, rows, (int)blocks, c->stream));
+        tim_mark(c, 7, rows);
     }
     return ESP_OK;
 }
@@ -295,6 +306,7 @@ int esp_destroy(esp_ctx *c) {
     c->rows.release();
     c->trace.release();
     c->btrace.release();
+    c->ftrace.release();
     if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
     c->stage.release();
     c->stage_out.release();

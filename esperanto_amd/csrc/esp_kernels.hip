@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
     __shared__ uint32_t h[PLAN_NCLASS * PLAN_NBIN];
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x) h[t] = 0;
     __syncthreads();
-    uint32_t smax = 0, emax = 0;
+    uint32_t smax = 0, emax = 0, enmax = 0;
     uint64_t wmax = 0, ewmax = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     // The loop bound is uniform per wave, so agg_inc sees whole waves.
@@ -399,6 +399,7 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
             } else if (a.hint_mode) {
                 // a band attempt that escalates reruns on the full matrix
                 emax = max(emax, m);
+                enmax = max(enmax, a.win_len[p]);
                 ewmax = max(ewmax, strip_trace_words(m, a.win_len[p]));
             }
         }
@@ -411,6 +412,7 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
     atomicMax(&a.plan->sum.strip_maxw, (unsigned long long)wmax);
     if (a.hint_mode) {
         atomicMax(&a.plan->sum.esc_maxm, emax);
+        atomicMax(&a.plan->sum.esc_maxn, enmax);
         atomicMax(&a.plan->sum.esc_maxw, (unsigned long long)ewmax);
     }
 }

@@ -41,6 +41,10 @@ struct GotohArgs {
     // Band kernel substitution rows: for read code 0..4, 5-bit signed scores
     // of window codes 0..5 packed at bit 5*code.
     uint32_t lut[5];
+    // Full-matrix kernel substitution columns: for window code 0..5, 5-bit
+    // signed scores of read codes 0..4 at bit 5*code, and -16 at bit 25 (the
+    // code of rows past the read).
+    uint32_t flut[6];
     // extend_hint escalation (band kernel in ESP_MODE_HINT): pairs whose band
     // attempt must be redone on the full matrix are appended to esc_list
     // through the counter esc_range[1].
@@ -75,8 +79,8 @@ struct PlanSummary {
     uint32_t strip_maxm;
     uint32_t bad;                   // pairs outside the length limits
     unsigned long long strip_maxw;  // largest strip-layout trace of the strip class (words)
-    uint32_t esc_maxm;              // extend_hint band pairs: largest read and
-    uint32_t pad;
+    uint32_t esc_maxm;              // extend_hint band pairs: largest read,
+    uint32_t esc_maxn;              // largest window and
     unsigned long long esc_maxw;    // largest full-matrix strip trace (words)
 };
 struct PlanDev {
@@ -129,6 +133,11 @@ bool band_half_supported(int half);
 hipError_t launch_gotoh_band(const GotohArgs &a, int half, int blocks, hipStream_t s);
 int gotoh_band_blocks_per_cu(int half);
 __host__ __device__ inline uint64_t band_trace_words(uint64_t m, uint64_t half) { return m * ((2 * half + 1 + 7) / 8); }
+// Full-matrix kernel for extend_hint's second attempt (reads up to 264
+// bases): rows = full_rows(max read length) picks the register tile.
+int full_rows(uint32_t maxm);
+hipError_t launch_gotoh_full(const GotohArgs &a, int rows, int blocks, hipStream_t s);
+int gotoh_full_blocks_per_cu(int rows);
 hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
