@@ -110,6 +110,15 @@ def bench_window(args, buf, params):
         out[name] = {"pairs_per_s": round(n / w), "ms_per_step": round(w * 1e3, 3),
                      "device_ms": round(k, 3), "stages_ms": stages}
         log(f"{name}: {n / w / 1e6:.2f} M pairs/s, {w * 1e3:.2f} ms/step, stages {stages}")
+        ps = db.last_plan_stats()
+        out[name]["escalated_to_full"] = ps["escalated"]
+        out[name]["band_attempts"] = ps["band_pairs"][30]
+        esc_ms = stages.get("escalated_full", 0.0)
+        if esc_ms and ps["escalated"]:
+            wlen = 600 if name == "spliced" else 1950
+            cells = ps["escalated"] * 150 * wlen
+            out[name]["full_kernel_gcups_est"] = round(cells / (esc_ms / 1e3) / 1e9, 1)
+        log(f"{name}: routing {ps}")
         if name == "mate_rescue":
             rec = db.raw_results()[0]
             out[name]["placed_fraction"] = round(float((rec["placed"] == 1).mean()), 4)
@@ -132,6 +141,124 @@ def bench_window(args, buf, params):
         log(f"cpu port: {out['cpu_port']}")
     dref.close()
     return out
+
+
+def bench_serving(args, buf, params):
+    """Serving loop: a pool of mixed requests (log-uniform 1-4,096 pairs of 50-250-nt
+    reads; 10% mate-rescue requests with ~2 kb windows) over a 256 Mb reference
+    resident in HBM.  Closed loop first (the whole pool at once: the batched peak),
+    then open-loop Poisson arrivals at 25/50/90% of that peak for a fixed time,
+    latency = completion minus scheduled arrival."""
+    import queue
+    import threading
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.server import Server, now_ns
+    from esperanto_amd.synth_window import serving_workload
+    log(f"serving workload: {args.serve_requests} requests on a {args.serve_ref_mb} Mb reference")
+    ref, pool = serving_workload(args.serve_requests, ref_len=args.serve_ref_mb * 1_000_000)
+    dref = DeviceReference(ref, buf)
+    srv = Server(dref, params, mem_budget=32 << 30, workers=args.serve_workers,
+                 max_batch_pairs=args.serve_batch_pairs)
+
+    def submit(r):
+        return srv.submit_arrays(r["kind"], r["reads"], r["read_len"], r["contig"],
+                                 r.get("win_start"), r.get("win_end"), r.get("hint"),
+                                 r.get("mate_pos"))
+
+    sizes = np.array([r["read_len"].shape[0] for r in pool])
+    # warm-up: one untimed pass of the whole pool grows every worker's scratch
+    # and pinned staging to the batch sizes the timed passes use
+    for t in [submit(r) for r in pool]:
+        t.wait()
+    t0 = now_ns()
+    tickets = [submit(r) for r in pool]
+    done = max(t.wait().done_ns for t in tickets)
+    peak = float(sizes.sum()) / ((done - t0) / 1e9)
+    log(f"serving closed loop: {len(pool)} requests, {sizes.sum()} pairs, {peak / 1e6:.2f} M pairs/s")
+    # bit-exactness of a sample against the oracle
+    verified = 0
+    if not args.no_cpu:
+        from oracle import oracle
+        from esperanto_amd.batch import PairBatch
+        from esperanto_amd.rescue import MateRequests
+        o = oracle.OracleRef(ref)
+        rng = np.random.default_rng(1)
+        for j in rng.choice(len(pool), min(24, len(pool)), replace=False):
+            r, t = pool[j], tickets[j]
+            n = r["read_len"].shape[0]
+            off = np.zeros(n + 1, dtype=np.int64)
+            off[1:] = np.cumsum(r["read_len"])
+            rec, cig, coff, cn = t.raw()
+            if r["kind"] == 0:
+                reads = [r["reads"][off[k]:off[k + 1]].tobytes() for k in range(n)]
+                wins = [oracle.decode(o, 0, int(r["win_start"][k]), int(r["win_end"][k])) for k in range(n)]
+                pb = PairBatch.from_pairs([(reads[k], wins[k], 32, 0) for k in range(n)])
+                e, _, ec, eo, en, _ = oracle.band_batch(pb, params, 2, nthreads=16)
+                got = rec.astype(np.int64)
+                got[:, 3] = rec[:, 3].view(np.int32)
+                ok = np.array_equal(got, e) and np.array_equal(cn, en)
+            else:
+                mq = MateRequests(r["reads"], off[:-1].astype(np.uint64), r["read_len"], r["contig"],
+                                  r["mate_pos"])
+                e, ec, eo, en = oracle.mate_batch(o, mq, params, nthreads=16)
+                ok = [x.tolist() for x in rec] == [x.tolist() for x in e] and np.array_equal(cn, en)
+            ok = ok and all(np.array_equal(cig[int(coff[k]):int(coff[k]) + int(cn[k])],
+                                           ec[int(coff[k]):int(coff[k]) + int(cn[k])]) for k in range(n))
+            if not ok:
+                raise RuntimeError(f"serving result of request {j} differs from the CPU path")
+            verified += 1
+    del tickets
+    loads = []
+    rng = np.random.default_rng(5)
+    for frac in (0.25, 0.5, 0.9):
+        rate = frac * peak / float(sizes.mean())  # requests per second
+        nreq = max(1, int(rate * args.serve_seconds))
+        gaps = rng.exponential(1.0 / rate, nreq)
+        order = rng.integers(0, len(pool), nreq)
+        q = queue.Queue()
+        lat = []
+
+        def completer():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                t_sched, t = item
+                lat.append((t.wait().done_ns - t_sched) / 1e6)
+
+        th = threading.Thread(target=completer)
+        th.start()
+        start = now_ns() + 20_000_000
+        sched = start + np.cumsum(gaps * 1e9).astype(np.int64)
+        pairs = 0
+        for k in range(nreq):
+            while True:
+                d = int(sched[k]) - now_ns()
+                if d <= 0:
+                    break
+                if d > 300_000:
+                    time.sleep((d - 200_000) / 1e9)
+            r = pool[order[k]]
+            q.put((int(sched[k]), submit(r)))
+            pairs += r["read_len"].shape[0]
+        q.put(None)
+        th.join()
+        span = (max(start, now_ns()) - start) / 1e9
+        lat_a = np.array(lat)
+        loads.append({"offered_fraction": frac, "offered_pairs_per_s": round(frac * peak),
+                      "achieved_pairs_per_s": round(pairs / span), "requests": nreq,
+                      "p50_ms": round(float(np.percentile(lat_a, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat_a, 99)), 3),
+                      "max_ms": round(float(lat_a.max()), 3)})
+        log(f"serving load {frac:.2f}: {loads[-1]}")
+    st = srv.stats()
+    srv.close()
+    dref.close()
+    return {"requests_in_pool": len(pool), "pairs_in_pool": int(sizes.sum()),
+            "workers": args.serve_workers, "mem_budget_bytes": 32 << 30,
+            "closed_loop_pairs_per_s": round(peak), "loads": loads,
+            "verified_requests_bit_exact": verified, "max_batch_requests": st["max_batch_requests"],
+            "batches": st["batches"], "peak_inflight_bytes": st["peak_inflight_bytes"]}
 
 
 def _subset_splice(sp, k):
@@ -165,6 +292,13 @@ def main() -> int:
     ap.add_argument("--window-cpu-pairs", type=int, default=20_000)
     ap.add_argument("--only-window", action="store_true",
                     help="run only the splice + mate-rescue leg")
+    ap.add_argument("--serve-requests", type=int, default=1500,
+                    help="requests in the serving pool (0 skips the serving leg)")
+    ap.add_argument("--serve-ref-mb", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-batch-pairs", type=int, default=131072)
+    ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -184,6 +318,10 @@ def main() -> int:
     buf = ExtendBuffer(0)
     if args.only_window:
         print(json.dumps(bench_window(args, buf, params)), flush=True)
+        buf.close()
+        return 0
+    if args.only_serve:
+        print(json.dumps(bench_serving(args, buf, params)), flush=True)
         buf.close()
         return 0
 
@@ -257,6 +395,9 @@ def main() -> int:
     win = None
     if args.window_pairs:
         win = bench_window(args, buf, params)
+    serve = None
+    if args.serve_requests:
+        serve = bench_serving(args, buf, params)
 
     cpu = None
     if not args.no_cpu:
@@ -306,7 +447,8 @@ def main() -> int:
                    "myers_pairs_per_s": round(myers_pps),
                    "myers_kernel_ms": round(k2, 3),
                    "myers_valu_tops": round(myers_tops, 2),
-                   "splice_rescue": win},
+                   "splice_rescue": win,
+                   "serving": serve},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
                      "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),

@@ -13,6 +13,7 @@ from .records import CExt, CParams, CTouch
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libesp_align.so")
 
 ESP_OK, ESP_E_INVAL, ESP_E_CAP, ESP_E_DEVICE, ESP_E_NOMEM, ESP_E_BUSY = range(6)
+ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET = 1, 2
 _STATUS = {1: "invalid argument", 2: "capacity", 3: "device error", 4: "out of memory",
            5: "busy"}
 
@@ -23,6 +24,8 @@ EXPORTS = [
     "esp_device_free", "esp_copy_to_device", "esp_copy_to_host", "esp_stitch_cigar",
     "esp_last_timings", "esp_device_copy", "esp_ref_upload", "esp_ref_free",
     "esp_ref_device_bytes", "esp_gather_batch", "esp_mate_rescue_batch", "esp_spliced_batch",
+    "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
+    "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
 ]
 
 
@@ -52,6 +55,11 @@ class CTiming(ctypes.Structure):
 class CHostMeta(ctypes.Structure):
     _fields_ = [("read_len", ctypes.c_void_p), ("win_len", ctypes.c_void_p),
                 ("offset", ctypes.c_void_p), ("half", ctypes.c_void_p)]
+
+
+class CPlanStats(ctypes.Structure):
+    _fields_ = [("band_pairs", ctypes.c_uint32 * 5), ("strip_pairs", ctypes.c_uint32),
+                ("escalated", ctypes.c_uint32)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -108,6 +116,8 @@ def lib():
     L.esp_copy_to_device.argtypes = [vp, vp, vp, sz]
     L.esp_copy_to_host.argtypes = [vp, vp, vp, sz]
     L.esp_stitch_cigar.argtypes = [u32p, sz, ctypes.c_uint32, u32p, sz, u32p, sz, u32p, sz, P(sz)]
+    L.esp_last_plan_stats.argtypes = [vp, P(CPlanStats)]
+    L.esp_ctx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
     L.esp_ref_upload.argtypes = [vp, sz, vp, vp, vp, vp, vp, P(vp)]
     L.esp_ref_free.argtypes = [vp, vp]
     L.esp_ref_device_bytes.argtypes = [vp, P(sz)]
@@ -122,3 +132,51 @@ def check(rc: int, ctx=None) -> None:
     if rc != ESP_OK:
         msg = lib().esp_last_error(ctx).decode() if ctx else ""
         raise EspError(rc, msg)
+
+
+class CServerConfig(ctypes.Structure):
+    _fields_ = [("params", CParams), ("mem_budget", ctypes.c_uint64), ("workers", ctypes.c_uint32),
+                ("max_batch_pairs", ctypes.c_uint32), ("max_pending", ctypes.c_uint32),
+                ("max_read_len", ctypes.c_uint32)]
+
+
+class CRequest(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("npairs", ctypes.c_uint32), ("reads", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("contig", ctypes.c_void_p),
+                ("win_start", ctypes.c_void_p), ("win_end", ctypes.c_void_p),
+                ("hint", ctypes.c_void_p), ("mate_pos", ctypes.c_void_p)]
+
+
+class CRequestOut(ctypes.Structure):
+    _fields_ = [("ext", ctypes.c_void_p), ("mate", ctypes.c_void_p), ("cigar", ctypes.c_void_p),
+                ("cig_n", ctypes.c_void_p)]
+
+
+class CRequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("done", ctypes.c_int32), ("submit_ns", ctypes.c_uint64),
+                ("done_ns", ctypes.c_uint64), ("batch", ctypes.c_uint64)]
+
+
+class CServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
+                ("inflight_bytes", ctypes.c_uint64), ("peak_inflight_bytes", ctypes.c_uint64),
+                ("pending", ctypes.c_uint32), ("max_batch_requests", ctypes.c_uint32),
+                ("stage_ns", ctypes.c_uint64), ("extend_ns", ctypes.c_uint64),
+                ("rescue_ns", ctypes.c_uint64), ("scatter_ns", ctypes.c_uint64)]
+
+
+def server_lib():
+    """The library with the serving-loop signatures bound."""
+    L = lib()
+    if not getattr(L, "_server_bound", False):
+        vp, P = ctypes.c_void_p, ctypes.POINTER
+        L.esp_server_create.argtypes = [ctypes.c_int, vp, P(CServerConfig), P(vp)]
+        L.esp_server_destroy.argtypes = [vp]
+        L.esp_server_submit.argtypes = [vp, P(CRequest), P(CRequestOut), P(ctypes.c_uint64)]
+        L.esp_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, P(CRequestInfo)]
+        L.esp_server_stats_get.argtypes = [vp, P(CServerStats)]
+        L.esp_now_ns.argtypes = []
+        L.esp_now_ns.restype = ctypes.c_uint64
+        L._server_bound = True
+    return L

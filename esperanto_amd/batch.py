@@ -202,9 +202,18 @@ class ContextTimings:
                       self.buf.handle)
         return ms.value
 
+    def last_plan_stats(self) -> dict:
+        """Routing of the last batch: pairs per band half, strip pairs, escalations."""
+        st = _native.CPlanStats()
+        _native.check(_native.lib().esp_last_plan_stats(self.buf.handle, ctypes.byref(st)),
+                      self.buf.handle)
+        return {"band_pairs": dict(zip((16, 30, 32, 48, 64), list(st.band_pairs))),
+                "strip_pairs": st.strip_pairs, "escalated": st.escalated}
+
     def last_timings(self):
         """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,
-        3 Myers, 5 window preparation, 6 selection / stitching."""
+        3 Myers, 5 window preparation, 6 selection / stitching, 7 full-matrix
+        rerun of escalated extend_hint attempts."""
         arr = (_native.CTiming * 16)()
         n = ctypes.c_size_t()
         _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),

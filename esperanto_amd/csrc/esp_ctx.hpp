@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/esp_align.h"
 #include "esp_kernels.hpp"
@@ -70,6 +71,11 @@ struct esp_ctx {
     std::string err;
     esph::DevBuf counter, rows, trace, btrace, ftrace, stage, stage_out, plan, order;
     int full_blocks[512] = {};  // occupancy of the full-matrix kernel per row tile
+    // Trace scratch caps per launch (grids shrink to fit); a server lowers them
+    // so that several contexts share one device-memory budget.
+    size_t trace_budget = esph::kTraceBudget;
+    size_t full_trace_budget = esph::kFullTraceBudget;
+    int full_kernel = 0;  // extend_hint reruns: 0 by batch size, 1 lane per pair
     // window pipelines: query arena, window / pseudo-reference arena,
     // per-pair descriptors and Myers results
     esph::DevBuf wquery, warena, wdesc, wmyers;
@@ -81,6 +87,26 @@ struct esp_ctx {
     hipEvent_t evs[kMaxEv + 1] = {};
     esp_timing tim[kMaxEv] = {};
     int nev = 0;
+};
+
+// The resident reference (esp_ref_upload): device tables plus a host copy of
+// the contig lengths for argument checks.
+struct esp_ref {
+    int device = 0;
+    uint32_t ncontig = 0;
+    std::vector<uint32_t> len;  // host copy for argument checks
+    void *packed = nullptr, *pk_off = nullptr, *dlen = nullptr, *iv = nullptr, *iv_off = nullptr;
+    size_t bytes = 0;
+    espk::RefView view() const {
+        espk::RefView v;
+        v.packed = (const uint8_t *)packed;
+        v.pk_off = (const uint64_t *)pk_off;
+        v.len = (const uint32_t *)dlen;
+        v.iv = (const uint32_t *)iv;
+        v.iv_off = (const uint64_t *)iv_off;
+        v.ncontig = ncontig;
+        return v;
+    }
 };
 
 namespace esph {
@@ -195,7 +221,7 @@ inline int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t
     uint64_t lanes = (uint64_t)c->ncu * per_cu * espk::GOTOH_BLOCK;
     lanes = std::min<uint64_t>(lanes, align_up(count, 64));
     const uint64_t tw = std::max<uint64_t>(maxw, 1), rw = 5 * std::max<uint64_t>(maxm, 1);
-    lanes = std::min<uint64_t>(lanes, kTraceBudget / (tw * 4));
+    lanes = std::min<uint64_t>(lanes, c->trace_budget / (tw * 4));
     lanes = std::min<uint64_t>(lanes, kRowBudget / (rw * 4));
     lanes = std::max<uint64_t>(lanes / 64 * 64, 64);
     a.nl = (uint32_t)lanes;

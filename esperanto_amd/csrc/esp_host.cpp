@@ -208,7 +208,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             if (!c->band_blocks[cls]) c->band_blocks[cls] = espk::gotoh_band_blocks_per_cu(half);
             uint64_t blocks = (uint64_t)c->ncu * c->band_blocks[cls];
             blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
-            blocks = std::min<uint64_t>(blocks, kTraceBudget / (maxw * 64 * 4));
+            blocks = std::min<uint64_t>(blocks, c->trace_budget / (maxw * 64 * 4));
             blocks = std::max<uint64_t>(blocks, 1);
             HIP_TRY(c, c->btrace.ensure(maxw * 64 * 4 * blocks));
             a.trace = (uint32_t *)c->btrace.p;
@@ -241,18 +241,21 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         f.touch = nullptr;  // extend_hint returns no TouchRec
         f.range = dp->esc_range;
         f.counter = &dp->counters[espk::PLAN_NCLASS];
-        const int rows = espk::full_rows(hp->esc_maxm);
-        const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
-        if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
-        uint64_t blocks = (uint64_t)c->ncu * c->full_blocks[rows & 511];
-        blocks = std::min<uint64_t>(blocks, ((uint64_t)hp->range[1][1] + 63) / 64);
-        blocks = std::min<uint64_t>(blocks, kFullTraceBudget / (maxw * 64 * 4));
-        blocks = std::max<uint64_t>(blocks, 1);
-        HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * blocks));
-        f.trace = (uint32_t *)c->ftrace.p;
-        f.maxw = (uint32_t)maxw;
-        HIP_TRY(c, espk::launch_gotoh_full(f, rows, (int)blocks, c->stream));
-        tim_mark(c, 7, rows);
+        const uint64_t bound = hp->range[1][1];
+        {
+            const int rows = espk::full_rows(hp->esc_maxm);
+            const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
+            if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
+            uint64_t blocks = (uint64_t)c->ncu * c->full_blocks[rows & 511];
+            blocks = std::min<uint64_t>(blocks, (bound + 63) / 64);
+            blocks = std::min<uint64_t>(blocks, c->full_trace_budget / (maxw * 64 * 4));
+            blocks = std::max<uint64_t>(blocks, 1);
+            HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * blocks));
+            f.trace = (uint32_t *)c->ftrace.p;
+            f.maxw = (uint32_t)maxw;
+            HIP_TRY(c, espk::launch_gotoh_full(f, rows, (int)blocks, c->stream));
+            tim_mark(c, 7, rows);
+        }
     }
     return ESP_OK;
 }
@@ -386,6 +389,42 @@ int esp_last_timings(esp_ctx *c, esp_timing *out, size_t cap, size_t *n) {
         out[k] = c->tim[k];
         out[k].ms = f;
     }
+    return ESP_OK;
+}
+
+int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
+    if (!c) return ESP_E_INVAL;
+    switch (option) {
+    case ESP_OPT_FULL_KERNEL:
+        // 0 (by batch size) and 1 (lane per pair) run the lane-per-pair kernel;
+        // the wave-per-pair kernel is not built.
+        if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full kernel choice not in {0,1}");
+        c->full_kernel = (int)value;
+        return ESP_OK;
+    case ESP_OPT_TRACE_BUDGET:
+        if (value < (1 << 20)) return fail(c, ESP_E_INVAL, "trace budget below 1 MiB");
+        c->trace_budget = (size_t)value;
+        c->full_trace_budget = (size_t)value * 2;
+        return ESP_OK;
+    default:
+        return fail(c, ESP_E_INVAL, "unknown option");
+    }
+}
+
+int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
+    if (!c || !out) return ESP_E_INVAL;
+    std::memset(out, 0, sizeof(*out));
+    if (!c->plan.p) return ESP_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    espk::PlanDev *dp = (espk::PlanDev *)c->plan.p;
+    espk::PlanSummary sum;
+    uint32_t esc[2];
+    HIP_TRY(c, hipMemcpy(&sum, &dp->sum, sizeof(sum), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(esc, dp->esc_range, sizeof(esc), hipMemcpyDeviceToHost));
+    for (int k = 0; k < espk::PLAN_STRIP; k++) out->band_pairs[k] = sum.range[k][1];
+    out->strip_pairs = sum.range[espk::PLAN_STRIP][1];
+    out->escalated = esc[1];
     return ESP_OK;
 }
 

@@ -358,13 +358,15 @@ __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint
 }
 
 // Wave-aggregated increment of counter[b]: lanes that share a bin add once
-// (one atomic per distinct bin in the wave, not per lane); returns the old
-// value plus this lane's rank among them.  Every lane of the wave calls it.
+// and take consecutive slots; returns this lane's slot.  Up to 4 bins per
+// wave are served that way (a batch dominated by one read length puts whole
+// waves on one address); lanes of further bins add on their own.  Every lane
+// of the wave calls it.
 __device__ __forceinline__ uint32_t agg_inc(uint32_t *counter, uint32_t b, bool active) {
     const uint32_t lane = threadIdx.x & 63u;
     uint64_t pending = __ballot(active);
     uint32_t mine = 0;
-    while (pending) {
+    for (int round = 0; round < 4 && pending; round++) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lb = __shfl(b, leader);
         const uint64_t same = __ballot(active && b == lb) & pending;
@@ -374,6 +376,7 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t *counter, uint32_t b, bool 
         if ((same >> lane) & 1ull) mine = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
         pending &= ~same;
     }
+    if ((pending >> lane) & 1ull) mine = atomicAdd(&counter[b], 1u);
     return mine;
 }
 

@@ -16,23 +16,6 @@
 
 using namespace esph;
 
-struct esp_ref {
-    int device = 0;
-    uint32_t ncontig = 0;
-    std::vector<uint32_t> len;  // host copy for argument checks
-    void *packed = nullptr, *pk_off = nullptr, *dlen = nullptr, *iv = nullptr, *iv_off = nullptr;
-    size_t bytes = 0;
-    espk::RefView view() const {
-        espk::RefView v;
-        v.packed = (const uint8_t *)packed;
-        v.pk_off = (const uint64_t *)pk_off;
-        v.len = (const uint32_t *)dlen;
-        v.iv = (const uint32_t *)iv;
-        v.iv_off = (const uint64_t *)iv_off;
-        v.ncontig = ncontig;
-        return v;
-    }
-};
 
 namespace {
 

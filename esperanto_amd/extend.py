@@ -35,6 +35,10 @@ class ExtendBuffer:
     def handle(self):
         return self._h
 
+    def set_option(self, option: int, value: int) -> None:
+        """esp_ctx_set_option (ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET)."""
+        _native.check(_native.lib().esp_ctx_set_option(self._h, option, value), self._h)
+
     def close(self) -> None:
         if self._h:
             _native.lib().esp_destroy(self._h)

@@ -68,3 +68,49 @@ def splice_rescue_workload(npairs: int, ref_len: int = 64_000_000, read_len: int
                         e.astype(np.uint32))
     truth = {"s": s, "e": e, "u": u, "p2": p2}
     return reference, splice, mate, truth
+
+
+def serving_workload(n_requests: int, ref_len: int = 256_000_000, seed: int = 0xE5A00005,
+                     rescue_fraction: float = 0.1, max_pairs: int = 4096, sub_rate: float = 0.01):
+    """Requests for the serving loop on a uniform-ACGT contig of ref_len bases.
+
+    Each request holds N ~ log-uniform[1, max_pairs] pairs with reads of
+    U[50, 250] bases planted at uniform loci with 1% substitutions.  Extension
+    requests (the rest) ask extend_hint over window [locus - 32, locus + m + 32)
+    with hint 32; a rescue_fraction of requests are mate-rescue requests
+    (mate position 0-300 bases before the planted locus, half of the reads
+    reverse-complemented), whose reference-geometry windows span 1650-2250
+    bases.  Returns (Reference, requests) with each request a dict of numpy
+    arrays: kind, reads (concatenated), read_len, contig, and win_start /
+    win_end / hint or mate_pos.
+    """
+    rng = np.random.default_rng(seed)
+    ref = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, ref_len)].copy()
+    reference = Reference.from_sequences([ref.tobytes()])
+    reqs = []
+    for _ in range(n_requests):
+        n = int(np.exp(rng.uniform(0.0, np.log(max_pairs))))
+        n = max(1, min(max_pairs, n))
+        rescue = rng.random() < rescue_fraction
+        m = rng.integers(50, 251, n)
+        loc = rng.integers(3000, ref_len - 3000, n)
+        off = np.zeros(n + 1, dtype=np.int64)
+        off[1:] = np.cumsum(m)
+        idx = np.repeat(loc - off[:-1], m) + np.arange(off[-1])
+        reads = ref[idx]
+        if rescue:
+            minus = rng.random(n) < 0.5
+            for k in np.nonzero(minus)[0]:
+                a, b = off[k], off[k + 1]
+                reads[a:b] = _COMP[reads[a:b][::-1]]
+        _substitute(rng, reads, sub_rate)
+        req = {"kind": 1 if rescue else 0, "reads": reads, "read_len": m.astype(np.uint32),
+               "contig": np.zeros(n, dtype=np.uint32)}
+        if rescue:
+            req["mate_pos"] = (loc - rng.integers(0, 301, n)).astype(np.uint32)
+        else:
+            req["win_start"] = (loc - 32).astype(np.uint32)
+            req["win_end"] = (loc + m + 32).astype(np.uint32)
+            req["hint"] = np.full(n, 32, dtype=np.int64)
+        reqs.append(req)
+    return reference, reqs

@@ -173,6 +173,25 @@ typedef struct {
 } esp_timing;
 int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
 
+/* Context options.  ESP_OPT_FULL_KERNEL picks the kernel of extend_hint's
+ * full-matrix reruns: 0 by batch size (default), 1 lane per pair; both run
+ * the lane-per-pair kernel in this build (2, wave per pair, is refused).
+ * ESP_OPT_TRACE_BUDGET caps the traceback scratch of one launch in bytes
+ * (the full-matrix kernels get twice that); grids shrink to fit. */
+enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2 };
+int esp_ctx_set_option(esp_ctx *ctx, int option, int64_t value);
+
+/* Routing of the last Gotoh batch (or pipeline) on this context: pairs per
+ * band-kernel half (16, 30, 32, 48, 64), pairs in the general strip kernel,
+ * and extend_hint band attempts that escalated to the full matrix.  Waits
+ * for the context stream. */
+typedef struct {
+    uint32_t band_pairs[5];
+    uint32_t strip_pairs;
+    uint32_t escalated;
+} esp_plan_stats;
+int esp_last_plan_stats(esp_ctx *ctx, esp_plan_stats *out);
+
 /* Device memory helpers for callers without an allocator of their own.
  * Copies are synchronous with respect to the context stream. */
 int esp_device_alloc(esp_ctx *ctx, size_t bytes, void **out);
@@ -296,6 +315,86 @@ typedef struct {
 int esp_spliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_splice_batch *b,
                       const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
                       const uint64_t *stitch_off, esp_splice_rec *rec);
+
+/* ---- serving loop ---------------------------------------------------------- */
+
+/* A request loop over the resident reference with continuous batching: the
+ * replacement for the per-read rayon pipeline (pipeline.rs:393-404) when the
+ * callers are many independent clients.  Requests queue in arrival order;
+ * each of `workers` worker threads owns a context (stream + scratch) and
+ * takes the next batch of queued requests that fits its pair cap and the
+ * device-memory budget, so batches overlap on the device and a request that
+ * arrives while others run joins the next free slot.  Results are identical
+ * to running each request alone (every pair is computed independently). */
+typedef struct esp_server esp_server;
+typedef struct {
+    esp_params params;          /* ExtendParams of every request */
+    uint64_t mem_budget;        /* device bytes for in-flight request data */
+    uint32_t workers;           /* batches in flight (contexts), 1..8 */
+    uint32_t max_batch_pairs;   /* pairs per batch; a larger request runs alone */
+    uint32_t max_pending;       /* queued requests before submit returns ESP_E_BUSY */
+    uint32_t max_read_len;      /* longest read accepted (<= 16000; rescue reads <= 256) */
+} esp_server_config;
+
+/* kind ESP_REQ_EXTEND: per pair, extend_hint(read, contig[win_start, win_end),
+ *   hint) (the window decoded from the resident reference, extend.rs:392);
+ * kind ESP_REQ_RESCUE: per pair, mate_window_placement(read, contig, mate_pos)
+ *   (align.rs:851).
+ * Inputs are host memory and are copied before submit returns. */
+enum { ESP_REQ_EXTEND = 0, ESP_REQ_RESCUE = 1 };
+typedef struct {
+    uint32_t kind;
+    uint32_t npairs;
+    const uint8_t *reads;       /* concatenated */
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *win_start;  /* EXTEND */
+    const uint32_t *win_end;    /* EXTEND */
+    const int64_t *hint;        /* EXTEND */
+    const uint32_t *mate_pos;   /* RESCUE */
+} esp_request;
+
+/* Host output buffers, written before the request completes; they must stay
+ * valid until esp_server_wait reports it done.  cigar holds, per pair k,
+ * esp_cigar_capacity(read_len[k]) words in order (pair k starts at the sum
+ * of the capacities before it). */
+typedef struct {
+    esp_ext *ext;           /* EXTEND: npairs records */
+    esp_mate_rec *mate;     /* RESCUE: npairs records */
+    uint32_t *cigar;
+    uint32_t *cig_n;
+} esp_request_out;
+
+typedef struct {
+    int32_t status;         /* ESP_OK, or the error the batch hit */
+    int32_t done;
+    uint64_t submit_ns;     /* steady-clock times */
+    uint64_t done_ns;
+    uint64_t batch;         /* batch sequence number that carried it */
+} esp_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, batches, pairs;
+    uint64_t inflight_bytes, peak_inflight_bytes;
+    uint32_t pending, max_batch_requests;
+    /* summed over batches (ns): host staging, extension phase (launch to
+     * results on the host), rescue phase, scattering results to callers */
+    uint64_t stage_ns, extend_ns, rescue_ns, scatter_ns;
+} esp_server_stats;
+
+int esp_server_create(int device, const esp_ref *ref, const esp_server_config *cfg,
+                      esp_server **out);
+int esp_server_destroy(esp_server *s);
+/* ESP_E_BUSY: queue full; ESP_E_NOMEM: the request alone exceeds the budget;
+ * ESP_E_INVAL: malformed request (lengths, contig, kind). */
+int esp_server_submit(esp_server *s, const esp_request *req, const esp_request_out *out,
+                      uint64_t *ticket);
+/* Waits up to timeout_us (< 0: forever) for the ticket; info->done tells.
+ * A completed ticket is released by this call. */
+int esp_server_wait(esp_server *s, uint64_t ticket, int64_t timeout_us, esp_request_info *info);
+int esp_server_stats_get(esp_server *s, esp_server_stats *out);
+/* Steady-clock now, the clock of submit_ns / done_ns. */
+uint64_t esp_now_ns(void);
 
 #ifdef __cplusplus
 }

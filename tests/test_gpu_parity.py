@@ -191,11 +191,16 @@ def test_extend_hint_band_route_and_escalation(buf):
                 r[q] = b"ACGT"[int(rng.integers(0, 4))]
             r = bytes(r)
         extra.append((r, w, 60, 0))
-    for part, label in ((hb, "hint_c1"), (PairBatch.from_pairs(extra), "hint_escalating")):
+    from esperanto_amd import _native
+    cases = [(hb, "hint_c1", 0), (PairBatch.from_pairs(extra), "hint_escalating", 0),
+             (PairBatch.from_pairs(extra), "hint_escalating_lane_kernel", 1)]
+    for part, label, kernel in cases:
         say(f"{label} {part.npairs} pairs")
+        buf.set_option(_native.ESP_OPT_FULL_KERNEL, kernel)
         db = DeviceBatch(part, buf)
         db.launch_gotoh(ExtendParams(), 2)
         db.sync()
+        buf.set_option(_native.ESP_OPT_FULL_KERNEL, 0)
         g = db.gotoh_results()
         o = oracle.band_batch(part, ExtendParams(), 2, nthreads=8)
         assert np.array_equal(g[0], o[0]), label
@@ -205,11 +210,22 @@ def test_extend_hint_band_route_and_escalation(buf):
         for k in range(part.npairs):
             a = int(off[k])
             assert np.array_equal(g[2][a:a + g[4][k]], o[2][a:a + o[4][k]]), (label, k)
-        if label == "hint_escalating":
+        if label.startswith("hint_escalating"):
             assert int((g[5] == 1).sum()) > 400
+            rerun = [t for t in db.last_timings() if t[0] == 7]
+            assert rerun and rerun[0][1] > 0  # the lane-per-pair full-matrix kernel ran
         kinds = {t[0] for t in db.last_timings()}
         assert 1 in kinds  # the band kernel ran the first attempts
         db.free()
+
+
+@pytest.mark.skip(reason="wave-per-pair full-matrix kernel (DPP wave_shr:1 column pipeline) hung "
+                         "on its first GPU run; its CPU emulation is bit-exact with the oracle and "
+                         "no cause is shown in the code, so the kernel is not built this round")
+def test_extend_hint_escalations_wave_per_pair_kernel(buf):
+    """The escalation cases of the test above with ESP_OPT_FULL_KERNEL = 2."""
+    from esperanto_amd import _native
+    buf.set_option(_native.ESP_OPT_FULL_KERNEL, 2)
 
 
 def test_config3_sample(buf):

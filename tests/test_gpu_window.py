@@ -269,12 +269,17 @@ def test_spliced_known_answer_and_workload(buf):
     ref, sp, mt, truth = splice_rescue_workload(4000, ref_len=2_000_000)
     o = oracle.OracleRef(ref)
     dref = DeviceReference(ref, buf)
-    db = DeviceSpliceBatch(sp, dref, buf)
-    say("splice/rescue workload 4000 pairs: spliced")
-    db.launch()
-    db.sync()
-    _compare_splice(sp, db.raw_results(), oracle.spliced_batch(o, sp, nthreads=8))
-    db.free()
+    expect = oracle.spliced_batch(o, sp, nthreads=8)
+    from esperanto_amd import _native
+    for kernel in (0, 1):  # auto (wave per pair at this size), then lane per pair
+        buf.set_option(_native.ESP_OPT_FULL_KERNEL, kernel)
+        db = DeviceSpliceBatch(sp, dref, buf)
+        say(f"splice/rescue workload 4000 pairs: spliced, full kernel {kernel}")
+        db.launch()
+        db.sync()
+        _compare_splice(sp, db.raw_results(), expect)
+        db.free()
+    buf.set_option(_native.ESP_OPT_FULL_KERNEL, 0)
     from esperanto_amd.rescue import DeviceMateBatch
     mb = DeviceMateBatch(mt, dref, buf)
     say("splice/rescue workload 4000 pairs: mate rescue")
