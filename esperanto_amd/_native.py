@@ -26,6 +26,7 @@ EXPORTS = [
     "esp_ref_device_bytes", "esp_gather_batch", "esp_mate_rescue_batch", "esp_spliced_batch",
     "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
+    "esp_unspliced_batch",
 ]
 
 
@@ -60,6 +61,15 @@ class CHostMeta(ctypes.Structure):
 class CPlanStats(ctypes.Structure):
     _fields_ = [("band_pairs", ctypes.c_uint32 * 5), ("strip_pairs", ctypes.c_uint32),
                 ("escalated", ctypes.c_uint32)]
+
+
+class CChainBatch(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_size_t), ("reads", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
+                ("contig", ctypes.c_void_p), ("minus", ctypes.c_void_p),
+                ("ref_start", ctypes.c_void_p), ("ref_end", ctypes.c_void_p),
+                ("read_start", ctypes.c_void_p), ("read_end", ctypes.c_void_p),
+                ("allow_dp", ctypes.c_void_p), ("max_read_len", ctypes.c_uint32)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -123,6 +133,7 @@ def lib():
     L.esp_ref_device_bytes.argtypes = [vp, P(sz)]
     L.esp_gather_batch.argtypes = [vp, vp, vp, sz, vp, vp]
     L.esp_mate_rescue_batch.argtypes = [vp, vp, P(CMateBatch), P(CParams), vp, vp, vp, vp]
+    L.esp_unspliced_batch.argtypes = [vp, vp, P(CChainBatch), P(CParams), vp, vp, vp, vp]
     L.esp_spliced_batch.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp]
     _lib = L
     return L

@@ -214,7 +214,36 @@ struct SpliceArgs {
     esp_splice_rec *rec;
 };
 
+struct UnspArgs {
+    uint32_t npairs;
+    RefView R;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig, *minus, *rs, *re, *qs, *qe, *allow_dp;
+    uint32_t max_read;
+    uint32_t qstride;
+    uint64_t wstride;
+    uint8_t *qarena;
+    uint8_t *warena;
+    espd::GParams P;
+    // per-pair scratch
+    uint64_t *q_off, *w_off, *s_start, *g_read_off;
+    int64_t *g_hint;
+    uint32_t *q_len, *w_len, *wstart, *g_read_len, *ea_bi, *ea_bj;
+    int32_t *state, *dist, *ea_sum;
+    esp_ext *ext;
+    // outputs
+    esp_unspliced_rec *out;
+    uint32_t *cigar;
+    const uint64_t *cig_off;
+    uint32_t *cig_n;
+};
+
 hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
+hipError_t launch_unsp_prep(const UnspArgs &a, hipStream_t s);
+hipError_t launch_unsp_ea(const UnspArgs &a, hipStream_t s);
+hipError_t launch_unsp_finish(const UnspArgs &a, hipStream_t s);
 hipError_t launch_mate_prep(const MateArgs &a, hipStream_t s);
 hipError_t launch_mate_select(const MateArgs &a, hipStream_t s);
 hipError_t launch_mate_finish(const MateArgs &a, hipStream_t s);

@@ -251,6 +251,115 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
     return ESP_OK;
 }
 
+int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,
+                        const esp_params *p, esp_unspliced_rec *out, uint32_t *cigar,
+                        const uint64_t *cig_off, uint32_t *cig_n) {
+    if (!c) return ESP_E_INVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!r || !b || !out || !cigar || !cig_off || !cig_n) return fail(c, ESP_E_INVAL, "NULL argument");
+    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
+    if (b->npairs == 0) return ESP_OK;
+    if (b->npairs > 0xFFFFFF00ull) return fail(c, ESP_E_INVAL, "too many pairs");
+    if (!b->reads || !b->read_off || !b->read_len || !b->contig || !b->minus || !b->ref_start ||
+        !b->ref_end || !b->read_start || !b->read_end || !b->allow_dp)
+        return fail(c, ESP_E_INVAL, "batch arrays are required");
+    if (b->max_read_len > kMaxRead) return fail(c, ESP_E_INVAL, "max_read_len above 16000");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = b->npairs;
+    const uint32_t maxm = std::max<uint32_t>(1, b->max_read_len);
+    espk::UnspArgs a{};
+    a.npairs = (uint32_t)n;
+    a.R = r->view();
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.contig = b->contig;
+    a.minus = b->minus;
+    a.rs = b->ref_start;
+    a.re = b->ref_end;
+    a.qs = b->read_start;
+    a.qe = b->read_end;
+    a.allow_dp = b->allow_dp;
+    a.max_read = maxm;
+    a.P = dev_params(p);
+    a.qstride = (uint32_t)align_up(maxm, 16);
+    a.wstride = align_up(2 * (uint64_t)maxm + 64, 16);  // read + 2 * (m/2 + 30) at most
+    HIP_TRY(c, c->wquery.ensure(n * a.qstride + 64));
+    HIP_TRY(c, c->warena.ensure(n * a.wstride + 64));
+    HIP_TRY(c, c->wdesc.ensure(n * (5 * 8 + 6 * 4 + 3 * 4 + sizeof(esp_ext)) + 16 * 256));
+    a.qarena = (uint8_t *)c->wquery.p;
+    a.warena = (uint8_t *)c->warena.p;
+    Carve cv{(uint8_t *)c->wdesc.p};
+    a.q_off = cv.take<uint64_t>(n);
+    a.w_off = cv.take<uint64_t>(n);
+    a.s_start = cv.take<uint64_t>(n);
+    a.g_read_off = cv.take<uint64_t>(n);
+    a.g_hint = cv.take<int64_t>(n);
+    a.q_len = cv.take<uint32_t>(n);
+    a.w_len = cv.take<uint32_t>(n);
+    a.wstart = cv.take<uint32_t>(n);
+    a.g_read_len = cv.take<uint32_t>(n);
+    a.ea_bi = cv.take<uint32_t>(n);
+    a.ea_bj = cv.take<uint32_t>(n);
+    a.state = cv.take<int32_t>(n);
+    a.dist = cv.take<int32_t>(n);
+    a.ea_sum = cv.take<int32_t>(n);
+    a.ext = cv.take<esp_ext>(n);
+    a.out = out;
+    a.cigar = cigar;
+    a.cig_off = cig_off;
+    a.cig_n = cig_n;
+    HIP_TRY(c, c->counter.ensure(256));
+
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_unsp_prep(a, c->stream));
+    tim_mark(c, 5, 0);
+    {  // EA-Myers best start of the read over its window (reads of 1..256 bases)
+        const int words = (int)((std::min<uint32_t>(maxm, 256) + 63) / 64);
+        uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(words), (n + 63) / 64);
+        espk::MyersArgs m{};
+        m.npairs = (uint32_t)n;
+        m.reads = a.qarena;
+        m.read_off = a.q_off;
+        m.read_len = a.q_len;
+        m.texts = a.warena;
+        m.text_off = a.w_off;
+        m.text_len = a.w_len;
+        m.dist = a.dist;
+        m.end = nullptr;
+        m.start = a.s_start;
+        m.counter = (uint32_t *)c->counter.p;
+        HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
+        HIP_TRY(c, espk::launch_myers(m, words, (int)std::max<uint64_t>(blocks, 1), c->stream));
+        tim_mark(c, 3, 0);
+    }
+    HIP_TRY(c, espk::launch_unsp_ea(a, c->stream));
+    tim_mark(c, 6, 0);
+    esp_batch gb{};
+    gb.npairs = n;
+    gb.reads = a.qarena;
+    gb.read_off = a.g_read_off;
+    gb.read_len = a.g_read_len;
+    gb.wins = a.warena;
+    gb.win_off = a.w_off;
+    gb.win_len = a.w_len;
+    gb.offset = a.g_hint;
+    gb.half = nullptr;
+    esp_batch_out go{};
+    go.ext = a.ext;
+    go.cigar = cigar;
+    go.cig_off = cig_off;
+    go.cig_n = cig_n;
+    rc = gotoh_batch_core(c, &gb, p, ESP_MODE_HINT, &go);
+    if (rc) return rc;
+    HIP_TRY(c, espk::launch_unsp_finish(a, c->stream));
+    tim_mark(c, 6, 0);
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
 int esp_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b,
                       const esp_params *p, esp_batch_out *o, uint32_t *stitched,
                       const uint64_t *stitch_off, esp_splice_rec *rec) {

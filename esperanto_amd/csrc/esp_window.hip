@@ -290,6 +290,157 @@ __global__ __launch_bounds__(WIN_BLOCK) void k_splice_finish(SpliceArgs a) {
     a.rec[k] = r;
 }
 
+// continuous_extension_window (align.rs:33-55), u32 saturating arithmetic.
+__device__ void ext_window(uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, uint32_t read_len,
+                           uint32_t clen, uint32_t &start, uint32_t &end) {
+    auto sadd = [](uint32_t x, uint32_t y) { return (uint32_t)min((uint64_t)x + y, (uint64_t)0xFFFFFFFFu); };
+    const uint32_t flank = read_len / 2 + 30;
+    const uint32_t head = max(flank, qs);
+    const uint32_t tail = max(flank, read_len > qe ? read_len - qe : 0u);
+    const uint32_t st = min(rs > head ? rs - head : 0u, clen);
+    uint32_t en = min(sadd(re, tail), clen);
+    en = min(en, sadd(sadd(rs, read_len), flank));
+    en = min(en, sadd(sadd(st, read_len), sadd(flank, flank)));
+    start = st;
+    end = max(en, st);
+}
+
+// Unspliced chain, stage 1 (align.rs:1776-1790, :1895-1906): window, decoded
+// window, the read in alignment orientation, Myers descriptors.
+__global__ __launch_bounds__(WIN_BLOCK) void k_unsp_prep(UnspArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (WIN_BLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= a.npairs) return;
+    const uint32_t m = a.read_len[k], ctg = a.contig[k];
+    int32_t state = 1;
+    uint32_t ws = 0, we = 0;
+    if (ctg >= a.R.ncontig || m > a.max_read) {
+        state = -1;
+    } else {
+        ext_window(a.rs[k], a.re[k], a.qs[k], a.qe[k], m, a.R.len[ctg], ws, we);
+    }
+    const uint64_t qo = (uint64_t)k * a.qstride, wo = (uint64_t)k * a.wstride;
+    if (lane == 0) {
+        a.state[k] = state;
+        a.wstart[k] = ws;
+        a.w_len[k] = we - ws;
+        a.w_off[k] = wo;
+        a.q_off[k] = qo;
+        a.q_len[k] = (state == 1 && m >= 1 && m <= 256 && we - ws >= m) ? m : 0u;
+    }
+    if (state != 1) return;
+    wave_decode(a.R, ctg, ws, we, false, a.warena + wo, lane);
+    const uint8_t *src = a.reads + a.read_off[k];
+    uint8_t *q = a.qarena + qo;
+    const bool rc = a.minus[k] != 0;
+    for (uint32_t k0 = lane * 16; k0 < m; k0 += 1024) {
+        const uint32_t cnt = min(16u, m - k0);
+        uint32_t wq[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (uint32_t t = 0; t < 16; t++)
+            if (t < cnt)
+                wq[t >> 2] |= (rc ? comp_ascii(src[m - 1 - (k0 + t)]) : (uint32_t)src[k0 + t]) << (8 * (t & 3));
+        store_run(q + k0, wq, cnt);
+    }
+}
+
+// extend.rs:87-93 on raw bytes: 1 plus (ref A / read G), 0 minus (ref T / read C), -1 none.
+__device__ __forceinline__ int32_t exempt_dir_b(uint32_t ref_b, uint32_t read_b) {
+    return (ref_b == 'A' && read_b == 'G') ? 1 : (ref_b == 'T' && read_b == 'C') ? 0 : -1;
+}
+
+// Unspliced chain, stage 2 (align.rs:1807-1877): the distance gate, the
+// per-base recount, Kadane over the diagonal; else the DP descriptors.
+__global__ __launch_bounds__(WIN_BLOCK) void k_unsp_ea(UnspArgs a) {
+    const uint32_t k = blockIdx.x * WIN_BLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    const uint32_t m = a.read_len[k];
+    int32_t sum = 0;
+    uint32_t bi = 0, bj = 0;
+    bool ea = false;
+    if (a.q_len[k]) {
+        const int32_t thr = max((int32_t)m / 3, 10), d = a.dist[k];
+        const uint64_t start = a.s_start[k];
+        const uint8_t *q = a.qarena + a.q_off[k], *w = a.warena + a.w_off[k] + start;
+        bool usable = d <= thr && start + m <= a.w_len[k];
+        if (usable) {
+            int32_t mm = 0;
+            for (uint32_t t = 0; t < m; t++) {
+                const uint32_t qb = q[t], rb = w[t];
+                if (qb == 'N' || rb == 'N') {
+                    usable = false;
+                    break;
+                }
+                if (qb != rb && exempt_dir_b(rb, qb) < 0) mm++;
+            }
+            if (mm != d) usable = false;
+        }
+        if (usable) {
+            int32_t cur = 0;
+            uint32_t ci = 0;
+            for (uint32_t t = 0; t < m; t++) {
+                const uint32_t qb = q[t], rb = w[t];
+                const int32_t ed = exempt_dir_b(rb, qb);
+                const bool ex = (a.P.edit == 1 && ed == 1) || (a.P.edit == 2 && ed == 0);
+                const int32_t s = qb == rb ? a.P.match : (ex ? 0 : a.P.mismatch);
+                if (cur <= 0) {
+                    cur = s;
+                    ci = t;
+                } else {
+                    cur += s;
+                }
+                if (cur > sum) {
+                    sum = cur;
+                    bi = ci;
+                    bj = t + 1;
+                }
+            }
+            ea = sum > 0 && (bj - bi) * 2 >= m;
+        }
+    }
+    a.ea_sum[k] = ea ? sum : 0;
+    a.ea_bi[k] = bi;
+    a.ea_bj[k] = bj;
+    const bool dp = !ea && a.state[k] == 1 && a.allow_dp[k] != 0;
+    a.g_read_off[k] = a.q_off[k];
+    a.g_read_len[k] = dp ? m : 0u;
+    a.g_hint[k] = (int64_t)(a.rs[k] - a.wstart[k]) - (int64_t)a.qs[k];
+}
+
+// Unspliced chain, stage 3 (align.rs:1850-1918): the placement.
+__global__ __launch_bounds__(WIN_BLOCK) void k_unsp_finish(UnspArgs a) {
+    const uint32_t k = blockIdx.x * WIN_BLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    esp_unspliced_rec r;
+    r.status = a.state[k] < 0 ? -1 : 0;
+    r.strand = a.minus[k] ? 1 : 0;
+    r.pos = 0;
+    r.score = 0;
+    r.dist = 0;
+    r.wstart = a.wstart[k];
+    r.wlen = a.w_len[k];
+    uint32_t nops = 0;
+    const uint64_t c0 = a.cig_off[k];
+    if (a.ea_sum[k] > 0) {
+        const uint32_t m = a.read_len[k], bi = a.ea_bi[k], bj = a.ea_bj[k];
+        uint32_t *cg = a.cigar + c0;
+        if (bi > 0) cg[nops++] = (bi << 4) | OP_S;
+        cg[nops++] = ((bj - bi) << 4) | OP_M;
+        if (m > bj) cg[nops++] = ((m - bj) << 4) | OP_S;
+        r.status = 1;
+        r.pos = r.wstart + (uint32_t)a.s_start[k] + bi;
+        r.score = a.ea_sum[k];
+        r.dist = a.dist[k];
+    } else if (a.g_read_len[k] && a.ext[k].score > 0) {
+        r.status = 2;
+        r.pos = r.wstart + a.ext[k].ref_start;
+        r.score = a.ext[k].score;
+        nops = a.cig_n[k];
+    }
+    a.out[k] = r;
+    a.cig_n[k] = nops;
+}
+
 uint32_t wave_blocks(uint32_t n) { return (n + WIN_BLOCK / 64 - 1) / (WIN_BLOCK / 64); }
 uint32_t lane_blocks(uint32_t n) { return (n + WIN_BLOCK - 1) / WIN_BLOCK; }
 
@@ -313,6 +464,21 @@ hipError_t launch_mate_select(const MateArgs &a, hipStream_t s) {
 
 hipError_t launch_mate_finish(const MateArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_mate_finish, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_unsp_prep(const UnspArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_unsp_prep, dim3(wave_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_unsp_ea(const UnspArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_unsp_ea, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_unsp_finish(const UnspArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_unsp_finish, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

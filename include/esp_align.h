@@ -316,6 +316,44 @@ int esp_spliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_splice_batch *
                       const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
                       const uint64_t *stitch_off, esp_splice_rec *rec);
 
+/* Unspliced chain extension, extend_unspliced_chain after its fast lane
+ * (align.rs:1770-1919): per chain, the window continuous_extension_window
+ * (align.rs:33-55) of the chain's reference span [ref_start, ref_end) and read
+ * span [read_start, read_end); the read in alignment orientation (reverse
+ * complement for minus chains); for reads of 1..256 bases the EA-Myers lane
+ * (best start within max(m/3, 10), a per-base mismatch recount equal to the
+ * distance, no N, Kadane-optimal soft clips kept when they cover half the
+ * read); otherwise, when allow_dp[k] != 0 (the caller's DP budget),
+ * extend_hint over the window at the chain diagonal.  Device pointers;
+ * max_read_len (host) bounds read_len and sizes the scratch. */
+typedef struct {
+    size_t npairs;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *minus;      /* 0 Plus, 1 Minus */
+    const uint32_t *ref_start;  /* Chain::ref_span */
+    const uint32_t *ref_end;
+    const uint32_t *read_start; /* Chain::read_span */
+    const uint32_t *read_end;
+    const uint32_t *allow_dp;
+    uint32_t max_read_len;
+} esp_chain_batch;
+typedef struct {
+    int32_t status;   /* 1 EA lane placement, 2 banded DP placement, 0 none,
+                         -1 invalid (contig out of range, read past max_read_len) */
+    int32_t strand;
+    uint32_t pos;
+    int32_t score;
+    int32_t dist;     /* EA-Myers distance of an EA-lane placement */
+    uint32_t wstart;  /* the window [wstart, wstart + wlen) */
+    uint32_t wlen;
+} esp_unspliced_rec;
+int esp_unspliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_chain_batch *b,
+                        const esp_params *p, esp_unspliced_rec *out, uint32_t *cigar,
+                        const uint64_t *cig_off, uint32_t *cig_n);
+
 /* ---- serving loop ---------------------------------------------------------- */
 
 /* A request loop over the resident reference with continuous batching: the

@@ -12,7 +12,9 @@
  *   crates/map/src/splice.rs    (stitch_cigar, align_spliced after
  *                                prepare_spliced, SplicedAlignment::genomic_start)
  *   crates/map/src/fasta.rs     (Contig::decode_into / base / is_n)
- *   crates/map/src/align.rs     (mate_window_placement, revcomp_into)
+ *   crates/map/src/align.rs     (mate_window_placement, revcomp_into,
+ *                                continuous_extension_window, the EA-Myers
+ *                                lane of extend_unspliced_chain)
  * plus two independent textbook checkers (local affine-gap DP max score over
  * a row-window set, semi-global EA edit distance) used to pin the
  * restatement where the reference has no literal expected outputs.
@@ -985,6 +987,134 @@ void espo_spliced_batch(const espo_ref *R, size_t npairs, const uint8_t *reads,
     J.cigar = cigar; J.cig_off = cig_off; J.cig_n = cig_n; J.stitched = stitched;
     J.stitch_off = stitch_off; J.srec = rec;
     run_window_jobs(&J, npairs, nthreads, splice_worker);
+}
+
+/* ---------------------------------------------- unspliced chain extension */
+
+static uint32_t sat_add32(uint32_t a, uint32_t b) { uint64_t s = (uint64_t)a + b; return s > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s; }
+static uint32_t sat_sub32(uint32_t a, uint32_t b) { return a > b ? a - b : 0u; }
+static uint32_t min32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+static uint32_t max32(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+/* continuous_extension_window: align.rs:33-55 (rs, re = Chain::ref_span,
+ * qs, qe = Chain::read_span). */
+void espo_ext_window(uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, uint32_t read_len,
+                     uint32_t contig_len, uint32_t *start, uint32_t *end) {
+    uint32_t flank = read_len / 2 + 30;
+    uint32_t head = max32(flank, qs);
+    uint32_t tail = max32(flank, sat_sub32(read_len, qe));
+    uint32_t st = min32(sat_sub32(rs, head), contig_len);
+    uint32_t en = min32(sat_add32(re, tail), contig_len);
+    en = min32(en, sat_add32(sat_add32(rs, read_len), flank));
+    uint32_t f2 = (uint32_t)((uint64_t)flank * 2 > 0xFFFFFFFFull ? 0xFFFFFFFFu : flank * 2);
+    en = min32(en, sat_add32(sat_add32(st, read_len), f2));
+    *start = st;
+    *end = max32(en, st);
+}
+
+typedef struct {
+    int32_t status; /* 1 EA lane, 2 banded DP, 0 none, -1 contig out of range */
+    int32_t strand;
+    uint32_t pos;
+    int32_t score;
+    int32_t dist;   /* EA-Myers distance (EA lane) */
+    uint32_t wstart, wlen;
+} espo_unspliced_rec;
+
+/* EditDir::exempt: extend.rs:71-93 (uppercase bytes only). */
+static int dir_exempt(int dir, uint8_t ref_b, uint8_t read_b) {
+    int ed = exempt_dir(ref_b, read_b); /* 1 plus (A/G), 0 minus (T/C), -1 none */
+    return (dir == 1 && ed == 1) || (dir == 2 && ed == 0);
+}
+
+/* extend_unspliced_chain after the fast lane (align.rs:1770-1919): the
+ * EA-Myers window lane for reads of 1..256 bases (best start within
+ * max(m/3, 10), per-base mismatch recount equal to the distance, no N, then
+ * the Kadane-optimal soft clips when the kept segment covers half the read),
+ * else, when allow_dp, extend_hint over the same window at the chain's
+ * diagonal. */
+int espo_unspliced(const espo_ref *R, const uint8_t *read, size_t m, uint32_t contig, int minus,
+                   uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, int allow_dp,
+                   const espo_params *p, espo_unspliced_rec *rec, uint32_t *cigar, size_t cap,
+                   size_t *n_ops) {
+    memset(rec, 0, sizeof(*rec));
+    *n_ops = 0;
+    rec->strand = minus ? 1 : 0;
+    if (contig >= R->ncontig) {
+        rec->status = -1;
+        return 0;
+    }
+    uint32_t read_len = (uint32_t)m;
+    uint32_t ws, we;
+    espo_ext_window(rs, re, qs, qe, read_len, R->len[contig], &ws, &we);
+    size_t n = we - ws;
+    rec->wstart = ws;
+    rec->wlen = (uint32_t)n;
+    uint8_t *win = (uint8_t *)malloc(n ? n : 1), *q = (uint8_t *)malloc(m ? m : 1);
+    espo_decode(R, contig, ws, we, 0, win);
+    if (minus) espo_revcomp(read, m, q);
+    else if (m) memcpy(q, read, m);
+    int rc = 0;
+    if (read_len > 0 && read_len <= 256 && n >= m) {
+        int32_t threshold = (int32_t)read_len / 3;
+        if (threshold < 10) threshold = 10;
+        int32_t d;
+        size_t start;
+        espo_myers_best_start(q, m, win, n, &d, &start);
+        int usable = d <= threshold && start + m <= n;
+        if (usable) {
+            int32_t mm = 0;
+            int has_n = 0;
+            for (size_t k = 0; k < m; k++) {
+                uint8_t qb = q[k], rb = win[start + k];
+                if (qb == 'N' || rb == 'N') { has_n = 1; break; }
+                if (qb != rb && exempt_dir(rb, qb) < 0) mm++;
+            }
+            if (has_n || mm != d) usable = 0;
+        }
+        if (usable) {
+            int32_t best_sum = 0, cur_sum = 0;
+            size_t bi = 0, bj = 0, cur_i = 0;
+            for (size_t k = 0; k < m; k++) {
+                uint8_t qb = q[k], rb = win[start + k];
+                int32_t sc = qb == rb ? p->match_score : (dir_exempt(p->edit_dir, rb, qb) ? 0 : p->mismatch);
+                if (cur_sum <= 0) { cur_sum = sc; cur_i = k; }
+                else cur_sum += sc;
+                if (cur_sum > best_sum) { best_sum = cur_sum; bi = cur_i; bj = k + 1; }
+            }
+            uint32_t seg = (uint32_t)(bj - bi);
+            if (best_sum > 0 && seg * 2 >= read_len) {
+                size_t k = 0;
+                if (bi > 0 && k < cap) cigar[k++] = ((uint32_t)bi << 4) | OP_S;
+                if (k < cap) cigar[k++] = (seg << 4) | OP_M;
+                if (m - bj > 0 && k < cap) cigar[k++] = ((uint32_t)(m - bj) << 4) | OP_S;
+                *n_ops = k;
+                rec->status = 1;
+                rec->pos = ws + (uint32_t)(start + bi);
+                rec->score = best_sum;
+                rec->dist = d;
+                free(win);
+                free(q);
+                return 0;
+            }
+        }
+    }
+    if (allow_dp) {
+        espo_ext e;
+        int32_t esc;
+        size_t nops = 0;
+        int64_t hint = (int64_t)(rs - ws) - (int64_t)qs;
+        rc = espo_extend_hint(q, m, win, n, p, hint, &e, cigar, cap, &nops, &esc);
+        if (e.score > 0) {
+            rec->status = 2;
+            rec->pos = ws + e.ref_start;
+            rec->score = e.score;
+            *n_ops = nops;
+        }
+    }
+    free(win);
+    free(q);
+    return rc;
 }
 
 /* ------------------------------------------------- independent checkers */

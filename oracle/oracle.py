@@ -72,6 +72,11 @@ def lib():
                                        P(CParams), vp, u32p, sz, P(sz)]
         L.espo_spliced.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, u32p, sz, ctypes.c_int64,
                                    P(CParams), P(CExt), u32p, sz, P(sz), u32p, sz, vp]
+        L.espo_ext_window.argtypes = [ctypes.c_uint32] * 6 + [u32p, u32p]
+        L.espo_unspliced.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_int, P(CParams), vp, u32p, sz,
+                                     P(sz)]
         L.espo_mate_batch.argtypes = [P(CRef), sz, vp, vp, vp, vp, vp, P(CParams), vp, vp, vp, vp,
                                       ctypes.c_int]
         L.espo_spliced_batch.argtypes = [P(CRef), sz, vp, vp, vp, vp, vp, vp, vp, P(CParams), vp,
@@ -357,3 +362,47 @@ def spliced_batch(oref: OracleRef, req, params: Optional[ExtendParams] = None, n
     e = ext[:n].astype(np.int64)
     e[:, 3] = ext[:n, 3].view(np.int32)
     return rec[:n], e, cig, off, cig_n[:n], st, soff
+
+
+UNSPLICED_REC_DTYPE = np.dtype([("status", "<i4"), ("strand", "<i4"), ("pos", "<u4"),
+                                ("score", "<i4"), ("dist", "<i4"), ("wstart", "<u4"),
+                                ("wlen", "<u4")])
+
+
+def ext_window(ref_span, read_span, read_len: int, contig_len: int):
+    """align.rs:33 continuous_extension_window."""
+    a, b = ctypes.c_uint32(), ctypes.c_uint32()
+    lib().espo_ext_window(ref_span[0], ref_span[1], read_span[0], read_span[1], read_len,
+                          contig_len, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def unspliced(oref: OracleRef, read: bytes, contig: int, minus: bool, ref_span, read_span,
+              allow_dp: bool, params: Optional[ExtendParams] = None):
+    """extend_unspliced_chain after the fast lane (align.rs:1770-1919): (record, ops)."""
+    r, m = _buf(read)
+    rec = np.zeros(1, dtype=UNSPLICED_REC_DTYPE)
+    cap = cigar_capacity(m)
+    cig = (ctypes.c_uint32 * cap)()
+    n = ctypes.c_size_t()
+    rc = lib().espo_unspliced(ctypes.byref(oref.c), r, m, contig, int(minus), ref_span[0],
+                              ref_span[1], read_span[0], read_span[1], int(allow_dp),
+                              ctypes.byref(_params(params)), rec.ctypes.data, cig, cap,
+                              ctypes.byref(n))
+    assert rc == 0, rc
+    return rec[0], cigar_from_words(cig[: n.value])
+
+
+def unspliced_batch(oref: OracleRef, req, params: Optional[ExtendParams] = None):
+    """unspliced() over esperanto_amd.unspliced.ChainRequests: (records, cigars)."""
+    recs, cigs = [], []
+    for k in range(req.npairs):
+        a = int(req.read_off[k])
+        read = req.reads[a:a + int(req.read_len[k])].tobytes()
+        rec, cig = unspliced(oref, read, int(req.contig[k]), bool(req.minus[k]),
+                             (int(req.ref_start[k]), int(req.ref_end[k])),
+                             (int(req.read_start[k]), int(req.read_end[k])),
+                             bool(req.allow_dp[k]), params)
+        recs.append(rec)
+        cigs.append(cig)
+    return recs, cigs

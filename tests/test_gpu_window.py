@@ -288,3 +288,96 @@ def test_spliced_known_answer_and_workload(buf):
     _compare_mate(mt, mb.raw_results(), oracle.mate_batch(o, mt, nthreads=8))
     mb.free()
     dref.close()
+
+
+def _chain_cases(rng, seqs, n: int):
+    reads, contigs, minus, rsp, qsp, dp = [], [], [], [], [], []
+    for _ in range(n):
+        c = int(rng.integers(0, len(seqs)))
+        seq = seqs[c]
+        L = len(seq)
+        m = int(rng.choice([int(rng.integers(1, 257)), int(rng.integers(257, 400)), 0]
+                           if rng.integers(0, 10) == 0 else [int(rng.integers(20, 257))]))
+        m = min(m, max(L - 1, 0))
+        P = int(rng.integers(0, max(1, L - m)))
+        r = bytearray(seq[P:P + m])
+        for _ in range(int(rng.integers(0, 5))):
+            if m:
+                r[int(rng.integers(0, m))] = b"ACGTN"[int(rng.integers(0, 5))]
+        if m > 10 and rng.integers(0, 5) == 0:
+            q = int(rng.integers(1, m - 1))
+            r = r[:q] + (b"T" * int(rng.integers(1, 4)) if rng.integers(0, 2) else b"") + r[q + 2:]
+        r = bytes(r)
+        mi = bool(rng.integers(0, 2))
+        if mi:
+            r = revcomp(r)
+        qs = int(rng.integers(0, max(1, len(r) - 15))) if len(r) > 15 else 0
+        qe = min(len(r), qs + int(rng.integers(15, 80)))
+        rs = max(0, P + qs + int(rng.integers(-3, 4)))
+        reads.append(r)
+        contigs.append(c)
+        minus.append(int(mi))
+        rsp.append((rs, rs + (qe - qs)))
+        qsp.append((qs, qe))
+        dp.append(int(rng.integers(0, 4) != 0))
+    return reads, contigs, minus, rsp, qsp, dp
+
+
+def test_unspliced_matches_oracle(buf):
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.unspliced import ChainRequests, DeviceChainBatch
+    ref = _ref_with_runs(51, [30000, 2000, 600])
+    o = oracle.OracleRef(ref)
+    seqs = [oracle.decode(o, k, 0, int(L)) for k, L in enumerate(ref.lens)]
+    rng = np.random.default_rng(10)
+    reads, contigs, minus, rsp, qsp, dp = _chain_cases(rng, seqs, 2500)
+    contigs[7] = 9  # an invalid contig index
+    req = ChainRequests.from_lists(reads, contigs, minus, rsp, qsp, dp)
+    dref = DeviceReference(ref, buf)
+    for params in (ExtendParams(), ExtendParams(edit_dir=2)):
+        db = DeviceChainBatch(req, dref, buf)
+        say(f"unspliced {req.npairs} chains params={params}")
+        db.launch(params)
+        db.sync()
+        rec, cig, off, cig_n = db.raw_results()
+        crec, ccig = oracle.unspliced_batch(o, req, params)
+        lanes = [0, 0, 0]
+        for k in range(req.npairs):
+            assert rec[k].tolist() == crec[k].tolist(), (k, rec[k], crec[k])
+            words = cig[int(off[k]):int(off[k]) + int(cig_n[k])].tolist()
+            assert words == [c.word for c in ccig[k]], k
+            lanes[max(int(rec[k]["status"]), 0)] += 1
+        say(f"none/EA/DP = {lanes}")
+        assert lanes[1] > 200 and lanes[2] > 50
+        db.free()
+    dref.close()
+
+
+def test_unspliced_reference_known_answers(buf):
+    """align.rs:2899-2972 on the device (see tests/test_oracle_window.py)."""
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.unspliced import ChainRequests, unspliced_batch
+    from test_oracle_window import _body, _sparse_chain
+    for n in (150, 300):
+        body = _body(n)
+        for left, right in ((1000, 1000), (0, 1000), (1000, 0)):
+            dref = DeviceReference(Reference.from_sequences([b"N" * left + body + b"N" * right]), buf)
+            reads, minus, rsp, qsp, dp = [], [], [], [], []
+            for near_head in (True, False):
+                for mi in (False, True):
+                    for allow in (1, 0):
+                        a, b = _sparse_chain(n, near_head, left)
+                        reads.append(revcomp(body) if mi else body)
+                        minus.append(int(mi))
+                        rsp.append(a)
+                        qsp.append(b)
+                        dp.append(allow)
+            req = ChainRequests.from_lists(reads, [0] * len(reads), minus, rsp, qsp, dp)
+            out = unspliced_batch(dref, req)
+            for p, allow in zip(out, dp):
+                if n > 256 and not allow:
+                    assert p is None
+                else:
+                    assert p is not None and p.pos == left and p.cigar == [Match(n)]
+                    assert p.lane == (1 if n <= 256 else 2)
+            dref.close()

@@ -175,3 +175,100 @@ def test_workload_truth_consistency():
         assert r2 == oracle.decode(o, 0, p2, p2 + 150, rc=True)
         rec, cig = oracle.mate_window(o, r2, 0, int(mt.mate_pos[k]))
         assert rec["placed"] == 1 and rec["pos"] == p2 and rec["strand"] == 1
+
+
+def _body(n: int) -> bytes:
+    """align.rs:2776-2786 sparse_window_body: xorshift64 from 0x75bcd15 over ACGT."""
+    s, out, M = 0x75BCD15, bytearray(), (1 << 64) - 1
+    for _ in range(n):
+        s ^= (s << 13) & M
+        s ^= s >> 7
+        s ^= (s << 17) & M
+        out.append(b"ACGT"[s & 3])
+    return bytes(out)
+
+
+def _sparse_chain(n: int, near_head: bool, pos: int, k: int = 15):
+    """align.rs:2877-2897 sparse_window_chain as (ref_span, read_span) with k = 15."""
+    qs = [2, 12, 27] if near_head else [n - 42, n - 32, n - 17]
+    return (pos + qs[0], pos + qs[-1] + k), (qs[0], qs[-1] + k)
+
+
+def test_continuous_extension_window_reference_properties():
+    """align.rs:3116-3173 sparse_window_geometry_preserves_old_windows_and_bounds."""
+    from esperanto_amd.unspliced import continuous_extension_window
+    for n in (60, 150, 300, 1024):
+        flank = n // 2 + 30
+        for qs in range(0, n - 15 + 1, 7):
+            for qe in range(qs + 15, n + 1, 11):
+                for pos in (0, 1, 1000):
+                    for extra in (0, 1000):
+                        rsp, qsp = (pos + qs, pos + qe - 15 + 15), (qs, qe)
+                        length = pos + n + extra
+                        start, end = oracle.ext_window(rsp, qsp, n, length)
+                        assert (start, end) == continuous_extension_window(rsp, qsp, n, length)
+                        assert start <= pos and end >= pos + n
+                        assert start <= end <= length
+                        assert end - start <= n + 2 * flank
+                        if qs <= flank and n - qe <= flank:
+                            assert (start, end) == (max(rsp[0] - flank, 0),
+                                                    min(rsp[1] + flank, length, rsp[0] + n + flank))
+                        disp = (rsp[0], rsp[1] + 50000)  # displaced last anchor
+                        a, b = oracle.ext_window(disp, qsp, n, length + 50000)
+                        assert a <= b <= length + 50000 and b - a <= n + 2 * flank
+    rsp, qsp = _sparse_chain(150, True, 0xFFFFFFFF - 100)
+    rsp = (rsp[0], min(rsp[1], 0xFFFFFFFF))
+    a, b = oracle.ext_window(rsp, qsp, 150, 0xFFFFFFFF)
+    assert a <= b and b - a <= 360
+
+
+def test_unspliced_reference_known_answers():
+    """align.rs:2899-2972: the sparse-window chain keeps the whole ordinary body
+    (EA lane for n <= 256, banded DP above), respects contig edges, and the EA
+    lane places without touching the DP budget."""
+    for n in (150, 300):
+        body = _body(n)
+        for left, right in ((1000, 1000), (0, 1000), (1000, 0)):
+            o = oracle.OracleRef(Reference.from_sequences([b"N" * left + body + b"N" * right]))
+            for near_head in (True, False):
+                for minus in (False, True):
+                    seq = revcomp(body) if minus else body
+                    rsp, qsp = _sparse_chain(n, near_head, left)
+                    rec, cig = oracle.unspliced(o, seq, 0, minus, rsp, qsp, True)
+                    assert rec["status"] == (1 if n <= 256 else 2)
+                    assert rec["pos"] == left and cig == [Match(n)]
+                    rec, cig = oracle.unspliced(o, seq, 0, minus, rsp, qsp, False)
+                    if n <= 256:
+                        assert rec["status"] == 1 and cig == [Match(n)]
+                    else:
+                        assert rec["status"] == 0 and cig == []
+
+
+def test_unspliced_ea_lane_soft_clips_and_fallback():
+    """Planted reads: ends mismatching on purpose are soft-clipped by Kadane; an
+    indel makes the recount differ from the distance and sends the read to the
+    banded DP (or to nothing without budget); an N on the diagonal does too."""
+    rng = np.random.default_rng(4)
+    contig = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, 20000)].tobytes()
+    o = oracle.OracleRef(Reference.from_sequences([contig]))
+    comp = {ord("A"): "C", ord("C"): "A", ord("G"): "T", ord("T"): "G"}
+    for _ in range(40):
+        m = int(rng.integers(60, 200))
+        P = int(rng.integers(500, 19000 - m))
+        read = bytearray(contig[P:P + m])
+        for q in (2, 3):  # 2 + 2 - 4 - 4 < 0: Kadane clips the first four bases
+            read[q] = ord(comp[read[q]])
+        rsp, qsp = (P + 20, P + 60), (20, 60)
+        rec, cig = oracle.unspliced(o, bytes(read), 0, False, rsp, qsp, True)
+        ws, we = oracle.ext_window(rsp, qsp, m, len(contig))
+        d, start = oracle.infix_best_start(bytes(read), contig[ws:we])
+        if ws + start == P:  # the substitution reading is the rightmost optimal start
+            assert rec["status"] == 1 and rec["dist"] == 2 == d
+            assert rec["pos"] == P + 4 and cig == [SoftClip(4), Match(m - 4)]
+        else:  # another optimal start: the recount disagrees and the DP decides
+            assert rec["status"] == 2
+        ind = bytes(read[:m // 2]) + b"T" + bytes(read[m // 2:])  # an insertion
+        rec, _ = oracle.unspliced(o, ind, 0, False, rsp, qsp, True)
+        assert rec["status"] in (0, 2)
+        rec, _ = oracle.unspliced(o, ind, 0, False, rsp, qsp, False)
+        assert rec["status"] in (0, 1)
