@@ -26,7 +26,7 @@ EXPORTS = [
     "esp_ref_device_bytes", "esp_gather_batch", "esp_mate_rescue_batch", "esp_spliced_batch",
     "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
-    "esp_unspliced_batch",
+    "esp_unspliced_batch", "esp_window_myers_batch",
 ]
 
 
@@ -133,6 +133,8 @@ def lib():
     L.esp_ref_device_bytes.argtypes = [vp, P(sz)]
     L.esp_gather_batch.argtypes = [vp, vp, vp, sz, vp, vp]
     L.esp_mate_rescue_batch.argtypes = [vp, vp, P(CMateBatch), P(CParams), vp, vp, vp, vp]
+    L.esp_window_myers_batch.argtypes = [vp, vp, vp, sz, vp, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_int, vp, vp, vp]
     L.esp_unspliced_batch.argtypes = [vp, vp, P(CChainBatch), P(CParams), vp, vp, vp, vp]
     L.esp_spliced_batch.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp]
     _lib = L

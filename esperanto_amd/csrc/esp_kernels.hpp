@@ -240,7 +240,21 @@ struct UnspArgs {
     uint32_t *cig_n;
 };
 
+struct WinMyersArgs {
+    uint32_t njobs;
+    RefView R;
+    const esp_window_job *jobs;
+    uint32_t max_pattern;
+    uint32_t max_text;
+    uint64_t tstride;
+    uint8_t *tarena;
+    uint64_t *p_off, *t_off;
+    uint32_t *p_len, *t_len;
+    int32_t *status;
+};
+
 hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
+hipError_t launch_winmyers_prep(const WinMyersArgs &a, hipStream_t s);
 hipError_t launch_unsp_prep(const UnspArgs &a, hipStream_t s);
 hipError_t launch_unsp_ea(const UnspArgs &a, hipStream_t s);
 hipError_t launch_unsp_finish(const UnspArgs &a, hipStream_t s);

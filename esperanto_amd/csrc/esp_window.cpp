@@ -251,6 +251,61 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
     return ESP_OK;
 }
 
+int esp_window_myers_batch(esp_ctx *c, const esp_ref *r, const esp_window_job *jobs,
+                           size_t njobs, const uint8_t *reads, uint32_t max_pattern,
+                           uint32_t max_text, int mode, int32_t *dist, uint64_t *pos,
+                           int32_t *status) {
+    if (!c) return ESP_E_INVAL;
+    if (!r || (njobs && (!jobs || !reads || !dist || (mode != 2 && !pos))))
+        return fail(c, ESP_E_INVAL, "NULL argument");
+    if (mode < 0 || mode > 2) return fail(c, ESP_E_INVAL, "mode not in {0,1,2}");
+    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
+    if (njobs == 0) return ESP_OK;
+    if (njobs > 0xFFFFFF00ull || max_text > kMaxWin) return fail(c, ESP_E_INVAL, "batch too large");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = njobs;
+    espk::WinMyersArgs a{};
+    a.njobs = (uint32_t)n;
+    a.R = r->view();
+    a.jobs = jobs;
+    a.max_pattern = max_pattern;
+    a.max_text = max_text;
+    a.tstride = align_up((uint64_t)max_text + 16, 16);
+    HIP_TRY(c, c->warena.ensure(n * a.tstride + 64));
+    HIP_TRY(c, c->wdesc.ensure(n * (2 * 8 + 2 * 4) + 8 * 256));
+    a.tarena = (uint8_t *)c->warena.p;
+    Carve cv{(uint8_t *)c->wdesc.p};
+    a.p_off = cv.take<uint64_t>(n);
+    a.t_off = cv.take<uint64_t>(n);
+    a.p_len = cv.take<uint32_t>(n);
+    a.t_len = cv.take<uint32_t>(n);
+    a.status = status;
+    HIP_TRY(c, c->counter.ensure(256));
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_winmyers_prep(a, c->stream));
+    tim_mark(c, 5, 0);
+    const int words = (int)((std::min<uint32_t>(std::max<uint32_t>(max_pattern, 1), 256) + 63) / 64);
+    uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(words), (n + 63) / 64);
+    espk::MyersArgs m{};
+    m.npairs = (uint32_t)n;
+    m.reads = reads;
+    m.read_off = a.p_off;
+    m.read_len = a.p_len;
+    m.texts = a.tarena;
+    m.text_off = a.t_off;
+    m.text_len = a.t_len;
+    m.dist = dist;
+    m.end = mode == 0 ? pos : nullptr;
+    m.start = mode == 1 ? pos : nullptr;
+    m.counter = (uint32_t *)c->counter.p;
+    HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
+    HIP_TRY(c, espk::launch_myers(m, words, (int)std::max<uint64_t>(blocks, 1), c->stream));
+    tim_mark(c, 3, 0);
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
 int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,
                         const esp_params *p, esp_unspliced_rec *out, uint32_t *cigar,
                         const uint64_t *cig_off, uint32_t *cig_n) {

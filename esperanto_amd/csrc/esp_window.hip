@@ -290,6 +290,28 @@ __global__ __launch_bounds__(WIN_BLOCK) void k_splice_finish(SpliceArgs a) {
     a.rec[k] = r;
 }
 
+// Window EA-Myers jobs: checks, the text decoded into the job's slot, and
+// the descriptors of the Myers launch (an invalid job gets a 0-length
+// pattern, which the Myers kernel answers with (0, 0)).
+__global__ __launch_bounds__(WIN_BLOCK) void k_winmyers_prep(WinMyersArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (WIN_BLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= a.njobs) return;
+    const esp_window_job j = a.jobs[k];
+    const bool ok = j.contig < a.R.ncontig && j.tstart <= j.tend && j.tend <= a.R.len[j.contig] &&
+                    j.qstart <= j.qend && j.qend - j.qstart <= a.max_pattern &&
+                    j.tend - j.tstart <= a.max_text;
+    const uint64_t to = (uint64_t)k * a.tstride;
+    if (lane == 0) {
+        if (a.status) a.status[k] = ok ? 0 : (int32_t)ESP_E_INVAL;
+        a.p_off[k] = j.read_off + j.qstart;
+        a.p_len[k] = ok ? j.qend - j.qstart : 0u;
+        a.t_off[k] = to;
+        a.t_len[k] = ok ? j.tend - j.tstart : 0u;
+    }
+    if (ok) wave_decode(a.R, j.contig, j.tstart, j.tend, false, a.tarena + to, lane);
+}
+
 // continuous_extension_window (align.rs:33-55), u32 saturating arithmetic.
 __device__ void ext_window(uint32_t rs, uint32_t re, uint32_t qs, uint32_t qe, uint32_t read_len,
                            uint32_t clen, uint32_t &start, uint32_t &end) {
@@ -464,6 +486,12 @@ hipError_t launch_mate_select(const MateArgs &a, hipStream_t s) {
 
 hipError_t launch_mate_finish(const MateArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_mate_finish, dim3(lane_blocks(a.npairs)), dim3(WIN_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_winmyers_prep(const WinMyersArgs &a, hipStream_t s) {
+    if (a.njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_winmyers_prep, dim3(wave_blocks(a.njobs)), dim3(WIN_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

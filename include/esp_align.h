@@ -354,6 +354,28 @@ int esp_unspliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_chain_batch 
                         const esp_params *p, esp_unspliced_rec *out, uint32_t *cigar,
                         const uint64_t *cig_off, uint32_t *cig_n);
 
+/* EA-Myers of read sub-patterns against reference windows: the breakpoint
+ * refinements of the intron-chain fast path (refine_donor_end: infix_best_end,
+ * refine_acceptor_start: infix_best_start, intron_chain.rs:285-342) and its
+ * split-read verification (verify_split_read: long::infix, :344-373).  Job k:
+ * pattern reads[read_off + qstart, read_off + qend), text contig[tstart, tend).
+ * mode 0: dist + best end, 1: dist + best start, 2: dist only (pos untouched).
+ * status[k] = ESP_E_INVAL for a job outside its contig or the host bounds
+ * max_pattern / max_text (its dist and pos are then 0).  Device pointers. */
+typedef struct {
+    uint64_t read_off;
+    uint32_t qstart;
+    uint32_t qend;
+    uint32_t contig;
+    uint32_t tstart;
+    uint32_t tend;
+    uint32_t pad;
+} esp_window_job;
+int esp_window_myers_batch(esp_ctx *ctx, const esp_ref *ref, const esp_window_job *jobs,
+                           size_t njobs, const uint8_t *reads, uint32_t max_pattern,
+                           uint32_t max_text, int mode, int32_t *dist, uint64_t *pos,
+                           int32_t *status);
+
 /* ---- serving loop ---------------------------------------------------------- */
 
 /* A request loop over the resident reference with continuous batching: the

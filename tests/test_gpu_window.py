@@ -381,3 +381,69 @@ def test_unspliced_reference_known_answers(buf):
                     assert p is not None and p.pos == left and p.cigar == [Match(n)]
                     assert p.lane == (1 if n <= 256 else 2)
             dref.close()
+
+
+def test_window_myers_and_intron_refinements(buf):
+    from esperanto_amd.intron_chain import (WINDOW_JOB_DTYPE, donor_plan, acceptor_plan,
+                                            refine_acceptor_start, refine_donor_end,
+                                            verify_split_read, window_myers)
+    from esperanto_amd.reference import DeviceReference
+    from test_oracle_window import _run_plan_on_oracle
+    ref = _ref_with_runs(61, [40000, 800])
+    o = oracle.OracleRef(ref)
+    seqs = [oracle.decode(o, k, 0, int(L)) for k, L in enumerate(ref.lens)]
+    dref = DeviceReference(ref, buf)
+    rng = np.random.default_rng(12)
+    reads = []
+    for _ in range(300):
+        c = 0
+        m = int(rng.integers(20, 300))
+        P = int(rng.integers(0, len(seqs[c]) - m))
+        r = bytearray(seqs[c][P:P + m])
+        for _ in range(int(rng.integers(0, 6))):
+            r[int(rng.integers(0, m))] = b"ACGTNacgt"[int(rng.integers(0, 9))]
+        reads.append(bytes(r))
+    jobs = []
+    for k, r in enumerate(reads):
+        for _ in range(3):
+            qs = int(rng.integers(0, len(r)))
+            qe = int(rng.integers(qs, min(len(r), qs + 260) + 1))
+            c = int(rng.integers(0, 2))
+            L = len(seqs[c])
+            ts = int(rng.integers(0, L))
+            te = min(L, ts + int(rng.integers(0, 400)))
+            jobs.append((k, qs, qe, c, ts, te, 0))
+    jobs.append((0, 0, 10, 5, 0, 10, 0))       # no such contig
+    jobs.append((0, 0, 10, 1, 700, 900, 0))    # past the contig end
+    J = np.array(jobs, dtype=WINDOW_JOB_DTYPE)
+    for mode in (0, 1, 2):
+        say(f"window myers {len(jobs)} jobs mode {mode}")
+        dist, pos, st = window_myers(dref, reads, J, mode)
+        exp = _run_plan_on_oracle(o, reads, J[:-2], mode)
+        for j in range(len(jobs) - 2):
+            assert st[j] == 0
+            assert dist[j] == exp[j][0], (mode, j)
+            if mode != 2:
+                assert int(pos[j]) == exp[j][1], (mode, j)
+        assert st[-1] == 1 and st[-2] == 1
+    # the refinement functions: searches on the device, same answers as the plan on the oracle
+    n = len(reads)
+    ctg = [0] * n
+    joff = [int(rng.integers(0, len(r) + 1)) for r in reads]
+    hi = [int(rng.integers(0, 40000)) for _ in reads]
+    imp = [h - int(rng.integers(0, 300)) for h in hi]
+    got = refine_donor_end(dref, reads, ctg, joff, hi, imp)
+    out, jobs2, who, base = donor_plan(ref.lens, reads, ctg, joff, hi, imp)
+    res = _run_plan_on_oracle(o, reads, jobs2, 0)
+    for j, k in enumerate(who):
+        out[k] = base[j] + res[j][1]
+    assert got == out
+    got = refine_acceptor_start(dref, reads, ctg, joff, hi, imp)
+    out, jobs2, who, base = acceptor_plan(ref.lens, reads, ctg, joff, hi, imp)
+    res = _run_plan_on_oracle(o, reads, jobs2, 1)
+    for j, k in enumerate(who):
+        out[k] = base[j] + res[j][1]
+    assert got == out
+    got = verify_split_read(dref, reads, ctg, imp, [h + 200 for h in hi], hi, [h + 100 for h in hi])
+    assert len(got) == n and all(isinstance(x, tuple) for x in got)
+    dref.close()

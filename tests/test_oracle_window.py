@@ -272,3 +272,55 @@ def test_unspliced_ea_lane_soft_clips_and_fallback():
         assert rec["status"] in (0, 2)
         rec, _ = oracle.unspliced(o, ind, 0, False, rsp, qsp, False)
         assert rec["status"] in (0, 1)
+
+
+def _run_plan_on_oracle(o, reads, jobs, mode):
+    out = []
+    for j in jobs:
+        read = reads[int(j["read_off"])]
+        pat = read[int(j["qstart"]):int(j["qend"])]
+        text = oracle.decode(o, int(j["contig"]), int(j["tstart"]), int(j["tend"]))
+        out.append(oracle.infix_best_end(pat, text) if mode == 0 else
+                   oracle.infix_best_start(pat, text) if mode == 1 else (oracle.infix(pat, text), 0))
+    return out
+
+
+def test_intron_chain_refinements_on_planted_junctions():
+    """intron_chain.rs:285-373 geometry with the oracle's Myers: on a read
+    spanning a planted junction the donor search ends at the donor and the
+    acceptor search starts at the acceptor (unique random sequence), and the
+    split-read verification finds both halves at distance 0."""
+    from esperanto_amd.intron_chain import acceptor_plan, donor_plan, verify_plan
+    rng = np.random.default_rng(6)
+    contig = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, 50000)].tobytes()
+    ref = Reference.from_sequences([contig])
+    o = oracle.OracleRef(ref)
+    reads, s_, e_, u_ = [], [], [], []
+    for _ in range(50):
+        m = int(rng.integers(80, 200))
+        u = int(rng.integers(30, m - 30))
+        s = int(rng.integers(1000, 20000))
+        e = s + int(rng.integers(60, 20000))
+        reads.append(contig[s - u:s] + contig[e:e + m - u])
+        s_.append(s)
+        e_.append(e)
+        u_.append(u)
+    n = len(reads)
+    out, jobs, who, base = donor_plan(ref.lens, reads, [0] * n, u_, s_, [s - u for s, u in zip(s_, u_)])
+    assert len(who) == n
+    res = _run_plan_on_oracle(o, reads, jobs, 0)
+    assert [base[j] + res[j][1] for j in range(n)] == s_
+    out, jobs, who, base = acceptor_plan(ref.lens, reads, [0] * n, u_, e_, [e - u for e, u in zip(e_, u_)])
+    res = _run_plan_on_oracle(o, reads, jobs, 1)
+    assert [base[j] + res[j][1] for j in range(n)] == e_
+    jobs, who = verify_plan(ref.lens, reads, [0] * n, [s - u for s, u in zip(s_, u_)],
+                            [e + len(r) - u for r, e, u in zip(reads, e_, u_)], [x - 1 for x in s_], e_)
+    res = _run_plan_on_oracle(o, reads, jobs, 2)
+    for j in range(len(who)):
+        k = who[j]
+        mid = len(reads[k]) // 2
+        # a half that crosses the junction is split by it; the other matches exactly
+        if mid <= u_[k]:
+            assert res[2 * j][0] == 0
+        if mid >= u_[k]:
+            assert res[2 * j + 1][0] == 0
