@@ -30,42 +30,99 @@ namespace espk {
 
 namespace {
 
+// Bytes p[lo .. lo + 15] as four words in which byte t is p[lo + t], for
+// the bytes up to index hi (later ones are unspecified; with hi < lo all
+// are).  Every load is an aligned dword holding at least one byte of
+// p[0 .. max(hi, 0)] (the caller guarantees p[0] is readable), so none
+// leaves the sequence's pages and no load needs a branch; one round trip.
+__device__ __forceinline__ void load_span16(const uint8_t *p, int32_t lo, int32_t hi, uint32_t out[4]) {
+    const int32_t last = max(hi, 0);
+    const uintptr_t a0 = (uintptr_t)(p + min(lo, last));
+    const uintptr_t d0 = a0 & ~(uintptr_t)3, dl = (uintptr_t)(p + last) & ~(uintptr_t)3;
+    const uint32_t sh = (uint32_t)(a0 & 3u);
+    uint32_t w[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) w[k] = *(const uint32_t *)(d0 + 4 * k <= dl ? d0 + 4 * k : dl);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
+}
+
+__device__ __forceinline__ uint32_t byte16(const uint32_t w[4], int32_t t) {
+    const uint64_t lo = (uint64_t)w[1] << 32 | w[0], hi = (uint64_t)w[3] << 32 | w[2];
+    return (uint32_t)((t < 8 ? lo >> (8 * t) : hi >> (8 * (t - 8))) & 0xFFu);
+}
+
+// Ring byte of logical row c: read code of base c-1 in bits 5-7 (rows up to
+// m), 5 * window code below.
+__device__ __forceinline__ uint8_t ring_byte(uint32_t wbyte, uint32_t rbyte, bool has_read) {
+    const uint32_t rcode = has_read ? (uint32_t)read_code(rbyte) << 5 : 0u;
+    return (uint8_t)(rcode | (uint32_t)(5 * win_code(wbyte)));
+}
+
+// Logical rows c0 .. c0 + nrows - 1 (1-based, nrows <= 16 and uniform)
+// into the LDS ring from physical row ph on: row c is the byte (read code of
+// base c-1 << 5 when c <= m) | 5 * window code of s0[c-1], at physical row
+// 1 + (c-1) mod RING and again at RING + that row for the first W-1
+// physical rows, so that a DP row reads its W codes from consecutive
+// physical rows.  Rows past this lane's last row L get unspecified bytes
+// (the caller only writes slots no row reads any more).
+template <int W, int RING>
+__device__ __forceinline__ void ring_batch(uint8_t *codes, uint32_t lane, const uint8_t *s0,
+                                           const uint8_t *read, int32_t m, int32_t L, int32_t c0,
+                                           int32_t ph, int32_t nrows) {
+    uint32_t ww[4], rw[4];
+    load_span16(s0, c0 - 1, L - 1, ww);
+    load_span16(read, c0 - 1, m - 1, rw);
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        if (t < nrows) {
+            const uint8_t v = ring_byte(byte16(ww, t), byte16(rw, t), c0 + t <= m);
+            codes[(size_t)ph * 64 + lane] = v;
+            if (ph <= W - 1) codes[(size_t)(ph + RING) * 64 + lane] = v;
+            ph = ph == RING ? 1 : ph + 1;
+        }
+    }
+}
+
 // Traceback over the band trace layout (extend.rs:884-930) with value
 // tracking; runs are written in walk order to slot[cap-1], slot[cap-2], ...
+// A step is branch-free apart from the cache refills and the run writes:
+// lanes of a wave walk in lockstep whatever their moves.
 template <int W>
-__device__ TbResult traceback_band(const PairView &pv, const GParams &P, const uint8_t *codes,
-                                   uint32_t lane, const uint32_t *tr, int32_t cb,
-                                   int32_t best, int32_t bi, int32_t bj, uint32_t *slot,
-                                   uint32_t cap) {
+__device__ TbResult traceback_band(const PairView &pv, const GParams &P, const uint32_t *tr,
+                                   int32_t cb, int32_t best, int32_t bi, int32_t bj,
+                                   uint32_t *slot, uint32_t cap) {
     constexpr int WW = (W + 7) / 8;
     int32_t i = bi, j = bj, V = best;
-    uint32_t state = 0;
+    uint32_t state = 0;  // 0 H, 1 E (insertion run), 2 F (deletion run)
     uint32_t cur_op = 0xFF, cur_len = 0, nrev = 0, nm = 0, ni = 0, nd = 0;
-    auto emit = [&](uint32_t op) {
-        if (op == cur_op) {
-            cur_len++;
-        } else {
-            if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
-            cur_op = op;
-            cur_len = 1;
-        }
-    };
     // Column cache: the nibbles of band cell ck for rows ci0-31 .. ci0, row r
     // at bit 4*(ci0 - r) of the 128-bit pair (lo, hi).  Diagonal steps keep
-    // k, so one batch of 32 independent loads serves up to 32 steps.
+    // k, so one batch of 32 independent loads serves up to 32 steps; loads
+    // of rows above row 1 are clamped to row 1 (never used).
     int32_t ci0 = -1, ck = -1;
     uint64_t nlo = 0, nhi = 0;
+    // Read and window bytes of the diagonal steps: every 16 iterations (the
+    // same iteration for every lane still walking, so one memory round trip
+    // serves the wave) the bytes the next 16 steps can reach are loaded.
+    int32_t it = 0, rb = 0, wb = 0;
+    uint32_t rw[4] = {}, ww[4] = {};
     while (i > 0 && j > 0) {
         const int32_t k = j - (cb + i);
         if (k < 0 || k >= W) break;
+        if ((it++ & 15) == 0) {
+            rb = max(i - 16, 0);
+            wb = max(j - 16, 0);
+            load_span16(pv.read, rb, i - 1, rw);
+            load_span16(pv.win, wb, j - 1, ww);
+        }
         if (k != ck || i > ci0 || i <= ci0 - 32) {
             const int32_t t = k >> 3;
             const int32_t c = min(8, W - 8 * t);
             const int32_t sh = 4 * (c - 1 - (k & 7));
             uint32_t w[32];
 #pragma unroll
-            for (int q = 0; q < 32; q++)
-                w[q] = (i - q >= 1) ? tr[((size_t)(i - 1 - q) * WW + t) * 64] : 0u;
+            for (int q = 0; q < 32; q++) w[q] = tr[((size_t)max(i - 1 - q, 0) * WW + t) * 64];
             nlo = 0;
             nhi = 0;
 #pragma unroll
@@ -76,59 +133,31 @@ __device__ TbResult traceback_band(const PairView &pv, const GParams &P, const u
             ci0 = i;
             ck = k;
         }
+        if (state == 0 && V == 0) break;  // H == 0: the reference's H_START
         const int32_t d = ci0 - i;
         const uint32_t nib = (uint32_t)((d < 16 ? (nlo >> (4 * d)) : (nhi >> (4 * (d - 16)))) & 0xFu);
-        if (state == 0) {
-            if (V == 0) break;  // H == 0: the reference's H_START
-            if (!(nib & 8u)) {
-                const int32_t rc = codes[(size_t)i * 64 + lane] >> 5;
-                const int32_t fc = (codes[(size_t)(j - cb) * 64 + lane] & 31) / 5;
-                V -= lut_score(P, rc, fc);
-                emit(OP_M);
-                nm++;
-                i--;
-                j--;
-            } else if (!(nib & 4u)) {
-                emit(OP_I);
-                ni++;
-                if (nib & 2u) {
-                    state = 1;
-                    V -= P.ext;
-                } else {
-                    V -= P.oe;
-                }
-                i--;
-            } else {
-                emit(OP_D);
-                nd++;
-                if (nib & 1u) {
-                    state = 2;
-                    V -= P.ext;
-                } else {
-                    V -= P.oe;
-                }
-                j--;
-            }
-        } else if (state == 1) {
-            emit(OP_I);
-            ni++;
-            if (nib & 2u) {
-                V -= P.ext;
-            } else {
-                state = 0;
-                V -= P.oe;
-            }
-            i--;
+        // State 0 takes the diagonal when D == H, else E when E == H, else F;
+        // a gap run continues while its cell's extension bit is set.
+        const bool isM = state == 0 && !(nib & 8u);
+        const bool isI = state == 1 || (state == 0 && (nib & 12u) == 8u);
+        const bool isD = !isM && !isI;
+        const bool gext = isI ? (nib & 2u) != 0 : (isD && (nib & 1u) != 0);
+        const int32_t rc = read_code(byte16(rw, i - 1 - rb));
+        const int32_t fc = win_code(byte16(ww, j - 1 - wb));
+        V -= isM ? lut_score(P, rc, fc) : (gext ? P.ext : P.oe);
+        state = gext ? (isI ? 1u : 2u) : 0u;
+        const uint32_t op = isM ? OP_M : (isI ? OP_I : OP_D);
+        nm += isM;
+        ni += isI;
+        nd += isD;
+        i -= isM || isI;
+        j -= isM || isD;
+        if (op != cur_op) {
+            if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
+            cur_op = op;
+            cur_len = 1;
         } else {
-            emit(OP_D);
-            nd++;
-            if (nib & 1u) {
-                V -= P.ext;
-            } else {
-                state = 0;
-                V -= P.oe;
-            }
-            j--;
+            cur_len++;
         }
     }
     if (cur_len) slot[cap - 1 - nrev++] = (cur_len << 4) | cur_op;
@@ -142,11 +171,21 @@ __device__ TbResult traceback_band(const PairView &pv, const GParams &P, const u
     return r;
 }
 
+constexpr int BAND_PF = 4;  // window-code prefetch distance in cells
+
+// pk = (low 16 bits of e + x) << 16 | (pk & 0xFFFF), in one VALU op.
+__device__ __forceinline__ void pack_hi(uint32_t &pk, int32_t e, int32_t x) {
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:DWORD"
+        : "+v"(pk)
+        : "v"(e), "v"(x));
+}
+
 template <int HALF>
 __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
     constexpr int W = 2 * HALF + 1;
     constexpr int WW = (W + 7) / 8;
-    extern __shared__ uint8_t codes[];  // [span][64]: (read code << 5) | 5 * window code
+    constexpr int RING = band_ring(HALF);
+    extern __shared__ uint8_t codes[];  // [RING + W][64]: (read code << 5) | 5 * window code
     uint32_t *lut_lds = (uint32_t *)codes;  // row 0 (never a code row): the 5 row LUTs
     const uint32_t lane = threadIdx.x;
     const uint32_t slot = blockIdx.x * BAND_BLOCK + lane;
@@ -154,6 +193,8 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
     const int32_t oe = P.oe, ext = P.ext;
     const int32_t noe = -oe;
     const int32_t negx = NEG + ext;
+    const int32_t zr = a.zero;
+    const int32_t extv = ext;
     const uint32_t first = a.range[0], total = a.range[1];
     const uint32_t nunits = (total + 63) / 64;
     for (;;) {
@@ -175,59 +216,23 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
         pv.half = HALF;
         const int32_t m = pv.m, n = pv.n;
         const int32_t cb = pv.off - HALF;
-        // Window codes of columns cb + 1 .. cb + m + W - 1 (interior: inside
-        // [1, n]) into LDS rows 1 .. m + W - 1.  Aligned dwords carry the
-        // body, eight loads in flight per round; head and tail go bytewise,
-        // so no load leaves the window.
-        {
-            const int32_t L = m > 0 ? m + W - 1 : 0;
-            const uint8_t *s0 = pv.win + cb;
-            const int32_t head = min(L, (int32_t)((4u - ((uint32_t)(uintptr_t)s0 & 3u)) & 3u));
-            for (int32_t q = 0; q < head; q++)
-                codes[(size_t)(1 + q) * 64 + lane] = (uint8_t)(5 * win_code(s0[q]));
-            const uint32_t *d = (const uint32_t *)(s0 + head);
-            const int32_t nd = (L - head) >> 2;
-            for (int32_t q = 0; q < nd; q += 8) {
-                uint32_t w[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) w[u] = q + u < nd ? d[q + u] : 0u;
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    if (q + u < nd) {
-                        const int32_t r = 1 + head + 4 * (q + u);
-#pragma unroll
-                        for (int b = 0; b < 4; b++)
-                            codes[(size_t)(r + b) * 64 + lane] =
-                                (uint8_t)(5 * win_code((w[u] >> (8 * b)) & 0xFFu));
-                    }
-                }
-            }
-            for (int32_t q = head + 4 * nd; q < L; q++)
-                codes[(size_t)(1 + q) * 64 + lane] = (uint8_t)(5 * win_code(s0[q]));
-            // Read code of row r (read base r-1) in bits 5-7 of LDS row r: the
-            // row loop then needs no global load (a vector-memory wait there
-            // would also wait for every trace store in flight).
-            for (int32_t q = 0; q < m; q += 8) {
-                uint32_t rb[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) rb[u] = q + u < m ? pv.read[q + u] : 0u;
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    if (q + u < m) {
-                        uint8_t *cell = codes + (size_t)(1 + q + u) * 64 + lane;
-                        *cell = (uint8_t)(*cell | (read_code(rb[u]) << 5));
-                    }
-                }
-            }
-        }
+        // Window codes of columns cb + 1 .. cb + L (interior: inside [1, n]),
+        // logical rows 1 .. L, pass through an LDS ring of RING rows; the
+        // first RING now, the rest as the row loop needs them.
+        const int32_t L = m > 0 ? m + W - 1 : 0;
+        const uint8_t *s0 = pv.win + cb;
+        for (int32_t c0 = 1; c0 <= RING; c0 += 16)
+            ring_batch<W, RING>(codes, lane, s0, pv.read, m, L, c0, c0, min(16, RING - c0 + 1));
         if (lane < 5) lut_lds[lane] = a.lut[lane];
         __syncthreads();  // one wave per block: orders the LUT writes before any lane reads them
 
         // Previous row, band index k: U = H + open_ext and B = E + ext.  Wide
-        // bands keep both as int16 halves of one VGPR (U low, B high): the
-        // reads fold into SDWA operands and one v_perm repacks, which halves
-        // the register arrays (the planner admits only pairs whose values fit
-        // int16; the rest run the int32 strip kernel).
+        // bands keep both as int16 halves of one VGPR (U low, B high), which
+        // halves the register arrays (the planner admits only pairs whose
+        // values fit int16; the rest run the int32 strip kernel): the reads
+        // fold into SDWA operands, and the cell after k, once its F has read
+        // U_k as a full word, writes E_k + ext into the high half in place
+        // (an SDWA add with WORD_1 destination), so packing costs no op.
         constexpr bool PACKED = HALF >= 48;
         constexpr int32_t NEG16 = -20000;  // packed "E = NEG": below every reachable U
         int32_t Ur[PACKED ? 1 : W], Bx[PACKED ? 1 : W];
@@ -249,13 +254,36 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
         // Per-read-code row LUT: 5-bit signed scores for window codes 0..5
         // (A C G T, -, other) at bit 5*code; the window codes in LDS are
         // stored pre-multiplied by 5.
-        for (int32_t i = 1; i <= umaxm; i++) {
+        // The ring advances 16 logical rows at a time: row i = 17, 33, ...
+        // loads logical rows RING + i - 16 .. RING + i - 1 (one memory round
+        // trip) into the slots of rows i - 16 .. i - 1, which no later row
+        // reads.  Rows i .. i + 15 read up to i + 15 + W - 1 <= RING + i - 1
+        // (RING >= W + 15).
+        int32_t pb = 1;  // physical ring row of logical row i
+        for (int32_t i = 1; i <= umaxm; i++, pb = pb == RING ? 1 : pb + 1) {
+            if (i > 1 && ((i - 1) & 15) == 0 && RING + i - 16 <= umaxm + W - 1) {
+                const int32_t ph = pb > 16 ? pb - 16 : pb - 16 + RING;  // slot of row i - 16
+                ring_batch<W, RING>(codes, lane, s0, pv.read, m, L, RING + i - 16, ph, 16);
+            }
             if (i <= m) {
-                const uint8_t *crow = codes + (size_t)i * 64 + lane;
+                const uint8_t *crow = codes + (size_t)pb * 64 + lane;
                 const uint32_t rowlut = lut_lds[crow[0] >> 5];
-                int32_t Ul = oe, Fx = negx, key = 0, H0 = 0, Hl = 0;
+                int32_t Ul = oe, Fx = negx, H0 = 0, Hl = 0;
+                // Row keys H << 8 | (64 - k') over segments of 65 cells
+                // (k' = k mod 65): the inline constant 64 - k' needs no scalar
+                // register, and the larger key is the larger H, then the
+                // smaller k.  Keys are folded pairwise with v_max3.
+                int32_t key1 = 0, key2 = 0, kpend = 0;
+                int32_t Eprev = 0;
                 uint32_t acc = 0;
                 uint32_t *trow_p = tr + (size_t)(i - 1) * WW * 64;
+                // Window codes are read BAND_PF cells ahead of their use (a
+                // rotating register buffer), and a scheduling barrier closes
+                // every cell, so the LDS latency hides behind the cells in
+                // between and no value lives longer than the pipeline needs.
+                uint32_t cbuf[BAND_PF];
+#pragma unroll
+                for (int q = 0; q < BAND_PF; q++) cbuf[q] = q < W ? crow[q * 64] : 0u;
 #pragma unroll
                 for (int k = 0; k < W; k++) {
                     int32_t Uup, Bup, Ud;
@@ -269,13 +297,18 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                         Bup = (k + 1 < W) ? Bx[k + 1] : negx;
                         Ud = Ur[k];
                     }
-                    const uint32_t code5 = crow[k * 64];
+                    const uint32_t code5 = cbuf[k % BAND_PF];
+                    if (k + BAND_PF < W) cbuf[k % BAND_PF] = crow[(k + BAND_PF) * 64];
                     const int32_t sc = __builtin_amdgcn_sbfe((int32_t)rowlut, code5, 5);
                     int32_t D;
                     asm("v_add3_u32 %0, %1, %2, %3" : "=v"(D) : "v"(Ud), "v"(sc), "s"(noe));
+                    // F is kept floored at 0, which folds H's floor into one
+                    // v_max3: H is unchanged, and a traceback only follows an F
+                    // run through cells whose F is positive, where the floored
+                    // value and its extension bit equal the unfloored ones.
                     const int32_t E = max(Uup, Bup);
-                    const int32_t F = max(Ul, Fx);
-                    const int32_t H = max(max(max(D, E), F), 0);
+                    const int32_t F = max(max(Ul, Fx), zr);
+                    const int32_t H = max(max(D, E), F);
                     // Sign bits: D < H, E < H, E from extension, F from extension,
                     // shifted into the trace word with v_alignbit.
                     const uint32_t x0 = (uint32_t)(D - H), x1 = (uint32_t)(E - H);
@@ -287,28 +320,50 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                                  : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3));
                     const int32_t Un = H + oe;
                     if constexpr (PACKED) {
-                        Pk[k] = __builtin_amdgcn_perm((uint32_t)(E + ext), (uint32_t)Un, 0x05040100u);
+                        // Ul (= U_{k-1}) has been read: pack B_{k-1} beside it.
+                        if (k > 0) pack_hi(Pk[k - 1], Eprev, extv);
+                        Pk[k] = (uint32_t)Un;
+                        if (k == W - 1) pack_hi(Pk[k], E, extv);
+                        Eprev = E;
                     } else {
                         Ur[k] = Un;
                         Bx[k] = E + ext;
                     }
                     Ul = Un;
                     Fx = F + ext;
-                    // Running row key (H << 8 | 255 - k): kept as a sequential chain
-                    // so the compiler cannot defer it into a tree that keeps every
-                    // cell's H live until the row ends.
-                    int32_t kk;
-                    asm("v_lshl_or_b32 %0, %1, 8, %2" : "=v"(kk) : "v"(H), "s"(255 - k));
-                    asm("v_max_i32 %0, %1, %2" : "=v"(key) : "v"(kk), "v"(key));
+                    // Running row keys, kept as sequential chains so the compiler
+                    // cannot defer them into a tree that keeps every cell's H
+                    // live until the row ends.
+                    {
+                        const int kq = k % 65;  // compile-time after unrolling
+                        const bool last = (k == W - 1) || (kq == 64);
+                        int32_t &key = k < 65 ? key1 : key2;
+                        int32_t kk;
+                        asm("v_lshl_or_b32 %0, %1, 8, %2" : "=v"(kk) : "v"(H), "I"(64 - kq));
+                        if ((kq & 1) == 1) {
+                            asm("v_max3_i32 %0, %1, %2, %3" : "=v"(key) : "v"(key), "v"(kpend), "v"(kk));
+                        } else if (last) {
+                            asm("v_max_i32 %0, %1, %2" : "=v"(key) : "v"(key), "v"(kk));
+                        } else {
+                            kpend = kk;
+                        }
+                    }
                     if (k == 0) H0 = H;
                     if (k == W - 1) Hl = H;
                     if ((k & 7) == 7 || k == W - 1) {
                         trow_p[(k >> 3) * 64] = acc;
                         acc = 0;
                     }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                const int32_t rmax = key >> 8;
-                const int32_t rarg = 255 - (key & 255);
+                int32_t rmax = key1 >> 8;
+                int32_t rarg = 64 - (key1 & 255);
+                if constexpr (W > 65) {
+                    if ((key2 >> 8) > rmax) {  // ties keep the first segment
+                        rmax = key2 >> 8;
+                        rarg = 65 + 64 - (key2 & 255);
+                    }
+                }
                 if (!touched) {
                     if (H0 == rmax) {
                         touched = 1;
@@ -341,7 +396,7 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
             const uint64_t c0 = a.cig_off[p];
             const uint32_t cap = (uint32_t)(a.cig_off[p + 1] - c0);
             uint32_t *cslot = a.cigar + c0;
-            TbResult tb = traceback_band<W>(pv, P, codes, lane, tr, cb, best, bi, bj, cslot, cap);
+            TbResult tb = traceback_band<W>(pv, P, tr, cb, best, bi, bj, cslot, cap);
             esp_ext e;
             uint32_t nops;
             finish_pair(pv, P, best, tb, cslot, cap, &e, &nops);
@@ -379,7 +434,7 @@ bool band_half_supported(int half) {
     return half == 16 || half == 30 || half == 32 || half == 48 || half == 64;
 }
 
-static size_t band_lds(int half) { return (size_t)band_span(half) * 64; }
+static size_t band_lds(int half) { return (size_t)(band_ring(half) + 2 * half + 1) * 64; }
 
 hipError_t launch_gotoh_band(const GotohArgs &a, int half, int blocks, hipStream_t s) {
     const size_t lds = band_lds(half);

@@ -180,7 +180,7 @@ inline void tim_mark(esp_ctx *c, int kind, int half) {
 // never occurs; 5 is any non-ACGT byte).  Mirrors SubstLut (extend.rs:161-186).
 // The band kernel's scores are 5-bit LUT fields, and its wide-band
 // specialisations keep U = H + open_ext and E + ext as int16: with reads of
-// at most band_span(64) bases and |score| <= 16, every such value lies in
+// at most band_max_read(half) <= 287 bases and |score| <= 16, every such value lies in
 // (-20000, 16384) when gap_open + gap_ext <= 8000.
 inline bool band_lut_fits(const esp_params *p) {
     return p->match_score >= -16 && p->match_score <= 15 && p->mismatch >= -16 &&

@@ -35,16 +35,20 @@ struct GParams {
     int32_t edit;      // 0 Off, 1 Ag (read G / ref A free), 2 Tc (read C / ref T free)
 };
 
+// Base code A0 C1 G2 T3 (either case), `other` for every other byte, with
+// no branch: after folding case (b & 0xDF) the four bases are the bytes
+// 0x40 + {1, 3, 7, 20}, and bits 1-2 of those bytes (0, 1, 3, 2) are a Gray
+// code whose decode t ^ (t >> 1) is the base code.
+__device__ __forceinline__ int32_t base_code_or(uint32_t b, int32_t other) {
+    const uint32_t u = b & 0xDFu;
+    const bool acgt = (u & 0xE0u) == 0x40u && ((0x10008Au >> (u & 31u)) & 1u);
+    const uint32_t t = (u >> 1) & 3u;
+    return acgt ? (int32_t)(t ^ (t >> 1)) : other;
+}
 // Read-side code: A0 C1 G2 T3, other 4.  Window-side code maps "other" to 5
 // so a read N never equals a window N (N scores mismatch, extend.rs:166-170).
-__device__ __forceinline__ int32_t read_code(uint32_t b) {
-    uint32_t u = b & 0xDFu;
-    return u == 'A' ? 0 : u == 'C' ? 1 : u == 'G' ? 2 : u == 'T' ? 3 : 4;
-}
-__device__ __forceinline__ int32_t win_code(uint32_t b) {
-    uint32_t u = b & 0xDFu;
-    return u == 'A' ? 0 : u == 'C' ? 1 : u == 'G' ? 2 : u == 'T' ? 3 : 5;
-}
+__device__ __forceinline__ int32_t read_code(uint32_t b) { return base_code_or(b, 4); }
+__device__ __forceinline__ int32_t win_code(uint32_t b) { return base_code_or(b, 5); }
 
 // The window code that scores 0 against this read code under the active
 // EditDir (extend.rs:176-183), or 6 (matches nothing) when none does.

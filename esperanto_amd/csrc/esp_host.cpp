@@ -203,8 +203,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
             const int half = espk::plan_class_half(cls);
             if (pa.hint_mode && half != 30) continue;
-            const uint64_t W = 2 * (uint64_t)half + 1;
-            const uint64_t maxw = espk::band_trace_words(espk::band_span(half) - W, half);
+            const uint64_t maxw = espk::band_trace_words(espk::band_max_read(half), half);
             if (!c->band_blocks[cls]) c->band_blocks[cls] = espk::gotoh_band_blocks_per_cu(half);
             uint64_t blocks = (uint64_t)c->ncu * c->band_blocks[cls];
             blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);

@@ -342,7 +342,7 @@ __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint
         // extend_hint attempt 1 is the band (hint, 30) (extend.rs:408-419):
         // interior ones run in the half-30 band kernel.
         const int64_t off = a.offset[p];
-        if (off >= 30 && off + (int64_t)m + 30 <= (int64_t)n && (int64_t)m + 61 <= band_span(30))
+        if (off >= 30 && off + (int64_t)m + 30 <= (int64_t)n && (int64_t)m <= band_max_read(30))
             cls = 1;
     } else if (a.band_ok && m >= 1) {
         const int64_t off = a.offset[p], half = a.half[p];
@@ -351,7 +351,7 @@ __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint
         for (int t = 0; t < 5; t++)
             if (half == kPlanHalves[t]) c = t;
         if (c >= 0 && off >= half && off + (int64_t)m + half <= (int64_t)n &&
-            (int64_t)m + 2 * half + 1 <= band_span((int)half))
+            (int64_t)m <= band_max_read((int)half))
             cls = (uint32_t)c;
     }
     return cls * PLAN_NBIN + (PLAN_NBIN - 1 - min(m, (uint32_t)PLAN_NBIN - 1));

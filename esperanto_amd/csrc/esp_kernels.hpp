@@ -50,6 +50,10 @@ struct GotohArgs {
     // through the counter esc_range[1].
     uint32_t *esc_range;
     uint32_t *esc_list;
+    // Always 0 (value-initialised).  Read from the argument block, the band
+    // kernel's floor at 0 is an operand the compiler cannot see through, so
+    // F's floor and H's max stay two v_max3.
+    int32_t zero;
 };
 
 struct MyersArgs {
@@ -119,15 +123,19 @@ __host__ __device__ inline uint64_t strip_trace_words(uint64_t m, uint64_t n) {
 hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s);
 
 // Diagonal-frame band kernel for interior bands (every row window inside
-// [1, n]); half must be one of band_halves().  LDS holds the window codes of
-// the unit's 64 pairs: (max m + 2*half) * 64 bytes.
+// [1, n]); half must be one of band_halves().
 constexpr int BAND_BLOCK = 64;
-// LDS rows of window codes per wave (64 bytes each): a pair fits when
-// m + 2*half + 1 <= span, i.e. reads up to 255 bases at half 64.  The spans
-// keep LDS at 20-24 KB per wave so that LDS does not cut occupancy below
-// what the register arrays allow.
-__host__ __device__ constexpr int band_span(int half) {
-    return half <= 32 ? 320 : (half <= 48 ? 352 : 384);
+// Longest read the band kernel takes at each half (longer ones run in the
+// strip kernel): 287 at half 16, 259 at 30, 255 at 32, 48 and 64.
+__host__ __device__ constexpr int band_max_read(int half) {
+    return (half <= 32 ? 320 : (half <= 48 ? 352 : 384)) - (2 * half + 1);
+}
+// LDS ring of window-code rows per wave (64 bytes a row, plus 2*half mirror
+// rows and the LUT row): sized so that LDS allows as many waves per CU as
+// the register arrays do (16 at half 16, 12 at 30-48, 8 at 64), whatever
+// the read length.
+__host__ __device__ constexpr int band_ring(int half) {
+    return half <= 16 ? 127 : (half <= 32 ? 148 : (half <= 48 ? 116 : 191));
 }
 bool band_half_supported(int half);
 hipError_t launch_gotoh_band(const GotohArgs &a, int half, int blocks, hipStream_t s);

@@ -9,6 +9,10 @@ full matrix.  Seeds, shapes and offsets follow crates/map/src/extend.rs:
   band_blocks       avx2_band_block_boundaries       :2828-2862  seed 0x13579BDF2468ACE0
   band_dispatch     avx2_band_dispatch_path_matches_scalar :2869-2913 seed 0x0BADC0DE12345678
 The long n = 20,000 windows are returned separately (long_full, long_band).
+
+band_interior is not one of the reference's sweeps: it places pairs inside
+the interior-band envelope of every specialised half (offset >= half and
+offset + m + half <= n) so that each one exercises the band kernel.
 """
 from __future__ import annotations
 
@@ -137,4 +141,44 @@ def hint_cases(seed=0x5EED_4711, count=400):
     for n in (1, 10, 29, 30, 31):
         read = rand_seq(rng, 25)
         cases.append((read, rand_window(rng, read, n), EditDir.Off, 0, 30))
+    return cases
+
+
+def _edit_near_edges(rng: Rng, read: bytes, half: int) -> bytes:
+    """The read with a gap of about half bases (inserted or deleted) in its
+    middle, so the optimal path runs along a band edge."""
+    mid = len(read) // 2
+    g = max(1, half - rng.below(3))
+    if rng.below(2) == 0:
+        return read[:mid] + rand_seq(rng, g) + read[mid:]
+    return read[:mid] + read[mid + g:] if len(read) > mid + g else read
+
+
+def band_interior(seed=0x1A7E_B0D5):
+    """(read, window, dir, offset, half) for halves 16/30/32/48/64 with the
+    band strictly inside the window: similar and random pairs, gaps near the
+    band edge, reads of 1..300 bases, offsets at both interior limits."""
+    rng = Rng(seed)
+    cases = []
+    for half in (16, 30, 32, 48, 64):
+        for m in (1, 2, 7, 31, 64, 65, 100, 129, 150, 199, 250, 300):
+            for rep in range(6):
+                read = rand_seq(rng, m)
+                extra = (0, 1, 17, 40)[rep % 4]
+                n = m + 2 * half + extra
+                lo, hi = half, n - m - half
+                off = (lo, hi, lo + rng.below(hi - lo + 1))[rep % 3]
+                if rep == 5:
+                    win = rand_seq(rng, n)  # unrelated: ties and zero rows
+                else:
+                    body = read if rep < 3 else _edit_near_edges(rng, read, half)
+                    w = bytearray(rand_seq(rng, n))
+                    for k, b in enumerate(body):
+                        j = off + k
+                        if j < n:
+                            w[j] = rng.base() if rng.below(25) == 0 else b
+                    win = bytes(w)
+                if rep == 4:
+                    read = read.lower()
+                cases.append((read, win, DIRS[rep % 3], off, half))
     return cases

@@ -102,6 +102,28 @@ def test_band_sweeps(buf):
     compare_batch(buf, sweeps.band_dispatch(), 0, label="band_dispatch")
 
 
+def test_band_kernel_interior_sweep(buf):
+    """Pairs inside the interior envelope of every specialised half: all of
+    them take the band kernel (planner routing checked), bit-exact."""
+    from esperanto_amd.batch import DeviceBatch, PairBatch
+    cases = sweeps.band_interior()
+    for d, group in _group_by_dir(cases).items():
+        hb = PairBatch.from_pairs([(c[0], c[1], c[3], c[4]) for c in group])
+        db = DeviceBatch(hb, buf)
+        db.launch_gotoh(ExtendParams(edit_dir=EditDir(d)), 0)
+        db.sync()
+        st = db.last_plan_stats()
+        # the band kernel's LDS holds m + 2 * half + 1 window rows: 320 for
+        # halves up to 32, 352 for 48, 384 for 64; longer reads take the strip
+        span = {16: 320, 30: 320, 32: 320, 48: 352, 64: 384}
+        want = {h: sum(1 for c in group if c[4] == h and 0 < len(c[0]) <= span[h] - 2 * h - 1)
+                for h in (16, 30, 32, 48, 64)}
+        say(f"interior dir={d} routing {st}")
+        assert st["band_pairs"] == want, (st, want)
+        db.free()
+    compare_batch(buf, cases, 0, label="band_interior")
+
+
 def test_long_windows(buf):
     _, lf = sweeps.full_sweep(include_long=True)
     _, lb = sweeps.band_sweep(include_long=True)
