@@ -30,27 +30,40 @@ namespace espk {
 
 namespace {
 
-// Bytes p[lo .. lo + 15] as four words in which byte t is p[lo + t], for
-// the bytes up to index hi (later ones are unspecified; with hi < lo all
-// are).  Every load is an aligned dword holding at least one byte of
+// Bytes p[lo .. lo + 4*NW - 1] as NW words in which byte t is p[lo + t],
+// for the bytes up to index hi (later ones are unspecified; with hi < lo
+// all are).  Every load is an aligned dword holding at least one byte of
 // p[0 .. max(hi, 0)] (the caller guarantees p[0] is readable), so none
 // leaves the sequence's pages and no load needs a branch; one round trip.
-__device__ __forceinline__ void load_span16(const uint8_t *p, int32_t lo, int32_t hi, uint32_t out[4]) {
+template <int NW>
+__device__ __forceinline__ void load_span(const uint8_t *p, int32_t lo, int32_t hi, uint32_t out[NW]) {
     const int32_t last = max(hi, 0);
     const uintptr_t a0 = (uintptr_t)(p + min(lo, last));
     const uintptr_t d0 = a0 & ~(uintptr_t)3, dl = (uintptr_t)(p + last) & ~(uintptr_t)3;
     const uint32_t sh = (uint32_t)(a0 & 3u);
-    uint32_t w[5];
+    uint32_t w[NW + 1];
 #pragma unroll
-    for (int k = 0; k < 5; k++) w[k] = *(const uint32_t *)(d0 + 4 * k <= dl ? d0 + 4 * k : dl);
+    for (int k = 0; k <= NW; k++) w[k] = *(const uint32_t *)(d0 + 4 * k <= dl ? d0 + 4 * k : dl);
 #pragma unroll
-    for (int k = 0; k < 4; k++) out[k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
+    for (int k = 0; k < NW; k++) out[k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], sh);
 }
 
-__device__ __forceinline__ uint32_t byte16(const uint32_t w[4], int32_t t) {
-    const uint64_t lo = (uint64_t)w[1] << 32 | w[0], hi = (uint64_t)w[3] << 32 | w[2];
-    return (uint32_t)((t < 8 ? lo >> (8 * t) : hi >> (8 * (t - 8))) & 0xFFu);
+__device__ __forceinline__ void load_span16(const uint8_t *p, int32_t lo, int32_t hi, uint32_t out[4]) {
+    load_span<4>(p, lo, hi, out);
 }
+
+template <int NW>
+__device__ __forceinline__ uint32_t byte_at(const uint32_t w[NW], int32_t t) {
+    if constexpr (NW == 2) {
+        const uint64_t v = (uint64_t)w[1] << 32 | w[0];
+        return (uint32_t)((v >> (8 * t)) & 0xFFu);
+    } else {
+        const uint64_t lo = (uint64_t)w[1] << 32 | w[0], hi = (uint64_t)w[3] << 32 | w[2];
+        return (uint32_t)((t < 8 ? lo >> (8 * t) : hi >> (8 * (t - 8))) & 0xFFu);
+    }
+}
+
+__device__ __forceinline__ uint32_t byte16(const uint32_t w[4], int32_t t) { return byte_at<4>(w, t); }
 
 // Ring byte of logical row c: read code of base c-1 in bits 5-7 (rows up to
 // m), 5 * window code below.
@@ -59,24 +72,24 @@ __device__ __forceinline__ uint8_t ring_byte(uint32_t wbyte, uint32_t rbyte, boo
     return (uint8_t)(rcode | (uint32_t)(5 * win_code(wbyte)));
 }
 
-// Logical rows c0 .. c0 + nrows - 1 (1-based, nrows <= 16 and uniform)
+// Logical rows c0 .. c0 + nrows - 1 (1-based, nrows <= 4*NW and uniform)
 // into the LDS ring from physical row ph on: row c is the byte (read code of
 // base c-1 << 5 when c <= m) | 5 * window code of s0[c-1], at physical row
 // 1 + (c-1) mod RING and again at RING + that row for the first W-1
 // physical rows, so that a DP row reads its W codes from consecutive
 // physical rows.  Rows past this lane's last row L get unspecified bytes
 // (the caller only writes slots no row reads any more).
-template <int W, int RING>
+template <int W, int RING, int NW>
 __device__ __forceinline__ void ring_batch(uint8_t *codes, uint32_t lane, const uint8_t *s0,
                                            const uint8_t *read, int32_t m, int32_t L, int32_t c0,
                                            int32_t ph, int32_t nrows) {
-    uint32_t ww[4], rw[4];
-    load_span16(s0, c0 - 1, L - 1, ww);
-    load_span16(read, c0 - 1, m - 1, rw);
+    uint32_t ww[NW], rw[NW];
+    load_span<NW>(s0, c0 - 1, L - 1, ww);
+    load_span<NW>(read, c0 - 1, m - 1, rw);
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
+    for (int t = 0; t < 4 * NW; t++) {
         if (t < nrows) {
-            const uint8_t v = ring_byte(byte16(ww, t), byte16(rw, t), c0 + t <= m);
+            const uint8_t v = ring_byte(byte_at<NW>(ww, t), byte_at<NW>(rw, t), c0 + t <= m);
             codes[(size_t)ph * 64 + lane] = v;
             if (ph <= W - 1) codes[(size_t)(ph + RING) * 64 + lane] = v;
             ph = ph == RING ? 1 : ph + 1;
@@ -172,6 +185,11 @@ __device__ TbResult traceback_band(const PairView &pv, const GParams &P, const u
 }
 
 constexpr int BAND_PF = 4;  // window-code prefetch distance in cells
+// Halves from this one up keep U and B packed as int16 pairs (half the
+// registers, one SDWA op more per cell).
+#ifndef BAND_PACK_FROM
+#define BAND_PACK_FROM 48
+#endif
 
 // pk = (low 16 bits of e + x) << 16 | (pk & 0xFFFF), in one VALU op.
 __device__ __forceinline__ void pack_hi(uint32_t &pk, int32_t e, int32_t x) {
@@ -180,7 +198,7 @@ __device__ __forceinline__ void pack_hi(uint32_t &pk, int32_t e, int32_t x) {
         : "v"(e), "v"(x));
 }
 
-template <int HALF>
+template <int HALF, bool FOLD>
 __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
     constexpr int W = 2 * HALF + 1;
     constexpr int WW = (W + 7) / 8;
@@ -194,7 +212,11 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
     const int32_t noe = -oe;
     const int32_t negx = NEG + ext;
     const int32_t zr = a.zero;
-    const int32_t extv = ext;
+    // VGPR copies of the penalties: v_add with two VGPR operands issues at a
+    // higher rate than with an SGPR operand.
+    int32_t oev, extv;
+    asm("v_mov_b32 %0, %1" : "=v"(oev) : "s"(oe));
+    asm("v_mov_b32 %0, %1" : "=v"(extv) : "s"(ext));
     const uint32_t first = a.range[0], total = a.range[1];
     const uint32_t nunits = (total + 63) / 64;
     for (;;) {
@@ -222,8 +244,8 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
         const int32_t L = m > 0 ? m + W - 1 : 0;
         const uint8_t *s0 = pv.win + cb;
         for (int32_t c0 = 1; c0 <= RING; c0 += 16)
-            ring_batch<W, RING>(codes, lane, s0, pv.read, m, L, c0, c0, min(16, RING - c0 + 1));
-        if (lane < 5) lut_lds[lane] = a.lut[lane];
+            ring_batch<W, RING, 4>(codes, lane, s0, pv.read, m, L, c0, c0, min(16, RING - c0 + 1));
+        if (lane < 5) lut_lds[lane] = FOLD ? a.lutf[lane] : a.lut[lane];
         __syncthreads();  // one wave per block: orders the LUT writes before any lane reads them
 
         // Previous row, band index k: U = H + open_ext and B = E + ext.  Wide
@@ -233,7 +255,7 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
         // fold into SDWA operands, and the cell after k, once its F has read
         // U_k as a full word, writes E_k + ext into the high half in place
         // (an SDWA add with WORD_1 destination), so packing costs no op.
-        constexpr bool PACKED = HALF >= 48;
+        constexpr bool PACKED = HALF >= BAND_PACK_FROM;
         constexpr int32_t NEG16 = -20000;  // packed "E = NEG": below every reachable U
         int32_t Ur[PACKED ? 1 : W], Bx[PACKED ? 1 : W];
         uint32_t Pk[PACKED ? W : 1];
@@ -254,21 +276,21 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
         // Per-read-code row LUT: 5-bit signed scores for window codes 0..5
         // (A C G T, -, other) at bit 5*code; the window codes in LDS are
         // stored pre-multiplied by 5.
-        // The ring advances 16 logical rows at a time: row i = 17, 33, ...
-        // loads logical rows RING + i - 16 .. RING + i - 1 (one memory round
-        // trip) into the slots of rows i - 16 .. i - 1, which no later row
-        // reads.  Rows i .. i + 15 read up to i + 15 + W - 1 <= RING + i - 1
-        // (RING >= W + 15).
+        // The ring advances 8 logical rows at a time: row i = 9, 17, ...
+        // loads logical rows RING + i - 8 .. RING + i - 1 (one memory round
+        // trip) into the slots of rows i - 8 .. i - 1, which no later row
+        // reads.  Rows i .. i + 7 read up to i + 7 + W - 1 <= RING + i - 1
+        // (RING >= W + 7).
         int32_t pb = 1;  // physical ring row of logical row i
         for (int32_t i = 1; i <= umaxm; i++, pb = pb == RING ? 1 : pb + 1) {
-            if (i > 1 && ((i - 1) & 15) == 0 && RING + i - 16 <= umaxm + W - 1) {
-                const int32_t ph = pb > 16 ? pb - 16 : pb - 16 + RING;  // slot of row i - 16
-                ring_batch<W, RING>(codes, lane, s0, pv.read, m, L, RING + i - 16, ph, 16);
+            if (i > 1 && ((i - 1) & 7) == 0 && RING + i - 8 <= umaxm + W - 1) {
+                const int32_t ph = pb > 8 ? pb - 8 : pb - 8 + RING;  // slot of row i - 8
+                ring_batch<W, RING, 2>(codes, lane, s0, pv.read, m, L, RING + i - 8, ph, 8);
             }
             if (i <= m) {
                 const uint8_t *crow = codes + (size_t)pb * 64 + lane;
                 const uint32_t rowlut = lut_lds[crow[0] >> 5];
-                int32_t Ul = oe, Fx = negx, H0 = 0, Hl = 0;
+                int32_t Ul = oe, Fx = negx, Hl = 0;
                 // Row keys H << 8 | (64 - k') over segments of 65 cells
                 // (k' = k mod 65): the inline constant 64 - k' needs no scalar
                 // register, and the larger key is the larger H, then the
@@ -301,7 +323,15 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                     if (k + BAND_PF < W) cbuf[k % BAND_PF] = crow[(k + BAND_PF) * 64];
                     const int32_t sc = __builtin_amdgcn_sbfe((int32_t)rowlut, code5, 5);
                     int32_t D;
-                    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(D) : "v"(Ud), "v"(sc), "s"(noe));
+                    if constexpr (FOLD && PACKED) {  // sc is score - oe
+                        asm("v_add_u32_sdwa %0, sext(%1), %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
+                            "src0_sel:WORD_0 src1_sel:DWORD"
+                            : "=v"(D) : "v"(Pk[k]), "v"(sc));
+                    } else if constexpr (FOLD) {
+                        D = Ud + sc;
+                    } else {
+                        asm("v_add3_u32 %0, %1, %2, %3" : "=v"(D) : "v"(Ud), "v"(sc), "s"(noe));
+                    }
                     // F is kept floored at 0, which folds H's floor into one
                     // v_max3: H is unchanged, and a traceback only follows an F
                     // run through cells whose F is positive, where the floored
@@ -318,7 +348,7 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                                  "v_alignbit_b32 %0, %0, %3, 31\n\t"
                                  "v_alignbit_b32 %0, %0, %4, 31"
                                  : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3));
-                    const int32_t Un = H + oe;
+                    const int32_t Un = H + oev;
                     if constexpr (PACKED) {
                         // Ul (= U_{k-1}) has been read: pack B_{k-1} beside it.
                         if (k > 0) pack_hi(Pk[k - 1], Eprev, extv);
@@ -327,10 +357,10 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                         Eprev = E;
                     } else {
                         Ur[k] = Un;
-                        Bx[k] = E + ext;
+                        Bx[k] = E + extv;
                     }
                     Ul = Un;
-                    Fx = F + ext;
+                    Fx = F + extv;
                     // Running row keys, kept as sequential chains so the compiler
                     // cannot defer them into a tree that keeps every cell's H
                     // live until the row ends.
@@ -348,7 +378,6 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                             kpend = kk;
                         }
                     }
-                    if (k == 0) H0 = H;
                     if (k == W - 1) Hl = H;
                     if ((k & 7) == 7 || k == W - 1) {
                         trow_p[(k >> 3) * 64] = acc;
@@ -365,7 +394,7 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                     }
                 }
                 if (!touched) {
-                    if (H0 == rmax) {
+                    if (rarg == 0) {  // the first maximum is cell 0: H[0] == rmax
                         touched = 1;
                         trow = i;
                         tleft = 1;
@@ -415,16 +444,23 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
     }
 }
 
+template <int HALF, bool FOLD>
+hipError_t launch_half(const GotohArgs &a, int blocks, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_gotoh_band<HALF, FOLD>), dim3(blocks), dim3(BAND_BLOCK), lds, s, a);
+    return hipGetLastError();
+}
+
+// The folded LUT when its fields fit, else the plain one plus -oe per cell.
 template <int HALF>
 hipError_t launch_half(const GotohArgs &a, int blocks, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_gotoh_band<HALF>), dim3(blocks), dim3(BAND_BLOCK), lds, s, a);
-    return hipGetLastError();
+    if (a.lut_folded) return launch_half<HALF, true>(a, blocks, lds, s);
+    return launch_half<HALF, false>(a, blocks, lds, s);
 }
 
 template <int HALF>
 int occupancy_half(size_t lds) {
     int nb = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_band<HALF>, BAND_BLOCK, lds);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_band<HALF, true>, BAND_BLOCK, lds);
     return nb > 0 ? nb : 1;
 }
 

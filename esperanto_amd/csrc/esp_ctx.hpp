@@ -214,6 +214,24 @@ inline void band_lut(const esp_params *p, uint32_t out[5]) {
     }
 }
 
+// band_lut with -open_ext added to every field (score + gap_open + gap_ext);
+// false when a field leaves the 5-bit signed range.
+inline bool band_lut_folded(const esp_params *p, uint32_t out[5]) {
+    const int noe = p->gap_open + p->gap_ext;
+    bool fits = true;
+    for (int rc = 0; rc < 5; rc++) {
+        const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;
+        uint32_t w = 0;
+        for (int f = 0; f < 6; f++) {
+            const int s = ((rc < 4 && f == rc) ? p->match_score : (f == partner ? 0 : p->mismatch)) + noe;
+            fits = fits && s >= -16 && s <= 15;
+            w |= ((uint32_t)s & 31u) << (5 * f);
+        }
+        out[rc] = w;
+    }
+    return fits;
+}
+
 // Launch geometry of the persistent strip grid for `count` pairs whose
 // largest has maxm rows and maxw trace words.
 inline int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t maxm, uint64_t maxw) {

@@ -159,6 +159,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     a.mode = mode;
     a.P = dev_params(p);
     band_lut(p, a.lut);
+    a.lut_folded = band_lut_folded(p, a.lutf) ? 1 : 0;
     full_lut(p, a.flut);
     a.ext = o->ext;
     a.touch = o->touch;

@@ -41,6 +41,11 @@ struct GotohArgs {
     // Band kernel substitution rows: for read code 0..4, 5-bit signed scores
     // of window codes 0..5 packed at bit 5*code.
     uint32_t lut[5];
+    // The same rows with -open_ext folded in (score - oe, still 5-bit
+    // signed fields); lut_folded says they fit.  The packed wide-band
+    // kernels then form D = U + (score - oe) from the packed U in one SDWA add.
+    uint32_t lutf[5];
+    int32_t lut_folded;
     // Full-matrix kernel substitution columns: for window code 0..5, 5-bit
     // signed scores of read codes 0..4 at bit 5*code, and -16 at bit 25 (the
     // code of rows past the read).

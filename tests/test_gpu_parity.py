@@ -122,6 +122,11 @@ def test_band_kernel_interior_sweep(buf):
         assert st["band_pairs"] == want, (st, want)
         db.free()
     compare_batch(buf, cases, 0, label="band_interior")
+    # gap_open + gap_ext = 43 does not fold into the 5-bit LUT: the packed
+    # halves take their unfolded kernels
+    compare_batch(buf, cases, 0, ExtendParams(gap_open=40, gap_ext=3), label="band_interior_unfolded")
+    compare_batch(buf, cases, 0, ExtendParams(match_score=1, mismatch=-3, gap_open=5, gap_ext=2),
+                  label="band_interior_custom")
 
 
 def test_long_windows(buf):
