@@ -368,14 +368,19 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
                         const int kq = k % 65;  // compile-time after unrolling
                         const bool last = (k == W - 1) || (kq == 64);
                         int32_t &key = k < 65 ? key1 : key2;
+                        // one asm statement per fold: a dependent pair of asm
+                        // statements gets a conservative s_nop between them
                         int32_t kk;
-                        asm("v_lshl_or_b32 %0, %1, 8, %2" : "=v"(kk) : "v"(H), "I"(64 - kq));
                         if ((kq & 1) == 1) {
-                            asm("v_max3_i32 %0, %1, %2, %3" : "=v"(key) : "v"(key), "v"(kpend), "v"(kk));
+                            asm("v_lshl_or_b32 %1, %2, 8, %4\n\t"
+                                "v_max3_i32 %0, %0, %3, %1"
+                                : "+v"(key), "=&v"(kk) : "v"(H), "v"(kpend), "I"(64 - kq));
                         } else if (last) {
-                            asm("v_max_i32 %0, %1, %2" : "=v"(key) : "v"(key), "v"(kk));
+                            asm("v_lshl_or_b32 %1, %2, 8, %3\n\t"
+                                "v_max_i32 %0, %0, %1"
+                                : "+v"(key), "=&v"(kk) : "v"(H), "I"(64 - kq));
                         } else {
-                            kpend = kk;
+                            asm("v_lshl_or_b32 %0, %1, 8, %2" : "=v"(kpend) : "v"(H), "I"(64 - kq));
                         }
                     }
                     if (k == W - 1) Hl = H;
