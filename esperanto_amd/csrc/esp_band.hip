@@ -52,14 +52,14 @@ __device__ __forceinline__ void load_span16(const uint8_t *p, int32_t lo, int32_
     load_span<4>(p, lo, hi, out);
 }
 
+// Byte t of the words; t is a compile-time constant where the ring uses it.
 template <int NW>
 __device__ __forceinline__ uint32_t byte_at(const uint32_t w[NW], int32_t t) {
-    if constexpr (NW == 2) {
-        const uint64_t v = (uint64_t)w[1] << 32 | w[0];
-        return (uint32_t)((v >> (8 * t)) & 0xFFu);
-    } else {
+    if constexpr (NW == 4) {
         const uint64_t lo = (uint64_t)w[1] << 32 | w[0], hi = (uint64_t)w[3] << 32 | w[2];
         return (uint32_t)((t < 8 ? lo >> (8 * t) : hi >> (8 * (t - 8))) & 0xFFu);
+    } else {
+        return (w[t >> 2] >> (8 * (t & 3))) & 0xFFu;
     }
 }
 
@@ -101,10 +101,23 @@ __device__ __forceinline__ void ring_batch(uint8_t *codes, uint32_t lane, const 
 // tracking; runs are written in walk order to slot[cap-1], slot[cap-2], ...
 // A step is branch-free apart from the cache refills and the run writes:
 // lanes of a wave walk in lockstep whatever their moves.
+// Bytes p[lo .. lo + 63] (valid up to index hi) into this lane's dwords
+// q0 .. q0 + 15 of the wave's LDS, dword q of lane at uint32 index q*64+lane.
+__device__ __forceinline__ void stage64(uint32_t *lds, uint32_t lane, const uint8_t *p, int32_t lo,
+                                        int32_t hi, int32_t q0) {
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        uint32_t w[4];
+        load_span<4>(p, lo + 16 * h, hi, w);
+#pragma unroll
+        for (int k = 0; k < 4; k++) lds[(size_t)(q0 + 4 * h + k) * 64 + lane] = w[k];
+    }
+}
+
 template <int W>
 __device__ TbResult traceback_band(const PairView &pv, const GParams &P, const uint32_t *tr,
                                    int32_t cb, int32_t best, int32_t bi, int32_t bj,
-                                   uint32_t *slot, uint32_t cap) {
+                                   uint32_t *slot, uint32_t cap, uint8_t *lds, uint32_t lane) {
     constexpr int WW = (W + 7) / 8;
     int32_t i = bi, j = bj, V = best;
     uint32_t state = 0;  // 0 H, 1 E (insertion run), 2 F (deletion run)
@@ -115,19 +128,21 @@ __device__ TbResult traceback_band(const PairView &pv, const GParams &P, const u
     // of rows above row 1 are clamped to row 1 (never used).
     int32_t ci0 = -1, ck = -1;
     uint64_t nlo = 0, nhi = 0;
-    // Read and window bytes of the diagonal steps: every 16 iterations (the
-    // same iteration for every lane still walking, so one memory round trip
-    // serves the wave) the bytes the next 16 steps can reach are loaded.
+    // Read and window bytes of the diagonal steps: every 64 iterations (the
+    // same iteration for every lane still walking) the 64 bytes of each that
+    // the next 64 steps can reach are staged in this lane's LDS words (the
+    // window-code ring is free once the DP is done): read bytes in dwords
+    // 0-15, window bytes in dwords 16-31.
+    uint32_t *ldsw = (uint32_t *)lds;
     int32_t it = 0, rb = 0, wb = 0;
-    uint32_t rw[4] = {}, ww[4] = {};
     while (i > 0 && j > 0) {
         const int32_t k = j - (cb + i);
         if (k < 0 || k >= W) break;
-        if ((it++ & 15) == 0) {
-            rb = max(i - 16, 0);
-            wb = max(j - 16, 0);
-            load_span16(pv.read, rb, i - 1, rw);
-            load_span16(pv.win, wb, j - 1, ww);
+        if ((it++ & 63) == 0) {
+            rb = max(i - 64, 0);
+            wb = max(j - 64, 0);
+            stage64(ldsw, lane, pv.read, rb, i - 1, 0);
+            stage64(ldsw, lane, pv.win, wb, j - 1, 16);
         }
         if (k != ck || i > ci0 || i <= ci0 - 32) {
             const int32_t t = k >> 3;
@@ -155,8 +170,9 @@ __device__ TbResult traceback_band(const PairView &pv, const GParams &P, const u
         const bool isI = state == 1 || (state == 0 && (nib & 12u) == 8u);
         const bool isD = !isM && !isI;
         const bool gext = isI ? (nib & 2u) != 0 : (isD && (nib & 1u) != 0);
-        const int32_t rc = read_code(byte16(rw, i - 1 - rb));
-        const int32_t fc = win_code(byte16(ww, j - 1 - wb));
+        const int32_t rq = i - 1 - rb, wq = j - 1 - wb;  // 0 .. 63 inside the stage
+        const int32_t rc = read_code(lds[(((size_t)(rq >> 2) * 64 + lane) << 2) + (rq & 3)]);
+        const int32_t fc = win_code(lds[(((size_t)(16 + (wq >> 2)) * 64 + lane) << 2) + (wq & 3)]);
         V -= isM ? lut_score(P, rc, fc) : (gext ? P.ext : P.oe);
         state = gext ? (isI ? 1u : 2u) : 0u;
         const uint32_t op = isM ? OP_M : (isI ? OP_I : OP_D);
@@ -198,11 +214,18 @@ __device__ __forceinline__ void pack_hi(uint32_t &pk, int32_t e, int32_t x) {
         : "v"(e), "v"(x));
 }
 
+// Waves per SIMD each specialisation is built for (the register budget:
+// 512 / waves VGPRs); the LDS ring is sized to match.
+__host__ __device__ constexpr int band_waves(int half) {
+    return half <= 16 ? 4 : (half <= 32 ? 2 : (half <= 48 ? 3 : 2));
+}
+
 template <int HALF, bool FOLD>
-__global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
+__global__ __launch_bounds__(BAND_BLOCK, band_waves(HALF)) void k_gotoh_band(GotohArgs a) {
     constexpr int W = 2 * HALF + 1;
     constexpr int WW = (W + 7) / 8;
     constexpr int RING = band_ring(HALF);
+    static_assert((RING + W) * 64 >= 32 * 256, "traceback staging needs 32 dwords per lane");
     extern __shared__ uint8_t codes[];  // [RING + W][64]: (read code << 5) | 5 * window code
     uint32_t *lut_lds = (uint32_t *)codes;  // row 0 (never a code row): the 5 row LUTs
     const uint32_t lane = threadIdx.x;
@@ -276,16 +299,26 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
         // Per-read-code row LUT: 5-bit signed scores for window codes 0..5
         // (A C G T, -, other) at bit 5*code; the window codes in LDS are
         // stored pre-multiplied by 5.
-        // The ring advances 8 logical rows at a time: row i = 9, 17, ...
-        // loads logical rows RING + i - 8 .. RING + i - 1 (one memory round
-        // trip) into the slots of rows i - 8 .. i - 1, which no later row
-        // reads.  Rows i .. i + 7 read up to i + 7 + W - 1 <= RING + i - 1
-        // (RING >= W + 7).
+        // The ring advances RF logical rows at a time: row i = RF + 1,
+        // 2 RF + 1, ... loads logical rows RING + i - RF .. RING + i - 1 (one
+        // memory round trip) into the slots of rows i - RF .. i - 1, which no
+        // later row reads.  Rows i .. i + RF - 1 read up to i + RF - 1 + W - 1
+        // <= RING + i - 1 (RING >= W + RF - 1).  Refilling many rows at once
+        // touches each cache line of the sequences in one burst instead of
+        // once per few rows (when it may have left the L2 in between).
+        constexpr int RF = (RING - W + 1) >= 32 ? 32 : 16;
+        static_assert(RING >= W + RF - 1, "ring too small for the refill");
         int32_t pb = 1;  // physical ring row of logical row i
         for (int32_t i = 1; i <= umaxm; i++, pb = pb == RING ? 1 : pb + 1) {
-            if (i > 1 && ((i - 1) & 7) == 0 && RING + i - 8 <= umaxm + W - 1) {
-                const int32_t ph = pb > 8 ? pb - 8 : pb - 8 + RING;  // slot of row i - 8
-                ring_batch<W, RING, 2>(codes, lane, s0, pv.read, m, L, RING + i - 8, ph, 8);
+            if (i > 1 && ((i - 1) & (RF - 1)) == 0 && RING + i - RF <= umaxm + W - 1) {
+                int32_t ph = pb > RF ? pb - RF : pb - RF + RING;  // slot of row i - RF
+                // 8 rows per round trip keeps the refill's registers below the
+                // cell loop's; the later rounds hit the lines the first fetched
+#pragma unroll
+                for (int q = 0; q < RF; q += 8) {
+                    ring_batch<W, RING, 2>(codes, lane, s0, pv.read, m, L, RING + i - RF + q, ph, 8);
+                    ph = ph + 8 > RING ? ph + 8 - RING : ph + 8;
+                }
             }
             if (i <= m) {
                 const uint8_t *crow = codes + (size_t)pb * 64 + lane;
@@ -430,7 +463,7 @@ __global__ __launch_bounds__(BAND_BLOCK) void k_gotoh_band(GotohArgs a) {
             const uint64_t c0 = a.cig_off[p];
             const uint32_t cap = (uint32_t)(a.cig_off[p + 1] - c0);
             uint32_t *cslot = a.cigar + c0;
-            TbResult tb = traceback_band<W>(pv, P, tr, cb, best, bi, bj, cslot, cap);
+            TbResult tb = traceback_band<W>(pv, P, tr, cb, best, bi, bj, cslot, cap, codes, lane);
             esp_ext e;
             uint32_t nops;
             finish_pair(pv, P, best, tb, cslot, cap, &e, &nops);
