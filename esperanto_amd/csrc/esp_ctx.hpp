@@ -202,6 +202,24 @@ inline void full_lut(const esp_params *p, uint32_t out[6]) {
     }
 }
 
+// full_lut with -open_ext added to every field (the rows past the read keep
+// -16 + gap_open + gap_ext); false when a field leaves the 5-bit range.
+inline bool full_lut_folded(const esp_params *p, uint32_t out[6]) {
+    const int noe = p->gap_open + p->gap_ext;
+    bool fits = -16 + noe <= 15;
+    for (int f = 0; f < 6; f++) {
+        uint32_t w = (uint32_t)((-16 + noe) & 31) << 25;
+        for (int rc = 0; rc < 5; rc++) {
+            const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;
+            const int s = ((rc < 4 && f == rc) ? p->match_score : (f == partner ? 0 : p->mismatch)) + noe;
+            fits = fits && s >= -16 && s <= 15;
+            w |= ((uint32_t)s & 31u) << (5 * rc);
+        }
+        out[f] = w;
+    }
+    return fits;
+}
+
 inline void band_lut(const esp_params *p, uint32_t out[5]) {
     for (int rc = 0; rc < 5; rc++) {
         const int partner = (p->edit_dir == 1 && rc == 2) ? 0 : (p->edit_dir == 2 && rc == 1) ? 3 : 6;

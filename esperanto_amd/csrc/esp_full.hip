@@ -168,7 +168,14 @@ __device__ TbResult traceback_full(const PairView &pv, const GParams &P, const u
     return r;
 }
 
-template <int MAXM>
+// pk = (low 16 bits of e + x) << 16 | (pk & 0xFFFF), in one VALU op.
+__device__ __forceinline__ void pack_hi(uint32_t &pk, int32_t e, int32_t x) {
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:DWORD src1_sel:DWORD"
+        : "+v"(pk)
+        : "v"(e), "v"(x));
+}
+
+template <int MAXM, bool FOLD>
 __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
     static_assert(MAXM % 8 == 0, "rows come in groups of 8");
     constexpr int WR = MAXM / 8;  // trace words per column
@@ -180,9 +187,16 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
     uint8_t *chunk = lds + 32 + (MAXM + 1) * 64 + lane;    // column q at chunk[q * 64]
     const GParams P = a.P;
     const int32_t oe = P.oe, ext = P.ext, noe = -oe;
+    const int32_t zr = a.zero;  // 0 the compiler cannot see through (one v_max3)
+    // VGPR copies of the penalties: v_add with two VGPR operands issues at a
+    // higher rate than with an SGPR operand.
+    int32_t oev, extv, noev;
+    asm("v_mov_b32 %0, %1" : "=v"(oev) : "s"(oe));
+    asm("v_mov_b32 %0, %1" : "=v"(extv) : "s"(ext));
+    asm("v_mov_b32 %0, %1" : "=v"(noev) : "s"(noe));
     const uint32_t first = a.range[0], total = a.range[1];
     const uint32_t nunits = (total + 63) / 64;
-    if (lane < 8) flut[lane] = a.flut[lane < 6 ? lane : 5];
+    if (lane < 8) flut[lane] = FOLD ? a.flutf[lane < 6 ? lane : 5] : a.flut[lane < 6 ? lane : 5];
     for (int q = 0; q < FULL_CHUNK; q++) chunk[q * 64] = 0;
     __syncthreads();
     uint32_t *tr = a.trace + (size_t)blockIdx.x * a.maxw * 64 + lane;
@@ -224,8 +238,10 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             for (int32_t q = 0; q < qend; q++) {
                 const int32_t j = j0 + q + 1;
                 const uint32_t clut = flut[chunk[q * 64]];
-                // row 0: H(0, j) = 0, E(0, j) = NEG; the diagonal of row 1 is H(0, j-1) = 0
-                int32_t Uc = oe, Bc = NEG + ext, Ud = oe, ckey = 0;
+                // row 0: H(0, j) = 0, E(0, j) = NEG; the diagonal of row 1 is
+                // H(0, j-1) = 0, i.e. U = oe in the low half of pdiag
+                int32_t Uc = oe, Bc = NEG + ext, ckey = 0, kpend = 0;
+                uint32_t pdiag = (uint32_t)oe & 0xFFFFu;
                 uint32_t *tcol = tr + (size_t)(j - 1) * WR * 64;
 #pragma unroll
                 for (int g = 0; g < WR; g++) {
@@ -234,31 +250,53 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
 #pragma unroll
                         for (int r = 0; r < 8; r++) {
                             const int i = 8 * g + r + 1;
+                            // pk = (F(i, j-1) + ext, H(i, j-1) + oe) as int16 halves;
+                            // SDWA operands read them sign-extended.
                             const uint32_t pk = Pk[i - 1];
-                            const int32_t Ul = (int32_t)(int16_t)(pk & 0xFFFFu);  // H(i, j-1) + oe
-                            const int32_t Fx = ((int32_t)pk) >> 16;                 // F(i, j-1) + ext
                             const uint32_t rc5 = rcode[(size_t)i * 64];
-                            const int32_t sc = __builtin_amdgcn_sbfe((int32_t)clut, rc5, 5);
-                            int32_t D;
-                            asm("v_add3_u32 %0, %1, %2, %3" : "=v"(D) : "v"(Ud), "v"(sc), "s"(noe));
-                            const int32_t E = max(Uc, Bc);
-                            const int32_t F = max(Ul, Fx);
-                            const int32_t H = max(max(max(D, E), F), 0);
+                            int32_t sc = __builtin_amdgcn_sbfe((int32_t)clut, rc5, 5);
+                            if constexpr (!FOLD) sc += noev;  // the folded LUT holds score - oe
+                            int32_t D, F, x3;
+                            asm("v_add_u32_sdwa %0, sext(%1), %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
+                                "src0_sel:WORD_0 src1_sel:DWORD"
+                                : "=v"(D) : "v"(pdiag), "v"(sc));
+                            // E is kept floored at 0, which folds H's floor into one
+                            // v_max3: H is unchanged, and a traceback only follows an
+                            // E run through cells whose E is positive, where the
+                            // floored value and its extension bit equal the
+                            // unfloored ones.
+                            const int32_t E = max(max(Uc, Bc), zr);
+                            asm("v_max_i32_sdwa %0, sext(%1), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+                                "src0_sel:WORD_0 src1_sel:WORD_1"
+                                : "=v"(F) : "v"(pk));
+                            asm("v_sub_u32_sdwa %0, sext(%1), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+                                "src0_sel:WORD_0 src1_sel:WORD_1"
+                                : "=v"(x3) : "v"(pk));
+                            const int32_t H = max(max(D, E), F);
                             const uint32_t x0 = (uint32_t)(D - H), x1 = (uint32_t)(E - H);
-                            const uint32_t x2 = (uint32_t)(Uc - Bc), x3 = (uint32_t)(Ul - Fx);
+                            const uint32_t x2 = (uint32_t)(Uc - Bc);
                             asm("v_alignbit_b32 %0, %0, %1, 31\n\t"
                                 "v_alignbit_b32 %0, %0, %2, 31\n\t"
                                 "v_alignbit_b32 %0, %0, %3, 31\n\t"
                                 "v_alignbit_b32 %0, %0, %4, 31"
-                                : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3));
-                            const int32_t Un = H + oe;
-                            Ud = Ul;  // H(i, j-1) + oe is the next row's diagonal
+                                : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"((uint32_t)x3));
+                            const int32_t Un = H + oev;
+                            pdiag = pk;  // H(i, j-1) + oe is the next row's diagonal
+                            uint32_t pn = (uint32_t)Un;
+                            pack_hi(pn, F, extv);
+                            Pk[i - 1] = pn;
                             Uc = Un;
-                            Bc = E + ext;
-                            Pk[i - 1] = __builtin_amdgcn_perm((uint32_t)(F + ext), (uint32_t)Un, 0x05040100u);
-                            int32_t kk;
-                            asm("v_lshl_or_b32 %0, %1, 16, %2" : "=v"(kk) : "v"(H), "s"(0xFFFF - i));
-                            asm("v_max_i32 %0, %1, %2" : "=v"(ckey) : "v"(kk), "v"(ckey));
+                            Bc = E + extv;
+                            // column key H << 16 | 0xFFFF - i, folded pairwise with
+                            // v_max3 (one asm statement per fold)
+                            if (r & 1) {
+                                int32_t kk;
+                                asm("v_lshl_or_b32 %1, %2, 16, %4\n\t"
+                                    "v_max3_i32 %0, %0, %3, %1"
+                                    : "+v"(ckey), "=&v"(kk) : "v"(H), "v"(kpend), "s"(0xFFFF - i));
+                            } else {
+                                asm("v_lshl_or_b32 %0, %1, 16, %2" : "=v"(kpend) : "v"(H), "s"(0xFFFF - i));
+                            }
                         }
                         tcol[g * 64] = acc;
                     }
@@ -292,16 +330,21 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
 template <int MAXM>
 size_t full_lds() { return 32 + (size_t)(MAXM + 1) * 64 + (size_t)FULL_CHUNK * 64; }
 
+// The folded LUT when its fields fit, else the plain one plus -oe per cell.
 template <int MAXM>
 hipError_t launch_full_t(const GotohArgs &a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_gotoh_full<MAXM>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
+    if (a.flut_folded)
+        hipLaunchKernelGGL((k_gotoh_full<MAXM, true>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
+    else
+        hipLaunchKernelGGL((k_gotoh_full<MAXM, false>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
     return hipGetLastError();
 }
 
 template <int MAXM>
 int full_blocks_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_full<MAXM>, 64, full_lds<MAXM>()) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_full<MAXM, true>, 64, full_lds<MAXM>()) !=
+        hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }

@@ -161,6 +161,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     band_lut(p, a.lut);
     a.lut_folded = band_lut_folded(p, a.lutf) ? 1 : 0;
     full_lut(p, a.flut);
+    a.flut_folded = full_lut_folded(p, a.flutf) ? 1 : 0;
     a.ext = o->ext;
     a.touch = o->touch;
     a.cigar = o->cigar;

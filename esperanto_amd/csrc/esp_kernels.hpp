@@ -50,6 +50,9 @@ struct GotohArgs {
     // signed scores of read codes 0..4 at bit 5*code, and -16 at bit 25 (the
     // code of rows past the read).
     uint32_t flut[6];
+    // flut with -open_ext folded in (as lutf for the band kernel).
+    uint32_t flutf[6];
+    int32_t flut_folded;
     // extend_hint escalation (band kernel in ESP_MODE_HINT): pairs whose band
     // attempt must be redone on the full matrix are appended to esc_list
     // through the counter esc_range[1].
