@@ -127,6 +127,18 @@ def test_band_kernel_interior_sweep(buf):
     compare_batch(buf, cases, 0, ExtendParams(gap_open=40, gap_ext=3), label="band_interior_unfolded")
     compare_batch(buf, cases, 0, ExtendParams(match_score=1, mismatch=-3, gap_open=5, gap_ext=2),
                   label="band_interior_custom")
+    # scores outside the band kernel's int16 envelope (5-bit LUT fields, gap
+    # penalties over 8000): every pair takes the int32 strip kernel instead
+    wide = ExtendParams(match_score=20, mismatch=-30, gap_open=9000, gap_ext=100)
+    group = [c for c in cases if int(c[2]) == 0]
+    hb = PairBatch.from_pairs([(c[0], c[1], c[3], c[4]) for c in group])
+    db = DeviceBatch(hb, buf)
+    db.launch_gotoh(wide, 0)
+    db.sync()
+    st = db.last_plan_stats()
+    assert sum(st["band_pairs"].values()) == 0 and st["strip_pairs"] == len(group), st
+    db.free()
+    compare_batch(buf, group, 0, wide, label="band_interior_int32")
 
 
 def test_long_windows(buf):
