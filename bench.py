@@ -143,6 +143,46 @@ def bench_window(args, buf, params):
     return out
 
 
+def bench_walk(args, buf, params):
+    """The three placement walks (rescore_with, recount_mm_ea_in,
+    exact_kmer_support) over the placements of the pair workload: per pair,
+    R1 as uM (intron)N (m-u)M and R2 as 150M, half of the planted junctions
+    in the library."""
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.synth_window import placement_walk_workload
+    from esperanto_amd.walk import DeviceJunctions, DeviceWalkBatch
+    n = args.walk_pairs
+    log(f"placement walks: {2 * n} placements on a {args.window_ref_mb} Mb reference")
+    ref, pl, jn = placement_walk_workload(n, ref_len=args.window_ref_mb * 1_000_000)
+    dref = DeviceReference(ref, buf)
+    dj = DeviceJunctions(jn, buf)
+    db = DeviceWalkBatch(pl, dref, buf)
+    w, k, tims = timed_steps(lambda: db.launch(params, None, dj), db, max(args.steps, 3), 1)
+    walk_ms = float(np.mean([sum(t[2] for t in st if t[0] == 8) for st in tims]))
+    # algorithmic bytes: query bytes, 37 B of per-placement arrays, CIGAR
+    # words, the 2-bit reference under the M columns (every query base sits
+    # in an M column here), 64 B of records out
+    nb = int(pl.read_len.sum()) + 37 * pl.n + 4 * int(pl.cig_n.sum()) + 64 * pl.n
+    nb += (int(pl.read_len.sum()) + 3) // 4
+    out = {"placements": pl.n, "placements_per_s": round(pl.n / w), "ms_per_step": round(w * 1e3, 3),
+           "kernel_ms": round(walk_ms, 3), "algorithmic_bytes": nb,
+           "kernel_gb_per_s": round(nb / (walk_ms / 1e3) / 1e9, 1), "junctions": len(jn)}
+    log(f"walks: {pl.n / w / 1e6:.2f} M placements/s, kernel {walk_ms:.3f} ms")
+    db.free()
+    dj.close()
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.OracleRef(ref)
+        t0 = time.perf_counter()
+        oracle.walk_batch(o, pl, params, oracle.SPLICE_DEFAULTS, jn, 15, threads)
+        dt = time.perf_counter() - t0
+        out["cpu_port"] = {"placements_per_s": round(pl.n / dt), "cores": threads, "sample": pl.n}
+        log(f"walks cpu port: {out['cpu_port']}")
+    dref.close()
+    return out
+
+
 def bench_serving(args, buf, params):
     """Serving loop: a pool of mixed requests (log-uniform 1-4,096 pairs of 50-250-nt
     reads; 10% mate-rescue requests with ~2 kb windows) over a 256 Mb reference
@@ -299,6 +339,9 @@ def main() -> int:
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-batch-pairs", type=int, default=131072)
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
+    ap.add_argument("--walk-pairs", type=int, default=1_000_000,
+                    help="read pairs of the placement-walk leg (two placements each; 0 skips it)")
+    ap.add_argument("--only-walk", action="store_true", help="run only the placement-walk leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -322,6 +365,10 @@ def main() -> int:
         return 0
     if args.only_serve:
         print(json.dumps(bench_serving(args, buf, params)), flush=True)
+        buf.close()
+        return 0
+    if args.only_walk:
+        print(json.dumps(bench_walk(args, buf, params)), flush=True)
         buf.close()
         return 0
 
@@ -395,6 +442,9 @@ def main() -> int:
     win = None
     if args.window_pairs:
         win = bench_window(args, buf, params)
+    walks = None
+    if args.walk_pairs:
+        walks = bench_walk(args, buf, params)
     serve = None
     if args.serve_requests:
         serve = bench_serving(args, buf, params)
@@ -448,6 +498,7 @@ def main() -> int:
                    "myers_kernel_ms": round(k2, 3),
                    "myers_valu_tops": round(myers_tops, 2),
                    "splice_rescue": win,
+                   "placement_walks": walks,
                    "serving": serve},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",

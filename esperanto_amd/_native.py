@@ -26,7 +26,8 @@ EXPORTS = [
     "esp_ref_device_bytes", "esp_gather_batch", "esp_mate_rescue_batch", "esp_spliced_batch",
     "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
-    "esp_unspliced_batch", "esp_window_myers_batch",
+    "esp_unspliced_batch", "esp_window_myers_batch", "esp_junctions_upload", "esp_junctions_free",
+    "esp_walk_placements", "esp_intron_length_penalty",
 ]
 
 
@@ -70,6 +71,19 @@ class CChainBatch(ctypes.Structure):
                 ("ref_start", ctypes.c_void_p), ("ref_end", ctypes.c_void_p),
                 ("read_start", ctypes.c_void_p), ("read_end", ctypes.c_void_p),
                 ("allow_dp", ctypes.c_void_p), ("max_read_len", ctypes.c_uint32)]
+
+
+class CWalkBatch(ctypes.Structure):
+    _fields_ = [("nplace", ctypes.c_size_t), ("reads", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
+                ("contig", ctypes.c_void_p), ("pos", ctypes.c_void_p), ("cigar", ctypes.c_void_p),
+                ("cig_off", ctypes.c_void_p), ("cig_n", ctypes.c_void_p), ("dir", ctypes.c_void_p)]
+
+
+class CWalkParams(ctypes.Structure):
+    _fields_ = [("known_bonus", ctypes.c_int32), ("gt_ag_bonus", ctypes.c_int32),
+                ("gc_ag_bonus", ctypes.c_int32), ("at_ac_bonus", ctypes.c_int32),
+                ("noncanonical_penalty", ctypes.c_int32), ("seed_k", ctypes.c_uint32)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -137,6 +151,12 @@ def lib():
                                          ctypes.c_int, vp, vp, vp]
     L.esp_unspliced_batch.argtypes = [vp, vp, P(CChainBatch), P(CParams), vp, vp, vp, vp]
     L.esp_spliced_batch.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp]
+    L.esp_junctions_upload.argtypes = [vp, sz, vp, P(vp)]
+    L.esp_junctions_free.argtypes = [vp, vp]
+    L.esp_walk_placements.argtypes = [vp, vp, vp, P(CWalkBatch), P(CParams), P(CWalkParams), vp, vp,
+                                      vp]
+    L.esp_intron_length_penalty.argtypes = [ctypes.c_uint32]
+    L.esp_intron_length_penalty.restype = ctypes.c_int32
     _lib = L
     return L
 

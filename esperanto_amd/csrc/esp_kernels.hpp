@@ -174,6 +174,36 @@ struct RefView {
     uint32_t ncontig;
 };
 
+// Placement walks (rescore_with, recount_mm_ea_in, exact_kmer_support).
+constexpr int PEN_UP = 64, PEN_DOWN = 32;
+struct WalkArgs {
+    RefView R;
+    const uint32_t *jn;  // sorted (contig, start, end) triples
+    uint32_t nj;
+    uint32_t n;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *pos;
+    const uint32_t *cigar;
+    const uint64_t *cig_off;
+    const uint32_t *cig_n;
+    const uint8_t *dir;
+    int32_t match, mismatch, gap_open, gap_ext;
+    int32_t known, gtag, gcag, atac, noncanon;
+    uint32_t k;
+    // intron_length_penalty thresholds: for len >= 1000 the penalty is the
+    // number of up[v] <= len; for len < 1000 the number of down[v] >= len.
+    uint32_t up[PEN_UP];
+    uint32_t down[PEN_DOWN];
+    int32_t nup, ndown;
+    esp_strict_score *strict;
+    esp_walk_counts *counts;
+    int32_t *status;
+};
+hipError_t launch_walk(const WalkArgs &a, int blocks, hipStream_t s);
+
 struct GatherArgs {
     RefView R;
     const esp_gather_job *jobs;

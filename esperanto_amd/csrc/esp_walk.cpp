@@ -1,0 +1,171 @@
+// esp_walk.cpp -- junction library upload and the placement-walk entry point
+// (rescore_with / recount_mm_ea_in / exact_kmer_support) behind the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/esp_align.h"
+#include "esp_ctx.hpp"
+#include "esp_kernels.hpp"
+
+using namespace esph;
+
+struct esp_junctions {
+    int device = 0;
+    uint32_t n = 0;
+    void *d = nullptr;
+};
+
+namespace {
+
+// intron_length_penalty (splice.rs:119-121) evaluated directly.
+int32_t penalty_formula(uint32_t len) {
+    return (int32_t)(8.0 * std::fabs(std::log10((double)std::max<uint32_t>(len, 1)) - 3.0));
+}
+
+// Threshold form of the penalty: it is non-increasing on [0, 1000] and
+// non-decreasing from 1000 up, so each side is a count of thresholds passed.
+struct PenaltyTable {
+    uint32_t up[espk::PEN_UP] = {};
+    uint32_t down[espk::PEN_DOWN] = {};
+    int nup = 0, ndown = 0;
+    PenaltyTable() {
+        // up[v-1]: smallest len >= 1000 with penalty >= v
+        for (int v = 1; v <= espk::PEN_UP && penalty_formula(0xFFFFFFFFu) >= v; v++) {
+            uint64_t lo = 1000, hi = 0xFFFFFFFFull;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (penalty_formula((uint32_t)mid) >= v) hi = mid;
+                else lo = mid + 1;
+            }
+            up[nup++] = (uint32_t)lo;
+        }
+        // down[v-1]: largest len <= 1000 with penalty >= v
+        for (int v = 1; v <= espk::PEN_DOWN && penalty_formula(0) >= v; v++) {
+            int64_t lo = 0, hi = 1000;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) / 2;
+                if (penalty_formula((uint32_t)mid) >= v) lo = mid;
+                else hi = mid - 1;
+            }
+            down[ndown++] = (uint32_t)lo;
+        }
+    }
+    int32_t eval(uint32_t len) const {
+        int32_t v = 0;
+        if (len >= 1000) {
+            for (int t = 0; t < nup; t++) v += up[t] <= len;
+        } else {
+            for (int t = 0; t < ndown; t++) v += down[t] >= len;
+        }
+        return v;
+    }
+};
+
+const PenaltyTable &penalty_table() {
+    static const PenaltyTable t;
+    return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t esp_intron_length_penalty(uint32_t len) { return penalty_table().eval(len); }
+
+int esp_junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, esp_junctions **out) {
+    if (!c) return ESP_E_INVAL;
+    if (!out || (n && !t)) return fail(c, ESP_E_INVAL, "NULL argument");
+    if (n > 0xFFFFFFFFull) return fail(c, ESP_E_INVAL, "too many junctions");
+    for (size_t j = 1; j < n; j++) {
+        const uint32_t *a = t + 3 * (j - 1), *b = t + 3 * j;
+        if (std::lexicographical_compare(b, b + 3, a, a + 3))
+            return fail(c, ESP_E_INVAL, "junctions are not sorted by (contig, start, end)");
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    esp_junctions *j = new esp_junctions;
+    j->device = c->device;
+    j->n = (uint32_t)n;
+    if (n) {
+        if (hipMalloc(&j->d, n * 12) != hipSuccess) {
+            delete j;
+            return fail(c, ESP_E_NOMEM, "junction library allocation failed");
+        }
+        if (hipMemcpy(j->d, t, n * 12, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(j->d);
+            delete j;
+            return hip_fail(c, hipErrorUnknown, "junction library upload");
+        }
+    }
+    *out = j;
+    return ESP_OK;
+}
+
+int esp_junctions_free(esp_ctx *c, esp_junctions *j) {
+    if (!j) return ESP_OK;
+    if (c) (void)hipSetDevice(j->device);
+    if (j->d) (void)hipFree(j->d);
+    delete j;
+    return ESP_OK;
+}
+
+int esp_walk_placements(esp_ctx *c, const esp_ref *r, const esp_junctions *jl, const esp_walk_batch *b,
+                        const esp_params *p, const esp_walk_params *wp, esp_strict_score *strict,
+                        esp_walk_counts *counts, int32_t *status) {
+    if (!c) return ESP_E_INVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!r || !b || !wp) return fail(c, ESP_E_INVAL, "NULL argument");
+    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
+    if (jl && jl->device != c->device) return fail(c, ESP_E_INVAL, "junctions live on another device");
+    if (b->nplace == 0) return ESP_OK;
+    if (b->nplace > 0x7FFFFFFFull) return fail(c, ESP_E_INVAL, "too many placements");
+    if (!b->reads || !b->read_off || !b->read_len || !b->contig || !b->pos || !b->cigar || !b->cig_off ||
+        !b->cig_n)
+        return fail(c, ESP_E_INVAL, "batch arrays are required");
+    HIP_TRY(c, hipSetDevice(c->device));
+    espk::WalkArgs a{};
+    a.R = r->view();
+    a.jn = jl ? (const uint32_t *)jl->d : nullptr;
+    a.nj = jl ? jl->n : 0;
+    a.n = (uint32_t)b->nplace;
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.contig = b->contig;
+    a.pos = b->pos;
+    a.cigar = b->cigar;
+    a.cig_off = b->cig_off;
+    a.cig_n = b->cig_n;
+    a.dir = b->dir;
+    a.match = p->match_score;
+    a.mismatch = p->mismatch;
+    a.gap_open = p->gap_open;
+    a.gap_ext = p->gap_ext;
+    a.known = wp->known_bonus;
+    a.gtag = wp->gt_ag_bonus;
+    a.gcag = wp->gc_ag_bonus;
+    a.atac = wp->at_ac_bonus;
+    a.noncanon = wp->noncanonical_penalty;
+    a.k = wp->seed_k;
+    const PenaltyTable &pt = penalty_table();
+    std::memcpy(a.up, pt.up, sizeof a.up);
+    std::memcpy(a.down, pt.down, sizeof a.down);
+    a.nup = pt.nup;
+    a.ndown = pt.ndown;
+    a.strict = strict;
+    a.counts = counts;
+    a.status = status;
+    const uint64_t blocks = std::min<uint64_t>((b->nplace + 255) / 256, (uint64_t)c->ncu * 8);
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_walk(a, (int)std::max<uint64_t>(blocks, 1), c->stream));
+    tim_mark(c, 8, 0);
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
+}  // extern "C"

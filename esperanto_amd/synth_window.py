@@ -114,3 +114,40 @@ def serving_workload(n_requests: int, ref_len: int = 256_000_000, seed: int = 0x
             req["hint"] = np.full(n, 32, dtype=np.int64)
         reqs.append(req)
     return reference, reqs
+
+
+def placement_walk_workload(npairs: int, ref_len: int = 64_000_000, read_len: int = 150,
+                            seed: int = 0xE5A00006, known_fraction: float = 0.5):
+    """Placements of the pair workload above, as the walks see them after the
+    DP: per pair, R1 spliced as uM (e-s)N (m-u)M at s-u, and R2 as mM at p2
+    with its query in the alignment's (forward) frame.  Returns (Reference,
+    Placements, junction triples) with the planted junctions of a
+    known_fraction of the pairs in the library."""
+    from .walk import Placements
+    ref, sp, mt, truth = splice_rescue_workload(npairs, ref_len, read_len, seed=seed)
+    m = read_len
+    s, e, u, p2 = (truth[k].astype(np.int64) for k in ("s", "e", "u", "p2"))
+    n = 2 * npairs
+    r2 = mt.reads.reshape(npairs, m)
+    q = np.empty((n, m), dtype=np.uint8)
+    q[0::2] = sp.reads.reshape(npairs, m)
+    q[1::2] = _COMP[r2[:, ::-1]]
+    M, N = 0, 3
+    cig = np.zeros((n, 3), dtype=np.uint32)
+    cig[0::2, 0] = (u << 4 | M).astype(np.uint32)
+    cig[0::2, 1] = ((e - s) << 4 | N).astype(np.uint32)
+    cig[0::2, 2] = ((m - u) << 4 | M).astype(np.uint32)
+    cig_n = np.tile(np.array([3, 1], dtype=np.uint32), npairs)
+    flat = np.concatenate([cig[0::2], np.stack([np.full(npairs, m << 4 | M, dtype=np.uint32)], 1)],
+                          axis=1).reshape(-1)
+    cig_off = np.zeros(n, dtype=np.uint64)
+    cig_off[1:] = np.cumsum(cig_n, dtype=np.uint64)[:-1]
+    pos = np.empty(n, dtype=np.uint32)
+    pos[0::2] = s - u
+    pos[1::2] = p2
+    pl = Placements(q.reshape(-1), np.arange(n, dtype=np.uint64) * m, np.full(n, m, dtype=np.uint32),
+                    np.zeros(n, dtype=np.uint32), pos, flat, cig_off, cig_n, np.zeros(n, dtype=np.uint8))
+    rng = np.random.default_rng(seed + 1)
+    pick = rng.random(npairs) < known_fraction
+    junctions = sorted(set(zip([0] * int(pick.sum()), s[pick].tolist(), e[pick].tolist())))
+    return ref, pl, junctions

@@ -165,7 +165,7 @@ int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
  * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
  * 6 selection / stitching, 7 extend_hint full-matrix rerun of escalated band
- * attempts), half (band kernel) and ms. */
+ * attempts, 8 placement walk kernel), half (band kernel) and ms. */
 typedef struct {
     int32_t kind;
     int32_t half;
@@ -375,6 +375,75 @@ int esp_window_myers_batch(esp_ctx *ctx, const esp_ref *ref, const esp_window_jo
                            size_t njobs, const uint8_t *reads, uint32_t max_pattern,
                            uint32_t max_text, int mode, int32_t *dist, uint64_t *pos,
                            int32_t *status);
+
+/* ---- placement walks ---------------------------------------------------------- */
+
+/* Annotated junctions (JunctionLib, crates/map/src/gtf.rs:120-169): n
+ * (contig, start, end) triples, sorted ascending as the library keeps them
+ * (host pointer; copied to the device and kept until esp_junctions_free).
+ * Unsorted input is rejected with ESP_E_INVAL. */
+typedef struct esp_junctions esp_junctions;
+int esp_junctions_upload(esp_ctx *ctx, size_t n, const uint32_t *triples, esp_junctions **out);
+int esp_junctions_free(esp_ctx *ctx, esp_junctions *j);
+
+/* The SpliceParams fields rescore_with reads (splice.rs:21-52; defaults 6,
+ * 4, 2, 1, 8) and the seed k-mer length of exact_kmer_support (SeedParams
+ * k, 15 in the reference's configurations). */
+typedef struct {
+    int32_t known_bonus;
+    int32_t gt_ag_bonus;
+    int32_t gc_ag_bonus;
+    int32_t at_ac_bonus;
+    int32_t noncanonical_penalty;
+    uint32_t seed_k;
+} esp_walk_params;
+
+/* StrictScore (align.rs:188-211). */
+typedef struct {
+    int32_t score;
+    uint32_t aligned, matches, mm, ea_ag, ea_tc, ins, del, clipped, n_splice;
+    int32_t splice_score;
+} esp_strict_score;
+
+/* recount_mm_ea_in's counts (align.rs:2473-2516) and exact_kmer_support's
+ * k-mer count (align.rs:1231-1288). */
+typedef struct {
+    uint32_t mm, ea_ag, ea_tc, kmers;
+} esp_walk_counts;
+
+/* Placements to walk: placement k is contig[k] at pos[k] with the BAM CIGAR
+ * cigar[cig_off[k] .. cig_off[k] + cig_n[k]) over the query bytes
+ * reads[read_off[k] .. + read_len[k]) in the alignment's frame (reverse
+ * complemented for minus placements, as the callers pass them).  dir[k] is
+ * the EditDir exempted (0 Off, 1 Ag, 2 Tc): rescore_with's `dir`, and the
+ * placement's ea_dir for the k-mer scan.  Device pointers. */
+typedef struct {
+    size_t nplace;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *pos;
+    const uint32_t *cigar;
+    const uint64_t *cig_off;
+    const uint32_t *cig_n;
+    const uint8_t *dir;
+} esp_walk_batch;
+
+/* Per placement: rescore_with (align.rs:1060-1163) into strict, and
+ * recount_mm_ea_in + exact_kmer_support into counts (either may be NULL).
+ * Reference bytes past a contig end and query bytes past the read read as
+ * N, as in the reference.  junctions may be NULL (an empty library).  A
+ * placement on a contig out of range gets all-zero records and
+ * status[k] = ESP_E_INVAL (status may be NULL).  Device pointers. */
+int esp_walk_placements(esp_ctx *ctx, const esp_ref *ref, const esp_junctions *junctions,
+                        const esp_walk_batch *b, const esp_params *p, const esp_walk_params *wp,
+                        esp_strict_score *strict, esp_walk_counts *counts, int32_t *status);
+
+/* intron_length_penalty (splice.rs:119-121) as the walk kernel evaluates
+ * it: from a threshold table built on the host with the reference's f64
+ * formula, so that device and host agree exactly. */
+int32_t esp_intron_length_penalty(uint32_t len);
 
 /* ---- serving loop ---------------------------------------------------------- */
 

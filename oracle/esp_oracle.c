@@ -14,7 +14,11 @@
  *   crates/map/src/fasta.rs     (Contig::decode_into / base / is_n)
  *   crates/map/src/align.rs     (mate_window_placement, revcomp_into,
  *                                continuous_extension_window, the EA-Myers
- *                                lane of extend_unspliced_chain)
+ *                                lane of extend_unspliced_chain, the
+ *                                placement walks rescore_with,
+ *                                recount_mm_ea_in, exact_kmer_support)
+ *   crates/map/src/splice.rs    (splice_signal, intron_length_penalty)
+ *   crates/map/src/gtf.rs       (JunctionLib::contains_any_strand)
  * plus two independent textbook checkers (local affine-gap DP max score over
  * a row-window set, semi-global EA edit distance) used to pin the
  * restatement where the reference has no literal expected outputs.
@@ -27,6 +31,7 @@
  * Result layout: CIGAR ops as BAM u32 (len << 4 | op), op M=0 I=1 D=2 N=3
  * S=4 (bamio/src/encode.rs:169-177).
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1320,4 +1325,283 @@ int espo_myers_batch(size_t npairs, const uint8_t *reads, const uint64_t *read_o
     free(jobs);
     free(th);
     return 0;
+}
+
+/* ====================================================== placement walks */
+
+/* SpliceParams fields used by rescore_with (splice.rs:21-52); defaults
+ * known 6, GT-AG 4, GC-AG 2, AT-AC 1, non-canonical 8. */
+typedef struct {
+    int32_t known_bonus;
+    int32_t gt_ag_bonus;
+    int32_t gc_ag_bonus;
+    int32_t at_ac_bonus;
+    int32_t noncanonical_penalty;
+} espo_splice_params;
+
+/* StrictScore (align.rs:188-211). */
+typedef struct {
+    int32_t score;
+    uint32_t aligned, matches, mm, ea_ag, ea_tc, ins, del, clipped, n_splice;
+    int32_t splice_score;
+} espo_strict;
+
+/* intron_length_penalty: splice.rs:119-121, 8 x |log10(max(len,1)) - 3| as
+ * i32 (f64 arithmetic, truncation toward zero). */
+int32_t espo_intron_length_penalty(uint32_t len) {
+    double v = 8.0 * fabs(log10((double)(len > 1 ? len : 1)) - 3.0);
+    return (int32_t)v;
+}
+
+static uint8_t upper(uint8_t b) { return (b >= 'a' && b <= 'z') ? (uint8_t)(b - 32) : b; }
+
+/* splice_signal(donor, acceptor, minus).score(sp): splice.rs:61-103. */
+int32_t espo_splice_signal_score(uint8_t d0, uint8_t d1, uint8_t a0, uint8_t a1, int minus,
+                                 const espo_splice_params *sp) {
+    d0 = upper(d0); d1 = upper(d1); a0 = upper(a0); a1 = upper(a1);
+    int sig; /* 0 GT-AG, 1 GC-AG, 2 AT-AC, 3 non-canonical */
+    if (!minus) {
+        if (d0 == 'G' && d1 == 'T' && a0 == 'A' && a1 == 'G') sig = 0;
+        else if (d0 == 'G' && d1 == 'C' && a0 == 'A' && a1 == 'G') sig = 1;
+        else if (d0 == 'A' && d1 == 'T' && a0 == 'A' && a1 == 'C') sig = 2;
+        else sig = 3;
+    } else if (d0 == 'C' && d1 == 'T' && a0 == 'A' && a1 == 'C') sig = 0;
+    else if (d0 == 'C' && d1 == 'T' && a0 == 'G' && a1 == 'C') sig = 1;
+    else if (d0 == 'G' && d1 == 'T' && a0 == 'A' && a1 == 'T') sig = 2;
+    else sig = 3;
+    switch (sig) {
+    case 0: return sp->gt_ag_bonus;
+    case 1: return sp->gc_ag_bonus;
+    case 2: return sp->at_ac_bonus;
+    default: return -sp->noncanonical_penalty;
+    }
+}
+
+/* JunctionLib::contains_any_strand (gtf.rs:165-169): binary search of
+ * (contig, start, end) in the sorted triples jn[3j .. 3j+2]. */
+static int jlib_contains(const uint32_t *jn, size_t nj, uint32_t contig, uint32_t start, uint32_t end) {
+    size_t lo = 0, hi = nj;
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        const uint32_t *t = jn + 3 * mid;
+        int c = t[0] != contig ? (t[0] < contig ? -1 : 1)
+              : t[1] != start  ? (t[1] < start ? -1 : 1)
+              : t[2] != end    ? (t[2] < end ? -1 : 1) : 0;
+        if (c == 0) return 1;
+        if (c < 0) lo = mid + 1;
+        else hi = mid;
+    }
+    return 0;
+}
+
+/* Reference byte at pos (N past the contig end): the decode_append reads of
+ * the walks, clamped at contig.len (align.rs:1082-1086, :2487-2492). */
+static uint8_t ref_byte(const espo_ref *R, uint32_t contig, uint32_t pos) {
+    return pos < R->len[contig] ? contig_base(R, contig, pos) : (uint8_t)'N';
+}
+
+/* Aligner::rescore_with (align.rs:1060-1163).  query is in the alignment's
+ * frame; bytes past its end read as N.  dir is the EditDir exempted. */
+int espo_rescore(const espo_ref *R, const uint8_t *query, size_t qlen, uint32_t contig,
+                 uint32_t pos, const uint32_t *cigar, size_t n_ops, int dir, const espo_params *ep,
+                 const espo_splice_params *sp, const uint32_t *jn, size_t nj, espo_strict *out) {
+    if (contig >= R->ncontig) return 1;
+    espo_strict s;
+    memset(&s, 0, sizeof s);
+    int32_t score = 0;
+    uint32_t ro = 0, rf = pos;
+    for (size_t o = 0; o < n_ops; o++) {
+        const uint32_t op = cigar[o] & 15u, n = cigar[o] >> 4;
+        if (op == OP_M) {
+            for (uint32_t k = 0; k < n; k++) {
+                const uint8_t qb = (size_t)ro + k < qlen ? query[ro + k] : (uint8_t)'N';
+                const uint8_t rb = ref_byte(R, contig, rf + k);
+                if (qb == 'N' || rb == 'N') continue; /* 0 score, uncounted */
+                s.aligned++;
+                if (qb == rb) {
+                    score += ep->match_score;
+                    s.matches++;
+                } else {
+                    if (!edit_exempt(dir, rb, qb)) score += ep->mismatch;
+                    const int e = exempt_dir(rb, qb);
+                    if (e == 1) s.ea_ag++;
+                    else if (e == 0) s.ea_tc++;
+                    else s.mm++;
+                }
+            }
+            ro += n;
+            rf += n;
+        } else if (op == OP_I) {
+            score -= ep->gap_open + ep->gap_ext * (int32_t)n;
+            s.aligned += n;
+            s.ins++;
+            ro += n;
+        } else if (op == OP_D) {
+            score -= ep->gap_open + ep->gap_ext * (int32_t)n;
+            s.aligned += n;
+            s.del++;
+            rf += n;
+        } else if (op == OP_S) {
+            s.clipped += n;
+            ro += n;
+        } else if (op == OP_N) {
+            const int32_t before = score;
+            if (jlib_contains(jn, nj, contig, rf, rf + n)) {
+                score += sp->known_bonus;
+            } else {
+                const uint32_t rf1 = rf == 0xFFFFFFFFu ? rf : rf + 1; /* saturating_add */
+                const uint8_t d0 = ref_byte(R, contig, rf), d1 = ref_byte(R, contig, rf1);
+                const uint8_t a0 = ref_byte(R, contig, rf + (n >= 2 ? n - 2 : 0));
+                const uint8_t a1 = ref_byte(R, contig, rf + (n >= 1 ? n - 1 : 0));
+                const int32_t sp_plus = espo_splice_signal_score(d0, d1, a0, a1, 0, sp);
+                const int32_t sp_minus = espo_splice_signal_score(d0, d1, a0, a1, 1, sp);
+                score += (sp_plus > sp_minus ? sp_plus : sp_minus) - espo_intron_length_penalty(n);
+            }
+            s.n_splice++;
+            s.splice_score += score - before;
+            rf += n;
+        }
+    }
+    s.score = score;
+    *out = s;
+    return 0;
+}
+
+/* recount_mm_ea_in (align.rs:2473-2516): (mm, ea_ag, ea_tc) over the M
+ * columns where neither byte is N and they differ. */
+int espo_recount(const espo_ref *R, const uint8_t *query, size_t qlen, uint32_t contig, uint32_t pos,
+                 const uint32_t *cigar, size_t n_ops, uint32_t out[3]) {
+    if (contig >= R->ncontig) return 1;
+    uint32_t ro = 0, rf = pos, mm = 0, ag = 0, tc = 0;
+    for (size_t o = 0; o < n_ops; o++) {
+        const uint32_t op = cigar[o] & 15u, n = cigar[o] >> 4;
+        if (op == OP_M) {
+            for (uint32_t k = 0; k < n; k++) {
+                const uint8_t qb = (size_t)ro + k < qlen ? query[ro + k] : (uint8_t)'N';
+                const uint8_t rb = ref_byte(R, contig, rf + k);
+                if (qb != 'N' && rb != 'N' && qb != rb) {
+                    const int e = exempt_dir(rb, qb);
+                    if (e == 1) ag++;
+                    else if (e == 0) tc++;
+                    else mm++;
+                }
+            }
+            ro += n;
+            rf += n;
+        } else if (op == OP_I || op == OP_S) {
+            ro += n;
+        } else if (op == OP_D || op == OP_N) {
+            rf += n;
+        }
+    }
+    out[0] = mm;
+    out[1] = ag;
+    out[2] = tc;
+    return 0;
+}
+
+/* Aligner::exact_kmer_support (align.rs:1231-1288): non-overlapping exact
+ * k-mers along the M blocks, scanned from each block's start (a k-mer with
+ * a bad column moves the scan one base on), exact in the reduced alphabet of
+ * dir (the placement's ea_dir). */
+uint32_t espo_kmer_support(const espo_ref *R, const uint8_t *query, size_t qlen, uint32_t contig,
+                           uint32_t pos, const uint32_t *cigar, size_t n_ops, int dir, uint32_t k) {
+    if (k == 0 || qlen == 0 || contig >= R->ncontig) return 0;
+    uint32_t count = 0, ro = 0, rf = pos;
+    for (size_t o = 0; o < n_ops; o++) {
+        const uint32_t op = cigar[o] & 15u, n = cigar[o] >> 4;
+        if (op == OP_M) {
+            uint32_t i = 0;
+            while ((uint64_t)i + k <= n) {
+                int ok = 1;
+                for (uint32_t t = 0; t < k; t++) {
+                    const uint8_t qb = (size_t)ro + i + t < qlen ? query[ro + i + t] : (uint8_t)'N';
+                    const uint8_t rb = ref_byte(R, contig, rf + i + t);
+                    if (qb != rb && !edit_exempt(dir, rb, qb)) {
+                        ok = 0;
+                        break;
+                    }
+                }
+                if (ok) {
+                    count++;
+                    i += k;
+                } else {
+                    i++;
+                }
+            }
+            ro += n;
+            rf += n;
+        } else if (op == OP_I || op == OP_S) {
+            ro += n;
+        } else if (op == OP_D || op == OP_N) {
+            rf += n;
+        }
+    }
+    return count;
+}
+
+/* The three walks over a batch of placements (nthreads POSIX threads): the
+ * CPU form of the device walk, used as the checker and the CPU baseline.
+ * counts[4k..4k+3] = (mm, ea_ag, ea_tc, k-mers). */
+typedef struct {
+    const espo_ref *R;
+    const uint8_t *reads;
+    const uint64_t *read_off, *cig_off;
+    const uint32_t *read_len, *contig, *pos, *cigar, *cig_n;
+    const uint8_t *dir;
+    const espo_params *ep;
+    const espo_splice_params *sp;
+    const uint32_t *jn;
+    size_t nj, lo, hi;
+    uint32_t k;
+    espo_strict *strict;
+    uint32_t *counts;
+    int32_t *status;
+} walk_job;
+
+static void *walk_worker(void *arg) {
+    walk_job *J = (walk_job *)arg;
+    for (size_t i = J->lo; i < J->hi; i++) {
+        const uint8_t *q = J->reads + J->read_off[i];
+        const uint32_t *c = J->cigar + J->cig_off[i];
+        const int d = J->dir ? J->dir[i] : 0;
+        espo_strict s;
+        memset(&s, 0, sizeof s);
+        int bad = espo_rescore(J->R, q, J->read_len[i], J->contig[i], J->pos[i], c, J->cig_n[i], d, J->ep,
+                               J->sp, J->jn, J->nj, &s);
+        J->strict[i] = s;
+        J->status[i] = bad;
+        uint32_t *o = J->counts + 4 * i;
+        o[0] = s.mm;
+        o[1] = s.ea_ag;
+        o[2] = s.ea_tc;
+        o[3] = bad ? 0 : espo_kmer_support(J->R, q, J->read_len[i], J->contig[i], J->pos[i], c, J->cig_n[i],
+                                           d, J->k);
+    }
+    return NULL;
+}
+
+void espo_walk_batch(const espo_ref *R, size_t n, const uint8_t *reads, const uint64_t *read_off,
+                     const uint32_t *read_len, const uint32_t *contig, const uint32_t *pos,
+                     const uint32_t *cigar, const uint64_t *cig_off, const uint32_t *cig_n,
+                     const uint8_t *dir, const espo_params *ep, const espo_splice_params *sp,
+                     const uint32_t *jn, size_t nj, uint32_t k, espo_strict *strict, uint32_t *counts,
+                     int32_t *status, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    walk_job *jobs = (walk_job *)calloc((size_t)nthreads, sizeof(walk_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    const size_t chunk = (n + (size_t)nthreads - 1) / (size_t)nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        walk_job *J = &jobs[t];
+        J->R = R; J->reads = reads; J->read_off = read_off; J->read_len = read_len;
+        J->contig = contig; J->pos = pos; J->cigar = cigar; J->cig_off = cig_off; J->cig_n = cig_n;
+        J->dir = dir; J->ep = ep; J->sp = sp; J->jn = jn; J->nj = nj; J->k = k;
+        J->strict = strict; J->counts = counts; J->status = status;
+        J->lo = (size_t)t * chunk < n ? (size_t)t * chunk : n;
+        J->hi = J->lo + chunk < n ? J->lo + chunk : n;
+        pthread_create(&th[t], NULL, walk_worker, J);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(jobs);
+    free(th);
 }

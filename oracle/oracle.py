@@ -28,6 +28,20 @@ class CRef(ctypes.Structure):
                 ("iv_off", ctypes.c_void_p)]
 
 
+class CSpliceParams(ctypes.Structure):
+    """The SpliceParams fields rescore_with reads (splice.rs:21-52)."""
+    _fields_ = [("known_bonus", ctypes.c_int32), ("gt_ag_bonus", ctypes.c_int32),
+                ("gc_ag_bonus", ctypes.c_int32), ("at_ac_bonus", ctypes.c_int32),
+                ("noncanonical_penalty", ctypes.c_int32)]
+
+
+SPLICE_DEFAULTS = (6, 4, 2, 1, 8)  # splice.rs:40-52: known, GT-AG, GC-AG, AT-AC, non-canonical
+
+STRICT_DTYPE = np.dtype([("score", "<i4"), ("aligned", "<u4"), ("matches", "<u4"), ("mm", "<u4"),
+                         ("ea_ag", "<u4"), ("ea_tc", "<u4"), ("ins", "<u4"), ("del", "<u4"),
+                         ("clipped", "<u4"), ("n_splice", "<u4"), ("splice_score", "<i4")])
+
+
 def build() -> None:
     import subprocess
     subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
@@ -81,6 +95,19 @@ def lib():
                                       ctypes.c_int]
         L.espo_spliced_batch.argtypes = [P(CRef), sz, vp, vp, vp, vp, vp, vp, vp, P(CParams), vp,
                                          vp, vp, vp, vp, vp, vp, ctypes.c_int]
+        L.espo_intron_length_penalty.argtypes = [ctypes.c_uint32]
+        L.espo_intron_length_penalty.restype = ctypes.c_int32
+        L.espo_splice_signal_score.argtypes = [ctypes.c_uint8] * 4 + [ctypes.c_int, P(CSpliceParams)]
+        L.espo_splice_signal_score.restype = ctypes.c_int32
+        L.espo_rescore.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, ctypes.c_uint32, u32p, sz,
+                                   ctypes.c_int, P(CParams), P(CSpliceParams), u32p, sz, vp]
+        L.espo_recount.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, ctypes.c_uint32, u32p, sz, u32p]
+        L.espo_kmer_support.argtypes = [P(CRef), u8p, sz, ctypes.c_uint32, ctypes.c_uint32, u32p, sz,
+                                        ctypes.c_int, ctypes.c_uint32]
+        L.espo_kmer_support.restype = ctypes.c_uint32
+        L.espo_walk_batch.argtypes = [P(CRef), sz] + [vp] * 9 + [P(CParams), P(CSpliceParams), vp, sz,
+                                                               ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
+        L.espo_walk_batch.restype = None
         _lib = L
     return _lib
 
@@ -406,3 +433,77 @@ def unspliced_batch(oref: OracleRef, req, params: Optional[ExtendParams] = None)
         recs.append(rec)
         cigs.append(cig)
     return recs, cigs
+
+
+# ------------------------------------------------------------ placement walks
+
+def intron_length_penalty(n: int) -> int:
+    """splice.rs:119 intron_length_penalty"""
+    return int(lib().espo_intron_length_penalty(n))
+
+
+def splice_signal_score(donor: bytes, acceptor: bytes, minus: bool, sp=SPLICE_DEFAULTS) -> int:
+    """splice.rs:78 splice_signal(..).score(&SpliceParams)"""
+    return int(lib().espo_splice_signal_score(donor[0], donor[1], acceptor[0], acceptor[1], int(minus),
+                                              ctypes.byref(CSpliceParams(*sp))))
+
+
+def _cigar_arr(cigar):
+    words = [int(w) for w in cigar]
+    return (ctypes.c_uint32 * max(len(words), 1))(*words), len(words)
+
+
+def _junction_arr(junctions):
+    flat = [int(x) for t in sorted(junctions or []) for x in t]
+    return (ctypes.c_uint32 * max(len(flat), 1))(*flat), len(flat) // 3
+
+
+def rescore(oref: OracleRef, query: bytes, contig: int, pos: int, cigar, dir: int = 0,
+            params: Optional[ExtendParams] = None, sp=SPLICE_DEFAULTS, junctions=None):
+    """align.rs:1060 Aligner::rescore_with -> StrictScore record (STRICT_DTYPE);
+    cigar is BAM words, junctions (contig, start, end) triples."""
+    q, m = _buf(query)
+    c, nc = _cigar_arr(cigar)
+    j, nj = _junction_arr(junctions)
+    out = np.zeros(1, dtype=STRICT_DTYPE)
+    rc = lib().espo_rescore(ctypes.byref(oref.c), q, m, contig, pos, c, nc, dir,
+                            ctypes.byref(_params(params)), ctypes.byref(CSpliceParams(*sp)), j, nj,
+                            out.ctypes.data)
+    assert rc == 0, rc
+    return out[0]
+
+
+def recount(oref: OracleRef, query: bytes, contig: int, pos: int, cigar):
+    """align.rs:2473 recount_mm_ea_in -> (mm, ea_ag, ea_tc)"""
+    q, m = _buf(query)
+    c, nc = _cigar_arr(cigar)
+    out = (ctypes.c_uint32 * 3)()
+    rc = lib().espo_recount(ctypes.byref(oref.c), q, m, contig, pos, c, nc, out)
+    assert rc == 0, rc
+    return tuple(out)
+
+
+def kmer_support(oref: OracleRef, query: bytes, contig: int, pos: int, cigar, dir: int = 0,
+                 k: int = 15) -> int:
+    """align.rs:1231 Aligner::exact_kmer_support (k from SeedParams, 15 by default)"""
+    q, m = _buf(query)
+    c, nc = _cigar_arr(cigar)
+    return int(lib().espo_kmer_support(ctypes.byref(oref.c), q, len(query), contig, pos, c, nc, dir, k))
+
+
+def walk_batch(oref: OracleRef, pl, params: Optional[ExtendParams] = None, sp=SPLICE_DEFAULTS,
+               junctions=None, k: int = 15, nthreads: int = 1):
+    """rescore_with + recount + exact_kmer_support over a Placements batch
+    (esperanto_amd.walk.Placements): (STRICT_DTYPE records, (n, 4) counts
+    mm/ea_ag/ea_tc/k-mers, status)."""
+    n = pl.n
+    arrs = [np.ascontiguousarray(getattr(pl, f)) for f in
+            ("reads", "read_off", "read_len", "contig", "pos", "cigar", "cig_off", "cig_n", "dir")]
+    flat = np.array([x for t in sorted(junctions or []) for x in t] or [0], dtype=np.uint32)
+    strict = np.zeros(max(n, 1), dtype=STRICT_DTYPE)
+    counts = np.zeros((max(n, 1), 4), dtype=np.uint32)
+    status = np.zeros(max(n, 1), dtype=np.int32)
+    lib().espo_walk_batch(ctypes.byref(oref.c), n, *(a.ctypes.data for a in arrs), ctypes.byref(_params(params)),
+                          ctypes.byref(CSpliceParams(*sp)), flat.ctypes.data, len(flat) // 3 if junctions else 0,
+                          k, strict.ctypes.data, counts.ctypes.data, status.ctypes.data, nthreads)
+    return strict[:n], counts[:n], status[:n]
