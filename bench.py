@@ -143,6 +143,20 @@ def bench_window(args, buf, params):
     return out
 
 
+def _copy_calibration(buf, src, nbytes):
+    """One plain device copy of nbytes (at most src's size): the yardstick that
+    the FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    import ctypes
+    from esperanto_amd import _native
+    from esperanto_amd.batch import DeviceArray
+    nbytes = min(nbytes, src.nbytes)
+    dst = DeviceArray(buf, nbytes)
+    _native.check(_native.lib().esp_device_copy(buf.handle, ctypes.c_void_p(dst.ptr),
+                                                ctypes.c_void_p(src.ptr), nbytes), buf.handle)
+    log(f"copy calibration: {nbytes} bytes")
+    dst.free()
+
+
 def bench_walk(args, buf, params):
     """The three placement walks (rescore_with, recount_mm_ea_in,
     exact_kmer_support) over the placements of the pair workload: per pair,
@@ -168,6 +182,8 @@ def bench_walk(args, buf, params):
            "kernel_ms": round(walk_ms, 3), "algorithmic_bytes": nb,
            "kernel_gb_per_s": round(nb / (walk_ms / 1e3) / 1e9, 1), "junctions": len(jn)}
     log(f"walks: {pl.n / w / 1e6:.2f} M placements/s, kernel {walk_ms:.3f} ms")
+    if args.copy_bytes:
+        _copy_calibration(buf, db.d["reads"], args.copy_bytes)
     db.free()
     dj.close()
     if not args.no_cpu:
@@ -395,16 +411,7 @@ def main() -> int:
         f"band {band_ms:.2f} {per_half}, strip {strip_ms:.3f}), {gcups:.1f} GCUPS")
     pairs_per_s = hb.npairs / wall
     if args.copy_bytes:
-        from esperanto_amd import _native
-        import ctypes
-        src = db.d["wins"]
-        nbytes = min(args.copy_bytes, src.nbytes)
-        from esperanto_amd.batch import DeviceArray
-        dst = DeviceArray(buf, nbytes)
-        _native.check(_native.lib().esp_device_copy(buf.handle, ctypes.c_void_p(dst.ptr),
-                                                    ctypes.c_void_p(src.ptr), nbytes), buf.handle)
-        log(f"copy calibration: {nbytes} bytes")
-        dst.free()
+        _copy_calibration(buf, db.d["wins"], args.copy_bytes)
     db.free()
     if args.only_gotoh:
         print(json.dumps({"gcups": round(gcups, 2), "band_ms": round(band_ms, 3),

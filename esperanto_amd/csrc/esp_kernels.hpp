@@ -163,8 +163,9 @@ int myers_blocks_per_cu(int words);
 // ---- resident reference and window pipelines (esp_window.hip) ----------
 
 // Device view of the resident 2-bit reference.  Contig k's packed bytes
-// start at packed + pk_off[k] and are followed by at least 8 zero bytes, so
-// a 5-byte window read at any in-range base stays inside the allocation.
+// start at packed + pk_off[k] (16-byte aligned) and are followed by at least
+// 16 zero bytes, so a 12-byte window read at any in-range base stays inside
+// the contig's region.
 struct RefView {
     const uint8_t *packed;
     const uint64_t *pk_off;
@@ -175,11 +176,19 @@ struct RefView {
 };
 
 // Placement walks (rescore_with, recount_mm_ea_in, exact_kmer_support).
+// Slot hash of a junction triple in the library's open-addressing set.
+__host__ __device__ inline uint32_t junction_hash(uint32_t c, uint32_t s, uint32_t e) {
+    uint32_t h = c * 0x9E3779B1u ^ s * 0x85EBCA77u ^ e * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h;
+}
 constexpr int PEN_UP = 64, PEN_DOWN = 32;
 struct WalkArgs {
     RefView R;
-    const uint32_t *jn;  // sorted (contig, start, end) triples
-    uint32_t nj;
+    const uint4 *jn;     // junction hash set slots (x contig, y start, z end, w used); NULL if none
+    uint32_t jmask;      // slots - 1
     uint32_t n;
     const uint8_t *reads;
     const uint64_t *read_off;
@@ -203,6 +212,7 @@ struct WalkArgs {
     int32_t *status;
 };
 hipError_t launch_walk(const WalkArgs &a, int blocks, hipStream_t s);
+int walk_blocks_per_cu();
 
 struct GatherArgs {
     RefView R;

@@ -13,9 +13,14 @@
 
 using namespace esph;
 
+// The library as an open-addressing hash set of its distinct triples: cap
+// (a power of two, at least four times the count) slots of (contig, start,
+// end, used), probed linearly from espk::junction_hash.  Membership is what
+// contains_any_strand asks of the sorted library, one probe or two per query.
 struct esp_junctions {
     int device = 0;
-    uint32_t n = 0;
+    uint32_t n = 0;    // distinct triples
+    uint32_t cap = 0;  // slots
     void *d = nullptr;
 };
 
@@ -79,22 +84,38 @@ int32_t esp_intron_length_penalty(uint32_t len) { return penalty_table().eval(le
 int esp_junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, esp_junctions **out) {
     if (!c) return ESP_E_INVAL;
     if (!out || (n && !t)) return fail(c, ESP_E_INVAL, "NULL argument");
-    if (n > 0xFFFFFFFFull) return fail(c, ESP_E_INVAL, "too many junctions");
-    for (size_t j = 1; j < n; j++) {
-        const uint32_t *a = t + 3 * (j - 1), *b = t + 3 * j;
-        if (std::lexicographical_compare(b, b + 3, a, a + 3))
-            return fail(c, ESP_E_INVAL, "junctions are not sorted by (contig, start, end)");
+    if (n > (1ull << 30)) return fail(c, ESP_E_INVAL, "too many junctions");
+    uint64_t cap = 16;
+    while (cap < 4 * (uint64_t)n) cap <<= 1;
+    std::vector<uint32_t> slots(4 * cap, 0);
+    uint32_t distinct = 0;
+    for (size_t k = 0; k < n; k++) {
+        const uint32_t *x = t + 3 * k;
+        uint64_t s = espk::junction_hash(x[0], x[1], x[2]) & (cap - 1);
+        while (slots[4 * s + 3]) {
+            if (slots[4 * s] == x[0] && slots[4 * s + 1] == x[1] && slots[4 * s + 2] == x[2]) break;
+            s = (s + 1) & (cap - 1);
+        }
+        if (!slots[4 * s + 3]) {
+            slots[4 * s] = x[0];
+            slots[4 * s + 1] = x[1];
+            slots[4 * s + 2] = x[2];
+            slots[4 * s + 3] = 1;
+            distinct++;
+        }
     }
     HIP_TRY(c, hipSetDevice(c->device));
     esp_junctions *j = new esp_junctions;
     j->device = c->device;
-    j->n = (uint32_t)n;
-    if (n) {
-        if (hipMalloc(&j->d, n * 12) != hipSuccess) {
+    j->n = distinct;
+    j->cap = (uint32_t)cap;
+    if (distinct) {
+        const size_t bytes = slots.size() * sizeof(uint32_t);
+        if (hipMalloc(&j->d, bytes) != hipSuccess) {
             delete j;
             return fail(c, ESP_E_NOMEM, "junction library allocation failed");
         }
-        if (hipMemcpy(j->d, t, n * 12, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpy(j->d, slots.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(j->d);
             delete j;
             return hip_fail(c, hipErrorUnknown, "junction library upload");
@@ -129,8 +150,8 @@ int esp_walk_placements(esp_ctx *c, const esp_ref *r, const esp_junctions *jl, c
     HIP_TRY(c, hipSetDevice(c->device));
     espk::WalkArgs a{};
     a.R = r->view();
-    a.jn = jl ? (const uint32_t *)jl->d : nullptr;
-    a.nj = jl ? jl->n : 0;
+    a.jn = jl && jl->n ? (const uint4 *)jl->d : nullptr;
+    a.jmask = jl && jl->n ? jl->cap - 1 : 0;
     a.n = (uint32_t)b->nplace;
     a.reads = b->reads;
     a.read_off = b->read_off;
@@ -159,7 +180,10 @@ int esp_walk_placements(esp_ctx *c, const esp_ref *r, const esp_junctions *jl, c
     a.strict = strict;
     a.counts = counts;
     a.status = status;
-    const uint64_t blocks = std::min<uint64_t>((b->nplace + 255) / 256, (uint64_t)c->ncu * 8);
+    // one resident generation of blocks: each lane then walks its share of
+    // the placements in rounds, with no partial last generation
+    const uint64_t blocks =
+        std::min<uint64_t>((b->nplace + 255) / 256, (uint64_t)c->ncu * (uint64_t)espk::walk_blocks_per_cu());
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     tim_begin(c);
     HIP_TRY(c, espk::launch_walk(a, (int)std::max<uint64_t>(blocks, 1), c->stream));

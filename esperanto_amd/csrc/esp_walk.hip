@@ -1,18 +1,34 @@
 // esp_walk.hip -- placement walks over the resident reference.
 //
 // One lane per placement walks its CIGAR against the 2-bit reference and the
-// query, in the reference's order (align.rs:1060-1163 rescore_with,
-// :2473-2516 recount_mm_ea_in, :1231-1288 exact_kmer_support), in one pass:
-//   * M columns: a reference byte from a per-lane cursor (a cached dword of
-//     16 packed bases, and a forward cursor over the contig's sorted N
-//     intervals, so no per-base binary search), a query byte from a cached
-//     dword of the read; strict score and counts per column; the k-mer scan
-//     as a streaming window (a bad column restarts the window after it,
-//     which is where the reference's one-step rescan first succeeds);
-//   * I/D/S: the gap and clip bookkeeping;
-//   * N: the junction library (binary search of sorted triples), else the
-//     donor/acceptor signal on both strands minus the intron-length penalty
-//     (a host-built threshold table, exact against the f64 formula).
+// query (align.rs:1060-1163 rescore_with, :2473-2516 recount_mm_ea_in,
+// :1231-1288 exact_kmer_support) in one pass.
+//
+// A lane runs a flat event loop: each iteration consumes one segment of up to
+// 32 columns of its current M op, after first handling every I/D/S/N op that
+// precedes the next segment.  The query and reference words of the next
+// segment are loaded before the current segment is scored, so every wave has
+// a segment's work in flight over each load's latency.  Columns are scored
+// four at a time in byte lanes of a dword:
+//   * reference bases come from the resident 2-bit contig through a 256-entry
+//     LDS table (packed byte -> four ASCII bytes) and v_alignbyte to the
+//     segment's base offset; N intervals and bases past the contig end are
+//     substituted on a slow path that only segments meeting one take;
+//   * the query's four bytes come from aligned dwords and v_alignbyte; bytes
+//     past the query end read as N;
+//   * byte equality is exact zero-byte detection on q ^ r, and the strict
+//     counts are popcounts of those byte masks.  The strict score is rebuilt
+//     from the counts at the end (it is linear in them: match x matches +
+//     mismatch x non-exempt substitutions - gap costs + splice terms);
+//   * exact_kmer_support's scan (restart one column on at a failing k-mer)
+//     counts the same k-mers as a run length over good columns that counts one
+//     k-mer and restarts each time k good columns accumulate; four columns at a
+//     time this is a first-bad / last-bad update when k >= 4, and a per-column
+//     loop otherwise.
+// RefSkips look the junction library up in its device hash set, else score
+// the donor / acceptor signal on both
+// strands minus the intron-length penalty, read from a host-built threshold
+// table (exact against the f64 formula) held in LDS.
 #include "esp_kernels.hpp"
 
 using namespace espd;
@@ -22,67 +38,27 @@ namespace espk {
 namespace {
 
 constexpr uint32_t OPM = 0, OPI = 1, OPD = 2, OPN = 3, OPS = 4;
+// Columns per segment; NQ query dwords and NR reference dwords hold one.
+constexpr uint32_t SEG = 32, NG = SEG / 4, NQ = NG + 1, NR = SEG / 16 + 1;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3), aligned(4)));
+constexpr uint32_t N4 = 0x4E4E4E4Eu, A4 = 0x41414141u, C4 = 0x43434343u, G4 = 0x47474747u,
+                   T4 = 0x54545454u, HI = 0x80808080u;
 
-// Reference bytes of one contig read mostly in ascending position order.
-struct RefCursor {
-    const uint32_t *pk;  // the contig's packed bytes as dwords (16-byte aligned)
-    const uint32_t *iv;  // (start, end) of its N intervals
-    uint32_t len, niv;
-    uint32_t ci = 0;            // first interval whose end > the last position
-    uint32_t last = 0;
-    uint32_t wi = 0xFFFFFFFFu;  // cached dword index
-    uint32_t w = 0;
+// 0x80 in every byte of x that is zero, 0 in the others (exact for all bytes).
+__device__ __forceinline__ uint32_t zbytes(uint32_t x) {
+    const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return ~(t | x) & HI;
+}
 
-    __device__ void seek(uint32_t p) {
-        uint32_t lo = 0, hi = niv;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (iv[2 * mid + 1] <= p) lo = mid + 1;
-            else hi = mid;
-        }
-        ci = lo;
-        last = p;
-    }
-    // ASCII base at p ('N' inside an N interval or past the contig end).
-    __device__ uint32_t get(uint32_t p) {
-        if (p >= len) return 'N';
-        if (p < last) seek(p);
-        while (ci < niv && iv[2 * ci + 1] <= p) ci++;
-        last = p;
-        if (ci < niv && iv[2 * ci] <= p) return 'N';
-        const uint32_t d = p >> 4;
-        if (d != wi) {
-            wi = d;
-            w = pk[d];
-        }
-        const uint32_t byte = (w >> (8 * ((p >> 2) & 3u))) & 0xFFu;
-        const uint32_t code = (byte >> (6 - 2 * (p & 3u))) & 3u;
-        return (0x54474341u >> (8 * code)) & 0xFFu;  // "ACGT"
-    }
-};
+// Four bytes of the eight {hi, lo} starting at byte sh (0..3).
+__device__ __forceinline__ uint32_t align4(uint32_t hi, uint32_t lo, uint32_t sh) {
+    return __builtin_amdgcn_alignbyte(hi, lo, sh);
+}
 
-// Query bytes read in ascending order through a cached aligned dword; the
-// dword holds the byte asked for, so no load leaves the read's pages.
-struct QueryCursor {
-    const uint8_t *q;
-    uint32_t len;
-    uintptr_t wa = 0;
-    uint32_t w = 0;
-    __device__ uint32_t get(uint32_t i) {
-        if (i >= len) return 'N';
-        const uintptr_t addr = (uintptr_t)(q + i);
-        const uintptr_t a = addr & ~(uintptr_t)3;
-        if (a != wa) {
-            wa = a;
-            w = *(const uint32_t *)a;
-        }
-        return (w >> (8 * (uint32_t)(addr & 3u))) & 0xFFu;
-    }
-};
-
-// exempt_dir (extend.rs:86-93): 1 ref A / read G, 0 ref T / read C, -1 else.
-__device__ __forceinline__ int exempt_dir_b(uint32_t rb, uint32_t qb) {
-    return (rb == 'A' && qb == 'G') ? 1 : (rb == 'T' && qb == 'C') ? 0 : -1;
+// Byte mask of the first c (clamped to 0..4) bytes of a dword.
+__device__ __forceinline__ uint32_t first_bytes(int32_t c) {
+    return c >= 4 ? 0xFFFFFFFFu : c <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * c));
 }
 
 __device__ __forceinline__ uint32_t up_ascii(uint32_t b) { return (b >= 'a' && b <= 'z') ? b - 32 : b; }
@@ -100,130 +76,455 @@ __device__ int32_t signal_score(const WalkArgs &a, uint32_t d0, uint32_t d1, uin
     return sig == 0 ? a.gtag : sig == 1 ? a.gcag : sig == 2 ? a.atac : -a.noncanon;
 }
 
-__device__ int32_t intron_penalty(const WalkArgs &a, uint32_t len) {
-    int32_t v = 0;
-    if (len >= 1000) {
-        for (int t = 0; t < a.nup; t++) v += a.up[t] <= len;
-    } else {
-        for (int t = 0; t < a.ndown; t++) v += a.down[t] >= len;
-    }
-    return v;
-}
-
-__device__ bool junction_known(const WalkArgs &a, uint32_t c, uint32_t s, uint32_t e) {
-    uint32_t lo = 0, hi = a.nj;
+// Number of entries of the ascending t[0..n) that are <= v.
+__device__ __forceinline__ int32_t count_le(const uint32_t *t, int32_t n, uint32_t v) {
+    int32_t lo = 0, hi = n;
     while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t *t = a.jn + 3 * (size_t)mid;
-        const bool less = t[0] != c ? t[0] < c : (t[1] != s ? t[1] < s : t[2] < e);
-        if (!less && t[0] == c && t[1] == s && t[2] == e) return true;
-        if (less) lo = mid + 1;
+        const int32_t mid = (lo + hi) >> 1;
+        if (t[mid] <= v) lo = mid + 1;
         else hi = mid;
     }
-    return false;
+    return lo;
+}
+
+// Number of entries of the descending t[0..n) that are >= v.
+__device__ __forceinline__ int32_t count_ge(const uint32_t *t, int32_t n, uint32_t v) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (t[mid] >= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// contains_any_strand (gtf.rs:165-169): probe the library's hash set from
+// the triple's slot (v0 holds that slot, loaded ahead) until the triple or an
+// empty slot (the set is at most a quarter full, so an empty slot exists).
+__device__ bool junction_known(const WalkArgs &a, uint4 v, uint32_t c, uint32_t s, uint32_t e) {
+    uint32_t slot = junction_hash(c, s, e) & a.jmask;
+    while (true) {
+        if (!v.w) return false;
+        if (v.x == c && v.y == s && v.z == e) return true;
+        slot = (slot + 1) & a.jmask;
+        v = a.jn[slot];
+    }
+}
+
+// First N interval of the contig whose end is > p.
+__device__ __forceinline__ uint32_t iv_seek(const uint32_t *iv, uint32_t niv, uint32_t p) {
+    uint32_t lo = 0, hi = niv;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (iv[2 * mid + 1] <= p) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ASCII reference base at p given the dword pk[p >> 4]: N inside an N
+// interval or past the contig end.
+__device__ uint32_t ref_base(uint32_t word, const uint32_t *iv, uint32_t niv, uint32_t len, uint32_t p) {
+    if (p >= len) return 'N';
+    if (niv) {
+        const uint32_t j = iv_seek(iv, niv, p);
+        if (j < niv && iv[2 * j] <= p) return 'N';
+    }
+    const uint32_t byte = (word >> (8 * ((p >> 2) & 3u))) & 0xFFu;
+    const uint32_t code = (byte >> (6 - 2 * (p & 3u))) & 3u;
+    return (0x54474341u >> (8 * code)) & 0xFFu;
+}
+
+struct Lane {
+    // placement
+    uint32_t idx;
+    const uint32_t *qw;  // the query's bytes as aligned dwords
+    uint32_t qoff0;      // byte offset of the query's first byte in qw[0]
+    uint32_t qlen, qjmax;
+    const uint32_t *pk;  // the contig's packed bases as dwords
+    const uint32_t *iv;
+    uint32_t niv, len, pkmax, contig;
+    const uint32_t *cig;
+    uint32_t nops, o, rem, ro, rf;
+    uint32_t nw;  // cig[o] when o < nops (loaded one op ahead)
+    // for a RefSkip that follows the current M op, loaded when the M op
+    // starts: its library slot and the dwords of its four signal bases
+    bool pre;
+    uint4 pslot;
+    uint32_t pw[4];
+    uint32_t exag, extc;  // all-ones when dir exempts A>G / T>C
+    bool fresh;           // the next segment starts an M op
+    // N-interval cursor: first interval whose end > the last scored position
+    uint32_t ci, ivs, ive;
+    // tallies
+    uint32_t matches, subs, ag, tc, kmers, run;
+};
+
+// Tallies that only non-M ops change, kept in LDS per lane (column
+// threadIdx.x): I/D lengths, I count, D count, clipped, RefSkip count, gap
+// cost, RefSkip score.
+enum { T_GAPLEN, T_INS, T_DEL, T_CLIP, T_NSPL, T_GAPS, T_SPL, NT };
+typedef uint32_t Tally[NT][256];
+
+__device__ __forceinline__ void iv_load(Lane &L) {
+    if (L.ci < L.niv) {
+        L.ivs = L.iv[2 * L.ci];
+        L.ive = L.iv[2 * L.ci + 1];
+    } else {
+        L.ivs = L.ive = 0xFFFFFFFFu;
+    }
+}
+
+// Library slot and signal-base dwords of a RefSkip of length n at rf.
+__device__ __forceinline__ void refskip_load(const WalkArgs &a, const Lane &L, uint32_t rf, uint32_t n,
+                                             uint4 &slot, uint32_t (&w)[4]) {
+    slot = a.jn ? a.jn[junction_hash(L.contig, rf, rf + n) & a.jmask] : make_uint4(0, 0, 0, 0);
+    const uint32_t last = L.pkmax + NR - 1;
+    const uint32_t p1 = rf == 0xFFFFFFFFu ? rf : rf + 1;
+    w[0] = L.pk[min(rf >> 4, last)];
+    w[1] = L.pk[min(p1 >> 4, last)];
+    w[2] = L.pk[min((rf + (n >= 2 ? n - 2 : 0u)) >> 4, last)];
+    w[3] = L.pk[min((rf + (n >= 1 ? n - 1 : 0u)) >> 4, last)];
+}
+
+// Handles every non-M op up to the next M op of nonzero length (rem > 0) or
+// the end of the CIGAR (rem == 0).  Entering an M op that a RefSkip follows
+// issues that RefSkip's loads.
+__device__ void advance(const WalkArgs &a, Lane &L, const uint32_t *s_up, const uint32_t *s_down, Tally &T) {
+    const uint32_t me = threadIdx.x;
+    while (L.rem == 0 && L.o < L.nops) {
+        const uint32_t w = L.nw;
+        L.o++;
+        if (L.o < L.nops) L.nw = L.cig[L.o];
+        const uint32_t op = w & 15u, n = w >> 4;
+        if (op == OPM) {
+            L.rem = n;
+            L.fresh = true;
+            if (n && L.o < L.nops && (L.nw & 15u) == OPN) {
+                refskip_load(a, L, L.rf + n, L.nw >> 4, L.pslot, L.pw);
+                L.pre = true;
+            }
+        } else if (op == OPI || op == OPD) {
+            T[T_GAPS][me] += (uint32_t)(a.gap_open + a.gap_ext * (int32_t)n);
+            T[T_GAPLEN][me] += n;
+            if (op == OPI) {
+                T[T_INS][me]++;
+                L.ro += n;
+            } else {
+                T[T_DEL][me]++;
+                L.rf += n;
+            }
+        } else if (op == OPS) {
+            T[T_CLIP][me] += n;
+            L.ro += n;
+        } else if (op == OPN) {
+            const uint32_t rf = L.rf;
+            uint4 slot;
+            uint32_t wd[4];
+            if (L.pre) {
+                slot = L.pslot;
+                for (int i = 0; i < 4; i++) wd[i] = L.pw[i];
+                L.pre = false;
+            } else {
+                refskip_load(a, L, rf, n, slot, wd);
+            }
+            int32_t d;
+            if (a.jn && junction_known(a, slot, L.contig, rf, rf + n)) {
+                d = a.known;
+            } else {
+                const uint32_t p1 = rf == 0xFFFFFFFFu ? rf : rf + 1;
+                const uint32_t p2 = rf + (n >= 2 ? n - 2 : 0u), p3 = rf + (n >= 1 ? n - 1 : 0u);
+                const uint32_t d0 = ref_base(wd[0], L.iv, L.niv, L.len, rf);
+                const uint32_t d1 = ref_base(wd[1], L.iv, L.niv, L.len, p1);
+                const uint32_t a0 = ref_base(wd[2], L.iv, L.niv, L.len, p2);
+                const uint32_t a1 = ref_base(wd[3], L.iv, L.niv, L.len, p3);
+                const int32_t sp = signal_score(a, d0, d1, a0, a1, false);
+                const int32_t sm = signal_score(a, d0, d1, a0, a1, true);
+                const int32_t pen = n >= 1000 ? count_le(s_up, a.nup, n) : count_ge(s_down, a.ndown, n);
+                d = max(sp, sm) - pen;
+            }
+            T[T_SPL][me] += (uint32_t)d;
+            T[T_NSPL][me]++;
+            L.rf += n;
+        }
+    }
+}
+
+// The NQ query dwords and NR reference dwords of the segment at (ro, rf).
+// Query dwords past the query's last are clamped to it (their bytes read as
+// N); reference dwords stay inside the contig's region, which holds at least
+// 16 zero bytes after its packed bases.
+__device__ __forceinline__ void seg_load(const Lane &L, uint32_t (&q)[NQ], uint32_t (&r)[NR]) {
+    if (L.qlen) {
+        const uint32_t j0 = (L.qoff0 + L.ro) >> 2;
+        if (j0 + NQ - 1 <= L.qjmax) {
+            const u32x4 v0 = *(const u32x4 *)(L.qw + j0);
+            const u32x4 v1 = *(const u32x4 *)(L.qw + j0 + 4);
+            q[0] = v0.x, q[1] = v0.y, q[2] = v0.z, q[3] = v0.w;
+            q[4] = v1.x, q[5] = v1.y, q[6] = v1.z, q[7] = v1.w;
+            q[8] = L.qw[j0 + 8];
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < NQ; j++) q[j] = L.qw[min(j0 + j, L.qjmax)];
+        }
+    }
+    const u32x3 v = *(const u32x3 *)(L.pk + min(L.rf >> 4, L.pkmax));
+    r[0] = v.x, r[1] = v.y, r[2] = v.z;
+}
+
+// A placement's own fields, loaded one placement ahead; the fields of its
+// contig and its first CIGAR word follow once those have arrived.
+struct Meta {
+    uint64_t read_off, cig_off;
+    uint32_t contig, read_len, pos, cig_n, dir;
+    uint64_t pk_off, iv0, iv1;
+    uint32_t len, w0;
+};
+
+__device__ __forceinline__ void meta_load2(const WalkArgs &a, Meta &m) {
+    if (m.contig < a.R.ncontig) {
+        m.pk_off = a.R.pk_off[m.contig];
+        m.len = a.R.len[m.contig];
+        m.iv0 = a.R.iv_off[m.contig];
+        m.iv1 = a.R.iv_off[m.contig + 1];
+        m.w0 = m.cig_n ? a.cigar[m.cig_off] : 0u;
+    }
+}
+
+__device__ __forceinline__ void meta_load(const WalkArgs &a, uint32_t idx, Meta &m) {
+    m.contig = a.contig[idx];
+    m.read_off = a.read_off[idx];
+    m.read_len = a.read_len[idx];
+    m.pos = a.pos[idx];
+    m.cig_off = a.cig_off[idx];
+    m.cig_n = a.cig_n[idx];
+    m.dir = a.dir ? a.dir[idx] : 0u;
+}
+
+__device__ void place_begin(const WalkArgs &a, Lane &L, uint32_t idx, const Meta &m, const uint32_t *s_up,
+                            const uint32_t *s_down, Tally &T) {
+    L.idx = idx;
+    L.contig = m.contig;
+    L.qlen = m.read_len;
+    const uintptr_t qa = (uintptr_t)(a.reads + m.read_off);
+    L.qw = (const uint32_t *)(qa & ~(uintptr_t)3);
+    L.qoff0 = (uint32_t)(qa & 3u);
+    L.qjmax = L.qlen ? (L.qoff0 + L.qlen - 1) >> 2 : 0;
+    L.pk = (const uint32_t *)(a.R.packed + m.pk_off);
+    L.len = m.len;
+    // first dword of the last NR the contig's region holds: the upload gives
+    // it ceil(len / 4) + 16 bytes, rounded up to 16
+    L.pkmax = ((((L.len + 3u) >> 2) + 16u + 15u) & ~15u) / 4u - NR;
+    L.iv = a.R.iv + 2 * m.iv0;
+    L.niv = (uint32_t)(m.iv1 - m.iv0);
+    L.cig = a.cigar + m.cig_off;
+    L.nops = m.cig_n;
+    L.exag = m.dir == 1 ? 0xFFFFFFFFu : 0u;
+    L.extc = m.dir == 2 ? 0xFFFFFFFFu : 0u;
+    L.o = L.rem = L.ro = 0;
+    L.nw = m.w0;
+    L.rf = m.pos;
+    L.fresh = false;
+    L.pre = false;
+    L.ci = L.niv ? iv_seek(L.iv, L.niv, L.rf) : 0u;
+    iv_load(L);
+    L.matches = L.subs = L.ag = L.tc = L.kmers = L.run = 0;
+    for (int f = 0; f < NT; f++) T[f][threadIdx.x] = 0;
+    advance(a, L, s_up, s_down, T);
+}
+
+__device__ void place_end(const WalkArgs &a, const Lane &L, const Tally &T) {
+    const uint32_t me = threadIdx.x;
+    esp_strict_score s;
+    const uint32_t ex = (L.ag & L.exag) | (L.tc & L.extc);
+    s.score = (int32_t)((uint32_t)(a.match * (int32_t)L.matches) + (uint32_t)(a.mismatch * (int32_t)(L.subs - ex)) -
+                        T[T_GAPS][me] + T[T_SPL][me]);
+    s.aligned = L.matches + L.subs + T[T_GAPLEN][me];
+    s.matches = L.matches;
+    s.mm = L.subs - L.ag - L.tc;
+    s.ea_ag = L.ag;
+    s.ea_tc = L.tc;
+    s.ins = T[T_INS][me];
+    s.del = T[T_DEL][me];
+    s.clipped = T[T_CLIP][me];
+    s.n_splice = T[T_NSPL][me];
+    s.splice_score = (int32_t)T[T_SPL][me];
+    esp_walk_counts c;
+    c.mm = s.mm;
+    c.ea_ag = L.ag;
+    c.ea_tc = L.tc;
+    c.kmers = (a.k && L.qlen) ? L.kmers : 0u;
+    if (a.strict) a.strict[L.idx] = s;
+    if (a.counts) a.counts[L.idx] = c;
+    if (a.status) a.status[L.idx] = 0;
+}
+
+// Byte mask of the set bits of a 4-bit mask.
+__device__ __forceinline__ uint32_t nibble_bytes(uint32_t m4) {
+    return 0xFFu * ((m4 & 1u) | ((m4 & 2u) << 7) | ((m4 & 4u) << 14) | ((m4 & 8u) << 21));
+}
+
+// Scores one segment: ncol columns at query index qa, reference position p.
+__device__ __forceinline__ void seg_score(const WalkArgs &a, Lane &L, const uint32_t *lut,
+                                          const uint32_t (&q)[NQ], const uint32_t (&r)[NR], uint32_t qa,
+                                          uint32_t p, uint32_t ncol, bool fresh) {
+    // reference: NG + 1 packed bytes from byte (p >> 2) on, as ASCII groups
+    const uint32_t bo = (p >> 2) & 3u;
+    uint32_t A[NG + 1];
+#pragma unroll
+    for (uint32_t i = 0; i + 1 < NR; i++) {
+        const uint32_t W = align4(r[i + 1], r[i], bo);
+#pragma unroll
+        for (uint32_t t = 0; t < 4; t++) A[4 * i + t] = lut[(W >> (8 * t)) & 0xFFu];
+    }
+    A[NG] = lut[(r[NR - 1] >> (8 * bo)) & 0xFFu];
+    // N intervals and the contig end: the cursor moves to the first interval
+    // ending after p; a segment that meets an interval or the end marks its
+    // reference N columns in nref
+    while (L.ci < L.niv && L.ive <= p) {
+        L.ci++;
+        iv_load(L);
+    }
+    uint32_t nref = 0;
+    if (p + ncol > L.len || L.ivs < p + ncol) {
+        for (uint32_t c = 0; c < ncol; c++) {
+            const uint32_t x = p + c;
+            while (L.ci < L.niv && L.ive <= x) {
+                L.ci++;
+                iv_load(L);
+            }
+            if (x >= L.len || L.ivs <= x) nref |= 1u << c;
+        }
+    }
+    // query bytes from qlen on read as N
+    const int32_t qv = (int32_t)(L.qlen > qa ? L.qlen - qa : 0u);
+    const bool slow = nref != 0 || qa + ncol > L.qlen;
+    const uint32_t qsh = (L.qoff0 + qa) & 3u, rsh = p & 3u;
+    const uint32_t k = a.k;
+    uint32_t run = fresh ? 0u : L.run;
+#pragma unroll
+    for (uint32_t t = 0; t < NG; t++) {
+        const uint32_t cm = first_bytes((int32_t)ncol - 4 * (int32_t)t) & HI;
+        uint32_t q4 = L.qlen ? align4(q[t + 1], q[t], qsh) : N4;
+        uint32_t r4 = align4(A[t + 1], A[t], rsh);
+        uint32_t rnh = 0;  // 0x80 in the bytes whose reference base is N
+        if (slow) {
+            const uint32_t qm = first_bytes(qv - 4 * (int32_t)t);
+            const uint32_t rn = nibble_bytes((nref >> (4 * t)) & 15u);
+            q4 = (q4 & qm) | (N4 & ~qm);
+            r4 = (r4 & ~rn) | (N4 & rn);
+            rnh = rn & HI;
+        }
+        const uint32_t eq = zbytes(q4 ^ r4);
+        const uint32_t valid = cm & ~(zbytes(q4 ^ N4) | rnh);
+        const uint32_t agm = zbytes((r4 ^ A4) | (q4 ^ G4)) & cm;
+        const uint32_t tcm = zbytes((r4 ^ T4) | (q4 ^ C4)) & cm;
+        L.matches += __builtin_popcount(eq & valid);
+        L.subs += __builtin_popcount(valid & ~eq);
+        L.ag += __builtin_popcount(agm);
+        L.tc += __builtin_popcount(tcm);
+        const uint32_t bh = ~((eq | (agm & L.exag) | (tcm & L.extc)) & cm) & HI;
+        if (k >= 4) {
+            // run of good columns: the ones before the first bad column may
+            // complete a k-mer (at most one, k >= 4); after a bad column the
+            // run is the good columns after the last one
+            const uint32_t f = bh ? ((uint32_t)__builtin_ctz(bh) >> 3) : 4u;
+            run += f;
+            const bool hit = run >= k;
+            L.kmers += hit ? 1u : 0u;
+            run = bh ? ((uint32_t)__builtin_clz(bh) >> 3) : (hit ? run - k : run);
+        } else if (k) {
+            for (uint32_t c = 0; c < 4; c++) {
+                if ((bh >> (8 * c + 7)) & 1u) {
+                    run = 0;
+                } else if (++run == k) {
+                    L.kmers++;
+                    run = 0;
+                }
+            }
+        }
+    }
+    L.run = run;
 }
 
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
+    __shared__ uint32_t lut[256];
+    __shared__ uint32_t s_up[PEN_UP];
+    __shared__ uint32_t s_down[PEN_DOWN];
+    __shared__ Tally T;
+    {
+        // packed byte (four bases, first in bits 7:6) -> four ASCII bytes
+        const uint32_t b = threadIdx.x;
+        uint32_t v = 0;
+        for (int t = 0; t < 4; t++) v |= ((0x54474341u >> (8 * ((b >> (6 - 2 * t)) & 3u))) & 0xFFu) << (8 * t);
+        lut[b] = v;
+        if (b < PEN_UP) s_up[b] = a.up[b];
+        if (b < PEN_DOWN) s_down[b] = a.down[b];
+    }
+    __syncthreads();
+
+    // Rounds: every lane of a wave begins its next placement together, with
+    // that placement's fields loaded during the previous one, so a wave pays
+    // each placement's set-up latency once rather than once per lane.
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < a.n; idx += stride) {
-        esp_strict_score s = {};
-        esp_walk_counts cnt = {};
-        const uint32_t contig = a.contig[idx];
-        if (contig >= a.R.ncontig) {
-            if (a.strict) a.strict[idx] = s;
-            if (a.counts) a.counts[idx] = cnt;
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    Meta m;
+    if (idx < a.n) {
+        meta_load(a, idx, m);
+        meta_load2(a, m);
+    }
+    uint32_t q[NQ], r[NR];
+#pragma unroll
+    for (uint32_t j = 0; j < NQ; j++) q[j] = N4;
+#pragma unroll
+    for (uint32_t j = 0; j < NR; j++) r[j] = 0;
+    for (; idx < a.n; idx += stride) {
+        const Meta cur = m;
+        if (idx + stride < a.n) meta_load(a, idx + stride, m);
+        if (cur.contig >= a.R.ncontig) {
+            // a placement on a contig out of range gets zeroed records
+            if (a.strict) a.strict[idx] = esp_strict_score{};
+            if (a.counts) a.counts[idx] = esp_walk_counts{};
             if (a.status) a.status[idx] = ESP_E_INVAL;
             continue;
         }
-        RefCursor ref;
-        ref.pk = (const uint32_t *)(a.R.packed + a.R.pk_off[contig]);
-        ref.iv = a.R.iv + 2 * a.R.iv_off[contig];
-        ref.niv = (uint32_t)(a.R.iv_off[contig + 1] - a.R.iv_off[contig]);
-        ref.len = a.R.len[contig];
-        QueryCursor q;
-        q.q = a.reads + a.read_off[idx];
-        q.len = a.read_len[idx];
-        const int dir = a.dir ? a.dir[idx] : 0;
-        const uint32_t k = a.k;
-        uint32_t rf = a.pos[idx], ro = 0;
-        ref.seek(rf);
-        int32_t score = 0;
-        const uint32_t *cig = a.cigar + a.cig_off[idx];
-        const uint32_t nops = a.cig_n[idx];
-        for (uint32_t o = 0; o < nops; o++) {
-            const uint32_t op = cig[o] & 15u, n = cig[o] >> 4;
-            if (op == OPM) {
-                uint32_t kstart = 0;  // candidate k-mer start inside this block
-                for (uint32_t x = 0; x < n; x++) {
-                    const uint32_t qb = q.get(ro + x), rb = ref.get(rf + x);
-                    const int ed = exempt_dir_b(rb, qb);
-                    const bool exempt = (dir == 1 && ed == 1) || (dir == 2 && ed == 0);
-                    if (qb != 'N' && rb != 'N') {
-                        s.aligned++;
-                        if (qb == rb) {
-                            score += a.match;
-                            s.matches++;
-                        } else {
-                            if (!exempt) score += a.mismatch;
-                            if (ed == 1) s.ea_ag++;
-                            else if (ed == 0) s.ea_tc++;
-                            else s.mm++;
-                        }
-                    }
-                    // k-mer scan: exact in the reduced alphabet of dir
-                    if (k) {
-                        if (qb != rb && !exempt) {
-                            kstart = x + 1;
-                        } else if (x + 1 - kstart == k) {
-                            cnt.kmers++;
-                            kstart = x + 1;
-                        }
-                    }
-                }
-                ro += n;
-                rf += n;
-            } else if (op == OPI) {
-                score -= a.gap_open + a.gap_ext * (int32_t)n;
-                s.aligned += n;
-                s.ins++;
-                ro += n;
-            } else if (op == OPD) {
-                score -= a.gap_open + a.gap_ext * (int32_t)n;
-                s.aligned += n;
-                s.del++;
-                rf += n;
-            } else if (op == OPS) {
-                s.clipped += n;
-                ro += n;
-            } else if (op == OPN) {
-                const int32_t before = score;
-                if (junction_known(a, contig, rf, rf + n)) {
-                    score += a.known;
-                } else {
-                    const uint32_t d0 = ref.get(rf), d1 = ref.get(rf == 0xFFFFFFFFu ? rf : rf + 1);
-                    const uint32_t a0 = ref.get(rf + (n >= 2 ? n - 2 : 0u));
-                    const uint32_t a1 = ref.get(rf + (n >= 1 ? n - 1 : 0u));
-                    const int32_t sp = signal_score(a, d0, d1, a0, a1, false);
-                    const int32_t sm = signal_score(a, d0, d1, a0, a1, true);
-                    score += max(sp, sm) - intron_penalty(a, n);
-                }
-                s.n_splice++;
-                s.splice_score += score - before;
-                rf += n;
+        Lane L;
+        place_begin(a, L, idx, cur, s_up, s_down, T);
+        if (L.rem) seg_load(L, q, r);
+        bool second = idx + stride < a.n;  // the next placement's contig fields are due
+        while (L.rem) {
+            if (second) {
+                meta_load2(a, m);
+                second = false;
             }
+            uint32_t cq[NQ], cr[NR];
+#pragma unroll
+            for (uint32_t j = 0; j < NQ; j++) cq[j] = q[j];
+#pragma unroll
+            for (uint32_t j = 0; j < NR; j++) cr[j] = r[j];
+            const uint32_t ncol = min(L.rem, SEG), qa = L.ro, p = L.rf;
+            const bool fresh = L.fresh;
+            L.fresh = false;
+            L.rem -= ncol;
+            L.ro += ncol;
+            L.rf += ncol;
+            if (L.rem == 0) advance(a, L, s_up, s_down, T);
+            if (L.rem) seg_load(L, q, r);
+            seg_score(a, L, lut, cq, cr, qa, p, ncol, fresh);
         }
-        s.score = score;
-        cnt.mm = s.mm;
-        cnt.ea_ag = s.ea_ag;
-        cnt.ea_tc = s.ea_tc;
-        if (!q.len) cnt.kmers = 0;  // exact_kmer_support of an empty query
-        if (a.strict) a.strict[idx] = s;
-        if (a.counts) a.counts[idx] = cnt;
-        if (a.status) a.status[idx] = 0;
+        if (second) meta_load2(a, m);
+        place_end(a, L, T);
     }
 }
 
 }  // namespace
+
+int walk_blocks_per_cu() {
+    static int nb = 0;
+    if (!nb && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk, 256, 0) != hipSuccess || nb < 1))
+        nb = 1;
+    return nb;
+}
 
 hipError_t launch_walk(const WalkArgs &a, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(256), 0, s, a);

@@ -69,12 +69,12 @@ int esp_ref_upload(esp_ctx *c, size_t n_contigs, const uint32_t *len, const uint
     }
     HIP_TRY(c, hipSetDevice(c->device));
     // Device layout: each contig's packed bytes 16-byte aligned and followed
-    // by at least 8 zero bytes.
+    // by at least 16 zero bytes.
     std::vector<uint64_t> doff(n_contigs);
     uint64_t total = 0;
     for (size_t k = 0; k < n_contigs; k++) {
         doff[k] = total;
-        total += align_up(((uint64_t)len[k] + 3) / 4 + 8, 16);
+        total += align_up(((uint64_t)len[k] + 3) / 4 + 16, 16);
     }
     std::vector<uint8_t> host(total, 0);
     for (size_t k = 0; k < n_contigs; k++)

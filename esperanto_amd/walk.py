@@ -54,11 +54,12 @@ def intron_length_penalty(n: int) -> int:
 
 
 class DeviceJunctions:
-    """JunctionLib::contains_any_strand's sorted (contig, start, end) triples, resident."""
+    """A JunctionLib's (contig, start, end) triples (any order, duplicates
+    allowed), resident as the device hash set contains_any_strand is asked of."""
 
     def __init__(self, triples: Sequence[Tuple[int, int, int]], buf: ExtendBuffer):
         self.buf = buf
-        t = np.ascontiguousarray(np.asarray(sorted(triples), dtype=np.uint32).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(list(triples), dtype=np.uint32).reshape(-1, 3))
         h = ctypes.c_void_p()
         _native.check(_native.lib().esp_junctions_upload(buf.handle, t.shape[0],
                                                          ctypes.c_void_p(t.ctypes.data) if t.size else None,

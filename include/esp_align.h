@@ -379,9 +379,10 @@ int esp_window_myers_batch(esp_ctx *ctx, const esp_ref *ref, const esp_window_jo
 /* ---- placement walks ---------------------------------------------------------- */
 
 /* Annotated junctions (JunctionLib, crates/map/src/gtf.rs:120-169): n
- * (contig, start, end) triples, sorted ascending as the library keeps them
- * (host pointer; copied to the device and kept until esp_junctions_free).
- * Unsorted input is rejected with ESP_E_INVAL. */
+ * (contig, start, end) triples in any order, duplicates allowed (host
+ * pointer).  The device keeps them as a hash set until esp_junctions_free;
+ * the walk asks it what contains_any_strand (gtf.rs:165-169) asks of the
+ * sorted library: is (contig, start, end) present. */
 typedef struct esp_junctions esp_junctions;
 int esp_junctions_upload(esp_ctx *ctx, size_t n, const uint32_t *triples, esp_junctions **out);
 int esp_junctions_free(esp_ctx *ctx, esp_junctions *j);

@@ -4,9 +4,10 @@ C ABI: every StrictScore field and count on random placements mixing
 M/I/D/N/S over references with N runs and soft-masked runs, queries with
 substitutions, N and lower-case bytes, positions past the contig end,
 queries shorter than their CIGAR, all three EditDirs, known junctions from
-an uploaded library, custom scoring, k = 0 / 5 / 15, empty queries and
-CIGARs, and an out-of-range contig; then the reference's own known answers
-(align.rs:4216-4317) through the device entry point."""
+an uploaded library (shuffled, with duplicates), custom scoring, k = 0 / 2 / 5 / 15 (k < 4 takes the
+per-column k-mer path), empty queries and CIGARs, and an out-of-range
+contig; then the reference's own known answers (align.rs:4216-4317) through
+the device entry point."""
 from __future__ import annotations
 
 import sys
@@ -82,10 +83,12 @@ def test_walk_matches_oracle(buf, seed, scoring):
     o = oracle.OracleRef(ref)
     dref = DeviceReference(ref, buf)
     jl_triples = _junctions(rng, cs, ps, cigs)
-    jl = DeviceJunctions(jl_triples, buf)
+    # the device set takes the triples in any order, duplicates included
+    shuffled = [jl_triples[i] for i in rng.permutation(len(jl_triples))] + jl_triples[: len(jl_triples) // 3]
+    jl = DeviceJunctions(shuffled, buf)
     pl = Placements.from_lists(qs, cs, ps, cigs, dirs)
     try:
-        for k in (15, 5, 0):
+        for k in (15, 5, 2, 0):
             wp = WalkParams(*sp, seed_k=k)
             say(f"walk seed {seed} {scoring} k {k}: {pl.n} placements, {len(jl_triples)} junctions")
             strict, counts, status = walk_batch(dref, pl, ep, wp, jl)
