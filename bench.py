@@ -185,6 +185,25 @@ def bench_walk(args, buf, params):
     if args.copy_bytes:
         _copy_calibration(buf, db.d["reads"], args.copy_bytes)
     db.free()
+    # The Kadane fast lane over the R2 placements as chains (anchors every 15
+    # bases on the true diagonal, the last one at m - 15).
+    from esperanto_amd.fast_lane import DeviceFastBatch
+    r2 = np.arange(1, pl.n, 2)
+    m = int(pl.read_len[1])
+    qp = np.unique(np.append(np.arange(0, m - 15 + 1, 15), m - 15)).astype(np.int64)
+    na = qp.shape[0]
+    anchors = np.stack([np.broadcast_to(qp, (r2.shape[0], na)),
+                        pl.pos[r2].astype(np.int64)[:, None] + qp[None, :]], axis=2)
+    fb = DeviceFastBatch(dref, pl.reads, pl.read_off[r2], pl.read_len[r2], pl.contig[r2], anchors,
+                         np.arange(r2.shape[0], dtype=np.uint64) * na, np.full(r2.shape[0], na, np.uint32))
+    wf, _, tf = timed_steps(lambda: fb.launch(params), fb, max(args.steps, 3), 1)
+    fast_ms = float(np.mean([sum(t[2] for t in st if t[0] == 9) for st in tf]))
+    rec, _ = fb.results()
+    out["fast_lane"] = {"chains": int(r2.shape[0]), "chains_per_s": round(r2.shape[0] / wf),
+                        "kernel_ms": round(fast_ms, 3), "accepted": round(float((rec["status"] == 1).mean()), 4)}
+    log(f"fast lane: {r2.shape[0] / wf / 1e6:.1f} M chains/s, kernel {fast_ms:.3f} ms, "
+        f"accepted {out['fast_lane']['accepted']}")
+    fb.free()
     dj.close()
     if not args.no_cpu:
         from oracle import oracle

@@ -27,7 +27,7 @@ EXPORTS = [
     "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
     "esp_unspliced_batch", "esp_window_myers_batch", "esp_junctions_upload", "esp_junctions_free",
-    "esp_walk_placements", "esp_intron_length_penalty",
+    "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
 ]
 
 
@@ -84,6 +84,13 @@ class CWalkParams(ctypes.Structure):
     _fields_ = [("known_bonus", ctypes.c_int32), ("gt_ag_bonus", ctypes.c_int32),
                 ("gc_ag_bonus", ctypes.c_int32), ("at_ac_bonus", ctypes.c_int32),
                 ("noncanonical_penalty", ctypes.c_int32), ("seed_k", ctypes.c_uint32)]
+
+
+class CFastBatch(ctypes.Structure):
+    _fields_ = [("nchain", ctypes.c_size_t), ("reads", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
+                ("contig", ctypes.c_void_p), ("anchors", ctypes.c_void_p),
+                ("anc_off", ctypes.c_void_p), ("anc_n", ctypes.c_void_p)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -157,6 +164,8 @@ def lib():
                                       vp]
     L.esp_intron_length_penalty.argtypes = [ctypes.c_uint32]
     L.esp_intron_length_penalty.restype = ctypes.c_int32
+    L.esp_fast_lane_batch.argtypes = [vp, vp, P(CFastBatch), P(CParams), ctypes.c_uint32, ctypes.c_uint32,
+                                      vp, vp]
     _lib = L
     return L
 

@@ -213,7 +213,7 @@ class ContextTimings:
     def last_timings(self):
         """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,
         3 Myers, 5 window preparation, 6 selection / stitching, 7 full-matrix
-        rerun of escalated extend_hint attempts, 8 placement walks."""
+        rerun of escalated extend_hint attempts, 8 placement walks, 9 fast lane."""
         arr = (_native.CTiming * 16)()
         n = ctypes.c_size_t()
         _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),

@@ -214,6 +214,24 @@ struct WalkArgs {
 hipError_t launch_walk(const WalkArgs &a, int blocks, hipStream_t s);
 int walk_blocks_per_cu();
 
+// try_fast_lane over chains (anchors as (qpos, rpos) pairs).
+struct FastArgs {
+    RefView R;
+    uint32_t n;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *anchors;
+    const uint64_t *anc_off;
+    const uint32_t *anc_n;
+    uint32_t k, w;
+    int32_t match, mismatch, edit_dir;
+    esp_fast_rec *out;
+    uint32_t *cigar;  // 3 words per chain
+};
+hipError_t launch_fast_lane(const FastArgs &a, int blocks, hipStream_t s);
+
 struct GatherArgs {
     RefView R;
     const esp_gather_job *jobs;

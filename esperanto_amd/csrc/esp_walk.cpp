@@ -192,4 +192,43 @@ int esp_walk_placements(esp_ctx *c, const esp_ref *r, const esp_junctions *jl, c
     return ESP_OK;
 }
 
+int esp_fast_lane_batch(esp_ctx *c, const esp_ref *r, const esp_fast_batch *b, const esp_params *p, uint32_t k,
+                        uint32_t w, esp_fast_rec *out, uint32_t *cigar) {
+    if (!c) return ESP_E_INVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!r || !b) return fail(c, ESP_E_INVAL, "NULL argument");
+    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
+    if (b->nchain == 0) return ESP_OK;
+    if (b->nchain > 0x7FFFFFFFull) return fail(c, ESP_E_INVAL, "too many chains");
+    if (!b->reads || !b->read_off || !b->read_len || !b->contig || !b->anchors || !b->anc_off || !b->anc_n ||
+        !out || !cigar)
+        return fail(c, ESP_E_INVAL, "batch arrays are required");
+    HIP_TRY(c, hipSetDevice(c->device));
+    espk::FastArgs a{};
+    a.R = r->view();
+    a.n = (uint32_t)b->nchain;
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.contig = b->contig;
+    a.anchors = b->anchors;
+    a.anc_off = b->anc_off;
+    a.anc_n = b->anc_n;
+    a.k = k;
+    a.w = w;
+    a.match = p->match_score;
+    a.mismatch = p->mismatch;
+    a.edit_dir = p->edit_dir;
+    a.out = out;
+    a.cigar = cigar;
+    const uint64_t blocks = std::min<uint64_t>((b->nchain + 255) / 256, (uint64_t)c->ncu * 8);
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_fast_lane(a, (int)std::max<uint64_t>(blocks, 1), c->stream));
+    tim_mark(c, 9, 0);
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
 }  // extern "C"

@@ -517,6 +517,165 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     }
 }
 
+// ------------------------------------------------------------ fast lane
+
+// try_fast_lane (align.rs:2274-2401), one lane per chain: the anchor
+// conditions, then Kadane along the chain diagonal four columns at a time
+// (column scores from exact byte masks as in the walks; the running sum's
+// updates stay per column), then the zero-mismatch gate and the at most
+// three-op CIGAR.
+__global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
+    __shared__ uint32_t lut[256];
+    {
+        const uint32_t b = threadIdx.x;
+        uint32_t v = 0;
+        for (int t = 0; t < 4; t++) v |= ((0x54474341u >> (8 * ((b >> (6 - 2 * t)) & 3u))) & 0xFFu) << (8 * t);
+        lut[b] = v;
+    }
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < a.n; idx += stride) {
+        esp_fast_rec rec = {};
+        const uint32_t contig = a.contig[idx];
+        if (contig >= a.R.ncontig) {
+            rec.status = -1;
+            a.out[idx] = rec;
+            continue;
+        }
+        const uint32_t m = a.read_len[idx];
+        const uint32_t na = a.anc_n[idx];
+        const uint32_t *anc = a.anchors + 2 * a.anc_off[idx];
+        // conditions 2-4 (align.rs:2284-2305)
+        bool ok = na > 0;
+        int64_t diag = 0;
+        uint32_t qstart = 0, qend = 0, rs = 0;
+        if (ok) {
+            qstart = anc[0];
+            rs = anc[1];
+            diag = (int64_t)rs - (int64_t)qstart;
+            uint32_t prevq = qstart;
+            for (uint32_t j = 1; j < na && ok; j++) {
+                const uint32_t q = anc[2 * j], r = anc[2 * j + 1];
+                ok = (int64_t)r - (int64_t)q == diag && q - prevq <= a.k + a.w + 5u;
+                prevq = q;
+            }
+            qend = prevq + a.k;
+            ok = ok && qstart <= 2 && m - qend <= 2;
+        }
+        if (!ok) {
+            a.out[idx] = rec;
+            continue;
+        }
+        const uint32_t flank = m / 2 + 30;
+        const uint32_t wstart = rs > flank ? rs - flank : 0u;
+        const uint32_t *pk = (const uint32_t *)(a.R.packed + a.R.pk_off[contig]);
+        const uint32_t clen = a.R.len[contig];
+        const uint32_t *iv = a.R.iv + 2 * a.R.iv_off[contig];
+        const uint32_t niv = (uint32_t)(a.R.iv_off[contig + 1] - a.R.iv_off[contig]);
+        const uintptr_t qaddr = (uintptr_t)(a.reads + a.read_off[idx]);
+        const uint32_t *qw = (const uint32_t *)(qaddr & ~(uintptr_t)3);
+        const uint32_t qoff0 = (uint32_t)(qaddr & 3u);
+        const uint32_t qjmax = m ? (qoff0 + m - 1) >> 2 : 0u;
+        // dwords the contig's region holds: ceil(len / 4) + 16 bytes, rounded up to 16
+        const uint32_t pklast = ((((clen + 3u) >> 2) + 16u + 15u) & ~15u) / 4u - 1u;
+        const uint32_t exag = a.edit_dir == 1 ? 0xFFFFFFFFu : 0u, extc = a.edit_dir == 2 ? 0xFFFFFFFFu : 0u;
+        uint32_t ci = 0;
+        {
+            const int64_t p0 = diag > 0 ? diag : 0;
+            ci = niv ? iv_seek(iv, niv, (uint32_t)min(p0, (int64_t)0xFFFFFFFF)) : 0u;
+        }
+        int32_t best = 0, cur = 0;
+        uint32_t bs = 0, be = 0, cs = 0, cmm = 0, bmm = 0;
+        for (uint32_t i0 = 0; i0 < m; i0 += 16) {
+            const uint32_t ncol = min(m - i0, 16u);
+            const int64_t p0 = (int64_t)i0 + diag;
+            // query: five aligned dwords (clamped to the query's last)
+            uint32_t q[5];
+            const uint32_t j0 = (qoff0 + i0) >> 2;
+#pragma unroll
+            for (uint32_t j = 0; j < 5; j++) q[j] = qw[min(j0 + j, qjmax)];
+            const uint32_t qsh = (qoff0 + i0) & 3u;
+            // reference: the LUT path when the 16 bases lie inside the contig
+            // and clear of N intervals, else base by base
+            while (ci < niv && (int64_t)iv[2 * ci + 1] <= p0) ci++;
+            const bool inside = p0 >= 0 && p0 + 16 <= (int64_t)clen;
+            const bool clear = ci >= niv || (int64_t)iv[2 * ci] >= p0 + 16;
+            uint32_t r4s[4];
+            if (inside && clear) {
+                const uint32_t p = (uint32_t)p0;
+                const uint32_t d = p >> 4, bo = (p >> 2) & 3u;
+                const uint32_t r0 = pk[d], r1 = pk[min(d + 1, pklast)];
+                const uint32_t W = align4(r1, r0, bo);
+                uint32_t A[5];
+#pragma unroll
+                for (int t = 0; t < 4; t++) A[t] = lut[(W >> (8 * t)) & 0xFFu];
+                A[4] = lut[(r1 >> (8 * bo)) & 0xFFu];
+#pragma unroll
+                for (int t = 0; t < 4; t++) r4s[t] = align4(A[t + 1], A[t], p & 3u);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    uint32_t v = 0;
+                    for (int c = 0; c < 4; c++) {
+                        const int64_t x = p0 + 4 * t + c;
+                        uint32_t b = 'N';
+                        if (x >= 0 && x < (int64_t)clen)
+                            b = ref_base(pk[min((uint32_t)x >> 4, pklast)], iv, niv, clen, (uint32_t)x);
+                        v |= b << (8 * c);
+                    }
+                    r4s[t] = v;
+                }
+            }
+#pragma unroll
+            for (uint32_t t = 0; t < 4; t++) {
+                const uint32_t q4 = align4(q[t + 1], q[t], qsh), r4 = r4s[t];
+                const uint32_t eq = zbytes(q4 ^ r4);
+                const uint32_t nb = zbytes(q4 ^ N4) | zbytes(r4 ^ N4);
+                const uint32_t ex = (zbytes((r4 ^ A4) | (q4 ^ G4)) & exag) | (zbytes((r4 ^ T4) | (q4 ^ C4)) & extc);
+                for (uint32_t c = 0; c < 4; c++) {
+                    const uint32_t i = i0 + 4 * t + c;
+                    if (i >= i0 + ncol) break;
+                    const uint32_t bit = 8 * c + 7;
+                    const bool isn = (nb >> bit) & 1u, ise = (eq >> bit) & 1u, isx = (ex >> bit) & 1u;
+                    const int32_t s = isn ? 0 : ise ? a.match : isx ? 0 : a.mismatch;
+                    const uint32_t mm = (!isn && !ise && !isx) ? 1u : 0u;
+                    // Kadane with <= 0 reset (align.rs:2346-2366)
+                    if (cur <= 0) {
+                        cur = s;
+                        cs = i;
+                        cmm = mm;
+                    } else {
+                        cur += s;
+                        cmm += mm;
+                    }
+                    if (cur > best) {
+                        best = cur;
+                        bs = cs;
+                        be = i + 1;
+                        bmm = cmm;
+                    }
+                }
+            }
+        }
+        if (best > 0 && bmm == 0) {
+            uint32_t *cg = a.cigar + 3 * (size_t)idx;
+            uint32_t n = 0;
+            if (bs > 0) cg[n++] = bs << 4 | OPS;
+            cg[n++] = (be - bs) << 4 | OPM;
+            if (m > be) cg[n++] = (m - be) << 4 | OPS;
+            rec.status = 1;
+            rec.score = best;
+            rec.wstart = wstart;
+            rec.ref_start = (uint32_t)((int64_t)bs + diag - (int64_t)wstart);
+            rec.read_start = bs;
+            rec.read_end = be;
+            rec.pos = wstart + rec.ref_start;
+            rec.n_ops = n;
+        }
+        a.out[idx] = rec;
+    }
+}
+
 }  // namespace
 
 int walk_blocks_per_cu() {
@@ -528,6 +687,11 @@ int walk_blocks_per_cu() {
 
 hipError_t launch_walk(const WalkArgs &a, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_walk, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_lane(const FastArgs &a, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_fast_lane, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -165,7 +165,8 @@ int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
  * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
  * 6 selection / stitching, 7 extend_hint full-matrix rerun of escalated band
- * attempts, 8 placement walk kernel), half (band kernel) and ms. */
+ * attempts, 8 placement walk kernel, 9 fast lane kernel), half (band kernel)
+ * and ms. */
 typedef struct {
     int32_t kind;
     int32_t half;
@@ -440,6 +441,37 @@ typedef struct {
 int esp_walk_placements(esp_ctx *ctx, const esp_ref *ref, const esp_junctions *junctions,
                         const esp_walk_batch *b, const esp_params *p, const esp_walk_params *wp,
                         esp_strict_score *strict, esp_walk_counts *counts, int32_t *status);
+
+/* The Kadane fast lane, Aligner::try_fast_lane (align.rs:2274-2401), over a
+ * batch of chains: chain k is contig[k] with its anchors (qpos, rpos) at
+ * anchors[2 * anc_off[k] .. 2 * (anc_off[k] + anc_n[k])) in the chain's
+ * (rpos, qpos) order, and its read reads[read_off[k] .. + read_len[k]) in the
+ * chain's orientation (reverse complemented for minus chains).  Device
+ * pointers. */
+typedef struct {
+    size_t nchain;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *anchors;
+    const uint64_t *anc_off;
+    const uint32_t *anc_n;
+} esp_fast_batch;
+/* status 1: the lane accepts with Extension (score, ref_start relative to
+ * wstart, read_start, read_end) and CIGAR cigar[3k .. 3k + n_ops); pos =
+ * wstart + ref_start is the placement's genomic start.  status 0: the lane
+ * refuses (the caller goes on to the EA lane / DP).  status -1: contig out of
+ * range. */
+typedef struct {
+    int32_t status;
+    int32_t score;
+    uint32_t wstart, ref_start, read_start, read_end, pos, n_ops;
+} esp_fast_rec;
+/* k, w: SeedParams (15, 5 in the reference's configurations); p supplies
+ * match_score, mismatch and edit_dir.  cigar: 3 words per chain. */
+int esp_fast_lane_batch(esp_ctx *ctx, const esp_ref *ref, const esp_fast_batch *b, const esp_params *p,
+                        uint32_t k, uint32_t w, esp_fast_rec *out, uint32_t *cigar);
 
 /* intron_length_penalty (splice.rs:119-121) as the walk kernel evaluates
  * it: from a threshold table built on the host with the reference's f64

@@ -1605,3 +1605,85 @@ void espo_walk_batch(const espo_ref *R, size_t n, const uint8_t *reads, const ui
     free(jobs);
     free(th);
 }
+
+/* ==================================================== Kadane fast lane */
+
+/* try_fast_lane's result (align.rs:2274-2401): status 1 with the Extension
+ * (score, ref_start, read_start, read_end, CIGAR of at most three ops) and
+ * the window start wstart, 0 when the lane refuses. */
+typedef struct {
+    int32_t status;
+    int32_t score;
+    uint32_t wstart, ref_start, read_start, read_end;
+    uint32_t n_ops;
+    uint32_t cigar[3];
+} espo_fast;
+
+/* Aligner::try_fast_lane (align.rs:2274-2401).  anchors holds the chain's
+ * (qpos, rpos) pairs in its (rpos, qpos) order; query is the read in the
+ * chain's orientation; k and w are SeedParams k and w. */
+int espo_fast_lane(const espo_ref *R, const uint8_t *query, uint32_t read_len, uint32_t contig,
+                   const uint32_t *anchors, size_t n_anchors, uint32_t k, uint32_t w, const espo_params *ep,
+                   espo_fast *out) {
+    memset(out, 0, sizeof *out);
+    if (contig >= R->ncontig) return 1;
+    if (n_anchors == 0) return 0; /* anchors.first()? */
+    const int64_t diag = (int64_t)anchors[1] - (int64_t)anchors[0];
+    for (size_t a = 0; a < n_anchors; a++)
+        if ((int64_t)anchors[2 * a + 1] - (int64_t)anchors[2 * a] != diag) return 0;
+    const uint32_t max_gap = k + w + 5;
+    for (size_t a = 1; a < n_anchors; a++)
+        if (anchors[2 * a] - anchors[2 * (a - 1)] > max_gap) return 0; /* u32 difference */
+    const uint32_t qstart = anchors[0], qend = anchors[2 * (n_anchors - 1)] + k;
+    if (qstart > 2 || read_len - qend > 2) return 0;
+    const uint32_t rs = anchors[1];
+    const uint32_t flank = read_len / 2 + 30;
+    const uint32_t wstart = rs > flank ? rs - flank : 0;
+    const uint32_t clen = R->len[contig];
+    int32_t best_score = 0, cur_score = 0;
+    size_t best_start = 0, best_end = 0, cur_start = 0;
+    uint32_t cur_mm = 0, best_mm = 0;
+    for (size_t i = 0; i < read_len; i++) {
+        const int64_t rpos = (int64_t)i + diag;
+        int32_t s = 0;
+        int is_mm = 0;
+        if (rpos >= 0 && rpos < (int64_t)clen) {
+            const uint8_t qb = query[i], rb = contig_base(R, contig, (uint32_t)rpos);
+            if (qb == 'N' || rb == 'N') s = 0;
+            else if (qb == rb) s = ep->match_score;
+            else if (edit_exempt(ep->edit_dir, rb, qb)) s = 0;
+            else {
+                s = ep->mismatch;
+                is_mm = 1;
+            }
+        }
+        if (cur_score <= 0) {
+            cur_score = s;
+            cur_start = i;
+            cur_mm = (uint32_t)is_mm;
+        } else {
+            cur_score += s;
+            cur_mm += (uint32_t)is_mm;
+        }
+        if (cur_score > best_score) {
+            best_score = cur_score;
+            best_start = cur_start;
+            best_end = i + 1;
+            best_mm = cur_mm;
+        }
+    }
+    if (best_score <= 0 || best_mm > 0) return 0;
+    const uint32_t read_start = (uint32_t)best_start, read_end = (uint32_t)best_end;
+    uint32_t n = 0;
+    if (read_start > 0) out->cigar[n++] = read_start << 4 | OP_S;
+    out->cigar[n++] = (read_end - read_start) << 4 | OP_M;
+    if (read_len - read_end > 0) out->cigar[n++] = (read_len - read_end) << 4 | OP_S;
+    out->status = 1;
+    out->score = best_score;
+    out->wstart = wstart;
+    out->ref_start = (uint32_t)((int64_t)read_start + diag - (int64_t)wstart);
+    out->read_start = read_start;
+    out->read_end = read_end;
+    out->n_ops = n;
+    return 0;
+}

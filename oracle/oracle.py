@@ -37,6 +37,8 @@ class CSpliceParams(ctypes.Structure):
 
 SPLICE_DEFAULTS = (6, 4, 2, 1, 8)  # splice.rs:40-52: known, GT-AG, GC-AG, AT-AC, non-canonical
 
+FAST_DTYPE = np.dtype([("status", "<i4"), ("score", "<i4"), ("wstart", "<u4"), ("ref_start", "<u4"),
+                       ("read_start", "<u4"), ("read_end", "<u4"), ("n_ops", "<u4"), ("cigar", "<u4", 3)])
 STRICT_DTYPE = np.dtype([("score", "<i4"), ("aligned", "<u4"), ("matches", "<u4"), ("mm", "<u4"),
                          ("ea_ag", "<u4"), ("ea_tc", "<u4"), ("ins", "<u4"), ("del", "<u4"),
                          ("clipped", "<u4"), ("n_splice", "<u4"), ("splice_score", "<i4")])
@@ -108,6 +110,8 @@ def lib():
         L.espo_walk_batch.argtypes = [P(CRef), sz] + [vp] * 9 + [P(CParams), P(CSpliceParams), vp, sz,
                                                                ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
         L.espo_walk_batch.restype = None
+        L.espo_fast_lane.argtypes = [P(CRef), u8p, ctypes.c_uint32, ctypes.c_uint32, u32p, sz,
+                                     ctypes.c_uint32, ctypes.c_uint32, P(CParams), vp]
         _lib = L
     return _lib
 
@@ -507,3 +511,23 @@ def walk_batch(oref: OracleRef, pl, params: Optional[ExtendParams] = None, sp=SP
                           ctypes.byref(CSpliceParams(*sp)), flat.ctypes.data, len(flat) // 3 if junctions else 0,
                           k, strict.ctypes.data, counts.ctypes.data, status.ctypes.data, nthreads)
     return strict[:n], counts[:n], status[:n]
+
+
+def fast_lane(oref: OracleRef, query: bytes, contig: int, anchors, k: int = 15, w: int = 5,
+              params: Optional[ExtendParams] = None):
+    """align.rs:2274 Aligner::try_fast_lane: None when the lane refuses, else
+    (Extension, wstart).  anchors: the chain's (qpos, rpos) pairs in its
+    (rpos, qpos) order; query in the chain's orientation."""
+    q, m = _buf(query)
+    flat = [int(x) for a in anchors for x in a]
+    arr = (ctypes.c_uint32 * max(len(flat), 1))(*flat)
+    out = np.zeros(1, dtype=FAST_DTYPE)
+    rc = lib().espo_fast_lane(ctypes.byref(oref.c), q, len(query), contig, arr, len(flat) // 2, k, w,
+                              ctypes.byref(_params(params)), out.ctypes.data)
+    assert rc == 0, rc
+    r = out[0]
+    if r["status"] != 1:
+        return None
+    ext = Extension(int(r["read_start"]), int(r["read_end"]), int(r["ref_start"]),
+                    cigar_from_words(r["cigar"][: int(r["n_ops"])].tolist()), int(r["score"]))
+    return ext, int(r["wstart"])
