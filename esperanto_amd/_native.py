@@ -28,6 +28,7 @@ EXPORTS = [
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
     "esp_unspliced_batch", "esp_window_myers_batch", "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
+    "esp_engine_myers_batch", "esp_engine_verify_batch",
 ]
 
 
@@ -91,6 +92,12 @@ class CFastBatch(ctypes.Structure):
                 ("read_off", ctypes.c_void_p), ("read_len", ctypes.c_void_p),
                 ("contig", ctypes.c_void_p), ("anchors", ctypes.c_void_p),
                 ("anc_off", ctypes.c_void_p), ("anc_n", ctypes.c_void_p)]
+
+
+class CEngineVerify(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("tx", ctypes.c_void_p), ("tx_off", ctypes.c_void_p),
+                ("tx_len", ctypes.c_void_p), ("anchor", ctypes.c_void_p)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -164,6 +171,8 @@ def lib():
                                       vp]
     L.esp_intron_length_penalty.argtypes = [ctypes.c_uint32]
     L.esp_intron_length_penalty.restype = ctypes.c_int32
+    L.esp_engine_myers_batch.argtypes = [vp, P(CBatch), P(CHostMeta), ctypes.c_int, vp]
+    L.esp_engine_verify_batch.argtypes = [vp, P(CEngineVerify), vp, ctypes.c_uint32, vp, vp]
     L.esp_fast_lane_batch.argtypes = [vp, vp, P(CFastBatch), P(CParams), ctypes.c_uint32, ctypes.c_uint32,
                                       vp, vp]
     _lib = L

@@ -267,6 +267,13 @@ class DeviceBatch(ContextTimings):
             ctypes.c_void_p(self.dist.ptr), ctypes.c_void_p(self.end.ptr),
             ctypes.c_void_p(self.start.ptr) if with_start else None), self.buf.handle)
 
+    def launch_engine_myers(self, mode: int = 0) -> None:
+        """The transcriptome engine's EA-Myers (engine/src/myers.rs) per pair:
+        mode 0 infix, 1 global; distances land in myers_results()[0]."""
+        _native.check(_native.lib().esp_engine_myers_batch(
+            self.buf.handle, ctypes.byref(self._cb), ctypes.byref(self._meta), mode,
+            ctypes.c_void_p(self.dist.ptr)), self.buf.handle)
+
     # ---- results ----
     def gotoh_results(self):
         """(ext[n,4] int64, touch[n,4] int64, cigar words, cig_off, cig_n, aux)."""

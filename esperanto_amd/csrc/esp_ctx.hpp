@@ -79,6 +79,8 @@ struct esp_ctx {
     // window pipelines: query arena, window / pseudo-reference arena,
     // per-pair descriptors and Myers results
     esph::DevBuf wquery, warena, wdesc, wmyers;
+    // transcriptome-engine verification windows (text offsets and lengths)
+    esph::DevBuf eng;
     hipEvent_t ev_plan = nullptr;
     int band_blocks[espk::PLAN_STRIP] = {};
     esph::HostBuf hstage, hout, hplan;

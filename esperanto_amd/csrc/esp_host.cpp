@@ -318,6 +318,7 @@ int esp_destroy(esp_ctx *c) {
     c->warena.release();
     c->wdesc.release();
     c->wmyers.release();
+    c->eng.release();
     c->hstage.release();
     c->hout.release();
     for (int k = 0; k <= esp_ctx::kMaxEv; k++)
@@ -474,6 +475,90 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     tim_begin(c);
     HIP_TRY(c, espk::launch_myers(a, words, (int)blocks, c->stream));
     tim_mark(c, 3, 0);
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
+// The engine's EA-Myers over (pattern, text) pairs: texts at
+// texts + text_off[k], text_len[k]; pattern lengths (host copy) in 1..=256.
+static int engine_myers_launch(esp_ctx *c, size_t n, const uint8_t *reads, const uint64_t *read_off,
+                               const uint32_t *read_len, const uint32_t *host_read_len, const uint8_t *texts,
+                               const uint64_t *text_off, const uint32_t *text_len, int global, int32_t *dist) {
+    uint32_t maxm = 1;
+    for (size_t k = 0; k < n; k++) {
+        const uint32_t m = host_read_len[k];
+        if (m == 0 || m > 256) return fail(c, ESP_E_INVAL, "engine EA-Myers patterns must be 1..256 bases");
+        maxm = std::max(maxm, m);
+    }
+    const int words = (int)((maxm + 63) / 64);
+    HIP_TRY(c, c->counter.ensure(256));
+    espk::MyersArgs a{};
+    a.npairs = (uint32_t)n;
+    a.reads = reads;
+    a.read_off = read_off;
+    a.read_len = read_len;
+    a.texts = texts;
+    a.text_off = text_off;
+    a.text_len = text_len;
+    a.dist = dist;
+    a.counter = (uint32_t *)c->counter.p;
+    a.global = global;
+    uint64_t blocks = (uint64_t)c->ncu * espk::myers_blocks_per_cu(words);
+    blocks = std::max<uint64_t>(std::min<uint64_t>(blocks, (n + 63) / 64), 1);
+    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
+    HIP_TRY(c, espk::launch_myers(a, words, (int)blocks, c->stream));
+    tim_mark(c, 3, 0);
+    return ESP_OK;
+}
+
+int esp_engine_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, int mode, int32_t *dist) {
+    if (!c) return ESP_E_INVAL;
+    if (!b || !meta || !meta->read_len || !dist)
+        return fail(c, ESP_E_INVAL, "batch, host lengths and outputs are required");
+    if (mode != 0 && mode != 1) return fail(c, ESP_E_INVAL, "mode must be 0 (infix) or 1 (global)");
+    if (b->npairs == 0) return ESP_OK;
+    if (meta->win_len)
+        for (size_t k = 0; k < b->npairs; k++)
+            if (meta->win_len[k] > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
+    if (b->npairs > 0xFFFFFFFFull) return fail(c, ESP_E_INVAL, "too many pairs");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    int rc = engine_myers_launch(c, b->npairs, b->reads, b->read_off, b->read_len, meta->read_len, b->wins,
+                                 b->win_off, b->win_len, mode, dist);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
+int esp_engine_verify_batch(esp_ctx *c, const esp_engine_verify *v, const uint32_t *host_read_len, uint32_t flank,
+                            int32_t *dist, int32_t *status) {
+    if (!c) return ESP_E_INVAL;
+    if (!v || !host_read_len || !dist || !status) return fail(c, ESP_E_INVAL, "NULL argument");
+    if (v->n == 0) return ESP_OK;
+    if (v->n > 0xFFFFFFFFull) return fail(c, ESP_E_INVAL, "too many candidates");
+    if (!v->reads || !v->read_off || !v->read_len || !v->tx || !v->tx_off || !v->tx_len || !v->anchor)
+        return fail(c, ESP_E_INVAL, "candidate arrays are required");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = v->n;
+    HIP_TRY(c, c->eng.ensure(n * 12 + 16));
+    espk::EnginePrepArgs pa{};
+    pa.n = (uint32_t)n;
+    pa.read_len = v->read_len;
+    pa.tx_off = v->tx_off;
+    pa.tx_len = v->tx_len;
+    pa.anchor = v->anchor;
+    pa.flank = flank;
+    pa.text_off = (uint64_t *)c->eng.p;
+    pa.text_len = (uint32_t *)((uint8_t *)c->eng.p + n * 8);
+    pa.status = status;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_engine_prep(pa, c->stream));
+    tim_mark(c, 5, 0);
+    int rc = engine_myers_launch(c, n, v->reads, v->read_off, v->read_len, host_read_len, v->tx, pa.text_off,
+                                 pa.text_len, 0, dist);
+    if (rc) return rc;
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
 }

@@ -78,7 +78,24 @@ struct MyersArgs {
     uint64_t *end;    // nullptr: no forward scan unless start is nullptr too (infix)
     uint64_t *start;  // nullptr: no reversed scan
     uint32_t *counter;
+    int32_t global;   // 1: global distance (both ends forced; engine/src/myers.rs:149-168)
 };
+
+// verify_dist windows (engine/src/lib.rs:851-876): per candidate the text is
+// its transcript's [wlo, whi), wlo = max(anchor - flank, 0), whi =
+// min(max(anchor + m + flank, 0), tx_len); an empty window is status 0.
+struct EnginePrepArgs {
+    uint32_t n;
+    const uint32_t *read_len;
+    const uint64_t *tx_off;
+    const uint32_t *tx_len;
+    const int64_t *anchor;
+    uint32_t flank;
+    uint64_t *text_off;
+    uint32_t *text_len;
+    int32_t *status;
+};
+hipError_t launch_engine_prep(const EnginePrepArgs &a, hipStream_t s);
 
 // Device-side batch planner: pairs are classified (one class per band
 // kernel specialisation, plus the general strip class) and counting-sorted

@@ -1,4 +1,6 @@
-// esp_myers.hip -- EA-Myers bit-parallel filter (crates/map/src/myers_ea.rs).
+// esp_myers.hip -- EA-Myers bit-parallel filter (crates/map/src/myers_ea.rs),
+// also the transcriptome engine's EA-Myers (crates/engine/src/myers.rs: the
+// same EA alphabet and recurrence, plus its global mode).
 //
 // One lane owns one (read, text) pair.  The pattern is carried in W 64-bit
 // words (W = ceil(m / 64) <= 4) with the edlib-form carry between words
@@ -93,9 +95,12 @@ __device__ void build_peq(uint64_t *peq, uint8_t *chunk, const uint8_t *read, in
 
 // Column scan over the text (reversed when `reverse`): (best, best_at) with
 // the strict-< earliest-end rule of infix_best_end (myers_ea.rs:141-156).
+// With `global` (myers::global, engine/src/myers.rs:149-168) every column
+// enters row 0 with a +1 delta (D[0][j] = j) and the result is the last
+// column's D[m][n], floored at 0.
 template <int W>
 __device__ void myers_scan(const uint64_t *peq, uint8_t *chunk, int32_t m, const uint8_t *text,
-                           int32_t n, bool reverse, int32_t &best_out, uint32_t &at_out) {
+                           int32_t n, bool reverse, bool global, int32_t &best_out, uint32_t &at_out) {
     const int32_t lastw = (m - 1) >> 6;
     const uint32_t lastb = (uint32_t)(m - 1) & 63u;
     uint64_t pv[W], mv[W];
@@ -112,7 +117,7 @@ __device__ void myers_scan(const uint64_t *peq, uint8_t *chunk, int32_t m, const
         for (int32_t q = 0; q < len; q++) {
             const uint32_t code = chunk[(reverse ? len - 1 - q : q) * 64];
             const uint64_t *row = peq + code * (W * 64);
-            int32_t hin = 0;
+            int32_t hin = global ? 1 : 0;
 #pragma unroll
             for (int w = 0; w < W; w++) {
                 if (w <= lastw) {
@@ -139,6 +144,10 @@ __device__ void myers_scan(const uint64_t *peq, uint8_t *chunk, int32_t m, const
                 best_at = (uint32_t)(c0 + q + 1);
             }
         }
+    }
+    if (global) {
+        best = max(score, 0);
+        best_at = (uint32_t)n;
     }
     best_out = best;
     at_out = best_at;
@@ -174,7 +183,7 @@ __global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
                 // the same minimum, so a start-only call scans once.
                 if (a.end || !a.start) {
                     build_peq<W>(peq, chunk, read, m, false);
-                    myers_scan<W>(peq, chunk, m, text, n, false, d, e);
+                    myers_scan<W>(peq, chunk, m, text, n, false, a.global != 0, d, e);
                     a.dist[p] = d;
                     if (a.end) a.end[p] = e;
                 }
@@ -182,7 +191,7 @@ __global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
                     // infix_best_start: best end of the doubly reversed pair
                     // (myers_ea.rs:166-185); start = n - rev_end.
                     build_peq<W>(peq, chunk, read, m, true);
-                    myers_scan<W>(peq, chunk, m, text, n, true, d, e);
+                    myers_scan<W>(peq, chunk, m, text, n, true, false, d, e);
                     a.dist[p] = d;
                     a.start[p] = (uint64_t)(n - (int32_t)e);
                 }
@@ -191,9 +200,29 @@ __global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
     }
 }
 
+__global__ void k_engine_prep(EnginePrepArgs a) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += gridDim.x * blockDim.x) {
+        const int64_t anchor = a.anchor[k];
+        const int64_t lo = anchor - (int64_t)a.flank;
+        const int64_t hi = anchor + (int64_t)a.read_len[k] + (int64_t)a.flank;
+        const uint64_t wlo = lo > 0 ? (uint64_t)lo : 0u;
+        const uint64_t whi = min(hi > 0 ? (uint64_t)hi : 0u, (uint64_t)a.tx_len[k]);
+        const bool ok = whi > wlo;
+        a.text_off[k] = a.tx_off[k] + (ok ? wlo : 0u);
+        a.text_len[k] = ok ? (uint32_t)(whi - wlo) : 0u;
+        a.status[k] = ok ? 1 : 0;
+    }
+}
+
 size_t myers_lds(int words) { return (size_t)5 * words * 64 * 8 + (size_t)MY_CHUNK * 64; }
 
 }  // namespace
+
+hipError_t launch_engine_prep(const EnginePrepArgs &a, hipStream_t s) {
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((a.n + 255) / 256, 4096u));
+    hipLaunchKernelGGL(k_engine_prep, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s) {
     const size_t lds = myers_lds(words);

@@ -442,6 +442,36 @@ int esp_walk_placements(esp_ctx *ctx, const esp_ref *ref, const esp_junctions *j
                         const esp_walk_batch *b, const esp_params *p, const esp_walk_params *wp,
                         esp_strict_score *strict, esp_walk_counts *counts, int32_t *status);
 
+/* The transcriptome engine's EA-Myers (crates/engine/src/myers.rs): edit
+ * distance with identity and the editing pairs (pattern G / text A, pattern
+ * C / text T) free, N matching nothing, case folded, indels 1.  Per pair,
+ * pattern = read, text = window of the batch:
+ *   mode 0  myers::infix / long::infix (text overhang free on both sides)
+ *   mode 1  myers::global / long::global (both ends forced)
+ * Pattern lengths (meta->read_len, host) must be 1..256, as the reference
+ * asserts; otherwise ESP_E_INVAL and nothing runs. */
+int esp_engine_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, int mode,
+                           int32_t *dist);
+
+/* verify_dist (crates/engine/src/lib.rs:851-876) per candidate: the read in
+ * the candidate's orientation, the candidate's transcript sequence at
+ * tx[tx_off[k] .. + tx_len[k]), anchor = ext.tx_lo - ext.read_lo; the window
+ * is [max(anchor - flank, 0), min(max(anchor + m + flank, 0), tx_len)).
+ * status[k] 1: dist[k] is myers infix distance over the window; 0: empty
+ * window (the reference's None).  Device pointers; host_read_len as above. */
+typedef struct {
+    size_t n;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint8_t *tx;
+    const uint64_t *tx_off;
+    const uint32_t *tx_len;
+    const int64_t *anchor;
+} esp_engine_verify;
+int esp_engine_verify_batch(esp_ctx *ctx, const esp_engine_verify *v, const uint32_t *host_read_len,
+                            uint32_t flank, int32_t *dist, int32_t *status);
+
 /* The Kadane fast lane, Aligner::try_fast_lane (align.rs:2274-2401), over a
  * batch of chains: chain k is contig[k] with its anchors (qpos, rpos) at
  * anchors[2 * anc_off[k] .. 2 * (anc_off[k] + anc_n[k])) in the chain's

@@ -1687,3 +1687,100 @@ int espo_fast_lane(const espo_ref *R, const uint8_t *query, uint32_t read_len, u
     out->n_ops = n;
     return 0;
 }
+
+/* ===================================== engine EA-Myers (128-bit blocks) */
+
+typedef unsigned __int128 u128;
+
+/* code5 (engine/src/myers.rs:19-28). */
+static int eng_code5(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+    }
+}
+
+typedef struct {
+    size_t m, blocks;
+    u128 peq[2][5], init_pv[2], top[2];
+} eng_myers;
+
+/* MyersEa::new (myers.rs:77-104). */
+static void eng_new(eng_myers *s, const uint8_t *pat, size_t m) {
+    memset(s, 0, sizeof *s);
+    s->m = m;
+    s->blocks = (m + 127) / 128;
+    for (size_t i = 0; i < m; i++) s->peq[i / 128][eng_code5(pat[i])] |= (u128)1 << (i % 128);
+    const size_t b0 = m < 128 ? m : 128;
+    s->init_pv[0] = b0 == 128 ? ~(u128)0 : (((u128)1 << b0) - 1);
+    s->top[0] = (u128)1 << (b0 - 1);
+    if (s->blocks == 2) {
+        const size_t b1 = m - 128;
+        s->init_pv[1] = b1 == 128 ? ~(u128)0 : (((u128)1 << b1) - 1);
+        s->top[1] = (u128)1 << (b1 - 1);
+    }
+}
+
+/* eq_vec (myers.rs:109-120): text A also pairs with pattern G, text T with
+ * pattern C; text N never matches. */
+static u128 eng_eq(const eng_myers *s, size_t blk, int tcode) {
+    if (tcode > 3) return 0;
+    u128 eq = s->peq[blk][tcode];
+    if (tcode == 0) eq |= s->peq[blk][2];
+    else if (tcode == 3) eq |= s->peq[blk][1];
+    return eq;
+}
+
+/* calculate_block (myers.rs:180-208): returns hout, writes the adder carry. */
+static int eng_block(u128 eq, u128 *pv, u128 *mv, int hin, int cin, u128 top, int *cout) {
+    const u128 xv = eq | *mv;
+    const u128 a = eq & *pv;
+    u128 sum = a + *pv;
+    int c = sum < a;
+    const u128 sum2 = sum + (u128)(cin ? 1 : 0);
+    c |= sum2 < sum;
+    sum = sum2;
+    *cout = c;
+    const u128 xh = (sum ^ *pv) | xv;
+    u128 ph = *mv | ~(xh | *pv);
+    u128 mh = *pv & xh;
+    int hout = 0;
+    if (ph & top) hout = 1;
+    else if (mh & top) hout = -1;
+    ph <<= 1;
+    mh <<= 1;
+    if (hin == 1) ph |= 1;
+    else if (hin == -1) mh |= 1;
+    *pv = mh | ~(xv | ph);
+    *mv = ph & xv;
+    return hout;
+}
+
+/* myers::infix / long::infix (infix_min, myers.rs:123-146) when global == 0,
+ * myers::global / long::global (myers.rs:149-168) when global == 1.
+ * Returns -1 for a pattern length outside 1..=256 (the reference asserts). */
+int64_t espo_engine_myers(const uint8_t *pat, size_t m, const uint8_t *text, size_t n, int global) {
+    if (m == 0 || m > 256) return -1;
+    eng_myers s;
+    eng_new(&s, pat, m);
+    u128 pv[2] = {s.init_pv[0], s.init_pv[1]}, mv[2] = {0, 0};
+    int32_t score = (int32_t)m;
+    uint32_t best = (uint32_t)m;
+    for (size_t j = 0; j < n; j++) {
+        const int tcode = eng_code5(text[j]);
+        int hin = global ? 1 : 0, cin = 0;
+        for (size_t blk = 0; blk < s.blocks; blk++) {
+            int cout = 0;
+            const int h = eng_block(eng_eq(&s, blk, tcode), &pv[blk], &mv[blk], hin, cin, s.top[blk], &cout);
+            hin = h;
+            cin = cout;
+        }
+        score += hin;
+        if (!global && score >= 0 && (uint32_t)score < best) best = (uint32_t)score;
+    }
+    if (global) return score > 0 ? score : 0;
+    return best;
+}

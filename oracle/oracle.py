@@ -110,6 +110,8 @@ def lib():
         L.espo_walk_batch.argtypes = [P(CRef), sz] + [vp] * 9 + [P(CParams), P(CSpliceParams), vp, sz,
                                                                ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
         L.espo_walk_batch.restype = None
+        L.espo_engine_myers.argtypes = [u8p, sz, u8p, sz, ctypes.c_int]
+        L.espo_engine_myers.restype = ctypes.c_int64
         L.espo_fast_lane.argtypes = [P(CRef), u8p, ctypes.c_uint32, ctypes.c_uint32, u32p, sz,
                                      ctypes.c_uint32, ctypes.c_uint32, P(CParams), vp]
         _lib = L
@@ -531,3 +533,11 @@ def fast_lane(oref: OracleRef, query: bytes, contig: int, anchors, k: int = 15, 
     ext = Extension(int(r["read_start"]), int(r["read_end"]), int(r["ref_start"]),
                     cigar_from_words(r["cigar"][: int(r["n_ops"])].tolist()), int(r["score"]))
     return ext, int(r["wstart"])
+
+
+def engine_myers(pattern: bytes, text: bytes, global_: bool = False) -> int:
+    """engine/src/myers.rs: infix (infix_min) or global EA distance; -1 for a
+    pattern length outside 1..=256."""
+    p, m = _buf(pattern)
+    t, n = _buf(text)
+    return int(lib().espo_engine_myers(p, m, t, n, int(global_)))
