@@ -71,6 +71,18 @@ def timed_steps(fn_launch, db, steps: int, warmup: int):
     return wall, float(np.mean(kms)), tims
 
 
+def _graph_steps(g, db, steps: int, warmup: int) -> float:
+    """Wall seconds per replay of a captured batch, each replay waited for."""
+    for _ in range(warmup):
+        g.launch()
+        db.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+        db.sync()
+    return (time.perf_counter() - t0) / steps
+
+
 def read_traffic(path: str):
     """HBM bytes per band-kernel launch from the committed PMC summary, if any."""
     try:
@@ -429,6 +441,12 @@ def main() -> int:
     log(f"gotoh batched: {wall*1e3:.2f} ms/step, device {kms:.2f} ms (plan {plan_ms:.3f}, "
         f"band {band_ms:.2f} {per_half}, strip {strip_ms:.3f}), {gcups:.1f} GCUPS")
     pairs_per_s = hb.npairs / wall
+    # The same batch captured once as a HIP graph and replayed.
+    g3 = db.capture_gotoh(params, 0)
+    graph_wall = _graph_steps(g3, db, args.steps, args.warmup)
+    g3.check()
+    g3.destroy()
+    log(f"gotoh batched, graph replay: {graph_wall*1e3:.2f} ms/step, {cells / graph_wall / 1e9:.1f} GCUPS")
     if args.copy_bytes:
         _copy_calibration(buf, db.d["wins"], args.copy_bytes)
     db.free()
@@ -443,6 +461,12 @@ def main() -> int:
     d1 = DeviceBatch(c1, buf)
     w1, k1, _ = timed_steps(lambda: d1.launch_gotoh(params, 0), d1, max(args.steps, 3), 1)
     c1_batched_pps = c1.npairs / w1
+    g1 = d1.capture_gotoh(params, 0)
+    w1g = _graph_steps(g1, d1, max(args.steps, 10), 2)
+    g1.check()
+    g1.destroy()
+    c1_graph_pps = c1.npairs / w1g
+    log(f"c1 graph replay: {c1_graph_pps:.0f} pairs/s ({w1g * 1e3:.3f} ms/step)")
     d1.free()
     ncalls = min(args.plain_calls, c1.npairs)
     pairs = [c1.pair(k) for k in range(ncalls)]
@@ -517,7 +541,10 @@ def main() -> int:
                    "device_ms": round(kms, 3), "planner_ms": round(plan_ms, 3),
                    "band_kernel_ms": round(band_ms, 3), "band_kernel_ms_per_half": per_half,
                    "kernel_gcups": round(kernel_gcups, 2),
+                   "graph_ms_per_step": round(graph_wall * 1e3, 3),
+                   "graph_gcups": round(cells / graph_wall / 1e9, 2),
                    "c1_batched_pairs_per_s": round(c1_batched_pps),
+                   "c1_graph_pairs_per_s": round(c1_graph_pps),
                    "c1_plain_pairs_per_s": round(plain_pps),
                    "batched_over_plain": round(c1_batched_pps / plain_pps, 1),
                    "myers_pairs_per_s": round(myers_pps),

@@ -28,7 +28,8 @@ EXPORTS = [
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
     "esp_unspliced_batch", "esp_window_myers_batch", "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
-    "esp_engine_myers_batch", "esp_engine_verify_batch",
+    "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
+    "esp_graph_check", "esp_graph_destroy",
 ]
 
 
@@ -171,6 +172,11 @@ def lib():
                                       vp]
     L.esp_intron_length_penalty.argtypes = [ctypes.c_uint32]
     L.esp_intron_length_penalty.restype = ctypes.c_int32
+    L.esp_gotoh_graph_create.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), ctypes.c_int, P(CBatchOut),
+                                         P(vp)]
+    L.esp_graph_launch.argtypes = [vp, vp]
+    L.esp_graph_check.argtypes = [vp, vp]
+    L.esp_graph_destroy.argtypes = [vp, vp]
     L.esp_engine_myers_batch.argtypes = [vp, P(CBatch), P(CHostMeta), ctypes.c_int, vp]
     L.esp_engine_verify_batch.argtypes = [vp, P(CEngineVerify), vp, ctypes.c_uint32, vp, vp]
     L.esp_fast_lane_batch.argtypes = [vp, vp, P(CFastBatch), P(CParams), ctypes.c_uint32, ctypes.c_uint32,

@@ -213,7 +213,8 @@ class ContextTimings:
     def last_timings(self):
         """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,
         3 Myers, 5 window preparation, 6 selection / stitching, 7 full-matrix
-        rerun of escalated extend_hint attempts, 8 placement walks, 9 fast lane."""
+        rerun of escalated extend_hint attempts, 8 placement walks, 9 fast lane,
+        10 a whole graph launch."""
         arr = (_native.CTiming * 16)()
         n = ctypes.c_size_t()
         _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),
@@ -260,6 +261,10 @@ class DeviceBatch(ContextTimings):
         _native.check(_native.lib().esp_gotoh_batch(self.buf.handle, ctypes.byref(self._cb),
                                                     ctypes.byref(self._meta), ctypes.byref(p),
                                                     mode, ctypes.byref(self._out)), self.buf.handle)
+
+    def capture_gotoh(self, params: Optional[ExtendParams] = None, mode: int = MODE_BAND) -> "GotohGraph":
+        """launch_gotoh recorded once as a HIP graph; GotohGraph.launch() replays it."""
+        return GotohGraph(self, params, mode)
 
     def launch_myers(self, with_start: bool = True) -> None:
         _native.check(_native.lib().esp_myers_batch(
@@ -310,3 +315,33 @@ class DeviceBatch(ContextTimings):
         for a in list(self.d.values()) + [self.cig_off, self.ext, self.touch, self.cigar,
                                           self.cig_n, self.aux, self.dist, self.end, self.start]:
             a.free()
+
+
+class GotohGraph:
+    """One batched Gotoh call of a DeviceBatch captured as a HIP graph
+    (esp_gotoh_graph_create): launch() redoes the batch on the device arrays'
+    current contents with one graph launch; check() waits and raises when a
+    launch met pairs longer than at capture."""
+
+    def __init__(self, db: DeviceBatch, params: Optional[ExtendParams], mode: int):
+        self.db = db
+        self.buf = db.buf
+        self._p = (params or ExtendParams()).to_c()
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().esp_gotoh_graph_create(
+            self.buf.handle, ctypes.byref(db._cb), ctypes.byref(db._meta), ctypes.byref(self._p), mode,
+            ctypes.byref(db._out), ctypes.byref(h)), self.buf.handle)
+        self.handle = h.value
+
+    def launch(self) -> None:
+        _native.check(_native.lib().esp_graph_launch(self.buf.handle, ctypes.c_void_p(self.handle)),
+                      self.buf.handle)
+
+    def check(self) -> None:
+        _native.check(_native.lib().esp_graph_check(self.buf.handle, ctypes.c_void_p(self.handle)),
+                      self.buf.handle)
+
+    def destroy(self) -> None:
+        if self.handle:
+            _native.lib().esp_graph_destroy(self.buf.handle, ctypes.c_void_p(self.handle))
+            self.handle = None

@@ -326,7 +326,11 @@ __global__ __launch_bounds__(GOTOH_BLOCK) void k_gotoh_strip(GotohArgs a) {
         const uint32_t idx = base + lane;
         if (idx < count) {
             const uint32_t p = a.order ? a.order[first + idx] : idx;
-            gotoh_one<T, EDIT>(a, p, S);
+            // A pair larger than the scratch was sized for is left alone (a
+            // captured graph replayed on longer pairs; the planner counts
+            // them as bad and the caller's check reports it).
+            if (a.read_len[p] <= a.maxm && strip_trace_words(a.read_len[p], a.win_len[p]) <= a.maxw)
+                gotoh_one<T, EDIT>(a, p, S);
         }
     }
 }

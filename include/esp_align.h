@@ -158,6 +158,25 @@ int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
 int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     int32_t *dist, uint64_t *end, uint64_t *start);
 
+/* One esp_gotoh_batch call captured as a HIP graph (band or full mode).
+ * Capture sizes everything from the host lengths (meta->read_len and
+ * meta->win_len are required) and records the planner, every band kernel and
+ * the strip kernel on the graph's own scratch; nothing runs until
+ * esp_graph_launch.  A launch redoes the whole batch on what the batch's
+ * device arrays hold at that time (same arrays, same pair count, reads and
+ * windows no longer than at capture) and writes the same outputs as
+ * esp_gotoh_batch would.  Pairs longer than at capture are left alone and
+ * counted; esp_graph_check (waits for the stream) returns ESP_E_INVAL when a
+ * launch met any.  Launches on one context run in stream order; a graph must
+ * not be launched on two contexts at once.  No counterpart in the
+ * reference: it is the batched form's replay without per-kernel launches. */
+typedef struct esp_graph esp_graph;
+int esp_gotoh_graph_create(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
+                           int mode, esp_batch_out *out, esp_graph **graph);
+int esp_graph_launch(esp_ctx *ctx, esp_graph *graph);
+int esp_graph_check(esp_ctx *ctx, esp_graph *graph);
+int esp_graph_destroy(esp_ctx *ctx, esp_graph *graph);
+
 /* Device time (ms) of the last batch call (planner plus every kernel),
  * measured with HIP events on the context stream. */
 int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
@@ -165,8 +184,8 @@ int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
  * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
  * 6 selection / stitching, 7 extend_hint full-matrix rerun of escalated band
- * attempts, 8 placement walk kernel, 9 fast lane kernel), half (band kernel)
- * and ms. */
+ * attempts, 8 placement walk kernel, 9 fast lane kernel, 10 a whole graph
+ * launch), half (band kernel) and ms. */
 typedef struct {
     int32_t kind;
     int32_t half;

@@ -1,0 +1,148 @@
+"""A batched Gotoh call captured as a HIP graph (esp_gotoh_graph_create /
+esp_graph_launch) against the eager esp_gotoh_batch on the same batch: an
+uneven band batch with a tenth of its pairs outside their band class (the
+strip kernel), and full mode; every replay checked field by field (extension,
+TouchRec, CIGAR words and counts, aux) after the outputs are cleared, with
+other eager work on the context between replays; then new read contents of
+the same lengths written into the batch's device arrays and replayed; then a
+graph captured for shorter pairs than the device holds, whose launch leaves
+the over-length strip pairs alone, reports them through esp_graph_check, and
+still gets every other pair right."""
+from __future__ import annotations
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from esperanto_amd import _native
+from esperanto_amd.records import ExtendParams
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def buf():
+    from esperanto_amd.extend import ExtendBuffer
+    say("creating device context")
+    b = ExtendBuffer(0)
+    yield b
+    b.close()
+
+
+def _clear_outputs(db):
+    n = db.npairs
+    for arr, nbytes in ((db.ext, 16 * n), (db.touch, 24 * n), (db.cig_n, 4 * n), (db.aux, 4 * n),
+                        (db.cigar, 4 * int(db.cig_off_host[-1]))):
+        z = np.full(max(nbytes, 1), 0xA5, dtype=np.uint8)
+        _native.check(_native.lib().esp_copy_to_device(db.buf.handle, ctypes.c_void_p(arr.ptr),
+                                                       ctypes.c_void_p(z.ctypes.data), nbytes), db.buf.handle)
+
+
+def _same(a, b, rows=None):
+    ext_a, touch_a, cig_a, off, n_a, aux_a = a
+    ext_b, touch_b, cig_b, _, n_b, aux_b = b
+    idx = range(len(n_a)) if rows is None else rows
+    for k in idx:
+        assert np.array_equal(ext_a[k], ext_b[k]), k
+        assert np.array_equal(touch_a[k], touch_b[k]), k
+        assert n_a[k] == n_b[k] and aux_a[k] == aux_b[k], k
+        o = int(off[k])
+        assert np.array_equal(cig_a[o:o + n_a[k]], cig_b[o:o + n_b[k]]), k
+
+
+def _mixed_batch(seed):
+    from esperanto_amd.batch import workload_c3
+    hb = workload_c3(20_000)
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(hb.npairs, hb.npairs // 10, replace=False)
+    hb.offset[pick] = rng.integers(0, 8, pick.shape[0])  # clamped bands: strip class
+    return hb
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_replays_equal_eager(buf, mode):
+    from esperanto_amd.batch import DeviceBatch, workload_c1
+    hb = _mixed_batch(5)
+    if mode == 1:
+        hb = workload_c1(3000)
+    P = ExtendParams()
+    db = DeviceBatch(hb, buf)
+    other = DeviceBatch(workload_c1(2000), buf)
+    say(f"graph mode {mode}: eager")
+    _clear_outputs(db)  # fields a mode leaves unwritten then match across runs
+    db.launch_gotoh(P, mode)
+    db.sync()
+    eager = db.gotoh_results()
+    g = db.capture_gotoh(P, mode)
+    try:
+        for rep in range(3):
+            _clear_outputs(db)
+            db.sync()
+            say(f"graph mode {mode}: replay {rep}")
+            g.launch()
+            g.check()
+            _same(db.gotoh_results(), eager)
+            other.launch_gotoh(P, 0)
+            other.sync()
+        # new contents, same lengths: substitutions in every read
+        rng = np.random.default_rng(9)
+        reads = hb.reads.copy()
+        hit = rng.random(reads.shape) < 0.02
+        reads[hit] = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, int(hit.sum()))]
+        _native.check(_native.lib().esp_copy_to_device(buf.handle, ctypes.c_void_p(db.d["reads"].ptr),
+                                                       ctypes.c_void_p(reads.ctypes.data), reads.nbytes),
+                      buf.handle)
+        hb2 = type(hb)(**{**hb.__dict__, "reads": reads})
+        ref = DeviceBatch(hb2, buf)
+        _clear_outputs(ref)
+        _clear_outputs(db)
+        ref.launch_gotoh(P, mode)
+        ref.sync()
+        say(f"graph mode {mode}: replay on new reads")
+        g.launch()
+        g.check()
+        _same(db.gotoh_results(), ref.gotoh_results())
+        ref.free()
+    finally:
+        g.destroy()
+        db.free()
+        other.free()
+
+
+def test_graph_reports_pairs_longer_than_at_capture(buf):
+    from esperanto_amd.batch import DeviceBatch
+    hb = _mixed_batch(6)
+    P = ExtendParams()
+    db = DeviceBatch(hb, buf)
+    _clear_outputs(db)
+    db.launch_gotoh(P, 0)
+    db.sync()
+    eager = db.gotoh_results()
+    cap = int(np.percentile(hb.read_len, 60))
+    rl = np.minimum(hb.read_len, cap).astype(np.uint32)
+    wl = hb.win_len.astype(np.uint32).copy()
+    meta = _native.CHostMeta(rl.ctypes.data, wl.ctypes.data, hb.offset.ctypes.data, hb.half.ctypes.data)
+    pc = P.to_c()
+    h = ctypes.c_void_p()
+    _native.check(_native.lib().esp_gotoh_graph_create(buf.handle, ctypes.byref(db._cb), ctypes.byref(meta),
+                                                       ctypes.byref(pc), 0, ctypes.byref(db._out),
+                                                       ctypes.byref(h)), buf.handle)
+    try:
+        say(f"graph captured for reads <= {cap}")
+        _clear_outputs(db)
+        db.sync()
+        _native.check(_native.lib().esp_graph_launch(buf.handle, h), buf.handle)
+        with pytest.raises(_native.EspError):
+            _native.check(_native.lib().esp_graph_check(buf.handle, h), buf.handle)
+        rows = np.nonzero(hb.read_len <= cap)[0]
+        _same(db.gotoh_results(), eager, rows)
+    finally:
+        _native.lib().esp_graph_destroy(buf.handle, h)
+        db.free()
