@@ -391,6 +391,7 @@ def main() -> int:
     ap.add_argument("--only-walk", action="store_true", help="run only the placement-walk leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the band grids at the occupancy limit")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -424,6 +425,12 @@ def main() -> int:
     cells = band_cells(hb)
     log(f"cells={cells} uploading")
     db = DeviceBatch(hb, buf)
+    tune = None
+    if not args.no_tune:
+        # the band kernels' grids, tuned on this batch (a pipeline tunes on
+        # its first chunk: the best persistent grid depends on the batch size)
+        tune = db.autotune_band(params, reps=1)
+        log(f"autotune (half: chosen blocks/CU, occupancy limit, ms at limit, ms chosen): {tune}")
     wall, kms, tims = timed_steps(lambda: db.launch_gotoh(params, 0), db, args.steps, args.warmup)
     gcups = cells / wall / 1e9
     kernel_gcups = cells / (kms / 1e3) / 1e9
@@ -541,6 +548,7 @@ def main() -> int:
                    "device_ms": round(kms, 3), "planner_ms": round(plan_ms, 3),
                    "band_kernel_ms": round(band_ms, 3), "band_kernel_ms_per_half": per_half,
                    "kernel_gcups": round(kernel_gcups, 2),
+                   "autotune": {str(h): v[0] for h, v in tune.items()} if tune else None,
                    "graph_ms_per_step": round(graph_wall * 1e3, 3),
                    "graph_gcups": round(cells / graph_wall / 1e9, 2),
                    "c1_batched_pairs_per_s": round(c1_batched_pps),

@@ -29,7 +29,7 @@ EXPORTS = [
     "esp_unspliced_batch", "esp_window_myers_batch", "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
     "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
-    "esp_graph_check", "esp_graph_destroy",
+    "esp_graph_check", "esp_graph_destroy", "esp_autotune_band",
 ]
 
 
@@ -99,6 +99,12 @@ class CEngineVerify(ctypes.Structure):
     _fields_ = [("n", ctypes.c_size_t), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
                 ("read_len", ctypes.c_void_p), ("tx", ctypes.c_void_p), ("tx_off", ctypes.c_void_p),
                 ("tx_len", ctypes.c_void_p), ("anchor", ctypes.c_void_p)]
+
+
+class CTuneReport(ctypes.Structure):
+    _fields_ = [("half", ctypes.c_int32 * 5), ("occupancy", ctypes.c_int32 * 5),
+                ("blocks_per_cu", ctypes.c_int32 * 5), ("ms_default", ctypes.c_double * 5),
+                ("ms_best", ctypes.c_double * 5), ("candidates", ctypes.c_int32)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -175,6 +181,8 @@ def lib():
     L.esp_gotoh_graph_create.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), ctypes.c_int, P(CBatchOut),
                                          P(vp)]
     L.esp_graph_launch.argtypes = [vp, vp]
+    L.esp_autotune_band.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), P(CBatchOut), ctypes.c_int,
+                                    P(CTuneReport)]
     L.esp_graph_check.argtypes = [vp, vp]
     L.esp_graph_destroy.argtypes = [vp, vp]
     L.esp_engine_myers_batch.argtypes = [vp, P(CBatch), P(CHostMeta), ctypes.c_int, vp]

@@ -262,6 +262,19 @@ class DeviceBatch(ContextTimings):
                                                     ctypes.byref(self._meta), ctypes.byref(p),
                                                     mode, ctypes.byref(self._out)), self.buf.handle)
 
+    def autotune_band(self, params: Optional[ExtendParams] = None, reps: int = 2) -> dict:
+        """Tunes the band kernels' grids on this batch (esp_autotune_band); the
+        context keeps the choice.  Returns {half: (chosen blocks per CU,
+        occupancy limit, ms at the limit, ms at the choice)}."""
+        p = (params or ExtendParams()).to_c()
+        r = _native.CTuneReport()
+        _native.check(_native.lib().esp_autotune_band(self.buf.handle, ctypes.byref(self._cb),
+                                                      ctypes.byref(self._meta), ctypes.byref(p),
+                                                      ctypes.byref(self._out), reps, ctypes.byref(r)),
+                      self.buf.handle)
+        return {int(r.half[k]): (int(r.blocks_per_cu[k]), int(r.occupancy[k]), round(float(r.ms_default[k]), 3),
+                                 round(float(r.ms_best[k]), 3)) for k in range(5)}
+
     def capture_gotoh(self, params: Optional[ExtendParams] = None, mode: int = MODE_BAND) -> "GotohGraph":
         """launch_gotoh recorded once as a HIP graph; GotohGraph.launch() replays it."""
         return GotohGraph(self, params, mode)

@@ -413,6 +413,66 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
     }
 }
 
+int esp_autotune_band(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
+                      esp_batch_out *o, int reps, esp_tune_report *rep) {
+    if (!c) return ESP_E_INVAL;
+    if (!b || !o || !rep || b->npairs == 0) return fail(c, ESP_E_INVAL, "probe batch, output and report are required");
+    if (reps < 1 || reps > 64) return fail(c, ESP_E_INVAL, "reps must be 1..64");
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::memset(rep, 0, sizeof *rep);
+    int occ[espk::PLAN_STRIP], maxocc = 1;
+    for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+        occ[cls] = espk::gotoh_band_blocks_per_cu(espk::plan_class_half(cls));
+        maxocc = std::max(maxocc, occ[cls]);
+        rep->half[cls] = espk::plan_class_half(cls);
+        rep->occupancy[cls] = occ[cls];
+    }
+    double best[espk::PLAN_STRIP];
+    for (int cls = 0; cls < espk::PLAN_STRIP; cls++) best[cls] = 1e30;
+    // Candidate k: every class with occupancy >= k runs k blocks (waves) per
+    // CU; each class keeps its fastest k (its kernel's time is its own).
+    for (int k = 1; k <= maxocc; k++) {
+        for (int cls = 0; cls < espk::PLAN_STRIP; cls++) c->band_blocks[cls] = std::min(k, occ[cls]);
+        double sum[espk::PLAN_STRIP] = {};
+        for (int r = 0; r <= reps; r++) {  // the first run warms up
+            int rc = esp_gotoh_batch(c, b, meta, p, ESP_MODE_BAND, o);
+            if (rc) return rc;
+            esp_timing tim[esp_ctx::kMaxEv];
+            size_t nt = 0;
+            rc = esp_last_timings(c, tim, esp_ctx::kMaxEv, &nt);
+            if (rc) return rc;
+            if (r == 0) continue;
+            for (size_t e = 0; e < nt; e++)
+                for (int cls = 0; cls < espk::PLAN_STRIP; cls++)
+                    if (tim[e].kind == 1 && tim[e].half == espk::plan_class_half(cls)) sum[cls] += tim[e].ms;
+        }
+        for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+            if (k > occ[cls]) continue;
+            const double ms = sum[cls] / reps;
+            if (k == occ[cls]) rep->ms_default[cls] = ms;
+            if (ms < best[cls]) {
+                best[cls] = ms;
+                rep->blocks_per_cu[cls] = k;
+                rep->ms_best[cls] = ms;
+            }
+        }
+        rep->candidates++;
+    }
+    // A half the probe does not use keeps its occupancy limit, and a choice
+    // within 2% of the limit's time is taken as noise.
+    esp_plan_stats st;
+    int rc = esp_last_plan_stats(c, &st);
+    if (rc) return rc;
+    for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+        if (st.band_pairs[cls] == 0 || rep->ms_best[cls] > 0.98 * rep->ms_default[cls]) {
+            rep->blocks_per_cu[cls] = occ[cls];
+            rep->ms_best[cls] = rep->ms_default[cls];
+        }
+        c->band_blocks[cls] = rep->blocks_per_cu[cls];
+    }
+    return ESP_OK;
+}
+
 int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
     if (!c || !out) return ESP_E_INVAL;
     std::memset(out, 0, sizeof(*out));

@@ -158,6 +158,24 @@ int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
 int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     int32_t *dist, uint64_t *end, uint64_t *start);
 
+/* Autotuner of the band kernels' persistent grids: runs the probe batch in
+ * band mode (1 + reps times per candidate) with every band half at k blocks
+ * (waves) per CU for k = 1 .. its occupancy limit, times each half's kernel
+ * with HIP events, keeps each half's fastest k in the context for every
+ * later batch and graph capture, and reports the choice.  The probe's outputs
+ * are written as by esp_gotoh_batch.  No counterpart in the reference: it
+ * picks launch shapes, not results. */
+typedef struct {
+    int32_t half[5];           /* band halves 16, 30, 32, 48, 64 */
+    int32_t occupancy[5];      /* blocks per CU the occupancy limit allows */
+    int32_t blocks_per_cu[5];  /* chosen */
+    double ms_default[5];      /* kernel ms at the occupancy limit */
+    double ms_best[5];         /* kernel ms at the choice */
+    int32_t candidates;
+} esp_tune_report;
+int esp_autotune_band(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
+                      esp_batch_out *out, int reps, esp_tune_report *report);
+
 /* One esp_gotoh_batch call captured as a HIP graph (band or full mode).
  * Capture sizes everything from the host lengths (meta->read_len and
  * meta->win_len are required) and records the planner, every band kernel and

@@ -146,3 +146,33 @@ def test_graph_reports_pairs_longer_than_at_capture(buf):
     finally:
         _native.lib().esp_graph_destroy(buf.handle, h)
         db.free()
+
+
+def test_autotune_changes_shapes_not_results(buf):
+    """esp_autotune_band picks each half's grid from 1 .. its occupancy limit
+    by timing a probe; batches after it give the same records as before."""
+    from esperanto_amd.batch import DeviceBatch, workload_c3
+    from esperanto_amd.extend import ExtendBuffer
+    P = ExtendParams()
+    hb = _mixed_batch(7)
+    b2 = ExtendBuffer(0)  # a context of its own, so other tests keep the default grids
+    try:
+        db = DeviceBatch(hb, b2)
+        _clear_outputs(db)
+        db.launch_gotoh(P, 0)
+        db.sync()
+        before = db.gotoh_results()
+        probe = DeviceBatch(workload_c3(30_000), b2)
+        say("autotune on a 30,000-pair probe")
+        rep = probe.autotune_band(P, reps=1)
+        probe.free()
+        for half, (chosen, occ, ms_def, ms_best) in rep.items():
+            assert half in (16, 30, 32, 48, 64) and 1 <= chosen <= occ
+            assert ms_best <= ms_def + 1e-9
+        _clear_outputs(db)
+        db.launch_gotoh(P, 0)
+        db.sync()
+        _same(db.gotoh_results(), before)
+        db.free()
+    finally:
+        b2.close()
