@@ -392,6 +392,8 @@ def main() -> int:
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the band grids at the occupancy limit")
+    ap.add_argument("--band-grid", type=str, default="",
+                    help="blocks per CU for halves 16,30,32,48,64 (an earlier autotune's choice; skips tuning)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -426,7 +428,17 @@ def main() -> int:
     log(f"cells={cells} uploading")
     db = DeviceBatch(hb, buf)
     tune = None
-    if not args.no_tune:
+    if args.band_grid:
+        # a grid chosen by an earlier autotune run, applied without tuning
+        from esperanto_amd import _native
+        import ctypes
+        tune = {}
+        for half, k in zip((16, 30, 32, 48, 64), (int(x) for x in args.band_grid.split(","))):
+            _native.check(_native.lib().esp_ctx_set_option(buf.handle, 3, ctypes.c_int64(half << 16 | k)),
+                          buf.handle)
+            tune[half] = (k,)
+        log(f"band grids set: {tune}")
+    elif not args.no_tune:
         # the band kernels' grids, tuned on this batch (a pipeline tunes on
         # its first chunk: the best persistent grid depends on the batch size)
         tune = db.autotune_band(params, reps=1)

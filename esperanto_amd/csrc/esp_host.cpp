@@ -408,6 +408,17 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
         c->trace_budget = (size_t)value;
         c->full_trace_budget = (size_t)value * 2;
         return ESP_OK;
+    case ESP_OPT_BAND_GRID: {
+        const int half = (int)(value >> 16), k = (int)(value & 0xFFFF);
+        for (int cls = 0; cls < espk::PLAN_STRIP; cls++) {
+            if (espk::plan_class_half(cls) != half) continue;
+            const int occ = espk::gotoh_band_blocks_per_cu(half);
+            if (k < 0 || k > occ) return fail(c, ESP_E_INVAL, "blocks per CU above the occupancy limit");
+            c->band_blocks[cls] = k ? k : occ;
+            return ESP_OK;
+        }
+        return fail(c, ESP_E_INVAL, "no band kernel for that half");
+    }
     default:
         return fail(c, ESP_E_INVAL, "unknown option");
     }

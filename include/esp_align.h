@@ -216,7 +216,10 @@ int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
  * the lane-per-pair kernel in this build (2, wave per pair, is refused).
  * ESP_OPT_TRACE_BUDGET caps the traceback scratch of one launch in bytes
  * (the full-matrix kernels get twice that); grids shrink to fit. */
-enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2 };
+enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2, ESP_OPT_BAND_GRID = 3 };
+/* ESP_OPT_BAND_GRID sets one band half's persistent grid as esp_autotune_band
+ * would: value = half << 16 | blocks per CU (1 .. the half's occupancy
+ * limit; 0 restores the limit). */
 int esp_ctx_set_option(esp_ctx *ctx, int option, int64_t value);
 
 /* Routing of the last Gotoh batch (or pipeline) on this context: pairs per

@@ -1784,3 +1784,97 @@ int64_t espo_engine_myers(const uint8_t *pat, size_t m, const uint8_t *text, siz
     if (global) return score > 0 ? score : 0;
     return best;
 }
+
+/* ================================ alignment-path checker (test invariants) */
+
+/* The affine score of the alignment path an Extension reports, scored as the
+ * extension DP scores it: SubstLut over the M columns (extend.rs:161-186),
+ * gap_open + gap_ext x len per I or D run (extend.rs:450-459).  bad[k] is 1
+ * when the path does not fit the pair: soft clips other than read_start
+ * at the front and m - read_end at the back, M/I consuming other than
+ * [read_start, read_end), or M/D leaving the window from ref_start. */
+typedef struct {
+    const uint8_t *reads, *wins;
+    const uint64_t *read_off, *win_off, *cig_off;
+    const uint32_t *read_len, *win_len, *cigar, *cig_n;
+    const espo_ext *ext;
+    const espo_params *p;
+    size_t lo, hi;
+    int64_t *score;
+    int32_t *bad;
+} path_job;
+
+static void *path_worker(void *arg) {
+    path_job *J = (path_job *)arg;
+    core_params core;
+    core_new(J->p, &core);
+    for (size_t k = J->lo; k < J->hi; k++) {
+        const uint8_t *r = J->reads + J->read_off[k], *w = J->wins + J->win_off[k];
+        const uint32_t m = J->read_len[k], n = J->win_len[k];
+        const uint32_t *cg = J->cigar + J->cig_off[k];
+        const uint32_t nops = J->cig_n[k];
+        const espo_ext e = J->ext[k];
+        int64_t s = 0;
+        int bad = 0;
+        uint64_t q = 0, t = e.ref_start;
+        for (uint32_t o = 0; o < nops; o++) {
+            const uint32_t op = cg[o] & 15u, len = cg[o] >> 4;
+            if (op == OP_S) {
+                if (o == 0) {
+                    if (len != e.read_start) bad = 1;
+                    q += len;
+                } else if (o == nops - 1) {
+                    if (q != e.read_end || len != m - e.read_end) bad = 1;
+                    q += len;
+                } else {
+                    bad = 1;
+                }
+            } else if (op == OP_M) {
+                if (o == 0 && e.read_start != 0) bad = 1;
+                for (uint32_t x = 0; x < len && !bad; x++) {
+                    if (q + x >= m || t + x >= n) {
+                        bad = 1;
+                        break;
+                    }
+                    s += core.lut[base_code(r[q + x]) * 5 + base_code(w[t + x])];
+                }
+                q += len;
+                t += len;
+            } else if (op == OP_I || op == OP_D) {
+                s -= (int64_t)J->p->gap_open + (int64_t)J->p->gap_ext * len;
+                if (op == OP_I) q += len;
+                else t += len;
+            } else {
+                bad = 1;
+            }
+        }
+        if (nops && (cg[nops - 1] & 15u) != OP_S && q != e.read_end) bad = 1;
+        if (q != (nops ? m : 0) && !(nops == 0 && e.read_start == e.read_end)) bad = 1;
+        if (t > n) bad = 1;
+        J->score[k] = s;
+        J->bad[k] = bad;
+    }
+    return NULL;
+}
+
+void espo_path_scores(size_t npairs, const uint8_t *reads, const uint64_t *read_off, const uint32_t *read_len,
+                      const uint8_t *wins, const uint64_t *win_off, const uint32_t *win_len, const espo_ext *ext,
+                      const uint32_t *cigar, const uint64_t *cig_off, const uint32_t *cig_n,
+                      const espo_params *p, int64_t *score, int32_t *bad, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    path_job *jobs = (path_job *)calloc((size_t)nthreads, sizeof(path_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    const size_t chunk = (npairs + (size_t)nthreads - 1) / (size_t)nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        path_job *J = &jobs[t];
+        J->reads = reads; J->wins = wins; J->read_off = read_off; J->win_off = win_off; J->cig_off = cig_off;
+        J->read_len = read_len; J->win_len = win_len; J->cigar = cigar; J->cig_n = cig_n; J->ext = ext;
+        J->p = p; J->score = score; J->bad = bad;
+        J->lo = (size_t)t * chunk < npairs ? (size_t)t * chunk : npairs;
+        J->hi = J->lo + chunk < npairs ? J->lo + chunk : npairs;
+        pthread_create(&th[t], NULL, path_worker, J);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(jobs);
+    free(th);
+}

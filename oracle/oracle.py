@@ -110,6 +110,9 @@ def lib():
         L.espo_walk_batch.argtypes = [P(CRef), sz] + [vp] * 9 + [P(CParams), P(CSpliceParams), vp, sz,
                                                                ctypes.c_uint32, vp, vp, vp, ctypes.c_int]
         L.espo_walk_batch.restype = None
+        L.espo_path_scores.argtypes = [sz, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, P(CParams), vp, vp,
+                                       ctypes.c_int]
+        L.espo_path_scores.restype = None
         L.espo_engine_myers.argtypes = [u8p, sz, u8p, sz, ctypes.c_int]
         L.espo_engine_myers.restype = ctypes.c_int64
         L.espo_fast_lane.argtypes = [P(CRef), u8p, ctypes.c_uint32, ctypes.c_uint32, u32p, sz,
@@ -541,3 +544,28 @@ def engine_myers(pattern: bytes, text: bytes, global_: bool = False) -> int:
     p, m = _buf(pattern)
     t, n = _buf(text)
     return int(lib().espo_engine_myers(p, m, t, n, int(global_)))
+
+
+EXT_DTYPE = np.dtype([("read_start", "<u4"), ("read_end", "<u4"), ("ref_start", "<u4"), ("score", "<i4")])
+
+
+def path_scores(batch, ext, cig, cig_off, cig_n, params: Optional[ExtendParams] = None, nthreads: int = 1):
+    """Affine score of every reported alignment path, rescored from its CIGAR
+    over the pair's read and window, and whether the path fits the pair
+    (espo_path_scores).  ext: (n, 4) read_start, read_end, ref_start, score."""
+    n = batch.npairs
+    e = np.zeros(max(n, 1), dtype=EXT_DTYPE)
+    ext = np.asarray(ext)
+    if n:
+        e["read_start"][:n], e["read_end"][:n], e["ref_start"][:n] = ext[:, 0], ext[:, 1], ext[:, 2]
+        e["score"][:n] = ext[:, 3]
+    cig = np.ascontiguousarray(cig, dtype=np.uint32)
+    cig_off = np.ascontiguousarray(cig_off, dtype=np.uint64)
+    cig_n = np.ascontiguousarray(cig_n, dtype=np.uint32)
+    score = np.zeros(max(n, 1), dtype=np.int64)
+    bad = np.zeros(max(n, 1), dtype=np.int32)
+    lib().espo_path_scores(n, batch.reads.ctypes.data, batch.read_off.ctypes.data, batch.read_len.ctypes.data,
+                           batch.wins.ctypes.data, batch.win_off.ctypes.data, batch.win_len.ctypes.data,
+                           e.ctypes.data, cig.ctypes.data, cig_off.ctypes.data, cig_n.ctypes.data,
+                           ctypes.byref(_params(params)), score.ctypes.data, bad.ctypes.data, nthreads)
+    return score[:n], bad[:n]

@@ -229,3 +229,24 @@ def test_planted_mistake_is_caught():
     assert good == 2 * len(q) - p.gap_open - 3 * p.gap_ext
     wrong = oracle.bf_local_affine(q, ref, ExtendParams(gap_open=p.gap_open + p.gap_ext, gap_ext=0))
     assert wrong != good
+
+
+def test_path_score_checker_agrees_with_the_oracle():
+    """The path checker the full-size GPU tests use (espo_path_scores) rescored
+    every path the oracle itself reports, on the 100/164-nt band-32 workload
+    and an uneven band sample: every path fits its pair and scores what the
+    DP reported; a planted wrong score or shifted ref_start is caught."""
+    from esperanto_amd.batch import workload_c1, workload_c3
+    for hb in (workload_c1(3000), workload_c3(3000)):
+        ext, _, cig, off, n, _ = oracle.band_batch(hb, ExtendParams(), 0, nthreads=8)
+        score, bad = oracle.path_scores(hb, ext, cig, off, n, nthreads=8)
+        assert not bad.any()
+        assert np.array_equal(score, ext[:, 3])
+        e2 = ext.copy()
+        e2[::7, 3] += 1
+        s2, _ = oracle.path_scores(hb, e2, cig, off, n, nthreads=8)
+        assert (s2[::7] != e2[::7, 3]).all()
+        e3 = ext.copy()
+        e3[:, 2] += 4000
+        _, b3 = oracle.path_scores(hb, e3, cig, off, n, nthreads=8)
+        assert b3[n > 0].all()
