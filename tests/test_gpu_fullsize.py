@@ -93,3 +93,84 @@ def test_myers_1m_pairs(buf):
         assert np.array_equal(start, os_)
     finally:
         db.free()
+
+
+def _query_len(words) -> int:
+    return int(sum(int(w) >> 4 for w in words if (int(w) & 15) in (0, 1, 4)))
+
+
+def _ref_len(words) -> int:
+    return int(sum(int(w) >> 4 for w in words if (int(w) & 15) in (0, 2, 3)))
+
+
+def test_splice_rescue_500k_pairs(buf):
+    """The 500,000-pair splice / mate-rescue workload on a 64 Mb reference:
+    every placed spliced read's stitched CIGAR consumes its 150 bases and
+    stays on the contig, planted junctions are found where the read was cut
+    (genomic start s - u with one RefSkip of e - s, or a read clipped at the
+    cut); every placed mate's CIGAR consumes the read inside its window and
+    lands at the planted position on the minus strand; the first 5,000 pairs
+    of each equal the oracle field by field."""
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.rescue import DeviceMateBatch, MateRequests
+    from esperanto_amd.splice import DeviceSpliceBatch, SpliceRequests
+    from esperanto_amd.synth_window import splice_rescue_workload
+    from test_gpu_window import _compare_mate, _compare_splice
+    say("generating the 500,000-pair workload")
+    ref, sp, mt, truth = splice_rescue_workload(500_000)
+    dref = DeviceReference(ref, buf)
+    L = int(ref.lens[0])
+    try:
+        say("spliced batch")
+        db = DeviceSpliceBatch(sp, dref, buf)
+        db.launch()
+        db.sync()
+        g = db.raw_results()
+        db.free()
+        rec, _, _, _, _, st, soff = g
+        placed = rec["status"] == 1
+        assert placed.mean() > 0.99
+        s, e, u = truth["s"], truth["e"], truth["u"]
+        at_cut = 0
+        for k in np.nonzero(placed)[0].tolist():
+            a = int(soff[k])
+            words = st[a:a + int(rec[k]["n_stitched"])]
+            assert _query_len(words) == 150, k
+            assert int(rec[k]["genomic_start"]) + _ref_len(words) <= L, k
+            skips = [int(w) >> 4 for w in words if (int(w) & 15) == 3]
+            if int(rec[k]["genomic_start"]) == int(s[k] - u[k]) and skips == [int(e[k] - s[k])]:
+                at_cut += 1
+        assert at_cut > 0.95 * placed.sum()
+        say("mate rescue batch")
+        mb = DeviceMateBatch(mt, dref, buf)
+        mb.launch()
+        mb.sync()
+        gm = mb.raw_results()
+        mb.free()
+        mrec, mcig, moff, mn = gm
+        mplaced = mrec["placed"] == 1
+        assert mplaced.mean() > 0.95
+        p2 = truth["p2"]
+        home = 0
+        for k in np.nonzero(mplaced)[0].tolist():
+            a = int(moff[k])
+            words = mcig[a:a + int(mn[k])]
+            assert _query_len(words) == 150, k
+            r = mrec[k]
+            assert r["wstart"] <= r["pos"] and r["pos"] + _ref_len(words) <= r["wstart"] + r["wlen"], k
+            home += int(r["strand"] == 1 and int(r["pos"]) == int(p2[k]) + int(r["read_start"]))
+        assert home > 0.95 * mplaced.sum()
+        say("oracle on the first 5,000 pairs")
+        o = oracle.OracleRef(ref)
+        k = 5000
+        r1 = int(sp.read_off[k - 1] + sp.read_len[k - 1])
+        sub = SpliceRequests(sp.reads[:r1], sp.read_off[:k], sp.read_len[:k], sp.contig[:k], sp.hint[:k],
+                             sp.exons[:int(sp.exon_off[k])], sp.exon_off[:k + 1])
+        exp = oracle.spliced_batch(o, sub, nthreads=16)
+        _compare_splice(sub, tuple(x[:k] if i in (0, 1, 4) else x for i, x in enumerate(g)), exp)
+        r1 = int(mt.read_off[k - 1] + mt.read_len[k - 1])
+        msub = MateRequests(mt.reads[:r1], mt.read_off[:k], mt.read_len[:k], mt.contig[:k], mt.mate_pos[:k])
+        mexp = oracle.mate_batch(o, msub, nthreads=16)
+        _compare_mate(msub, (mrec[:k], mcig, moff[:k + 1], mn[:k]), mexp)
+    finally:
+        dref.close()
