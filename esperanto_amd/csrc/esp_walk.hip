@@ -4,12 +4,11 @@
 // query (align.rs:1060-1163 rescore_with, :2473-2516 recount_mm_ea_in,
 // :1231-1288 exact_kmer_support) in one pass.
 //
-// A lane runs a flat event loop: each iteration consumes one segment of up to
-// 32 columns of its current M op, after first handling every I/D/S/N op that
-// precedes the next segment.  The query and reference words of the next
-// segment are loaded before the current segment is scored, so every wave has
-// a segment's work in flight over each load's latency.  Columns are scored
-// four at a time in byte lanes of a dword:
+// A lane runs a flat event loop: each iteration consumes one tile of up to 64
+// columns of its current M op (its query and reference words loaded at once,
+// whole lines rather than a few words at a time), handles every I/D/S/N op
+// that precedes the next tile, and scores the tile's 32-column segments from
+// registers.  Columns are scored four at a time in byte lanes of a dword:
 //   * reference bases come from the resident 2-bit contig through a 256-entry
 //     LDS table (packed byte -> four ASCII bytes) and v_alignbyte to the
 //     segment's base offset; N intervals and bases past the contig end are
@@ -41,7 +40,6 @@ constexpr uint32_t OPM = 0, OPI = 1, OPD = 2, OPN = 3, OPS = 4;
 // Columns per segment; NQ query dwords and NR reference dwords hold one.
 constexpr uint32_t SEG = 32, NG = SEG / 4, NQ = NG + 1, NR = SEG / 16 + 1;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4), aligned(4)));
-typedef uint32_t u32x3 __attribute__((ext_vector_type(3), aligned(4)));
 constexpr uint32_t N4 = 0x4E4E4E4Eu, A4 = 0x41414141u, C4 = 0x43434343u, G4 = 0x47474747u,
                    T4 = 0x54545454u, HI = 0x80808080u;
 
@@ -250,28 +248,6 @@ __device__ void advance(const WalkArgs &a, Lane &L, const uint32_t *s_up, const 
     }
 }
 
-// The NQ query dwords and NR reference dwords of the segment at (ro, rf).
-// Query dwords past the query's last are clamped to it (their bytes read as
-// N); reference dwords stay inside the contig's region, which holds at least
-// 16 zero bytes after its packed bases.
-__device__ __forceinline__ void seg_load(const Lane &L, uint32_t (&q)[NQ], uint32_t (&r)[NR]) {
-    if (L.qlen) {
-        const uint32_t j0 = (L.qoff0 + L.ro) >> 2;
-        if (j0 + NQ - 1 <= L.qjmax) {
-            const u32x4 v0 = *(const u32x4 *)(L.qw + j0);
-            const u32x4 v1 = *(const u32x4 *)(L.qw + j0 + 4);
-            q[0] = v0.x, q[1] = v0.y, q[2] = v0.z, q[3] = v0.w;
-            q[4] = v1.x, q[5] = v1.y, q[6] = v1.z, q[7] = v1.w;
-            q[8] = L.qw[j0 + 8];
-        } else {
-#pragma unroll
-            for (uint32_t j = 0; j < NQ; j++) q[j] = L.qw[min(j0 + j, L.qjmax)];
-        }
-    }
-    const u32x3 v = *(const u32x3 *)(L.pk + min(L.rf >> 4, L.pkmax));
-    r[0] = v.x, r[1] = v.y, r[2] = v.z;
-}
-
 // A placement's own fields, loaded one placement ahead; the fields of its
 // contig and its first CIGAR word follow once those have arrived.
 struct Meta {
@@ -299,6 +275,40 @@ __device__ __forceinline__ void meta_load(const WalkArgs &a, uint32_t idx, Meta 
     m.cig_off = a.cig_off[idx];
     m.cig_n = a.cig_n[idx];
     m.dir = a.dir ? a.dir[idx] : 0u;
+}
+
+// Columns per tile of the walk loop, and the tile's query and reference words.
+constexpr uint32_t WT = 64;
+__device__ __forceinline__ void tile_load(const Lane &L, uint32_t (&q)[WT / 4 + 1], uint32_t (&r)[WT / 16 + 1]) {
+    if (L.qlen) {
+        const uint32_t j0 = (L.qoff0 + L.ro) >> 2;
+        if (j0 + WT / 4 <= L.qjmax) {
+#pragma unroll
+            for (uint32_t j = 0; j < WT / 16; j++) {
+                const u32x4 v = *(const u32x4 *)(L.qw + j0 + 4 * j);
+                q[4 * j] = v.x, q[4 * j + 1] = v.y, q[4 * j + 2] = v.z, q[4 * j + 3] = v.w;
+            }
+            q[WT / 4] = L.qw[j0 + WT / 4];
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j <= WT / 4; j++) q[j] = L.qw[min(j0 + j, L.qjmax)];
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j <= WT / 4; j++) q[j] = N4;
+    }
+    const uint32_t last = L.pkmax + NR - 1, d = L.rf >> 4;
+    if (d + WT / 16 <= last) {
+#pragma unroll
+        for (uint32_t j = 0; j < WT / 64; j++) {
+            const u32x4 v = *(const u32x4 *)(L.pk + d + 4 * j);
+            r[4 * j] = v.x, r[4 * j + 1] = v.y, r[4 * j + 2] = v.z, r[4 * j + 3] = v.w;
+        }
+        r[WT / 16] = L.pk[d + WT / 16];
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j <= WT / 16; j++) r[j] = L.pk[min(d + j, last)];
+    }
 }
 
 __device__ void place_begin(const WalkArgs &a, Lane &L, uint32_t idx, const Meta &m, const uint32_t *s_up,
@@ -366,7 +376,7 @@ __device__ __forceinline__ uint32_t nibble_bytes(uint32_t m4) {
 
 // Scores one segment: ncol columns at query index qa, reference position p.
 __device__ __forceinline__ void seg_score(const WalkArgs &a, Lane &L, const uint32_t *lut,
-                                          const uint32_t (&q)[NQ], const uint32_t (&r)[NR], uint32_t qa,
+                                          const uint32_t *q, const uint32_t *r, uint32_t qa,
                                           uint32_t p, uint32_t ncol, bool fresh) {
     // reference: NG + 1 packed bytes from byte (p >> 2) on, as ASCII groups
     const uint32_t bo = (p >> 2) & 3u;
@@ -473,11 +483,6 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         meta_load(a, idx, m);
         meta_load2(a, m);
     }
-    uint32_t q[NQ], r[NR];
-#pragma unroll
-    for (uint32_t j = 0; j < NQ; j++) q[j] = N4;
-#pragma unroll
-    for (uint32_t j = 0; j < NR; j++) r[j] = 0;
     for (; idx < a.n; idx += stride) {
         const Meta cur = m;
         if (idx + stride < a.n) meta_load(a, idx + stride, m);
@@ -490,27 +495,29 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         }
         Lane L;
         place_begin(a, L, idx, cur, s_up, s_down, T);
-        if (L.rem) seg_load(L, q, r);
         bool second = idx + stride < a.n;  // the next placement's contig fields are due
         while (L.rem) {
             if (second) {
                 meta_load2(a, m);
                 second = false;
             }
-            uint32_t cq[NQ], cr[NR];
-#pragma unroll
-            for (uint32_t j = 0; j < NQ; j++) cq[j] = q[j];
-#pragma unroll
-            for (uint32_t j = 0; j < NR; j++) cr[j] = r[j];
-            const uint32_t ncol = min(L.rem, SEG), qa = L.ro, p = L.rf;
+            // one tile: up to WT columns of the M op, its words loaded at once
+            uint32_t tq[WT / 4 + 1], tr[WT / 16 + 1];
+            tile_load(L, tq, tr);
+            const uint32_t tcol = min(L.rem, WT), qa = L.ro, p = L.rf;
             const bool fresh = L.fresh;
             L.fresh = false;
-            L.rem -= ncol;
-            L.ro += ncol;
-            L.rf += ncol;
+            L.rem -= tcol;
+            L.ro += tcol;
+            L.rf += tcol;
             if (L.rem == 0) advance(a, L, s_up, s_down, T);
-            if (L.rem) seg_load(L, q, r);
-            seg_score(a, L, lut, cq, cr, qa, p, ncol, fresh);
+#pragma unroll
+            for (uint32_t sg = 0; sg < WT / SEG; sg++) {
+                const uint32_t c0 = sg * SEG;
+                if (c0 < tcol)
+                    seg_score(a, L, lut, tq + sg * (SEG / 4), tr + sg * (SEG / 16), qa + c0, p + c0,
+                              min(tcol - c0, SEG), fresh && sg == 0);
+            }
         }
         if (second) meta_load2(a, m);
         place_end(a, L, T);
@@ -518,6 +525,88 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
 }
 
 // ------------------------------------------------------------ fast lane
+
+// Columns per fast-lane tile.
+constexpr uint32_t FT = 128;
+
+// Kadane state of try_fast_lane (align.rs:2346-2366).
+struct FastKadane {
+    int32_t best = 0, cur = 0;
+    uint32_t bs = 0, be = 0, cs = 0, cmm = 0, bmm = 0;
+};
+
+// One tile of FT columns from registers: query words q (aligned, shifted by
+// qsh), reference words r (dwords from (p0 >> 4), used when fast), else the
+// bases come one by one (tile off the contig or meeting an N interval).
+// FULL: every column of the tile is in the read.
+template <bool FULL>
+__device__ __forceinline__ void fast_tile(const FastArgs &a, FastKadane &K, const uint32_t *lut,
+                                          const uint32_t (&q)[FT / 4 + 1], const uint32_t (&r)[FT / 16 + 1],
+                                          uint32_t qsh, bool fast, int64_t p0, uint32_t i0, uint32_t ncol,
+                                          const uint32_t *pk, uint32_t pklast, const uint32_t *iv, uint32_t niv,
+                                          uint32_t clen, uint32_t exag, uint32_t extc) {
+    const uint32_t bo = fast ? ((uint32_t)p0 >> 2) & 3u : 0u, rsh = fast ? (uint32_t)p0 & 3u : 0u;
+#pragma unroll
+    for (uint32_t g = 0; g < FT / 16; g++) {
+        // reference: four ASCII words of this 16-column group
+        uint32_t r4s[4];
+        if (fast) {
+            const uint32_t W = align4(r[g + 1], r[g], bo);
+            uint32_t A[5];
+#pragma unroll
+            for (int t = 0; t < 4; t++) A[t] = lut[(W >> (8 * t)) & 0xFFu];
+            A[4] = lut[(r[g + 1] >> (8 * bo)) & 0xFFu];
+#pragma unroll
+            for (int t = 0; t < 4; t++) r4s[t] = align4(A[t + 1], A[t], rsh);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                uint32_t v = 0;
+                for (int c = 0; c < 4; c++) {
+                    const int64_t x = p0 + 16 * g + 4 * t + c;
+                    uint32_t base = 'N';
+                    if (x >= 0 && x < (int64_t)clen)
+                        base = ref_base(pk[min((uint32_t)x >> 4, pklast)], iv, niv, clen, (uint32_t)x);
+                    v |= base << (8 * c);
+                }
+                r4s[t] = v;
+            }
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < 4; t++) {
+            const uint32_t q4 = align4(q[4 * g + t + 1], q[4 * g + t], qsh), r4 = r4s[t];
+            const uint32_t eq = zbytes(q4 ^ r4);
+            const uint32_t nb = zbytes(q4 ^ N4) | zbytes(r4 ^ N4);
+            const uint32_t ex = (zbytes((r4 ^ A4) | (q4 ^ G4)) & exag) | (zbytes((r4 ^ T4) | (q4 ^ C4)) & extc);
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++) {
+                const uint32_t col = 16 * g + 4 * t + c;
+                const uint32_t i = i0 + col;
+                const uint32_t bit = 8 * c + 7;
+                const bool isn = (nb >> bit) & 1u, ise = (eq >> bit) & 1u, isx = (ex >> bit) & 1u;
+                const int32_t s = isn ? 0 : ise ? a.match : isx ? 0 : a.mismatch;
+                const uint32_t mm = (!isn && !ise && !isx) ? 1u : 0u;
+                const bool on = FULL || col < ncol;
+                // Kadane with <= 0 reset (align.rs:2346-2366)
+                const bool reset = K.cur <= 0;
+                const int32_t ncur = reset ? s : K.cur + s;
+                const uint32_t ncs = reset ? i : K.cs, ncmm = reset ? mm : K.cmm + mm;
+                if (on) {
+                    K.cur = ncur;
+                    K.cs = ncs;
+                    K.cmm = ncmm;
+                    if (ncur > K.best) {
+                        K.best = ncur;
+                        K.bs = ncs;
+                        K.be = i + 1;
+                        K.bmm = ncmm;
+                    }
+                }
+            }
+        }
+    }
+}
+
 
 // try_fast_lane (align.rs:2274-2401), one lane per chain: the anchor
 // conditions, then Kadane along the chain diagonal four columns at a time
@@ -584,79 +673,40 @@ __global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
             const int64_t p0 = diag > 0 ? diag : 0;
             ci = niv ? iv_seek(iv, niv, (uint32_t)min(p0, (int64_t)0xFFFFFFFF)) : 0u;
         }
-        int32_t best = 0, cur = 0;
-        uint32_t bs = 0, be = 0, cs = 0, cmm = 0, bmm = 0;
-        for (uint32_t i0 = 0; i0 < m; i0 += 16) {
-            const uint32_t ncol = min(m - i0, 16u);
-            const int64_t p0 = (int64_t)i0 + diag;
-            // query: five aligned dwords (clamped to the query's last)
-            uint32_t q[5];
+        FastKadane K;
+        for (uint32_t i0 = 0; i0 < m; i0 += FT) {
+            // one tile: its query and reference words load at once, then the
+            // FT columns run from registers
+            const uint32_t ncol = min(m - i0, FT);
+            uint32_t q[FT / 4 + 1], r[FT / 16 + 1];
             const uint32_t j0 = (qoff0 + i0) >> 2;
+            if (j0 + FT / 4 <= qjmax) {
 #pragma unroll
-            for (uint32_t j = 0; j < 5; j++) q[j] = qw[min(j0 + j, qjmax)];
-            const uint32_t qsh = (qoff0 + i0) & 3u;
-            // reference: the LUT path when the 16 bases lie inside the contig
-            // and clear of N intervals, else base by base
-            while (ci < niv && (int64_t)iv[2 * ci + 1] <= p0) ci++;
-            const bool inside = p0 >= 0 && p0 + 16 <= (int64_t)clen;
-            const bool clear = ci >= niv || (int64_t)iv[2 * ci] >= p0 + 16;
-            uint32_t r4s[4];
-            if (inside && clear) {
-                const uint32_t p = (uint32_t)p0;
-                const uint32_t d = p >> 4, bo = (p >> 2) & 3u;
-                const uint32_t r0 = pk[d], r1 = pk[min(d + 1, pklast)];
-                const uint32_t W = align4(r1, r0, bo);
-                uint32_t A[5];
-#pragma unroll
-                for (int t = 0; t < 4; t++) A[t] = lut[(W >> (8 * t)) & 0xFFu];
-                A[4] = lut[(r1 >> (8 * bo)) & 0xFFu];
-#pragma unroll
-                for (int t = 0; t < 4; t++) r4s[t] = align4(A[t + 1], A[t], p & 3u);
+                for (uint32_t j = 0; j < FT / 16; j++) {
+                    const u32x4 v = *(const u32x4 *)(qw + j0 + 4 * j);
+                    q[4 * j] = v.x, q[4 * j + 1] = v.y, q[4 * j + 2] = v.z, q[4 * j + 3] = v.w;
+                }
+                q[FT / 4] = qw[j0 + FT / 4];
             } else {
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    uint32_t v = 0;
-                    for (int c = 0; c < 4; c++) {
-                        const int64_t x = p0 + 4 * t + c;
-                        uint32_t b = 'N';
-                        if (x >= 0 && x < (int64_t)clen)
-                            b = ref_base(pk[min((uint32_t)x >> 4, pklast)], iv, niv, clen, (uint32_t)x);
-                        v |= b << (8 * c);
-                    }
-                    r4s[t] = v;
-                }
+                for (uint32_t j = 0; j <= FT / 4; j++) q[j] = qw[min(j0 + j, qjmax)];
             }
+            const uint32_t qsh = (qoff0 + i0) & 3u;
+            const int64_t p0 = (int64_t)i0 + diag;
+            while (ci < niv && (int64_t)iv[2 * ci + 1] <= p0) ci++;
+            const bool inside = p0 >= 0 && p0 + FT <= (int64_t)clen;
+            const bool clear = ci >= niv || (int64_t)iv[2 * ci] >= p0 + FT;
+            const bool fast = inside && clear;
+            const uint32_t d0 = fast ? (uint32_t)p0 >> 4 : 0u;
 #pragma unroll
-            for (uint32_t t = 0; t < 4; t++) {
-                const uint32_t q4 = align4(q[t + 1], q[t], qsh), r4 = r4s[t];
-                const uint32_t eq = zbytes(q4 ^ r4);
-                const uint32_t nb = zbytes(q4 ^ N4) | zbytes(r4 ^ N4);
-                const uint32_t ex = (zbytes((r4 ^ A4) | (q4 ^ G4)) & exag) | (zbytes((r4 ^ T4) | (q4 ^ C4)) & extc);
-                for (uint32_t c = 0; c < 4; c++) {
-                    const uint32_t i = i0 + 4 * t + c;
-                    if (i >= i0 + ncol) break;
-                    const uint32_t bit = 8 * c + 7;
-                    const bool isn = (nb >> bit) & 1u, ise = (eq >> bit) & 1u, isx = (ex >> bit) & 1u;
-                    const int32_t s = isn ? 0 : ise ? a.match : isx ? 0 : a.mismatch;
-                    const uint32_t mm = (!isn && !ise && !isx) ? 1u : 0u;
-                    // Kadane with <= 0 reset (align.rs:2346-2366)
-                    if (cur <= 0) {
-                        cur = s;
-                        cs = i;
-                        cmm = mm;
-                    } else {
-                        cur += s;
-                        cmm += mm;
-                    }
-                    if (cur > best) {
-                        best = cur;
-                        bs = cs;
-                        be = i + 1;
-                        bmm = cmm;
-                    }
-                }
-            }
+            for (uint32_t j = 0; j <= FT / 16; j++) r[j] = pk[min(d0 + j, pklast)];
+            if (ncol == FT)
+                fast_tile<true>(a, K, lut, q, r, qsh, fast, p0, i0, FT, pk, pklast, iv, niv, clen, exag, extc);
+            else
+                fast_tile<false>(a, K, lut, q, r, qsh, fast, p0, i0, ncol, pk, pklast, iv, niv, clen, exag, extc);
         }
+        const int32_t best = K.best;
+        const uint32_t bs = K.bs, be = K.be, bmm = K.bmm;
         if (best > 0 && bmm == 0) {
             uint32_t *cg = a.cigar + 3 * (size_t)idx;
             uint32_t n = 0;
