@@ -685,6 +685,10 @@ static int myers_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t
     return ESP_OK;
 }
 
+int esp_myers_infix(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *text, size_t n, int32_t *dist) {
+    return myers_single(c, read, m, text, n, dist, nullptr, nullptr);
+}
+
 int esp_myers_best_end(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *text, size_t n,
                        int32_t *dist, size_t *end) {
     if (!end) return fail(c, ESP_E_INVAL, "NULL argument");

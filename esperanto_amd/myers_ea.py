@@ -33,7 +33,12 @@ def infix_best_end(read: bytes, text: bytes, buf: Optional[ExtendBuffer] = None)
 
 def infix(read: bytes, text: bytes, buf: Optional[ExtendBuffer] = None) -> int:
     """myers_ea.rs:118 / :255"""
-    return infix_best_end(read, text, buf)[0]
+    buf = buf or default_buffer()
+    r, m = _b(read)
+    t, n = _b(text)
+    d = ctypes.c_int32()
+    _native.check(_native.lib().esp_myers_infix(buf.handle, r, m, t, n, ctypes.byref(d)), buf.handle)
+    return d.value
 
 
 def infix_best_start(read: bytes, text: bytes, buf: Optional[ExtendBuffer] = None) -> Tuple[int, int]:

@@ -97,8 +97,11 @@ int esp_band(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *win, si
              const esp_params *p, int has_band, int64_t offset, int64_t half, esp_ext *out,
              esp_touch *touch, uint32_t *cigar, size_t cap, size_t *n_ops);
 
-/* myers_ea::long::infix_best_end / infix_best_start (m == 0 -> (0,0),
- * m > 256 -> (m,0)); the short-module functions are the same for m <= 128. */
+/* myers_ea::long::infix / infix_best_end / infix_best_start (myers_ea.rs:
+ * 118-185, :249-310; m == 0 -> (0,0), m > 256 -> (m,0)); the short-module
+ * functions are the same for m <= 128. */
+int esp_myers_infix(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *text, size_t n,
+                    int32_t *dist);
 int esp_myers_best_end(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *text,
                        size_t n, int32_t *dist, size_t *end);
 int esp_myers_best_start(esp_ctx *ctx, const uint8_t *read, size_t m, const uint8_t *text,

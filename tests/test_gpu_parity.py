@@ -337,6 +337,7 @@ def test_myers_plain(buf):
     for r, t, _, _ in _myers_cases()[::7]:
         assert M.infix_best_end(r, t, buf) == oracle.infix_best_end(r, t)
         assert M.infix_best_start(r, t, buf) == oracle.infix_best_start(r, t)
+        assert M.infix(r, t, buf) == oracle.infix(r, t)
 
 
 def test_out_of_envelope_scoring_is_rejected(buf):
