@@ -515,13 +515,16 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     if (!b || !meta || !meta->read_len || !dist)
         return fail(c, ESP_E_INVAL, "batch, host lengths and outputs are required");
     if (b->npairs == 0) return ESP_OK;
-    uint32_t maxm = 0;
-    for (size_t k = 0; k < b->npairs; k++) {
-        const uint32_t m = meta->read_len[k];
-        if (m <= 256) maxm = std::max(maxm, m);
-        if (meta->win_len && meta->win_len[k] > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
+    // Branch-free scans (they vectorize): the longest pattern of at most 256
+    // bases sizes the kernel's words; longer ones take the guard.
+    uint32_t maxm = 0, maxn = 0;
+    const uint32_t *rl = meta->read_len;
+    for (size_t k = 0; k < b->npairs; k++) maxm = std::max(maxm, rl[k] <= 256u ? rl[k] : 0u);
+    if (meta->win_len) {
+        const uint32_t *wl = meta->win_len;
+        for (size_t k = 0; k < b->npairs; k++) maxn = std::max(maxn, wl[k]);
+        if (maxn > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
     }
-    const int words = std::max(1, (int)((maxm + 63) / 64));
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, c->counter.ensure(256));
     espk::MyersArgs a{};
@@ -537,14 +540,14 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     a.end = end;
     a.start = start;
     a.counter = (uint32_t *)c->counter.p;
-    const int per_cu = espk::myers_blocks_per_cu(words);
+    const int per_cu = espk::myers_blocks_per_cu((int)maxm, false);
     uint64_t blocks = (uint64_t)c->ncu * per_cu;
     blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
     blocks = std::max<uint64_t>(blocks, 1);
     HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     tim_begin(c);
-    HIP_TRY(c, espk::launch_myers(a, words, (int)blocks, c->stream));
+    HIP_TRY(c, espk::launch_myers(a, (int)maxm, (int)blocks, c->stream));
     tim_mark(c, 3, 0);
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
@@ -561,7 +564,6 @@ static int engine_myers_launch(esp_ctx *c, size_t n, const uint8_t *reads, const
         if (m == 0 || m > 256) return fail(c, ESP_E_INVAL, "engine EA-Myers patterns must be 1..256 bases");
         maxm = std::max(maxm, m);
     }
-    const int words = (int)((maxm + 63) / 64);
     HIP_TRY(c, c->counter.ensure(256));
     espk::MyersArgs a{};
     a.npairs = (uint32_t)n;
@@ -574,10 +576,10 @@ static int engine_myers_launch(esp_ctx *c, size_t n, const uint8_t *reads, const
     a.dist = dist;
     a.counter = (uint32_t *)c->counter.p;
     a.global = global;
-    uint64_t blocks = (uint64_t)c->ncu * espk::myers_blocks_per_cu(words);
+    uint64_t blocks = (uint64_t)c->ncu * espk::myers_blocks_per_cu((int)maxm, global != 0);
     blocks = std::max<uint64_t>(std::min<uint64_t>(blocks, (n + 63) / 64), 1);
     HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
-    HIP_TRY(c, espk::launch_myers(a, words, (int)blocks, c->stream));
+    HIP_TRY(c, espk::launch_myers(a, (int)maxm, (int)blocks, c->stream));
     tim_mark(c, 3, 0);
     return ESP_OK;
 }
@@ -670,9 +672,8 @@ static int myers_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t
     a.end = (uint64_t *)(dout + 8);
     a.start = start ? (uint64_t *)(dout + 16) : nullptr;
     a.counter = (uint32_t *)c->counter.p;
-    const int words = m > 256 ? 1 : std::max(1, (int)((m + 63) / 64));
     HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
-    HIP_TRY(c, espk::launch_myers(a, words, 1, c->stream));
+    HIP_TRY(c, espk::launch_myers(a, m > 256 ? 1 : (int)m, 1, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->hout.p, dout, 24, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint8_t *ho = (const uint8_t *)c->hout.p;

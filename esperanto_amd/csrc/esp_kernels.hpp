@@ -171,11 +171,12 @@ __host__ __device__ inline uint64_t band_trace_words(uint64_t m, uint64_t half) 
 int full_rows(uint32_t maxm);
 hipError_t launch_gotoh_full(const GotohArgs &a, int rows, int blocks, hipStream_t s);
 int gotoh_full_blocks_per_cu(int rows);
-hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s);
+// maxm: the longest pattern of the launch (1..256 after clamping).
+hipError_t launch_myers(const MyersArgs &a, int maxm, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 int gotoh_strip_blocks_per_cu(int edit);
-int myers_blocks_per_cu(int words);
+int myers_blocks_per_cu(int maxm, bool global);
 
 // ---- resident reference and window pipelines (esp_window.hip) ----------
 

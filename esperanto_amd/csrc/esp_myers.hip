@@ -153,6 +153,154 @@ __device__ void myers_scan(const uint64_t *peq, uint8_t *chunk, int32_t m, const
     at_out = best_at;
 }
 
+// ---- 32-bit words with a full carry chain (infix modes) ----------------
+//
+// The pattern sits left-aligned in NW 32-bit words: row i at bit
+// off + i, off = 32 NW - m, so row m - 1 is always bit 31 of the last word.
+// The off rows below it match every text byte and start with zero vertical
+// deltas; by induction every one of them keeps D = 0 in every column, which
+// is exactly the free top row of the infix recurrence, so D[m][j] is
+// unchanged.  (Global distances, whose top row is D[0][j] = j, keep the
+// 64-bit kernel above.)  Each column is one multi-word step of the textbook
+// recurrence: the addition carries across words and the two shifts are
+// funnel shifts, so no per-word horizontal carry is needed.
+
+// Peq rows (A, C, G, T, other) for the read, left-aligned, padding rows set.
+template <int NW>
+__device__ void build_peq32(uint32_t *peq, uint8_t *chunk, const uint8_t *read, int32_t m, bool reverse) {
+    const int32_t off = 32 * NW - m;
+    uint32_t A[NW], C[NW], G[NW], T[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) A[w] = C[w] = G[w] = T[w] = 0;
+    for (int32_t q0 = 0; q0 < m; q0 += MY_CHUNK) {
+        const int32_t len = min(MY_CHUNK, m - q0);
+        stage_codes(read + (reverse ? m - q0 - len : q0), len, chunk);
+        for (int32_t q = 0; q < len; q++) {
+            const uint32_t rc = chunk[(reverse ? len - 1 - q : q) * 64];
+            const int32_t pos = off + q0 + q;
+            const uint32_t bit = 1u << (pos & 31);
+            const int32_t wi = pos >> 5;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                if (w == wi) {
+                    if (rc == 0) A[w] |= bit;
+                    if (rc == 1) { C[w] |= bit; T[w] |= bit; }
+                    if (rc == 2) { G[w] |= bit; A[w] |= bit; }
+                    if (rc == 3) T[w] |= bit;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const int32_t lo = 32 * w;
+        const uint32_t pad = off >= lo + 32 ? 0xFFFFFFFFu : (off <= lo ? 0u : ((1u << (off - lo)) - 1u));
+        peq[(0 * NW + w) * 64] = A[w] | pad;
+        peq[(1 * NW + w) * 64] = C[w] | pad;
+        peq[(2 * NW + w) * 64] = G[w] | pad;
+        peq[(3 * NW + w) * 64] = T[w] | pad;
+        peq[(4 * NW + w) * 64] = pad;
+    }
+}
+
+// Column scan: (best, best_at) with the strict-< earliest-end rule of
+// infix_best_end (myers_ea.rs:141-156).
+template <int NW>
+__device__ void myers_scan32(const uint32_t *peq, uint8_t *chunk, int32_t m, const uint8_t *text, int32_t n,
+                             bool reverse, int32_t &best_out, uint32_t &at_out) {
+    const int32_t off = 32 * NW - m;
+    uint32_t pv[NW], mv[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const int32_t lo = 32 * w;
+        const uint32_t pad = off >= lo + 32 ? 0xFFFFFFFFu : (off <= lo ? 0u : ((1u << (off - lo)) - 1u));
+        pv[w] = ~pad;
+        mv[w] = 0;
+    }
+    int32_t score = m, best = m;
+    uint32_t best_at = 0;
+    for (int32_t c0 = 0; c0 < n; c0 += MY_CHUNK) {
+        const int32_t len = min(MY_CHUNK, n - c0);
+        stage_codes(text + (reverse ? n - c0 - len : c0), len, chunk);
+        for (int32_t q = 0; q < len; q++) {
+            const uint32_t code = chunk[(reverse ? len - 1 - q : q) * 64];
+            const uint32_t *row = peq + code * (NW * 64);
+            uint32_t ph[NW], mh[NW], xv[NW];
+            uint32_t carry = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                const uint32_t eq = row[w * 64], p = pv[w];
+                xv[w] = eq | mv[w];
+                uint32_t c1 = 0;
+                const uint32_t sum = __builtin_addc(eq & p, p, carry, &c1);
+                carry = c1;
+                const uint32_t xh = (sum ^ p) | eq;
+                ph[w] = mv[w] | ~(xh | p);
+                mh[w] = p & xh;
+            }
+            score += (int32_t)(ph[NW - 1] >> 31) - (int32_t)(mh[NW - 1] >> 31);
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                const uint32_t phs = w ? __builtin_amdgcn_alignbit(ph[w], ph[w - 1], 31) : ph[0] << 1;
+                const uint32_t mhs = w ? __builtin_amdgcn_alignbit(mh[w], mh[w - 1], 31) : mh[0] << 1;
+                pv[w] = mhs | ~(xv[w] | phs);
+                mv[w] = phs & xv[w];
+            }
+            if (score < best) {
+                best = score;
+                best_at = (uint32_t)(c0 + q + 1);
+            }
+        }
+    }
+    best_out = best;
+    at_out = best_at;
+}
+
+template <int NW>
+__global__ __launch_bounds__(MYERS_BLOCK) void k_myers32(MyersArgs a) {
+    extern __shared__ uint32_t lds32[];  // peq [5][NW][64] u32, then chunk [128][64] bytes
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t *peq = lds32 + lane;
+    uint8_t *chunk = (uint8_t *)(lds32 + 5 * NW * 64) + lane;
+    const uint32_t first = a.range ? a.range[0] : 0u;
+    const uint32_t count = a.range ? a.range[1] : a.npairs;
+    for (;;) {
+        const uint32_t base = wave_pop(a.counter, lane);
+        if (base >= count) break;
+        const uint32_t idx = base + lane;
+        if (idx < count) {
+            const uint32_t p = a.order ? a.order[first + idx] : idx;
+            const int32_t m = (int32_t)a.read_len[p], n = (int32_t)a.text_len[p];
+            const uint8_t *read = a.reads + a.read_off[p];
+            const uint8_t *text = a.texts + a.text_off[p];
+            if (m == 0 || m > 32 * NW) {
+                // long::infix_best_end guards (myers_ea.rs:275-280); the host
+                // sizes NW for every m <= 256, so only the guards land here
+                a.dist[p] = m == 0 ? 0 : m;
+                if (a.end) a.end[p] = 0;
+                if (a.start) a.start[p] = 0;
+            } else {
+                int32_t d;
+                uint32_t e;
+                if (a.end || !a.start) {
+                    build_peq32<NW>(peq, chunk, read, m, false);
+                    myers_scan32<NW>(peq, chunk, m, text, n, false, d, e);
+                    a.dist[p] = d;
+                    if (a.end) a.end[p] = e;
+                }
+                if (a.start) {
+                    build_peq32<NW>(peq, chunk, read, m, true);
+                    myers_scan32<NW>(peq, chunk, m, text, n, true, d, e);
+                    a.dist[p] = d;
+                    a.start[p] = (uint64_t)(n - (int32_t)e);
+                }
+            }
+        }
+    }
+}
+
+size_t myers32_lds(int nw) { return (size_t)5 * nw * 64 * 4 + (size_t)MY_CHUNK * 64; }
+
 template <int W>
 __global__ __launch_bounds__(MYERS_BLOCK) void k_myers(MyersArgs a) {
     extern __shared__ uint64_t lds[];  // peq [5][W][64] u64, then chunk [128][64] bytes
@@ -224,28 +372,67 @@ hipError_t launch_engine_prep(const EnginePrepArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_myers(const MyersArgs &a, int words, int blocks, hipStream_t s) {
-    const size_t lds = myers_lds(words);
-    switch (words) {
-    case 1: hipLaunchKernelGGL(k_myers<1>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    case 2: hipLaunchKernelGGL(k_myers<2>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    case 3: hipLaunchKernelGGL(k_myers<3>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
-    default: hipLaunchKernelGGL(k_myers<4>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+// Pattern lengths up to maxm (clamped to 1..256): infix modes run the 32-bit
+// kernel with ceil(maxm / 32) words, global mode the 64-bit kernel.
+hipError_t launch_myers(const MyersArgs &a, int maxm, int blocks, hipStream_t s) {
+    maxm = std::min(std::max(maxm, 1), 256);
+    if (a.global) {
+        const int words = (maxm + 63) / 64;
+        const size_t lds = myers_lds(words);
+        switch (words) {
+        case 1: hipLaunchKernelGGL(k_myers<1>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+        case 2: hipLaunchKernelGGL(k_myers<2>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+        case 3: hipLaunchKernelGGL(k_myers<3>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+        default: hipLaunchKernelGGL(k_myers<4>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+        }
+        return hipGetLastError();
+    }
+    const int nw = (maxm + 31) / 32;
+    const size_t lds = myers32_lds(nw);
+    switch (nw) {
+    case 1: hipLaunchKernelGGL(k_myers32<1>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 2: hipLaunchKernelGGL(k_myers32<2>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 3: hipLaunchKernelGGL(k_myers32<3>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 4: hipLaunchKernelGGL(k_myers32<4>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 5: hipLaunchKernelGGL(k_myers32<5>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 6: hipLaunchKernelGGL(k_myers32<6>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    case 7: hipLaunchKernelGGL(k_myers32<7>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_myers32<8>, dim3(blocks), dim3(MYERS_BLOCK), lds, s, a); break;
     }
     return hipGetLastError();
 }
 
-int myers_blocks_per_cu(int words) {
+template <typename K>
+static int occupancy(K kernel, size_t lds) {
     int nb = 0;
-    const size_t lds = myers_lds(words);
-    hipError_t e = hipSuccess;
-    switch (words) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<1>, MYERS_BLOCK, lds); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<2>, MYERS_BLOCK, lds); break;
-    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<3>, MYERS_BLOCK, lds); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_myers<4>, MYERS_BLOCK, lds); break;
-    }
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, MYERS_BLOCK, lds);
     return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+int myers_blocks_per_cu(int maxm, bool global) {
+    maxm = std::min(std::max(maxm, 1), 256);
+    if (global) {
+        const int words = (maxm + 63) / 64;
+        const size_t lds = myers_lds(words);
+        switch (words) {
+        case 1: return occupancy(k_myers<1>, lds);
+        case 2: return occupancy(k_myers<2>, lds);
+        case 3: return occupancy(k_myers<3>, lds);
+        default: return occupancy(k_myers<4>, lds);
+        }
+    }
+    const int nw = (maxm + 31) / 32;
+    const size_t lds = myers32_lds(nw);
+    switch (nw) {
+    case 1: return occupancy(k_myers32<1>, lds);
+    case 2: return occupancy(k_myers32<2>, lds);
+    case 3: return occupancy(k_myers32<3>, lds);
+    case 4: return occupancy(k_myers32<4>, lds);
+    case 5: return occupancy(k_myers32<5>, lds);
+    case 6: return occupancy(k_myers32<6>, lds);
+    case 7: return occupancy(k_myers32<7>, lds);
+    default: return occupancy(k_myers32<8>, lds);
+    }
 }
 
 }  // namespace espk

@@ -204,8 +204,8 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
     HIP_TRY(c, espk::launch_mate_prep(a, c->stream));
     tim_mark(c, 5, 0);
     // EA-Myers best start of both orientations over the window.
-    const int words = (int)((maxm + 63) / 64);
-    const int per_cu = espk::myers_blocks_per_cu(words);
+    const int mp = (int)std::min<uint64_t>(std::max<uint64_t>(maxm, 1), 256);
+    const int per_cu = espk::myers_blocks_per_cu(mp, false);
     uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * per_cu, (n + 63) / 64);
     blocks = std::max<uint64_t>(blocks, 1);
     for (int strand = 0; strand < 2; strand++) {
@@ -222,7 +222,7 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
         m.start = strand ? a.s_minus : a.s_plus;
         m.counter = (uint32_t *)c->counter.p + 16 * strand;
         HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
-        HIP_TRY(c, espk::launch_myers(m, words, (int)blocks, c->stream));
+        HIP_TRY(c, espk::launch_myers(m, mp, (int)blocks, c->stream));
         tim_mark(c, 3, 0);
     }
     HIP_TRY(c, espk::launch_mate_select(a, c->stream));
@@ -285,8 +285,8 @@ int esp_window_myers_batch(esp_ctx *c, const esp_ref *r, const esp_window_job *j
     tim_begin(c);
     HIP_TRY(c, espk::launch_winmyers_prep(a, c->stream));
     tim_mark(c, 5, 0);
-    const int words = (int)((std::min<uint32_t>(std::max<uint32_t>(max_pattern, 1), 256) + 63) / 64);
-    uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(words), (n + 63) / 64);
+    const int mp = (int)std::min<uint32_t>(std::max<uint32_t>(max_pattern, 1), 256);
+    uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(mp, false), (n + 63) / 64);
     espk::MyersArgs m{};
     m.npairs = (uint32_t)n;
     m.reads = reads;
@@ -300,7 +300,7 @@ int esp_window_myers_batch(esp_ctx *c, const esp_ref *r, const esp_window_job *j
     m.start = mode == 1 ? pos : nullptr;
     m.counter = (uint32_t *)c->counter.p;
     HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
-    HIP_TRY(c, espk::launch_myers(m, words, (int)std::max<uint64_t>(blocks, 1), c->stream));
+    HIP_TRY(c, espk::launch_myers(m, mp, (int)std::max<uint64_t>(blocks, 1), c->stream));
     tim_mark(c, 3, 0);
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
@@ -372,8 +372,8 @@ int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,
     HIP_TRY(c, espk::launch_unsp_prep(a, c->stream));
     tim_mark(c, 5, 0);
     {  // EA-Myers best start of the read over its window (reads of 1..256 bases)
-        const int words = (int)((std::min<uint32_t>(maxm, 256) + 63) / 64);
-        uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(words), (n + 63) / 64);
+        const int mp = (int)std::min<uint32_t>(std::max<uint32_t>(maxm, 1), 256);
+        uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(mp, false), (n + 63) / 64);
         espk::MyersArgs m{};
         m.npairs = (uint32_t)n;
         m.reads = a.qarena;
@@ -387,7 +387,7 @@ int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,
         m.start = a.s_start;
         m.counter = (uint32_t *)c->counter.p;
         HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
-        HIP_TRY(c, espk::launch_myers(m, words, (int)std::max<uint64_t>(blocks, 1), c->stream));
+        HIP_TRY(c, espk::launch_myers(m, mp, (int)std::max<uint64_t>(blocks, 1), c->stream));
         tim_mark(c, 3, 0);
     }
     HIP_TRY(c, espk::launch_unsp_ea(a, c->stream));
