@@ -169,6 +169,23 @@ def _copy_calibration(buf, src, nbytes):
     dst.free()
 
 
+def bench_myers(args, buf):
+    """Batched EA-Myers on the 150-nt / 214-nt workload (distance and end):
+    (pairs/s, kernel ms, TOP/s at the algorithmic 31 ops per 64-bit word-column
+    + 3 per column)."""
+    from esperanto_amd.batch import DeviceBatch, workload_c2
+    c2 = workload_c2(args.myers_pairs)
+    d2 = DeviceBatch(c2, buf)
+    w2, k2, _ = timed_steps(lambda: d2.launch_myers(False), d2, max(args.steps, 3), 1)
+    myers_pps = c2.npairs / w2
+    words = 3
+    myers_ops = c2.npairs * 214 * (MYERS_OPS_PER_WORD_COL * words + 3)
+    myers_tops = myers_ops / (k2 / 1e3) / 1e12
+    log(f"myers: {myers_pps/1e6:.1f} M pairs/s (kernel {k2:.2f} ms)")
+    d2.free()
+    return myers_pps, k2, myers_tops
+
+
 def bench_walk(args, buf, params):
     """The three placement walks (rescore_with, recount_mm_ea_in,
     exact_kmer_support) over the placements of the pair workload: per pair,
@@ -389,6 +406,7 @@ def main() -> int:
     ap.add_argument("--walk-pairs", type=int, default=1_000_000,
                     help="read pairs of the placement-walk leg (two placements each; 0 skips it)")
     ap.add_argument("--only-walk", action="store_true", help="run only the placement-walk leg")
+    ap.add_argument("--only-myers", action="store_true", help="run only the batched Myers leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the band grids at the occupancy limit")
@@ -415,6 +433,12 @@ def main() -> int:
         return 0
     if args.only_serve:
         print(json.dumps(bench_serving(args, buf, params)), flush=True)
+        buf.close()
+        return 0
+    if args.only_myers:
+        pps, kms, tops = bench_myers(args, buf)
+        print(json.dumps({"myers_pairs_per_s": round(pps), "kernel_ms": round(kms, 3),
+                          "valu_tops": round(tops, 2)}), flush=True)
         buf.close()
         return 0
     if args.only_walk:
@@ -496,16 +520,7 @@ def main() -> int:
     plain_pps = ncalls / (time.perf_counter() - t0)
     log(f"c1: batched {c1_batched_pps:.0f} pairs/s, plain {plain_pps:.0f} pairs/s")
 
-    # Myers batched.
-    c2 = workload_c2(args.myers_pairs)
-    d2 = DeviceBatch(c2, buf)
-    w2, k2, _ = timed_steps(lambda: d2.launch_myers(False), d2, max(args.steps, 3), 1)
-    myers_pps = c2.npairs / w2
-    words = 3
-    myers_ops = c2.npairs * 214 * (MYERS_OPS_PER_WORD_COL * words + 3)
-    myers_tops = myers_ops / (k2 / 1e3) / 1e12
-    log(f"myers: {myers_pps/1e6:.1f} M pairs/s (kernel {k2:.2f} ms)")
-    d2.free()
+    myers_pps, k2, myers_tops = bench_myers(args, buf)
 
     # Splice + mate-rescue window pipelines on a 64 Mb reference.
     win = None
