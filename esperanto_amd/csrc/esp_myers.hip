@@ -166,6 +166,10 @@ __device__ void myers_scan(const uint64_t *peq, uint8_t *chunk, int32_t m, const
 // funnel shifts, so no per-word horizontal carry is needed.
 
 // Peq rows (A, C, G, T, other) for the read, left-aligned, padding rows set.
+// Word by word: the bases that land in word w set their bit in each row whose
+// letter they match (ea_matches, myers_ea.rs:28-38), read from the nibble
+// table 0x85A1 (A: A; C: C,T; G: G,A; T: T; other: none; rows A C G T =
+// bits 0 1 2 3).
 template <int NW>
 __device__ void build_peq32(uint32_t *peq, uint8_t *chunk, const uint8_t *read, int32_t m, bool reverse) {
     const int32_t off = 32 * NW - m;
@@ -175,19 +179,18 @@ __device__ void build_peq32(uint32_t *peq, uint8_t *chunk, const uint8_t *read, 
     for (int32_t q0 = 0; q0 < m; q0 += MY_CHUNK) {
         const int32_t len = min(MY_CHUNK, m - q0);
         stage_codes(read + (reverse ? m - q0 - len : q0), len, chunk);
-        for (int32_t q = 0; q < len; q++) {
-            const uint32_t rc = chunk[(reverse ? len - 1 - q : q) * 64];
-            const int32_t pos = off + q0 + q;
-            const uint32_t bit = 1u << (pos & 31);
-            const int32_t wi = pos >> 5;
 #pragma unroll
-            for (int w = 0; w < NW; w++) {
-                if (w == wi) {
-                    if (rc == 0) A[w] |= bit;
-                    if (rc == 1) { C[w] |= bit; T[w] |= bit; }
-                    if (rc == 2) { G[w] |= bit; A[w] |= bit; }
-                    if (rc == 3) T[w] |= bit;
-                }
+        for (int w = 0; w < NW; w++) {
+            // pattern indices of this pass that land in word w
+            const int32_t lo = max(32 * w - off, q0), hi = min(32 * w + 32 - off, q0 + len);
+            for (int32_t i = lo; i < hi; i++) {
+                const uint32_t rc = chunk[(reverse ? len - 1 - (i - q0) : i - q0) * 64];
+                const uint32_t mk = (0x85A1u >> (4 * rc)) & 15u;
+                const uint32_t b = (uint32_t)(i + off - 32 * w);
+                A[w] |= (mk & 1u) << b;
+                C[w] |= ((mk >> 1) & 1u) << b;
+                G[w] |= ((mk >> 2) & 1u) << b;
+                T[w] |= (mk >> 3) << b;
             }
         }
     }
