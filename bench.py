@@ -10,7 +10,8 @@ cell census sum_pairs L*(2b+1).
 
 Reported beside it (same JSON line): the plain one-launch-per-call form on the
 10,000-pair 100/164-nt workload, the batched form on that workload, the
-batched EA-Myers filter on 1,000,000 150/214-nt pairs, the integer-VALU
+batched EA-Myers filter on 1,000,000 150/214-nt pairs and its plain
+one-launch-per-pair form, the integer-VALU
 roofline of the Gotoh kernel, and the CPU restatement of the reference path
 timed on this host's cores on a bounded sample of the same workload.
 
@@ -172,7 +173,7 @@ def _copy_calibration(buf, src, nbytes):
 def bench_myers(args, buf):
     """Batched EA-Myers on the 150-nt / 214-nt workload (distance and end):
     (pairs/s, kernel ms, TOP/s at the algorithmic 31 ops per 64-bit word-column
-    + 3 per column)."""
+    + 3 per column, pairs/s of the plain one-launch-per-pair form)."""
     from esperanto_amd.batch import DeviceBatch, workload_c2
     c2 = workload_c2(args.myers_pairs)
     d2 = DeviceBatch(c2, buf)
@@ -183,7 +184,17 @@ def bench_myers(args, buf):
     myers_tops = myers_ops / (k2 / 1e3) / 1e12
     log(f"myers: {myers_pps/1e6:.1f} M pairs/s (kernel {k2:.2f} ms)")
     d2.free()
-    return myers_pps, k2, myers_tops
+    # The plain form: one esp_myers_best_end call (one launch) per pair.
+    from esperanto_amd.myers_ea import infix_best_end
+    ncalls = min(args.plain_calls, c2.npairs)
+    pairs = [c2.pair(k) for k in range(ncalls)]
+    infix_best_end(pairs[0][0], pairs[0][1], buf)
+    t0 = time.perf_counter()
+    for r, w in pairs:
+        infix_best_end(r, w, buf)
+    plain_pps = ncalls / (time.perf_counter() - t0)
+    log(f"myers plain: {plain_pps:.0f} pairs/s ({ncalls} calls)")
+    return myers_pps, k2, myers_tops, plain_pps
 
 
 def bench_walk(args, buf, params):
@@ -436,9 +447,9 @@ def main() -> int:
         buf.close()
         return 0
     if args.only_myers:
-        pps, kms, tops = bench_myers(args, buf)
+        pps, kms, tops, plain = bench_myers(args, buf)
         print(json.dumps({"myers_pairs_per_s": round(pps), "kernel_ms": round(kms, 3),
-                          "valu_tops": round(tops, 2)}), flush=True)
+                          "valu_tops": round(tops, 2), "plain_pairs_per_s": round(plain)}), flush=True)
         buf.close()
         return 0
     if args.only_walk:
@@ -520,7 +531,7 @@ def main() -> int:
     plain_pps = ncalls / (time.perf_counter() - t0)
     log(f"c1: batched {c1_batched_pps:.0f} pairs/s, plain {plain_pps:.0f} pairs/s")
 
-    myers_pps, k2, myers_tops = bench_myers(args, buf)
+    myers_pps, k2, myers_tops, myers_plain_pps = bench_myers(args, buf)
 
     # Splice + mate-rescue window pipelines on a 64 Mb reference.
     win = None
@@ -585,6 +596,8 @@ def main() -> int:
                    "myers_pairs_per_s": round(myers_pps),
                    "myers_kernel_ms": round(k2, 3),
                    "myers_valu_tops": round(myers_tops, 2),
+                   "myers_plain_pairs_per_s": round(myers_plain_pps),
+                   "myers_batched_over_plain": round(myers_pps / myers_plain_pps, 1),
                    "splice_rescue": win,
                    "placement_walks": walks,
                    "serving": serve},

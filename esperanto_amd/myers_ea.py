@@ -1,7 +1,7 @@
 """Host mirror of crates/map/src/myers_ea.rs over the HIP library.
 
-The device kernel carries the pattern in 64-bit words (ceil(m/64) <= 4) with
-an edlib-form carry between words, so one code path serves both the
+The device kernel carries the pattern in 32-bit words (ceil(m/32) <= 8) with
+a full carry chain between words, so one code path serves both the
 reference's single-block (m <= 128) and long:: two-block (m <= 256)
 functions; results depend only on the DP.  Guards are kept: m == 0 -> (0, 0),
 m > 256 -> (m, 0) (myers_ea.rs:123-125, :254-256).
