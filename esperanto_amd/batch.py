@@ -203,18 +203,24 @@ class ContextTimings:
         return ms.value
 
     def last_plan_stats(self) -> dict:
-        """Routing of the last batch: pairs per band half, strip pairs, escalations."""
+        """Routing of the last batch: pairs per band half, strip pairs, escalations,
+        and of those the pairs the pruned full-matrix passes finished, the pairs
+        the unpruned kernel ran and the pairs the second pruned pass took."""
         st = _native.CPlanStats()
         _native.check(_native.lib().esp_last_plan_stats(self.buf.handle, ctypes.byref(st)),
                       self.buf.handle)
         return {"band_pairs": dict(zip((16, 30, 32, 48, 64), list(st.band_pairs))),
-                "strip_pairs": st.strip_pairs, "escalated": st.escalated}
+                "strip_pairs": st.strip_pairs, "escalated": st.escalated,
+                "full_pruned": st.full_pruned, "full_unpruned": st.full_unpruned,
+                "full_second": st.full_second}
 
     def last_timings(self):
         """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,
         3 Myers, 5 window preparation, 6 selection / stitching, 7 full-matrix
-        rerun of escalated extend_hint attempts, 8 placement walks, 9 fast lane,
-        10 a whole graph launch."""
+        rerun of escalated extend_hint attempts (the first pruned pass when
+        pruning is on), 8 placement walks, 9 fast lane, 10 a whole graph launch,
+        11 the unpruned pass over what the pruned passes could not prove, 12 the
+        sort of the escalation list, 13 the second pruned pass."""
         arr = (_native.CTiming * 16)()
         n = ctypes.c_size_t()
         _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),

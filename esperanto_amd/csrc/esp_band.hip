@@ -458,6 +458,10 @@ __global__ __launch_bounds__(BAND_BLOCK, band_waves(HALF)) void k_gotoh_band(Got
         if (escalate) {
             a.esc_list[atomicAdd(&a.esc_range[1], 1u)] = p;
             if (a.aux) a.aux[p] = 1;
+            if (a.esc_info) {
+                a.esc_info[2 * (size_t)p] = best;
+                a.esc_info[2 * (size_t)p + 1] = bj - bi;
+            }
         }
         if (active && !escalate) {
             const uint64_t c0 = a.cig_off[p];

@@ -76,6 +76,13 @@ struct esp_ctx {
     size_t trace_budget = esph::kTraceBudget;
     size_t full_trace_budget = esph::kFullTraceBudget;
     int full_kernel = 0;  // extend_hint reruns: 0 by batch size, 1 lane per pair
+    int full_prune = 1;   // extend_hint reruns: pruned pass first (1) or the full matrix only (0)
+    bool last_pruned = false;  // the last batch ran the pruned pass (its counts are in escbuf)
+    // stripe half-width and head pad of the two pruned passes
+    int prune_half_a = espk::ESC_HALF_A, prune_pad_a = espk::ESC_PAD_A;
+    int prune_half_b = espk::ESC_HALF_B, prune_pad_b = espk::ESC_PAD_B;
+    // pruned reruns: EscDev, per-pair band results, the sorted and fallback lists
+    esph::DevBuf escbuf;
     // window pipelines: query arena, window / pseudo-reference arena,
     // per-pair descriptors and Myers results
     esph::DevBuf wquery, warena, wdesc, wmyers;
@@ -220,6 +227,14 @@ inline bool full_lut_folded(const esp_params *p, uint32_t out[6]) {
         out[f] = w;
     }
     return fits;
+}
+
+// The largest score of the substitution table (match, the EditDir-exempt 0,
+// or mismatch): no path gains more on one read row.
+inline int max_subst(const esp_params *p) {
+    int s = p->match_score > p->mismatch ? p->match_score : p->mismatch;
+    if (p->edit_dir == 1 || p->edit_dir == 2) s = s > 0 ? s : 0;
+    return s;
 }
 
 inline void band_lut(const esp_params *p, uint32_t out[5]) {

@@ -20,6 +20,8 @@
 // so no such cell can beat or tie a real row's maximum at an earlier row.
 // The best cell is the first row-major strict maximum (extend.rs:751-772):
 // a column key (H << 16 | 0xFFFF - i) taken strictly across columns.
+#include <algorithm>
+
 #include "esp_kernels.hpp"
 
 using namespace espd;
@@ -175,7 +177,23 @@ __device__ __forceinline__ void pack_hi(uint32_t &pk, int32_t e, int32_t x) {
         : "v"(e), "v"(x));
 }
 
-template <int MAXM, bool FOLD>
+// Pruned mode (PRUNE).  Let L be the band attempt's best score (a lower
+// bound on the full matrix's best: band cells see margins no higher than the
+// full matrix's values) and S the largest substitution score.  A value V at
+// row i can lie on, or decide a tie on, the optimal path only if
+// V + S * (m - i) >= L (call it live): a path gains at most S a row.  Every
+// maximiser of a live value is live, so skipping cells that are not live,
+// with margins (H = 0, E = F = NEG) standing in for them, leaves every live
+// value, every trace bit the traceback reads and the first row-major
+// maximum unchanged.  Per column the wave computes the head groups (every
+// row from which a path starting at 0 can reach L, plus a pad) and the
+// groups meeting rows [j - dmax - h, j - dmin + h] (its pairs' band-maximum
+// diagonals dmin..dmax, h = prune_half); the stripe only moves down, so
+// a group enters it once, with its initial state.  The proof that nothing
+// skipped was live: every computed cell with an uncomputed down, right or
+// diagonal neighbour must hold H, E and F below L - S * (m - i).  A pair that
+// fails it goes to the fallback list (the unpruned kernel) untraced.
+template <int MAXM, bool FOLD, bool PRUNE>
 __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
     static_assert(MAXM % 8 == 0, "rows come in groups of 8");
     constexpr int WR = MAXM / 8;  // trace words per column
@@ -195,7 +213,8 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
     asm("v_mov_b32 %0, %1" : "=v"(extv) : "s"(ext));
     asm("v_mov_b32 %0, %1" : "=v"(noev) : "s"(noe));
     const uint32_t first = a.range[0], total = a.range[1];
-    const uint32_t nunits = (total + 63) / 64;
+    const uint32_t usz = a.unit ? a.unit : 64u;
+    const uint32_t nunits = (total + usz - 1) / usz;
     if (lane < 8) flut[lane] = FOLD ? a.flutf[lane < 6 ? lane : 5] : a.flut[lane < 6 ? lane : 5];
     for (int q = 0; q < FULL_CHUNK; q++) chunk[q * 64] = 0;
     __syncthreads();
@@ -203,8 +222,8 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
     for (;;) {
         const uint32_t u = wave_pop(a.counter, lane, 1u);
         if (u >= nunits) break;
-        const uint32_t ubegin = first + 64 * u;
-        const uint32_t ucount = min(64u, total - 64 * u);
+        const uint32_t ubegin = first + usz * u;
+        const uint32_t ucount = min(usz, total - usz * u);
         const bool active = lane < ucount;
         const uint32_t p = a.order[ubegin + (active ? lane : 0u)];
         PairView pv;
@@ -222,7 +241,30 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             umaxm = max(umaxm, __shfl_xor(umaxm, sh));
             umaxn = max(umaxn, __shfl_xor(umaxn, sh));
         }
-        const int32_t ngroups = (umaxm + 7) >> 3;
+        const int32_t ngroups = PRUNE ? __builtin_amdgcn_readfirstlane((umaxm + 7) >> 3) : (umaxm + 7) >> 3;
+        // Pruned mode: the lane's threshold base L - S * m, and the wave's head
+        // groups and diagonal range (uniform).
+        int32_t tbase = 0, nhead = ngroups, dmin = 0, dmax = 0;
+        if constexpr (PRUNE) {
+            const int32_t S = a.prune_s;
+            const int32_t L = active ? a.esc_info[2 * (size_t)p] : 0;
+            const int32_t d = active ? a.esc_info[2 * (size_t)p + 1] : 0;
+            tbase = L - S * m;
+            int32_t nh = active ? esc_head_groups(m, L, S, a.prune_pad) : 0;
+            int32_t lo = active ? d : 0x7FFFFFFF, hi = active ? d : -0x7FFFFFFF;
+#pragma unroll
+            for (int sh = 32; sh >= 1; sh >>= 1) {
+                nh = max(nh, __shfl_xor(nh, sh));
+                lo = min(lo, __shfl_xor(lo, sh));
+                hi = max(hi, __shfl_xor(hi, sh));
+            }
+            // uniform after the reductions: scalar copies keep the group
+            // branches scalar
+            nhead = __builtin_amdgcn_readfirstlane(min(nh, ngroups));
+            dmin = __builtin_amdgcn_readfirstlane(lo);
+            dmax = __builtin_amdgcn_readfirstlane(hi);
+        }
+        bool fail = false;
         // Read codes x5 into LDS rows 1..m; rows past m get code 5 (-16 vs all).
         for (int32_t i = 1; i <= 8 * ngroups; i++)
             rcode[(size_t)i * 64] = (uint8_t)(5 * (i <= m ? read_code(pv.read[i - 1]) : 5u));
@@ -238,6 +280,27 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             for (int32_t q = 0; q < qend; q++) {
                 const int32_t j = j0 + q + 1;
                 const uint32_t clut = flut[chunk[q * 64]];
+                // Stripe groups at columns j - 1, j and j + 1 (uniform): rows
+                // [j - dmax - h, j - dmin + h] meet groups glo .. ghi.
+                int32_t glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+                if constexpr (PRUNE) {
+#pragma unroll
+                    for (int t3 = 0; t3 < 3; t3++) {
+                        const int32_t jj = j - 1 + t3;
+                        const int32_t rlo = jj - dmax - a.prune_half, rhi = jj - dmin + a.prune_half;
+                        glo[t3] = rlo <= 1 ? 0 : (rlo - 1) >> 3;
+                        ghi[t3] = rhi < 1 ? -1 : (rhi - 1) >> 3;
+                    }
+                }
+                // group g is computed at column j - 1 + t3
+                auto computed = [&](int g, int t3) {
+                    return g < ngroups && (g < nhead || (g >= glo[t3] && g <= ghi[t3]));
+                };
+                // m and the threshold base, opaque per column: otherwise every
+                // row's exit predicate is hoisted out of the column loop and
+                // held in registers.
+                int32_t mc = m, tbc = tbase;
+                if constexpr (PRUNE) asm volatile("" : "+v"(mc), "+v"(tbc));
                 // row 0: H(0, j) = 0, E(0, j) = NEG; the diagonal of row 1 is
                 // H(0, j-1) = 0, i.e. U = oe in the low half of pdiag
                 int32_t Uc = oe, Bc = NEG + ext, ckey = 0, kpend = 0;
@@ -245,7 +308,17 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
                 uint32_t *tcol = tr + (size_t)(j - 1) * WR * 64;
 #pragma unroll
                 for (int g = 0; g < WR; g++) {
-                    if (g < ngroups) {
+                    if (PRUNE ? computed(g, 1) : g < ngroups) {
+                        // Pruned: after a skipped group the carries come from margins,
+                        // and the diagonal from column j - 1 only if row 8g was
+                        // computed there.
+                        if constexpr (PRUNE) {
+                            if (g > 0 && !computed(g - 1, 1)) {
+                                Uc = oe;
+                                Bc = NEG + ext;
+                                pdiag = (j > 1 && computed(g - 1, 0)) ? Pk[8 * g - 1] : ((uint32_t)oe & 0xFFFFu);
+                            }
+                        }
                         uint32_t acc = 0;
 #pragma unroll
                         for (int r = 0; r < 8; r++) {
@@ -299,6 +372,27 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
                             }
                         }
                         tcol[g * 64] = acc;
+                        // Pruned exit checks, on H(i, j) = the low half of Pk[i - 1]
+                        // minus oe (H bounds E and F): every row when the group
+                        // is dropped at j + 1 (right and diagonal exits, j < n),
+                        // the last row when the group below is not computed at j
+                        // or j + 1 (down and diagonal exits, i < m).
+                        if constexpr (PRUNE) {
+                            const bool chk_all = !computed(g, 2);
+                            const bool chk_last = !computed(g + 1, 1) || !computed(g + 1, 2);
+                            if (chk_all || chk_last) {
+#pragma unroll
+                                for (int r = 0; r < 8; r++) {
+                                    const int i = 8 * g + r + 1;
+                                    if (chk_all || r == 7) {
+                                        const int32_t H = (int32_t)(int16_t)(Pk[i - 1] & 0xFFFFu) - oe;
+                                        const bool side = chk_all && j < n && i <= mc;
+                                        const bool down = r == 7 && chk_last && j <= n && i < mc;
+                                        fail |= (side || down) && H >= tbc + a.prune_s * i;
+                                    }
+                                }
+                            }
+                        }
                     }
                 }
                 if (j <= n && ckey > bestkey) {
@@ -307,7 +401,9 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
                 }
             }
         }
-        if (active) {
+        if (PRUNE && active && fail) {
+            a.fb_list[atomicAdd(&a.fb_range[1], 1u)] = p;
+        } else if (active) {
             int32_t best = bestkey >> 16, bi = 0xFFFF - (bestkey & 0xFFFF);
             if (best <= 0) {
                 best = 0;
@@ -331,19 +427,29 @@ template <int MAXM>
 size_t full_lds() { return 32 + (size_t)(MAXM + 1) * 64 + (size_t)FULL_CHUNK * 64; }
 
 // The folded LUT when its fields fit, else the plain one plus -oe per cell.
-template <int MAXM>
-hipError_t launch_full_t(const GotohArgs &a, int blocks, hipStream_t s) {
+template <int MAXM, bool PRUNE>
+hipError_t launch_full_p(const GotohArgs &a, int blocks, hipStream_t s) {
     if (a.flut_folded)
-        hipLaunchKernelGGL((k_gotoh_full<MAXM, true>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
+        hipLaunchKernelGGL((k_gotoh_full<MAXM, true, PRUNE>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
     else
-        hipLaunchKernelGGL((k_gotoh_full<MAXM, false>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
+        hipLaunchKernelGGL((k_gotoh_full<MAXM, false, PRUNE>), dim3(blocks), dim3(64), full_lds<MAXM>(), s, a);
     return hipGetLastError();
+}
+
+// The pruned pass exists for tiles up to 192 rows (at 264 its checks stop the
+// group loop from unrolling and the row state would leave the registers).
+template <int MAXM>
+hipError_t launch_full_t(const GotohArgs &a, bool prune, int blocks, hipStream_t s) {
+    if constexpr (MAXM <= FULL_PRUNE_MAX_ROWS)
+        if (prune) return launch_full_p<MAXM, true>(a, blocks, s);
+    if (prune) return hipErrorInvalidValue;
+    return launch_full_p<MAXM, false>(a, blocks, s);
 }
 
 template <int MAXM>
 int full_blocks_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_full<MAXM, true>, 64, full_lds<MAXM>()) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_full<MAXM, true, false>, 64, full_lds<MAXM>()) !=
         hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
@@ -355,14 +461,83 @@ int full_rows(uint32_t maxm) {
     return maxm <= 64 ? 64 : maxm <= 128 ? 128 : maxm <= 160 ? 160 : maxm <= 192 ? 192 : 264;
 }
 
-hipError_t launch_gotoh_full(const GotohArgs &a, int rows, int blocks, hipStream_t s) {
+hipError_t launch_gotoh_full(const GotohArgs &a, int rows, bool prune, int blocks, hipStream_t s) {
     switch (rows) {
-    case 64: return launch_full_t<64>(a, blocks, s);
-    case 128: return launch_full_t<128>(a, blocks, s);
-    case 160: return launch_full_t<160>(a, blocks, s);
-    case 192: return launch_full_t<192>(a, blocks, s);
-    default: return launch_full_t<264>(a, blocks, s);
+    case 64: return launch_full_t<64>(a, prune, blocks, s);
+    case 128: return launch_full_t<128>(a, prune, blocks, s);
+    case 160: return launch_full_t<160>(a, prune, blocks, s);
+    case 192: return launch_full_t<192>(a, prune, blocks, s);
+    default: return launch_full_t<264>(a, prune, blocks, s);
     }
+}
+
+namespace {
+
+// Escalation sort key: head-group class * ESC_NDIAG + diagonal bucket, or
+// ESC_NB for a pair the pruned pass cannot help (no positive band score, no
+// positive substitution score, or a head of more than 12 groups or covering
+// the read).
+__device__ __forceinline__ uint32_t esc_key(const EscSortArgs &a, uint32_t p) {
+    const int32_t L = a.esc_info[2 * (size_t)p], d = a.esc_info[2 * (size_t)p + 1];
+    const int32_t m = (int32_t)a.read_len[p];
+    if (a.s <= 0 || L <= 0) return ESC_NB;
+    const int32_t nh = esc_head_groups(m, L, a.s, a.pad);
+    if (nh > 12 || 8 * nh >= m) return ESC_NB;
+    const int32_t cls = nh <= 3 ? 0 : (nh <= 6 ? 1 : 2);
+    const int32_t db = min(max((d + 256) >> 1, 0), ESC_NDIAG - 1);
+    return (uint32_t)(cls * ESC_NDIAG + db);
+}
+
+__global__ void k_esc_hist(EscSortArgs a) {
+    const uint32_t n = *a.count;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t k = esc_key(a, a.list[t]);
+        if (k < ESC_NB) atomicAdd(&a.esc->hist[k], 1u);
+    }
+}
+
+// One block of 1024 threads: exclusive scan of the ESC_NB bucket counts into
+// the scatter cursors, three buckets a thread.
+__global__ __launch_bounds__(1024) void k_esc_scan(EscSortArgs a) {
+    static_assert(ESC_NB == 3 * 1024, "three buckets a thread");
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t h0 = a.esc->hist[3 * t], h1 = a.esc->hist[3 * t + 1], h2 = a.esc->hist[3 * t + 2];
+    part[t] = h0 + h1 + h2;
+    __syncthreads();
+    for (uint32_t sh = 1; sh < 1024; sh <<= 1) {
+        const uint32_t v = t >= sh ? part[t - sh] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const uint32_t base = part[t] - (h0 + h1 + h2);
+    a.esc->cursor[3 * t] = base;
+    a.esc->cursor[3 * t + 1] = base + h0;
+    a.esc->cursor[3 * t + 2] = base + h0 + h1;
+    if (t == 1023) a.esc->prange[1] = part[t];
+}
+
+__global__ void k_esc_scatter(EscSortArgs a) {
+    const uint32_t n = *a.count;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t p = a.list[t];
+        const uint32_t k = esc_key(a, p);
+        if (k < ESC_NB)
+            a.plist[atomicAdd(&a.esc->cursor[k], 1u)] = p;
+        else
+            a.fb2list[atomicAdd(&a.esc->fb2range[1], 1u)] = p;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_esc_sort(const EscSortArgs &a, uint32_t bound, hipStream_t s) {
+    const uint32_t blocks = std::max(1u, std::min(1024u, (bound + 255) / 256));
+    hipLaunchKernelGGL(k_esc_hist, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_esc_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_esc_scatter, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 int gotoh_full_blocks_per_cu(int rows) {

@@ -198,6 +198,24 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     espk::PlanDev *dp = pa.plan;
     a.esc_range = dp->esc_range;
     a.esc_list = pa.order + b->npairs;
+    // Pruned reruns: EscDev, then per-pair band results (2 x int32), the sorted
+    // list and the two fallback lists (npairs entries each).
+    const int prune_s = max_subst(p);
+    const bool prune = hint_band && c->full_prune && prune_s > 0;
+    c->last_pruned = false;
+    espk::EscDev *esc = nullptr;
+    uint32_t *plist = nullptr, *fblist = nullptr, *fb2list = nullptr;
+    if (prune) {
+        const size_t head = align_up(sizeof(espk::EscDev), 256);
+        HIP_TRY(c, c->escbuf.ensure(head + (size_t)b->npairs * 20));
+        uint8_t *base = (uint8_t *)c->escbuf.p;
+        esc = (espk::EscDev *)base;
+        a.esc_info = (int32_t *)(base + head);
+        plist = (uint32_t *)(base + head + (size_t)b->npairs * 8);
+        fblist = plist + b->npairs;
+        fb2list = fblist + b->npairs;
+        HIP_TRY(c, espk::launch_zero_u32((uint32_t *)esc, sizeof(espk::EscDev) / 4, c->stream));
+    }
 
     // 2. Band kernel per half specialisation (an empty class exits at once);
     //    extend_hint's band attempt uses only the half-30 class.
@@ -235,26 +253,73 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     // 4. extend_hint attempt 2: the full matrix for every band pair that
     //    escalated (its count is known only on the device; the grid drains
     //    the list and exits).  aux already holds 1 for these pairs.
+    //    With pruning the list is first sorted by (head groups, band-maximum
+    //    diagonal).  Pass A takes the sorted list 64 pairs a wave; pass B takes
+    //    A's failures one pair a wave with a wider stripe and a deeper head;
+    //    the unpruned kernel drains B's failures and the pairs no pruned pass
+    //    can help (usually nothing: it then exits at once).
     if (pa.hint_mode && hp->range[1][1]) {
         espk::GotohArgs f = a;
         f.mode = ESP_MODE_FULL;
         f.aux = nullptr;
         f.touch = nullptr;  // extend_hint returns no TouchRec
+        f.esc_info = nullptr;
         f.range = dp->esc_range;
         f.counter = &dp->counters[espk::PLAN_NCLASS];
         const uint64_t bound = hp->range[1][1];
-        {
-            const int rows = espk::full_rows(hp->esc_maxm);
-            const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
-            if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
-            uint64_t blocks = (uint64_t)c->ncu * c->full_blocks[rows & 511];
-            blocks = std::min<uint64_t>(blocks, (bound + 63) / 64);
-            blocks = std::min<uint64_t>(blocks, c->full_trace_budget / (maxw * 64 * 4));
-            blocks = std::max<uint64_t>(blocks, 1);
-            HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * blocks));
-            f.trace = (uint32_t *)c->ftrace.p;
-            f.maxw = (uint32_t)maxw;
-            HIP_TRY(c, espk::launch_gotoh_full(f, rows, (int)blocks, c->stream));
+        const int rows = espk::full_rows(hp->esc_maxm);
+        const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
+        if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
+        uint64_t blocks = (uint64_t)c->ncu * c->full_blocks[rows & 511];
+        blocks = std::min<uint64_t>(blocks, (bound + 63) / 64);
+        blocks = std::min<uint64_t>(blocks, c->full_trace_budget / (maxw * 64 * 4));
+        blocks = std::max<uint64_t>(blocks, 1);
+        HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * blocks));
+        f.trace = (uint32_t *)c->ftrace.p;
+        f.maxw = (uint32_t)maxw;
+        if (prune && rows <= espk::FULL_PRUNE_MAX_ROWS) {
+            espk::EscSortArgs sa{};
+            sa.list = a.esc_list;
+            sa.count = &dp->esc_range[1];
+            sa.read_len = b->read_len;
+            sa.esc_info = a.esc_info;
+            sa.s = prune_s;
+            sa.pad = c->prune_pad_a;
+            sa.esc = esc;
+            sa.plist = plist;
+            sa.fb2list = fb2list;
+            HIP_TRY(c, espk::launch_esc_sort(sa, (uint32_t)bound, c->stream));
+            tim_mark(c, 12, 0);
+            c->last_pruned = true;
+            espk::GotohArgs g = f;
+            g.order = plist;
+            g.range = esc->prange;
+            g.counter = &esc->pcounter;
+            g.esc_info = a.esc_info;
+            g.prune_s = prune_s;
+            g.prune_half = c->prune_half_a;
+            g.prune_pad = c->prune_pad_a;
+            g.fb_list = fblist;
+            g.fb_range = esc->fbrange;
+            HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
+            tim_mark(c, 7, rows);
+            g.order = fblist;
+            g.range = esc->fbrange;
+            g.counter = &esc->fbcounter;
+            g.prune_half = c->prune_half_b;
+            g.prune_pad = c->prune_pad_b;
+            g.fb_list = fb2list;
+            g.fb_range = esc->fb2range;
+            g.unit = 1;
+            HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
+            tim_mark(c, 13, rows);
+            f.order = fb2list;
+            f.range = esc->fb2range;
+            f.counter = &esc->fb2counter;
+            HIP_TRY(c, espk::launch_gotoh_full(f, rows, false, (int)blocks, c->stream));
+            tim_mark(c, 11, rows);
+        } else {
+            HIP_TRY(c, espk::launch_gotoh_full(f, rows, false, (int)blocks, c->stream));
             tim_mark(c, 7, rows);
         }
     }
@@ -319,6 +384,7 @@ int esp_destroy(esp_ctx *c) {
     c->wdesc.release();
     c->wmyers.release();
     c->eng.release();
+    c->escbuf.release();
     c->hstage.release();
     c->hout.release();
     for (int k = 0; k <= esp_ctx::kMaxEv; k++)
@@ -403,6 +469,28 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
         if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full kernel choice not in {0,1}");
         c->full_kernel = (int)value;
         return ESP_OK;
+    case ESP_OPT_FULL_PRUNE:
+        if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full-matrix pruning not in {0,1}");
+        c->full_prune = (int)value;
+        return ESP_OK;
+    case ESP_OPT_FULL_PRUNE_SHAPE: {
+        if (value == 0) {
+            c->prune_half_a = espk::ESC_HALF_A;
+            c->prune_pad_a = espk::ESC_PAD_A;
+            c->prune_half_b = espk::ESC_HALF_B;
+            c->prune_pad_b = espk::ESC_PAD_B;
+            return ESP_OK;
+        }
+        const int f[4] = {(int)(value & 255), (int)((value >> 8) & 255), (int)((value >> 16) & 255),
+                          (int)((value >> 24) & 255)};
+        if (value < 0 || value > 0xFFFFFFFFll || !f[0] || !f[1] || !f[2] || !f[3])
+            return fail(c, ESP_E_INVAL, "pruning shape fields must be 1..255");
+        c->prune_half_a = f[0];
+        c->prune_pad_a = f[1];
+        c->prune_half_b = f[2];
+        c->prune_pad_b = f[3];
+        return ESP_OK;
+    }
     case ESP_OPT_TRACE_BUDGET:
         if (value < (1 << 20)) return fail(c, ESP_E_INVAL, "trace budget below 1 MiB");
         c->trace_budget = (size_t)value;
@@ -498,6 +586,17 @@ int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
     for (int k = 0; k < espk::PLAN_STRIP; k++) out->band_pairs[k] = sum.range[k][1];
     out->strip_pairs = sum.range[espk::PLAN_STRIP][1];
     out->escalated = esc[1];
+    out->full_unpruned = esc[1];
+    if (c->last_pruned && c->escbuf.p) {
+        uint32_t fb[2], fa[2];
+        const espk::EscDev *ed = (const espk::EscDev *)c->escbuf.p;
+        HIP_TRY(c, hipMemcpy(fb, ed->fb2range, sizeof(fb), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(fa, ed->fbrange, sizeof(fa), hipMemcpyDeviceToHost));
+        out->full_unpruned = fb[1];
+        out->full_pruned = esc[1] - fb[1];
+        // pass B's failures are in fb2 beside the pairs no pass could help
+        out->full_second = fa[1];
+    }
     return ESP_OK;
 }
 

@@ -58,6 +58,21 @@ struct GotohArgs {
     // through the counter esc_range[1].
     uint32_t *esc_range;
     uint32_t *esc_list;
+    // With esc_info set, an escalating pair p also records its band attempt's
+    // best score and the diagonal bj - bi of its best cell at esc_info[2p],
+    // esc_info[2p + 1]: the pruned full-matrix pass reads them.
+    int32_t *esc_info;
+    // Pruned full-matrix pass (k_gotoh_full with pruning): prune_s is the
+    // largest substitution score; a pair whose check fails is appended to
+    // fb_list through the counter fb_range[1] instead of being traced back.
+    int32_t prune_s;
+    int32_t prune_half;  // stripe rows each side of a pair's diagonal
+    int32_t prune_pad;   // head rows past the last row a 0 start can reach L from
+    uint32_t *fb_list;
+    uint32_t *fb_range;
+    // Pairs per work unit of the full-matrix kernel: 64 (0 means 64) or 1
+    // (one pair a wave, for short lists whose pairs share no stripe).
+    uint32_t unit;
     // Always 0 (value-initialised).  Read from the argument block, the band
     // kernel's floor at 0 is an operand the compiler cannot see through, so
     // F's floor and H's max stay two v_max3.
@@ -167,10 +182,53 @@ hipError_t launch_gotoh_band(const GotohArgs &a, int half, int blocks, hipStream
 int gotoh_band_blocks_per_cu(int half);
 __host__ __device__ inline uint64_t band_trace_words(uint64_t m, uint64_t half) { return m * ((2 * half + 1 + 7) / 8); }
 // Full-matrix kernel for extend_hint's second attempt (reads up to 264
-// bases): rows = full_rows(max read length) picks the register tile.
+// bases): rows = full_rows(max read length) picks the register tile.  With
+// prune set it computes only the head rows and a diagonal stripe of each
+// wave's pairs (esc_info, prune_s, fb_list / fb_range must be set).
 int full_rows(uint32_t maxm);
-hipError_t launch_gotoh_full(const GotohArgs &a, int rows, int blocks, hipStream_t s);
+constexpr int FULL_PRUNE_MAX_ROWS = 192;  // largest row tile with a pruned pass
+hipError_t launch_gotoh_full(const GotohArgs &a, int rows, bool prune, int blocks, hipStream_t s);
 int gotoh_full_blocks_per_cu(int rows);
+
+// Ordering of the escalation list for the pruned pass: a counting sort by
+// (head-row class, diagonal of the band attempt's best cell), so that the 64
+// pairs of a wave share one stripe; pairs the pruned pass cannot help go to
+// the fallback list at once.  Every count lives in EscDev (zeroed before).
+// The first pruned pass takes the sorted list with (ESC_HALF_A, ESC_PAD_A),
+// the second its failures one pair a wave with (ESC_HALF_B, ESC_PAD_B): a
+// stripe row costs the read's span of columns, a head row the whole window.
+constexpr int ESC_HALF_A = 32, ESC_PAD_A = 16;
+constexpr int ESC_HALF_B = 64, ESC_PAD_B = 16;
+constexpr int ESC_NDIAG = 1024;    // diagonal buckets per class (two diagonals each)
+constexpr int ESC_NCLASS = 3;      // head groups <= 3, <= 6, <= 12
+constexpr int ESC_NB = ESC_NCLASS * ESC_NDIAG;
+struct EscDev {
+    uint32_t prange[2];   // {0, pairs in the sorted list}
+    uint32_t fbrange[2];  // {0, failures of the first pruned pass}
+    uint32_t fb2range[2]; // {0, pairs for the unpruned kernel}
+    uint32_t pcounter, fbcounter, fb2counter;  // work-queue heads of the three passes
+    uint32_t hist[ESC_NB];
+    uint32_t cursor[ESC_NB];
+};
+struct EscSortArgs {
+    const uint32_t *list;   // escalated pairs
+    const uint32_t *count;  // device count of list
+    const uint32_t *read_len;
+    const int32_t *esc_info;
+    int32_t s;              // largest substitution score
+    int32_t pad;            // head pad of the first pruned pass
+    EscDev *esc;
+    uint32_t *plist;
+    uint32_t *fb2list;  // pairs no pruned pass can help go straight to the unpruned kernel
+};
+hipError_t launch_esc_sort(const EscSortArgs &a, uint32_t bound, hipStream_t s);
+// Head groups (8 rows each) a pruned pass computes for a pair: every row
+// from which a path starting at 0 can still reach L, plus pad rows.
+__host__ __device__ inline int32_t esc_head_groups(int32_t m, int32_t L, int32_t s, int32_t pad) {
+    const int32_t r = m - (L + s - 1) / s;
+    const int32_t need = r > 0 ? r : 0;
+    return (need + pad + 7) >> 3;
+}
 // maxm: the longest pattern of the launch (1..256 after clamping).
 hipError_t launch_myers(const MyersArgs &a, int maxm, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);

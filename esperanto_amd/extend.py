@@ -36,7 +36,8 @@ class ExtendBuffer:
         return self._h
 
     def set_option(self, option: int, value: int) -> None:
-        """esp_ctx_set_option (ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET)."""
+        """esp_ctx_set_option (ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET,
+        ESP_OPT_BAND_GRID, ESP_OPT_FULL_PRUNE)."""
         _native.check(_native.lib().esp_ctx_set_option(self._h, option, value), self._h)
 
     def close(self) -> None:

@@ -205,8 +205,11 @@ int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
  * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
  * 6 selection / stitching, 7 extend_hint full-matrix rerun of escalated band
- * attempts, 8 placement walk kernel, 9 fast lane kernel, 10 a whole graph
- * launch), half (band kernel) and ms. */
+ * attempts (the first pruned pass when pruning is on), 8 placement walk
+ * kernel, 9 fast lane kernel, 10 a whole graph launch, 11 the unpruned
+ * full-matrix pass over what the pruned passes could not prove, 12 the sort
+ * of the escalation list, 13 the second pruned pass), half (band kernel) and
+ * ms. */
 typedef struct {
     int32_t kind;
     int32_t half;
@@ -218,8 +221,16 @@ int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
  * full-matrix reruns: 0 by batch size (default), 1 lane per pair; both run
  * the lane-per-pair kernel in this build (2, wave per pair, is refused).
  * ESP_OPT_TRACE_BUDGET caps the traceback scratch of one launch in bytes
- * (the full-matrix kernels get twice that); grids shrink to fit. */
-enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2, ESP_OPT_BAND_GRID = 3 };
+ * (the full-matrix kernels get twice that); grids shrink to fit.
+ * ESP_OPT_FULL_PRUNE (1, the default, or 0): the full-matrix reruns first
+ * take a pruned pass that computes only the cells that can still reach the
+ * band attempt's score, proves per pair that no skipped cell could, and
+ * leaves every pair it cannot prove to the unpruned kernel; results are the
+ * same either way.  ESP_OPT_FULL_PRUNE_SHAPE sets the stripe half-width and
+ * head pad (rows) of the two pruned passes: value = halfA | padA << 8 |
+ * halfB << 16 | padB << 24, each 1..255 (0 restores 32, 16, 64, 16). */
+enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2, ESP_OPT_BAND_GRID = 3, ESP_OPT_FULL_PRUNE = 4,
+       ESP_OPT_FULL_PRUNE_SHAPE = 5 };
 /* ESP_OPT_BAND_GRID sets one band half's persistent grid as esp_autotune_band
  * would: value = half << 16 | blocks per CU (1 .. the half's occupancy
  * limit; 0 restores the limit). */
@@ -227,12 +238,17 @@ int esp_ctx_set_option(esp_ctx *ctx, int option, int64_t value);
 
 /* Routing of the last Gotoh batch (or pipeline) on this context: pairs per
  * band-kernel half (16, 30, 32, 48, 64), pairs in the general strip kernel,
- * and extend_hint band attempts that escalated to the full matrix.  Waits
- * for the context stream. */
+ * extend_hint band attempts that escalated to the full matrix, and of those
+ * the pairs the pruned pass finished and the pairs the unpruned kernel ran
+ * (every escalated pair when pruning is off).  Waits for the context
+ * stream. */
 typedef struct {
     uint32_t band_pairs[5];
     uint32_t strip_pairs;
     uint32_t escalated;
+    uint32_t full_pruned;
+    uint32_t full_unpruned;
+    uint32_t full_second;  /* escalated pairs the second pruned pass took */
 } esp_plan_stats;
 int esp_last_plan_stats(esp_ctx *ctx, esp_plan_stats *out);
 

@@ -1,0 +1,182 @@
+"""Exactness of the pruned full-matrix pass of extend_hint's escalations
+(ESP_OPT_FULL_PRUNE), through the C ABI.
+
+extend_hint (extend.rs:392-428) reruns an escalated band attempt on the full
+matrix.  The pruned pass computes only the cells that can still reach the
+band attempt's score and proves per pair that no skipped cell could; a pair
+it cannot prove goes to the unpruned kernel.  Every record, CIGAR and path
+must equal the oracle's extend_hint and the unpruned run's, on batches built
+so that the pass both succeeds (mate-rescue-like 1,950-nt windows, spliced-
+like pseudo-references whose read sits off the hint diagonal) and must fall
+back (indels that walk the path off its stripe, tandem copies that tie at
+other diagonals, heavy substitution that leaves a low band score, N and
+lower-case bytes), at read lengths of every row tile (64 .. 192 pruned; a
+batch of 200-250-nt reads takes the 264-row tile, which has no pruned pass),
+with the default and a custom scoring (EditDir Ag included).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from esperanto_amd.records import EditDir, ExtendParams
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def buf():
+    from esperanto_amd.extend import ExtendBuffer
+    say("creating device context")
+    b = ExtendBuffer(0)
+    yield b
+    b.close()
+
+
+def _rand(rng, n):
+    return ACGT[rng.integers(0, 4, n)].tobytes()
+
+
+def _subst(rng, seq, rate):
+    s = bytearray(seq)
+    for q in np.nonzero(rng.random(len(s)) < rate)[0]:
+        s[q] = b"ACGT"[(b"ACGT".index(bytes([s[q]])) + int(rng.integers(1, 4))) % 4]
+    return bytes(s)
+
+
+def _mate_like(rng, n):
+    """150-nt reads with 1% substitutions in 1,950-nt windows, hint = the planted start."""
+    out = []
+    for _ in range(n):
+        w = _rand(rng, 1950)
+        p = int(rng.integers(0, 1950 - 150))
+        out.append((_subst(rng, w[p:p + 150], 0.01), w, p, 0))
+    return out
+
+
+def _spliced_like(rng, n):
+    """Reads of 80-190 nt placed 70 nt into 600-nt windows, hint offset 40 (the
+    band attempt runs 30 diagonals off the path and touches its edge)."""
+    out = []
+    for _ in range(n):
+        w = _rand(rng, 600)
+        m = int(rng.integers(80, 191))
+        out.append((_subst(rng, w[70:70 + m], 0.01), w, 40, 0))
+    return out
+
+
+def _adversarial(rng, n):
+    out = []
+    for k in range(n):
+        kind = k % 6
+        m = int(rng.integers(50, 191))
+        wlen = int(rng.integers(m + 60, 1400))
+        w = bytearray(_rand(rng, wlen))
+        p = int(rng.integers(0, wlen - m - 20))
+        if kind == 0:  # an indel of 1-12 bases in the middle of the read
+            d = int(rng.integers(1, 13))
+            r = bytes(w[p:p + m // 2]) + (bytes(w[p + m // 2 + d:p + m + d]) if rng.random() < 0.5
+                                          else _rand(rng, d) + bytes(w[p + m // 2:p + m - d]))
+        elif kind == 1:  # a tandem copy of the locus elsewhere in the window: ties
+            r = bytes(w[p:p + m])
+            q = int(rng.integers(0, wlen - m))
+            w[q:q + m] = r
+        elif kind == 2:  # heavy substitution: a low band score, a deep head
+            r = _subst(rng, bytes(w[p:p + m]), float(rng.choice([0.08, 0.15, 0.3])))
+        elif kind == 3:  # N runs and lower-case bytes in both sequences
+            r = bytearray(_subst(rng, bytes(w[p:p + m]), 0.01))
+            for q in rng.choice(m, 4, replace=False):
+                r[q] = ord("N")
+            for q in rng.choice(wlen, 20, replace=False):
+                w[q] = w[q] | 0x20
+            w[p + 5:p + 9] = b"NNNN"
+            r = bytes(r)
+        elif kind == 4:  # a hint off by a few diagonals
+            r = _subst(rng, bytes(w[p:p + m]), 0.02)
+            p = max(0, p + int(rng.integers(-20, 21)))
+        else:  # a random read: nothing to find
+            r = _rand(rng, m)
+        out.append((r, bytes(w), p, 0))
+    return out
+
+
+def _long_reads(rng, n):
+    """200-250-nt reads: the 264-row tile, which has no pruned pass."""
+    out = []
+    for _ in range(n):
+        m = int(rng.integers(200, 251))
+        w = _rand(rng, 1200)
+        p = int(rng.integers(0, 1200 - m))
+        out.append((_subst(rng, w[p:p + m], 0.01), w, p, 0))
+    return out
+
+
+def _run(buf, part, params, prune):
+    from esperanto_amd import _native
+    from esperanto_amd.batch import DeviceBatch
+    buf.set_option(_native.ESP_OPT_FULL_PRUNE, int(prune))
+    try:
+        db = DeviceBatch(part, buf)
+        db.launch_gotoh(params, 2)
+        db.sync()
+        g = db.gotoh_results()
+        st = db.last_plan_stats()
+        kinds = [t[0] for t in db.last_timings()]
+        db.free()
+    finally:
+        buf.set_option(_native.ESP_OPT_FULL_PRUNE, 1)
+    return g, st, kinds
+
+
+def _same(g, o, label):
+    assert np.array_equal(g[0], o[0]), label
+    assert np.array_equal(g[4], o[4]), label
+    assert np.array_equal(g[5], o[5]), label
+    off = g[3]
+    for k in range(len(g[4])):
+        a = int(off[k])
+        assert np.array_equal(g[2][a:a + g[4][k]], o[2][a:a + o[4][k]]), (label, k)
+
+
+@pytest.mark.parametrize("scoring", ["default", "custom"])
+def test_pruned_full_matrix_equals_oracle_and_unpruned(buf, scoring):
+    from esperanto_amd.batch import PairBatch
+    params = ExtendParams() if scoring == "default" else ExtendParams(
+        match_score=1, mismatch=-3, gap_open=5, gap_ext=1, edit_dir=EditDir.Ag)
+    rng = np.random.default_rng(91 if scoring == "default" else 92)
+    for label, pairs in (("mate-like", _mate_like(rng, 1500)),
+                         ("spliced-like", _spliced_like(rng, 1500)),
+                         ("adversarial", _adversarial(rng, 1800)),
+                         ("long reads", _long_reads(rng, 400))):
+        part = PairBatch.from_pairs(pairs)
+        say(f"{scoring} {label}: {part.npairs} pairs, pruning on")
+        gp, sp, kp = _run(buf, part, params, True)
+        say(f"{scoring} {label}: routing {sp}")
+        say(f"{scoring} {label}: pruning off")
+        gu, su, ku = _run(buf, part, params, False)
+        _same(gp, gu, f"{label} pruned vs unpruned")
+        o = oracle.band_batch(part, params, 2, nthreads=8)
+        _same(gp, o, f"{label} pruned vs oracle")
+        assert sp["escalated"] == su["escalated"] > 0
+        assert sp["full_pruned"] + sp["full_unpruned"] == sp["escalated"]
+        assert su["full_pruned"] == 0 and su["full_unpruned"] == su["escalated"]
+        assert 12 not in ku
+        if label == "long reads":
+            assert 12 not in kp and sp["full_pruned"] == 0, sp
+            continue
+        assert 12 in kp and 13 in kp and 11 in kp
+        if label != "adversarial":
+            # the pass proves nearly every escalation of a clean read
+            assert sp["full_pruned"] >= 0.9 * sp["escalated"], sp
+        else:
+            assert sp["full_unpruned"] > 0, sp  # fallbacks were exercised
