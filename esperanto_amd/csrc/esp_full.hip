@@ -488,12 +488,28 @@ __device__ __forceinline__ uint32_t esc_key(const EscSortArgs &a, uint32_t p) {
     return (uint32_t)(cls * ESC_NDIAG + db);
 }
 
-__global__ void k_esc_hist(EscSortArgs a) {
-    const uint32_t n = *a.count;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+// Block b owns the list entries [b * C, (b + 1) * C), C = ceil(count / grid):
+// its bucket counts gather in LDS, and each touched bucket costs one global
+// atomic per block (the keys of a pipeline crowd into few buckets).
+__device__ __forceinline__ void esc_chunk(uint32_t n, uint32_t &lo, uint32_t &hi) {
+    const uint32_t c = (n + gridDim.x - 1) / gridDim.x;
+    lo = min(n, blockIdx.x * c);
+    hi = min(n, lo + c);
+}
+
+__global__ __launch_bounds__(256) void k_esc_hist(EscSortArgs a) {
+    __shared__ uint32_t h[ESC_NB];
+    for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    uint32_t lo, hi;
+    esc_chunk(*a.count, lo, hi);
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
         const uint32_t k = esc_key(a, a.list[t]);
-        if (k < ESC_NB) atomicAdd(&a.esc->hist[k], 1u);
+        if (k < ESC_NB) atomicAdd(&h[k], 1u);
     }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x)
+        if (h[b]) atomicAdd(&a.esc->hist[b], h[b]);
 }
 
 // One block of 1024 threads: exclusive scan of the ESC_NB bucket counts into
@@ -518,22 +534,40 @@ __global__ __launch_bounds__(1024) void k_esc_scan(EscSortArgs a) {
     if (t == 1023) a.esc->prange[1] = part[t];
 }
 
-__global__ void k_esc_scatter(EscSortArgs a) {
-    const uint32_t n = *a.count;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+// The same chunks as k_esc_hist: count per bucket in LDS, reserve each
+// touched bucket's run with one global atomic, then place each entry at its
+// run's base plus its rank in LDS.
+__global__ __launch_bounds__(256) void k_esc_scatter(EscSortArgs a) {
+    __shared__ uint32_t h[ESC_NB];
+    __shared__ uint32_t fb_count, fb_base;
+    for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x) h[b] = 0;
+    if (threadIdx.x == 0) fb_count = 0;
+    __syncthreads();
+    uint32_t lo, hi;
+    esc_chunk(*a.count, lo, hi);
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+        const uint32_t k = esc_key(a, a.list[t]);
+        atomicAdd(k < ESC_NB ? &h[k] : &fb_count, 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x)
+        if (h[b]) h[b] = atomicAdd(&a.esc->cursor[b], h[b]);
+    if (threadIdx.x == 0) fb_base = fb_count ? atomicAdd(&a.esc->fb2range[1], fb_count) : 0u;
+    __syncthreads();
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
         const uint32_t p = a.list[t];
         const uint32_t k = esc_key(a, p);
         if (k < ESC_NB)
-            a.plist[atomicAdd(&a.esc->cursor[k], 1u)] = p;
+            a.plist[atomicAdd(&h[k], 1u)] = p;
         else
-            a.fb2list[atomicAdd(&a.esc->fb2range[1], 1u)] = p;
+            a.fb2list[atomicAdd(&fb_base, 1u)] = p;
     }
 }
 
 }  // namespace
 
 hipError_t launch_esc_sort(const EscSortArgs &a, uint32_t bound, hipStream_t s) {
-    const uint32_t blocks = std::max(1u, std::min(1024u, (bound + 255) / 256));
+    const uint32_t blocks = std::max(1u, std::min(512u, (bound + 1023) / 1024));
     hipLaunchKernelGGL(k_esc_hist, dim3(blocks), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_esc_scan, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(k_esc_scatter, dim3(blocks), dim3(256), 0, s, a);
