@@ -242,6 +242,7 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             umaxn = max(umaxn, __shfl_xor(umaxn, sh));
         }
         const int32_t ngroups = PRUNE ? __builtin_amdgcn_readfirstlane((umaxm + 7) >> 3) : (umaxm + 7) >> 3;
+        if constexpr (PRUNE) umaxn = __builtin_amdgcn_readfirstlane(umaxn);  // a scalar column loop
         // Pruned mode: the lane's threshold base L - S * m, and the wave's head
         // groups and diagonal range (uniform).
         int32_t tbase = 0, nhead = ngroups, dmin = 0, dmax = 0;
@@ -265,6 +266,23 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             dmax = __builtin_amdgcn_readfirstlane(hi);
         }
         bool fail = false;
+        // Pruned: bit g of gmask(jj) says group g is computed at column jj
+        // (uniform): the head groups, and the groups meeting rows
+        // [jj - dmax - h, jj - dmin + h].  The masks of columns j - 1, j, j + 1
+        // roll along the sweep.
+        auto gmask = [&](int32_t jj) -> uint32_t {
+            const int32_t rlo = jj - dmax - a.prune_half, rhi = jj - dmin + a.prune_half;
+            const int32_t glo = rlo <= 1 ? 0 : (rlo - 1) >> 3;
+            const int32_t ghi = min(rhi < 1 ? -1 : (rhi - 1) >> 3, ngroups - 1);
+            uint32_t mk = (1u << nhead) - 1u;
+            if (glo <= ghi) mk |= (2u << ghi) - (1u << glo);
+            return mk;
+        };
+        uint32_t mprev = 0, mcur = 0, mnext = 0;
+        if constexpr (PRUNE) {
+            mcur = gmask(0);
+            mnext = gmask(1);
+        }
         // Read codes x5 into LDS rows 1..m; rows past m get code 5 (-16 vs all).
         for (int32_t i = 1; i <= 8 * ngroups; i++)
             rcode[(size_t)i * 64] = (uint8_t)(5 * (i <= m ? read_code(pv.read[i - 1]) : 5u));
@@ -280,21 +298,15 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             for (int32_t q = 0; q < qend; q++) {
                 const int32_t j = j0 + q + 1;
                 const uint32_t clut = flut[chunk[q * 64]];
-                // Stripe groups at columns j - 1, j and j + 1 (uniform): rows
-                // [j - dmax - h, j - dmin + h] meet groups glo .. ghi.
-                int32_t glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
                 if constexpr (PRUNE) {
-#pragma unroll
-                    for (int t3 = 0; t3 < 3; t3++) {
-                        const int32_t jj = j - 1 + t3;
-                        const int32_t rlo = jj - dmax - a.prune_half, rhi = jj - dmin + a.prune_half;
-                        glo[t3] = rlo <= 1 ? 0 : (rlo - 1) >> 3;
-                        ghi[t3] = rhi < 1 ? -1 : (rhi - 1) >> 3;
-                    }
+                    mprev = mcur;
+                    mcur = mnext;
+                    mnext = gmask(j + 1);
                 }
                 // group g is computed at column j - 1 + t3
                 auto computed = [&](int g, int t3) {
-                    return g < ngroups && (g < nhead || (g >= glo[t3] && g <= ghi[t3]));
+                    const uint32_t mk = t3 == 0 ? mprev : (t3 == 1 ? mcur : mnext);
+                    return g < 32 && ((mk >> g) & 1u);
                 };
                 // m and the threshold base, opaque per column: otherwise every
                 // row's exit predicate is hoisted out of the column loop and
@@ -308,6 +320,9 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
                 uint32_t *tcol = tr + (size_t)(j - 1) * WR * 64;
 #pragma unroll
                 for (int g = 0; g < WR; g++) {
+                    // scalar masks, opaque per group: every bit test stays at its
+                    // use instead of being hoisted into a lane mask per group
+                    if constexpr (PRUNE) asm volatile("" : "+s"(mprev), "+s"(mcur), "+s"(mnext));
                     if (PRUNE ? computed(g, 1) : g < ngroups) {
                         // Pruned: after a skipped group the carries come from margins,
                         // and the diagonal from column j - 1 only if row 8g was
