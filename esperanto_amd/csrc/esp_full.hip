@@ -241,8 +241,10 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             umaxm = max(umaxm, __shfl_xor(umaxm, sh));
             umaxn = max(umaxn, __shfl_xor(umaxn, sh));
         }
-        const int32_t ngroups = PRUNE ? __builtin_amdgcn_readfirstlane((umaxm + 7) >> 3) : (umaxm + 7) >> 3;
-        if constexpr (PRUNE) umaxn = __builtin_amdgcn_readfirstlane(umaxn);  // a scalar column loop
+        // uniform after the reductions: scalar copies keep the column loop and
+        // the group branches scalar
+        const int32_t ngroups = __builtin_amdgcn_readfirstlane((umaxm + 7) >> 3);
+        umaxn = __builtin_amdgcn_readfirstlane(umaxn);
         // Pruned mode: the lane's threshold base L - S * m, and the wave's head
         // groups and diagonal range (uniform).
         int32_t tbase = 0, nhead = ngroups, dmin = 0, dmax = 0;
