@@ -167,11 +167,13 @@ static __device__ void finish_pair(const PairView &pv, const GParams &P, int32_t
     ext->score = best;
 }
 
-// Work-queue pop by lane 0 of the wave, broadcast to all lanes.
+// Work-queue pop by lane 0 of the wave, broadcast to all lanes.  Every
+// caller runs it with the whole wave active; the scalar copy lets the
+// compiler see that the unit, and every bound derived from it, is uniform.
 __device__ __forceinline__ uint32_t wave_pop(uint32_t *counter, uint32_t lane, uint32_t step = 64u) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(counter, step);
-    return __shfl(base, 0);
+    return __builtin_amdgcn_readfirstlane(__shfl(base, 0));
 }
 
 }  // namespace espd
