@@ -270,14 +270,16 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         const int rows = espk::full_rows(hp->esc_maxm);
         const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
         if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
-        uint64_t blocks = (uint64_t)c->ncu * c->full_blocks[rows & 511];
-        blocks = std::min<uint64_t>(blocks, (bound + 63) / 64);
-        blocks = std::min<uint64_t>(blocks, c->full_trace_budget / (maxw * 64 * 4));
-        blocks = std::max<uint64_t>(blocks, 1);
-        HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * blocks));
+        const uint64_t resident = (uint64_t)c->ncu * c->full_blocks[rows & 511];
+        const uint64_t cap = std::max<uint64_t>(1, c->full_trace_budget / (maxw * 64 * 4));
+        // 64 pairs a wave; pass B (one pair a wave) may fill the chip on fewer pairs
+        const uint64_t blocks = std::max<uint64_t>(1, std::min({resident, (bound + 63) / 64, cap}));
+        const uint64_t blocks_b = std::max<uint64_t>(1, std::min({resident, bound, cap}));
+        const bool pruned = prune && rows <= espk::FULL_PRUNE_MAX_ROWS;
+        HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * (pruned ? std::max(blocks, blocks_b) : blocks)));
         f.trace = (uint32_t *)c->ftrace.p;
         f.maxw = (uint32_t)maxw;
-        if (prune && rows <= espk::FULL_PRUNE_MAX_ROWS) {
+        if (pruned) {
             espk::EscSortArgs sa{};
             sa.list = a.esc_list;
             sa.count = &dp->esc_range[1];
@@ -311,7 +313,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             g.fb_list = fb2list;
             g.fb_range = esc->fb2range;
             g.unit = 1;
-            HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
+            HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks_b, c->stream));
             tim_mark(c, 13, rows);
             f.order = fb2list;
             f.range = esc->fb2range;
