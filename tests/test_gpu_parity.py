@@ -260,7 +260,7 @@ def test_extend_hint_band_route_and_escalation(buf):
 
 @pytest.mark.skip(reason="wave-per-pair full-matrix kernel (DPP wave_shr:1 column pipeline) hung "
                          "on its first GPU run; its CPU emulation is bit-exact with the oracle and "
-                         "no cause is shown in the code, so the kernel is not built this round")
+                         "no cause is shown in the code, so the kernel is not built")
 def test_extend_hint_escalations_wave_per_pair_kernel(buf):
     """The escalation cases of the test above with ESP_OPT_FULL_KERNEL = 2."""
     from esperanto_amd import _native
