@@ -616,18 +616,20 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     if (!b || !meta || !meta->read_len || !dist)
         return fail(c, ESP_E_INVAL, "batch, host lengths and outputs are required");
     if (b->npairs == 0) return ESP_OK;
-    // Branch-free scans (they vectorize): the longest pattern of at most 256
-    // bases sizes the kernel's words; longer ones take the guard.
-    uint32_t maxm = 0, maxn = 0;
-    const uint32_t *rl = meta->read_len;
-    for (size_t k = 0; k < b->npairs; k++) maxm = std::max(maxm, rl[k] <= 256u ? rl[k] : 0u);
-    if (meta->win_len) {
-        const uint32_t *wl = meta->win_len;
-        for (size_t k = 0; k < b->npairs; k++) maxn = std::max(maxn, wl[k]);
-        if (maxn > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
-    }
+    if (b->npairs > 0xFFFFFFFFull) return fail(c, ESP_E_INVAL, "too many pairs");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, c->counter.ensure(256));
+    HIP_TRY(c, c->hout.ensure(64));
+    // The longest pattern of at most 256 bases sizes the kernel's words
+    // (longer ones take the guard) and the longest text is checked: both
+    // come from one reduction over the device length arrays, 8 bytes read
+    // back, which costs far less than scanning the host copies.
+    uint32_t *lmax = (uint32_t *)c->counter.p + 16;
+    HIP_TRY(c, espk::launch_len_max(b->read_len, b->win_len, b->npairs, lmax, c->ncu, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->hout.p, lmax, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint32_t maxm = ((const uint32_t *)c->hout.p)[0], maxn = ((const uint32_t *)c->hout.p)[1];
+    if (maxn > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
     espk::MyersArgs a{};
     a.npairs = (uint32_t)b->npairs;
     a.order = nullptr;

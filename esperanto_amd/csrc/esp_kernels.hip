@@ -489,6 +489,28 @@ __global__ void k_zero_u32(uint32_t *p, size_t n) {
     if (i < n) p[i] = 0;
 }
 
+// The longest pattern of at most 256 bases (longer ones take the guard) and
+// the longest text of a Myers batch: out[0] and out[1], zeroed beforehand.
+// The launch sizes its words from the first and checks the second.
+__global__ __launch_bounds__(256) void k_len_max(const uint32_t *__restrict__ rl,
+                                                 const uint32_t *__restrict__ wl, size_t n,
+                                                 uint32_t *out) {
+    uint32_t mm = 0, mn = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t m = rl[i];
+        mm = max(mm, m <= 256u ? m : 0u);
+        mn = max(mn, wl[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mm = max(mm, (uint32_t)__shfl_xor((int)mm, o));
+        mn = max(mn, (uint32_t)__shfl_xor((int)mn, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out, mm);
+        atomicMax(out + 1, mn);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_copy16(uint4 *dst, const uint4 *src, size_t n16) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -530,6 +552,15 @@ hipError_t launch_plan(const PlanArgs &a, int ncu, hipStream_t s) {
 }
 
 int plan_class_half(int c) { return c >= 0 && c < 5 ? kPlanHalves[c] : -1; }
+
+hipError_t launch_len_max(const uint32_t *rl, const uint32_t *wl, size_t n, uint32_t *out, int ncu,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, out, (size_t)2);
+    if (n == 0) return hipGetLastError();
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ncu * 4, (n + 255) / 256));
+    hipLaunchKernelGGL(k_len_max, dim3((unsigned)blocks), dim3(256), 0, s, rl, wl, n, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;

@@ -232,6 +232,8 @@ __host__ __device__ inline int32_t esc_head_groups(int32_t m, int32_t L, int32_t
 // maxm: the longest pattern of the launch (1..256 after clamping).
 hipError_t launch_myers(const MyersArgs &a, int maxm, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+hipError_t launch_len_max(const uint32_t *rl, const uint32_t *wl, size_t n, uint32_t *out, int ncu,
+                          hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 int gotoh_strip_blocks_per_cu(int edit);
 int myers_blocks_per_cu(int maxm, bool global);

@@ -157,7 +157,10 @@ int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
  *   end != NULL            long::infix_best_end    -> dist, end
  *   start != NULL          long::infix_best_start  -> dist, start
  *   both                   both (dist is the same minimum)
- *   neither                long::infix             -> dist only */
+ *   neither                long::infix             -> dist only
+ * The call reduces the device length arrays to the longest pattern and text
+ * (one 8-byte read-back: the pattern length sizes the kernel's words) and
+ * then returns with the kernel queued on the context stream. */
 int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     int32_t *dist, uint64_t *end, uint64_t *start);
 
