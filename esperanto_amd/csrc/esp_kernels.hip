@@ -505,9 +505,18 @@ __global__ __launch_bounds__(256) void k_len_max(const uint32_t *__restrict__ rl
         mm = max(mm, (uint32_t)__shfl_xor((int)mm, o));
         mn = max(mn, (uint32_t)__shfl_xor((int)mn, o));
     }
+    // One atomic pair per block: same-address atomics serialize in the L2,
+    // so a pair per wave over a full grid cost about 0.1 ms.
+    __shared__ uint32_t part[2][4];
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(out, mm);
-        atomicMax(out + 1, mn);
+        part[0][wave] = mm;
+        part[1][wave] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(out, max(max(part[0][0], part[0][1]), max(part[0][2], part[0][3])));
+        atomicMax(out + 1, max(max(part[1][0], part[1][1]), max(part[1][2], part[1][3])));
     }
 }
 
@@ -557,7 +566,7 @@ hipError_t launch_len_max(const uint32_t *rl, const uint32_t *wl, size_t n, uint
                           hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, out, (size_t)2);
     if (n == 0) return hipGetLastError();
-    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ncu * 4, (n + 255) / 256));
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ncu, (n + 255) / 256));
     hipLaunchKernelGGL(k_len_max, dim3((unsigned)blocks), dim3(256), 0, s, rl, wl, n, out);
     return hipGetLastError();
 }
