@@ -135,8 +135,10 @@ inline int hip_fail(esp_ctx *c, hipError_t e, const char *where) {
         if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
     } while (0)
 
-// Scoring envelope (include/esp_align.h).  Inside it the reference's i16 and
-// i32 accumulators agree and the int32 device datapath cannot overflow.
+// Scoring envelope (include/esp_align.h).  Inside it the int32 device
+// datapath cannot overflow, and the only place where the reference's i16
+// accumulator leaves the i16 range is a diagonal candidate above 32767 (the
+// strip kernel drops it for pure pairs; no other kernel admits such pairs).
 inline int check_params(esp_ctx *c, const esp_params *p) {
     if (!p) return fail(c, ESP_E_INVAL, "params is NULL");
     if (p->gap_open < 0 || p->gap_ext < 0 || p->gap_open + p->gap_ext > 16384)

@@ -9,8 +9,10 @@
 //   finish_extension     :864-1068  traceback, neutral flanks, soft clips
 //
 // Everything is int32.  Inside the scoring envelope the host validates
-// (esp_align.h), the reference's i16 and i32 accumulators produce the same
-// values, so one int32 datapath reproduces both.
+// (esp_align.h), the reference's i16 accumulator differs from its i32 one
+// only by dropping diagonal candidates above 32767 on pure pairs; the strip
+// kernel reproduces that (i16_wraps), and the band and full-matrix kernels
+// only admit scores and read lengths where it cannot happen.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -79,9 +79,12 @@ int enqueue_batch(esp_ctx *c, esp_graph *g, const esp_batch *b, const esp_params
     pa.half = b->half;
     pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
     pa.hint_mode = 0;
-    // pairs longer than at capture count as bad
+    // pairs longer than at capture, and strip pairs whose trace would not fit
+    // the captured strip scratch (a longer read with a longer window than any
+    // one captured pair), count as bad
     pa.max_read = maxm;
     pa.max_win = maxn;
+    pa.max_strip_w = maxw;
     pa.plan = (espk::PlanDev *)g->plan.p;
     pa.order = (uint32_t *)g->order.p;
     HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));

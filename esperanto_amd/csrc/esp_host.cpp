@@ -185,6 +185,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     pa.hint_mode = hint_band;
     pa.max_read = kMaxRead;
     pa.max_win = (uint32_t)kMaxWin;
+    pa.max_strip_w = ~0ull;
     pa.plan = (espk::PlanDev *)c->plan.p;
     pa.order = (uint32_t *)c->order.p;
     HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));
@@ -242,7 +243,12 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     // 3. Strip kernel for everything else, sized from the plan summary.
     HIP_TRY(c, hipEventSynchronize(c->ev_plan));
     const espk::PlanSummary *hp = (const espk::PlanSummary *)c->hplan.p;
-    if (hp->bad) return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+    if (hp->bad) {
+        // The band kernels already queued read this batch's arrays: let them
+        // finish so that the caller may free them as soon as this returns.
+        (void)hipStreamSynchronize(c->stream);
+        return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+    }
     if (hp->range[espk::PLAN_STRIP][1]) {
         a.range = &dp->sum.range[espk::PLAN_STRIP][0];
         a.counter = &dp->counters[espk::PLAN_STRIP];

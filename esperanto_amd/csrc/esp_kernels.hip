@@ -43,8 +43,12 @@ enum { R_BNDH = 0, R_BNDF = 1, R_MAX = 2, R_HL = 3, R_HR = 4 };
 
 // Fills the DP of one pair strip by strip.  Returns the best cell (first
 // row-major strict maximum, extend.rs:789-792, :823-827) and the first touch
-// (extend.rs:803-821).
-template <int T, bool EDIT>
+// (extend.rs:803-821).  WRAP16 reproduces the reference's i16 accumulator
+// (extend.rs:473-501) for pure pairs whose values can pass i16's range: a
+// diagonal candidate above 32767 wraps there to a negative value (or to
+// the -30000 sentinel) and never wins, so it is dropped; inside the scoring
+// envelope no other add of the i16 route leaves the i16 range.
+template <int T, bool EDIT, bool WRAP16>
 __device__ FillResult fill_strips(const PairView &pv, const GParams &P, const Slot &S) {
     const int32_t m = pv.m, n = pv.n, off = pv.off, half = pv.half;
     for (int32_t i = 1; i <= m; i++) {
@@ -104,7 +108,8 @@ __device__ FillResult fill_strips(const PairView &pv, const GParams &P, const Sl
                 const uint32_t fdir = fe > fo;
                 int32_t sc = rc == fc[c] ? P.match : P.mismatch;
                 if (EDIT) sc = fc[c] == partner ? 0 : sc;
-                const int32_t D = hdiag + sc;
+                const int32_t D0 = hdiag + sc;
+                const int32_t D = WRAP16 && D0 > 32767 ? NEG : D0;
                 const int32_t H = max(max(D, E), max(F, 0));
                 const uint32_t hd = H == 0 ? HD_START : (D == H ? HD_DIAG : (E == H ? HD_E : HD_F));
                 const bool valid = c >= clo && c <= chi;
@@ -242,6 +247,41 @@ __device__ TbResult traceback_strips(const PairView &pv, const Slot &S, int32_t 
     return t;
 }
 
+// run_dispatch (extend.rs:293-344) sends pairs whose every byte is one of
+// A, C, G, T, N (upper case) to the i16 kernels and every other pair to the
+// i32 one.
+__device__ bool pure_bytes(const uint8_t *s, uint32_t n) {
+    bool ok = true;
+    for (uint32_t k = 0; k < n && ok; k++) {
+        const uint32_t b = s[k];
+        ok = b == 'A' || b == 'C' || b == 'G' || b == 'T' || b == 'N';
+    }
+    return ok;
+}
+
+// True when the reference's i16 route can leave the i16 range on this pair:
+// it is pure and a diagonal candidate can exceed 32767 (a path gains at most
+// max(match, mismatch, 0) a row, so m times that bounds every H).
+__device__ bool i16_wraps(const PairView &pv, const GParams &P) {
+    const int64_t s = max(max(P.match, P.mismatch), 0);
+    if ((int64_t)pv.m * s <= 32767) return false;
+    return pure_bytes(pv.read, (uint32_t)pv.m) && pure_bytes(pv.win, (uint32_t)pv.n);
+}
+
+template <int T, bool EDIT>
+__device__ FillResult fill_route(const PairView &pv, const GParams &P, const Slot &S, bool wrap16) {
+    if (wrap16) return fill_strips<T, EDIT, true>(pv, P, S);
+    return fill_strips<T, EDIT, false>(pv, P, S);
+}
+
+// extend_hint's escalation test `score * 20 < perfect * 17` (extend.rs:424)
+// in the reference's wrapping i32 arithmetic.
+__device__ __forceinline__ bool hint_escalates(int32_t touched, int32_t best, int32_t m, int32_t match) {
+    const int32_t lhs = (int32_t)((uint32_t)best * 20u);
+    const int32_t rhs = (int32_t)((uint32_t)m * (uint32_t)match * 17u);
+    return touched || lhs < rhs;
+}
+
 template <int T, bool EDIT>
 __device__ void gotoh_one(const GotohArgs &a, uint32_t p, const Slot &S) {
     const uint32_t m = a.read_len[p], n = a.win_len[p];
@@ -283,13 +323,13 @@ __device__ void gotoh_one(const GotohArgs &a, uint32_t p, const Slot &S) {
     }
     pv.off = band ? clamp_off(a.offset[p], pv.m, pv.n) : 0;
     pv.half = band ? clamp_half(a.mode == ESP_MODE_HINT ? 30 : a.half[p], pv.n) : pv.n;
-    FillResult fr = fill_strips<T, EDIT>(pv, a.P, S);
+    const bool wrap16 = i16_wraps(pv, a.P);
+    FillResult fr = fill_route<T, EDIT>(pv, a.P, S, wrap16);
     if (a.mode == ESP_MODE_HINT && band) {
-        const int32_t perfect = pv.m * a.P.match;
-        if (fr.touched || fr.best * 20 < perfect * 17) {
+        if (hint_escalates(fr.touched, fr.best, pv.m, a.P.match)) {
             pv.off = 0;
             pv.half = pv.n;
-            fr = fill_strips<T, EDIT>(pv, a.P, S);
+            fr = fill_route<T, EDIT>(pv, a.P, S, wrap16);
             path = 1;
         }
     }
@@ -398,11 +438,13 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
         uint32_t cls = PLAN_STRIP, b = 0;
         if (active) {
             b = plan_bin(a, p, cls);
-            if (a.read_len[p] > a.max_read || a.win_len[p] > a.max_win) atomicAdd(&a.plan->sum.bad, 1u);
             const uint32_t m = a.read_len[p];
+            const uint64_t sw = strip_trace_words(m, a.win_len[p]);
+            if (m > a.max_read || a.win_len[p] > a.max_win || (cls == PLAN_STRIP && sw > a.max_strip_w))
+                atomicAdd(&a.plan->sum.bad, 1u);
             if (cls == PLAN_STRIP) {
                 smax = max(smax, m);
-                wmax = max(wmax, strip_trace_words(m, a.win_len[p]));
+                wmax = max(wmax, sw);
             } else if (a.hint_mode) {
                 // a band attempt that escalates reruns on the full matrix
                 emax = max(emax, m);

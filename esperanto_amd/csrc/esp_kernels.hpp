@@ -144,6 +144,9 @@ struct PlanArgs {
     int32_t hint_mode;  // 1 when mode is extend_hint and the scoring fits the band kernel
     uint32_t max_read;
     uint32_t max_win;
+    // strip-class pairs whose strip trace exceeds this many words are counted
+    // as bad (a captured graph's strip scratch); ~0 when the call sizes its own
+    unsigned long long max_strip_w;
     PlanDev *plan;
     uint32_t *order;
 };

@@ -373,9 +373,9 @@ int esp_server_create(int device, const esp_ref *ref, const esp_server_config *c
     if (cfg->workers < 1 || cfg->workers > 8 || cfg->max_batch_pairs < 1 || cfg->max_pending < 1 ||
         cfg->max_read_len < 1 || cfg->max_read_len > kMaxRead || ref->device != device)
         return ESP_E_INVAL;
-    if (cfg->params.gap_open < 0 || cfg->params.gap_ext < 0 || cfg->params.edit_dir < 0 ||
-        cfg->params.edit_dir > 2)
-        return ESP_E_INVAL;
+    // The batch calls check the same envelope; refusing here keeps a server
+    // from accepting requests that every batch would fail.
+    if (check_params(nullptr, &cfg->params) != ESP_OK) return ESP_E_INVAL;
     // Scratch per context: band trace, full-matrix trace, planner and slack.
     const size_t trace = (size_t)1 << 30, full_trace = (size_t)4 << 30, slack = (size_t)256 << 20;
     const uint64_t reserve = (uint64_t)cfg->workers * (trace + full_trace + slack);

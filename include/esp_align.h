@@ -22,8 +22,16 @@
  * Scoring envelope accepted by the device path (ESP_E_INVAL otherwise):
  *   gap_open >= 0, gap_ext >= 0, gap_open + gap_ext <= 16384,
  *   |match_score| <= 1024, |mismatch| <= 16384, read length <= 16000,
- *   window length <= 1 << 24.  Inside it the i16 and i32 reference kernels
- *   agree, so results equal the reference's for every byte pattern.
+ *   window length <= 1 << 24.  Inside it results equal the reference's for
+ *   every byte pattern, on both of its accumulator routes (run_dispatch,
+ *   extend.rs:293-344): inputs whose bytes are all A, C, G, T or N follow the
+ *   i16 route, whose wrapping add (extend.rs:473-501) drops every diagonal
+ *   candidate above 32767 -- reachable here when read length times
+ *   max(match_score, mismatch) exceeds 32767 -- and all other inputs follow
+ *   the i32 route.
+ *
+ * A batch call that fails after queueing kernels waits for them before it
+ * returns, so its arrays may be freed as soon as it returns.
  */
 #ifndef ESP_ALIGN_H
 #define ESP_ALIGN_H

@@ -305,11 +305,21 @@ static void empty_extension(uint32_t m, espo_ext *out, opvec *cig) {
     if (m > 0) push_op(cig, OP_S, m);
 }
 
-/* run_band<T>: extend.rs:606-851.  is16 selects the i16 Acc. */
+/* Planted mistakes for the tests that must catch them (0 = none): 1 tests
+ * the right band edge before the left one, 2 keeps the last row maximum
+ * (>=) instead of the first, 3 lets E win a tie with the diagonal, 4 takes a
+ * gap extension on a tie with the open.  Only tests/ set it. */
+static int g_mutation = 0;
+void espo_set_mutation(int k) { g_mutation = k; }
+
+/* run_band<T>: extend.rs:606-851.  is16 selects the i16 Acc.  best_cell, when
+ * not NULL, receives (best_i, best_j). */
 static void run_band(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t n,
                      const core_params *core, int is16, int has_band, int64_t b_offset,
-                     int64_t b_half, espo_ext *out, opvec *cig, espo_touch *touch) {
+                     int64_t b_half, espo_ext *out, opvec *cig, espo_touch *touch,
+                     int64_t *best_cell) {
     memset(touch, 0, sizeof(*touch));
+    if (best_cell) best_cell[0] = best_cell[1] = 0;
     if (m == 0 || n == 0) {
         empty_extension((uint32_t)m, out, cig);
         return;
@@ -389,13 +399,15 @@ static void run_band(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t 
             int32_t e_open = acc_add(&A, h_up, open_ext);
             int32_t e_ext = acc_add(&A, e_up, ext_p);
             int32_t e_val; int edir;
-            if (e_ext > e_open) { e_val = e_ext; edir = 1; } else { e_val = e_open; edir = 0; }
+            int e_better = g_mutation == 4 ? e_ext >= e_open : e_ext > e_open;
+            if (e_better) { e_val = e_ext; edir = 1; } else { e_val = e_open; edir = 0; }
             int32_t h_left = jj > lo ? h_cur[cidx - 1] : zero;
             int32_t f_left = jj > lo ? f_cur[cidx - 1] : A.NEG;
             int32_t f_open = acc_add(&A, h_left, open_ext);
             int32_t f_ext = acc_add(&A, f_left, ext_p);
             int32_t f_val; int fdir;
-            if (f_ext > f_open) { f_val = f_ext; fdir = 1; } else { f_val = f_open; fdir = 0; }
+            int f_better = g_mutation == 4 ? f_ext >= f_open : f_ext > f_open;
+            if (f_better) { f_val = f_ext; fdir = 1; } else { f_val = f_open; fdir = 0; }
             int64_t hp = jj - 1;
             int32_t h_diag = (hp >= plo && hp <= phi) ? h_prev[hp - plo] : zero;
             int32_t s = core->lut[rc * 5 + rf_codes[jj - 1]];
@@ -403,18 +415,32 @@ static void run_band(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t 
             int32_t h_val = zero;
             int hdir = H_START;
             if (f_val != A.NEG && !(h_val > f_val)) { h_val = f_val; hdir = H_FROM_F; }
-            if (e_val != A.NEG && !(h_val > e_val)) { h_val = e_val; hdir = H_FROM_E; }
-            if (diag != A.NEG && !(h_val > diag)) { h_val = diag; hdir = H_DIAG; }
+            if (g_mutation == 3) {
+                if (diag != A.NEG && !(h_val > diag)) { h_val = diag; hdir = H_DIAG; }
+                if (e_val != A.NEG && !(h_val > e_val)) { h_val = e_val; hdir = H_FROM_E; }
+            } else {
+                if (e_val != A.NEG && !(h_val > e_val)) { h_val = e_val; hdir = H_FROM_E; }
+                if (diag != A.NEG && !(h_val > diag)) { h_val = diag; hdir = H_DIAG; }
+            }
             if (h_val == zero) hdir = H_START;
             h_cur[cidx] = h_val;
             e_cur[cidx] = e_val;
             f_cur[cidx] = f_val;
             flat[off + cidx] = (uint8_t)(hdir | (edir << 2) | (fdir << 4));
-            if (h_val > row_max) { row_max = h_val; row_max_cell = (int64_t)cidx; }
+            if (g_mutation == 2 ? h_val >= row_max : h_val > row_max) {
+                row_max = h_val;
+                row_max_cell = (int64_t)cidx;
+            }
         }
         int left_strict = lo > 0;
         int right_strict = hi < n;
-        if (!touch->touched) {
+        if (!touch->touched && g_mutation == 1) {
+            if (right_strict && h_cur[len - 1] == row_max) {
+                touch->touched = 1; touch->row = i; touch->left = 0; touch->clamped = hi >= n;
+            } else if (left_strict && h_cur[0] == row_max) {
+                touch->touched = 1; touch->row = i; touch->left = 1; touch->clamped = lo <= 1;
+            }
+        } else if (!touch->touched) {
             if (left_strict && h_cur[0] == row_max) {
                 touch->touched = 1; touch->row = i; touch->left = 1; touch->clamped = lo <= 1;
             } else if (right_strict && h_cur[len - 1] == row_max) {
@@ -433,6 +459,7 @@ static void run_band(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t 
         plo = lo; phi = hi;
     }
 #undef FLAT_GROW
+    if (best_cell) { best_cell[0] = best_i; best_cell[1] = best_j; }
     finish_extension(m, core, rows, flat, rc_codes, rf_codes, n, best_i, best_j, out, cig);
     free(h_prev); free(h_cur); free(e_prev); free(e_cur); free(f_cur);
     free(rows); free(flat); free(rc_codes); free(rf_codes);
@@ -453,7 +480,7 @@ static void run_dispatch(const uint8_t *read, int64_t m, const uint8_t *rf, int6
     int pure = 1;
     for (int64_t k = 0; k < m && pure; k++) pure = pure_byte(read[k]);
     for (int64_t k = 0; k < n && pure; k++) pure = pure_byte(rf[k]);
-    run_band(read, m, rf, n, &core, pure, has_band, offset, half, out, cig, touch);
+    run_band(read, m, rf, n, &core, pure, has_band, offset, half, out, cig, touch, NULL);
 }
 
 #define INITIAL_BAND 30 /* extend.rs:126 */
@@ -470,7 +497,21 @@ int espo_run_band(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
     core_params core;
     core_new(p, &core);
     opvec cig = {cigar, 0, cap, 0};
-    run_band(read, (int64_t)m, rf, (int64_t)n, &core, is16, has_band, offset, half, out, &cig, touch);
+    run_band(read, (int64_t)m, rf, (int64_t)n, &core, is16, has_band, offset, half, out, &cig, touch,
+             NULL);
+    return finish_call(&cig, n_ops);
+}
+
+/* espo_run_band plus the best cell (best_i, best_j) the traceback starts from. */
+int espo_run_band_cell(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
+                       const espo_params *p, int is16, int has_band, int64_t offset, int64_t half,
+                       espo_ext *out, espo_touch *touch, uint32_t *cigar, size_t cap, size_t *n_ops,
+                       int64_t *best_cell) {
+    core_params core;
+    core_new(p, &core);
+    opvec cig = {cigar, 0, cap, 0};
+    run_band(read, (int64_t)m, rf, (int64_t)n, &core, is16, has_band, offset, half, out, &cig, touch,
+             best_cell);
     return finish_call(&cig, n_ops);
 }
 
@@ -508,8 +549,11 @@ int espo_extend_hint(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
         return finish_call(&cig, n_ops);
     }
     run_dispatch(read, (int64_t)m, rf, (int64_t)n, p, 1, hint_offset, INITIAL_BAND, out, &cig, &t);
-    int32_t perfect = (int32_t)m * p->match_score;
-    if (t.touched || out->score * 20 < perfect * 17) {
+    /* extend.rs:423-424 in release i32 arithmetic, which wraps on overflow
+     * (the scaled scores of the envelope can take score * 20 past 2^31). */
+    int32_t perfect = (int32_t)((uint32_t)m * (uint32_t)p->match_score);
+    int32_t lhs = (int32_t)((uint32_t)out->score * 20u), rhs = (int32_t)((uint32_t)perfect * 17u);
+    if (t.touched || lhs < rhs) {
         cig.n = 0;
         cig.overflow = 0;
         run_dispatch(read, (int64_t)m, rf, (int64_t)n, p, 0, 0, 0, out, &cig, &t);
@@ -1170,6 +1214,82 @@ int32_t espo_bf_local_affine(const uint8_t *read, size_t m, const uint8_t *rf, s
     }
     free(H); free(E); free(F);
     return (int32_t)best;
+}
+
+/* The same textbook DP held as whole matrices, read back for what the
+ * band's row scan reports, from the values alone:
+ *   best      the first maximum in row-major order (i ascending, then j),
+ *             (0, 0) when no cell is positive;
+ *   touch     the first row i whose H at the left window edge equals the
+ *             row's maximum (left = 1, clamped = lo_i <= 1), or else whose
+ *             H at a right edge below n does (left = 0, clamped = 0);
+ * with diag_cap >= 0 a diagonal candidate above diag_cap is not taken
+ * (the reference's i16 route, whose wrapped sum is negative there).
+ * Written from the definitions, not from run_band. */
+typedef struct {
+    int32_t best;
+    int64_t bi, bj;
+    int32_t touched;
+    int64_t row;
+    int32_t left, clamped;
+} espo_bf_rows_out;
+
+void espo_bf_band_rows(const uint8_t *read, size_t m, const uint8_t *rf, size_t n,
+                       const espo_params *p, int has_band, int64_t offset, int64_t half,
+                       int64_t diag_cap, espo_bf_rows_out *out) {
+    memset(out, 0, sizeof(*out));
+    if (m == 0 || n == 0) return;
+    const int64_t NINF = -((int64_t)1 << 40);
+    size_t W = n + 1;
+    int64_t *H = (int64_t *)calloc((m + 1) * W, sizeof(int64_t));
+    int64_t *E = (int64_t *)malloc((m + 1) * W * sizeof(int64_t));
+    int64_t *F = (int64_t *)malloc((m + 1) * W * sizeof(int64_t));
+    int64_t *LO = (int64_t *)malloc((m + 1) * sizeof(int64_t));
+    int64_t *HI = (int64_t *)malloc((m + 1) * sizeof(int64_t));
+    for (size_t k = 0; k < (m + 1) * W; k++) { E[k] = NINF; F[k] = NINF; }
+    int64_t go = p->gap_open, ge = p->gap_ext;
+    if (!has_band) { offset = 0; half = (int64_t)n; }
+    if (half < 0) half = 0;
+    for (size_t i = 1; i <= m; i++) {
+        int64_t c = offset + (int64_t)i;
+        c = c < 1 ? 1 : (c > (int64_t)n ? (int64_t)n : c);
+        LO[i] = c - half > 1 ? c - half : 1;
+        HI[i] = c + half < (int64_t)n ? c + half : (int64_t)n;
+        int rcode = base_code(read[i - 1]);
+        for (int64_t j = LO[i]; j <= HI[i]; j++) {
+            int fcode = base_code(rf[j - 1]);
+            int64_t s;
+            if (rcode < 4 && rcode == fcode) s = p->match_score;
+            else if (rcode < 4 && fcode < 4 && edit_exempt(p->edit_dir, "ACGT"[fcode], "ACGT"[rcode])) s = 0;
+            else s = p->mismatch;
+            size_t ij = i * W + (size_t)j, up = (i - 1) * W + (size_t)j;
+            int64_t e = H[up] - go - ge, f = H[ij - 1] - go - ge;
+            if (E[up] > NINF && E[up] - ge > e) e = E[up] - ge;
+            if (F[ij - 1] > NINF && F[ij - 1] - ge > f) f = F[ij - 1] - ge;
+            int64_t d = H[up - 1] + s, h = 0;
+            if (!(diag_cap >= 0 && d > diag_cap) && d > h) h = d;
+            if (e > h) h = e;
+            if (f > h) h = f;
+            H[ij] = h; E[ij] = e; F[ij] = f;
+        }
+    }
+    int64_t best = 0;
+    for (size_t i = 1; i <= m; i++)
+        for (int64_t j = LO[i]; j <= HI[i]; j++)
+            if (H[i * W + (size_t)j] > best) { best = H[i * W + (size_t)j]; out->bi = (int64_t)i; out->bj = j; }
+    out->best = (int32_t)best;
+    for (size_t i = 1; i <= m && !out->touched; i++) {
+        if (LO[i] > HI[i]) continue;
+        int64_t rmax = NINF;
+        for (int64_t j = LO[i]; j <= HI[i]; j++)
+            if (H[i * W + (size_t)j] > rmax) rmax = H[i * W + (size_t)j];
+        if (H[i * W + (size_t)LO[i]] == rmax) {
+            out->touched = 1; out->row = (int64_t)i; out->left = 1; out->clamped = LO[i] <= 1;
+        } else if (HI[i] < (int64_t)n && H[i * W + (size_t)HI[i]] == rmax) {
+            out->touched = 1; out->row = (int64_t)i; out->left = 0; out->clamped = 0;
+        }
+    }
+    free(H); free(E); free(F); free(LO); free(HI);
 }
 
 /* Textbook semi-global unit-cost edit distance under EA equality

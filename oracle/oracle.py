@@ -28,6 +28,13 @@ class CRef(ctypes.Structure):
                 ("iv_off", ctypes.c_void_p)]
 
 
+class CBfRows(ctypes.Structure):
+    """espo_bf_rows_out: what the brute-force matrices say of the row scan."""
+    _fields_ = [("best", ctypes.c_int32), ("bi", ctypes.c_int64), ("bj", ctypes.c_int64),
+                ("touched", ctypes.c_int32), ("row", ctypes.c_int64), ("left", ctypes.c_int32),
+                ("clamped", ctypes.c_int32)]
+
+
 class CSpliceParams(ctypes.Structure):
     """The SpliceParams fields rescore_with reads (splice.rs:21-52)."""
     _fields_ = [("known_bonus", ctypes.c_int32), ("gt_ag_bonus", ctypes.c_int32),
@@ -75,6 +82,14 @@ def lib():
                                            ctypes.c_int64, ctypes.c_int64]
         L.espo_bf_local_affine.restype = ctypes.c_int32
         L.espo_bf_ea_edit.argtypes = [u8p, sz, u8p, sz, i32p, P(sz)]
+        L.espo_run_band_cell.argtypes = [u8p, sz, u8p, sz, P(CParams), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int64, ctypes.c_int64, P(CExt), P(CTouch), u32p, sz,
+                                         P(sz), i64p]
+        L.espo_set_mutation.argtypes = [ctypes.c_int]
+        L.espo_set_mutation.restype = None
+        L.espo_bf_band_rows.argtypes = [u8p, sz, u8p, sz, P(CParams), ctypes.c_int, ctypes.c_int64,
+                                        ctypes.c_int64, ctypes.c_int64, P(CBfRows)]
+        L.espo_bf_band_rows.restype = None
         L.espo_band_batch.argtypes = [sz, u8p, u64p, u32p, u8p, u64p, u32p, i64p, i64p,
                                       P(CParams), ctypes.c_int, P(CExt), P(CTouch), u32p, u64p,
                                       u32p, i32p, ctypes.c_int]
@@ -145,6 +160,41 @@ def run_band(read: bytes, window: bytes, params: Optional[ExtendParams] = None, 
                              cig, cap, ctypes.byref(nops))
     assert rc == 0, rc
     return Extension.from_c(e, cig[: nops.value]), TouchRec.from_c(t)
+
+
+def run_band_cell(read: bytes, window: bytes, params: Optional[ExtendParams] = None, is16: bool = True,
+                  band: Optional[Tuple[int, int]] = None):
+    """run_band plus the best cell (best_i, best_j) its traceback starts from."""
+    r, m = _buf(read)
+    w, n = _buf(window)
+    cap = cigar_capacity(m)
+    cig = (ctypes.c_uint32 * cap)()
+    e, t, nops = CExt(), CTouch(), ctypes.c_size_t()
+    cell = (ctypes.c_int64 * 2)()
+    off, half = band if band is not None else (0, 0)
+    rc = lib().espo_run_band_cell(r, m, w, n, ctypes.byref(_params(params)), int(is16),
+                                  int(band is not None), off, half, ctypes.byref(e), ctypes.byref(t),
+                                  cig, cap, ctypes.byref(nops), cell)
+    assert rc == 0, rc
+    return Extension.from_c(e, cig[: nops.value]), TouchRec.from_c(t), (cell[0], cell[1])
+
+
+def set_mutation(k: int) -> None:
+    """Plants mistake k in run_band (0 restores it); for tests that must catch them."""
+    lib().espo_set_mutation(int(k))
+
+
+def bf_band_rows(read: bytes, window: bytes, params: Optional[ExtendParams] = None,
+                 band: Optional[Tuple[int, int]] = None, diag_cap: int = -1) -> dict:
+    """Brute-force matrices: best score, first row-major best cell, TouchRec."""
+    r, m = _buf(read)
+    w, n = _buf(window)
+    o = CBfRows()
+    off, half = band if band is not None else (0, 0)
+    lib().espo_bf_band_rows(r, m, w, n, ctypes.byref(_params(params)), int(band is not None), off, half,
+                            int(diag_cap), ctypes.byref(o))
+    return {"best": o.best, "cell": (o.bi, o.bj),
+            "touch": TouchRec(bool(o.touched), o.row, bool(o.left), bool(o.clamped))}
 
 
 def band(read: bytes, window: bytes, params: Optional[ExtendParams] = None,

@@ -182,3 +182,26 @@ def band_interior(seed=0x1A7E_B0D5):
                     read = read.lower()
                 cases.append((read, win, DIRS[rep % 3], off, half))
     return cases
+
+
+def i16_wrap_cases():
+    """Pure pairs whose i16 route (extend.rs:473-501) can pass 32767: scaled
+    scores on 40-nt reads, a 3000-nt read at a 5-bit match score, positive
+    mismatches.  Cases are (read, window, ExtendParams); the callers add
+    lowercase (impure, i32-route) copies where they need them."""
+    from esperanto_amd.records import ExtendParams
+    rng = Rng(0x16_0001)
+    out = []
+    for match, mm, go, ge, m, n in [(1024, -4, 4, 2, 40, 50), (900, -4, 4, 2, 40, 45),
+                                    (1024, -2048, 4000, 50, 40, 50), (1000, -300, 100, 10, 64, 90),
+                                    (15, -8, 3, 1, 3000, 3060), (1024, 1024, 0, 0, 36, 40),
+                                    (800, 300, 16, 8, 50, 70)]:
+        for rep in range(3):
+            read = rand_seq(rng, m)
+            win = rand_window(rng, read, n) if rep != 2 else rand_seq(rng, n)
+            if rep == 1:
+                b = bytearray(read)
+                b[m // 2] = ord("N")
+                read = bytes(b)
+            out.append((read, win, ExtendParams(match, mm, go, ge, DIRS[rep % 3])))
+    return out

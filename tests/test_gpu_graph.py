@@ -148,6 +148,47 @@ def test_graph_reports_pairs_longer_than_at_capture(buf):
         db.free()
 
 
+def test_graph_reports_a_long_read_in_a_long_window(buf):
+    """Captured with a long read in a short window and a short read in a long
+    one; a replay whose pair has both the long read and the long window passes
+    the separate length limits but not the strip scratch: it is reported, and
+    the other pairs are still right."""
+    from esperanto_amd.batch import DeviceBatch, PairBatch
+    from seqgen import Rng, rand_seq, rand_window
+    rng = Rng(0x6A_0001)
+    pairs = []
+    for m, n in [(200, 300), (200, 3000), (60, 90), (150, 200)]:
+        r = rand_seq(rng, m)
+        pairs.append((r, rand_window(rng, r, n), 0, 0))
+    hb = PairBatch.from_pairs(pairs)
+    P = ExtendParams()
+    db = DeviceBatch(hb, buf)
+    _clear_outputs(db)
+    db.launch_gotoh(P, 1)
+    db.sync()
+    eager = db.gotoh_results()
+    rl = hb.read_len.astype(np.uint32).copy()
+    rl[1] = 20  # at capture: a short read in the long window
+    wl = hb.win_len.astype(np.uint32).copy()
+    meta = _native.CHostMeta(rl.ctypes.data, wl.ctypes.data, hb.offset.ctypes.data, hb.half.ctypes.data)
+    pc = P.to_c()
+    h = ctypes.c_void_p()
+    _native.check(_native.lib().esp_gotoh_graph_create(buf.handle, ctypes.byref(db._cb), ctypes.byref(meta),
+                                                       ctypes.byref(pc), 1, ctypes.byref(db._out),
+                                                       ctypes.byref(h)), buf.handle)
+    try:
+        say("graph captured with (200, 300) and (20, 3000); replayed with (200, 3000)")
+        _clear_outputs(db)
+        db.sync()
+        _native.check(_native.lib().esp_graph_launch(buf.handle, h), buf.handle)
+        with pytest.raises(_native.EspError):
+            _native.check(_native.lib().esp_graph_check(buf.handle, h), buf.handle)
+        _same(db.gotoh_results(), eager, [0, 2, 3])
+    finally:
+        _native.lib().esp_graph_destroy(buf.handle, h)
+        db.free()
+
+
 def test_autotune_changes_shapes_not_results(buf):
     """esp_autotune_band picks each half's grid from 1 .. its occupancy limit
     by timing a probe; batches after it give the same records as before."""
