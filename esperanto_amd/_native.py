@@ -27,7 +27,8 @@ EXPORTS = [
     "esp_ref_device_bytes", "esp_gather_batch", "esp_mate_rescue_batch", "esp_spliced_batch",
     "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
-    "esp_unspliced_batch", "esp_window_myers_batch", "esp_junctions_upload", "esp_junctions_free",
+    "esp_unspliced_batch", "esp_window_myers_batch", "esp_window_myers_batch_noalloc",
+    "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
     "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
     "esp_graph_check", "esp_graph_destroy", "esp_autotune_band", "esp_myers_infix",
@@ -172,6 +173,7 @@ def lib():
     L.esp_mate_rescue_batch.argtypes = [vp, vp, P(CMateBatch), P(CParams), vp, vp, vp, vp]
     L.esp_window_myers_batch.argtypes = [vp, vp, vp, sz, vp, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_int, vp, vp, vp]
+    L.esp_window_myers_batch_noalloc.argtypes = L.esp_window_myers_batch.argtypes
     L.esp_unspliced_batch.argtypes = [vp, vp, P(CChainBatch), P(CParams), vp, vp, vp, vp]
     L.esp_spliced_batch.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp]
     L.esp_junctions_upload.argtypes = [vp, sz, vp, P(vp)]

@@ -291,6 +291,20 @@ inline int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t
 }
 
 
+// Window EA-Myers batch: argument checks, scratch sizes and the launches on
+// caller-provided scratch (esp_window.cpp).
+int window_myers_check(esp_ctx *c, const esp_ref *r, const esp_window_job *jobs, size_t njobs,
+                       const uint8_t *reads, uint32_t max_text, int mode, int32_t *dist, uint64_t *pos);
+size_t window_myers_desc_bytes(size_t njobs);
+size_t window_myers_arena_bytes(size_t njobs, uint32_t max_text);
+int window_myers_core(esp_ctx *c, const esp_ref *r, const esp_window_job *jobs, size_t njobs,
+                      const uint8_t *reads, uint32_t max_pattern, uint32_t max_text, int mode, int32_t *dist,
+                      uint64_t *pos, int32_t *status, uint8_t *desc, uint8_t *tarena, uint32_t *counter);
+
+// Set-up of the allocation-free entry points' device scratch on the current
+// device (esp_noalloc.hip); called by esp_create.
+void noalloc_setup(int device);
+
 // Gotoh batch on the context stream without resetting the launch timings
 // (the pipelines of esp_window.cpp time their own stages around it).
 int gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, int mode,

@@ -356,6 +356,8 @@ int esp_create(int device, esp_ctx **out) {
         return ESP_E_DEVICE;
     }
     c->ncu = prop.multiProcessorCount;
+    // the allocation-free entry points' device scratch, once per device
+    esph::noalloc_setup(device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming) != hipSuccess) {

@@ -45,6 +45,74 @@ size_t mate_desc_bytes(size_t n) {
 
 }  // namespace
 
+// The window EA-Myers batch (esp_window_myers_batch and its allocation-free
+// variant): argument checks, then the prep kernel decodes every job's text
+// into `tarena` (job k at k * tstride) and writes the Myers descriptors into
+// `desc`; the Myers kernel drains the jobs through `counter`, which is
+// cleared here by a fill kernel.  Every descriptor is written for every job,
+// so the counter is the only scratch that must start at zero.
+int esph::window_myers_check(esp_ctx *c, const esp_ref *r, const esp_window_job *jobs, size_t njobs,
+                             const uint8_t *reads, uint32_t max_text, int mode, int32_t *dist,
+                             uint64_t *pos) {
+    if (!c) return ESP_E_INVAL;
+    if (!r || (njobs && (!jobs || !reads || !dist || (mode != 2 && !pos))))
+        return fail(c, ESP_E_INVAL, "NULL argument");
+    if (mode < 0 || mode > 2) return fail(c, ESP_E_INVAL, "mode not in {0,1,2}");
+    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
+    if (njobs > 0xFFFFFF00ull || max_text > kMaxWin) return fail(c, ESP_E_INVAL, "batch too large");
+    return ESP_OK;
+}
+
+size_t esph::window_myers_desc_bytes(size_t njobs) { return njobs * (2 * 8 + 2 * 4) + 8 * 256; }
+
+size_t esph::window_myers_arena_bytes(size_t njobs, uint32_t max_text) {
+    return njobs * align_up((uint64_t)max_text + 16, 16) + 64;
+}
+
+int esph::window_myers_core(esp_ctx *c, const esp_ref *r, const esp_window_job *jobs, size_t njobs,
+                            const uint8_t *reads, uint32_t max_pattern, uint32_t max_text, int mode,
+                            int32_t *dist, uint64_t *pos, int32_t *status, uint8_t *desc, uint8_t *tarena,
+                            uint32_t *counter) {
+    const size_t n = njobs;
+    espk::WinMyersArgs a{};
+    a.njobs = (uint32_t)n;
+    a.R = r->view();
+    a.jobs = jobs;
+    a.max_pattern = max_pattern;
+    a.max_text = max_text;
+    a.tstride = align_up((uint64_t)max_text + 16, 16);
+    a.tarena = tarena;
+    Carve cv{desc};
+    a.p_off = cv.take<uint64_t>(n);
+    a.t_off = cv.take<uint64_t>(n);
+    a.p_len = cv.take<uint32_t>(n);
+    a.t_len = cv.take<uint32_t>(n);
+    a.status = status;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_winmyers_prep(a, c->stream));
+    tim_mark(c, 5, 0);
+    const int mp = (int)std::min<uint32_t>(std::max<uint32_t>(max_pattern, 1), 256);
+    uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(mp, false), (n + 63) / 64);
+    espk::MyersArgs m{};
+    m.npairs = (uint32_t)n;
+    m.reads = reads;
+    m.read_off = a.p_off;
+    m.read_len = a.p_len;
+    m.texts = a.tarena;
+    m.text_off = a.t_off;
+    m.text_len = a.t_len;
+    m.dist = dist;
+    m.end = mode == 0 ? pos : nullptr;
+    m.start = mode == 1 ? pos : nullptr;
+    m.counter = counter;
+    HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
+    HIP_TRY(c, espk::launch_myers(m, mp, (int)std::max<uint64_t>(blocks, 1), c->stream));
+    tim_mark(c, 3, 0);
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
 extern "C" {
 
 int esp_ref_upload(esp_ctx *c, size_t n_contigs, const uint32_t *len, const uint8_t *packed,
@@ -255,55 +323,14 @@ int esp_window_myers_batch(esp_ctx *c, const esp_ref *r, const esp_window_job *j
                            size_t njobs, const uint8_t *reads, uint32_t max_pattern,
                            uint32_t max_text, int mode, int32_t *dist, uint64_t *pos,
                            int32_t *status) {
-    if (!c) return ESP_E_INVAL;
-    if (!r || (njobs && (!jobs || !reads || !dist || (mode != 2 && !pos))))
-        return fail(c, ESP_E_INVAL, "NULL argument");
-    if (mode < 0 || mode > 2) return fail(c, ESP_E_INVAL, "mode not in {0,1,2}");
-    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
-    if (njobs == 0) return ESP_OK;
-    if (njobs > 0xFFFFFF00ull || max_text > kMaxWin) return fail(c, ESP_E_INVAL, "batch too large");
+    int rc = window_myers_check(c, r, jobs, njobs, reads, max_text, mode, dist, pos);
+    if (rc || njobs == 0) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
-    const size_t n = njobs;
-    espk::WinMyersArgs a{};
-    a.njobs = (uint32_t)n;
-    a.R = r->view();
-    a.jobs = jobs;
-    a.max_pattern = max_pattern;
-    a.max_text = max_text;
-    a.tstride = align_up((uint64_t)max_text + 16, 16);
-    HIP_TRY(c, c->warena.ensure(n * a.tstride + 64));
-    HIP_TRY(c, c->wdesc.ensure(n * (2 * 8 + 2 * 4) + 8 * 256));
-    a.tarena = (uint8_t *)c->warena.p;
-    Carve cv{(uint8_t *)c->wdesc.p};
-    a.p_off = cv.take<uint64_t>(n);
-    a.t_off = cv.take<uint64_t>(n);
-    a.p_len = cv.take<uint32_t>(n);
-    a.t_len = cv.take<uint32_t>(n);
-    a.status = status;
+    HIP_TRY(c, c->warena.ensure(window_myers_arena_bytes(njobs, max_text)));
+    HIP_TRY(c, c->wdesc.ensure(window_myers_desc_bytes(njobs)));
     HIP_TRY(c, c->counter.ensure(256));
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    tim_begin(c);
-    HIP_TRY(c, espk::launch_winmyers_prep(a, c->stream));
-    tim_mark(c, 5, 0);
-    const int mp = (int)std::min<uint32_t>(std::max<uint32_t>(max_pattern, 1), 256);
-    uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(mp, false), (n + 63) / 64);
-    espk::MyersArgs m{};
-    m.npairs = (uint32_t)n;
-    m.reads = reads;
-    m.read_off = a.p_off;
-    m.read_len = a.p_len;
-    m.texts = a.tarena;
-    m.text_off = a.t_off;
-    m.text_len = a.t_len;
-    m.dist = dist;
-    m.end = mode == 0 ? pos : nullptr;
-    m.start = mode == 1 ? pos : nullptr;
-    m.counter = (uint32_t *)c->counter.p;
-    HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
-    HIP_TRY(c, espk::launch_myers(m, mp, (int)std::max<uint64_t>(blocks, 1), c->stream));
-    tim_mark(c, 3, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-    return ESP_OK;
+    return window_myers_core(c, r, jobs, njobs, reads, max_pattern, max_text, mode, dist, pos, status,
+                             (uint8_t *)c->wdesc.p, (uint8_t *)c->warena.p, (uint32_t *)c->counter.p);
 }
 
 int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,

@@ -447,6 +447,21 @@ int esp_window_myers_batch(esp_ctx *ctx, const esp_ref *ref, const esp_window_jo
                            uint32_t max_text, int mode, int32_t *dist, uint64_t *pos,
                            int32_t *status);
 
+/* esp_window_myers_batch without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results; its scratch (per job: the text, max_text + 16 bytes
+ * rounded up to 16, and 24 bytes of descriptors; plus about 2.3 KiB) lives in
+ * a 32 MiB device array that every GPU the library runs on holds until the
+ * process ends.  A batch needing more returns ESP_E_INVAL before anything
+ * runs (split it).  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (several contexts or threads) race; calls queued on one
+ * context's stream run in order. */
+int esp_window_myers_batch_noalloc(esp_ctx *ctx, const esp_ref *ref, const esp_window_job *jobs,
+                                   size_t njobs, const uint8_t *reads, uint32_t max_pattern,
+                                   uint32_t max_text, int mode, int32_t *dist, uint64_t *pos,
+                                   int32_t *status);
+
 /* ---- placement walks ---------------------------------------------------------- */
 
 /* Annotated junctions (JunctionLib, crates/map/src/gtf.rs:120-169): n
