@@ -28,6 +28,8 @@ EXPORTS = [
     "esp_last_plan_stats", "esp_server_create", "esp_server_destroy", "esp_server_submit",
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
     "esp_unspliced_batch", "esp_window_myers_batch", "esp_window_myers_batch_noalloc",
+    "esp_junctions_upload_stranded", "esp_refine_junction_batch", "esp_align_spliced_batch",
+    "esp_reextend_batch_run",
     "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
     "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
@@ -126,6 +128,26 @@ class CSpliceBatch(ctypes.Structure):
                 ("n_exons", ctypes.c_size_t)]
 
 
+class CSpliceCfg(ctypes.Structure):
+    """esp_splice_params: SpliceParams (splice.rs:21-53)."""
+    _fields_ = [("min_intron", ctypes.c_uint32), ("max_intron", ctypes.c_uint32),
+                ("gt_ag_bonus", ctypes.c_int32), ("gc_ag_bonus", ctypes.c_int32), ("at_ac_bonus", ctypes.c_int32),
+                ("noncanonical_penalty", ctypes.c_int32), ("known_bonus", ctypes.c_int32),
+                ("refine_radius", ctypes.c_uint32)]
+
+
+class CSplicedChainBatch(ctypes.Structure):
+    _fields_ = [("nchain", ctypes.c_size_t), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("contig", ctypes.c_void_p), ("minus", ctypes.c_void_p),
+                ("anchors", ctypes.c_void_p), ("anc_off", ctypes.c_void_p)]
+
+
+class CReextendBatch(ctypes.Structure):
+    _fields_ = [("nplace", ctypes.c_size_t), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("contig", ctypes.c_void_p), ("pos", ctypes.c_void_p),
+                ("cigar", ctypes.c_void_p), ("cig_off", ctypes.c_void_p), ("cig_n", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -177,6 +199,11 @@ def lib():
     L.esp_unspliced_batch.argtypes = [vp, vp, P(CChainBatch), P(CParams), vp, vp, vp, vp]
     L.esp_spliced_batch.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp]
     L.esp_junctions_upload.argtypes = [vp, sz, vp, P(vp)]
+    L.esp_junctions_upload_stranded.argtypes = [vp, sz, vp, P(vp)]
+    L.esp_refine_junction_batch.argtypes = [vp, vp, vp, vp, sz, vp, P(CSpliceCfg), vp]
+    L.esp_align_spliced_batch.argtypes = [vp, vp, vp, P(CSplicedChainBatch), ctypes.c_uint32, P(CSpliceCfg),
+                                          P(CParams), P(CBatchOut), vp, vp, vp, vp, vp]
+    L.esp_reextend_batch_run.argtypes = [vp, vp, P(CReextendBatch), P(CParams), ctypes.c_int, vp, vp, vp, vp]
     L.esp_junctions_free.argtypes = [vp, vp]
     L.esp_walk_placements.argtypes = [vp, vp, vp, P(CWalkBatch), P(CParams), P(CWalkParams), vp, vp,
                                       vp]

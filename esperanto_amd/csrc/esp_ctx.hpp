@@ -88,6 +88,8 @@ struct esp_ctx {
     esph::DevBuf wquery, warena, wdesc, wmyers;
     // transcriptome-engine verification windows (text offsets and lengths)
     esph::DevBuf eng;
+    // chain preparation and re-extension: per-item state, offsets and arenas
+    esph::DevBuf jscratch, jarena;
     hipEvent_t ev_plan = nullptr;
     int band_blocks[espk::PLAN_STRIP] = {};
     esph::HostBuf hstage, hout, hplan;
@@ -116,6 +118,18 @@ struct esp_ref {
         v.ncontig = ncontig;
         return v;
     }
+};
+
+// The junction library (esp_junctions_upload*): an open-addressing hash set
+// of (contig, start, end) slots, at most a quarter full; a stranded upload
+// also keeps, per slot, the support count of the plus and of the minus
+// junction.
+struct esp_junctions {
+    int device = 0;
+    uint32_t n = 0;    // distinct triples
+    uint32_t cap = 0;  // slots
+    void *d = nullptr;       // uint4 per slot: contig, start, end, used
+    void *counts = nullptr;  // uint2 per slot (stranded uploads only)
 };
 
 namespace esph {
@@ -300,6 +314,10 @@ size_t window_myers_arena_bytes(size_t njobs, uint32_t max_text);
 int window_myers_core(esp_ctx *c, const esp_ref *r, const esp_window_job *jobs, size_t njobs,
                       const uint8_t *reads, uint32_t max_pattern, uint32_t max_text, int mode, int32_t *dist,
                       uint64_t *pos, int32_t *status, uint8_t *desc, uint8_t *tarena, uint32_t *counter);
+
+// The spliced window pipeline (esp_window.cpp) on caller-checked arguments.
+int spliced_core(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b, const esp_params *p,
+                 esp_batch_out *o, uint32_t *stitched, const uint64_t *stitch_off, esp_splice_rec *rec);
 
 // Set-up of the allocation-free entry points' device scratch on the current
 // device (esp_noalloc.hip); called by esp_create.

@@ -394,6 +394,8 @@ int esp_destroy(esp_ctx *c) {
     c->wdesc.release();
     c->wmyers.release();
     c->eng.release();
+    c->jscratch.release();
+    c->jarena.release();
     c->escbuf.release();
     c->hstage.release();
     c->hout.release();

@@ -1,0 +1,461 @@
+// esp_junction.hip -- read-aware junction refinement and what is built on it:
+// the chain preparation of align_spliced and the re-extension of soft-clipped
+// placements.
+//
+// Semantics restated from the reference (crates/map/src/):
+//   refine_junction_read  splice.rs:208-317  boundary shift over the read gap, flank match, signal, library
+//   splice_signal         splice.rs:75-104   plus tests and their minus-strand mirrors
+//   dinuc                 splice.rs:330-347  donor [p, p+2), acceptor [p-2, p), N past the contig
+//   splice_support_pass   splice.rs:107-115
+//   segment_chain         splice.rs:364-396
+//   prepare_spliced       splice.rs:545-621  guards and the pseudo-reference bounds
+//   junction_signal       splice.rs:864-868  (the intron-chain fallback, intron_chain.rs:475-486)
+//   reextend_with         align.rs:1170-1225 window, hint, acceptance
+//
+// One lane owns one query, chain or placement: each is a short serial walk
+// (at most 25 boundary shifts of 8 base comparisons; a chain's anchors), so
+// the cost is a few memory round trips per item; window decodes run a wave
+// per window as in the window pipelines.
+#include "esp_kernels.hpp"
+#include "esp_refdev.hpp"
+
+using namespace espd;
+
+namespace espk {
+
+namespace {
+
+constexpr int JBLOCK = 256;
+
+inline unsigned lane_blocks(size_t n) { return (unsigned)((n + JBLOCK - 1) / JBLOCK); }
+inline unsigned wave_blocks(size_t n) { return (unsigned)((n + JBLOCK / 64 - 1) / (JBLOCK / 64)); }
+
+__device__ __forceinline__ uint32_t up_ascii(uint32_t b) { return (b >= 'a' && b <= 'z') ? b - 32 : b; }
+
+// splice_signal: 0 GT-AG, 1 GC-AG, 2 AT-AC, 3 non-canonical.
+__device__ int splice_sig(uint32_t d0, uint32_t d1, uint32_t a0, uint32_t a1, bool minus) {
+    d0 = up_ascii(d0);
+    d1 = up_ascii(d1);
+    a0 = up_ascii(a0);
+    a1 = up_ascii(a1);
+    if (!minus) {
+        if (d0 == 'G' && d1 == 'T' && a0 == 'A' && a1 == 'G') return 0;
+        if (d0 == 'G' && d1 == 'C' && a0 == 'A' && a1 == 'G') return 1;
+        if (d0 == 'A' && d1 == 'T' && a0 == 'A' && a1 == 'C') return 2;
+        return 3;
+    }
+    if (d0 == 'C' && d1 == 'T' && a0 == 'A' && a1 == 'C') return 0;
+    if (d0 == 'C' && d1 == 'T' && a0 == 'G' && a1 == 'C') return 1;
+    if (d0 == 'G' && d1 == 'T' && a0 == 'A' && a1 == 'T') return 2;
+    return 3;
+}
+
+__device__ __forceinline__ int32_t sig_bonus(int sig, const SpliceCfgDev &sp) {
+    return sig == 0 ? sp.gt_ag : sig == 1 ? sp.gc_ag : sig == 2 ? sp.at_ac : -sp.noncanon;
+}
+
+// dinuc: the donor dinucleotide at p or the acceptor one ending at p.
+__device__ void dinuc(const RefView &R, uint32_t ctg, uint32_t p, bool donor, uint32_t &c0, uint32_t &c1) {
+    c0 = c1 = 'N';
+    if (!donor && p < 2) return;
+    const uint32_t len = R.len[ctg];
+    const uint32_t a = donor ? p : p - 2;
+    if (a >= len) return;
+    const uint64_t b = min((uint64_t)a + 2, (uint64_t)len);
+    c0 = base_ascii(R, ctg, a);
+    if ((uint64_t)a + 1 < b) c1 = base_ascii(R, ctg, a + 1);
+}
+
+// Base::from_ascii code (A0 C1 G2 T3, every other byte N = 4) and edit_equiv
+// (splice.rs:320-325): equal bases (N equals N) or A/G, T/C either way.
+__device__ __forceinline__ int32_t base4(uint32_t b) { return base_code_or(b, 4); }
+__device__ __forceinline__ bool edit_equiv(int32_t a, int32_t b) {
+    return a == b || (a ^ b) == 2;  // A0^G2 and C1^T3
+}
+
+// JunctionLib::support through the hash set: the stranded count, 0 if absent.
+__device__ uint32_t jsupport(const JunctionView &J, uint32_t c, uint32_t s, uint32_t e, bool minus) {
+    if (!J.jn) return 0u;
+    uint32_t slot = junction_hash(c, s, e) & J.jmask;
+    while (true) {
+        const uint4 v = J.jn[slot];
+        if (!v.w) return 0u;
+        if (v.x == c && v.y == s && v.z == e) {
+            const uint2 n = J.cnt[slot];
+            return minus ? n.y : n.x;
+        }
+        slot = (slot + 1) & J.jmask;
+    }
+}
+
+// refine_junction_read.  The reference indexes read[q1] only through its
+// own bounds; a query position past the read (an anchor beyond it) is
+// skipped here instead of read.
+__device__ esp_refined_junction refine_read(const RefView &R, const JunctionView &J, const SpliceCfgDev &sp,
+                                            const uint8_t *read, uint32_t read_len, uint32_t ctg, uint32_t ns,
+                                            uint32_t ne, uint32_t qa, uint32_t qbv, bool minus) {
+    esp_refined_junction best;
+    best.found = 0;
+    best.start = best.end = best.minus = best.support = 0;
+    best.signal = 0;
+    if (ctg >= R.ncontig) return best;
+    const uint32_t len = R.len[ctg];
+    const int64_t ref_gap = (int64_t)ne - (int64_t)ns, read_gap = (int64_t)qbv - (int64_t)qa;
+    if (read_gap > 20) return best;
+    const int64_t il = ref_gap - read_gap;
+    if (il < (int64_t)sp.min_intron || il > (int64_t)sp.max_intron) return best;
+    const int64_t lo = min(read_gap, (int64_t)0) - 2, hi = max(read_gap, (int64_t)0) + 2;
+    int32_t bs = 0;
+    for (int64_t delta = lo; delta <= hi; delta++) {
+        const int64_t s = (int64_t)ns + delta;
+        if (s < 0) continue;
+        const int64_t e = s + il;
+        if (e < 2 || e > (int64_t)len) continue;
+        const int64_t qb = (int64_t)qa + delta;
+        if (qb < 1 || (uint32_t)qb >= read_len) continue;
+        int32_t mt = 0;
+#pragma unroll
+        for (int64_t i = 1; i <= 4; i++) {
+            const int64_t q1 = qb - i, g1 = s - i;
+            if (q1 >= 0 && g1 >= 0 && (uint64_t)q1 < read_len &&
+                edit_equiv(base4(read[q1]), base4(base_ascii(R, ctg, (uint32_t)g1))))
+                mt++;
+            const int64_t q2 = qb + i - 1, g2 = e + i - 1;
+            if ((uint32_t)q2 < read_len && (uint64_t)q2 < read_len && g2 < (int64_t)len &&
+                edit_equiv(base4(read[q2]), base4(base_ascii(R, ctg, (uint32_t)g2))))
+                mt++;
+        }
+        uint32_t d0, d1, a0, a1;
+        dinuc(R, ctg, (uint32_t)s, true, d0, d1);
+        dinuc(R, ctg, (uint32_t)e, false, a0, a1);
+        const int32_t ad = (int32_t)(delta < 0 ? -delta : delta);
+        for (int t = 0; t < 2; t++) {
+            const bool tm = t == 0 ? minus : !minus;
+            const int sig = splice_sig(d0, d1, a0, a1, tm);
+            const uint32_t support = jsupport(J, ctg, (uint32_t)s, (uint32_t)e, tm);
+            int32_t score = mt * 50 + sig_bonus(sig, sp) * 5 - ad * 5;
+            if (support > 0) score += sp.known;
+            if (!best.found || score > bs) {
+                bs = score;
+                best.found = 1;
+                best.start = (uint32_t)s;
+                best.end = (uint32_t)e;
+                best.minus = tm ? 1u : 0u;
+                best.signal = sig;
+                best.support = support;
+            }
+        }
+    }
+    return best;
+}
+
+__global__ __launch_bounds__(JBLOCK) void k_refine(RefineArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    const esp_junction_query q = a.q[k];
+    esp_refined_junction r = refine_read(a.R, a.J, a.sp, a.reads + q.read_off, q.read_len, q.contig,
+                                         q.naive_start, q.naive_end, q.q_a, q.q_b, q.minus != 0);
+    if (!r.found && q.use_fallback && q.contig < a.R.ncontig) {
+        // intron_chain.rs:475-486: the fragment pair's own junction, with its
+        // junction_signal (splice.rs:864-868) and library support.
+        uint32_t d0, d1, a0, a1;
+        dinuc(a.R, q.contig, q.fallback_start, true, d0, d1);
+        dinuc(a.R, q.contig, q.fallback_end, false, a0, a1);
+        r.found = 2;
+        r.start = q.fallback_start;
+        r.end = q.fallback_end;
+        r.minus = q.minus ? 1u : 0u;
+        r.signal = splice_sig(d0, d1, a0, a1, q.minus != 0);
+        r.support = jsupport(a.J, q.contig, q.fallback_start, q.fallback_end, q.minus != 0);
+    }
+    a.out[k] = r;
+}
+
+// prepare_spliced up to the pseudo-reference bounds, per chain.
+__global__ __launch_bounds__(JBLOCK) void k_chain_prep(ChainPrepArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    const uint64_t a0 = a.anc_off[k], na = a.anc_off[k + 1] - a0;
+    const uint32_t ctg = a.contig[k], m = a.read_len[k], K = a.k;
+    const bool minus = a.minus[k] != 0;
+    const uint32_t *anc = a.anchors + 2 * a0;
+    uint32_t *segs = a.segs + 4 * a0;
+    // segment_chain
+    uint32_t ns = 0;
+    if (na > 0) {
+        uint32_t c0 = anc[0], c1 = anc[0] + K, c2 = anc[1], c3 = anc[1] + K;
+        for (uint64_t t = 1; t < na; t++) {
+            const uint32_t q1 = anc[2 * t - 2], r1 = anc[2 * t - 1], q2 = anc[2 * t], r2 = anc[2 * t + 1];
+            const uint32_t ref_gap = r2 > r1 + K ? r2 - (r1 + K) : 0u;
+            const uint32_t read_gap = q2 > q1 + K ? q2 - (q1 + K) : 0u;
+            if (ref_gap > a.sp.min_intron && read_gap <= 10u) {
+                segs[4 * ns] = c0;
+                segs[4 * ns + 1] = c1;
+                segs[4 * ns + 2] = c2;
+                segs[4 * ns + 3] = c3;
+                ns++;
+                c0 = q2;
+                c1 = q2 + K;
+                c2 = r2;
+                c3 = r2 + K;
+            } else {
+                c1 = q2 + K;
+                c3 = r2 + K;
+            }
+        }
+        segs[4 * ns] = c0;
+        segs[4 * ns + 1] = c1;
+        segs[4 * ns + 2] = c2;
+        segs[4 * ns + 3] = c3;
+        ns++;
+    }
+    int32_t state = 1, refusal = 0;
+    if (ns < 2) {
+        state = 0;
+        refusal = 1;
+    } else if (ctg >= a.R.ncontig) {
+        state = -1;
+    }
+    const uint8_t *read = a.reads + a.read_off[k];
+    esp_refined_junction *jr = a.jrec + a0;
+    for (uint32_t w = 0; state == 1 && w + 1 < ns; w++) {
+        const esp_refined_junction r = refine_read(a.R, a.J, a.sp, read, m, ctg, segs[4 * w + 3],
+                                                   segs[4 * (w + 1) + 2], segs[4 * w + 1], segs[4 * (w + 1)], minus);
+        jr[w] = r;
+        if (!r.found) {
+            state = 0;
+            refusal = 2;
+            break;
+        }
+        const uint32_t il = r.end - r.start;
+        if (il < a.sp.min_intron || il > a.sp.max_intron) {
+            state = 0;
+            refusal = 3;
+        } else if ((il < 50u && r.support == 0) || (r.signal == 3 && r.support == 0)) {
+            state = 0;
+            refusal = 4;
+        }
+    }
+    uint64_t total = 0;
+    uint32_t *bnd = a.bounds + 3 * a0;
+    for (uint32_t i = 0; state == 1 && i < ns; i++) {
+        const uint32_t rs = i == 0 ? (segs[2] > 30u ? segs[2] - 30u : 0u) : jr[i - 1].end;
+        const uint32_t re = i + 1 == ns ? (uint32_t)min((uint64_t)segs[4 * i + 3] + 30u, (uint64_t)a.R.len[ctg])
+                                        : jr[i].start;
+        if (re <= rs) {
+            state = 0;
+            refusal = 5;
+            break;
+        }
+        bnd[3 * i] = (uint32_t)total;
+        bnd[3 * i + 1] = rs;
+        bnd[3 * i + 2] = re;
+        total += re - rs;
+    }
+    a.state[k] = state;
+    a.refusal[k] = refusal;
+    a.nseg[k] = ns;
+    a.nex[k] = state == 1 ? ns : 0u;
+    a.plen[k] = state == 1 ? total : 0ull;
+}
+
+// Compacted exon list and zero hints for the spliced pipeline.
+__global__ __launch_bounds__(JBLOCK) void k_chain_exons(ChainPrepArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    a.hint[k] = 0;  // align_spliced extends at DiagHint offset 0
+    const uint64_t e0 = a.exon_off[k], ne = a.exon_off[k + 1] - e0;
+    const uint32_t *bnd = a.bounds + 3 * a.anc_off[k];
+    for (uint64_t i = 0; i < ne; i++) {
+        a.exons[2 * (e0 + i)] = bnd[3 * i + 1];
+        a.exons[2 * (e0 + i) + 1] = bnd[3 * i + 2];
+    }
+}
+
+__global__ __launch_bounds__(JBLOCK) void k_chain_rec(ChainPrepArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.npairs) return;
+    esp_spliced_chain_rec r;
+    r.status = a.state[k];
+    r.refusal = a.refusal[k];
+    r.n_segments = a.nseg[k];
+    r.genomic_start = r.pseudo_len = r.n_stitched = 0;
+    if (r.status == 1) {
+        const esp_splice_rec s = a.srec[k];
+        r.status = s.status;
+        r.genomic_start = s.genomic_start;
+        r.pseudo_len = s.pseudo_len;
+        r.n_stitched = s.n_stitched;
+    }
+    a.rec[k] = r;
+}
+
+// reextend_with, stage 1: the CIGAR's clips, span and aligned count, the
+// window and the hint.
+__global__ __launch_bounds__(JBLOCK) void k_reext_prep(ReextArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    const uint32_t *cg = a.cigar + a.cig_off[k];
+    const uint32_t nops = a.cig_n[k], ctg = a.contig[k], pos = a.pos[k];
+    uint32_t lead = 0, tail = 0, span = 0, aligned = 0;
+    bool seen_m = false, spliced = false;
+    for (uint32_t t = 0; t < nops; t++) {
+        const uint32_t w = cg[t], op = w & 0xFu, n = w >> 4;
+        if (op == OP_N) spliced = true;
+        else if (op == OP_S) (seen_m ? tail : lead) += n;
+        else if (op == OP_M) {
+            seen_m = true;
+            span += n;
+            aligned += n;
+        } else if (op == OP_I) aligned += n;
+        else if (op == OP_D) span += n;
+    }
+    esp_reextend_rec r;
+    r.status = 0;
+    r.pos = pos;
+    r.score = 0;
+    r.wstart = r.wlen = 0;
+    r.aligned = 0;
+    r.new_aligned = 0;
+    int32_t state = 1;
+    if (spliced) state = 2;
+    else if (ctg >= a.R.ncontig) state = -1;
+    uint32_t ws = 0, we = 0;
+    if (state == 1) {
+        const uint32_t back = lead + 30u;
+        ws = pos > back ? pos - back : 0u;
+        we = min(pos + span + tail + 30u, a.R.len[ctg]);  // u32 arithmetic, as the reference's
+        if (we <= ws) state = 3;
+        r.aligned = aligned;
+    }
+    if (state == 1) {
+        r.wstart = ws;
+        r.wlen = we - ws;
+    }
+    r.status = state == 1 ? 0 : state;
+    a.rec[k] = r;
+    a.state[k] = state;
+    a.wlen[k] = state == 1 ? we - ws : 0u;
+    a.g_read_len[k] = state == 1 ? a.read_len[k] : 0u;
+    a.hint[k] = state == 1 ? (int64_t)(pos - ws) - (int64_t)lead : 0;
+}
+
+__global__ __launch_bounds__(JBLOCK) void k_reext_gather(ReextArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * (JBLOCK / 64) + (threadIdx.x >> 6);
+    if (k >= a.n) return;
+    const uint32_t wl = a.wlen[k];
+    if (wl == 0) return;
+    const uint32_t ws = a.rec[k].wstart;
+    wave_decode(a.R, a.contig[k], ws, ws + wl, false, a.arena + a.win_off[k], lane);
+}
+
+// reextend_with, stage 3: the acceptance rule on the new extension.
+__global__ __launch_bounds__(JBLOCK) void k_reext_finish(ReextArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    if (a.state[k] != 1) {
+        a.cig_out_n[k] = 0;
+        return;
+    }
+    esp_reextend_rec r = a.rec[k];
+    const esp_ext e = a.ext[k];
+    const uint32_t *cg = a.cig_out + a.cig_out_off[k];
+    const uint32_t nops = a.cig_out_n[k];
+    uint32_t na = 0;
+    for (uint32_t t = 0; t < nops; t++) {
+        const uint32_t op = cg[t] & 0xFu;
+        if (op == OP_M || op == OP_I) na += cg[t] >> 4;
+    }
+    r.score = e.score;
+    r.new_aligned = na;
+    if (e.score > 0 && na >= r.aligned && na <= a.read_len[k]) {
+        r.status = 1;
+        r.pos = r.wstart + e.ref_start;
+    }
+    a.rec[k] = r;
+}
+
+// Exclusive scans of up to two arrays in one block: each thread sums a
+// contiguous chunk, the block scans the chunk sums, each thread writes its
+// chunk's prefixes; thread 0 of the last chunk writes the totals at [n].
+__global__ __launch_bounds__(1024) void k_scan2(const uint32_t *cnt, const uint64_t *len, uint64_t n,
+                                                uint64_t *cnt_off, uint64_t *len_off) {
+    __shared__ uint64_t sc[1024], sl[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t chunk = (n + 1023) / 1024;
+    const uint64_t b = min((uint64_t)t * chunk, n), e = min(b + chunk, n);
+    uint64_t c = 0, l = 0;
+    for (uint64_t i = b; i < e; i++) {
+        if (cnt) c += cnt[i];
+        if (len) l += len[i];
+    }
+    sc[t] = c;
+    sl[t] = l;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint64_t vc = t >= off ? sc[t - off] : 0ull, vl = t >= off ? sl[t - off] : 0ull;
+        __syncthreads();
+        sc[t] += vc;
+        sl[t] += vl;
+        __syncthreads();
+    }
+    c = sc[t] - c;
+    l = sl[t] - l;
+    for (uint64_t i = b; i < e; i++) {
+        if (cnt_off) cnt_off[i] = c;
+        if (len_off) len_off[i] = l;
+        if (cnt) c += cnt[i];
+        if (len) l += len[i];
+    }
+    if (t == 1023) {
+        if (cnt_off) cnt_off[n] = sc[1023];
+        if (len_off) len_off[n] = sl[1023];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_refine(const RefineArgs &a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_refine, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_prep(const ChainPrepArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_chain_prep, dim3(lane_blocks(a.npairs)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_exons(const ChainPrepArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_chain_exons, dim3(lane_blocks(a.npairs)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_rec(const ChainPrepArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_chain_rec, dim3(lane_blocks(a.npairs)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reext_prep(const ReextArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reext_prep, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reext_gather(const ReextArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reext_gather, dim3(wave_blocks(a.n)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_reext_finish(const ReextArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reext_finish, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan2(const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off, uint64_t *len_off,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, cnt, len, (uint64_t)n, cnt_off, len_off);
+    return hipGetLastError();
+}
+
+}  // namespace espk

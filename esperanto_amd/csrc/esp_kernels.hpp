@@ -408,6 +408,84 @@ struct WinMyersArgs {
     int32_t *status;
 };
 
+// ---- junction refinement, chain preparation, re-extension (esp_junction.hip)
+
+// The stranded junction library: hash slots (x contig, y start, z end, w
+// used) and per slot the plus / minus support counts; jn == nullptr is the
+// empty library.
+struct JunctionView {
+    const uint4 *jn;
+    const uint2 *cnt;
+    uint32_t jmask;
+};
+struct SpliceCfgDev {
+    uint32_t min_intron, max_intron;
+    int32_t gt_ag, gc_ag, at_ac, noncanon, known;
+};
+struct RefineArgs {
+    RefView R;
+    JunctionView J;
+    SpliceCfgDev sp;
+    uint32_t n;
+    const esp_junction_query *q;
+    const uint8_t *reads;
+    esp_refined_junction *out;
+};
+hipError_t launch_refine(const RefineArgs &a, hipStream_t s);
+
+struct ChainPrepArgs {
+    RefView R;
+    JunctionView J;
+    SpliceCfgDev sp;
+    uint32_t npairs, k;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig, *minus, *anchors;
+    const uint64_t *anc_off;
+    uint32_t *segs;                // 4 per anchor slot: q_start, q_end, r_start, r_end
+    esp_refined_junction *jrec;    // per anchor slot
+    uint32_t *bounds;              // 3 per anchor slot: pseudo_start, ref_start, ref_end
+    int32_t *state;                // 1 prepared, 0 refused, -1 invalid
+    int32_t *refusal;
+    uint32_t *nseg, *nex;          // segments; exons handed on (0 unless prepared)
+    uint64_t *plen;                // pseudo-reference length (0 unless prepared)
+    // after the scans
+    const uint64_t *exon_off, *pseudo_off;
+    uint32_t *exons;               // 2 per exon: ref_start, ref_end
+    int64_t *hint;
+    const esp_splice_rec *srec;    // the spliced pipeline's records
+    esp_spliced_chain_rec *rec;
+};
+hipError_t launch_chain_prep(const ChainPrepArgs &a, hipStream_t s);
+hipError_t launch_chain_exons(const ChainPrepArgs &a, hipStream_t s);
+hipError_t launch_chain_rec(const ChainPrepArgs &a, hipStream_t s);
+
+struct ReextArgs {
+    RefView R;
+    uint32_t n;
+    const uint32_t *read_len, *contig, *pos, *cigar, *cig_n;
+    const uint64_t *cig_off;
+    int32_t *state;       // 1 runs the DP, else the final status
+    uint32_t *g_read_len, *wlen;
+    int64_t *hint;
+    const uint64_t *win_off;
+    uint8_t *arena;
+    const esp_ext *ext;
+    const uint32_t *cig_out;
+    const uint64_t *cig_out_off;
+    uint32_t *cig_out_n;
+    esp_reextend_rec *rec;
+};
+hipError_t launch_reext_prep(const ReextArgs &a, hipStream_t s);
+hipError_t launch_reext_gather(const ReextArgs &a, hipStream_t s);
+hipError_t launch_reext_finish(const ReextArgs &a, hipStream_t s);
+
+// Exclusive prefix sums of up to two per-item arrays (either may be NULL)
+// into n + 1 entries each; the last entry is the total.
+hipError_t launch_scan2(const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off,
+                        uint64_t *len_off, hipStream_t s);
+
 hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
 hipError_t launch_winmyers_prep(const WinMyersArgs &a, hipStream_t s);
 hipError_t launch_unsp_prep(const UnspArgs &a, hipStream_t s);

@@ -17,13 +17,6 @@ using namespace esph;
 // (a power of two, at least four times the count) slots of (contig, start,
 // end, used), probed linearly from espk::junction_hash.  Membership is what
 // contains_any_strand asks of the sorted library, one probe or two per query.
-struct esp_junctions {
-    int device = 0;
-    uint32_t n = 0;    // distinct triples
-    uint32_t cap = 0;  // slots
-    void *d = nullptr;
-};
-
 namespace {
 
 // intron_length_penalty (splice.rs:119-121) evaluated directly.
@@ -77,20 +70,22 @@ const PenaltyTable &penalty_table() {
 
 }  // namespace
 
-extern "C" {
-
-int32_t esp_intron_length_penalty(uint32_t len) { return penalty_table().eval(len); }
-
-int esp_junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, esp_junctions **out) {
+// Builds the library's hash set from n junctions of `width` u32 each:
+// (contig, start, end) triples, or (contig, start, end, minus) quads whose
+// per-strand counts (JunctionLib::build merges identical junctions into a
+// count, gtf.rs:84-114) go into a parallel uint2 array.
+static int junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, int width, esp_junctions **out) {
     if (!c) return ESP_E_INVAL;
     if (!out || (n && !t)) return fail(c, ESP_E_INVAL, "NULL argument");
+    *out = nullptr;
     if (n > (1ull << 30)) return fail(c, ESP_E_INVAL, "too many junctions");
     uint64_t cap = 16;
     while (cap < 4 * (uint64_t)n) cap <<= 1;
     std::vector<uint32_t> slots(4 * cap, 0);
+    std::vector<uint32_t> counts(width == 4 ? 2 * cap : 0, 0);
     uint32_t distinct = 0;
     for (size_t k = 0; k < n; k++) {
-        const uint32_t *x = t + 3 * k;
+        const uint32_t *x = t + (size_t)width * k;
         uint64_t s = espk::junction_hash(x[0], x[1], x[2]) & (cap - 1);
         while (slots[4 * s + 3]) {
             if (slots[4 * s] == x[0] && slots[4 * s + 1] == x[1] && slots[4 * s + 2] == x[2]) break;
@@ -103,6 +98,7 @@ int esp_junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, esp_junctions 
             slots[4 * s + 3] = 1;
             distinct++;
         }
+        if (width == 4) counts[2 * s + (x[3] ? 1 : 0)]++;
     }
     HIP_TRY(c, hipSetDevice(c->device));
     esp_junctions *j = new esp_junctions;
@@ -111,24 +107,41 @@ int esp_junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, esp_junctions 
     j->cap = (uint32_t)cap;
     if (distinct) {
         const size_t bytes = slots.size() * sizeof(uint32_t);
-        if (hipMalloc(&j->d, bytes) != hipSuccess) {
-            delete j;
+        const size_t cbytes = counts.size() * sizeof(uint32_t);
+        hipError_t e = hipMalloc(&j->d, bytes);
+        if (e == hipSuccess && cbytes) e = hipMalloc(&j->counts, cbytes);
+        if (e != hipSuccess) {
+            esp_junctions_free(c, j);
             return fail(c, ESP_E_NOMEM, "junction library allocation failed");
         }
-        if (hipMemcpy(j->d, slots.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
-            (void)hipFree(j->d);
-            delete j;
-            return hip_fail(c, hipErrorUnknown, "junction library upload");
+        e = hipMemcpy(j->d, slots.data(), bytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess && cbytes) e = hipMemcpy(j->counts, counts.data(), cbytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            esp_junctions_free(c, j);
+            return hip_fail(c, e, "junction library upload");
         }
     }
     *out = j;
     return ESP_OK;
 }
 
+extern "C" {
+
+int32_t esp_intron_length_penalty(uint32_t len) { return penalty_table().eval(len); }
+
+int esp_junctions_upload(esp_ctx *c, size_t n, const uint32_t *t, esp_junctions **out) {
+    return junctions_upload(c, n, t, 3, out);
+}
+
+int esp_junctions_upload_stranded(esp_ctx *c, size_t n, const uint32_t *quads, esp_junctions **out) {
+    return junctions_upload(c, n, quads, 4, out);
+}
+
 int esp_junctions_free(esp_ctx *c, esp_junctions *j) {
     if (!j) return ESP_OK;
     if (c) (void)hipSetDevice(j->device);
     if (j->d) (void)hipFree(j->d);
+    if (j->counts) (void)hipFree(j->counts);
     delete j;
     return ESP_OK;
 }

@@ -458,6 +458,20 @@ int esp_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b,
         !b->exon_off || !b->pseudo_off)
         return fail(c, ESP_E_INVAL, "batch arrays are required");
     HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    rc = spliced_core(c, r, b, p, o, stitched, stitch_off, rec);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
+}  // extern "C"
+
+// The spliced window pipeline on the context stream (no timing reset):
+// pseudo-reference prep, extend_hint, stitch.  Arguments already checked.
+int esph::spliced_core(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b, const esp_params *p,
+                       esp_batch_out *o, uint32_t *stitched, const uint64_t *stitch_off, esp_splice_rec *rec) {
     const size_t n = b->npairs;
     espk::SpliceArgs a{};
     a.npairs = (uint32_t)n;
@@ -485,8 +499,6 @@ int esp_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b,
     a.stitch_off = stitch_off;
     a.rec = rec;
 
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    tim_begin(c);
     HIP_TRY(c, espk::launch_splice_prep(a, c->stream));
     tim_mark(c, 5, 0);
     esp_batch gb{};
@@ -499,12 +511,9 @@ int esp_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b,
     gb.win_len = a.p_len;
     gb.offset = b->hint;
     gb.half = nullptr;
-    rc = gotoh_batch_core(c, &gb, p, ESP_MODE_HINT, o);
+    const int rc = gotoh_batch_core(c, &gb, p, ESP_MODE_HINT, o);
     if (rc) return rc;
     HIP_TRY(c, espk::launch_splice_finish(a, c->stream));
     tim_mark(c, 6, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     return ESP_OK;
 }
-
-}  // extern "C"

@@ -473,6 +473,142 @@ typedef struct esp_junctions esp_junctions;
 int esp_junctions_upload(esp_ctx *ctx, size_t n, const uint32_t *triples, esp_junctions **out);
 int esp_junctions_free(esp_ctx *ctx, esp_junctions *j);
 
+/* JunctionLib::build (gtf.rs:84-114) with strands: n (contig, start, end,
+ * minus) quads in any order, identical ones counted (host pointer).  Besides
+ * contains_any_strand, the library then answers JunctionLib::support
+ * (gtf.rs:142-149, the count of an exact stranded junction), which the
+ * junction refinement below needs.  Freed by esp_junctions_free. */
+int esp_junctions_upload_stranded(esp_ctx *ctx, size_t n, const uint32_t *quads, esp_junctions **out);
+
+/* SpliceParams (splice.rs:21-53; defaults 20, 1000000, 4, 2, 1, 8, 6, 12). */
+typedef struct {
+    uint32_t min_intron;
+    uint32_t max_intron;
+    int32_t gt_ag_bonus;
+    int32_t gc_ag_bonus;
+    int32_t at_ac_bonus;
+    int32_t noncanonical_penalty;
+    int32_t known_bonus;
+    uint32_t refine_radius;
+} esp_splice_params;
+
+/* RefinedJunction (gtf.rs:52-61).  signal: 0 GT-AG, 1 GC-AG, 2 AT-AC,
+ * 3 non-canonical. */
+typedef struct {
+    int32_t found;    /* 1 Some, 0 None */
+    uint32_t start;   /* intron [start, end), plus-strand coordinates */
+    uint32_t end;
+    uint32_t minus;   /* transcript strand */
+    int32_t signal;
+    uint32_t support; /* JunctionLib::support of the stranded junction */
+} esp_refined_junction;
+
+/* One refine_junction_read call (splice.rs:208-317): the read
+ * reads[read_off .. + read_len) in alignment orientation, naive breakpoints
+ * and the read positions of the two anchors.  With use_fallback set, a None
+ * becomes the junction [fallback_start, fallback_end) on the query's strand
+ * with its junction_signal and support (intron_chain.rs:475-486); found is
+ * then 2. */
+typedef struct {
+    uint64_t read_off;
+    uint32_t read_len;
+    uint32_t contig;
+    uint32_t naive_start;
+    uint32_t naive_end;
+    uint32_t q_a;
+    uint32_t q_b;
+    uint32_t minus;
+    uint32_t use_fallback;
+    uint32_t fallback_start;
+    uint32_t fallback_end;
+} esp_junction_query;
+
+/* refine_junction_read per query (device pointers; sp is host).  junctions
+ * must come from esp_junctions_upload_stranded (or be NULL: an empty
+ * library).  A query on a contig out of range gets found = 0. */
+int esp_refine_junction_batch(esp_ctx *ctx, const esp_ref *ref, const esp_junctions *junctions,
+                              const esp_junction_query *queries, size_t n, const uint8_t *reads,
+                              const esp_splice_params *sp, esp_refined_junction *out);
+
+/* align_spliced (splice.rs:625-664) after prepare_spliced (:545-621) per
+ * chain: segment_chain (:364-396) over the chain's anchors (qpos, rpos) at
+ * anchors[2 * anc_off[k] .. 2 * anc_off[k + 1]) in (rpos, qpos) order with
+ * seed length k; read_aware refinement of every adjacent segment pair, the
+ * intron and support guards, the pseudo-reference with 30-base outer flanks,
+ * extend_hint at offset 0 and the stitched genomic CIGAR.  Device pointers;
+ * the chain's read is in alignment orientation. */
+typedef struct {
+    size_t nchain;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *minus;     /* chain strand: 0 Plus, 1 Minus */
+    const uint32_t *anchors;
+    const uint64_t *anc_off;   /* nchain + 1 */
+} esp_spliced_chain_batch;
+typedef struct {
+    int32_t status;        /* 1 aligned, 0 refused (prepare_spliced returned None),
+                              -1 invalid (contig out of range), -2 stitched CIGAR
+                              capacity too small (n_stitched = required) */
+    int32_t refusal;       /* with status 0: 1 fewer than two segments, 2 a
+                              refinement found nothing, 3 a refined intron
+                              outside [min_intron, max_intron], 4 the splice
+                              support guard, 5 an exon piece with end <= start */
+    uint32_t n_segments;
+    uint32_t genomic_start;
+    uint32_t pseudo_len;
+    uint32_t n_stitched;
+} esp_spliced_chain_rec;
+/* Outputs: ext_out as for esp_gotoh_batch in extend_hint mode (pseudo-
+ * reference CIGARs; pair k needs esp_cigar_capacity(read_len[k]) words);
+ * stitched/stitch_off the genomic CIGARs (pair k needs
+ * esp_cigar_capacity(read_len[k]) + 2 * anchors words); junctions and
+ * bounds: chain k's refined junctions at junctions[anc_off[k] + j] and its
+ * exon bounds (pseudo_start, ref_start, ref_end) at bounds[3 * (anc_off[k] + s)]
+ * (either may be NULL). */
+int esp_align_spliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_junctions *junctions,
+                            const esp_spliced_chain_batch *b, uint32_t k, const esp_splice_params *sp,
+                            const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
+                            const uint64_t *stitch_off, esp_spliced_chain_rec *rec,
+                            esp_refined_junction *junctions_out, uint32_t *bounds);
+
+/* Aligner::reextend_with (align.rs:1170-1225) per genomic placement, with
+ * one EditDir for the whole call (the mapper's candidate_dir; callers group
+ * placements by it): a placement whose CIGAR holds a RefSkip is left alone;
+ * otherwise the window [pos - (lead + 30), min(pos + ref_span + tail + 30,
+ * contig length)) around its footprint and soft clips is extended with
+ * extend_hint at hint (pos - wstart) - lead, and the result replaces the
+ * placement when its score is positive and it aligns at least as many read
+ * bases (M + I) as before, but no more than the read has. */
+typedef struct {
+    size_t nplace;
+    const uint8_t *reads;      /* the query in the placement's orientation */
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *contig;
+    const uint32_t *pos;
+    const uint32_t *cigar;
+    const uint64_t *cig_off;
+    const uint32_t *cig_n;
+} esp_reextend_batch;
+typedef struct {
+    int32_t status;        /* 1 replaced, 0 kept, 2 spliced (left alone), 3 empty
+                              window, -1 contig out of range */
+    uint32_t pos;          /* the placement's position after the call */
+    int32_t score;         /* the new extension's score */
+    uint32_t wstart;
+    uint32_t wlen;
+    uint32_t aligned;      /* M + I before */
+    uint32_t new_aligned;  /* M + I of the new extension */
+} esp_reextend_rec;
+/* cigar_out/cig_out_off/cig_out_n: the new extension's CIGAR for every
+ * placement whose DP ran (placement k needs esp_cigar_capacity(read_len[k])
+ * words; cig_out_n = 0 otherwise).  Device pointers. */
+int esp_reextend_batch_run(esp_ctx *ctx, const esp_ref *ref, const esp_reextend_batch *b, const esp_params *p,
+                           int edit_dir, esp_reextend_rec *rec, uint32_t *cigar_out, const uint64_t *cig_out_off,
+                           uint32_t *cig_out_n);
+
 /* The SpliceParams fields rescore_with reads (splice.rs:21-52; defaults 6,
  * 4, 2, 1, 8) and the seed k-mer length of exact_kmer_support (SeedParams
  * k, 15 in the reference's configurations). */

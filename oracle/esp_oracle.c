@@ -1998,3 +1998,374 @@ void espo_path_scores(size_t npairs, const uint8_t *reads, const uint64_t *read_
     free(jobs);
     free(th);
 }
+
+/* ------------------------------------------- junction refinement / re-extension */
+
+/* SpliceParams (splice.rs:21-53). */
+typedef struct {
+    uint32_t min_intron, max_intron;
+    int32_t gt_ag_bonus, gc_ag_bonus, at_ac_bonus, noncanonical_penalty, known_bonus;
+    uint32_t refine_radius;
+} espo_splice_cfg;
+
+/* JunctionLib::build + support (gtf.rs:84-149): junctions sorted by
+ * (contig, start, end, minus) with identical ones merged into counts. */
+typedef struct {
+    uint32_t *q; /* 4 per junction */
+    uint32_t *count;
+    size_t n;
+} espo_jlib;
+
+static int quad_cmp(const void *a, const void *b) {
+    const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+    for (int k = 0; k < 4; k++)
+        if (x[k] != y[k]) return x[k] < y[k] ? -1 : 1;
+    return 0;
+}
+
+espo_jlib *espo_jlib_build(const uint32_t *quads, size_t n) {
+    espo_jlib *L = (espo_jlib *)calloc(1, sizeof(espo_jlib));
+    uint32_t *tmp = (uint32_t *)malloc((n ? n : 1) * 4 * sizeof(uint32_t));
+    for (size_t k = 0; k < n; k++) {
+        memcpy(tmp + 4 * k, quads + 4 * k, 4 * sizeof(uint32_t));
+        tmp[4 * k + 3] = tmp[4 * k + 3] ? 1u : 0u;
+    }
+    qsort(tmp, n, 4 * sizeof(uint32_t), quad_cmp);
+    L->q = (uint32_t *)malloc((n ? n : 1) * 4 * sizeof(uint32_t));
+    L->count = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    for (size_t k = 0; k < n; k++) {
+        if (L->n && quad_cmp(L->q + 4 * (L->n - 1), tmp + 4 * k) == 0) {
+            L->count[L->n - 1]++;
+        } else {
+            memcpy(L->q + 4 * L->n, tmp + 4 * k, 4 * sizeof(uint32_t));
+            L->count[L->n++] = 1;
+        }
+    }
+    free(tmp);
+    return L;
+}
+
+void espo_jlib_free(espo_jlib *L) {
+    if (!L) return;
+    free(L->q);
+    free(L->count);
+    free(L);
+}
+
+uint32_t espo_jlib_support(const espo_jlib *L, uint32_t contig, uint32_t start, uint32_t end, uint32_t minus) {
+    if (!L) return 0;
+    const uint32_t key[4] = {contig, start, end, minus ? 1u : 0u};
+    size_t lo = 0, hi = L->n;
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        int c = quad_cmp(L->q + 4 * mid, key);
+        if (c == 0) return L->count[mid];
+        if (c < 0) lo = mid + 1;
+        else hi = mid;
+    }
+    return 0;
+}
+
+/* splice_signal (splice.rs:75-104): 0 GT-AG, 1 GC-AG, 2 AT-AC, 3 non-canonical. */
+static int signal_of(const uint8_t d[2], const uint8_t a[2], int minus) {
+    uint8_t d0 = upper(d[0]), d1 = upper(d[1]), a0 = upper(a[0]), a1 = upper(a[1]);
+    if (!minus) {
+        if (d0 == 'G' && d1 == 'T' && a0 == 'A' && a1 == 'G') return 0;
+        if (d0 == 'G' && d1 == 'C' && a0 == 'A' && a1 == 'G') return 1;
+        if (d0 == 'A' && d1 == 'T' && a0 == 'A' && a1 == 'C') return 2;
+        return 3;
+    }
+    if (d0 == 'C' && d1 == 'T' && a0 == 'A' && a1 == 'C') return 0;
+    if (d0 == 'C' && d1 == 'T' && a0 == 'G' && a1 == 'C') return 1;
+    if (d0 == 'G' && d1 == 'T' && a0 == 'A' && a1 == 'T') return 2;
+    return 3;
+}
+
+/* SpliceSignal::score (splice.rs:62-71). */
+static int32_t signal_bonus(int sig, const espo_splice_cfg *sp) {
+    switch (sig) {
+    case 0: return sp->gt_ag_bonus;
+    case 1: return sp->gc_ag_bonus;
+    case 2: return sp->at_ac_bonus;
+    default: return -sp->noncanonical_penalty;
+    }
+}
+
+/* dinuc (splice.rs:330-347): donor [p, p+2), acceptor [p-2, p), N-padded
+ * past the contig end. */
+static void dinuc_at(const espo_ref *R, uint32_t contig, uint32_t p, int donor, uint8_t out[2]) {
+    out[0] = out[1] = 'N';
+    if (!donor && p < 2) return;
+    uint32_t a = donor ? p : p - 2, b = donor ? p + 2 : p;
+    if (a >= R->len[contig]) return;
+    if (b > R->len[contig]) b = R->len[contig];
+    for (uint32_t x = a; x < b; x++) out[x - a] = contig_base(R, contig, x);
+}
+
+/* Base::from_ascii (fasta.rs:30-38) as a code, N = 4. */
+static int base_of(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+    }
+}
+
+/* edit_equiv (splice.rs:320-325): equal Bases (N equals N), or A/G, T/C either way. */
+static int edit_equiv(int a, int b) {
+    return a == b || (a == 0 && b == 2) || (a == 2 && b == 0) || (a == 3 && b == 1) || (a == 1 && b == 3);
+}
+
+typedef struct {
+    int32_t found; /* 1: Some(RefinedJunction), 0: None */
+    uint32_t start, end, minus;
+    int32_t signal; /* 0 GT-AG, 1 GC-AG, 2 AT-AC, 3 non-canonical */
+    uint32_t support;
+} espo_refined;
+
+/* refine_junction_read: splice.rs:208-317. */
+int espo_refine_junction_read(const espo_ref *R, const uint8_t *read, size_t read_len_sz, uint32_t contig,
+                              uint32_t naive_start, uint32_t naive_end, uint32_t q_a, uint32_t q_b, int minus,
+                              const espo_jlib *L, const espo_splice_cfg *sp, espo_refined *out) {
+    memset(out, 0, sizeof(*out));
+    if (contig >= R->ncontig) return 0;
+    const uint32_t len = R->len[contig];
+    const uint32_t read_len = (uint32_t)read_len_sz;
+    const int64_t ref_gap = (int64_t)naive_end - (int64_t)naive_start;
+    const int64_t read_gap = (int64_t)q_b - (int64_t)q_a;
+    if (read_gap > 20) return 0;
+    const int64_t intron_len = ref_gap - read_gap;
+    if (intron_len < (int64_t)sp->min_intron || intron_len > (int64_t)sp->max_intron) return 0;
+    const int64_t lo = (read_gap < 0 ? read_gap : 0) - 2, hi = (read_gap > 0 ? read_gap : 0) + 2;
+    int have = 0;
+    int32_t best = 0;
+    for (int64_t delta = lo; delta <= hi; delta++) {
+        const int64_t s = (int64_t)naive_start + delta;
+        if (s < 0) continue;
+        const int64_t e = s + intron_len;
+        if (e < 2 || (uint64_t)e > (uint64_t)len) continue;
+        const int64_t qb = (int64_t)q_a + delta;
+        if (qb < 1 || (uint32_t)qb >= read_len) continue;
+        int32_t mt = 0;
+        for (int64_t i = 1; i <= 4; i++) {
+            const int64_t q1 = qb - i, g1 = s - i;
+            /* (u64)q < read_len: the reference would panic past the read */
+            if (q1 >= 0 && g1 >= 0 && (uint64_t)q1 < read_len &&
+                edit_equiv(base_of(read[q1]), base_of(contig_base(R, contig, (uint32_t)g1))))
+                mt++;
+            const int64_t q2 = qb + i - 1, g2 = e + i - 1;
+            if ((uint32_t)q2 < read_len && (uint64_t)q2 < read_len && (uint64_t)g2 < (uint64_t)len &&
+                edit_equiv(base_of(read[q2]), base_of(contig_base(R, contig, (uint32_t)g2))))
+                mt++;
+        }
+        uint8_t dn[2], ac[2];
+        dinuc_at(R, contig, (uint32_t)s, 1, dn);
+        dinuc_at(R, contig, (uint32_t)e, 0, ac);
+        for (int t = 0; t < 2; t++) {
+            const int tm = t == 0 ? minus : !minus;
+            const int sig = signal_of(dn, ac, tm);
+            const uint32_t support = espo_jlib_support(L, contig, (uint32_t)s, (uint32_t)e, (uint32_t)tm);
+            const int64_t ad = delta < 0 ? -delta : delta;
+            int32_t score = mt * 50 + signal_bonus(sig, sp) * 5 - (int32_t)ad * 5;
+            if (support > 0) score += sp->known_bonus;
+            if (!have || score > best) {
+                have = 1;
+                best = score;
+                out->found = 1;
+                out->start = (uint32_t)s;
+                out->end = (uint32_t)e;
+                out->minus = (uint32_t)tm;
+                out->signal = sig;
+                out->support = support;
+            }
+        }
+    }
+    return have;
+}
+
+/* splice_support_pass: splice.rs:107-115 (MIN_INTRON_WITHOUT_SUPPORT = 50). */
+static int support_pass(uint32_t intron_len, int sig, uint32_t support) {
+    if (intron_len < 50 && support == 0) return 0;
+    if (sig == 3 && support == 0) return 0;
+    return 1;
+}
+
+/* segment_chain: splice.rs:364-396.  anchors as (qpos, rpos) pairs in the
+ * chain's (rpos, qpos) order; segs receives 4 u32 (q_start, q_end, r_start,
+ * r_end) per segment (capacity n_anchors).  Returns the segment count. */
+size_t espo_segment_chain(const uint32_t *anchors, size_t n, uint32_t k, uint32_t min_intron, uint32_t *segs) {
+    if (n == 0) return 0;
+    size_t ns = 0;
+    uint32_t cur[4] = {anchors[0], anchors[0] + k, anchors[1], anchors[1] + k};
+    for (size_t a = 1; a < n; a++) {
+        const uint32_t q1 = anchors[2 * (a - 1)], r1 = anchors[2 * (a - 1) + 1];
+        const uint32_t q2 = anchors[2 * a], r2 = anchors[2 * a + 1];
+        const uint32_t ref_gap = r2 > r1 + k ? r2 - (r1 + k) : 0u;
+        const uint32_t read_gap = q2 > q1 + k ? q2 - (q1 + k) : 0u;
+        if (ref_gap > min_intron && read_gap <= 10) {
+            memcpy(segs + 4 * ns++, cur, sizeof(cur));
+            cur[0] = q2; cur[1] = q2 + k; cur[2] = r2; cur[3] = r2 + k;
+        } else {
+            cur[1] = q2 + k;
+            cur[3] = r2 + k;
+        }
+    }
+    memcpy(segs + 4 * ns++, cur, sizeof(cur));
+    return ns;
+}
+
+/* Refusal codes of prepare_spliced (splice.rs:545-621), each a `None`:
+ * 1 fewer than two segments, 2 a refinement found nothing, 3 a refined intron
+ * outside [min_intron, max_intron], 4 the support guard, 5 an exon piece with
+ * re <= rs; 6 a contig out of range (`contigs.get(..)?`). */
+typedef struct {
+    int32_t status;   /* 1 aligned, 0 refused (refusal says why), -2 stitched capacity too small */
+    int32_t refusal;
+    uint32_t n_segments, genomic_start, pseudo_len, n_stitched;
+} espo_align_splice_rec;
+
+/* align_spliced: splice.rs:625-664 after prepare_spliced (:545-621).
+ * junctions receives n_segments - 1 records, bounds 3 u32 per segment. */
+int espo_align_spliced(const espo_ref *R, const espo_jlib *L, const uint8_t *read, size_t m, uint32_t contig,
+                       int minus, const uint32_t *anchors, size_t n_anchors, uint32_t k,
+                       const espo_splice_cfg *sp, const espo_params *p, espo_ext *ext, uint32_t *cigar,
+                       size_t cap, size_t *n_ops, uint32_t *stitched, size_t scap,
+                       espo_align_splice_rec *rec, espo_refined *junctions, uint32_t *bounds) {
+    memset(rec, 0, sizeof(*rec));
+    memset(ext, 0, sizeof(*ext));
+    *n_ops = 0;
+    uint32_t *segs = (uint32_t *)malloc((n_anchors ? n_anchors : 1) * 4 * sizeof(uint32_t));
+    const size_t ns = espo_segment_chain(anchors, n_anchors, k, sp->min_intron, segs);
+    rec->n_segments = (uint32_t)ns;
+    int refusal = 0;
+    if (ns < 2) refusal = 1;
+    else if (contig >= R->ncontig) refusal = 6;
+    for (size_t w = 0; !refusal && w + 1 < ns; w++) {
+        espo_refined *j = &junctions[w];
+        if (!espo_refine_junction_read(R, read, m, contig, segs[4 * w + 3], segs[4 * (w + 1) + 2], segs[4 * w + 1],
+                                       segs[4 * (w + 1)], minus, L, sp, j)) {
+            refusal = 2;
+            break;
+        }
+        const uint32_t il = j->end - j->start;
+        if (il < sp->min_intron || il > sp->max_intron) refusal = 3;
+        else if (!support_pass(il, j->signal, j->support)) refusal = 4;
+    }
+    uint64_t total = 0;
+    for (size_t i = 0; !refusal && i < ns; i++) {
+        const uint32_t rs = i == 0 ? (segs[2] > 30 ? segs[2] - 30 : 0u) : junctions[i - 1].end;
+        uint32_t re;
+        if (i + 1 == ns) {
+            const uint64_t x = (uint64_t)segs[4 * i + 3] + 30u;
+            re = (uint32_t)(x < R->len[contig] ? x : R->len[contig]);
+        } else {
+            re = junctions[i].start;
+        }
+        if (re <= rs) {
+            refusal = 5;
+            break;
+        }
+        bounds[3 * i] = (uint32_t)total;
+        bounds[3 * i + 1] = rs;
+        bounds[3 * i + 2] = re;
+        total += re - rs;
+    }
+    free(segs);
+    if (refusal) {
+        rec->refusal = refusal;
+        return 0;
+    }
+    uint8_t *pseudo = (uint8_t *)malloc(total ? total : 1);
+    uint32_t *skips = (uint32_t *)malloc(ns * sizeof(uint32_t));
+    for (size_t i = 0; i < ns; i++) {
+        espo_decode(R, contig, bounds[3 * i + 1], bounds[3 * i + 2], 0, pseudo + bounds[3 * i]);
+        if (i + 1 < ns) skips[i] = junctions[i].end - junctions[i].start;
+    }
+    int32_t esc;
+    int rc = espo_extend_hint(read, m, pseudo, total, p, 0, ext, cigar, cap, n_ops, &esc);
+    size_t nst = 0;
+    int src = espo_stitch_cigar(cigar, *n_ops, ext->ref_start, bounds, ns, skips, ns - 1, stitched, scap, &nst);
+    rec->status = src ? -2 : 1;
+    rec->n_stitched = (uint32_t)nst;
+    rec->pseudo_len = (uint32_t)total;
+    rec->genomic_start = genomic_start_of(ext->ref_start, bounds, ns);
+    free(pseudo);
+    free(skips);
+    return rc;
+}
+
+typedef struct {
+    int32_t status; /* 1 replaced, 0 kept, 2 spliced (left alone), 3 empty window, -1 contig out of range */
+    uint32_t pos;   /* the placement's position after the call */
+    int32_t score;  /* the new extension's score (0 when none ran) */
+    uint32_t wstart, wlen, aligned, new_aligned;
+} espo_reext_rec;
+
+/* Aligner::reextend_with: align.rs:1170-1225.  dir replaces p->edit_dir.
+ * cigar_out holds the new extension's CIGAR whenever the DP ran. */
+int espo_reextend_with(const espo_ref *R, const uint8_t *query, size_t m, uint32_t contig, uint32_t pos,
+                       const uint32_t *cigar, size_t n_ops, const espo_params *p, int dir, espo_reext_rec *rec,
+                       uint32_t *cigar_out, size_t cap, size_t *n_out) {
+    memset(rec, 0, sizeof(*rec));
+    rec->pos = pos;
+    *n_out = 0;
+    for (size_t k = 0; k < n_ops; k++)
+        if (op_kind(cigar[k]) == OP_N) {
+            rec->status = 2;
+            return 0;
+        }
+    if (contig >= R->ncontig) {
+        rec->status = -1;
+        return 0;
+    }
+    const uint32_t read_len = (uint32_t)m;
+    uint32_t lead = 0, tail = 0, ref_span = 0, aligned = 0;
+    int seen_m = 0;
+    for (size_t k = 0; k < n_ops; k++) {
+        const uint32_t kind = op_kind(cigar[k]), n = op_len(cigar[k]);
+        if (kind == OP_S) {
+            if (seen_m) tail += n;
+            else lead += n;
+        } else if (kind == OP_M) {
+            seen_m = 1;
+            ref_span += n;
+            aligned += n;
+        } else if (kind == OP_I) {
+            aligned += n;
+        } else if (kind == OP_D) {
+            ref_span += n;
+        }
+    }
+    rec->aligned = aligned;
+    const uint32_t back = lead + 30u;
+    const uint32_t wstart = pos > back ? pos - back : 0u;
+    uint32_t wend = pos + ref_span + tail + 30u; /* u32 arithmetic, as the reference's */
+    if (wend > R->len[contig]) wend = R->len[contig];
+    if (wend <= wstart) {
+        rec->status = 3;
+        return 0;
+    }
+    rec->wstart = wstart;
+    rec->wlen = wend - wstart;
+    uint8_t *win = (uint8_t *)malloc(wend - wstart);
+    espo_decode(R, contig, wstart, wend, 0, win);
+    espo_params ep = *p;
+    ep.edit_dir = dir;
+    espo_ext e;
+    int32_t esc;
+    const int64_t hint = (int64_t)(pos - wstart) - (int64_t)lead;
+    int rc = espo_extend_hint(query, m, win, wend - wstart, &ep, hint, &e, cigar_out, cap, n_out, &esc);
+    free(win);
+    uint32_t new_aligned = 0;
+    for (size_t k = 0; k < *n_out; k++)
+        if (op_kind(cigar_out[k]) == OP_M || op_kind(cigar_out[k]) == OP_I) new_aligned += op_len(cigar_out[k]);
+    rec->score = e.score;
+    rec->new_aligned = new_aligned;
+    if (e.score > 0 && new_aligned >= aligned && new_aligned <= read_len) {
+        rec->status = 1;
+        rec->pos = wstart + e.ref_start;
+    }
+    return rc;
+}

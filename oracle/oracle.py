@@ -619,3 +619,155 @@ def path_scores(batch, ext, cig, cig_off, cig_n, params: Optional[ExtendParams] 
                            e.ctypes.data, cig.ctypes.data, cig_off.ctypes.data, cig_n.ctypes.data,
                            ctypes.byref(_params(params)), score.ctypes.data, bad.ctypes.data, nthreads)
     return score[:n], bad[:n]
+
+
+# ---- junction refinement, align_spliced, reextend_with (splice.rs, align.rs) ----
+
+class CSpliceCfg(ctypes.Structure):
+    """SpliceParams (splice.rs:21-53)."""
+    _fields_ = [("min_intron", ctypes.c_uint32), ("max_intron", ctypes.c_uint32),
+                ("gt_ag_bonus", ctypes.c_int32), ("gc_ag_bonus", ctypes.c_int32), ("at_ac_bonus", ctypes.c_int32),
+                ("noncanonical_penalty", ctypes.c_int32), ("known_bonus", ctypes.c_int32),
+                ("refine_radius", ctypes.c_uint32)]
+
+
+SPLICE_CFG_DEFAULTS = (20, 1_000_000, 4, 2, 1, 8, 6, 12)  # SpliceParams::default, splice.rs:40-53
+
+
+class CRefined(ctypes.Structure):
+    _fields_ = [("found", ctypes.c_int32), ("start", ctypes.c_uint32), ("end", ctypes.c_uint32),
+                ("minus", ctypes.c_uint32), ("signal", ctypes.c_int32), ("support", ctypes.c_uint32)]
+
+
+class CAlignSpliceRec(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("refusal", ctypes.c_int32), ("n_segments", ctypes.c_uint32),
+                ("genomic_start", ctypes.c_uint32), ("pseudo_len", ctypes.c_uint32),
+                ("n_stitched", ctypes.c_uint32)]
+
+
+class CReextRec(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("pos", ctypes.c_uint32), ("score", ctypes.c_int32),
+                ("wstart", ctypes.c_uint32), ("wlen", ctypes.c_uint32), ("aligned", ctypes.c_uint32),
+                ("new_aligned", ctypes.c_uint32)]
+
+
+def _splice_lib():
+    L = lib()
+    if getattr(L, "_splice_bound", False):
+        return L
+    P, vp, sz = ctypes.POINTER, ctypes.c_void_p, ctypes.c_size_t
+    u32p = P(ctypes.c_uint32)
+    L.espo_jlib_build.argtypes = [u32p, sz]
+    L.espo_jlib_build.restype = vp
+    L.espo_jlib_free.argtypes = [vp]
+    L.espo_jlib_free.restype = None
+    L.espo_jlib_support.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    L.espo_jlib_support.restype = ctypes.c_uint32
+    L.espo_refine_junction_read.argtypes = [P(CRef), P(ctypes.c_uint8), sz, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp,
+                                            P(CSpliceCfg), P(CRefined)]
+    L.espo_segment_chain.argtypes = [u32p, sz, ctypes.c_uint32, ctypes.c_uint32, u32p]
+    L.espo_segment_chain.restype = sz
+    L.espo_align_spliced.argtypes = [P(CRef), vp, P(ctypes.c_uint8), sz, ctypes.c_uint32, ctypes.c_int, u32p, sz,
+                                     ctypes.c_uint32, P(CSpliceCfg), P(CParams), P(CExt), u32p, sz, P(sz), u32p,
+                                     sz, P(CAlignSpliceRec), P(CRefined), u32p]
+    L.espo_reextend_with.argtypes = [P(CRef), P(ctypes.c_uint8), sz, ctypes.c_uint32, ctypes.c_uint32, u32p, sz,
+                                     P(CParams), ctypes.c_int, P(CReextRec), u32p, sz, P(sz)]
+    L._splice_bound = True
+    return L
+
+
+def splice_cfg(min_intron=20, max_intron=1_000_000, gt_ag=4, gc_ag=2, at_ac=1, noncanonical=8, known=6,
+               refine_radius=12) -> CSpliceCfg:
+    return CSpliceCfg(min_intron, max_intron, gt_ag, gc_ag, at_ac, noncanonical, known, refine_radius)
+
+
+class JLib:
+    """JunctionLib::build (gtf.rs:84-114) of (contig, start, end, minus) quads."""
+
+    def __init__(self, quads: Sequence[Tuple[int, int, int, int]]):
+        L = _splice_lib()
+        q = np.ascontiguousarray(np.array(quads, dtype=np.uint32).reshape(-1, 4) if len(quads) else
+                                 np.zeros((0, 4), np.uint32))
+        self.quads = q
+        self.h = L.espo_jlib_build(q.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), q.shape[0])
+
+    def support(self, contig, start, end, minus) -> int:
+        return int(_splice_lib().espo_jlib_support(self.h, contig, start, end, int(minus)))
+
+    def close(self):
+        if self.h:
+            _splice_lib().espo_jlib_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+SIGNALS = ("GtAg", "GcAg", "AtAc", "NonCanonical")
+
+
+def refine_junction_read(oref: OracleRef, read: bytes, contig: int, naive_start: int, naive_end: int,
+                         q_a: int, q_b: int, minus: bool, jlib: Optional[JLib], cfg: Optional[CSpliceCfg] = None):
+    """splice.rs:208 refine_junction_read -> (start, end, minus, signal, support) or None."""
+    L = _splice_lib()
+    r, m = _buf(read)
+    o = CRefined()
+    found = L.espo_refine_junction_read(ctypes.byref(oref.c), r, m, contig, naive_start, naive_end, q_a, q_b,
+                                        int(minus), jlib.h if jlib else None, ctypes.byref(cfg or splice_cfg()),
+                                        ctypes.byref(o))
+    return (o.start, o.end, bool(o.minus), SIGNALS[o.signal], o.support) if found else None
+
+
+def segment_chain(anchors: Sequence[Tuple[int, int]], k: int, min_intron: int = 20):
+    """splice.rs:364 segment_chain over (qpos, rpos) anchors -> [(q_start, q_end, r_start, r_end)]."""
+    L = _splice_lib()
+    a = np.ascontiguousarray(np.array(anchors, dtype=np.uint32).reshape(-1))
+    out = np.zeros(max(len(anchors), 1) * 4, np.uint32)
+    ns = L.espo_segment_chain(a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(anchors), k, min_intron,
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return [tuple(int(x) for x in out[4 * s:4 * s + 4]) for s in range(ns)]
+
+
+def align_spliced(oref: OracleRef, jlib: Optional[JLib], read: bytes, contig: int, minus: bool,
+                  anchors: Sequence[Tuple[int, int]], k: int, cfg: Optional[CSpliceCfg] = None,
+                  params: Optional[ExtendParams] = None) -> dict:
+    """splice.rs:625 align_spliced (after prepare_spliced, :545)."""
+    L = _splice_lib()
+    r, m = _buf(read)
+    na = len(anchors)
+    a = np.ascontiguousarray(np.array(anchors, dtype=np.uint32).reshape(-1)) if na else np.zeros(2, np.uint32)
+    cap = cigar_capacity(m)
+    cig = (ctypes.c_uint32 * cap)()
+    scap = cap + 2 * max(na, 1)
+    st = (ctypes.c_uint32 * scap)()
+    js = (CRefined * max(na, 1))()
+    bounds = (ctypes.c_uint32 * (3 * max(na, 1)))()
+    e, rec, nops = CExt(), CAlignSpliceRec(), ctypes.c_size_t()
+    u32 = ctypes.POINTER(ctypes.c_uint32)
+    L.espo_align_spliced(ctypes.byref(oref.c), jlib.h if jlib else None, r, m, contig, int(minus), a.ctypes.data_as(u32),
+                         na, k, ctypes.byref(cfg or splice_cfg()), ctypes.byref(_params(params)), ctypes.byref(e),
+                         cig, cap, ctypes.byref(nops), st, scap, ctypes.byref(rec), js, bounds)
+    ns = rec.n_segments if rec.status != 0 else 0
+    return {"status": rec.status, "refusal": rec.refusal, "n_segments": rec.n_segments,
+            "genomic_start": rec.genomic_start, "pseudo_len": rec.pseudo_len,
+            "ext": Extension.from_c(e, cig[: nops.value]) if rec.status != 0 else None,
+            "stitched": cigar_from_words(st[: rec.n_stitched]) if rec.status == 1 else [],
+            "junctions": [(js[j].start, js[j].end, bool(js[j].minus), SIGNALS[js[j].signal], js[j].support)
+                          for j in range(max(ns - 1, 0))],
+            "bounds": [tuple(bounds[3 * s:3 * s + 3]) for s in range(ns)]}
+
+
+def reextend_with(oref: OracleRef, query: bytes, contig: int, pos: int, cigar_words: Sequence[int],
+                  params: Optional[ExtendParams], edit_dir: int) -> dict:
+    """align.rs:1170 Aligner::reextend_with."""
+    L = _splice_lib()
+    q, m = _buf(query)
+    cw = (ctypes.c_uint32 * max(len(cigar_words), 1))(*cigar_words)
+    cap = cigar_capacity(m)
+    out = (ctypes.c_uint32 * cap)()
+    rec, n = CReextRec(), ctypes.c_size_t()
+    L.espo_reextend_with(ctypes.byref(oref.c), q, m, contig, pos, cw, len(cigar_words), ctypes.byref(_params(params)),
+                         int(edit_dir), ctypes.byref(rec), out, cap, ctypes.byref(n))
+    return {"status": rec.status, "pos": rec.pos, "score": rec.score, "wstart": rec.wstart, "wlen": rec.wlen,
+            "aligned": rec.aligned, "new_aligned": rec.new_aligned, "cigar": list(out[: n.value])}

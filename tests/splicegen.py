@@ -1,0 +1,109 @@
+"""Planted splice junctions for the junction-refinement, align_spliced and
+reextend_with tests: one contig with a spliced read per locus, chains of
+exact k-mer anchors on both exons, and the cases that make prepare_spliced
+refuse (one segment, short or non-canonical introns without library support,
+reads whose anchors disagree with the motif)."""
+from __future__ import annotations
+
+import numpy as np
+
+from esperanto_amd.reference import Reference
+
+K = 15
+MOTIF = {False: (b"GT", b"AG"), True: (b"CT", b"AC")}  # plus / minus transcript strand
+
+
+def _mutate(rng, s: bytearray, rate: float):
+    for i in range(len(s)):
+        if rng.random() < rate:
+            s[i] = b"ACGT"[(b"ACGT".index(s[i]) + int(rng.integers(1, 4))) % 4] if s[i] in b"ACGT" else s[i]
+
+
+def _anchors(rng, read_len: int, u: int, left_r: int, right_r: int, stride: int):
+    """Exact k-mers at qpos multiples of stride on each exon, in (rpos, qpos) order."""
+    out = []
+    for q in range(0, max(u - K, -1) + 1, stride):
+        out.append((q, left_r + q))
+    for q in range(u, read_len - K + 1, stride):
+        out.append((q, right_r + (q - u)))
+    return sorted(out, key=lambda a: (a[1], a[0]))
+
+
+def spliced_workload(seed: int, nreads: int = 200, read_len: int = 150):
+    """Returns (reference, chains, library quads, truth) where each chain is
+    (read, contig, minus, anchors) and truth[c] the planted (start, end, kind)."""
+    rng = np.random.default_rng(seed)
+    span = 14000
+    genome = bytearray(np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, span * nreads + 2000)].tobytes())
+    chains, truth, quads = [], [], []
+    for c in range(nreads):
+        base = 1000 + c * span
+        kind = ["canon", "canon", "canon", "canon", "short", "noncanon", "onesegment", "three"][c % 8]
+        minus = bool(rng.integers(0, 2))
+        L = int(np.exp(rng.uniform(np.log(60), np.log(5000)))) if kind not in ("short",) else int(rng.integers(25, 50))
+        s = base + 400
+        e = s + L
+        d, a = MOTIF[minus]
+        if kind != "noncanon":
+            genome[s:s + 2] = d
+            genome[e - 2:e] = a
+        else:
+            genome[s:s + 2] = b"CA"
+            genome[e - 2:e] = b"TT"
+        u = int(rng.integers(30, read_len - (70 if kind == "three" else 30)))
+        if kind == "three":
+            # a second intron inside the right exon
+            u2 = int(rng.integers(u + 30, read_len - 25))
+            L2 = int(rng.integers(80, 800))
+            s2 = e + (u2 - u)
+            e2 = s2 + L2
+            genome[s2:s2 + 2] = d
+            genome[e2 - 2:e2] = a
+            read = bytearray(genome[s - u:s] + genome[e:s2] + genome[e2:e2 + (read_len - u2)])
+            anchors = _anchors(rng, u2, u, s - u, e, 7) + [(q, e2 + (q - u2)) for q in range(u2, read_len - K + 1, 7)]
+            anchors = sorted(anchors, key=lambda x: (x[1], x[0]))
+        elif kind == "onesegment":
+            read = bytearray(genome[s - u:s - u + read_len])
+            anchors = [(q, s - u + q) for q in range(0, read_len - K + 1, 9)]
+        else:
+            read = bytearray(genome[s - u:s] + genome[e:e + (read_len - u)])
+            anchors = _anchors(rng, read_len, u, s - u, e, 7 if c % 3 else 11)
+        _mutate(rng, read, 0.01)
+        # keep only anchors still exact after mutation (as a seeder would)
+        rb = bytes(read)
+        anchors = [(q, r) for q, r in anchors if rb[q:q + K] == bytes(genome[r:r + K])]
+        if len(read) and rng.random() < 0.15:
+            read = bytearray(rb.lower())
+        chains.append((bytes(read), 0, minus, anchors))
+        truth.append((s, e, kind))
+        if kind in ("canon", "short", "noncanon", "three") and rng.random() < 0.5:
+            quads.append((0, s, e, int(minus)))
+            if rng.random() < 0.3:
+                quads.append((0, s, e, int(minus)))  # a duplicate counts twice
+        if rng.random() < 0.2:
+            quads.append((0, s + 3, e + 3, int(not minus)))  # a decoy nearby
+    ref = Reference.from_sequences([bytes(genome)])
+    return ref, chains, quads, truth
+
+
+def refine_queries(seed: int, chains, truth, n: int = 600):
+    """refine_junction_read queries around the planted junctions: naive
+    breakpoints shifted by small amounts, read gaps of -3..25, both strands,
+    some on a contig out of range, some with a fallback junction."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        c = int(rng.integers(0, len(chains)))
+        read, contig, minus, _ = chains[c]
+        s, e, _ = truth[c]
+        shift = int(rng.integers(-4, 5))
+        gap = int(rng.integers(-3, 26))
+        qa = int(rng.integers(1, max(len(read) - 1, 2)))
+        qb = qa + gap
+        ns = s + shift
+        ne = e + shift + gap
+        q = (c, 5 if i % 97 == 0 else contig, max(ns, 0), max(ne, 0), qa, max(qb, 0), bool(rng.integers(0, 2)))
+        if i % 5 == 0:
+            q = q + ((s + int(rng.integers(-3, 4)), e),)
+        out.append(q)
+    return out
