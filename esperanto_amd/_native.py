@@ -29,7 +29,7 @@ EXPORTS = [
     "esp_server_wait", "esp_server_stats_get", "esp_now_ns", "esp_ctx_set_option",
     "esp_unspliced_batch", "esp_window_myers_batch", "esp_window_myers_batch_noalloc",
     "esp_junctions_upload_stranded", "esp_refine_junction_batch", "esp_align_spliced_batch",
-    "esp_reextend_batch_run",
+    "esp_reextend_batch_run", "esp_intron_chain_batch_run",
     "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
     "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
@@ -148,6 +148,17 @@ class CReextendBatch(ctypes.Structure):
                 ("cigar", ctypes.c_void_p), ("cig_off", ctypes.c_void_p), ("cig_n", ctypes.c_void_p)]
 
 
+class CIntronChainBatch(ctypes.Structure):
+    _fields_ = [("nread", ctypes.c_size_t), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("anchors", ctypes.c_void_p), ("anc_off", ctypes.c_void_p),
+                ("max_read_len", ctypes.c_uint32)]
+
+
+class CIntronParams(ctypes.Structure):
+    _fields_ = [("min_frag_cov", ctypes.c_uint32), ("refine_window", ctypes.c_uint32),
+                ("refine_pattern_len", ctypes.c_uint32), ("intron_max", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -203,6 +214,8 @@ def lib():
     L.esp_refine_junction_batch.argtypes = [vp, vp, vp, vp, sz, vp, P(CSpliceCfg), vp]
     L.esp_align_spliced_batch.argtypes = [vp, vp, vp, P(CSplicedChainBatch), ctypes.c_uint32, P(CSpliceCfg),
                                           P(CParams), P(CBatchOut), vp, vp, vp, vp, vp]
+    L.esp_intron_chain_batch_run.argtypes = [vp, vp, vp, P(CIntronChainBatch), ctypes.c_uint32, P(CIntronParams),
+                                             P(CParams), P(CBatchOut), vp, vp, vp]
     L.esp_reextend_batch_run.argtypes = [vp, vp, P(CReextendBatch), P(CParams), ctypes.c_int, vp, vp, vp, vp]
     L.esp_junctions_free.argtypes = [vp, vp]
     L.esp_walk_placements.argtypes = [vp, vp, vp, P(CWalkBatch), P(CParams), P(CWalkParams), vp, vp,

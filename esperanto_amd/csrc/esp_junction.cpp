@@ -261,4 +261,157 @@ int esp_reextend_batch_run(esp_ctx *c, const esp_ref *r, const esp_reextend_batc
     return ESP_OK;
 }
 
+int esp_intron_chain_batch_run(esp_ctx *c, const esp_ref *r, const esp_junctions *j,
+                               const esp_intron_chain_batch *b, uint32_t k, const esp_intron_params *ip,
+                               const esp_params *p, esp_batch_out *o, uint32_t *stitched,
+                               const uint64_t *stitch_off, esp_intron_chain_rec *rec) {
+    if (!c) return ESP_E_INVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!r || !b || !ip || !o || !o->ext || !o->cigar || !o->cig_off || !o->cig_n || !stitched || !stitch_off ||
+        !rec)
+        return fail(c, ESP_E_INVAL, "NULL argument");
+    if (r->device != c->device) return fail(c, ESP_E_INVAL, "reference lives on another device");
+    espk::IcArgs a{};
+    rc = junction_view(c, j, a.J);
+    if (rc) return rc;
+    const size_t n = b->nread;
+    if (n == 0) return ESP_OK;
+    if (n > 0x7FFFFF00ull) return fail(c, ESP_E_INVAL, "too many reads");
+    if (!b->reads || !b->read_off || !b->read_len || !b->anchors || !b->anc_off)
+        return fail(c, ESP_E_INVAL, "batch arrays are required");
+    if (b->max_read_len > kMaxRead) return fail(c, ESP_E_INVAL, "max_read_len above 16000");
+    HIP_TRY(c, hipSetDevice(c->device));
+    uint64_t nanc = 0, unused = 0;
+    rc = read_totals(c, b->anc_off + n, nullptr, nanc, unused);
+    if (rc) return rc;
+    const size_t slots = std::max<uint64_t>(nanc, 1), nj = 2 * n;
+    HIP_TRY(c, c->jscratch.ensure(slots * 56 + n * (sizeof(espk::IcState) + 96) + nj * 64 + 32 * 256));
+    Carve cv{(uint8_t *)c->jscratch.p};
+    a.mems = cv.take<uint32_t>(6 * slots);
+    a.frags = cv.take<uint32_t>(8 * slots);
+    a.st = cv.take<espk::IcState>(n);
+    a.p_off = cv.take<uint64_t>(nj);
+    a.p_len = cv.take<uint32_t>(nj);
+    a.t_ctg = cv.take<uint32_t>(nj);
+    a.t_lo = cv.take<uint32_t>(nj);
+    a.t_len = cv.take<uint32_t>(nj);
+    uint64_t *t_off = cv.take<uint64_t>(nj + 1);
+    int32_t *dist = cv.take<int32_t>(nj);
+    uint64_t *endp = cv.take<uint64_t>(nj);
+    uint64_t *startp = cv.take<uint64_t>(nj);
+    a.exons = cv.take<uint32_t>(4 * n);
+    a.nex = cv.take<uint32_t>(n);
+    a.ctg_out = cv.take<uint32_t>(n);
+    a.exon_off = cv.take<uint64_t>(n + 1);
+    a.plen = cv.take<uint64_t>(n);
+    uint64_t *pseudo_off = cv.take<uint64_t>(n + 1);
+    a.hint = cv.take<int64_t>(n);
+    esp_splice_rec *srec = cv.take<esp_splice_rec>(n);
+    a.R = r->view();
+    a.sp.min_intron = 20;  // SpliceParams::default with max_intron = intron_max (intron_chain.rs:462-473)
+    a.sp.max_intron = ip->intron_max;
+    a.sp.gt_ag = 4;
+    a.sp.gc_ag = 2;
+    a.sp.at_ac = 1;
+    a.sp.noncanon = 8;
+    a.sp.known = 6;
+    a.n = (uint32_t)n;
+    a.k = k;
+    a.max_read = b->max_read_len;
+    a.min_frag_cov = ip->min_frag_cov;
+    a.refine_window = ip->refine_window;
+    a.refine_pattern_len = ip->refine_pattern_len;
+    a.intron_max = ip->intron_max;
+    a.reads = b->reads;
+    a.read_off = b->read_off;
+    a.read_len = b->read_len;
+    a.anchors = b->anchors;
+    a.anc_off = b->anc_off;
+    a.t_off = t_off;
+    a.dist = dist;
+    a.end = endp;
+    a.start = startp;
+    a.srec = srec;
+    a.ext = o->ext;
+    a.stitched = stitched;
+    a.stitch_off = stitch_off;
+    a.rec = rec;
+    HIP_TRY(c, c->counter.ensure(256));
+    uint32_t *counter = (uint32_t *)c->counter.p;
+
+    // Myers over jobs [first, first + count) of the current job arrays.
+    auto myers = [&](size_t first, size_t count, int maxm, bool want_end, bool want_start) -> int {
+        espk::MyersArgs m{};
+        m.npairs = (uint32_t)count;
+        m.reads = b->reads;
+        m.read_off = a.p_off + first;
+        m.read_len = a.p_len + first;
+        m.texts = a.arena;
+        m.text_off = t_off + first;
+        m.text_len = a.t_len + first;
+        m.dist = dist + first;
+        m.end = want_end ? endp + first : nullptr;
+        m.start = want_start ? startp + first : nullptr;
+        m.counter = counter;
+        const int mp = std::min(std::max(maxm, 1), 256);
+        uint64_t blocks = std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(mp, false), (count + 63) / 64);
+        HIP_TRY(c, espk::launch_zero_u32(counter, 1, c->stream));
+        HIP_TRY(c, espk::launch_myers(m, mp, (int)std::max<uint64_t>(blocks, 1), c->stream));
+        return ESP_OK;
+    };
+    // Texts of the current jobs: offsets, one read-back of the total, decode.
+    auto gather = [&]() -> int {
+        HIP_TRY(c, espk::launch_scan2(a.t_len, nullptr, nj, t_off, nullptr, c->stream));
+        uint64_t tbytes = 0, u = 0;
+        int r2 = read_totals(c, t_off + nj, nullptr, tbytes, u);
+        if (r2) return r2;
+        HIP_TRY(c, c->jarena.ensure(tbytes + 64));
+        a.arena = (uint8_t *)c->jarena.p;
+        HIP_TRY(c, espk::launch_ic(a, 1, c->stream));
+        return ESP_OK;
+    };
+
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    tim_begin(c);
+    HIP_TRY(c, espk::launch_ic(a, 0, c->stream));
+    rc = gather();
+    if (rc) return rc;
+    tim_mark(c, 5, 0);
+    rc = myers(0, n, (int)ip->refine_pattern_len, true, false);
+    if (rc) return rc;
+    rc = myers(n, n, (int)ip->refine_pattern_len, false, true);
+    if (rc) return rc;
+    tim_mark(c, 3, 0);
+    HIP_TRY(c, espk::launch_ic(a, 2, c->stream));
+    rc = gather();
+    if (rc) return rc;
+    rc = myers(0, nj, (int)(b->max_read_len - b->max_read_len / 2), false, false);
+    if (rc) return rc;
+    tim_mark(c, 3, 0);
+    HIP_TRY(c, espk::launch_ic(a, 3, c->stream));
+    HIP_TRY(c, espk::launch_scan2(nullptr, a.plen, n, nullptr, pseudo_off, c->stream));
+    uint64_t pbytes = 0, u2 = 0;
+    rc = read_totals(c, pseudo_off + n, nullptr, pbytes, u2);
+    if (rc) return rc;
+    tim_mark(c, 6, 0);
+    esp_splice_batch sb{};
+    sb.npairs = n;
+    sb.reads = b->reads;
+    sb.read_off = b->read_off;
+    sb.read_len = b->read_len;
+    sb.contig = a.ctg_out;
+    sb.hint = a.hint;
+    sb.exons = a.exons;
+    sb.exon_off = a.exon_off;
+    sb.pseudo_off = pseudo_off;
+    sb.pseudo_bytes = pbytes;
+    sb.n_exons = 2 * n;
+    rc = spliced_core(c, r, &sb, p, o, stitched, stitch_off, srec);
+    if (rc) return rc;
+    HIP_TRY(c, espk::launch_ic(a, 4, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    return ESP_OK;
+}
+
 }  // extern "C"

@@ -376,6 +376,403 @@ __global__ __launch_bounds__(JBLOCK) void k_reext_finish(ReextArgs a) {
     a.rec[k] = r;
 }
 
+// ---- try_intron_chain_placement (intron_chain.rs:392-538) -------------------
+
+// base_eq (intron_chain.rs:113-117): the same upper-case base, A C G T only.
+__device__ __forceinline__ bool ic_base_eq(uint32_t a, uint32_t b) {
+    a &= 0xDFu;
+    b &= 0xDFu;
+    return a == b && (a == 'A' || a == 'C' || a == 'G' || a == 'T');
+}
+
+// MEM fields: contig, strand, q_lo, q_hi, r_lo, r_hi.  Order key of
+// cluster_frags (:174-183): (contig, strand, q_lo - r_lo, q_lo, r_lo).
+__device__ __forceinline__ bool mem_after(const uint32_t *x, const uint32_t *y) {
+    if (x[0] != y[0]) return x[0] > y[0];
+    if (x[1] != y[1]) return x[1] > y[1];
+    const int64_t dx = (int64_t)x[2] - (int64_t)x[4], dy = (int64_t)y[2] - (int64_t)y[4];
+    if (dx != dy) return dx > dy;
+    if (x[2] != y[2]) return x[2] > y[2];
+    return x[4] > y[4];
+}
+
+// Frag fields: contig, strand, q_lo, q_hi, r_lo, r_hi, first member, members.
+__device__ uint32_t ic_cov(const uint32_t *f, const uint32_t *mems) {
+    uint32_t total = 0, end = 0;
+    for (uint32_t i = 0; i < f[7]; i++) {
+        const uint32_t *m = mems + 6 * (f[6] + i);
+        if (m[3] > end) {
+            total += m[3] - max(m[2], end);
+            end = m[3];
+        }
+    }
+    return total;
+}
+
+// Stage 0: MEMs (extend_mems :122-166), fragments (cluster_frags :171-214),
+// the min_frag_cov filter and dominant check (:405-418), the best pair
+// (best_frag_pair :233-277) and the two refinement searches' geometry
+// (refine_donor_end :285-309, refine_acceptor_start :318-342).
+__global__ __launch_bounds__(JBLOCK) void k_ic_frags(IcArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    const uint64_t a0 = a.anc_off[k], na = a.anc_off[k + 1] - a0;
+    const uint32_t rl = a.read_len[k], K = a.k;
+    const uint8_t *read = a.reads + a.read_off[k];
+    uint32_t *mems = a.mems + 6 * a0;
+    uint32_t *frags = a.frags + 8 * a0;
+    IcState S;
+    S.reason = 0;
+    S.ctg = S.minus = S.junction_off = 0;
+    S.l_q_lo = S.l_q_hi = S.l_r_lo = S.l_r_hi = S.l_nm = 0;
+    S.r_q_lo = S.r_q_hi = S.r_r_lo = S.r_r_hi = S.r_nm = 0;
+    S.istart = S.iend = 0;
+    S.d_run = S.a_run = S.d_lo = S.a_lo = S.v_run = 0;
+    S.donor_end = S.acceptor_start = 0;
+    // both searches default to empty jobs
+    a.p_off[k] = a.p_off[a.n + k] = a.read_off[k];
+    a.p_len[k] = a.p_len[a.n + k] = 0;
+    a.t_ctg[k] = a.t_ctg[a.n + k] = 0;
+    a.t_lo[k] = a.t_lo[a.n + k] = 0;
+    a.t_len[k] = a.t_len[a.n + k] = 0;
+    if (na == 0 || rl == 0 || rl > a.max_read) {
+        S.reason = rl > a.max_read ? -1 : 1;  // -1: past the batch's max_read_len
+        a.st[k] = S;
+        return;
+    }
+    uint32_t nm = 0;
+    for (uint64_t t = 0; t < na; t++) {
+        const uint32_t *an = a.anchors + 4 * (a0 + t);
+        const uint32_t qpos = an[0], rpos = an[1], ctg = an[2];
+        if (ctg >= a.R.ncontig) continue;
+        const uint32_t clen = a.R.len[ctg];
+        const uint32_t win_lo = rpos > rl ? rpos - rl : 0u;
+        const uint32_t win_hi = (uint32_t)min((uint64_t)rpos + K + rl, (uint64_t)clen);
+        if (win_hi <= win_lo) continue;
+        uint32_t q_lo = qpos, r_lo = rpos;
+        // q_lo <= rl, r_lo <= win_hi: an anchor past the read or the contig
+        // does not extend (the reference would index out of its slices)
+        while (q_lo > 0 && r_lo > win_lo && q_lo <= rl && r_lo <= win_hi &&
+               ic_base_eq(read[q_lo - 1], base_ascii(a.R, ctg, r_lo - 1))) {
+            q_lo--;
+            r_lo--;
+        }
+        uint32_t q_hi = qpos + K, r_hi = rpos + K;
+        while (q_hi < rl && r_hi < win_hi && ic_base_eq(read[q_hi], base_ascii(a.R, ctg, r_hi))) {
+            q_hi++;
+            r_hi++;
+        }
+        if (q_hi > q_lo) {
+            uint32_t x[6] = {ctg, an[3] ? 1u : 0u, q_lo, q_hi, r_lo, r_hi};
+            // stable insertion into the sorted prefix
+            uint32_t j = nm;
+            while (j > 0 && mem_after(mems + 6 * (j - 1), x)) {
+                for (int f = 0; f < 6; f++) mems[6 * j + f] = mems[6 * (j - 1) + f];
+                j--;
+            }
+            for (int f = 0; f < 6; f++) mems[6 * j + f] = x[f];
+            nm++;
+        }
+    }
+    uint32_t nf = 0;
+    for (uint32_t i = 0; i < nm; i++) {
+        const uint32_t *x = mems + 6 * i;
+        bool start_new = true;
+        if (nf > 0) {
+            const uint32_t *f = frags + 8 * (nf - 1);
+            if (f[0] == x[0] && f[1] == x[1])
+                start_new = ((int64_t)x[2] - (int64_t)x[4]) - ((int64_t)f[2] - (int64_t)f[4]) > 5;
+        }
+        if (start_new) {
+            uint32_t *f = frags + 8 * nf++;
+            for (int q = 0; q < 6; q++) f[q] = x[q];
+            f[6] = i;
+            f[7] = 1;
+        } else {
+            uint32_t *f = frags + 8 * (nf - 1);
+            f[2] = min(f[2], x[2]);
+            f[3] = max(f[3], x[3]);
+            f[4] = min(f[4], x[4]);
+            f[5] = max(f[5], x[5]);
+            f[7]++;
+        }
+    }
+    for (uint32_t g = 0; g < nf; g++) {  // members by q_lo, stable
+        const uint32_t m0 = frags[8 * g + 6], mn = frags[8 * g + 7];
+        for (uint32_t i = m0 + 1; i < m0 + mn; i++) {
+            uint32_t x[6];
+            for (int q = 0; q < 6; q++) x[q] = mems[6 * i + q];
+            uint32_t j = i;
+            while (j > m0 && mems[6 * (j - 1) + 2] > x[2]) {
+                for (int q = 0; q < 6; q++) mems[6 * j + q] = mems[6 * (j - 1) + q];
+                j--;
+            }
+            for (int q = 0; q < 6; q++) mems[6 * j + q] = x[q];
+        }
+    }
+    uint32_t nk = 0, dom = 0;
+    for (uint32_t g = 0; g < nf; g++) {
+        const uint32_t cv = ic_cov(frags + 8 * g, mems);
+        if (cv >= a.min_frag_cov) {
+            for (int q = 0; q < 8; q++) frags[8 * nk + q] = frags[8 * g + q];
+            nk++;
+            dom = max(dom, cv);
+        }
+    }
+    if (nk == 0) S.reason = 2;
+    else if ((uint64_t)dom * 20 >= (uint64_t)rl * 11) S.reason = 3;
+    bool have = false;
+    uint32_t bi = 0, bj = 0;
+    int32_t bscore = 0;
+    for (uint32_t i = 0; S.reason == 0 && i < nk; i++) {
+        const uint32_t *fa = frags + 8 * i;
+        const uint32_t ca = ic_cov(fa, mems);
+        for (uint32_t j = 0; j < nk; j++) {
+            if (i == j) continue;
+            const uint32_t *fb = frags + 8 * j;
+            if (fa[0] != fb[0] || fa[1] != fb[1] || fa[3] > fb[2] + 3u) continue;
+            const uint32_t sp = min(fa[3], fb[2]);
+            const int64_t off_a = (int64_t)fa[5] - (int64_t)fa[3], off_b = (int64_t)fb[4] - (int64_t)fb[2];
+            const int64_t len0 = off_b - off_a;
+            if (len0 < 20 || len0 > (int64_t)a.intron_max) continue;
+            const uint32_t cb = ic_cov(fb, mems);
+            const int64_t bal = (int64_t)ca - (int64_t)cb;
+            const int32_t score = (int32_t)((ca + cb) * 4u) + (int32_t)((fa[7] + fb[7]) * 3u) -
+                                  (int32_t)(bal < 0 ? -bal : bal);
+            if (!have || score > bscore) {
+                have = true;
+                bscore = score;
+                bi = i;
+                bj = j;
+                S.istart = (uint32_t)((int64_t)sp + off_a);
+                S.iend = (uint32_t)((int64_t)sp + off_b);
+            }
+        }
+    }
+    if (S.reason == 0 && !have) S.reason = 4;
+    if (S.reason == 0 && frags[8 * bi] >= a.R.ncontig) S.reason = 5;
+    if (S.reason) {
+        S.istart = S.iend = 0;
+        a.st[k] = S;
+        return;
+    }
+    const uint32_t *L = frags + 8 * bi, *Rf = frags + 8 * bj;
+    S.ctg = L[0];
+    S.minus = L[1];
+    S.l_q_lo = L[2];
+    S.l_q_hi = L[3];
+    S.l_r_lo = L[4];
+    S.l_r_hi = L[5];
+    S.l_nm = L[7];
+    S.r_q_lo = Rf[2];
+    S.r_q_hi = Rf[3];
+    S.r_r_lo = Rf[4];
+    S.r_r_hi = Rf[5];
+    S.r_nm = Rf[7];
+    const uint32_t j_est = rl / 2;
+    S.junction_off = S.l_q_hi >= S.r_q_lo ? j_est : min(max(j_est, S.l_q_hi), S.r_q_lo);
+    const uint32_t clen = a.R.len[S.ctg];
+    const int64_t w = a.refine_window;
+    const uint32_t plen = a.refine_pattern_len;
+    const uint32_t off = min(S.junction_off, rl);
+    {  // refine_donor_end
+        const uint32_t lo = (uint32_t)max((int64_t)S.l_r_hi - w, (int64_t)0);
+        const uint32_t hi = (uint32_t)min((int64_t)S.l_r_hi + w, (int64_t)clen);
+        S.donor_end = S.l_r_hi > 0 ? S.l_r_hi - 1 : 0;
+        if (!(hi <= lo || off == 0)) {
+            const int64_t tl = (int64_t)lo - ((int64_t)S.l_r_lo - (int64_t)S.l_q_lo) + 2;
+            const uint64_t text_lo_in_read = (uint64_t)max(tl, (int64_t)0);
+            const uint64_t pat_start = max((uint64_t)(off > plen ? off - plen : 0u), text_lo_in_read);
+            if (pat_start < off) {
+                S.d_run = 1;
+                S.d_lo = lo;
+                a.p_off[k] = a.read_off[k] + pat_start;
+                a.p_len[k] = (uint32_t)(off - pat_start);
+                a.t_ctg[k] = S.ctg;
+                a.t_lo[k] = lo;
+                a.t_len[k] = hi - lo;
+            }
+        }
+    }
+    {  // refine_acceptor_start
+        const uint32_t lo = (uint32_t)max((int64_t)S.r_r_lo - w, (int64_t)0);
+        const uint32_t hi = (uint32_t)min((int64_t)S.r_r_lo + w, (int64_t)clen);
+        S.acceptor_start = S.r_r_lo;
+        if (hi > lo) {
+            const int64_t th = (int64_t)hi - ((int64_t)S.r_r_lo - (int64_t)S.r_q_lo) - 2;
+            const uint64_t text_hi_in_read = (uint64_t)max(th, (int64_t)0);
+            const uint64_t pat_end = min(min((uint64_t)off + plen, (uint64_t)rl), text_hi_in_read);
+            if (pat_end > off) {
+                S.a_run = 1;
+                S.a_lo = lo;
+                a.p_off[a.n + k] = a.read_off[k] + off;
+                a.p_len[a.n + k] = (uint32_t)(pat_end - off);
+                a.t_ctg[a.n + k] = S.ctg;
+                a.t_lo[a.n + k] = lo;
+                a.t_len[a.n + k] = hi - lo;
+            }
+        }
+    }
+    a.st[k] = S;
+}
+
+// Decodes job j's text (contig t_ctg[j], [t_lo, t_lo + t_len)) to arena + t_off[j].
+__global__ __launch_bounds__(JBLOCK) void k_ic_gather(IcArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = blockIdx.x * (JBLOCK / 64) + (threadIdx.x >> 6);
+    if (j >= 2 * a.n) return;
+    const uint32_t tl = a.t_len[j];
+    if (tl == 0) return;
+    wave_decode(a.R, a.t_ctg[j], a.t_lo[j], a.t_lo[j] + tl, false, a.arena + a.t_off[j], lane);
+}
+
+// Stage 2: the refined breakpoints, then verify_split_read's two windows
+// (:346-373) as the next two jobs.
+__global__ __launch_bounds__(JBLOCK) void k_ic_verify_jobs(IcArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    IcState S = a.st[k];
+    const uint32_t rl = a.read_len[k];
+    a.p_off[k] = a.p_off[a.n + k] = a.read_off[k];
+    a.p_len[k] = a.p_len[a.n + k] = 0;
+    a.t_len[k] = a.t_len[a.n + k] = 0;
+    if (S.reason) return;
+    if (S.d_run) S.donor_end = (int64_t)S.d_lo + (int64_t)a.end[k];
+    if (S.a_run) S.acceptor_start = (int64_t)S.a_lo + (int64_t)a.start[a.n + k];
+    const int64_t clen = a.R.len[S.ctg];
+    const uint32_t mid = rl / 2;
+    const uint32_t d_start = (uint32_t)max((int64_t)S.l_r_lo - (int64_t)S.l_q_lo - 30, (int64_t)0);
+    const uint32_t d_end = (uint32_t)min(max(S.donor_end + 1 + 30, (int64_t)1), clen);
+    const uint32_t a_start = (uint32_t)min(max(S.acceptor_start - 30, (int64_t)0), clen);
+    const uint32_t a_end = (uint32_t)min(max((int64_t)S.r_r_hi + 30, (int64_t)a_start + 1), clen);
+    S.v_run = !(d_start >= d_end || a_start >= a_end || mid == 0 || mid >= rl);
+    if (S.v_run) {
+        a.p_len[k] = mid;
+        a.t_ctg[k] = S.ctg;
+        a.t_lo[k] = d_start;
+        a.t_len[k] = d_end - d_start;
+        a.p_off[a.n + k] = a.read_off[k] + mid;
+        a.p_len[a.n + k] = rl - mid;
+        a.t_ctg[a.n + k] = S.ctg;
+        a.t_lo[a.n + k] = a_start;
+        a.t_len[a.n + k] = a_end - a_start;
+    }
+    a.st[k] = S;
+}
+
+// Stage 3: the gates (:436-446), the junction (:452-486) and its checks
+// (:487-496), the exon bounds and hint (:497-511).  Every read owns two exon
+// slots; a refused read gets exons the spliced pipeline rejects (end <= start).
+__global__ __launch_bounds__(JBLOCK) void k_ic_junction(IcArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    IcState S = a.st[k];
+    const uint32_t rl = a.read_len[k];
+    esp_intron_chain_rec r;
+    r.status = S.reason == -1 ? -1 : 0;
+    r.reason = S.reason == -1 ? 0 : S.reason;
+    r.genomic_start = 0;
+    r.score = 0;
+    r.left_start = r.jstart = r.jend = r.right_end = 0;
+    r.signal = 0;
+    r.support = r.minus = r.n_stitched = 0;
+    r.donor_end = r.acceptor_start = 0;
+    r.vd = r.va = 0;
+    uint32_t *ex = a.exons + 4 * k;
+    ex[0] = 1;
+    ex[1] = 0;
+    ex[2] = 1;
+    ex[3] = 0;
+    a.plen[k] = 0;
+    a.hint[k] = 0;
+    if (S.reason == 0) {
+        r.donor_end = S.donor_end;
+        r.acceptor_start = S.acceptor_start;
+        r.minus = S.minus;
+        const int32_t vd = S.v_run ? a.dist[k] : INT32_MAX, va = S.v_run ? a.dist[a.n + k] : INT32_MAX;
+        r.vd = vd;
+        r.va = va;
+        const int64_t intron_len = S.acceptor_start - S.donor_end - 1;
+        const int32_t max_v = max(vd, va);
+        const int32_t sum_v = (int32_t)((uint32_t)vd + (uint32_t)va);  // release i32 wraps
+        const int32_t max_thr = (int32_t)((double)rl * 0.17), sum_thr = (int32_t)((double)rl * 0.24);
+        if (S.l_nm + S.r_nm < 2) r.reason = 6;
+        else if (intron_len < 20 || intron_len > (int64_t)a.intron_max) r.reason = 7;
+        else if (max_v > max_thr || sum_v > sum_thr) r.reason = 8;
+    }
+    if (r.status == 0 && S.reason == 0 && r.reason == 0) {
+        SpliceCfgDev sp = a.sp;
+        sp.max_intron = a.intron_max;
+        const uint8_t *read = a.reads + a.read_off[k];
+        esp_refined_junction j = refine_read(a.R, a.J, sp, read, rl, S.ctg, S.l_r_hi, S.r_r_lo, S.l_q_hi, S.r_q_lo,
+                                             S.minus != 0);
+        if (!j.found) {
+            uint32_t d0, d1, a0, a1;
+            dinuc(a.R, S.ctg, S.istart, true, d0, d1);
+            dinuc(a.R, S.ctg, S.iend, false, a0, a1);
+            j.found = 2;
+            j.start = S.istart;
+            j.end = S.iend;
+            j.minus = S.minus;
+            j.signal = splice_sig(d0, d1, a0, a1, S.minus != 0);
+            j.support = jsupport(a.J, S.ctg, S.istart, S.iend, S.minus != 0);
+        }
+        r.jstart = j.start;
+        r.jend = j.end;
+        r.signal = j.signal;
+        r.support = j.support;
+        if (j.end <= j.start || j.end - j.start < 20u || j.end - j.start > a.intron_max) r.reason = 9;
+        if (r.reason == 0) {
+            const uint32_t left_start = S.l_r_lo > rl ? S.l_r_lo - rl : 0u;
+            const uint32_t right_end = (uint32_t)min((uint64_t)S.r_r_hi + rl, (uint64_t)a.R.len[S.ctg]);
+            if (left_start >= j.start || j.end >= right_end) {
+                r.reason = 10;
+            } else {
+                r.left_start = left_start;
+                r.right_end = right_end;
+                ex[0] = left_start;
+                ex[1] = j.start;
+                ex[2] = j.end;
+                ex[3] = right_end;
+                a.plen[k] = (uint64_t)(j.start - left_start) + (right_end - j.end);
+                a.hint[k] = ((int64_t)S.l_r_lo - (int64_t)S.l_q_lo) - (int64_t)left_start;
+            }
+        }
+    }
+    a.nex[k] = 2;
+    a.ctg_out[k] = S.reason == 0 ? S.ctg : 0u;
+    a.exon_off[k] = 2ull * k;
+    if (k + 1 == a.n) a.exon_off[a.n] = 2ull * a.n;
+    a.rec[k] = r;
+}
+
+// Stage 4: the acceptance (:513-530).
+__global__ __launch_bounds__(JBLOCK) void k_ic_finish(IcArgs a) {
+    const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    esp_intron_chain_rec r = a.rec[k];
+    if (r.reason != 0 || r.status != 0) return;
+    const esp_splice_rec s = a.srec[k];
+    const esp_ext e = a.ext[k];
+    r.genomic_start = s.genomic_start;
+    r.score = e.score;
+    r.n_stitched = s.n_stitched;
+    const uint32_t *cg = a.stitched + a.stitch_off[k];
+    uint64_t q = 0;
+    bool has_n = false;
+    for (uint32_t t = 0; s.status == 1 && t < s.n_stitched; t++) {
+        const uint32_t op = cg[t] & 0xFu;
+        if (op == OP_M || op == OP_I || op == OP_S) q += cg[t] >> 4;
+        if (op == OP_N) has_n = true;
+    }
+    if (s.status != 1) r.reason = 12;
+    else if (e.score <= 0) r.reason = 11;
+    else if (q != a.read_len[k]) r.reason = 12;
+    else if (!has_n) r.reason = 13;
+    r.status = r.reason ? 0 : 1;
+    a.rec[k] = r;
+}
+
 // Exclusive scans of up to two arrays in one block: each thread sums a
 // contiguous chunk, the block scans the chunk sums, each thread writes its
 // chunk's prefixes; thread 0 of the last chunk writes the totals at [n].
@@ -449,6 +846,17 @@ hipError_t launch_reext_gather(const ReextArgs &a, hipStream_t s) {
 
 hipError_t launch_reext_finish(const ReextArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_reext_finish, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ic(const IcArgs &a, int stage, hipStream_t s) {
+    switch (stage) {
+    case 0: hipLaunchKernelGGL(k_ic_frags, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a); break;
+    case 1: hipLaunchKernelGGL(k_ic_gather, dim3(wave_blocks(2 * (size_t)a.n)), dim3(JBLOCK), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_ic_verify_jobs, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_ic_junction, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_ic_finish, dim3(lane_blocks(a.n)), dim3(JBLOCK), 0, s, a); break;
+    }
     return hipGetLastError();
 }
 

@@ -481,6 +481,50 @@ hipError_t launch_reext_prep(const ReextArgs &a, hipStream_t s);
 hipError_t launch_reext_gather(const ReextArgs &a, hipStream_t s);
 hipError_t launch_reext_finish(const ReextArgs &a, hipStream_t s);
 
+// Intron-chain fast path: per-read state between the stages.
+struct IcState {
+    int32_t reason;
+    uint32_t ctg, minus, junction_off;
+    uint32_t l_q_lo, l_q_hi, l_r_lo, l_r_hi, l_nm;
+    uint32_t r_q_lo, r_q_hi, r_r_lo, r_r_hi, r_nm;
+    uint32_t istart, iend;
+    uint32_t d_run, a_run, d_lo, a_lo, v_run;
+    int64_t donor_end, acceptor_start;
+};
+struct IcArgs {
+    RefView R;
+    JunctionView J;
+    SpliceCfgDev sp;
+    uint32_t n, k, max_read;
+    uint32_t min_frag_cov, refine_window, refine_pattern_len, intron_max;
+    const uint8_t *reads;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *anchors;
+    const uint64_t *anc_off;
+    uint32_t *mems;   // 6 per anchor slot
+    uint32_t *frags;  // 8 per anchor slot
+    IcState *st;
+    // EA-Myers jobs, two per read (job k and n + k)
+    uint64_t *p_off;
+    uint32_t *p_len, *t_ctg, *t_lo, *t_len;
+    const uint64_t *t_off;
+    uint8_t *arena;
+    const int32_t *dist;
+    const uint64_t *end, *start;
+    // the spliced pipeline's inputs and records
+    uint32_t *exons, *nex, *ctg_out;
+    uint64_t *exon_off;
+    uint64_t *plen;
+    int64_t *hint;
+    const esp_splice_rec *srec;
+    const esp_ext *ext;
+    const uint32_t *stitched;
+    const uint64_t *stitch_off;
+    esp_intron_chain_rec *rec;
+};
+hipError_t launch_ic(const IcArgs &a, int stage, hipStream_t s);
+
 // Exclusive prefix sums of up to two per-item arrays (either may be NULL)
 // into n + 1 entries each; the last entry is the total.
 hipError_t launch_scan2(const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off,

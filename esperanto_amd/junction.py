@@ -278,3 +278,77 @@ def reextend_batch(dref: DeviceReference, queries: Sequence[bytes], contigs: Seq
     finally:
         for a in d.values():
             a.free()
+
+
+IC_REC_DTYPE = np.dtype([("status", "<i4"), ("reason", "<i4"), ("genomic_start", "<u4"), ("score", "<i4"),
+                         ("left_start", "<u4"), ("jstart", "<u4"), ("jend", "<u4"), ("right_end", "<u4"),
+                         ("signal", "<i4"), ("support", "<u4"), ("minus", "<u4"), ("n_stitched", "<u4"),
+                         ("donor_end", "<i8"), ("acceptor_start", "<i8"), ("vd", "<i4"), ("va", "<i4")])
+
+
+@dataclass(frozen=True)
+class IntronParams:
+    """intron_chain.rs:20-42 (defaults :32-42)."""
+
+    min_frag_cov: int = 15
+    refine_window: int = 30
+    refine_pattern_len: int = 50
+    intron_max: int = 105_000
+
+
+def intron_chain_batch(dref: DeviceReference, jlib: Optional[JunctionLib], reads: Sequence[bytes],
+                       anchors: Sequence[Sequence[Tuple[int, int, int, int]]], k: int = 15,
+                       params: IntronParams = IntronParams(), extend_params: Optional[ExtendParams] = None):
+    """try_intron_chain_placement per read; anchors[r] = the read's (qpos, rpos, contig, minus) pool.
+    Returns (records, extensions, stitched CIGARs)."""
+    buf = dref.buf
+    n = len(reads)
+    if n == 0:
+        return np.zeros(0, IC_REC_DTYPE), [], []
+    rb, ro, rl = _concat(reads)
+    na = np.array([len(a) for a in anchors], dtype=np.uint64)
+    ao = np.zeros(n + 1, dtype=np.uint64)
+    ao[1:] = np.cumsum(na)
+    flat = np.array([x for a in anchors for an in a for x in an], dtype=np.uint32) if int(ao[-1]) else \
+        np.zeros(4, np.uint32)
+    caps = 2 * rl.astype(np.uint64) + 5
+    co = np.zeros(n + 1, dtype=np.uint64)
+    co[1:] = np.cumsum(caps)
+    so = np.zeros(n + 1, dtype=np.uint64)
+    so[1:] = np.cumsum(caps + 4)
+    d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
+         (("reads", rb), ("read_off", ro), ("read_len", rl), ("anchors", flat), ("anc_off", ao), ("cig_off", co),
+          ("stitch_off", so))}
+    d["ext"] = DeviceArray(buf, 16 * n)
+    d["cigar"] = DeviceArray(buf, 4 * int(co[-1]))
+    d["cig_n"] = DeviceArray(buf, 4 * n)
+    d["stitched"] = DeviceArray(buf, 4 * int(so[-1]))
+    d["rec"] = DeviceArray(buf, IC_REC_DTYPE.itemsize * n)
+    try:
+        cb = _native.CIntronChainBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr, d["anchors"].ptr,
+                                       d["anc_off"].ptr, int(rl.max()) if n else 1)
+        ip = _native.CIntronParams(params.min_frag_cov, params.refine_window, params.refine_pattern_len,
+                                   params.intron_max)
+        out = _native.CBatchOut(d["ext"].ptr, None, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr, None)
+        ep = (extend_params or ExtendParams()).to_c()
+        _native.check(_native.lib().esp_intron_chain_batch_run(
+            buf.handle, dref.handle, jlib.handle if jlib else None, ctypes.byref(cb), k, ctypes.byref(ip),
+            ctypes.byref(ep), ctypes.byref(out), ctypes.c_void_p(d["stitched"].ptr),
+            ctypes.c_void_p(d["stitch_off"].ptr), ctypes.c_void_p(d["rec"].ptr)), buf.handle)
+        _native.check(_native.lib().esp_sync(buf.handle), buf.handle)
+        rec = d["rec"].to_numpy(np.uint8, IC_REC_DTYPE.itemsize * n).view(IC_REC_DTYPE)
+        e = d["ext"].to_numpy(np.uint32, 4 * n).reshape(n, 4)
+        cig = d["cigar"].to_numpy(np.uint32, int(co[-1]))
+        cn = d["cig_n"].to_numpy(np.uint32, n)
+        st = d["stitched"].to_numpy(np.uint32, int(so[-1]))
+        exts, stitched = [], []
+        for c in range(n):
+            ran = int(rec[c]["left_start"]) or int(rec[c]["status"]) == 1
+            exts.append(Extension(int(e[c, 0]), int(e[c, 1]), int(e[c, 2]),
+                                  cigar_from_words(cig[int(co[c]):int(co[c]) + int(cn[c])]), _i32(int(e[c, 3])))
+                        if ran else None)
+            stitched.append(cigar_from_words(st[int(so[c]):int(so[c]) + int(rec[c]["n_stitched"])]))
+        return rec, exts, stitched
+    finally:
+        for a in d.values():
+            a.free()

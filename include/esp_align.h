@@ -609,6 +609,63 @@ int esp_reextend_batch_run(esp_ctx *ctx, const esp_ref *ref, const esp_reextend_
                            int edit_dir, esp_reextend_rec *rec, uint32_t *cigar_out, const uint64_t *cig_out_off,
                            uint32_t *cig_out_n);
 
+/* The intron-chain fast path, try_intron_chain_placement
+ * (crates/map/src/intron_chain.rs:392-538), per read: anchors extended into
+ * maximal exact matches, clustered into fragments by implied read start,
+ * the best fragment pair, EA-Myers refinement of the donor end and acceptor
+ * start, split-read verification, the read-aware junction (or the pair's own
+ * junction when refinement finds none), the two-exon pseudo-reference
+ * extension and its acceptance (positive score, read-conserving stitched
+ * CIGAR holding a RefSkip).  Anchors: 4 u32 each (qpos, rpos, contig, strand
+ * 0 Plus / 1 Minus) at anchors[4 * anc_off[k] .. 4 * anc_off[k + 1]), the
+ * read's whole anchor pool in any order.  Device pointers; max_read_len
+ * (host) bounds read_len over the batch. */
+typedef struct {
+    size_t nread;
+    const uint8_t *reads;      /* alignment orientation */
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *anchors;
+    const uint64_t *anc_off;   /* nread + 1 */
+    uint32_t max_read_len;
+} esp_intron_chain_batch;
+/* IntronParams (intron_chain.rs:20-42; defaults 15, 30, 50, 105000). */
+typedef struct {
+    uint32_t min_frag_cov;
+    uint32_t refine_window;
+    uint32_t refine_pattern_len;
+    uint32_t intron_max;
+} esp_intron_params;
+typedef struct {
+    int32_t status;          /* 1 placed, 0 None, -1 read longer than max_read_len */
+    int32_t reason;          /* with status 0: 1 no anchors or empty read, 2 no fragment
+                                of min_frag_cov, 3 a dominant fragment (unspliced read),
+                                4 no fragment pair, 5 contig out of range, 6 fewer than
+                                two anchors, 7 intron length, 8 split verification,
+                                9 junction, 10 exon bounds, 11 extension score, 12 CIGAR
+                                not read-conserving, 13 no RefSkip */
+    uint32_t genomic_start;
+    int32_t score;
+    uint32_t left_start;     /* exon bounds [left_start, jstart) ++ [jend, right_end) */
+    uint32_t jstart;
+    uint32_t jend;
+    uint32_t right_end;
+    int32_t signal;          /* the junction's signal (0 GT-AG .. 3 non-canonical) */
+    uint32_t support;
+    uint32_t minus;
+    uint32_t n_stitched;
+    int64_t donor_end;       /* the EA-Myers refinements (reasons >= 6) */
+    int64_t acceptor_start;
+    int32_t vd;              /* split verification distances */
+    int32_t va;
+} esp_intron_chain_rec;
+/* ext_out: pseudo-reference extensions (pair k needs esp_cigar_capacity(m)
+ * words); stitched/stitch_off: genomic CIGARs (esp_cigar_capacity(m) + 4). */
+int esp_intron_chain_batch_run(esp_ctx *ctx, const esp_ref *ref, const esp_junctions *junctions,
+                               const esp_intron_chain_batch *b, uint32_t k, const esp_intron_params *ip,
+                               const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
+                               const uint64_t *stitch_off, esp_intron_chain_rec *rec);
+
 /* The SpliceParams fields rescore_with reads (splice.rs:21-52; defaults 6,
  * 4, 2, 1, 8) and the seed k-mer length of exact_kmer_support (SeedParams
  * k, 15 in the reference's configurations). */

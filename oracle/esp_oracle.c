@@ -2369,3 +2369,349 @@ int espo_reextend_with(const espo_ref *R, const uint8_t *query, size_t m, uint32
     }
     return rc;
 }
+
+/* ------------------------------------------------------ intron-chain fast path */
+
+/* IntronParams (intron_chain.rs:20-42). */
+typedef struct {
+    uint32_t min_frag_cov, refine_window, refine_pattern_len, intron_max;
+} espo_intron_cfg;
+
+typedef struct {
+    uint32_t contig, strand, q_lo, q_hi, r_lo, r_hi;
+} ic_mem;
+
+typedef struct {
+    uint32_t contig, strand, q_lo, q_hi, r_lo, r_hi;
+    size_t m0, nm; /* member MEMs mems[m0 .. m0 + nm), sorted by q_lo */
+} ic_frag;
+
+/* base_eq: intron_chain.rs:113-117. */
+static int ic_base_eq(uint8_t a, uint8_t b) {
+    a &= 0xDF;
+    b &= 0xDF;
+    return a >= 'A' && a <= 'T' && a == b && (a == 'A' || a == 'C' || a == 'G' || a == 'T');
+}
+
+static int64_t mem_diag(const ic_mem *m) { return (int64_t)m->q_lo - (int64_t)m->r_lo; }
+
+static int mem_key_cmp(const ic_mem *x, const ic_mem *y) {
+    if (x->contig != y->contig) return x->contig < y->contig ? -1 : 1;
+    if (x->strand != y->strand) return x->strand < y->strand ? -1 : 1;
+    if (mem_diag(x) != mem_diag(y)) return mem_diag(x) < mem_diag(y) ? -1 : 1;
+    if (x->q_lo != y->q_lo) return x->q_lo < y->q_lo ? -1 : 1;
+    if (x->r_lo != y->r_lo) return x->r_lo < y->r_lo ? -1 : 1;
+    return 0;
+}
+
+/* Frag::cov (intron_chain.rs:91-102) over q_lo-sorted members. */
+static uint32_t frag_cov(const ic_frag *f, const ic_mem *mems) {
+    uint32_t total = 0, end = 0;
+    for (size_t i = 0; i < f->nm; i++) {
+        const ic_mem *m = &mems[f->m0 + i];
+        if (m->q_hi > end) {
+            total += m->q_hi - (m->q_lo > end ? m->q_lo : end);
+            end = m->q_hi;
+        }
+    }
+    return total;
+}
+
+static int64_t frag_off_lo(const ic_frag *f) { return (int64_t)f->r_lo - (int64_t)f->q_lo; }
+
+typedef struct {
+    int32_t status;   /* 1 placed, 0 None (reason says which stage) */
+    int32_t reason;   /* 1 no anchors/read, 2 no fragment, 3 dominant fragment, 4 no pair, 5 contig,
+                         6 anchors, 7 intron length, 8 split verification, 9 junction, 10 exon bounds,
+                         11 extension score, 12 CIGAR not conserving, 13 no RefSkip */
+    uint32_t genomic_start;
+    int32_t score;
+    uint32_t left_start, jstart, jend, right_end;
+    int32_t signal;
+    uint32_t support, minus, n_stitched;
+    int64_t donor_end, acceptor_start;
+    int32_t vd, va;
+} espo_ic_rec;
+
+/* try_intron_chain_placement: intron_chain.rs:392-538 with extend_mems
+ * (:122-166), cluster_frags (:171-214), best_frag_pair (:233-277),
+ * refine_donor_end (:285-309), refine_acceptor_start (:318-342) and
+ * verify_split_read (:346-373).  anchors: 4 u32 each (qpos, rpos, contig,
+ * strand 0 Plus / 1 Minus).  stitched receives the genomic CIGAR. */
+int espo_intron_chain(const espo_ref *R, const espo_jlib *L, const uint8_t *read, size_t m,
+                      const uint32_t *anchors, size_t na, uint32_t k, const espo_intron_cfg *ip,
+                      const espo_params *p, espo_ext *ext, uint32_t *cigar, size_t cap, size_t *n_ops,
+                      uint32_t *stitched, size_t scap, espo_ic_rec *rec) {
+    memset(rec, 0, sizeof(*rec));
+    memset(ext, 0, sizeof(*ext));
+    *n_ops = 0;
+    if (na == 0 || m == 0) { rec->reason = 1; return 0; }
+    const uint32_t rl = (uint32_t)m;
+    ic_mem *mems = (ic_mem *)malloc(na * sizeof(ic_mem));
+    size_t nm = 0;
+    for (size_t a = 0; a < na; a++) {
+        const uint32_t qpos = anchors[4 * a], rpos = anchors[4 * a + 1], ctg = anchors[4 * a + 2];
+        if (ctg >= R->ncontig) continue;
+        const uint32_t clen = R->len[ctg];
+        const uint32_t win_lo = rpos > rl ? rpos - rl : 0u;
+        uint64_t wh = (uint64_t)rpos + k + rl;
+        const uint32_t win_hi = (uint32_t)(wh < clen ? wh : clen);
+        if (win_hi <= win_lo) continue;
+        uint32_t q_lo = qpos, r_lo = rpos;
+        /* q_lo <= rl, r_lo <= win_hi: an anchor past the read or the contig (the
+         * reference would panic) does not extend */
+        while (q_lo > 0 && r_lo > win_lo && q_lo <= rl && r_lo <= win_hi &&
+               ic_base_eq(read[q_lo - 1], contig_base(R, ctg, r_lo - 1))) {
+            q_lo--;
+            r_lo--;
+        }
+        uint32_t q_hi = qpos + k, r_hi = rpos + k;
+        while (q_hi < rl && r_hi < win_hi && ic_base_eq(read[q_hi], contig_base(R, ctg, r_hi))) { q_hi++; r_hi++; }
+        if (q_hi > q_lo) {
+            ic_mem x = {ctg, anchors[4 * a + 3] ? 1u : 0u, q_lo, q_hi, r_lo, r_hi};
+            mems[nm++] = x;
+        }
+    }
+    /* stable insertion sort by (contig, strand, diag, q_lo, r_lo) */
+    for (size_t i = 1; i < nm; i++) {
+        ic_mem x = mems[i];
+        size_t j = i;
+        while (j > 0 && mem_key_cmp(&mems[j - 1], &x) > 0) { mems[j] = mems[j - 1]; j--; }
+        mems[j] = x;
+    }
+    ic_frag *frags = (ic_frag *)malloc((nm ? nm : 1) * sizeof(ic_frag));
+    size_t nf = 0;
+    for (size_t i = 0; i < nm; i++) {
+        const ic_mem *x = &mems[i];
+        int start_new = 1;
+        if (nf > 0 && frags[nf - 1].contig == x->contig && frags[nf - 1].strand == x->strand)
+            start_new = mem_diag(x) - ((int64_t)frags[nf - 1].q_lo - (int64_t)frags[nf - 1].r_lo) > 5;
+        if (start_new) {
+            ic_frag f = {x->contig, x->strand, x->q_lo, x->q_hi, x->r_lo, x->r_hi, i, 1};
+            frags[nf++] = f;
+        } else {
+            ic_frag *f = &frags[nf - 1];
+            if (x->q_lo < f->q_lo) f->q_lo = x->q_lo;
+            if (x->q_hi > f->q_hi) f->q_hi = x->q_hi;
+            if (x->r_lo < f->r_lo) f->r_lo = x->r_lo;
+            if (x->r_hi > f->r_hi) f->r_hi = x->r_hi;
+            f->nm++;
+        }
+    }
+    for (size_t f = 0; f < nf; f++) /* members by q_lo, stable */
+        for (size_t i = frags[f].m0 + 1; i < frags[f].m0 + frags[f].nm; i++) {
+            ic_mem x = mems[i];
+            size_t j = i;
+            while (j > frags[f].m0 && mems[j - 1].q_lo > x.q_lo) { mems[j] = mems[j - 1]; j--; }
+            mems[j] = x;
+        }
+    /* keep fragments with cov >= min_frag_cov */
+    size_t nk = 0;
+    uint32_t dom = 0;
+    for (size_t f = 0; f < nf; f++) {
+        const uint32_t cv = frag_cov(&frags[f], mems);
+        if (cv >= ip->min_frag_cov) {
+            frags[nk++] = frags[f];
+            if (cv > dom) dom = cv;
+        }
+    }
+    int reason = 0;
+    if (nk == 0) reason = 2;
+    else if ((uint64_t)dom * 20 >= (uint64_t)m * 11) reason = 3;
+    int have = 0;
+    size_t bi = 0, bj = 0;
+    int32_t bscore = 0;
+    uint32_t split = 0, istart = 0, iend = 0;
+    for (size_t i = 0; !reason && i < nk; i++)
+        for (size_t j = 0; j < nk; j++) {
+            if (i == j) continue;
+            const ic_frag *a = &frags[i], *b = &frags[j];
+            if (a->contig != b->contig || a->strand != b->strand || a->q_hi > b->q_lo + 3) continue;
+            const uint32_t sp = a->q_hi < b->q_lo ? a->q_hi : b->q_lo;
+            const int64_t off_a = (int64_t)a->r_hi - (int64_t)a->q_hi, off_b = frag_off_lo(b);
+            const int64_t len0 = off_b - off_a;
+            if (len0 < 20 || len0 > (int64_t)ip->intron_max) continue;
+            const uint32_t ca = frag_cov(a, mems), cb = frag_cov(b, mems);
+            const int64_t bal = (int64_t)ca - (int64_t)cb;
+            const int32_t score = (int32_t)((ca + cb) * 4) + (int32_t)((a->nm + b->nm) * 3) -
+                                  (int32_t)(bal < 0 ? -bal : bal);
+            if (!have || score > bscore) {
+                have = 1;
+                bscore = score;
+                bi = i;
+                bj = j;
+                split = sp;
+                istart = (uint32_t)((int64_t)sp + off_a);
+                iend = (uint32_t)((int64_t)sp + off_b);
+            }
+        }
+    if (!reason && !have) reason = 4;
+    ic_frag left = {0}, right = {0};
+    if (!reason) {
+        left = frags[bi];
+        right = frags[bj];
+        if (left.contig >= R->ncontig) reason = 5;
+    }
+    (void)split;
+    if (reason) {
+        rec->reason = reason;
+        free(mems);
+        free(frags);
+        return 0;
+    }
+    const uint32_t ctg = left.contig, clen = R->len[ctg];
+    const int minus = left.strand == 1;
+    const uint32_t j_est = (uint32_t)(m / 2);
+    const uint32_t junction_off =
+        left.q_hi >= right.q_lo ? j_est : (j_est < left.q_hi ? left.q_hi : (j_est > right.q_lo ? right.q_lo : j_est));
+    const int64_t w = ip->refine_window;
+    const size_t plen = ip->refine_pattern_len;
+    /* refine_donor_end */
+    int64_t donor_end;
+    {
+        const size_t off = junction_off < m ? junction_off : m;
+        const int64_t lo64 = (int64_t)left.r_hi - w, hi64 = (int64_t)left.r_hi + w;
+        const uint32_t lo = (uint32_t)(lo64 > 0 ? lo64 : 0), hi = (uint32_t)(hi64 < (int64_t)clen ? hi64 : clen);
+        const uint32_t dflt = left.r_hi > 0 ? left.r_hi - 1 : 0u;
+        donor_end = dflt;
+        if (!(hi <= lo || off == 0)) {
+            const int64_t tl = (int64_t)lo - frag_off_lo(&left) + 2;
+            const size_t text_lo_in_read = (size_t)(tl > 0 ? tl : 0);
+            size_t pat_start = off > plen ? off - plen : 0;
+            if (text_lo_in_read > pat_start) pat_start = text_lo_in_read;
+            if (pat_start < off) {
+                uint8_t *text = (uint8_t *)malloc(hi - lo);
+                espo_decode(R, ctg, lo, hi, 0, text);
+                int32_t d;
+                size_t end;
+                espo_myers_best_end(read + pat_start, off - pat_start, text, hi - lo, &d, &end);
+                donor_end = (int64_t)lo + (int64_t)end;
+                free(text);
+            }
+        }
+    }
+    /* refine_acceptor_start */
+    int64_t acceptor_start;
+    {
+        const size_t off = junction_off < m ? junction_off : m;
+        const int64_t lo64 = (int64_t)right.r_lo - w, hi64 = (int64_t)right.r_lo + w;
+        const uint32_t lo = (uint32_t)(lo64 > 0 ? lo64 : 0), hi = (uint32_t)(hi64 < (int64_t)clen ? hi64 : clen);
+        acceptor_start = right.r_lo;
+        if (hi > lo) {
+            const int64_t th = (int64_t)hi - frag_off_lo(&right) - 2;
+            const size_t text_hi_in_read = (size_t)(th > 0 ? th : 0);
+            size_t pat_end = off + plen;
+            if (pat_end > m) pat_end = m;
+            if (pat_end > text_hi_in_read) pat_end = text_hi_in_read;
+            if (pat_end > off) {
+                uint8_t *text = (uint8_t *)malloc(hi - lo);
+                espo_decode(R, ctg, lo, hi, 0, text);
+                int32_t d;
+                size_t start;
+                espo_myers_best_start(read + off, pat_end - off, text, hi - lo, &d, &start);
+                acceptor_start = (int64_t)lo + (int64_t)start;
+                free(text);
+            }
+        }
+    }
+    rec->donor_end = donor_end;
+    rec->acceptor_start = acceptor_start;
+    const int64_t intron_len = acceptor_start - donor_end - 1;
+    /* verify_split_read */
+    int32_t vd = INT32_MAX, va = INT32_MAX;
+    {
+        const size_t mid = m / 2;
+        const int64_t ds = frag_off_lo(&left) - 30;
+        const uint32_t d_start = (uint32_t)(ds > 0 ? ds : 0);
+        int64_t de = donor_end + 1 + 30;
+        if (de < 1) de = 1;
+        if (de > (int64_t)clen) de = clen;
+        const uint32_t d_end = (uint32_t)de;
+        int64_t as = acceptor_start - 30;
+        if (as < 0) as = 0;
+        if (as > (int64_t)clen) as = clen;
+        const uint32_t a_start = (uint32_t)as;
+        int64_t ae = (int64_t)right.r_hi + 30;
+        if (ae < (int64_t)a_start + 1) ae = (int64_t)a_start + 1;
+        if (ae > (int64_t)clen) ae = clen;
+        const uint32_t a_end = (uint32_t)ae;
+        if (!(d_start >= d_end || a_start >= a_end || mid == 0 || mid >= m)) {
+            uint8_t *t1 = (uint8_t *)malloc(d_end - d_start), *t2 = (uint8_t *)malloc(a_end - a_start);
+            espo_decode(R, ctg, d_start, d_end, 0, t1);
+            espo_decode(R, ctg, a_start, a_end, 0, t2);
+            size_t e;
+            espo_myers_best_end(read, mid, t1, d_end - d_start, &vd, &e);
+            espo_myers_best_end(read + mid, m - mid, t2, a_end - a_start, &va, &e);
+            free(t1);
+            free(t2);
+        }
+    }
+    rec->vd = vd;
+    rec->va = va;
+    const size_t total_anchors = left.nm + right.nm;
+    const int32_t max_v = vd > va ? vd : va;
+    const int32_t sum_v = (int32_t)((uint32_t)vd + (uint32_t)va); /* release i32 wraps */
+    const int32_t max_thr = (int32_t)((double)m * 0.17), sum_thr = (int32_t)((double)m * 0.24);
+    if (total_anchors < 2) reason = 6;
+    else if (intron_len < 20 || intron_len > (int64_t)ip->intron_max) reason = 7;
+    else if (max_v > max_thr || sum_v > sum_thr) reason = 8;
+    espo_refined rj;
+    memset(&rj, 0, sizeof(rj));
+    uint32_t left_start = 0, right_end = 0;
+    if (!reason) {
+        espo_splice_cfg sp = {20, ip->intron_max, 4, 2, 1, 8, 6, 12};
+        if (!espo_refine_junction_read(R, read, m, ctg, left.r_hi, right.r_lo, left.q_hi, right.q_lo, minus, L, &sp,
+                                       &rj)) {
+            uint8_t dn[2], ac[2];
+            dinuc_at(R, ctg, istart, 1, dn);
+            dinuc_at(R, ctg, iend, 0, ac);
+            rj.found = 2;
+            rj.start = istart;
+            rj.end = iend;
+            rj.minus = (uint32_t)minus;
+            rj.signal = signal_of(dn, ac, minus);
+            rj.support = espo_jlib_support(L, ctg, istart, iend, (uint32_t)minus);
+        }
+        if (rj.end <= rj.start || rj.end - rj.start < 20 || rj.end - rj.start > ip->intron_max) reason = 9;
+    }
+    if (!reason) {
+        left_start = left.r_lo > rl ? left.r_lo - rl : 0u;
+        const uint64_t re = (uint64_t)right.r_hi + rl;
+        right_end = (uint32_t)(re < clen ? re : clen);
+        if (left_start >= rj.start || rj.end >= right_end) reason = 10;
+    }
+    free(mems);
+    free(frags);
+    rec->jstart = rj.start;
+    rec->jend = rj.end;
+    rec->signal = rj.signal;
+    rec->support = rj.support;
+    rec->minus = (uint32_t)minus;
+    if (reason) {
+        rec->reason = reason;
+        return 0;
+    }
+    rec->left_start = left_start;
+    rec->right_end = right_end;
+    const uint32_t exons[4] = {left_start, rj.start, rj.end, right_end};
+    const int64_t hint = frag_off_lo(&left) - (int64_t)left_start;
+    espo_splice_rec srec;
+    int rc = espo_spliced(R, read, m, ctg, exons, 2, hint, p, ext, cigar, cap, n_ops, stitched, scap, &srec);
+    rec->genomic_start = srec.genomic_start;
+    rec->score = ext->score;
+    rec->n_stitched = srec.n_stitched;
+    uint64_t q = 0;
+    int has_n = 0;
+    for (size_t i = 0; i < srec.n_stitched; i++) {
+        const uint32_t kind = op_kind(stitched[i]);
+        if (kind == OP_M || kind == OP_I || kind == OP_S) q += op_len(stitched[i]);
+        if (kind == OP_N) has_n = 1;
+    }
+    if (srec.status != 1) reason = 12;
+    else if (ext->score <= 0) reason = 11;
+    else if (q != m) reason = 12;
+    else if (!has_n) reason = 13;
+    rec->reason = reason;
+    rec->status = reason ? 0 : 1;
+    return rc;
+}

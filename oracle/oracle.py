@@ -771,3 +771,51 @@ def reextend_with(oref: OracleRef, query: bytes, contig: int, pos: int, cigar_wo
                          int(edit_dir), ctypes.byref(rec), out, cap, ctypes.byref(n))
     return {"status": rec.status, "pos": rec.pos, "score": rec.score, "wstart": rec.wstart, "wlen": rec.wlen,
             "aligned": rec.aligned, "new_aligned": rec.new_aligned, "cigar": list(out[: n.value])}
+
+
+class CIntronCfg(ctypes.Structure):
+    """IntronParams (intron_chain.rs:20-42)."""
+    _fields_ = [("min_frag_cov", ctypes.c_uint32), ("refine_window", ctypes.c_uint32),
+                ("refine_pattern_len", ctypes.c_uint32), ("intron_max", ctypes.c_uint32)]
+
+
+INTRON_CFG_DEFAULTS = (15, 30, 50, 105_000)  # IntronParams::default, intron_chain.rs:32-42
+
+
+class CIcRec(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("reason", ctypes.c_int32), ("genomic_start", ctypes.c_uint32),
+                ("score", ctypes.c_int32), ("left_start", ctypes.c_uint32), ("jstart", ctypes.c_uint32),
+                ("jend", ctypes.c_uint32), ("right_end", ctypes.c_uint32), ("signal", ctypes.c_int32),
+                ("support", ctypes.c_uint32), ("minus", ctypes.c_uint32), ("n_stitched", ctypes.c_uint32),
+                ("donor_end", ctypes.c_int64), ("acceptor_start", ctypes.c_int64), ("vd", ctypes.c_int32),
+                ("va", ctypes.c_int32)]
+
+
+def intron_chain(oref: OracleRef, jlib: Optional[JLib], read: bytes, anchors: Sequence[Tuple[int, int, int, int]],
+                 k: int = 15, cfg: Tuple[int, int, int, int] = INTRON_CFG_DEFAULTS,
+                 params: Optional[ExtendParams] = None) -> dict:
+    """intron_chain.rs:392 try_intron_chain_placement; anchors (qpos, rpos, contig, minus)."""
+    L = _splice_lib()
+    if not getattr(L, "_ic_bound", False):
+        P, vp, sz = ctypes.POINTER, ctypes.c_void_p, ctypes.c_size_t
+        u32p = P(ctypes.c_uint32)
+        L.espo_intron_chain.argtypes = [P(CRef), vp, P(ctypes.c_uint8), sz, u32p, sz, ctypes.c_uint32, P(CIntronCfg),
+                                        P(CParams), P(CExt), u32p, sz, P(sz), u32p, sz, P(CIcRec)]
+        L._ic_bound = True
+    r, m = _buf(read)
+    a = np.ascontiguousarray(np.array(anchors, dtype=np.uint32).reshape(-1)) if len(anchors) else \
+        np.zeros(4, np.uint32)
+    cap = cigar_capacity(m)
+    cig = (ctypes.c_uint32 * cap)()
+    st = (ctypes.c_uint32 * (cap + 4))()
+    e, rec, n = CExt(), CIcRec(), ctypes.c_size_t()
+    c = CIntronCfg(*cfg)
+    L.espo_intron_chain(ctypes.byref(oref.c), jlib.h if jlib else None, r, m,
+                        a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(anchors), k, ctypes.byref(c),
+                        ctypes.byref(_params(params)), ctypes.byref(e), cig, cap, ctypes.byref(n), st, cap + 4,
+                        ctypes.byref(rec))
+    out = {f: getattr(rec, f) for f, _ in CIcRec._fields_}
+    out["signal"] = SIGNALS[rec.signal]
+    out["ext"] = Extension.from_c(e, cig[: n.value]) if rec.left_start or rec.status else None
+    out["stitched"] = cigar_from_words(st[: rec.n_stitched])
+    return out

@@ -107,3 +107,55 @@ def refine_queries(seed: int, chains, truth, n: int = 600):
             q = q + ((s + int(rng.integers(-3, 4)), e),)
         out.append(q)
     return out
+
+
+def intron_chain_workload(seed: int, nreads: int = 300, read_len: int = 150):
+    """Reads over planted two-exon loci with their anchor pools (qpos, rpos,
+    contig, minus): exact k-mers on both exons at a stride of 3..7, a few
+    noise anchors elsewhere (and on the other strand), plus unspliced reads
+    (one dominant fragment) and reads with a short exon side.  Returns
+    (reference, reads, anchor pools, library quads)."""
+    rng = np.random.default_rng(seed)
+    span = 9000
+    genome = bytearray(np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, span * nreads + 4000)].tobytes())
+    reads, pools, quads = [], [], []
+    for c in range(nreads):
+        base = 2000 + c * span
+        kind = c % 6
+        minus = bool(rng.integers(0, 2))
+        L = int(np.exp(rng.uniform(np.log(60), np.log(6000))))
+        s = base + 700
+        e = s + L
+        d, a = MOTIF[minus]
+        genome[s:s + 2] = d
+        genome[e - 2:e] = a
+        # the fast path places balanced splits (no fragment over 55% of the read)
+        u = int(rng.integers(68, 83)) if kind < 4 else int(rng.integers(8, 40))
+        if kind == 5:  # unspliced
+            read = bytearray(genome[s - u:s - u + read_len])
+            exons = [(0, s - u, read_len)]
+        else:
+            read = bytearray(genome[s - u:s] + genome[e:e + (read_len - u)])
+            exons = [(0, s - u, u), (u, e, read_len - u)]
+        _mutate(rng, read, 0.01)
+        if rng.random() < 0.1:  # a technical clip
+            nn = int(rng.integers(1, 12))
+            read[:nn] = b"N" * nn
+        stride = int(rng.integers(3, 8))
+        pool = []
+        rb = bytes(read)
+        for q0, r0, ln in exons:
+            for off in range(int(rng.integers(0, stride)), ln - K + 1, stride):
+                q, r = q0 + off, r0 + off
+                if rb[q:q + K] == bytes(genome[r:r + K]):
+                    pool.append((q, r, 0, int(minus)))
+        for _ in range(int(rng.integers(0, 4))):  # noise anchors
+            q = int(rng.integers(0, read_len - K))
+            pool.append((q, int(rng.integers(0, len(genome) - K)), 0, int(rng.integers(0, 2))))
+        order = rng.permutation(len(pool))
+        pools.append([pool[i] for i in order])
+        reads.append(rb)
+        if rng.random() < 0.3:
+            quads.append((0, s, e, int(minus)))
+    ref = Reference.from_sequences([bytes(genome)])
+    return ref, reads, pools, quads

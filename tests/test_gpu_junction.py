@@ -234,3 +234,63 @@ def test_reextend_batch_matches_oracle(buf, edit_dir):
         assert moved > 50, moved
     finally:
         dref.close()
+
+
+IC_FIELDS = ("status", "reason", "genomic_start", "score", "left_start", "jstart", "jend", "right_end", "support",
+             "minus", "n_stitched", "donor_end", "acceptor_start", "vd", "va")
+SIG = ("GtAg", "GcAg", "AtAc", "NonCanonical")
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_intron_chain_batch_matches_oracle(buf, seed):
+    from esperanto_amd.junction import JunctionLib, intron_chain_batch
+    from esperanto_amd.reference import DeviceReference
+    ref, reads, pools, quads = splicegen.intron_chain_workload(seed, 600)
+    o = oracle.OracleRef(ref)
+    jl = oracle.JLib(quads)
+    dref = DeviceReference(ref, buf)
+    dlib = JunctionLib(dref, quads)
+    try:
+        say(f"intron chain: {len(reads)} reads")
+        rec, exts, stitched = intron_chain_batch(dref, dlib, reads, pools)
+        placed = 0
+        for k, (r, p) in enumerate(zip(reads, pools)):
+            w = oracle.intron_chain(o, jl, r, p)
+            got = {f: int(rec[k][f]) for f in IC_FIELDS}
+            want = {f: int(w[f]) for f in IC_FIELDS}
+            assert got == want, (k, got, want)
+            assert SIG[int(rec[k]["signal"])] == w["signal"], k
+            assert stitched[k] == w["stitched"], k
+            if w["status"] == 1:
+                assert exts[k] == w["ext"], k
+                placed += 1
+        assert placed > 300, placed
+    finally:
+        dlib.close()
+        dref.close()
+        jl.close()
+
+
+def test_intron_chain_known_answer(buf):
+    """align.rs:5067-5120: pos 1000, [S11, M60, N1940, M79], score 278."""
+    from esperanto_amd.junction import intron_chain_batch
+    from esperanto_amd.reference import DeviceReference
+    from test_oracle_junction import _xorshift_genome
+    genome = _xorshift_genome(5000, 0x517CC1B7)
+    for donor, acceptor in ((b"GT", b"AG"), (b"CT", b"AC")):
+        genome[1060:1062] = donor
+        genome[2998:3000] = acceptor
+        genome[1059] = ord("T")
+        genome[3000] = ord("A")
+        query = b"N" * 11 + bytes(genome[1000:1060]) + bytes(genome[3000:3079])
+        dref = DeviceReference(Reference.from_sequences([bytes(genome)]), buf)
+        try:
+            pools = [[(q + off, r + off, 0, strand) for q, r, span in ((11, 1000, 60), (71, 3000, 79))
+                      for off in range(3, span - 14, 4)] for strand in (0, 1)]
+            rec, exts, stitched = intron_chain_batch(dref, None, [query, query], pools)
+            for k in range(2):
+                assert int(rec[k]["status"]) == 1 and int(rec[k]["genomic_start"]) == 1000
+                assert stitched[k] == [SoftClip(11), Match(60), RefSkip(1940), Match(79)]
+                assert int(rec[k]["score"]) == 278
+        finally:
+            dref.close()

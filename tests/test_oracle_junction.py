@@ -223,3 +223,53 @@ def test_reextend_under_the_matching_edit_dir():
     # spliced placements are left alone; a contig out of range is refused
     assert oracle.reextend_with(o, read, 0, pos, [Match(50).word, RefSkip(100).word, Match(62).word], p, 1)["status"] == 2
     assert oracle.reextend_with(o, read, 3, pos, words, p, 1)["status"] == -1
+
+
+# ---- try_intron_chain_placement ----------------------------------------------------
+
+def test_intron_chain_known_answer():
+    """intron_fast_path_reconstructs_asymmetric_exons_with_technical_clips
+    (crates/map/src/align.rs:5067-5120): pos 1000, [S11, M60, N1940, M79], score 278,
+    for both motif orientations and both anchor strands."""
+    genome = _xorshift_genome(5000, 0x517CC1B7)
+    for donor, acceptor in ((b"GT", b"AG"), (b"CT", b"AC")):
+        genome[1060:1062] = donor
+        genome[2998:3000] = acceptor
+        genome[1059] = ord("T")
+        genome[3000] = ord("A")
+        query = b"N" * 11 + bytes(genome[1000:1060]) + bytes(genome[3000:3079])
+        o = oracle.OracleRef(Reference.from_sequences([bytes(genome)]))
+        for strand in (0, 1):
+            anchors = [(q + off, r + off, 0, strand) for q, r, span in ((11, 1000, 60), (71, 3000, 79))
+                       for off in range(3, span - 14, 4)]
+            h = oracle.intron_chain(o, None, query, anchors)
+            assert h["status"] == 1, h
+            assert h["genomic_start"] == 1000
+            assert h["stitched"] == [SoftClip(11), Match(60), RefSkip(1940), Match(79)]
+            assert h["score"] == 278
+            assert (h["jstart"], h["jend"]) == (1060, 3000)
+
+
+def test_intron_chain_places_planted_junctions():
+    """Balanced two-exon reads land on their planted intron with a read-conserving
+    [.., N(intron), ..] CIGAR; reads with one dominant fragment go to the
+    unspliced paths (reason 3)."""
+    ref, reads, pools, quads = splicegen.intron_chain_workload(5, 240)
+    o = oracle.OracleRef(ref)
+    jl = oracle.JLib(quads)
+    placed = hits = 0
+    for c, (r, p) in enumerate(zip(reads, pools)):
+        h = oracle.intron_chain(o, jl, r, p)
+        kind = c % 6
+        if kind == 5:
+            assert h["status"] == 0 and h["reason"] == 3
+        if h["status"] != 1:
+            continue
+        placed += 1
+        assert sum(op.len for op in h["stitched"] if op.op in (0, 1, 4)) == len(r)
+        assert any(op.op == 3 and op.len == h["jend"] - h["jstart"] for op in h["stitched"])
+        # a substitution near the seam can stop an exon's MEM early; when the
+        # read gap then exceeds 20 the pair's own naive junction is kept
+        hits += h["jstart"] == 2000 + c * 9000 + 700
+    assert placed > 120 and hits >= 0.85 * placed, (placed, hits)
+    jl.close()
