@@ -396,6 +396,51 @@ def _subset_mate(mt, k):
                         mt.mate_pos[:k])
 
 
+def bench_cpu(args, params):
+    """The reference's CPU route restated in C (oracle/esp_oracle.c): run_dispatch
+    sends these pure pairs to the AVX2 i16 anti-diagonal kernel avx2_band
+    (crates/map/src/extend.rs:293-344, :1526-2026), one worker per thread with
+    static chunks as rayon's map_init would give.  Timed on all host threads (up
+    to 16) and on one, plus the scalar run_band restatement on all threads;
+    each leg is a bounded prefix of the same uneven workload."""
+    from oracle import oracle
+    from esperanto_amd.batch import workload_c3
+    threads = min(16, os.cpu_count() or 1)
+    avx = oracle.avx2_available()
+
+    def leg(route, nthreads, seconds):
+        oracle.set_route(route)
+        try:
+            probe = workload_c3(2000)
+            t0 = time.perf_counter()
+            oracle.band_batch(probe, params, 0, nthreads=nthreads)
+            per_pair = (time.perf_counter() - t0) / probe.npairs
+            n = int(min(args.pairs, max(2000, seconds / max(per_pair, 1e-9))))
+            sample = workload_c3(n)
+            cells = band_cells(sample)
+            t0 = time.perf_counter()
+            oracle.band_batch(sample, params, 0, nthreads=nthreads)
+            dt = time.perf_counter() - t0
+        finally:
+            oracle.set_route(0)
+        return cells / dt / 1e9, n, n / dt
+
+    route = 1 if avx else 0
+    g_all, n_all, pps_all = leg(route, threads, args.cpu_seconds)
+    g_one, n_one, pps_one = leg(route, 1, args.cpu_seconds / 3)
+    g_scalar, n_scalar, _ = leg(0, threads, args.cpu_seconds / 3)
+    name = ("C restatement of the reference's AVX2 route (run_dispatch -> avx2_band i16 anti-diagonal kernel "
+            "+ finish_extension)" if avx else "scalar C restatement of run_dispatch/run_band (no AVX2 on this host)")
+    log(f"cpu baseline: {name}: {g_all:.3f} GCUPS on {threads} threads ({n_all} pairs), "
+        f"{g_one:.3f} on 1 thread ({n_one} pairs); scalar port {g_scalar:.3f} on {threads} threads")
+    return {"value": round(g_all, 3), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "sample": f"first {n_all} pairs of the same uneven workload (seed 0xE5A00003), {name}, "
+                      f"{pps_all:.0f} pairs/s",
+            "route": "avx2_band" if avx else "run_band",
+            "single_thread_gcups": round(g_one, 3), "single_thread_sample_pairs": n_one,
+            "scalar_port_gcups": round(g_scalar, 3), "scalar_port_sample_pairs": n_scalar}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -551,25 +596,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle
-        threads = min(16, os.cpu_count() or 1)
-        probe = workload_c3(2000)
-        t0 = time.perf_counter()
-        oracle.band_batch(probe, params, 0, nthreads=threads)
-        dt = time.perf_counter() - t0
-        per_pair = dt / probe.npairs
-        nsample = int(min(args.pairs, max(2000, args.cpu_seconds / max(per_pair, 1e-9))))
-        sample = workload_c3(nsample)
-        scells = band_cells(sample)
-        log(f"cpu baseline: {nsample} pairs on {threads} threads")
-        t0 = time.perf_counter()
-        oracle.band_batch(sample, params, 0, nthreads=threads)
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(scells / dt / 1e9, 3), "unit": "GCUPS", "cores": threads,
-               "kind": "port",
-               "sample": f"first {nsample} pairs of the same uneven workload (seed 0xE5A00003), "
-                         f"scalar C restatement of run_dispatch/run_band/finish_extension, "
-                         f"{nsample / dt:.0f} pairs/s"}
+        cpu = bench_cpu(args, params)
 
     out = {
         "metric": "Gotoh GCUPS + Myers pairs/s, batched vs plain on 1 MI355X; p99 latency",

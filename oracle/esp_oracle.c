@@ -465,13 +465,323 @@ static void run_band(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t 
     free(rows); free(flat); free(rc_codes); free(rf_codes);
 }
 
+/* ------------------------------------------------------------ AVX2 kernels */
+
+#include <immintrin.h>
+
+/* Route of run_dispatch for pure inputs: 0 the scalar run_band<i16> (the
+ * checker's default: the reference requires its AVX2 kernels to equal it),
+ * 1 the AVX2 kernels exactly where the reference takes them (pure, m and n
+ * >= 8, AVX2 present).  Set by tests and the CPU baseline. */
+static int g_route = 0;
+void espo_set_route(int r) { g_route = r; }
+int espo_avx2_available(void) { return __builtin_cpu_supports("avx2"); }
+
+/* Per-thread scratch of the AVX2 kernels, grown on demand (ExtendBuffer's
+ * packed buffers, extend.rs:231). */
+typedef struct {
+    int16_t *h[3], *e[2], *f[2], *rmax, *hl, *hr, *rd, *rr;
+    int32_t *rarg;
+    int64_t *lo, *hi, *alo, *ahi, *base;
+    size_t cap_m, cap_n;
+} avx_scratch;
+static __thread avx_scratch g_avx;
+
+static void avx_reserve(size_t m, size_t n) {
+    avx_scratch *S = &g_avx;
+    if (m > S->cap_m) {
+        size_t c = m + 64;
+        for (int k = 0; k < 3; k++) S->h[k] = (int16_t *)realloc(S->h[k], (c + 33) * 2);
+        for (int k = 0; k < 2; k++) S->e[k] = (int16_t *)realloc(S->e[k], (c + 33) * 2);
+        for (int k = 0; k < 2; k++) S->f[k] = (int16_t *)realloc(S->f[k], (c + 33) * 2);
+        S->rmax = (int16_t *)realloc(S->rmax, (c + 1) * 2);
+        S->hl = (int16_t *)realloc(S->hl, (c + 1) * 2);
+        S->hr = (int16_t *)realloc(S->hr, (c + 1) * 2);
+        S->rd = (int16_t *)realloc(S->rd, (c + 16) * 2);
+        S->rarg = (int32_t *)realloc(S->rarg, (c + 1) * 4);
+        S->lo = (int64_t *)realloc(S->lo, (c + 2) * 8);
+        S->hi = (int64_t *)realloc(S->hi, (c + 2) * 8);
+        S->alo = (int64_t *)realloc(S->alo, (c + 2) * 8);
+        S->ahi = (int64_t *)realloc(S->ahi, (c + 2) * 8);
+        S->base = (int64_t *)realloc(S->base, (c + 2) * 8);
+        S->cap_m = c;
+    }
+    if (n > S->cap_n) {
+        size_t c = n + 64;
+        S->rr = (int16_t *)realloc(S->rr, (c + 16) * 2);
+        S->cap_n = c;
+    }
+}
+
+/* The vector body shared by avx2_full / avx2_band (extend.rs:1352-1430,
+ * :1855-1933): one 16-lane block of anti-diagonal d starting at row v. */
+typedef struct {
+    __m256i neg, zero, ones, one, two, three, four, open, ext, sdiag, smm, sga, stc, cg, ca, cc, ct;
+} avx_consts;
+
+static inline __m256i avx_addc(const avx_consts *K, __m256i x, __m256i c) {
+    const __m256i isn = _mm256_cmpeq_epi16(x, K->neg);
+    const __m256i r = _mm256_max_epi16(_mm256_add_epi16(x, c), K->neg);
+    return _mm256_blendv_epi8(r, K->neg, isn);
+}
+
+__attribute__((target("avx2"))) static inline void avx_block(const avx_consts *K, const int16_t *rd,
+                                                             const int16_t *rr, const int16_t *h2p,
+                                                             const int16_t *h1p, int16_t *h0p, const int16_t *e1p,
+                                                             int16_t *e0p, const int16_t *f1p, int16_t *f0p,
+                                                             int64_t v, int64_t n, int64_t d, int16_t hb[16],
+                                                             int16_t db[16]) {
+    const __m256i rc = _mm256_loadu_si256((const __m256i *)(rd + (v - 1)));
+    const __m256i rfc = _mm256_loadu_si256((const __m256i *)(rr + (v + (n - d))));
+    const __m256i eq = _mm256_cmpeq_epi16(rc, rfc);
+    const __m256i is_match = _mm256_and_si256(eq, _mm256_cmpgt_epi16(K->four, rc));
+    const __m256i ga = _mm256_and_si256(_mm256_cmpeq_epi16(rc, K->cg), _mm256_cmpeq_epi16(rfc, K->ca));
+    const __m256i tc = _mm256_and_si256(_mm256_cmpeq_epi16(rc, K->cc), _mm256_cmpeq_epi16(rfc, K->ct));
+    __m256i sc = _mm256_blendv_epi8(K->smm, K->sga, ga);
+    sc = _mm256_blendv_epi8(sc, K->stc, tc);
+    sc = _mm256_blendv_epi8(sc, K->sdiag, is_match);
+    const __m256i h_up = _mm256_loadu_si256((const __m256i *)(h1p + (v - 1)));
+    const __m256i e_up = _mm256_loadu_si256((const __m256i *)(e1p + (v - 1)));
+    const __m256i e_open = avx_addc(K, h_up, K->open), e_ext = avx_addc(K, e_up, K->ext);
+    const __m256i egt = _mm256_cmpgt_epi16(e_ext, e_open);
+    const __m256i e_val = _mm256_blendv_epi8(e_open, e_ext, egt);
+    const __m256i edir = _mm256_and_si256(egt, K->one);
+    const __m256i h_left = _mm256_loadu_si256((const __m256i *)(h1p + v));
+    const __m256i f_left = _mm256_loadu_si256((const __m256i *)(f1p + v));
+    const __m256i f_open = avx_addc(K, h_left, K->open), f_ext = avx_addc(K, f_left, K->ext);
+    const __m256i fgt = _mm256_cmpgt_epi16(f_ext, f_open);
+    const __m256i f_val = _mm256_blendv_epi8(f_open, f_ext, fgt);
+    const __m256i fdir = _mm256_and_si256(fgt, K->one);
+    const __m256i h_diag = _mm256_loadu_si256((const __m256i *)(h2p + (v - 1)));
+    const __m256i diag = avx_addc(K, h_diag, sc);
+    __m256i hv = K->zero, hd = K->three;
+    const __m256i fmask = _mm256_andnot_si256(
+        _mm256_or_si256(_mm256_cmpeq_epi16(f_val, K->neg), _mm256_cmpgt_epi16(hv, f_val)), K->ones);
+    hv = _mm256_blendv_epi8(hv, f_val, fmask);
+    hd = _mm256_blendv_epi8(hd, K->two, fmask);
+    const __m256i emask = _mm256_andnot_si256(
+        _mm256_or_si256(_mm256_cmpeq_epi16(e_val, K->neg), _mm256_cmpgt_epi16(hv, e_val)), K->ones);
+    hv = _mm256_blendv_epi8(hv, e_val, emask);
+    hd = _mm256_blendv_epi8(hd, K->one, emask);
+    const __m256i dmask = _mm256_andnot_si256(
+        _mm256_or_si256(_mm256_cmpeq_epi16(diag, K->neg), _mm256_cmpgt_epi16(hv, diag)), K->ones);
+    hv = _mm256_blendv_epi8(hv, diag, dmask);
+    hd = _mm256_blendv_epi8(hd, K->zero, dmask);
+    hd = _mm256_blendv_epi8(hd, K->three, _mm256_cmpeq_epi16(hv, K->zero));
+    const __m256i dir16 =
+        _mm256_or_si256(hd, _mm256_or_si256(_mm256_slli_epi16(edir, 2), _mm256_slli_epi16(fdir, 4)));
+    _mm256_storeu_si256((__m256i *)(h0p + v), hv);
+    _mm256_storeu_si256((__m256i *)(e0p + v), e_val);
+    _mm256_storeu_si256((__m256i *)(f0p + v), f_val);
+    _mm256_storeu_si256((__m256i *)hb, hv);
+    _mm256_storeu_si256((__m256i *)db, dir16);
+}
+
+__attribute__((target("avx2"))) static void avx_consts_init(avx_consts *K, int16_t oe, int16_t ex, int16_t sd,
+                                                            int16_t sm, int16_t sg, int16_t st) {
+    K->neg = _mm256_set1_epi16(NEG16);
+    K->zero = _mm256_set1_epi16(0);
+    K->ones = _mm256_set1_epi16(-1);
+    K->one = _mm256_set1_epi16(1);
+    K->two = _mm256_set1_epi16(2);
+    K->three = _mm256_set1_epi16(3);
+    K->four = _mm256_set1_epi16(4);
+    K->open = _mm256_set1_epi16(oe);
+    K->ext = _mm256_set1_epi16(ex);
+    K->sdiag = _mm256_set1_epi16(sd);
+    K->smm = _mm256_set1_epi16(sm);
+    K->sga = _mm256_set1_epi16(sg);
+    K->stc = _mm256_set1_epi16(st);
+    K->cg = _mm256_set1_epi16(2);
+    K->ca = _mm256_set1_epi16(0);
+    K->cc = _mm256_set1_epi16(1);
+    K->ct = _mm256_set1_epi16(3);
+}
+
+/* LUT constants of both kernels and their scalar fallback (extend.rs:1158-1168):
+ * returns 0 when one of them is the i16 sentinel. */
+static int avx_lut(const core_params *core, int16_t *sd, int16_t *sm, int16_t *sg, int16_t *st, int16_t *oe,
+                   int16_t *ex) {
+    *sd = core->lut[0];
+    *sm = core->lut[1];
+    *sg = core->lut[10];
+    *st = core->lut[8];
+    *oe = (int16_t)core->open_ext;
+    *ex = (int16_t)core->ext;
+    return !(*sd == NEG16 || *sm == NEG16 || *sg == NEG16 || *st == NEG16 || *oe == NEG16 || *ex == NEG16);
+}
+
+/* avx2_full::run + dp_diagonals: extend.rs:1118-1488. */
+__attribute__((target("avx2"))) static void avx2_full_run(const uint8_t *read, int64_t m, const uint8_t *rf,
+                                                          int64_t n, const core_params *core, espo_ext *out,
+                                                          opvec *cig, espo_touch *touch, int64_t *best_cell) {
+    int16_t sd, sm, sg, st, oe, ex;
+    if (!avx_lut(core, &sd, &sm, &sg, &st, &oe, &ex)) {
+        run_band(read, m, rf, n, core, 1, 0, 0, 0, out, cig, touch, best_cell);
+        return;
+    }
+    memset(touch, 0, sizeof(*touch));
+    avx_reserve((size_t)m, (size_t)n);
+    avx_scratch *S = &g_avx;
+    int *rc_codes = (int *)malloc((size_t)m * sizeof(int)), *rf_codes = (int *)malloc((size_t)n * sizeof(int));
+    for (int64_t k = 0; k < m; k++) rc_codes[k] = base_code(read[k]);
+    for (int64_t k = 0; k < n; k++) rf_codes[k] = base_code(rf[k]);
+    const size_t flat_n = (size_t)(n + 1 + m * n);
+    uint8_t *flat = (uint8_t *)calloc(flat_n, 1);
+    memset(flat, H_START, (size_t)n + 1);
+    row_trace *rows = (row_trace *)malloc((size_t)(m + 1) * sizeof(row_trace));
+    rows[0].lo = 0; rows[0].off = 0; rows[0].len = (size_t)n + 1;
+    for (int64_t i = 1; i <= m; i++) {
+        rows[i].lo = 1;
+        rows[i].off = (size_t)((n + 1) + (i - 1) * n);
+        rows[i].len = (size_t)n;
+    }
+    const size_t cap = (size_t)m + 33;
+    for (int k = 0; k < 3; k++) memset(S->h[k], 0, cap * 2);
+    for (int k = 0; k < 2; k++) { memset(S->e[k], 0, cap * 2); memset(S->f[k], 0, cap * 2); }
+    for (int64_t i = 0; i <= m; i++) { S->rmax[i] = NEG16; S->rarg[i] = 1; S->hl[i] = NEG16; }
+    for (int64_t i = 0; i < m + 16; i++) S->rd[i] = i < m ? (int16_t)rc_codes[i] : 0;
+    for (int64_t q = 0; q < n + 16; q++) S->rr[q] = q < n ? (int16_t)rf_codes[n - 1 - q] : 0;
+    avx_consts K;
+    avx_consts_init(&K, oe, ex, sd, sm, sg, st);
+    int h2 = 0, h1 = 1, h0 = 2, e1 = 0, e0 = 1;
+    int16_t hb[16], db[16];
+    for (int64_t d = 0; d <= m + n; d++) {
+        const int64_t ilo = d - n > 1 ? d - n : 1, ihi = m < d - 1 ? m : d - 1;
+        for (int64_t v = ilo; v <= ihi; v += 16) {
+            avx_block(&K, S->rd, S->rr, S->h[h2], S->h[h1], S->h[h0], S->e[e1], S->e[e0], S->f[e1], S->f[e0], v,
+                      n, d, hb, db);
+            for (int k = 0; k < 16; k++) {
+                const int64_t i = v + k;
+                if (i > ihi) break;
+                flat[(size_t)(i * (n - 1) + d)] = (uint8_t)db[k];
+                if (hb[k] > S->rmax[i]) { S->rmax[i] = hb[k]; S->rarg[i] = (int32_t)(d - i); }
+                if (d - i == 1) S->hl[i] = hb[k];
+            }
+        }
+        if (d <= n) { S->h[h0][0] = 0; S->e[e0][0] = NEG16; S->f[e0][0] = NEG16; }
+        if (d <= m) { S->h[h0][d] = 0; S->e[e0][d] = NEG16; S->f[e0][d] = NEG16; }
+        const int t = h2; h2 = h1; h1 = h0; h0 = t;
+        const int t2 = e1; e1 = e0; e0 = t2;
+    }
+    int16_t best = 0;
+    int64_t bi = 0, bj = 0;
+    for (int64_t i = 1; i <= m; i++) {
+        const int16_t rm = S->rmax[i];
+        if (!touch->touched && S->hl[i] == rm) { touch->touched = 1; touch->row = i; touch->left = 1; touch->clamped = 1; }
+        if (rm > best) { best = rm; bi = i; bj = S->rarg[i]; }
+    }
+    if (best_cell) { best_cell[0] = bi; best_cell[1] = bj; }
+    finish_extension(m, core, rows, flat, rc_codes, rf_codes, n, bi, bj, out, cig);
+    free(rows); free(flat); free(rc_codes); free(rf_codes);
+}
+
+/* avx2_band::run + dp_diagonals: extend.rs:1538-2026. */
+__attribute__((target("avx2"))) static void avx2_band_run(const uint8_t *read, int64_t m, const uint8_t *rf,
+                                                          int64_t n, const core_params *core, int64_t offset,
+                                                          int64_t half, espo_ext *out, opvec *cig,
+                                                          espo_touch *touch, int64_t *best_cell) {
+    if (half < 0) half = 0;
+    int16_t sd, sm, sg, st, oe, ex;
+    if (!avx_lut(core, &sd, &sm, &sg, &st, &oe, &ex)) {
+        run_band(read, m, rf, n, core, 1, 1, offset, half, out, cig, touch, best_cell);
+        return;
+    }
+    memset(touch, 0, sizeof(*touch));
+    avx_reserve((size_t)m, (size_t)n);
+    avx_scratch *S = &g_avx;
+    int *rc_codes = (int *)malloc((size_t)m * sizeof(int)), *rf_codes = (int *)malloc((size_t)n * sizeof(int));
+    for (int64_t k = 0; k < m; k++) rc_codes[k] = base_code(read[k]);
+    for (int64_t k = 0; k < n; k++) rf_codes[k] = base_code(rf[k]);
+    row_trace *rows = (row_trace *)malloc((size_t)(m + 1) * sizeof(row_trace));
+    const int64_t lo0 = offset - half > 0 ? offset - half : 0, hi0 = offset + half < n ? offset + half : n;
+    const size_t len0 = lo0 <= hi0 ? (size_t)(hi0 - lo0 + 1) : 0;
+    rows[0].lo = lo0; rows[0].off = 0; rows[0].len = len0;
+    size_t off_acc = len0;
+    for (int64_t i = 1; i <= m; i++) {
+        int64_t c = offset + i;
+        c = c < 1 ? 1 : (c > n ? n : c);
+        const int64_t lo = c - half > 1 ? c - half : 1, hi = c + half < n ? c + half : n;
+        const size_t len = (size_t)(hi - lo + 1);
+        S->lo[i] = lo; S->hi[i] = hi; S->alo[i] = lo + i; S->ahi[i] = hi + i;
+        S->base[i] = (int64_t)off_acc - lo;
+        rows[i].lo = lo; rows[i].off = off_acc; rows[i].len = len;
+        off_acc += len;
+    }
+    uint8_t *flat = (uint8_t *)calloc(off_acc ? off_acc : 1, 1);
+    memset(flat, H_START, len0);
+    const int64_t dmin = S->alo[1], dmax = S->ahi[m];
+    const size_t cap = (size_t)m + 33;
+    for (int k = 0; k < 3; k++) memset(S->h[k], 0, cap * 2);
+    for (int k = 0; k < 2; k++)
+        for (size_t q = 0; q < cap; q++) { S->e[k][q] = NEG16; S->f[k][q] = NEG16; }
+    for (int64_t i = 0; i <= m; i++) { S->rmax[i] = NEG16; S->rarg[i] = 1; S->hl[i] = NEG16; S->hr[i] = NEG16; }
+    for (int64_t i = 0; i < m + 16; i++) S->rd[i] = i < m ? (int16_t)rc_codes[i] : 0;
+    for (int64_t q = 0; q < n + 16; q++) S->rr[q] = q < n ? (int16_t)rf_codes[n - 1 - q] : 0;
+    avx_consts K;
+    avx_consts_init(&K, oe, ex, sd, sm, sg, st);
+    int h2 = 0, h1 = 1, h0 = 2, e1 = 0, e0 = 1;
+    int64_t blo_c = 1, bhi_c = 1, me_r = 1, mf_r = 1, md_r = 1;
+    int16_t hb[16], db[16];
+    for (int64_t d = dmin; d <= dmax; d++) {
+        while (blo_c <= m && S->ahi[blo_c] < d) blo_c++;
+        while (bhi_c < m && S->alo[bhi_c + 1] <= d) bhi_c++;
+        const int64_t blo = blo_c, bhi = bhi_c;
+        for (int64_t v = blo; v <= bhi; v += 16) {
+            avx_block(&K, S->rd, S->rr, S->h[h2], S->h[h1], S->h[h0], S->e[e1], S->e[e0], S->f[e1], S->f[e0], v,
+                      n, d, hb, db);
+            for (int k = 0; k < 16; k++) {
+                const int64_t i = v + k;
+                if (i > bhi) break;
+                const int64_t j = d - i;
+                flat[(size_t)(S->base[i] + j)] = (uint8_t)db[k];
+                if (hb[k] > S->rmax[i]) { S->rmax[i] = hb[k]; S->rarg[i] = (int32_t)j; }
+                if (j == S->lo[i]) S->hl[i] = hb[k];
+                if (j == S->hi[i]) S->hr[i] = hb[k];
+            }
+        }
+        while (me_r <= m && S->ahi[me_r] < d + 1) me_r++;
+        if (me_r <= m && S->ahi[me_r] == d + 1 && me_r >= 2 && S->hi[me_r] == S->hi[me_r - 1] + 1) {
+            S->h[h0][me_r - 1] = 0; S->e[e0][me_r - 1] = NEG16; S->f[e0][me_r - 1] = NEG16;
+        }
+        while (mf_r <= m && S->alo[mf_r] < d + 1) mf_r++;
+        if (mf_r <= m && S->alo[mf_r] == d + 1) {
+            S->h[h0][mf_r] = 0; S->e[e0][mf_r] = NEG16; S->f[e0][mf_r] = NEG16;
+        }
+        while (md_r <= m && S->alo[md_r] < d + 2) md_r++;
+        if (md_r <= m && S->alo[md_r] == d + 2 && md_r >= 2 && S->lo[md_r] == S->lo[md_r - 1]) {
+            S->h[h0][md_r - 1] = 0; S->e[e0][md_r - 1] = NEG16; S->f[e0][md_r - 1] = NEG16;
+        }
+        const int t = h2; h2 = h1; h1 = h0; h0 = t;
+        const int t2 = e1; e1 = e0; e0 = t2;
+    }
+    int16_t best = 0;
+    int64_t bi = 0, bj = 0;
+    for (int64_t i = 1; i <= m; i++) {
+        const int16_t rm = S->rmax[i];
+        if (!touch->touched) {
+            if (S->lo[i] > 0 && S->hl[i] == rm) {
+                touch->touched = 1; touch->row = i; touch->left = 1; touch->clamped = S->lo[i] <= 1;
+            } else if (S->hi[i] < n && S->hr[i] == rm) {
+                touch->touched = 1; touch->row = i; touch->left = 0; touch->clamped = S->hi[i] >= n;
+            }
+        }
+        if (rm > best) { best = rm; bi = i; bj = S->rarg[i]; }
+    }
+    if (best_cell) { best_cell[0] = bi; best_cell[1] = bj; }
+    finish_extension(m, core, rows, flat, rc_codes, rf_codes, n, bi, bj, out, cig);
+    free(rows); free(flat); free(rc_codes); free(rf_codes);
+}
+
 /* ---------------------------------------------------------------- API */
 
 static int pure_byte(uint8_t b) { return b == 'A' || b == 'C' || b == 'G' || b == 'T' || b == 'N'; }
 
 /* run_dispatch: extend.rs:293-344.  The AVX2 branches are cell-for-cell the
  * i16 scalar kernel by the reference's own contract, so pure inputs run the
- * i16 restatement and everything else the i32 one. */
+ * i16 restatement and everything else the i32 one; with espo_set_route(1)
+ * pure inputs of m, n >= 8 take the AVX2 restatements instead, as the
+ * reference does on an AVX2 host. */
 static void run_dispatch(const uint8_t *read, int64_t m, const uint8_t *rf, int64_t n,
                          const espo_params *p, int has_band, int64_t offset, int64_t half,
                          espo_ext *out, opvec *cig, espo_touch *touch) {
@@ -480,6 +790,11 @@ static void run_dispatch(const uint8_t *read, int64_t m, const uint8_t *rf, int6
     int pure = 1;
     for (int64_t k = 0; k < m && pure; k++) pure = pure_byte(read[k]);
     for (int64_t k = 0; k < n && pure; k++) pure = pure_byte(rf[k]);
+    if (g_route == 1 && pure && m >= 8 && n >= 8 && espo_avx2_available()) {
+        if (has_band) avx2_band_run(read, m, rf, n, &core, offset, half, out, cig, touch, NULL);
+        else avx2_full_run(read, m, rf, n, &core, out, cig, touch, NULL);
+        return;
+    }
     run_band(read, m, rf, n, &core, pure, has_band, offset, half, out, cig, touch, NULL);
 }
 

@@ -819,3 +819,18 @@ def intron_chain(oref: OracleRef, jlib: Optional[JLib], read: bytes, anchors: Se
     out["ext"] = Extension.from_c(e, cig[: n.value]) if rec.left_start or rec.status else None
     out["stitched"] = cigar_from_words(st[: rec.n_stitched])
     return out
+
+
+def set_route(route: int) -> None:
+    """0: pure inputs run the scalar run_band<i16> restatement; 1: the AVX2
+    restatements where run_dispatch takes them (extend.rs:293-344)."""
+    L = lib()
+    L.espo_set_route.argtypes = [ctypes.c_int]
+    L.espo_set_route.restype = None
+    L.espo_set_route(int(route))
+
+
+def avx2_available() -> bool:
+    L = lib()
+    L.espo_avx2_available.restype = ctypes.c_int
+    return bool(L.espo_avx2_available())
