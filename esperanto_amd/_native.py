@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libe
 ESP_OK, ESP_E_INVAL, ESP_E_CAP, ESP_E_DEVICE, ESP_E_NOMEM, ESP_E_BUSY = range(6)
 ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET, ESP_OPT_BAND_GRID, ESP_OPT_FULL_PRUNE = 1, 2, 3, 4
 ESP_OPT_FULL_PRUNE_SHAPE = 5
+ESP_OPT_PLAIN_KERNEL = 6
 _STATUS = {1: "invalid argument", 2: "capacity", 3: "device error", 4: "out of memory",
            5: "busy"}
 

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libesp_align.so")
 SOURCES = ["esp_kernels.hip", "esp_band.hip", "esp_full.hip", "esp_myers.hip", "esp_window.hip",
-           "esp_walk.hip", "esp_host.cpp", "esp_window.cpp", "esp_walk.cpp", "esp_server.cpp",
+           "esp_walk.hip", "esp_wave.hip", "esp_host.cpp", "esp_window.cpp", "esp_walk.cpp", "esp_server.cpp",
            "esp_graph.cpp", "esp_noalloc.hip", "esp_junction.hip", "esp_junction.cpp",
            "esp_synth.cpp"]
 ARCH = os.environ.get("ESP_OFFLOAD_ARCH", "gfx950")

@@ -77,6 +77,7 @@ struct esp_ctx {
     size_t full_trace_budget = esph::kFullTraceBudget;
     int full_kernel = 0;  // extend_hint reruns: 0 by batch size, 1 lane per pair
     int full_prune = 1;   // extend_hint reruns: pruned pass first (1) or the full matrix only (0)
+    int plain_kernel = 0;  // single-pair calls: 0 one wave per pair (windows up to 4096), 1 one lane
     bool last_pruned = false;  // the last batch ran the pruned pass (its counts are in escbuf)
     // stripe half-width and head pad of the two pruned passes
     int prune_half_a = espk::ESC_HALF_A, prune_pad_a = espk::ESC_PAD_A;

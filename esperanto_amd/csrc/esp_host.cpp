@@ -108,16 +108,30 @@ int run_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, si
     a.cigar = (uint32_t *)(dout + O.o_cig);
     HIP_TRY(c, c->counter.ensure(256));
     a.counter = (uint32_t *)c->counter.p;
-    const uint64_t maxw = espk::strip_trace_words(m, n);
-    HIP_TRY(c, c->rows.ensure(5 * std::max<size_t>(m, 1) * 64 * 4));
-    HIP_TRY(c, c->trace.ensure(std::max<uint64_t>(maxw, 1) * 64 * 4));
-    a.rows = (int32_t *)c->rows.p;
-    a.trace = (uint32_t *)c->trace.p;
-    a.maxm = (uint32_t)std::max<size_t>(m, 1);
-    a.maxw = (uint32_t)std::max<uint64_t>(maxw, 1);
-    a.nl = 64;
     HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
-    HIP_TRY(c, espk::launch_gotoh_strip(a, 1, c->stream));
+    if (c->plain_kernel == 0 && n <= (size_t)espk::WAVE_NMAX) {
+        // one wave sweeps the pair; its trace stays in LDS when small enough
+        const uint64_t need = espk::wave_trace_words(m, n, mode, half);
+        const uint64_t maxw = need > espk::WAVE_TR_LDS ? need : 1;
+        HIP_TRY(c, c->trace.ensure(maxw * 4));
+        a.trace = (uint32_t *)c->trace.p;
+        a.maxw = (uint32_t)maxw;
+        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+        HIP_TRY(c, espk::launch_gotoh_wave(a, 1, c->stream));
+        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    } else {
+        const uint64_t maxw = espk::strip_trace_words(m, n);
+        HIP_TRY(c, c->rows.ensure(5 * std::max<size_t>(m, 1) * 64 * 4));
+        HIP_TRY(c, c->trace.ensure(std::max<uint64_t>(maxw, 1) * 64 * 4));
+        a.rows = (int32_t *)c->rows.p;
+        a.trace = (uint32_t *)c->trace.p;
+        a.maxm = (uint32_t)std::max<size_t>(m, 1);
+        a.maxw = (uint32_t)std::max<uint64_t>(maxw, 1);
+        a.nl = 64;
+        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+        HIP_TRY(c, espk::launch_gotoh_strip(a, 1, c->stream));
+        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    }
     HIP_TRY(c, hipMemcpyAsync(c->hout.p, c->stage_out.p, O.total, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint8_t *ho = (const uint8_t *)c->hout.p;
@@ -480,6 +494,10 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
         // the wave-per-pair kernel is not built.
         if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full kernel choice not in {0,1}");
         c->full_kernel = (int)value;
+        return ESP_OK;
+    case ESP_OPT_PLAIN_KERNEL:
+        if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "plain kernel choice not in {0,1}");
+        c->plain_kernel = (int)value;
         return ESP_OK;
     case ESP_OPT_FULL_PRUNE:
         if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full-matrix pruning not in {0,1}");

@@ -163,6 +163,23 @@ __host__ __device__ inline uint64_t strip_trace_words(uint64_t m, uint64_t n) {
     return ((n + STRIP_T - 1) / STRIP_T) * m * (STRIP_T / 8);
 }
 
+// One pair per wave (esp_wave.hip): the longest window it takes, and the
+// trace words it keeps in LDS (more go to the wave's global scratch).
+constexpr int WAVE_NMAX = 4096;
+constexpr uint32_t WAVE_TR_LDS = 4096;
+// Trace words of one pair in the wave kernel: rows times the widest row
+// window in words; FULL and HINT size for the full matrix (an extend_hint
+// band attempt may escalate), BAND for its clamped half-width.
+__host__ __device__ inline uint64_t wave_trace_words(uint64_t m, uint64_t n, int mode, int64_t half) {
+    uint64_t w = n;
+    if (mode == ESP_MODE_BAND) {
+        const int64_t h = half < 0 ? 0 : (half > (int64_t)n ? (int64_t)n : half);
+        w = (uint64_t)(2 * h + 1) < n ? (uint64_t)(2 * h + 1) : n;
+    }
+    return m * ((w + 7) / 8);
+}
+hipError_t launch_gotoh_wave(const GotohArgs &a, int blocks, hipStream_t s);
+
 hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s);
 
 // Diagonal-frame band kernel for interior bands (every row window inside

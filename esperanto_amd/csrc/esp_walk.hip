@@ -279,8 +279,25 @@ __device__ __forceinline__ void meta_load(const WalkArgs &a, uint32_t idx, Meta 
 
 // Columns per tile of the walk loop, and the tile's query and reference words.
 constexpr uint32_t WT = 64;
-__device__ __forceinline__ void tile_load(const Lane &L, uint32_t (&q)[WT / 4 + 1], uint32_t (&r)[WT / 16 + 1]) {
-    if (L.qlen) {
+// Dwords of LDS per wave for its round's queries: 64 queries of up to 150
+// bytes, and three blocks of four waves still fit a CU's 160 KB.
+constexpr uint32_t STAGE_DW = 2432;
+// sbase: the (wave-uniform) address the wave's LDS copy of its round's
+// queries starts at, 0 when the round's queries were not copied.
+__device__ __forceinline__ void tile_load(const Lane &L, const uint32_t *stage, uintptr_t sbase,
+                                          uint32_t (&q)[WT / 4 + 1], uint32_t (&r)[WT / 16 + 1]) {
+    if (L.qlen && sbase) {
+        // from the wave's LDS copy (every dword up to qjmax is there)
+        const uint32_t j0 = (L.qoff0 + L.ro) >> 2;
+        const uint32_t *sp = stage + (uint32_t)(((uintptr_t)L.qw - sbase) >> 2);
+        if (j0 + WT / 4 <= L.qjmax) {
+#pragma unroll
+            for (uint32_t j = 0; j <= WT / 4; j++) q[j] = sp[j0 + j];
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j <= WT / 4; j++) q[j] = sp[min(j0 + j, L.qjmax)];
+        }
+    } else if (L.qlen) {
         const uint32_t j0 = (L.qoff0 + L.ro) >> 2;
         if (j0 + WT / 4 <= L.qjmax) {
 #pragma unroll
@@ -457,11 +474,12 @@ __device__ __forceinline__ void seg_score(const WalkArgs &a, Lane &L, const uint
     L.run = run;
 }
 
-__global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_walk(WalkArgs a) {
     __shared__ uint32_t lut[256];
     __shared__ uint32_t s_up[PEN_UP];
     __shared__ uint32_t s_down[PEN_DOWN];
     __shared__ Tally T;
+    __shared__ __align__(16) uint32_t s_stage[4][STAGE_DW];
     {
         // packed byte (four bases, first in bits 7:6) -> four ASCII bytes
         const uint32_t b = threadIdx.x;
@@ -475,52 +493,103 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
 
     // Rounds: every lane of a wave begins its next placement together, with
     // that placement's fields loaded during the previous one, so a wave pays
-    // each placement's set-up latency once rather than once per lane.
+    // each placement's set-up latency once rather than once per lane.  The
+    // wave's 64 placements are consecutive, and so are their queries in a
+    // batch laid out in placement order: when the round's queries span at most
+    // STAGE_DW dwords the wave copies them into LDS with coalesced loads first,
+    // so each query byte leaves HBM once instead of once per tile.
     const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t *const stage = s_stage[wave];
     Meta m;
-    if (idx < a.n) {
-        meta_load(a, idx, m);
-        meta_load2(a, m);
+    {
+        const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+        if (idx < a.n) {
+            meta_load(a, idx, m);
+            meta_load2(a, m);
+        }
     }
-    for (; idx < a.n; idx += stride) {
+    for (uint32_t first = blockIdx.x * blockDim.x + 64 * wave; first < a.n; first += stride) {
+        const uint32_t idx = first + lane;
+        const bool act = idx < a.n;
         const Meta cur = m;
-        if (idx + stride < a.n) meta_load(a, idx + stride, m);
-        if (cur.contig >= a.R.ncontig) {
+        if (act && idx + stride < a.n) meta_load(a, idx + stride, m);
+        const bool walk = act && cur.contig < a.R.ncontig;
+        uint64_t lo = walk && cur.read_len ? cur.read_off : ~0ull;
+        uint64_t hi = walk && cur.read_len ? cur.read_off + cur.read_len : 0ull;
+#pragma unroll
+        for (int o = 32; o; o >>= 1) {
+            lo = min(lo, (uint64_t)__shfl_xor((unsigned long long)lo, o, 64));
+            hi = max(hi, (uint64_t)__shfl_xor((unsigned long long)hi, o, 64));
+        }
+        // every lane holds the same span now; keep it in scalar registers
+        lo = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)lo) |
+             (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(lo >> 32)) << 32;
+        hi = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)hi) |
+             (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(hi >> 32)) << 32;
+        uintptr_t sbase = 0;
+        if (hi > lo) {
+            const uintptr_t g0 = (uintptr_t)(a.reads + lo) & ~(uintptr_t)3;
+            const uintptr_t g1 = ((uintptr_t)(a.reads + hi) + 3) & ~(uintptr_t)3;
+            const uint32_t ndw = (uint32_t)min((uint64_t)(g1 - g0) >> 2, (uint64_t)STAGE_DW + 1);
+            if (ndw <= STAGE_DW) {
+                // every dword copied lies in [g0, g1): the dwords holding the
+                // round's query bytes, which the global path reads too
+                const uint32_t *src = (const uint32_t *)g0;
+                for (uint32_t d = 4 * lane; d < ndw; d += 256) {
+                    if (d + 4 <= ndw) {
+                        const u32x4 v = *(const u32x4 *)(src + d);
+                        stage[d] = v.x, stage[d + 1] = v.y, stage[d + 2] = v.z, stage[d + 3] = v.w;
+                    } else {
+                        for (uint32_t k = d; k < ndw; k++) stage[k] = src[k];
+                    }
+                }
+                sbase = g0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        if (act && !walk) {
             // a placement on a contig out of range gets zeroed records
             if (a.strict) a.strict[idx] = esp_strict_score{};
             if (a.counts) a.counts[idx] = esp_walk_counts{};
             if (a.status) a.status[idx] = ESP_E_INVAL;
-            continue;
         }
-        Lane L;
-        place_begin(a, L, idx, cur, s_up, s_down, T);
-        bool second = idx + stride < a.n;  // the next placement's contig fields are due
-        while (L.rem) {
-            if (second) {
-                meta_load2(a, m);
-                second = false;
-            }
-            // one tile: up to WT columns of the M op, its words loaded at once
-            uint32_t tq[WT / 4 + 1], tr[WT / 16 + 1];
-            tile_load(L, tq, tr);
-            const uint32_t tcol = min(L.rem, WT), qa = L.ro, p = L.rf;
-            const bool fresh = L.fresh;
-            L.fresh = false;
-            L.rem -= tcol;
-            L.ro += tcol;
-            L.rf += tcol;
-            if (L.rem == 0) advance(a, L, s_up, s_down, T);
+        if (walk) {
+            Lane L;
+            place_begin(a, L, idx, cur, s_up, s_down, T);
+            bool second = idx + stride < a.n;  // the next placement's contig fields are due
+            while (L.rem) {
+                if (second) {
+                    meta_load2(a, m);
+                    second = false;
+                }
+                // one tile: up to WT columns of the M op, its words loaded at once
+                uint32_t tq[WT / 4 + 1], tr[WT / 16 + 1];
+                tile_load(L, stage, sbase, tq, tr);
+                const uint32_t tcol = min(L.rem, WT), qa = L.ro, p = L.rf;
+                const bool fresh = L.fresh;
+                L.fresh = false;
+                L.rem -= tcol;
+                L.ro += tcol;
+                L.rf += tcol;
+                if (L.rem == 0) advance(a, L, s_up, s_down, T);
 #pragma unroll
-            for (uint32_t sg = 0; sg < WT / SEG; sg++) {
-                const uint32_t c0 = sg * SEG;
-                if (c0 < tcol)
-                    seg_score(a, L, lut, tq + sg * (SEG / 4), tr + sg * (SEG / 16), qa + c0, p + c0,
-                              min(tcol - c0, SEG), fresh && sg == 0);
+                for (uint32_t sg = 0; sg < WT / SEG; sg++) {
+                    const uint32_t c0 = sg * SEG;
+                    if (c0 < tcol)
+                        seg_score(a, L, lut, tq + sg * (SEG / 4), tr + sg * (SEG / 16), qa + c0, p + c0,
+                                  min(tcol - c0, SEG), fresh && sg == 0);
+                }
             }
+            if (second) meta_load2(a, m);
+            place_end(a, L, T);
         }
-        if (second) meta_load2(a, m);
-        place_end(a, L, T);
+        // the next round's copy overwrites this one only after every lane's
+        // reads of it (same-wave LDS operations complete in order)
+        __builtin_amdgcn_wave_barrier();
     }
 }
 

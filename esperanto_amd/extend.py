@@ -7,7 +7,9 @@ Same names, argument order and results as the reference:
       -> (Extension, TouchRec), band_spec = (offset, num, den, half) | None
 ExtendBuffer (extend.rs:231) is a device context: one HIP stream plus
 scratch; like the reference's buffer it has no semantic effect.  Every call
-here is the plain form: one kernel launch per call.
+here is the plain form: one kernel launch per call, one wavefront sweeping
+the pair (windows up to 4096 bases; longer ones, or ESP_OPT_PLAIN_KERNEL = 1,
+take the lane-per-pair kernel).
 """
 from __future__ import annotations
 
@@ -37,8 +39,15 @@ class ExtendBuffer:
 
     def set_option(self, option: int, value: int) -> None:
         """esp_ctx_set_option (ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET,
-        ESP_OPT_BAND_GRID, ESP_OPT_FULL_PRUNE)."""
+        ESP_OPT_BAND_GRID, ESP_OPT_FULL_PRUNE, ESP_OPT_FULL_PRUNE_SHAPE,
+        ESP_OPT_PLAIN_KERNEL)."""
         _native.check(_native.lib().esp_ctx_set_option(self._h, option, value), self._h)
+
+    def last_kernel_ms(self) -> float:
+        """esp_last_kernel_ms: device time of the last call's kernel (or batch)."""
+        ms = ctypes.c_double()
+        _native.check(_native.lib().esp_last_kernel_ms(self._h, ctypes.byref(ms)), self._h)
+        return ms.value
 
     def close(self) -> None:
         if self._h:

@@ -239,9 +239,14 @@ int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
  * leaves every pair it cannot prove to the unpruned kernel; results are the
  * same either way.  ESP_OPT_FULL_PRUNE_SHAPE sets the stripe half-width and
  * head pad (rows) of the two pruned passes: value = halfA | padA << 8 |
- * halfB << 16 | padB << 24, each 1..255 (0 restores 32, 16, 64, 16). */
+ * halfB << 16 | padB << 24, each 1..255 (0 restores 32, 16, 64, 16).
+ * ESP_OPT_PLAIN_KERNEL picks the kernel of the single-pair calls (esp_band,
+ * esp_extend, esp_extend_hint): 0 (default) one wavefront per pair, its 64
+ * lanes sweeping 64 rows at once, for windows up to 4096 bases (longer ones
+ * take the lane kernel); 1 the lane-per-pair strip kernel.  Results are the
+ * same either way; esp_last_kernel_ms reads the call's kernel time. */
 enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2, ESP_OPT_BAND_GRID = 3, ESP_OPT_FULL_PRUNE = 4,
-       ESP_OPT_FULL_PRUNE_SHAPE = 5 };
+       ESP_OPT_FULL_PRUNE_SHAPE = 5, ESP_OPT_PLAIN_KERNEL = 6 };
 /* ESP_OPT_BAND_GRID sets one band half's persistent grid as esp_autotune_band
  * would: value = half << 16 | blocks per CU (1 .. the half's occupancy
  * limit; 0 restores the limit). */
