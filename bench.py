@@ -32,6 +32,9 @@ sys.path.insert(0, ROOT)
 
 # MI355X integer VALU peak: 256 CUs x 64 int32 lane-ops/clk x 2.4 GHz.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# One wave64 issues at most one VALU instruction per 4 cycles on its SIMD
+# (16 lanes a cycle): 1/1024 of the chip, the ceiling of a one-wave call.
+ONE_WAVE_PEAK_TOPS = 16 * 2.4e9 / 1e12
 GOTOH_OPS_PER_CELL = 18      # E 3, F 3, subst 1, diag 1, H 3, dirs 2, hdir 3, pack 1, best 1
 MYERS_OPS_PER_WORD_COL = 31  # per 64-bit word and text column, +3 per column
 
@@ -128,14 +131,30 @@ def bench_window(args, buf, params):
         out[name]["escalated_to_full"] = ps["escalated"]
         out[name]["escalated_pruned"] = ps["full_pruned"]
         out[name]["band_attempts"] = ps["band_pairs"][30]
-        # every full-matrix stage (sort, pruned passes, unpruned pass) against
-        # the cells of the full matrices they stand for
+        # Full-matrix reruns: one more (untimed) launch with the cell census
+        # on gives the cells the passes really computed; their rate over the
+        # timed full-matrix stages (sort, pruned passes, unpruned pass) is the
+        # stage's GCUPS and roofline fraction.
         esc_ms = sum(stages.get(k, 0.0) for k in ("escalation_sort", "escalated_full",
                                                     "escalated_pruned_b", "escalated_unpruned"))
         if esc_ms and ps["escalated"]:
-            wlen = 600 if name == "spliced" else 1950
-            cells = ps["escalated"] * 150 * wlen
-            out[name]["full_equiv_gcups_est"] = round(cells / (esc_ms / 1e3) / 1e9, 1)
+            from esperanto_amd import _native
+            buf.set_option(_native.ESP_OPT_FULL_CENSUS, 1)
+            db.launch(params)
+            db.sync()
+            cs = db.last_plan_stats()
+            buf.set_option(_native.ESP_OPT_FULL_CENSUS, 0)
+            comp, iss, mat = (cs["full_cells_computed"], cs["full_lane_cells_issued"],
+                              cs["full_cells_matrix"])
+            tops = comp * GOTOH_OPS_PER_CELL / (esc_ms / 1e3) / 1e12
+            out[name]["full_matrix"] = {
+                "ms": round(esc_ms, 3), "cells_computed": comp, "lane_cells_issued": iss,
+                "cells_of_full_matrices": mat,
+                "computed_fraction": round(comp / mat, 4) if mat else None,
+                "issue_efficiency": round(comp / iss, 4) if iss else None,
+                "gcups": round(comp / (esc_ms / 1e3) / 1e9, 1),
+                "roofline_frac": round(tops / VALU_PEAK_TOPS, 4)}
+            log(f"{name}: full-matrix census {out[name]['full_matrix']}")
         log(f"{name}: routing {ps}")
         if name == "mate_rescue":
             rec = db.raw_results()[0]
@@ -579,7 +598,19 @@ def main() -> int:
     for r, w in pairs:
         plain_band(r, w, params, (32, 32), buf)
     plain_pps = ncalls / (time.perf_counter() - t0)
-    log(f"c1: batched {c1_batched_pps:.0f} pairs/s, plain {plain_pps:.0f} pairs/s")
+    # The plain form's kernel alone: each call's HIP-event kernel time (a
+    # second pass, so that the read-back stays out of the wall-clock rate).
+    kms_plain = []
+    for r, w in pairs:
+        plain_band(r, w, params, (32, 32), buf)
+        kms_plain.append(buf.last_kernel_ms())
+    from esperanto_amd.batch import band_cells as one_band_cells
+    plain_cells = sum(one_band_cells(len(r), len(w), 32, 32) for r, w in pairs)
+    plain_kernel_s = float(np.sum(kms_plain)) / 1e3
+    plain_kernel_gcups = plain_cells / plain_kernel_s / 1e9
+    plain_tops = plain_cells * GOTOH_OPS_PER_CELL / plain_kernel_s / 1e12
+    log(f"c1: batched {c1_batched_pps:.0f} pairs/s, plain {plain_pps:.0f} pairs/s, plain kernel "
+        f"{1e6 * plain_kernel_s / ncalls:.1f} us/call, {plain_kernel_gcups:.3f} GCUPS")
 
     myers_pps, k2, myers_tops, myers_plain_pps = bench_myers(args, buf)
 
@@ -625,6 +656,12 @@ def main() -> int:
                    "c1_graph_pairs_per_s": round(c1_graph_pps),
                    "c1_plain_pairs_per_s": round(plain_pps),
                    "batched_over_plain": round(c1_batched_pps / plain_pps, 1),
+                   "c1_plain_kernel": {
+                       "kernel": "k_gotoh_wave", "calls": ncalls,
+                       "us_per_call": round(1e6 * plain_kernel_s / ncalls, 2),
+                       "gcups": round(plain_kernel_gcups, 4),
+                       "roofline_frac_chip": round(plain_tops / VALU_PEAK_TOPS, 7),
+                       "roofline_frac_one_wave": round(plain_tops / ONE_WAVE_PEAK_TOPS, 4)},
                    "myers_pairs_per_s": round(myers_pps),
                    "myers_kernel_ms": round(k2, 3),
                    "myers_valu_tops": round(myers_tops, 2),

@@ -16,6 +16,7 @@ ESP_OK, ESP_E_INVAL, ESP_E_CAP, ESP_E_DEVICE, ESP_E_NOMEM, ESP_E_BUSY = range(6)
 ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET, ESP_OPT_BAND_GRID, ESP_OPT_FULL_PRUNE = 1, 2, 3, 4
 ESP_OPT_FULL_PRUNE_SHAPE = 5
 ESP_OPT_PLAIN_KERNEL = 6
+ESP_OPT_FULL_CENSUS = 7
 _STATUS = {1: "invalid argument", 2: "capacity", 3: "device error", 4: "out of memory",
            5: "busy"}
 
@@ -69,7 +70,9 @@ class CHostMeta(ctypes.Structure):
 class CPlanStats(ctypes.Structure):
     _fields_ = [("band_pairs", ctypes.c_uint32 * 5), ("strip_pairs", ctypes.c_uint32),
                 ("escalated", ctypes.c_uint32), ("full_pruned", ctypes.c_uint32),
-                ("full_unpruned", ctypes.c_uint32), ("full_second", ctypes.c_uint32)]
+                ("full_unpruned", ctypes.c_uint32), ("full_second", ctypes.c_uint32),
+                ("full_cells_computed", ctypes.c_uint64), ("full_lane_cells_issued", ctypes.c_uint64),
+                ("full_cells_matrix", ctypes.c_uint64)]
 
 
 class CChainBatch(ctypes.Structure):

@@ -205,14 +205,19 @@ class ContextTimings:
     def last_plan_stats(self) -> dict:
         """Routing of the last batch: pairs per band half, strip pairs, escalations,
         and of those the pairs the pruned full-matrix passes finished, the pairs
-        the unpruned kernel ran and the pairs the second pruned pass took."""
+        the unpruned kernel ran and the pairs the second pruned pass took; with
+        the census option on, the full-matrix passes' computed cells, issued
+        lane-cells and the cells of the full matrices they stand for."""
         st = _native.CPlanStats()
         _native.check(_native.lib().esp_last_plan_stats(self.buf.handle, ctypes.byref(st)),
                       self.buf.handle)
         return {"band_pairs": dict(zip((16, 30, 32, 48, 64), list(st.band_pairs))),
                 "strip_pairs": st.strip_pairs, "escalated": st.escalated,
                 "full_pruned": st.full_pruned, "full_unpruned": st.full_unpruned,
-                "full_second": st.full_second}
+                "full_second": st.full_second,
+                "full_cells_computed": st.full_cells_computed,
+                "full_lane_cells_issued": st.full_lane_cells_issued,
+                "full_cells_matrix": st.full_cells_matrix}
 
     def last_timings(self):
         """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,

@@ -79,6 +79,11 @@ struct esp_ctx {
     int full_prune = 1;   // extend_hint reruns: pruned pass first (1) or the full matrix only (0)
     int plain_kernel = 0;  // single-pair calls: 0 one wave per pair (windows up to 4096), 1 one lane
     bool last_pruned = false;  // the last batch ran the pruned pass (its counts are in escbuf)
+    // full-matrix cell census (ESP_OPT_FULL_CENSUS): on/off, its device
+    // counters, and whether the last batch ran the full-matrix stage with it
+    int census = 0;
+    esph::DevBuf censusbuf;
+    bool last_census = false;
     // stripe half-width and head pad of the two pruned passes
     int prune_half_a = espk::ESC_HALF_A, prune_pad_a = espk::ESC_PAD_A;
     int prune_half_b = espk::ESC_HALF_B, prune_pad_b = espk::ESC_PAD_B;

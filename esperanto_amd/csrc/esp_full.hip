@@ -293,6 +293,9 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
 #pragma unroll
         for (int r = 0; r < MAXM; r++) Pk[r] = pinit;
         int32_t bestkey = 0xFFFF, bj = 0;
+        // census (a.census set): cells this lane computed inside its matrix,
+        // and the wave's computed group-columns
+        uint32_t ccells = 0, cgroups = 0;
         for (int32_t j0 = 0; j0 < umaxn; j0 += FULL_CHUNK) {
             const int32_t len = max(0, min(FULL_CHUNK, n - j0));
             stage_win(pv.win + j0, len, chunk);
@@ -416,7 +419,27 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
                     bestkey = ckey;
                     bj = j;
                 }
+                if constexpr (PRUNE) {
+                    if (a.census) {
+                        // rows 1..m of the computed groups: whole groups below
+                        // m >> 3, then the m & 7 rows of the partial one
+                        const uint32_t gm = (uint32_t)m >> 3;
+                        const uint32_t rows = 8u * (uint32_t)__builtin_popcount(mcur & ((1u << gm) - 1u)) +
+                                              (((mcur >> gm) & 1u) ? ((uint32_t)m & 7u) : 0u);
+                        ccells += j <= n ? rows : 0u;
+                        cgroups += (uint32_t)__builtin_popcount(mcur);
+                    }
+                }
             }
+        }
+        if (a.census) {
+            if constexpr (!PRUNE) {
+                ccells = (uint32_t)m * (uint32_t)n;
+                cgroups = (uint32_t)ngroups * (uint32_t)umaxn;
+            }
+            if (active) atomicAdd(&a.census[0], (unsigned long long)ccells);
+            if (lane == 0) atomicAdd(&a.census[1], (unsigned long long)cgroups * 8ull * 64ull);
+            if (a.census_first && active) atomicAdd(&a.census[2], (unsigned long long)m * (unsigned long long)n);
         }
         if (PRUNE && active && fail) {
             a.fb_list[atomicAdd(&a.fb_range[1], 1u)] = p;
@@ -518,15 +541,23 @@ __global__ __launch_bounds__(256) void k_esc_hist(EscSortArgs a) {
     __shared__ uint32_t h[ESC_NB];
     for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x) h[b] = 0;
     __syncthreads();
+    __shared__ unsigned long long cells;
+    if (threadIdx.x == 0) cells = 0;
+    __syncthreads();
     uint32_t lo, hi;
     esc_chunk(*a.count, lo, hi);
+    unsigned long long mine = 0;
     for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-        const uint32_t k = esc_key(a, a.list[t]);
+        const uint32_t p = a.list[t];
+        const uint32_t k = esc_key(a, p);
         if (k < ESC_NB) atomicAdd(&h[k], 1u);
+        if (a.census) mine += (unsigned long long)a.read_len[p] * a.win_len[p];
     }
+    if (a.census && mine) atomicAdd(&cells, mine);
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x)
         if (h[b]) atomicAdd(&a.esc->hist[b], h[b]);
+    if (a.census && threadIdx.x == 0 && cells) atomicAdd(&a.census[2], cells);
 }
 
 // One block of 1024 threads: exclusive scan of the ESC_NB bucket counts into

@@ -218,6 +218,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     const int prune_s = max_subst(p);
     const bool prune = hint_band && c->full_prune && prune_s > 0;
     c->last_pruned = false;
+    c->last_census = false;
     espk::EscDev *esc = nullptr;
     uint32_t *plist = nullptr, *fblist = nullptr, *fb2list = nullptr;
     if (prune) {
@@ -299,6 +300,13 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * (pruned ? std::max(blocks, blocks_b) : blocks)));
         f.trace = (uint32_t *)c->ftrace.p;
         f.maxw = (uint32_t)maxw;
+        if (c->census) {
+            HIP_TRY(c, c->censusbuf.ensure(64));
+            HIP_TRY(c, espk::launch_zero_u32((uint32_t *)c->censusbuf.p, 16, c->stream));
+            f.census = (unsigned long long *)c->censusbuf.p;
+            f.census_first = 1;
+            c->last_census = true;
+        }
         if (pruned) {
             espk::EscSortArgs sa{};
             sa.list = a.esc_list;
@@ -310,6 +318,8 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             sa.esc = esc;
             sa.plist = plist;
             sa.fb2list = fb2list;
+            sa.win_len = b->win_len;
+            sa.census = f.census;
             HIP_TRY(c, espk::launch_esc_sort(sa, (uint32_t)bound, c->stream));
             tim_mark(c, 12, 0);
             c->last_pruned = true;
@@ -323,6 +333,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             g.prune_pad = c->prune_pad_a;
             g.fb_list = fblist;
             g.fb_range = esc->fbrange;
+            g.census_first = 0;  // the sort counted the matrices
             HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
             tim_mark(c, 7, rows);
             g.order = fblist;
@@ -333,11 +344,13 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             g.fb_list = fb2list;
             g.fb_range = esc->fb2range;
             g.unit = 1;
+            g.census_first = 0;
             HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks_b, c->stream));
             tim_mark(c, 13, rows);
             f.order = fb2list;
             f.range = esc->fb2range;
             f.counter = &esc->fb2counter;
+            f.census_first = 0;
             HIP_TRY(c, espk::launch_gotoh_full(f, rows, false, (int)blocks, c->stream));
             tim_mark(c, 11, rows);
         } else {
@@ -495,6 +508,10 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
         if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full kernel choice not in {0,1}");
         c->full_kernel = (int)value;
         return ESP_OK;
+    case ESP_OPT_FULL_CENSUS:
+        if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full-matrix census not in {0,1}");
+        c->census = (int)value;
+        return ESP_OK;
     case ESP_OPT_PLAIN_KERNEL:
         if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "plain kernel choice not in {0,1}");
         c->plain_kernel = (int)value;
@@ -626,6 +643,13 @@ int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
         out->full_pruned = esc[1] - fb[1];
         // pass B's failures are in fb2 beside the pairs no pass could help
         out->full_second = fa[1];
+    }
+    if (c->last_census && c->censusbuf.p) {
+        unsigned long long cs[3];
+        HIP_TRY(c, hipMemcpy(cs, c->censusbuf.p, sizeof(cs), hipMemcpyDeviceToHost));
+        out->full_cells_computed = cs[0];
+        out->full_lane_cells_issued = cs[1];
+        out->full_cells_matrix = cs[2];
     }
     return ESP_OK;
 }

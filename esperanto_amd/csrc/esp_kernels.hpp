@@ -73,6 +73,14 @@ struct GotohArgs {
     // Pairs per work unit of the full-matrix kernel: 64 (0 means 64) or 1
     // (one pair a wave, for short lists whose pairs share no stripe).
     uint32_t unit;
+    // Cell census of the full-matrix kernel (nullptr: off).  census[0] +=
+    // cells computed inside the pairs' matrices (rows 1..m, columns 1..n),
+    // census[1] += lane-cells issued (64 lanes x 8-row groups x columns,
+    // padding included), census[2] += m x n of each pair when census_first
+    // is set (the unpruned kernel run alone; with pruning the escalation sort
+    // counts the matrices).
+    unsigned long long *census;
+    int32_t census_first;
     // Always 0 (value-initialised).  Read from the argument block, the band
     // kernel's floor at 0 is an operand the compiler cannot see through, so
     // F's floor and H's max stay two v_max3.
@@ -240,6 +248,9 @@ struct EscSortArgs {
     EscDev *esc;
     uint32_t *plist;
     uint32_t *fb2list;  // pairs no pruned pass can help go straight to the unpruned kernel
+    // census (nullptr: off): census[2] += m x n of every listed pair
+    const uint32_t *win_len;
+    unsigned long long *census;
 };
 hipError_t launch_esc_sort(const EscSortArgs &a, uint32_t bound, hipStream_t s);
 // Head groups (8 rows each) a pruned pass computes for a pair: every row

@@ -32,23 +32,17 @@ namespace espk {
 
 namespace {
 
-// Moves x one lane down the wave: lane l returns lane l-1's x (lane 0 gets
-// 0).  row_shr:1 moves values inside each row of 16 lanes; the first lanes
-// of rows 1-3 take the last lane of the row before it through scalar
-// registers.  Called with all 64 lanes active.
-// The lane masks (all ones on lane 16, 32, 48 respectively) are computed
-// once; the merge is plain bit arithmetic so that no branch splits the wave.
-struct LaneFix {
-    int32_t a, b, c;
-    __device__ explicit LaneFix(int32_t lane)
-        : a(-(int32_t)(lane == 16)), b(-(int32_t)(lane == 32)), c(-(int32_t)(lane == 48)) {}
-};
-__device__ __forceinline__ int32_t lane_down1(int32_t x, const LaneFix &f) {
-    const int32_t r = __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
-    const int32_t a = __builtin_amdgcn_readlane(x, 15);
-    const int32_t b = __builtin_amdgcn_readlane(x, 31);
-    const int32_t c = __builtin_amdgcn_readlane(x, 47);
-    return (r & ~(f.a | f.b | f.c)) | (a & f.a) | (b & f.b) | (c & f.c);
+// Moves x one lane down the wave: lane l returns lane l-1's x, and lane 0
+// returns first (a value of its own).  Two DPP moves: row_bcast:15 (rows 1-3)
+// puts the last lane of each 16-lane row into every lane of the next row,
+// then row_shr:1 (bound_ctrl off, so the first lane of each row keeps that
+// value) moves every other lane's neighbour in; lane 0 is set by one select.
+// Called with all 64 lanes active, so no source lane is disabled and no
+// branch splits the wave.
+__device__ __forceinline__ int32_t lane_down1(int32_t x, int32_t first, bool lane0) {
+    const int32_t b = __builtin_amdgcn_update_dpp(0, x, 0x142, 0xE, 0xF, false);
+    const int32_t r = __builtin_amdgcn_update_dpp(b, x, 0x111, 0xF, 0xF, false);
+    return lane0 ? first : r;
 }
 
 __device__ __forceinline__ int32_t wave_max(int32_t v) {
@@ -69,12 +63,15 @@ __device__ __forceinline__ uint32_t wave_wpr(int32_t n, int32_t half) {
 }
 
 // Fills one pair.  win: the window's codes in LDS; bh/be: LDS copy of the
-// row above the current block (columns 1..n); tr: trace, wpr words per row.
-template <bool EDIT>
-__device__ WaveFill wave_fill(const PairView &pv, const GParams P, bool wrap16, const uint8_t *win,
+// row above the current block (columns 1..n); tr: trace, wpr words per row,
+// in LDS (TRL) or in the wave's global scratch.  Each step's LDS reads (the
+// lane's window code, lane 0's row above) are issued one step ahead, so no
+// step waits on LDS.
+template <bool EDIT, bool TRL>
+__device__ __forceinline__ WaveFill wave_fill(const PairView &pv, const GParams P, bool wrap16, const uint8_t *win,
                               int32_t *bh, int32_t *be, uint32_t *tr, uint32_t wpr) {
     const int32_t lane = (int32_t)(threadIdx.x & 63u);
-    const LaneFix fix(lane);
+    const bool lane0 = lane == 0, lane63 = lane == 63;
     const int32_t m = pv.m, n = pv.n, off = pv.off, half = pv.half;
     WaveFill r;
     r.best = 0;
@@ -93,6 +90,10 @@ __device__ WaveFill wave_fill(const PairView &pv, const GParams P, bool wrap16, 
         row_window(off, half, n, i0, lo0, hi0);
         row_window(off, half, n, i1, lo1, hi1);
         if (i0 > 1) row_window(off, half, n, i0 - 1, plo, phi);
+        if (!rowok) {
+            lo = 1;
+            hi = 0;  // no valid cell
+        }
         const int32_t J0 = lo0;
         const int32_t T = hi1 - lo0 + (i1 - i0) + 1;
         const int32_t rc = rowok ? read_code(pv.read[(rowok ? i : 1) - 1]) : 4;
@@ -103,41 +104,51 @@ __device__ WaveFill wave_fill(const PairView &pv, const GParams P, bool wrap16, 
         // H(i-1, j-1) for the first step: lane 0 reads it from the row above;
         // every other lane's first cell lies left of its window
         int32_t dH = 0;
-        if (lane == 0 && i0 > 1 && plo <= J0 - 1 && J0 - 1 <= phi) dH = bh[max(J0 - 1, 0)];
+        if (lane0 && i0 > 1 && plo <= J0 - 1 && J0 - 1 <= phi) dH = bh[max(J0 - 1, 0)];
         int32_t rmax = -1, rarg = 0, hl = -1, hr = -1;
         uint32_t word = 0;
-        uint32_t *trow = tr + (size_t)(i - 1) * wpr;
+        uint32_t *trow = tr + (size_t)(rowok ? i - 1 : 0) * wpr;
+        // step 0's loads: the lane's window code at column J0 - lane, lane
+        // 0's row above at column J0 (every index read is in range)
+        auto wcode = [&](int32_t j) -> int32_t { return (j >= 1 && j <= n) ? (int32_t)win[(j >= 1 && j <= n) ? j - 1 : 0] : 7; };
+        auto above = [&](int32_t j, const int32_t *v, int32_t dflt) -> int32_t {
+            return (i0 > 1 && plo <= j && j <= phi) ? v[(i0 > 1 && plo <= j && j <= phi) ? j : 0] : dflt;
+        };
+        int32_t fcn = wcode(J0 - lane);
+        int32_t bhn = above(J0, bh, 0), ben = above(J0, be, NEG);
         for (int32_t t = 0; t < T; t++) {
             const int32_t j = J0 + t - lane;
-            int32_t uH = lane_down1(oH, fix), uE = lane_down1(oE, fix);
-            if (lane == 0) {
-                const bool in = i0 > 1 && plo <= j && j <= phi;
-                const int32_t jj = in ? j : 0;  // every index read is in [0, n]
-                uH = in ? bh[jj] : 0;
-                uE = in ? be[jj] : NEG;
-            }
-            const bool valid = rowok && j >= lo && j <= hi;
-            const bool jin = j >= 1 && j <= n;
-            const int32_t fc = jin ? (int32_t)win[jin ? j - 1 : 0] : 7;
+            const int32_t fc = fcn, ah = bhn, ae = ben;
+            fcn = wcode(j + 1);
+            bhn = above(J0 + t + 1, bh, 0);
+            ben = above(J0 + t + 1, be, NEG);
+            const int32_t uH = lane_down1(oH, ah, lane0), uE = lane_down1(oE, ae, lane0);
+            const bool valid = j >= lo && j <= hi;
             const int32_t eo = uH + P.oe, ee = uE + P.ext;
             const int32_t E = max(eo, ee);
-            const uint32_t edir = ee > eo;
+            const uint32_t edir = (uint32_t)(eo - ee) >> 31;  // ee > eo
             const int32_t fo = oH + P.oe, fe = oF + P.ext;
             const int32_t F = max(fo, fe);
-            const uint32_t fdir = fe > fo;
+            const uint32_t fdir = (uint32_t)(fo - fe) >> 31;  // fe > fo
             int32_t sc = rc == fc ? P.match : P.mismatch;
             if (EDIT) sc = fc == partner ? 0 : sc;
             const int32_t D0 = dH + sc;
             const int32_t D = wrap16 && D0 > 32767 ? NEG : D0;
             const int32_t H = max(max(D, E), max(F, 0));
-            const uint32_t hd = H == 0 ? HD_START : (D == H ? HD_DIAG : (E == H ? HD_E : HD_F));
+            // the cell's nibble from sign bits (H >= D, E and H >= 0):
+            // HD_DIAG when D == H, else HD_E when E == H, else HD_F; HD_START
+            // when H == 0; then the E and F extension bits
+            const uint32_t a = (uint32_t)(D - H) >> 31, b = (uint32_t)(E - H) >> 31;
+            const uint32_t z = (uint32_t)(H - 1) >> 31;
+            const uint32_t hd = (a + (a & b)) | (z * 3u);
+            const uint32_t nib = hd | (edir << 2) | (fdir << 3);
             dH = uH;
             oH = valid ? H : 0;
             oE = valid ? E : NEG;
             oF = valid ? F : NEG;
             if (valid) {
                 const int32_t k = j - lo;
-                word |= (hd | (edir << 2) | (fdir << 3)) << (4 * (k & 7));
+                word |= nib << (4 * (k & 7));
                 if ((k & 7) == 7 || j == hi) {
                     trow[k >> 3] = word;
                     word = 0;
@@ -149,7 +160,7 @@ __device__ WaveFill wave_fill(const PairView &pv, const GParams P, bool wrap16, 
                 if (j == lo) hl = H;
                 if (j == hi) hr = H;
                 // the block's last row is the next block's row above
-                if (lane == 63) {
+                if (lane63) {
                     bh[j] = H;
                     be[j] = E;
                 }
@@ -308,9 +319,15 @@ __device__ void wave_one(const GotohArgs &a, uint32_t p, uint32_t mu, uint32_t n
     const bool wrap16 = (int64_t)m * smax > 32767 && !wave_impure(pv.read, m) && !wave_impure(pv.win, n);
     for (int32_t k = (int32_t)lane; k < n; k += 64) s_win[k] = (uint8_t)win_code(pv.win[k]);
     __syncthreads();
+    // the trace in LDS when it fits (each instance of the fill then stores it
+    // with LDS or global instructions, never flat ones)
+    auto fill = [&](uint32_t w) -> WaveFill {
+        if ((uint64_t)m * w <= WAVE_TR_LDS) return wave_fill<EDIT, true>(pv, a.P, wrap16, s_win, s_bh, s_be, s_tr, w);
+        return wave_fill<EDIT, false>(pv, a.P, wrap16, s_win, s_bh, s_be, g_tr, w);
+    };
     uint32_t wpr = wave_wpr(n, pv.half);
     uint32_t *tr = (uint64_t)m * wpr <= WAVE_TR_LDS ? s_tr : g_tr;
-    WaveFill fr = wave_fill<EDIT>(pv, a.P, wrap16, s_win, s_bh, s_be, tr, wpr);
+    WaveFill fr = fill(wpr);
     if (a.mode == ESP_MODE_HINT && band) {
         // extend_hint's escalation test in the reference's wrapping i32
         const int32_t lhs = (int32_t)((uint32_t)fr.best * 20u);
@@ -320,7 +337,7 @@ __device__ void wave_one(const GotohArgs &a, uint32_t p, uint32_t mu, uint32_t n
             pv.half = n;
             wpr = wave_wpr(n, n);
             tr = (uint64_t)m * wpr <= WAVE_TR_LDS ? s_tr : g_tr;
-            fr = wave_fill<EDIT>(pv, a.P, wrap16, s_win, s_bh, s_be, tr, wpr);
+            fr = fill(wpr);
             path = 1;
         }
     }

@@ -40,7 +40,7 @@ class ExtendBuffer:
     def set_option(self, option: int, value: int) -> None:
         """esp_ctx_set_option (ESP_OPT_FULL_KERNEL, ESP_OPT_TRACE_BUDGET,
         ESP_OPT_BAND_GRID, ESP_OPT_FULL_PRUNE, ESP_OPT_FULL_PRUNE_SHAPE,
-        ESP_OPT_PLAIN_KERNEL)."""
+        ESP_OPT_PLAIN_KERNEL, ESP_OPT_FULL_CENSUS)."""
         _native.check(_native.lib().esp_ctx_set_option(self._h, option, value), self._h)
 
     def last_kernel_ms(self) -> float:

@@ -244,9 +244,12 @@ int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
  * esp_extend, esp_extend_hint): 0 (default) one wavefront per pair, its 64
  * lanes sweeping 64 rows at once, for windows up to 4096 bases (longer ones
  * take the lane kernel); 1 the lane-per-pair strip kernel.  Results are the
- * same either way; esp_last_kernel_ms reads the call's kernel time. */
+ * same either way; esp_last_kernel_ms reads the call's kernel time.
+ * ESP_OPT_FULL_CENSUS (0, the default, or 1): the full-matrix reruns also
+ * count the cells they compute (esp_plan_stats.full_cells_*), at the cost of
+ * a few atomics per wave; results are unchanged. */
 enum { ESP_OPT_FULL_KERNEL = 1, ESP_OPT_TRACE_BUDGET = 2, ESP_OPT_BAND_GRID = 3, ESP_OPT_FULL_PRUNE = 4,
-       ESP_OPT_FULL_PRUNE_SHAPE = 5, ESP_OPT_PLAIN_KERNEL = 6 };
+       ESP_OPT_FULL_PRUNE_SHAPE = 5, ESP_OPT_PLAIN_KERNEL = 6, ESP_OPT_FULL_CENSUS = 7 };
 /* ESP_OPT_BAND_GRID sets one band half's persistent grid as esp_autotune_band
  * would: value = half << 16 | blocks per CU (1 .. the half's occupancy
  * limit; 0 restores the limit). */
@@ -265,6 +268,13 @@ typedef struct {
     uint32_t full_pruned;
     uint32_t full_unpruned;
     uint32_t full_second;  /* escalated pairs the second pruned pass took */
+    /* With ESP_OPT_FULL_CENSUS on (else 0), over every full-matrix pass of
+     * the batch: cells computed inside the pairs' matrices, lane-cells issued
+     * (64 lanes x 8-row groups x columns, padding included), and the cells
+     * of the escalated pairs' full matrices (sum of m x n). */
+    uint64_t full_cells_computed;
+    uint64_t full_lane_cells_issued;
+    uint64_t full_cells_matrix;
 } esp_plan_stats;
 int esp_last_plan_stats(esp_ctx *ctx, esp_plan_stats *out);
 

@@ -180,3 +180,46 @@ def test_pruned_full_matrix_equals_oracle_and_unpruned(buf, scoring):
             assert sp["full_pruned"] >= 0.9 * sp["escalated"], sp
         else:
             assert sp["full_unpruned"] > 0, sp  # fallbacks were exercised
+
+
+def test_full_matrix_census(buf):
+    """ESP_OPT_FULL_CENSUS counts the cells the full-matrix passes compute
+    without changing any result.  Unpruned, the computed cells are exactly the
+    matrices of the pairs the full-matrix kernel ran; on mate-like pairs whose
+    hint band lies inside the window (so that no band attempt runs in the
+    general kernel, which escalates by itself) those are the m x n of every
+    pair whose path says it escalated, summed on the host.  Pruned, the
+    computed cells are a fraction of them and at most the lane-cells the waves
+    issued."""
+    from esperanto_amd import _native
+    from esperanto_amd.batch import PairBatch
+    params = ExtendParams()
+    rng = np.random.default_rng(93)
+    mate = [q for q in _mate_like(rng, 900) if 31 <= q[2] <= 1950 - 150 - 31][:700]
+    for label, pairs in (("mate-like", mate), ("adversarial", _adversarial(rng, 600))):
+        part = PairBatch.from_pairs(pairs)
+        say(f"census {label}: {part.npairs} pairs")
+        ref_p, _, _ = _run(buf, part, params, True)
+        buf.set_option(_native.ESP_OPT_FULL_CENSUS, 1)
+        try:
+            gp, sp, _ = _run(buf, part, params, True)
+            gu, su, _ = _run(buf, part, params, False)
+        finally:
+            buf.set_option(_native.ESP_OPT_FULL_CENSUS, 0)
+        _same(gp, ref_p, f"{label} census on vs off")
+        _same(gu, ref_p, f"{label} census unpruned vs pruned")
+        esc = np.nonzero(gp[5] == 1)[0]
+        mn = int(sum(int(part.read_len[k]) * int(part.win_len[k]) for k in esc))
+        say(f"census {label}: pruned {sp['full_cells_computed']} of {su['full_cells_matrix']} cells, "
+            f"issued {sp['full_lane_cells_issued']}; unpruned {su['full_cells_computed']}; "
+            f"{len(esc)} escalated, {sp['escalated']} through the full-matrix kernel")
+        assert sp["escalated"] > 0
+        assert su["full_cells_computed"] == su["full_cells_matrix"] == sp["full_cells_matrix"]
+        assert su["full_lane_cells_issued"] >= su["full_cells_computed"]
+        assert 0 < sp["full_cells_computed"] <= sp["full_lane_cells_issued"]
+        if label == "mate-like":
+            assert len(esc) == sp["escalated"]
+            assert su["full_cells_matrix"] == mn
+            assert sp["full_cells_computed"] < mn
+        else:
+            assert su["full_cells_matrix"] <= mn
