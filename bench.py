@@ -156,6 +156,14 @@ def bench_window(args, buf, params):
                 "roofline_frac": round(tops / VALU_PEAK_TOPS, 4)}
             log(f"{name}: full-matrix census {out[name]['full_matrix']}")
         log(f"{name}: routing {ps}")
+        # The whole pipeline recorded as one HIP graph and replayed.
+        g = db.capture(params)
+        gw = _graph_steps(g, db, max(args.steps, 3), 1)
+        g.check()
+        g.destroy()
+        out[name]["graph_ms_per_step"] = round(gw * 1e3, 3)
+        out[name]["graph_pairs_per_s"] = round(n / gw)
+        log(f"{name}: graph replay {n / gw / 1e6:.2f} M pairs/s, {gw * 1e3:.2f} ms/step")
         if name == "mate_rescue":
             rec = db.raw_results()[0]
             out[name]["placed_fraction"] = round(float((rec["placed"] == 1).mean()), 4)

@@ -36,6 +36,7 @@ EXPORTS = [
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
     "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
     "esp_graph_check", "esp_graph_destroy", "esp_autotune_band", "esp_myers_infix",
+    "esp_mate_rescue_graph_create", "esp_spliced_graph_create",
 ]
 
 
@@ -229,6 +230,9 @@ def lib():
     L.esp_gotoh_graph_create.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), ctypes.c_int, P(CBatchOut),
                                          P(vp)]
     L.esp_graph_launch.argtypes = [vp, vp]
+    L.esp_mate_rescue_graph_create.argtypes = [vp, vp, P(CMateBatch), P(CParams), vp, vp, vp, vp, P(vp)]
+    L.esp_spliced_graph_create.argtypes = [vp, vp, P(CSpliceBatch), P(CParams), P(CBatchOut), vp, vp, vp,
+                                           ctypes.c_uint32, ctypes.c_uint32, P(vp)]
     L.esp_myers_infix.argtypes = [vp, u8p, sz, u8p, sz, P(ctypes.c_int32)]
     L.esp_autotune_band.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), P(CBatchOut), ctypes.c_int,
                                     P(CTuneReport)]

@@ -341,21 +341,15 @@ class DeviceBatch(ContextTimings):
             a.free()
 
 
-class GotohGraph:
-    """One batched Gotoh call of a DeviceBatch captured as a HIP graph
-    (esp_gotoh_graph_create): launch() redoes the batch on the device arrays'
-    current contents with one graph launch; check() waits and raises when a
-    launch met pairs longer than at capture."""
+class BatchGraph:
+    """A batched call captured as a HIP graph (esp_*_graph_create): launch()
+    redoes the call on the device arrays' current contents with one graph
+    launch; check() waits and raises when a launch met pairs past the bounds
+    the graph was recorded with."""
 
-    def __init__(self, db: DeviceBatch, params: Optional[ExtendParams], mode: int):
-        self.db = db
-        self.buf = db.buf
-        self._p = (params or ExtendParams()).to_c()
-        h = ctypes.c_void_p()
-        _native.check(_native.lib().esp_gotoh_graph_create(
-            self.buf.handle, ctypes.byref(db._cb), ctypes.byref(db._meta), ctypes.byref(self._p), mode,
-            ctypes.byref(db._out), ctypes.byref(h)), self.buf.handle)
-        self.handle = h.value
+    def __init__(self, buf, handle):
+        self.buf = buf
+        self.handle = handle
 
     def launch(self) -> None:
         _native.check(_native.lib().esp_graph_launch(self.buf.handle, ctypes.c_void_p(self.handle)),
@@ -369,3 +363,17 @@ class GotohGraph:
         if self.handle:
             _native.lib().esp_graph_destroy(self.buf.handle, ctypes.c_void_p(self.handle))
             self.handle = None
+
+
+class GotohGraph(BatchGraph):
+    """One batched Gotoh call of a DeviceBatch (any mode) captured as a HIP
+    graph (esp_gotoh_graph_create); recording runs the call once."""
+
+    def __init__(self, db: DeviceBatch, params: Optional[ExtendParams], mode: int):
+        self.db = db
+        self._p = (params or ExtendParams()).to_c()
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().esp_gotoh_graph_create(
+            db.buf.handle, ctypes.byref(db._cb), ctypes.byref(db._meta), ctypes.byref(self._p), mode,
+            ctypes.byref(db._out), ctypes.byref(h)), db.buf.handle)
+        super().__init__(db.buf, h.value)

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/esp_align.h"
@@ -63,6 +64,31 @@ inline constexpr size_t kFullTraceBudget = (size_t)16 << 30;  // full-matrix ker
 
 }  // namespace esph
 
+namespace esph {
+
+// Host-side bounds of a batch recorded into a HIP graph (esp_graph.cpp): the
+// longest read and window any replay may hold, and the strip kernel's trace
+// words per pair.  While a context captures, the Gotoh core sizes every
+// launch from these instead of reading the planner's summary back.
+struct CaptureBounds {
+    uint32_t maxm = 1, maxn = 0;
+    uint64_t strip_maxw = 1;
+};
+
+// Device scratch of the batch calls, as one set: a captured graph owns one
+// and swaps it into its context while it records.
+struct ScratchSet {
+    DevBuf counter, rows, trace, btrace, ftrace, plan, order, escbuf, censusbuf;
+    DevBuf wquery, warena, wdesc, wmyers, eng, jscratch, jarena;
+    void release() {
+        for (DevBuf *b : {&counter, &rows, &trace, &btrace, &ftrace, &plan, &order, &escbuf, &censusbuf, &wquery,
+                          &warena, &wdesc, &wmyers, &eng, &jscratch, &jarena})
+            b->release();
+    }
+};
+
+}  // namespace esph
+
 struct esp_ctx {
     int device = 0;
     int ncu = 0;
@@ -99,6 +125,8 @@ struct esp_ctx {
     hipEvent_t ev_plan = nullptr;
     int band_blocks[espk::PLAN_STRIP] = {};
     esph::HostBuf hstage, hout, hplan;
+    // Set while a graph records a call on this context (see CaptureBounds).
+    const esph::CaptureBounds *cap = nullptr;
     // Per-launch timing of the last batch call.
     static constexpr int kMaxEv = 16;
     hipEvent_t evs[kMaxEv + 1] = {};
@@ -192,14 +220,44 @@ inline int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Window stride of the mate-rescue pipeline for reads of at most maxm
+// bases: its windows span 2 * 750 + m bases at most.
+inline uint64_t mate_window_stride(uint64_t maxm) { return align_up(1500 + 3 * maxm, 16); }
+
+// The call's start / end events (esp_last_kernel_ms); not recorded while a
+// graph captures.
+inline hipError_t ev_record(esp_ctx *c, hipEvent_t e) { return c->cap ? hipSuccess : hipEventRecord(e, c->stream); }
+
+// Swaps the context's batch scratch with a graph's set.
+inline void swap_scratch(esp_ctx *c, ScratchSet &s) {
+    std::swap(c->counter, s.counter);
+    std::swap(c->rows, s.rows);
+    std::swap(c->trace, s.trace);
+    std::swap(c->btrace, s.btrace);
+    std::swap(c->ftrace, s.ftrace);
+    std::swap(c->plan, s.plan);
+    std::swap(c->order, s.order);
+    std::swap(c->escbuf, s.escbuf);
+    std::swap(c->censusbuf, s.censusbuf);
+    std::swap(c->wquery, s.wquery);
+    std::swap(c->warena, s.warena);
+    std::swap(c->wdesc, s.wdesc);
+    std::swap(c->wmyers, s.wmyers);
+    std::swap(c->eng, s.eng);
+    std::swap(c->jscratch, s.jscratch);
+    std::swap(c->jarena, s.jarena);
+}
+
 // Launch timing: mark() before the first launch and after every launch of
 // a batch call; esp_last_timings() reports the differences.
+// Under capture nothing is timed: a graph launch is timed as one unit.
 inline void tim_begin(esp_ctx *c) {
+    if (c->cap) return;
     c->nev = 0;
     (void)hipEventRecord(c->evs[0], c->stream);
 }
 inline void tim_mark(esp_ctx *c, int kind, int half) {
-    if (c->nev >= esp_ctx::kMaxEv) return;
+    if (c->cap || c->nev >= esp_ctx::kMaxEv) return;
     c->tim[c->nev].kind = kind;
     c->tim[c->nev].half = half;
     c->nev++;
@@ -291,12 +349,13 @@ inline bool band_lut_folded(const esp_params *p, uint32_t out[5]) {
 
 // Launch geometry of the persistent strip grid for `count` pairs whose
 // largest has maxm rows and maxw trace words.
-inline int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t maxm, uint64_t maxw) {
+inline int strip_launch(esp_ctx *c, espk::GotohArgs &a, uint64_t count, uint64_t maxm, uint64_t maxw,
+                        size_t trace_budget) {
     const int per_cu = espk::gotoh_strip_blocks_per_cu(a.P.edit);
     uint64_t lanes = (uint64_t)c->ncu * per_cu * espk::GOTOH_BLOCK;
     lanes = std::min<uint64_t>(lanes, align_up(count, 64));
     const uint64_t tw = std::max<uint64_t>(maxw, 1), rw = 5 * std::max<uint64_t>(maxm, 1);
-    lanes = std::min<uint64_t>(lanes, c->trace_budget / (tw * 4));
+    lanes = std::min<uint64_t>(lanes, trace_budget / (tw * 4));
     lanes = std::min<uint64_t>(lanes, kRowBudget / (rw * 4));
     lanes = std::max<uint64_t>(lanes / 64 * 64, 64);
     a.nl = (uint32_t)lanes;

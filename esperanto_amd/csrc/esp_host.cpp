@@ -116,9 +116,9 @@ int run_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, si
         HIP_TRY(c, c->trace.ensure(maxw * 4));
         a.trace = (uint32_t *)c->trace.p;
         a.maxw = (uint32_t)maxw;
-        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+        HIP_TRY(c, ev_record(c, c->ev0));
         HIP_TRY(c, espk::launch_gotoh_wave(a, 1, c->stream));
-        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(c, ev_record(c, c->ev1));
     } else {
         const uint64_t maxw = espk::strip_trace_words(m, n);
         HIP_TRY(c, c->rows.ensure(5 * std::max<size_t>(m, 1) * 64 * 4));
@@ -128,9 +128,9 @@ int run_single(esp_ctx *c, const uint8_t *read, size_t m, const uint8_t *win, si
         a.maxm = (uint32_t)std::max<size_t>(m, 1);
         a.maxw = (uint32_t)std::max<uint64_t>(maxw, 1);
         a.nl = 64;
-        HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+        HIP_TRY(c, ev_record(c, c->ev0));
         HIP_TRY(c, espk::launch_gotoh_strip(a, 1, c->stream));
-        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(c, ev_record(c, c->ev1));
     }
     HIP_TRY(c, hipMemcpyAsync(c->hout.p, c->stage_out.p, O.total, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -197,18 +197,24 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     pa.half = b->half;
     pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
     pa.hint_mode = hint_band;
-    pa.max_read = kMaxRead;
-    pa.max_win = (uint32_t)kMaxWin;
-    pa.max_strip_w = ~0ull;
+    // Under capture, pairs past the graph's bounds are counted as bad and
+    // left alone (esp_graph_check reports them).
+    const esph::CaptureBounds *capb = c->cap;
+    pa.max_read = capb ? capb->maxm : kMaxRead;
+    pa.max_win = capb ? capb->maxn : (uint32_t)kMaxWin;
+    pa.max_strip_w = capb ? capb->strip_maxw : ~0ull;
     pa.plan = (espk::PlanDev *)c->plan.p;
     pa.order = (uint32_t *)c->order.p;
     HIP_TRY(c, espk::launch_plan(pa, c->ncu, c->stream));
     tim_mark(c, 0, 0);
     // The plan summary travels back while the band kernels run: they read
     // their class ranges from device memory and need nothing from the host.
-    HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanSummary),
-                              hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ev_plan, c->stream));
+    // A graph sizes from its bounds instead and reads nothing back.
+    if (!capb) {
+        HIP_TRY(c, hipMemcpyAsync(c->hplan.p, c->plan.p, sizeof(espk::PlanSummary),
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipEventRecord(c->ev_plan, c->stream));
+    }
     a.order = pa.order;
     espk::PlanDev *dp = pa.plan;
     a.esc_range = dp->esc_range;
@@ -255,19 +261,43 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         }
     }
 
-    // 3. Strip kernel for everything else, sized from the plan summary.
-    HIP_TRY(c, hipEventSynchronize(c->ev_plan));
-    const espk::PlanSummary *hp = (const espk::PlanSummary *)c->hplan.p;
-    if (hp->bad) {
-        // The band kernels already queued read this batch's arrays: let them
-        // finish so that the caller may free them as soon as this returns.
-        (void)hipStreamSynchronize(c->stream);
-        return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+    // 3. Strip kernel for everything else, sized from the plan summary (or,
+    //    under capture, from the graph's bounds: its grid then drains
+    //    whatever the class holds at each replay, possibly nothing).
+    uint64_t strip_count, strip_maxm, strip_maxw, esc_bound = 0, esc_maxm = 0, esc_maxn = 0;
+    if (capb) {
+        strip_count = b->npairs;
+        strip_maxm = capb->maxm;
+        strip_maxw = capb->strip_maxw;
+        if (hint_band) {
+            esc_bound = b->npairs;
+            esc_maxm = std::min<uint64_t>(capb->maxm, espk::band_max_read(30));
+            esc_maxn = capb->maxn;
+        }
+    } else {
+        HIP_TRY(c, hipEventSynchronize(c->ev_plan));
+        const espk::PlanSummary *hp = (const espk::PlanSummary *)c->hplan.p;
+        if (hp->bad) {
+            // The band kernels already queued read this batch's arrays: let them
+            // finish so that the caller may free them as soon as this returns.
+            (void)hipStreamSynchronize(c->stream);
+            return fail(c, ESP_E_INVAL, "read longer than 16000 or window longer than 1<<24");
+        }
+        strip_count = hp->range[espk::PLAN_STRIP][1];
+        strip_maxm = hp->strip_maxm;
+        strip_maxw = hp->strip_maxw;
+        if (pa.hint_mode) {
+            esc_bound = hp->range[1][1];
+            esc_maxm = hp->esc_maxm;
+            esc_maxn = hp->esc_maxn;
+        }
     }
-    if (hp->range[espk::PLAN_STRIP][1]) {
+    if (strip_count) {
         a.range = &dp->sum.range[espk::PLAN_STRIP][0];
         a.counter = &dp->counters[espk::PLAN_STRIP];
-        rc = strip_launch(c, a, hp->range[espk::PLAN_STRIP][1], hp->strip_maxm, hp->strip_maxw);
+        // a graph's strip grid is sized for every pair but holds at most a
+        // quarter of the trace budget
+        rc = strip_launch(c, a, strip_count, strip_maxm, strip_maxw, capb ? c->trace_budget / 4 : c->trace_budget);
         if (rc) return rc;
         tim_mark(c, 2, 0);
     }
@@ -279,7 +309,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     //    A's failures one pair a wave with a wider stripe and a deeper head;
     //    the unpruned kernel drains B's failures and the pairs no pruned pass
     //    can help (usually nothing: it then exits at once).
-    if (pa.hint_mode && hp->range[1][1]) {
+    if (pa.hint_mode && esc_bound) {
         espk::GotohArgs f = a;
         f.mode = ESP_MODE_FULL;
         f.aux = nullptr;
@@ -287,9 +317,9 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         f.esc_info = nullptr;
         f.range = dp->esc_range;
         f.counter = &dp->counters[espk::PLAN_NCLASS];
-        const uint64_t bound = hp->range[1][1];
-        const int rows = espk::full_rows(hp->esc_maxm);
-        const uint64_t maxw = std::max<uint64_t>(1, (uint64_t)hp->esc_maxn * (rows / 8));
+        const uint64_t bound = esc_bound;
+        const int rows = espk::full_rows((uint32_t)esc_maxm);
+        const uint64_t maxw = std::max<uint64_t>(1, esc_maxn * (rows / 8));
         if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
         const uint64_t resident = (uint64_t)c->ncu * c->full_blocks[rows & 511];
         const uint64_t cap = std::max<uint64_t>(1, c->full_trace_budget / (maxw * 64 * 4));
@@ -300,7 +330,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * (pruned ? std::max(blocks, blocks_b) : blocks)));
         f.trace = (uint32_t *)c->ftrace.p;
         f.maxw = (uint32_t)maxw;
-        if (c->census) {
+        if (c->census && !c->cap) {
             HIP_TRY(c, c->censusbuf.ensure(64));
             HIP_TRY(c, espk::launch_zero_u32((uint32_t *)c->censusbuf.p, 16, c->stream));
             f.census = (unsigned long long *)c->censusbuf.p;
@@ -468,11 +498,11 @@ int esp_gotoh_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
     (void)meta;  // the device planner reads the lengths itself
     if (!c) return ESP_E_INVAL;
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     const int rc = gotoh_batch_core(c, b, p, mode, o);
     if (rc) return rc;
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -700,11 +730,11 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
     blocks = std::max<uint64_t>(blocks, 1);
     HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_myers(a, (int)maxm, (int)blocks, c->stream));
     tim_mark(c, 3, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -750,12 +780,12 @@ int esp_engine_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *
             if (meta->win_len[k] > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
     if (b->npairs > 0xFFFFFFFFull) return fail(c, ESP_E_INVAL, "too many pairs");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     int rc = engine_myers_launch(c, b->npairs, b->reads, b->read_off, b->read_len, meta->read_len, b->wins,
                                  b->win_off, b->win_len, mode, dist);
     if (rc) return rc;
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -780,14 +810,14 @@ int esp_engine_verify_batch(esp_ctx *c, const esp_engine_verify *v, const uint32
     pa.text_off = (uint64_t *)c->eng.p;
     pa.text_len = (uint32_t *)((uint8_t *)c->eng.p + n * 8);
     pa.status = status;
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_engine_prep(pa, c->stream));
     tim_mark(c, 5, 0);
     int rc = engine_myers_launch(c, n, v->reads, v->read_off, v->read_len, host_read_len, v->tx, pa.text_off,
                                  pa.text_len, 0, dist);
     if (rc) return rc;
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 

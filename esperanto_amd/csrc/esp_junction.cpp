@@ -92,11 +92,11 @@ int esp_refine_junction_batch(esp_ctx *c, const esp_ref *r, const esp_junctions 
     a.q = q;
     a.reads = reads;
     a.out = out;
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_refine(a, c->stream));
     tim_mark(c, 5, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -156,7 +156,7 @@ int esp_align_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_junctions *j
     a.srec = srec;
     a.rec = rec;
 
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_chain_prep(a, c->stream));
     HIP_TRY(c, espk::launch_scan2(a.nex, a.plen, n, exon_off, pseudo_off, c->stream));
@@ -182,7 +182,7 @@ int esp_align_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_junctions *j
     rc = spliced_core(c, r, &sb, p, o, stitched, stitch_off, srec);
     if (rc) return rc;
     HIP_TRY(c, espk::launch_chain_rec(a, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -227,7 +227,7 @@ int esp_reextend_batch_run(esp_ctx *c, const esp_ref *r, const esp_reextend_batc
     a.cig_out_n = cig_out_n;
     a.rec = rec;
 
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_reext_prep(a, c->stream));
     HIP_TRY(c, espk::launch_scan2(a.wlen, nullptr, n, win_off, nullptr, c->stream));
@@ -257,7 +257,7 @@ int esp_reextend_batch_run(esp_ctx *c, const esp_ref *r, const esp_reextend_batc
     if (rc) return rc;
     HIP_TRY(c, espk::launch_reext_finish(a, c->stream));
     tim_mark(c, 6, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -372,7 +372,7 @@ int esp_intron_chain_batch_run(esp_ctx *c, const esp_ref *r, const esp_junctions
         return ESP_OK;
     };
 
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_ic(a, 0, c->stream));
     rc = gather();
@@ -410,7 +410,7 @@ int esp_intron_chain_batch_run(esp_ctx *c, const esp_ref *r, const esp_junctions
     rc = spliced_core(c, r, &sb, p, o, stitched, stitch_off, srec);
     if (rc) return rc;
     HIP_TRY(c, espk::launch_ic(a, 4, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 

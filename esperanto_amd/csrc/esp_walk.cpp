@@ -197,11 +197,11 @@ int esp_walk_placements(esp_ctx *c, const esp_ref *r, const esp_junctions *jl, c
     // the placements in rounds, with no partial last generation
     const uint64_t blocks =
         std::min<uint64_t>((b->nplace + 255) / 256, (uint64_t)c->ncu * (uint64_t)espk::walk_blocks_per_cu());
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_walk(a, (int)std::max<uint64_t>(blocks, 1), c->stream));
     tim_mark(c, 8, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -236,11 +236,11 @@ int esp_fast_lane_batch(esp_ctx *c, const esp_ref *r, const esp_fast_batch *b, c
     a.out = out;
     a.cigar = cigar;
     const uint64_t blocks = std::min<uint64_t>((b->nchain + 255) / 256, (uint64_t)c->ncu * 8);
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_fast_lane(a, (int)std::max<uint64_t>(blocks, 1), c->stream));
     tim_mark(c, 9, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 

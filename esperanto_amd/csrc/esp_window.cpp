@@ -88,7 +88,7 @@ int esph::window_myers_core(esp_ctx *c, const esp_ref *r, const esp_window_job *
     a.p_len = cv.take<uint32_t>(n);
     a.t_len = cv.take<uint32_t>(n);
     a.status = status;
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_winmyers_prep(a, c->stream));
     tim_mark(c, 5, 0);
@@ -109,7 +109,7 @@ int esph::window_myers_core(esp_ctx *c, const esp_ref *r, const esp_window_job *
     HIP_TRY(c, espk::launch_zero_u32(m.counter, 1, c->stream));
     HIP_TRY(c, espk::launch_myers(m, mp, (int)std::max<uint64_t>(blocks, 1), c->stream));
     tim_mark(c, 3, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -205,11 +205,11 @@ int esp_gather_batch(esp_ctx *c, const esp_ref *r, const esp_gather_job *jobs, s
     a.njobs = (uint32_t)njobs;
     a.dst = dst;
     a.status = status;
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_gather(a, c->stream));
     tim_mark(c, 5, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -239,7 +239,7 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
     a.mate_pos = b->mate_pos;
     a.max_read = maxm;
     a.qstride = (uint32_t)(2 * align_up(maxm, 16));
-    a.wstride = align_up(1500 + 3 * (uint64_t)maxm, 16);  // 2 * span + m at most
+    a.wstride = mate_window_stride(maxm);
     HIP_TRY(c, c->wquery.ensure(n * a.qstride + 64));
     HIP_TRY(c, c->warena.ensure(n * a.wstride + 64));
     HIP_TRY(c, c->wdesc.ensure(mate_desc_bytes(n)));
@@ -267,7 +267,7 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
     a.cig_n = cig_n;
     HIP_TRY(c, c->counter.ensure(256));
 
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_mate_prep(a, c->stream));
     tim_mark(c, 5, 0);
@@ -315,7 +315,7 @@ int esp_mate_rescue_batch(esp_ctx *c, const esp_ref *r, const esp_mate_batch *b,
     if (rc) return rc;
     HIP_TRY(c, espk::launch_mate_finish(a, c->stream));
     tim_mark(c, 6, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -394,7 +394,7 @@ int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,
     a.cig_n = cig_n;
     HIP_TRY(c, c->counter.ensure(256));
 
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_unsp_prep(a, c->stream));
     tim_mark(c, 5, 0);
@@ -438,7 +438,7 @@ int esp_unspliced_batch(esp_ctx *c, const esp_ref *r, const esp_chain_batch *b,
     if (rc) return rc;
     HIP_TRY(c, espk::launch_unsp_finish(a, c->stream));
     tim_mark(c, 6, 0);
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 
@@ -458,11 +458,11 @@ int esp_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_splice_batch *b,
         !b->exon_off || !b->pseudo_off)
         return fail(c, ESP_E_INVAL, "batch arrays are required");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     rc = spliced_core(c, r, b, p, o, stitched, stitch_off, rec);
     if (rc) return rc;
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }
 

@@ -100,6 +100,19 @@ class DeviceMateBatch(ContextTimings):
             ctypes.c_void_p(self.rec.ptr), ctypes.c_void_p(self.cigar.ptr),
             ctypes.c_void_p(self.cig_off.ptr), ctypes.c_void_p(self.cig_n.ptr)), self.buf.handle)
 
+    def capture(self, params: Optional[ExtendParams] = None):
+        """launch() recorded as one HIP graph (esp_mate_rescue_graph_create;
+        recording runs the pipeline once); the returned BatchGraph replays it."""
+        from .batch import BatchGraph
+        self._gp = (params or ExtendParams()).to_c()
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().esp_mate_rescue_graph_create(
+            self.buf.handle, self.dref.handle, ctypes.byref(self._cb), ctypes.byref(self._gp),
+            ctypes.c_void_p(self.rec.ptr), ctypes.c_void_p(self.cigar.ptr),
+            ctypes.c_void_p(self.cig_off.ptr), ctypes.c_void_p(self.cig_n.ptr), ctypes.byref(h)),
+            self.buf.handle)
+        return BatchGraph(self.buf, h.value)
+
     def raw_results(self):
         """(records, cigar words, cig_off, cig_n) as numpy arrays."""
         n = self.npairs

@@ -148,6 +148,26 @@ class DeviceSpliceBatch(ContextTimings):
             ctypes.byref(self._out), ctypes.c_void_p(self.stitched.ptr),
             ctypes.c_void_p(self.stitch_off.ptr), ctypes.c_void_p(self.rec.ptr)), self.buf.handle)
 
+    def capture(self, params: Optional[ExtendParams] = None, max_read_len: Optional[int] = None,
+                max_pseudo_len: Optional[int] = None):
+        """launch() recorded as one HIP graph (esp_spliced_graph_create;
+        recording runs the pipeline once); bounds default to this batch's
+        longest read and pseudo-reference.  The returned BatchGraph replays it."""
+        from .batch import BatchGraph
+        n = self.npairs
+        if max_read_len is None:
+            max_read_len = int(self.req.read_len.max()) if n else 1
+        if max_pseudo_len is None:
+            max_pseudo_len = int(np.diff(self.pseudo_off_host.astype(np.int64)).max()) if n else 1
+        self._gp = (params or ExtendParams()).to_c()
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().esp_spliced_graph_create(
+            self.buf.handle, self.dref.handle, ctypes.byref(self._cb), ctypes.byref(self._gp),
+            ctypes.byref(self._out), ctypes.c_void_p(self.stitched.ptr),
+            ctypes.c_void_p(self.stitch_off.ptr), ctypes.c_void_p(self.rec.ptr),
+            max(1, max_read_len), max(1, max_pseudo_len), ctypes.byref(h)), self.buf.handle)
+        return BatchGraph(self.buf, h.value)
+
     def raw_results(self):
         """(records, ext[n,4] int64, cigar, cig_off, cig_n, stitched, stitch_off)."""
         n = self.npairs

@@ -190,18 +190,21 @@ typedef struct {
 int esp_autotune_band(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
                       esp_batch_out *out, int reps, esp_tune_report *report);
 
-/* One esp_gotoh_batch call captured as a HIP graph (band or full mode).
- * Capture sizes everything from the host lengths (meta->read_len and
- * meta->win_len are required) and records the planner, every band kernel and
- * the strip kernel on the graph's own scratch; nothing runs until
- * esp_graph_launch.  A launch redoes the whole batch on what the batch's
- * device arrays hold at that time (same arrays, same pair count, reads and
- * windows no longer than at capture) and writes the same outputs as
- * esp_gotoh_batch would.  Pairs longer than at capture are left alone and
- * counted; esp_graph_check (waits for the stream) returns ESP_E_INVAL when a
- * launch met any.  Launches on one context run in stream order; a graph must
- * not be launched on two contexts at once.  No counterpart in the
- * reference: it is the batched form's replay without per-kernel launches. */
+/* One esp_gotoh_batch call captured as a HIP graph, in any mode (extend_hint's
+ * escalations included: the escalation list, its sort and the pruned and
+ * unpruned full-matrix passes all run from device counters).  The graph
+ * sizes every launch from the host lengths (meta->read_len and meta->win_len
+ * are required) instead of reading the planner back, owns its own scratch,
+ * and is recorded by running the call once eagerly (which writes the outputs)
+ * and once under stream capture.  esp_graph_launch redoes the whole batch on
+ * what the batch's device arrays hold at that time (same arrays, same pair
+ * count, reads and windows no longer than at capture) and writes the same
+ * outputs as esp_gotoh_batch would.  Pairs past those bounds are left alone
+ * and counted; esp_graph_check (waits for the stream) returns ESP_E_INVAL
+ * when a launch met any.  Launches on one context run in stream order; a
+ * graph must not be launched on two contexts at once.  No counterpart in the
+ * reference: it is the batched form's replay without per-kernel launches or
+ * host round trips. */
 typedef struct esp_graph esp_graph;
 int esp_gotoh_graph_create(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
                            int mode, esp_batch_out *out, esp_graph **graph);
@@ -401,6 +404,23 @@ typedef struct {
 int esp_spliced_batch(esp_ctx *ctx, const esp_ref *ref, const esp_splice_batch *b,
                       const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
                       const uint64_t *stitch_off, esp_splice_rec *rec);
+
+/* The mate-rescue and spliced window pipelines (esp_mate_rescue_batch,
+ * esp_spliced_batch) captured as HIP graphs the same way as
+ * esp_gotoh_graph_create (launch, check and destroy with esp_graph_*): one
+ * graph holds the window preparation, the EA-Myers launches, the extend_hint
+ * batch with its escalations and the selection or stitching.  Bounds: the mate batch's
+ * max_read_len (and the window stride it implies); for the spliced batch the
+ * caller states the longest read and the longest pseudo-reference any replay
+ * may hold.  A replay reads the batch arrays again (mate positions, exons,
+ * hints, reads may change; pair count, pseudo_bytes and n_exons may not). */
+int esp_mate_rescue_graph_create(esp_ctx *ctx, const esp_ref *ref, const esp_mate_batch *b,
+                                 const esp_params *p, esp_mate_rec *out, uint32_t *cigar,
+                                 const uint64_t *cig_off, uint32_t *cig_n, esp_graph **graph);
+int esp_spliced_graph_create(esp_ctx *ctx, const esp_ref *ref, const esp_splice_batch *b,
+                             const esp_params *p, esp_batch_out *ext_out, uint32_t *stitched,
+                             const uint64_t *stitch_off, esp_splice_rec *rec, uint32_t max_read_len,
+                             uint32_t max_pseudo_len, esp_graph **graph);
 
 /* Unspliced chain extension, extend_unspliced_chain after its fast lane
  * (align.rs:1770-1919): per chain, the window continuous_extension_window

@@ -1,13 +1,17 @@
-"""A batched Gotoh call captured as a HIP graph (esp_gotoh_graph_create /
-esp_graph_launch) against the eager esp_gotoh_batch on the same batch: an
-uneven band batch with a tenth of its pairs outside their band class (the
-strip kernel), and full mode; every replay checked field by field (extension,
-TouchRec, CIGAR words and counts, aux) after the outputs are cleared, with
-other eager work on the context between replays; then new read contents of
-the same lengths written into the batch's device arrays and replayed; then a
-graph captured for shorter pairs than the device holds, whose launch leaves
-the over-length strip pairs alone, reports them through esp_graph_check, and
-still gets every other pair right."""
+"""Batched calls captured as HIP graphs (esp_gotoh_graph_create,
+esp_mate_rescue_graph_create, esp_spliced_graph_create / esp_graph_launch)
+against the eager calls on the same batch: an uneven band batch with a tenth
+of its pairs outside their band class (the strip kernel), full mode, and
+extend_hint mode with its escalations (the sort, the pruned passes and the
+unpruned kernel all inside the graph); the mate-rescue and spliced window
+pipelines.  Every replay is checked field by field (extension, TouchRec,
+CIGAR words and counts, aux; pipeline records and stitched CIGARs) after the
+outputs are cleared, with other eager work on the context between replays;
+then new read contents of the same lengths (new mate positions for the
+rescue pipeline) are written into the device arrays and replayed; then a
+graph captured for shorter pairs than the device holds leaves the
+over-length pairs alone, reports them through esp_graph_check, and still gets
+every other pair right."""
 from __future__ import annotations
 
 import ctypes
@@ -66,7 +70,7 @@ def _mixed_batch(seed):
     return hb
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_graph_replays_equal_eager(buf, mode):
     from esperanto_amd.batch import DeviceBatch, workload_c1
     hb = _mixed_batch(5)
@@ -217,3 +221,105 @@ def test_autotune_changes_shapes_not_results(buf):
         db.free()
     finally:
         b2.close()
+
+
+def _clear(buf, arr, nbytes):
+    z = np.full(max(nbytes, 1), 0xA5, dtype=np.uint8)
+    _native.check(_native.lib().esp_copy_to_device(buf.handle, ctypes.c_void_p(arr.ptr),
+                                                   ctypes.c_void_p(z.ctypes.data), nbytes), buf.handle)
+
+
+def _mate_state(mb):
+    rec, cig, off, cig_n = mb.raw_results()
+    words = [cig[int(off[k]):int(off[k]) + int(cig_n[k])].tobytes() for k in range(mb.npairs)]
+    return rec.tobytes(), cig_n.tobytes(), words
+
+
+def _splice_state(sb):
+    rec, ext, cig, off, cig_n, st, soff = sb.raw_results()
+    words = [cig[int(off[k]):int(off[k]) + int(cig_n[k])].tobytes() for k in range(sb.npairs)]
+    stw = [st[int(soff[k]):int(soff[k]) + int(rec[k]["n_stitched"])].tobytes() for k in range(sb.npairs)]
+    return rec.tobytes(), ext.tobytes(), cig_n.tobytes(), words, stw
+
+
+def test_window_pipeline_graphs_equal_eager(buf):
+    """Mate rescue (prep, EA-Myers on both strands, select, extend_hint with
+    escalations, finish) and the spliced pipeline (pseudo-reference decode,
+    extend_hint, stitch) each recorded as one graph: three replays equal the
+    eager call; then the mates are moved and a replay equals a fresh eager
+    call on the moved batch."""
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.rescue import MATE_REC_DTYPE, DeviceMateBatch, MateRequests
+    from esperanto_amd.splice import SPLICE_REC_DTYPE, DeviceSpliceBatch
+    from esperanto_amd.synth_window import splice_rescue_workload
+    from esperanto_amd.batch import DeviceBatch, workload_c1
+    P = ExtendParams()
+    ref, sp, mt, _ = splice_rescue_workload(3000, ref_len=2_000_000)
+    dref = DeviceReference(ref, buf)
+    other = DeviceBatch(workload_c1(1000), buf)
+    mb = DeviceMateBatch(mt, dref, buf)
+    sb = DeviceSpliceBatch(sp, dref, buf)
+    gm = gs = None
+    try:
+        n = mb.npairs
+        mb.launch(P)
+        mb.sync()
+        eager_m = _mate_state(mb)
+        sb.launch(P)
+        sb.sync()
+        eager_s = _splice_state(sb)
+        say("mate rescue graph: recording")
+        gm = mb.capture(P)
+        say("spliced graph: recording")
+        gs = sb.capture(P)
+        for rep in range(3):
+            _clear(buf, mb.rec, MATE_REC_DTYPE.itemsize * n)
+            _clear(buf, mb.cig_n, 4 * n)
+            _clear(buf, mb.cigar, 4 * int(mb.cig_off_host[-1]))
+            _clear(buf, sb.rec, SPLICE_REC_DTYPE.itemsize * n)
+            _clear(buf, sb.ext, 16 * n)
+            _clear(buf, sb.cig_n, 4 * n)
+            _clear(buf, sb.cigar, 4 * int(sb.cig_off_host[-1]))
+            _clear(buf, sb.stitched, 4 * int(sb.stitch_off_host[-1]))
+            buf_sync(buf)
+            say(f"window graphs: replay {rep}")
+            gm.launch()
+            gs.launch()
+            gm.check()
+            gs.check()
+            assert _mate_state(mb) == eager_m
+            assert _splice_state(sb) == eager_s
+            other.launch_gotoh(P, 0)
+            other.sync()
+        # move every mate by a few bases: a replay reads the new positions
+        rng = np.random.default_rng(12)
+        pos = (mt.mate_pos.astype(np.int64) + rng.integers(-40, 41, n)).clip(0).astype(np.uint32)
+        _native.check(_native.lib().esp_copy_to_device(buf.handle, ctypes.c_void_p(mb.d["mate_pos"].ptr),
+                                                       ctypes.c_void_p(pos.ctypes.data), pos.nbytes),
+                      buf.handle)
+        moved = MateRequests(mt.reads, mt.read_off, mt.read_len, mt.contig, pos)
+        fresh = DeviceMateBatch(moved, dref, buf)
+        fresh.launch(P)
+        fresh.sync()
+        want = _mate_state(fresh)
+        fresh.free()
+        _clear(buf, mb.rec, MATE_REC_DTYPE.itemsize * n)
+        _clear(buf, mb.cig_n, 4 * n)
+        _clear(buf, mb.cigar, 4 * int(mb.cig_off_host[-1]))
+        buf_sync(buf)
+        say("mate rescue graph: replay on moved mates")
+        gm.launch()
+        gm.check()
+        assert _mate_state(mb) == want
+    finally:
+        for g in (gm, gs):
+            if g is not None:
+                g.destroy()
+        mb.free()
+        sb.free()
+        other.free()
+        dref.close()
+
+
+def buf_sync(buf):
+    _native.check(_native.lib().esp_sync(buf.handle), buf.handle)
