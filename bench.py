@@ -215,6 +215,11 @@ def bench_myers(args, buf):
     myers_ops = c2.npairs * 214 * (MYERS_OPS_PER_WORD_COL * words + 3)
     myers_tops = myers_ops / (k2 / 1e3) / 1e12
     log(f"myers: {myers_pps/1e6:.1f} M pairs/s (kernel {k2:.2f} ms)")
+    # The same batch with its read-length bound stated: no host round trip.
+    maxm = int(c2.read_len.max())
+    wb, kb, _ = timed_steps(lambda: d2.launch_myers(False, max_read_len=maxm), d2, max(args.steps, 3), 1)
+    args._myers_bounded = (c2.npairs / wb, kb)
+    log(f"myers, bound {maxm} stated: {c2.npairs / wb / 1e6:.1f} M pairs/s (kernels {kb:.2f} ms)")
     d2.free()
     # The plain form: one esp_myers_best_end call (one launch) per pair.
     from esperanto_amd.myers_ea import infix_best_end
@@ -673,6 +678,7 @@ def main() -> int:
                    "myers_pairs_per_s": round(myers_pps),
                    "myers_kernel_ms": round(k2, 3),
                    "myers_valu_tops": round(myers_tops, 2),
+                   "myers_bounded_pairs_per_s": round(args._myers_bounded[0]),
                    "myers_plain_pairs_per_s": round(myers_plain_pps),
                    "myers_batched_over_plain": round(myers_pps / myers_plain_pps, 1),
                    "splice_rescue": win,

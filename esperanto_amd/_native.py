@@ -65,7 +65,8 @@ class CTiming(ctypes.Structure):
 
 class CHostMeta(ctypes.Structure):
     _fields_ = [("read_len", ctypes.c_void_p), ("win_len", ctypes.c_void_p),
-                ("offset", ctypes.c_void_p), ("half", ctypes.c_void_p)]
+                ("offset", ctypes.c_void_p), ("half", ctypes.c_void_p),
+                ("max_read_len", ctypes.c_uint32)]
 
 
 class CPlanStats(ctypes.Structure):

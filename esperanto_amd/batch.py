@@ -290,11 +290,17 @@ class DeviceBatch(ContextTimings):
         """launch_gotoh recorded once as a HIP graph; GotohGraph.launch() replays it."""
         return GotohGraph(self, params, mode)
 
-    def launch_myers(self, with_start: bool = True) -> None:
-        _native.check(_native.lib().esp_myers_batch(
-            self.buf.handle, ctypes.byref(self._cb), ctypes.byref(self._meta),
-            ctypes.c_void_p(self.dist.ptr), ctypes.c_void_p(self.end.ptr),
-            ctypes.c_void_p(self.start.ptr) if with_start else None), self.buf.handle)
+    def launch_myers(self, with_start: bool = True, max_read_len: int = 0) -> None:
+        """esp_myers_batch; max_read_len > 0 states a bound on the read lengths
+        (the call then reads nothing back; longer reads are still exact)."""
+        self._meta.max_read_len = int(max_read_len)
+        try:
+            _native.check(_native.lib().esp_myers_batch(
+                self.buf.handle, ctypes.byref(self._cb), ctypes.byref(self._meta),
+                ctypes.c_void_p(self.dist.ptr), ctypes.c_void_p(self.end.ptr),
+                ctypes.c_void_p(self.start.ptr) if with_start else None), self.buf.handle)
+        finally:
+            self._meta.max_read_len = 0
 
     def launch_engine_myers(self, mode: int = 0) -> None:
         """The transcriptome engine's EA-Myers (engine/src/myers.rs) per pair:

@@ -79,10 +79,10 @@ struct CaptureBounds {
 // and swaps it into its context while it records.
 struct ScratchSet {
     DevBuf counter, rows, trace, btrace, ftrace, plan, order, escbuf, censusbuf;
-    DevBuf wquery, warena, wdesc, wmyers, eng, jscratch, jarena;
+    DevBuf wquery, warena, wdesc, wmyers, eng, jscratch, jarena, mlist;
     void release() {
         for (DevBuf *b : {&counter, &rows, &trace, &btrace, &ftrace, &plan, &order, &escbuf, &censusbuf, &wquery,
-                          &warena, &wdesc, &wmyers, &eng, &jscratch, &jarena})
+                          &warena, &wdesc, &wmyers, &eng, &jscratch, &jarena, &mlist})
             b->release();
     }
 };
@@ -122,6 +122,8 @@ struct esp_ctx {
     esph::DevBuf eng;
     // chain preparation and re-extension: per-item state, offsets and arenas
     esph::DevBuf jscratch, jarena;
+    // EA-Myers batch: pairs longer than the caller's bound, for the 8-word launch
+    esph::DevBuf mlist;
     hipEvent_t ev_plan = nullptr;
     int band_blocks[espk::PLAN_STRIP] = {};
     esph::HostBuf hstage, hout, hplan;
@@ -246,6 +248,7 @@ inline void swap_scratch(esp_ctx *c, ScratchSet &s) {
     std::swap(c->eng, s.eng);
     std::swap(c->jscratch, s.jscratch);
     std::swap(c->jarena, s.jarena);
+    std::swap(c->mlist, s.mlist);
 }
 
 // Launch timing: mark() before the first launch and after every launch of

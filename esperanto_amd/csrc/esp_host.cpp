@@ -453,6 +453,7 @@ int esp_destroy(esp_ctx *c) {
     c->eng.release();
     c->jscratch.release();
     c->jarena.release();
+    c->mlist.release();
     c->escbuf.release();
     c->hstage.release();
     c->hout.release();
@@ -702,16 +703,6 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, c->counter.ensure(256));
     HIP_TRY(c, c->hout.ensure(64));
-    // The longest pattern of at most 256 bases sizes the kernel's words
-    // (longer ones take the guard) and the longest text is checked: both
-    // come from one reduction over the device length arrays, 8 bytes read
-    // back, which costs far less than scanning the host copies.
-    uint32_t *lmax = (uint32_t *)c->counter.p + 16;
-    HIP_TRY(c, espk::launch_len_max(b->read_len, b->win_len, b->npairs, lmax, c->ncu, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->hout.p, lmax, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const uint32_t maxm = ((const uint32_t *)c->hout.p)[0], maxn = ((const uint32_t *)c->hout.p)[1];
-    if (maxn > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
     espk::MyersArgs a{};
     a.npairs = (uint32_t)b->npairs;
     a.order = nullptr;
@@ -724,16 +715,52 @@ int esp_myers_batch(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, i
     a.dist = dist;
     a.end = end;
     a.start = start;
-    a.counter = (uint32_t *)c->counter.p;
+    uint32_t *cnt = (uint32_t *)c->counter.p;  // [0] queue, [16..17] lengths, [32] overflow queue, [48..49] its range
+    a.counter = cnt;
+    uint32_t maxm;
+    if (meta->max_read_len) {
+        // The caller's bound sizes the words; longer patterns (up to 256)
+        // are listed on the device and an 8-word launch drains the list
+        // (an empty list ends it at once).  Nothing is read back.
+        maxm = std::min<uint32_t>(meta->max_read_len, 256);
+        HIP_TRY(c, c->mlist.ensure(b->npairs * 4));
+        a.ovf_list = (uint32_t *)c->mlist.p;
+        a.ovf_range = cnt + 48;
+        HIP_TRY(c, espk::launch_zero_u32(cnt, 64, c->stream));
+    } else {
+        // The longest pattern of at most 256 bases sizes the kernel's words
+        // (longer ones take the guard) and the longest text is checked: both
+        // come from one reduction over the device length arrays, 8 bytes read
+        // back, which costs far less than scanning the host copies.
+        uint32_t *lmax = cnt + 16;
+        HIP_TRY(c, espk::launch_len_max(b->read_len, b->win_len, b->npairs, lmax, c->ncu, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->hout.p, lmax, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        maxm = ((const uint32_t *)c->hout.p)[0];
+        const uint32_t maxn = ((const uint32_t *)c->hout.p)[1];
+        if (maxn > kMaxWin) return fail(c, ESP_E_INVAL, "text too long");
+        HIP_TRY(c, espk::launch_zero_u32(cnt, 1, c->stream));
+    }
     const int per_cu = espk::myers_blocks_per_cu((int)maxm, false);
     uint64_t blocks = (uint64_t)c->ncu * per_cu;
     blocks = std::min<uint64_t>(blocks, (b->npairs + 63) / 64);
     blocks = std::max<uint64_t>(blocks, 1);
-    HIP_TRY(c, espk::launch_zero_u32(a.counter, 1, c->stream));
     HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_myers(a, (int)maxm, (int)blocks, c->stream));
     tim_mark(c, 3, 0);
+    if (a.ovf_list && maxm < 256) {
+        espk::MyersArgs o = a;
+        o.order = a.ovf_list;
+        o.range = a.ovf_range;
+        o.ovf_list = nullptr;
+        o.ovf_range = nullptr;
+        o.counter = cnt + 32;
+        const uint64_t ob = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->ncu * espk::myers_blocks_per_cu(256, false),
+                                                                   (b->npairs + 63) / 64));
+        HIP_TRY(c, espk::launch_myers(o, 256, (int)ob, c->stream));
+        tim_mark(c, 3, 256);
+    }
     HIP_TRY(c, ev_record(c, c->ev1));
     return ESP_OK;
 }

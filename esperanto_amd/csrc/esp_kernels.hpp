@@ -102,6 +102,11 @@ struct MyersArgs {
     uint64_t *start;  // nullptr: no reversed scan
     uint32_t *counter;
     int32_t global;   // 1: global distance (both ends forced; engine/src/myers.rs:149-168)
+    // With ovf_list set, the 32-bit kernel appends the pairs whose pattern is
+    // longer than its words hold (up to 256 bases) to ovf_list through the
+    // counter ovf_range[1], for an 8-word launch over that list.
+    uint32_t *ovf_list;
+    uint32_t *ovf_range;
 };
 
 // verify_dist windows (engine/src/lib.rs:851-876): per candidate the text is

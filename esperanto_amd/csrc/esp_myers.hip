@@ -276,9 +276,13 @@ __global__ __launch_bounds__(MYERS_BLOCK) void k_myers32(MyersArgs a) {
             const int32_t m = (int32_t)a.read_len[p], n = (int32_t)a.text_len[p];
             const uint8_t *read = a.reads + a.read_off[p];
             const uint8_t *text = a.texts + a.text_off[p];
-            if (m == 0 || m > 32 * NW) {
-                // long::infix_best_end guards (myers_ea.rs:275-280); the host
-                // sizes NW for every m <= 256, so only the guards land here
+            if (m > 32 * NW && m <= 256 && a.ovf_list) {
+                // a wider instance takes it (the host's bound was short)
+                a.ovf_list[atomicAdd(&a.ovf_range[1], 1u)] = p;
+            } else if (m == 0 || m > 32 * NW) {
+                // long::infix_best_end guards (myers_ea.rs:275-280); without
+                // an overflow list the host sizes NW for every m <= 256, so
+                // only the guards land here
                 a.dist[p] = m == 0 ? 0 : m;
                 if (a.end) a.end[p] = 0;
                 if (a.start) a.start[p] = 0;

@@ -156,6 +156,11 @@ typedef struct {
     const uint32_t *win_len;
     const int64_t *offset;
     const int64_t *half;
+    /* Optional (0: unknown): a bound on the batch's read lengths.  With it,
+     * esp_myers_batch sizes its kernel from the bound and reads nothing back;
+     * reads past the bound still get exact results (a wider kernel takes
+     * them, more slowly). */
+    uint32_t max_read_len;
 } esp_host_meta;
 
 int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
@@ -166,9 +171,12 @@ int esp_gotoh_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
  *   start != NULL          long::infix_best_start  -> dist, start
  *   both                   both (dist is the same minimum)
  *   neither                long::infix             -> dist only
- * The call reduces the device length arrays to the longest pattern and text
- * (one 8-byte read-back: the pattern length sizes the kernel's words) and
- * then returns with the kernel queued on the context stream. */
+ * With meta->max_read_len set, the call queues its kernels and returns
+ * without waiting (no host round trip; it can be captured in a graph).
+ * Without it, the call reduces the device length arrays to the longest
+ * pattern and text (one 8-byte read-back, which waits for the work already
+ * queued on the context stream), refuses texts over 1<<24, and returns with
+ * the kernel queued. */
 int esp_myers_batch(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta,
                     int32_t *dist, uint64_t *end, uint64_t *start);
 

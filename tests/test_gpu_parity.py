@@ -308,11 +308,11 @@ def _myers_cases():
     return cases
 
 
-def _check_myers(buf, hb, label):
+def _check_myers(buf, hb, label, bound=0):
     from esperanto_amd.batch import DeviceBatch
-    say(f"myers {label} pairs={hb.npairs}")
+    say(f"myers {label} pairs={hb.npairs} bound={bound}")
     db = DeviceBatch(hb, buf)
-    db.launch_myers(True)
+    db.launch_myers(True, max_read_len=bound)
     db.sync()
     gd, ge, gs = db.myers_results()
     od, oe, os_ = oracle.myers_batch(hb, True, nthreads=8)
@@ -325,6 +325,16 @@ def _check_myers(buf, hb, label):
 def test_myers_sweep(buf):
     from esperanto_amd.batch import PairBatch
     _check_myers(buf, PairBatch.from_pairs(_myers_cases()), "sweep")
+
+
+@pytest.mark.parametrize("bound", [300, 256, 150, 64, 1])
+def test_myers_sweep_with_a_stated_bound(buf, bound):
+    """esp_myers_batch with meta->max_read_len: the kernel's words come from
+    the bound and nothing is read back; patterns past a short bound go to the
+    8-word launch over the overflow list, so every answer equals the oracle's
+    whatever the bound (300 and 256 need no list, 1 sends nearly every pair)."""
+    from esperanto_amd.batch import PairBatch
+    _check_myers(buf, PairBatch.from_pairs(_myers_cases()), "sweep", bound)
 
 
 def test_myers_config2_sample(buf):
