@@ -188,6 +188,90 @@ def bench_window(args, buf, params):
     return out
 
 
+def bench_junction(args, buf, params):
+    """align_spliced over chains (segment_chain, refine_junction_read per seam,
+    the guards, then the spliced pipeline), try_intron_chain_placement (MEMs,
+    fragments, EA-Myers refinements and verification, junction, DP) and
+    reextend_with (window from the clips, extend_hint, acceptance) over
+    junction_workload reads on a 64 Mb reference: reads/s per call with
+    inputs resident, device ms and stage times, and the first reads checked
+    against the oracle."""
+    from esperanto_amd.junction import (DeviceChainBatch, DeviceIntronChainBatch, DeviceReextendBatch,
+                                        JunctionLib)
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.synth_window import junction_workload
+    n = args.junction_reads
+    log(f"junction workload: {n} reads on a {args.window_ref_mb} Mb reference")
+    w = junction_workload(n, ref_len=args.window_ref_mb * 1_000_000)
+    dref = DeviceReference(w["reference"], buf)
+    jlib = JunctionLib(dref, [tuple(int(x) for x in q) for q in w["quads"]])
+    k = w["k"]
+    legs = (
+        ("align_spliced", DeviceChainBatch(dref, w["reads"], w["read_off"], w["read_len"], w["contig"],
+                                           w["minus"], w["chain_anchors"], w["chain_off"]),
+         lambda db: db.launch(jlib, k, extend_params=params)),
+        ("intron_chain", DeviceIntronChainBatch(dref, w["reads"], w["read_off"], w["read_len"],
+                                                w["pool_anchors"], w["pool_off"]),
+         lambda db: db.launch(jlib, k, extend_params=params)),
+        ("reextend", DeviceReextendBatch(dref, w["rx_reads"], w["read_off"], w["read_len"], w["contig"],
+                                         w["rx_pos"], w["rx_cigar"], w["rx_cig_off"], w["rx_cig_n"]),
+         lambda db: db.launch(params, 0)),
+    )
+    out = {"reads": n, "reference_bases": int(args.window_ref_mb * 1_000_000)}
+    kinds = {0: "plan", 1: "band30", 2: "strip", 3: "myers", 5: "prep", 6: "select_finish",
+             7: "escalated_full", 11: "escalated_unpruned", 12: "escalation_sort", 13: "escalated_pruned_b"}
+    for name, db, launch in legs:
+        wall, kms, tims = timed_steps(lambda: launch(db), db, max(args.steps, 3), 1)
+        stages = {}
+        for st in tims:
+            for kind, half, ms in st:
+                stages.setdefault(kinds.get(kind, str(kind)), []).append(ms)
+        stages = {k2: round(float(np.sum(v)) / len(tims), 3) for k2, v in stages.items()}
+        rec = db.records()
+        ok = int((rec["status"] == 1).sum())
+        out[name] = {"reads_per_s": round(n / wall), "ms_per_step": round(wall * 1e3, 3),
+                     "device_ms": round(kms, 3), "stages_ms": stages, "status_1_fraction": round(ok / n, 4)}
+        log(f"{name}: {n / wall / 1e6:.2f} M reads/s, {wall * 1e3:.2f} ms/step, device {kms:.2f} ms, "
+            f"status 1 on {ok / n:.3f}, stages {stages}")
+    if not args.no_cpu:
+        # the first reads against the oracle, timed as the CPU port (one thread)
+        from oracle import oracle
+        o = oracle.OracleRef(w["reference"])
+        jl = oracle.JLib([tuple(int(x) for x in q) for q in w["quads"]])
+        m = int(w["read_len"][0])
+        ns = min(n, args.junction_check)
+        cpu = {}
+        chains, icres, rxres = legs[0][1].records(), legs[1][1].records(), legs[2][1].records()
+        t0 = time.perf_counter()
+        for c in range(ns):
+            a0, a1 = int(w["chain_off"][c]), int(w["chain_off"][c + 1])
+            anc = [(int(w["chain_anchors"][2 * i]), int(w["chain_anchors"][2 * i + 1])) for i in range(a0, a1)]
+            r = oracle.align_spliced(o, jl, bytes(w["reads"][m * c:m * c + m]), 0, False, anc, k, params=params)
+            assert r["status"] == int(chains[c]["status"]) and r["genomic_start"] == int(chains[c]["genomic_start"]), c
+        t1 = time.perf_counter()
+        for c in range(ns):
+            a0, a1 = int(w["pool_off"][c]), int(w["pool_off"][c + 1])
+            pool = [tuple(int(x) for x in w["pool_anchors"][4 * i:4 * i + 4]) for i in range(a0, a1)]
+            r = oracle.intron_chain(o, jl, bytes(w["reads"][m * c:m * c + m]), pool, k, params=params)
+            assert r["status"] == int(icres[c]["status"]) and r["score"] == int(icres[c]["score"]), c
+        t2 = time.perf_counter()
+        for c in range(ns):
+            r = oracle.reextend_with(o, bytes(w["rx_reads"][m * c:m * c + m]), 0, int(w["rx_pos"][c]),
+                                     [int(x) for x in w["rx_cigar"][3 * c:3 * c + 3]], params, 0)
+            assert r["status"] == int(rxres[c]["status"]) and r["pos"] == int(rxres[c]["pos"]), c
+        t3 = time.perf_counter()
+        cpu = {"align_spliced_reads_per_s": round(ns / (t1 - t0)), "intron_chain_reads_per_s": round(ns / (t2 - t1)),
+               "reextend_reads_per_s": round(ns / (t3 - t2)), "cores": 1, "sample_reads": ns,
+               "matches_device": True}
+        out["cpu_port"] = cpu
+        log(f"junction cpu port (and device check on {ns} reads): {cpu}")
+    for _, db, _ in legs:
+        db.free()
+    jlib.close()
+    dref.close()
+    return out
+
+
 def _copy_calibration(buf, src, nbytes):
     """One plain device copy of nbytes (at most src's size): the yardstick that
     the FETCH_SIZE / WRITE_SIZE counters are scaled by."""
@@ -499,6 +583,11 @@ def main() -> int:
     ap.add_argument("--walk-pairs", type=int, default=1_000_000,
                     help="read pairs of the placement-walk leg (two placements each; 0 skips it)")
     ap.add_argument("--only-walk", action="store_true", help="run only the placement-walk leg")
+    ap.add_argument("--junction-reads", type=int, default=200_000,
+                    help="reads of the align_spliced / intron-chain / reextend leg (0 skips it)")
+    ap.add_argument("--junction-check", type=int, default=2000,
+                    help="reads of that leg checked against the oracle and timed on the CPU")
+    ap.add_argument("--only-junction", action="store_true", help="run only the junction leg")
     ap.add_argument("--only-myers", action="store_true", help="run only the batched Myers leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
@@ -536,6 +625,10 @@ def main() -> int:
         return 0
     if args.only_walk:
         print(json.dumps(bench_walk(args, buf, params)), flush=True)
+        buf.close()
+        return 0
+    if args.only_junction:
+        print(json.dumps(bench_junction(args, buf, params)), flush=True)
         buf.close()
         return 0
 
@@ -634,6 +727,9 @@ def main() -> int:
     walks = None
     if args.walk_pairs:
         walks = bench_walk(args, buf, params)
+    junction = None
+    if args.junction_reads:
+        junction = bench_junction(args, buf, params)
     serve = None
     if args.serve_requests:
         serve = bench_serving(args, buf, params)
@@ -683,6 +779,7 @@ def main() -> int:
                    "myers_batched_over_plain": round(myers_pps / myers_plain_pps, 1),
                    "splice_rescue": win,
                    "placement_walks": walks,
+                   "junction": junction,
                    "serving": serve},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",

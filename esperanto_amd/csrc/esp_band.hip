@@ -20,8 +20,19 @@
 // extension, F from extension), each the sign bit of one subtraction shifted
 // in with v_alignbit; the traceback carries the cell value along the path, so
 // H == 0 (the reference's H_START) needs no stored bit.
-// Pairs that are not interior, or whose half has no specialisation, run the
-// general strip kernel (esp_kernels.hip).
+// Bands clamped on the left (0 <= offset < half, offset + m + half <= n,
+// mismatch score <= 0) run here too.  The reference's row window is then
+// [max(c - half, 1), c + half]; the band's columns cb + i + k <= 0 read the
+// window code of 'N'.  By induction those cells hold H = 0: their diagonal
+// is such a cell or row 0, so D = 0 + mismatch <= 0, their E comes from the
+// cell above (such a cell, U = open_ext <= 0) and F is floored at 0.  A real
+// cell therefore reads exactly the reference's margins through them (H = 0
+// left and diagonal; the floored F is 0 either way), the traceback stops at
+// column 0 as the reference's loop does, and the row maximum is positive
+// only at real cells, so the left touch is "the first maximum lies at or
+// before the first real cell" (with a zero row, that cell holds 0 as well).
+// Other pairs, and halves with no specialisation, run the general strip
+// kernel (esp_kernels.hip).
 #include "esp_kernels.hpp"
 
 using namespace espd;
@@ -90,6 +101,32 @@ __device__ __forceinline__ void ring_batch(uint8_t *codes, uint32_t lane, const 
     for (int t = 0; t < 4 * NW; t++) {
         if (t < nrows) {
             const uint8_t v = ring_byte(byte_at<NW>(ww, t), byte_at<NW>(rw, t), c0 + t <= m);
+            codes[(size_t)ph * 64 + lane] = v;
+            if (ph <= W - 1) codes[(size_t)(ph + RING) * 64 + lane] = v;
+            ph = ph == RING ? 1 : ph + 1;
+        }
+    }
+}
+
+// ring_batch for a band clamped on the left (cb < 0): logical row c's window
+// byte is win[cb + c - 1] when that index is >= 0 and c <= L, else 'N'.
+template <int W, int RING>
+__device__ void ring_batch_clamped(uint8_t *codes, uint32_t lane, const uint8_t *win, int32_t cb,
+                                   const uint8_t *read, int32_t m, int32_t L, int32_t c0, int32_t nrows) {
+    uint32_t rw[4];
+    load_span<4>(read, c0 - 1, m - 1, rw);
+    uint32_t wb[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int32_t c = c0 + t, idx = cb + c - 1;
+        const bool in = t < nrows && idx >= 0 && c <= L;
+        wb[t] = in ? (uint32_t)win[in ? idx : 0] : (uint32_t)'N';
+    }
+    int32_t ph = c0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        if (t < nrows) {
+            const uint8_t v = ring_byte(wb[t], byte_at<4>(rw, t), c0 + t <= m);
             codes[(size_t)ph * 64 + lane] = v;
             if (ph <= W - 1) codes[(size_t)(ph + RING) * 64 + lane] = v;
             ph = ph == RING ? 1 : ph + 1;
@@ -266,8 +303,16 @@ __global__ __launch_bounds__(BAND_BLOCK, band_waves(HALF)) void k_gotoh_band(Got
         // first RING now, the rest as the row loop needs them.
         const int32_t L = m > 0 ? m + W - 1 : 0;
         const uint8_t *s0 = pv.win + cb;
-        for (int32_t c0 = 1; c0 <= RING; c0 += 16)
-            ring_batch<W, RING, 4>(codes, lane, s0, pv.read, m, L, c0, c0, min(16, RING - c0 + 1));
+        if (cb >= 0) {
+            for (int32_t c0 = 1; c0 <= RING; c0 += 16)
+                ring_batch<W, RING, 4>(codes, lane, s0, pv.read, m, L, c0, c0, min(16, RING - c0 + 1));
+        } else {
+            // left-clamped band: logical rows c <= -cb are 'N' columns, the
+            // rest are read byte by byte from the window (every row the ring
+            // refills later lies past RING > half >= -cb, inside the window)
+            for (int32_t c0 = 1; c0 <= RING; c0 += 16)
+                ring_batch_clamped<W, RING>(codes, lane, pv.win, cb, pv.read, m, L, c0, min(16, RING - c0 + 1));
+        }
         if (lane < 5) lut_lds[lane] = FOLD ? a.lutf[lane] : a.lut[lane];
         __syncthreads();  // one wave per block: orders the LUT writes before any lane reads them
 
@@ -432,7 +477,9 @@ __global__ __launch_bounds__(BAND_BLOCK, band_waves(HALF)) void k_gotoh_band(Got
                     }
                 }
                 if (!touched) {
-                    if (rarg == 0) {  // the first maximum is cell 0: H[0] == rmax
+                    // the first maximum is the window's first cell (band cell
+                    // 0, or the first real one of a left-clamped row)
+                    if (rarg <= -min(cb + i - 1, 0)) {
                         touched = 1;
                         trow = i;
                         tleft = 1;

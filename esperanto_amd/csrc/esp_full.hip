@@ -224,8 +224,12 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
         if (u >= nunits) break;
         const uint32_t ubegin = first + usz * u;
         const uint32_t ucount = min(usz, total - usz * u);
-        const bool active = lane < ucount;
-        const uint32_t p = a.order[ubegin + (active ? lane : 0u)];
+        const uint32_t p = a.order[ubegin + (lane < ucount ? lane : 0u)];
+        // A pair longer than the tile or the trace slab was sized for (a
+        // graph replayed on pairs past its bounds, which the planner counts
+        // as bad) is left alone.
+        const bool active = lane < ucount && a.read_len[p] <= (uint32_t)MAXM &&
+                            (uint64_t)a.win_len[p] * WR <= (uint64_t)a.maxw;
         PairView pv;
         pv.read = a.reads + a.read_off[p];
         pv.win = a.wins + a.win_off[p];

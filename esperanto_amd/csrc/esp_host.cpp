@@ -197,6 +197,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     pa.half = b->half;
     pa.band_ok = mode == ESP_MODE_BAND && band_lut_fits(p);
     pa.hint_mode = hint_band;
+    pa.left_clamp_ok = p->mismatch <= 0;
     // Under capture, pairs past the graph's bounds are counted as bad and
     // left alone (esp_graph_check reports them).
     const esph::CaptureBounds *capb = c->cap;

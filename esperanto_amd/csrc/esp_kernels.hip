@@ -382,11 +382,16 @@ constexpr int kPlanHalves[5] = {16, 30, 32, 48, 64};
 __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint32_t &cls) {
     const uint32_t m = a.read_len[p], n = a.win_len[p];
     cls = PLAN_STRIP;
+    // The band kernel takes bands whose every row window lies inside [1, n]
+    // (offset >= half and offset + m + half <= n) and, with left_clamp_ok,
+    // bands clamped only on the left (0 <= offset < half): there the row
+    // centres offset + i stay inside [1, n] and the columns left of the
+    // window become 'N' columns that keep H = 0.
     if (a.hint_mode && m >= 1 && n > 30u) {
-        // extend_hint attempt 1 is the band (hint, 30) (extend.rs:408-419):
-        // interior ones run in the half-30 band kernel.
+        // extend_hint attempt 1 is the band (hint, 30) (extend.rs:408-419)
         const int64_t off = a.offset[p];
-        if (off >= 30 && off + (int64_t)m + 30 <= (int64_t)n && (int64_t)m <= band_max_read(30))
+        if (off >= (a.left_clamp_ok ? 0 : 30) && off + (int64_t)m + 30 <= (int64_t)n &&
+            (int64_t)m <= band_max_read(30))
             cls = 1;
     } else if (a.band_ok && m >= 1) {
         const int64_t off = a.offset[p], half = a.half[p];
@@ -394,7 +399,7 @@ __device__ __forceinline__ uint32_t plan_bin(const PlanArgs &a, uint32_t p, uint
 #pragma unroll
         for (int t = 0; t < 5; t++)
             if (half == kPlanHalves[t]) c = t;
-        if (c >= 0 && off >= half && off + (int64_t)m + half <= (int64_t)n &&
+        if (c >= 0 && off >= (a.left_clamp_ok ? 0 : half) && off + (int64_t)m + half <= (int64_t)n &&
             (int64_t)m <= band_max_read((int)half))
             cls = (uint32_t)c;
     }

@@ -155,6 +155,10 @@ struct PlanArgs {
     const int64_t *half;
     int32_t band_ok;    // 1 when mode is band and the scoring fits the band kernel
     int32_t hint_mode;  // 1 when mode is extend_hint and the scoring fits the band kernel
+    // 1 when bands clamped on the left (0 <= offset < half) may take the band
+    // kernel too: the mismatch score is <= 0, so the 'N' columns that stand
+    // for the columns left of the window keep H = 0 (see the band kernel)
+    int32_t left_clamp_ok;
     uint32_t max_read;
     uint32_t max_win;
     // strip-class pairs whose strip trace exceeds this many words are counted

@@ -18,7 +18,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native
-from .batch import DeviceArray
+from .batch import ContextTimings, DeviceArray
 from .records import CigarOp, Extension, ExtendParams, cigar_from_words
 from .reference import DeviceReference
 
@@ -153,56 +153,62 @@ class SplicedChainResult:
     genomic_start: int = 0
 
 
-def align_spliced_batch(dref: DeviceReference, jlib: Optional[JunctionLib], reads: Sequence[bytes],
-                        contigs: Sequence[int], minus: Sequence[bool], anchors: Sequence[Sequence[Tuple[int, int]]],
-                        k: int, params: SpliceParams = SpliceParams(),
-                        extend_params: Optional[ExtendParams] = None) -> List[SplicedChainResult]:
-    """align_spliced per chain: anchors[c] is chain c's (qpos, rpos) list in (rpos, qpos) order."""
-    buf = dref.buf
-    n = len(reads)
-    if n == 0:
-        return []
-    rb, ro, rl = _concat(reads)
-    na = np.array([len(a) for a in anchors], dtype=np.uint64)
-    ao = np.zeros(n + 1, dtype=np.uint64)
-    ao[1:] = np.cumsum(na)
-    flat = np.array([x for a in anchors for pr in a for x in pr], dtype=np.uint32) if int(ao[-1]) else \
-        np.zeros(2, np.uint32)
-    caps = 2 * rl.astype(np.uint64) + 5
-    co = np.zeros(n + 1, dtype=np.uint64)
-    co[1:] = np.cumsum(caps)
-    so = np.zeros(n + 1, dtype=np.uint64)
-    so[1:] = np.cumsum(caps + 2 * na)
-    slots = max(int(ao[-1]), 1)
-    d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
-         (("reads", rb), ("read_off", ro), ("read_len", rl), ("contig", np.asarray(contigs, dtype=np.uint32)),
-          ("minus", np.asarray(minus, dtype=np.uint32)), ("anchors", flat), ("anc_off", ao), ("cig_off", co),
-          ("stitch_off", so))}
-    d["ext"] = DeviceArray(buf, 16 * n)
-    d["cigar"] = DeviceArray(buf, 4 * int(co[-1]))
-    d["cig_n"] = DeviceArray(buf, 4 * n)
-    d["stitched"] = DeviceArray(buf, 4 * int(so[-1]))
-    d["rec"] = DeviceArray(buf, CHAIN_REC_DTYPE.itemsize * n)
-    d["junctions"] = DeviceArray(buf, REFINED_DTYPE.itemsize * slots)
-    d["bounds"] = DeviceArray(buf, 12 * slots)
-    try:
-        cb = _native.CSplicedChainBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr, d["contig"].ptr,
-                                        d["minus"].ptr, d["anchors"].ptr, d["anc_off"].ptr)
-        out = _native.CBatchOut(d["ext"].ptr, None, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr, None)
-        sp, ep = params.to_c(), (extend_params or ExtendParams()).to_c()
+class DeviceChainBatch(ContextTimings):
+    """align_spliced's chains resident in HBM with their result arenas
+    (esp_align_spliced_batch): reads (concatenated, u64 offsets, u32
+    lengths), contig and strand per chain, and the flat (qpos, rpos) anchor
+    list with its u64 prefix offsets."""
+
+    def __init__(self, dref: DeviceReference, rb, ro, rl, contigs, minus, flat, ao):
+        self.buf = buf = dref.buf
+        self.dref = dref
+        n = self.npairs = int(rl.shape[0])
+        self.ao = ao
+        caps = 2 * rl.astype(np.uint64) + 5
+        self.co = np.zeros(n + 1, dtype=np.uint64)
+        self.co[1:] = np.cumsum(caps)
+        self.so = np.zeros(n + 1, dtype=np.uint64)
+        self.so[1:] = np.cumsum(caps + 2 * np.diff(ao.astype(np.uint64)))
+        self.slots = slots = max(int(ao[-1]), 1)
+        self.d = d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
+                      (("reads", rb), ("read_off", ro), ("read_len", rl),
+                       ("contig", np.asarray(contigs, dtype=np.uint32)),
+                       ("minus", np.asarray(minus, dtype=np.uint32)), ("anchors", flat), ("anc_off", ao),
+                       ("cig_off", self.co), ("stitch_off", self.so))}
+        d["ext"] = DeviceArray(buf, 16 * n)
+        d["cigar"] = DeviceArray(buf, 4 * int(self.co[-1]))
+        d["cig_n"] = DeviceArray(buf, 4 * n)
+        d["stitched"] = DeviceArray(buf, 4 * int(self.so[-1]))
+        d["rec"] = DeviceArray(buf, CHAIN_REC_DTYPE.itemsize * n)
+        d["junctions"] = DeviceArray(buf, REFINED_DTYPE.itemsize * slots)
+        d["bounds"] = DeviceArray(buf, 12 * slots)
+        self._cb = _native.CSplicedChainBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr,
+                                              d["contig"].ptr, d["minus"].ptr, d["anchors"].ptr, d["anc_off"].ptr)
+        self._out = _native.CBatchOut(d["ext"].ptr, None, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr,
+                                      None)
+
+    def launch(self, jlib: Optional[JunctionLib], k: int, params: SpliceParams = SpliceParams(),
+               extend_params: Optional[ExtendParams] = None) -> None:
+        d = self.d
+        self._sp, self._ep = params.to_c(), (extend_params or ExtendParams()).to_c()
         _native.check(_native.lib().esp_align_spliced_batch(
-            buf.handle, dref.handle, jlib.handle if jlib else None, ctypes.byref(cb), k, ctypes.byref(sp),
-            ctypes.byref(ep), ctypes.byref(out), ctypes.c_void_p(d["stitched"].ptr),
-            ctypes.c_void_p(d["stitch_off"].ptr), ctypes.c_void_p(d["rec"].ptr),
-            ctypes.c_void_p(d["junctions"].ptr), ctypes.c_void_p(d["bounds"].ptr)), buf.handle)
-        _native.check(_native.lib().esp_sync(buf.handle), buf.handle)
-        rec = d["rec"].to_numpy(np.uint8, CHAIN_REC_DTYPE.itemsize * n).view(CHAIN_REC_DTYPE)
+            self.buf.handle, self.dref.handle, jlib.handle if jlib else None, ctypes.byref(self._cb), k,
+            ctypes.byref(self._sp), ctypes.byref(self._ep), ctypes.byref(self._out),
+            ctypes.c_void_p(d["stitched"].ptr), ctypes.c_void_p(d["stitch_off"].ptr), ctypes.c_void_p(d["rec"].ptr),
+            ctypes.c_void_p(d["junctions"].ptr), ctypes.c_void_p(d["bounds"].ptr)), self.buf.handle)
+
+    def records(self) -> np.ndarray:
+        return self.d["rec"].to_numpy(np.uint8, CHAIN_REC_DTYPE.itemsize * self.npairs).view(CHAIN_REC_DTYPE)
+
+    def results(self) -> List[SplicedChainResult]:
+        d, n, ao, co, so = self.d, self.npairs, self.ao, self.co, self.so
+        rec = self.records()
         e = d["ext"].to_numpy(np.uint32, 4 * n).reshape(n, 4)
         cig = d["cigar"].to_numpy(np.uint32, int(co[-1]))
         cn = d["cig_n"].to_numpy(np.uint32, n)
         st = d["stitched"].to_numpy(np.uint32, int(so[-1]))
-        js = d["junctions"].to_numpy(np.uint8, REFINED_DTYPE.itemsize * slots).view(REFINED_DTYPE)
-        bd = d["bounds"].to_numpy(np.uint32, 3 * slots)
+        js = d["junctions"].to_numpy(np.uint8, REFINED_DTYPE.itemsize * self.slots).view(REFINED_DTYPE)
+        bd = d["bounds"].to_numpy(np.uint32, 3 * self.slots)
         res = []
         for c in range(n):
             r = rec[c]
@@ -219,9 +225,33 @@ def align_spliced_batch(dref: DeviceReference, jlib: Optional[JunctionLib], read
                     x.cigar = cigar_from_words(st[int(so[c]):int(so[c]) + int(r["n_stitched"])])
             res.append(x)
         return res
-    finally:
-        for a in d.values():
+
+    def free(self) -> None:
+        for a in self.d.values():
             a.free()
+
+
+def align_spliced_batch(dref: DeviceReference, jlib: Optional[JunctionLib], reads: Sequence[bytes],
+                        contigs: Sequence[int], minus: Sequence[bool], anchors: Sequence[Sequence[Tuple[int, int]]],
+                        k: int, params: SpliceParams = SpliceParams(),
+                        extend_params: Optional[ExtendParams] = None) -> List[SplicedChainResult]:
+    """align_spliced per chain: anchors[c] is chain c's (qpos, rpos) list in (rpos, qpos) order."""
+    n = len(reads)
+    if n == 0:
+        return []
+    rb, ro, rl = _concat(reads)
+    na = np.array([len(a) for a in anchors], dtype=np.uint64)
+    ao = np.zeros(n + 1, dtype=np.uint64)
+    ao[1:] = np.cumsum(na)
+    flat = np.array([x for a in anchors for pr in a for x in pr], dtype=np.uint32) if int(ao[-1]) else \
+        np.zeros(2, np.uint32)
+    db = DeviceChainBatch(dref, rb, ro, rl, contigs, minus, flat, ao)
+    try:
+        db.launch(jlib, k, params, extend_params)
+        db.sync()
+        return db.results()
+    finally:
+        db.free()
 
 
 @dataclass
@@ -236,11 +266,58 @@ class Reextension:
     cigar: List[CigarOp]  # the new extension's CIGAR (empty when no DP ran)
 
 
+class DeviceReextendBatch(ContextTimings):
+    """reextend_with's placements resident in HBM (esp_reextend_batch_run):
+    queries (concatenated), contig and pos per placement, and the flat BAM
+    CIGAR words with their u64 offsets and u32 counts."""
+
+    def __init__(self, dref: DeviceReference, rb, ro, rl, contigs, positions, cw, cof, cn):
+        self.buf = buf = dref.buf
+        self.dref = dref
+        n = self.npairs = int(rl.shape[0])
+        caps = 2 * rl.astype(np.uint64) + 5
+        self.oo = np.zeros(n + 1, dtype=np.uint64)
+        self.oo[1:] = np.cumsum(caps)
+        self.d = d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
+                      (("reads", rb), ("read_off", ro), ("read_len", rl),
+                       ("contig", np.asarray(contigs, dtype=np.uint32)),
+                       ("pos", np.asarray(positions, dtype=np.uint32)), ("cigar", cw), ("cig_off", cof),
+                       ("cig_n", cn), ("out_off", self.oo))}
+        d["rec"] = DeviceArray(buf, REEXT_REC_DTYPE.itemsize * n)
+        d["out"] = DeviceArray(buf, 4 * int(self.oo[-1]))
+        d["out_n"] = DeviceArray(buf, 4 * n)
+        self._b = _native.CReextendBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr, d["contig"].ptr,
+                                         d["pos"].ptr, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr)
+
+    def launch(self, extend_params: Optional[ExtendParams], edit_dir: int) -> None:
+        d = self.d
+        self._ep = (extend_params or ExtendParams()).to_c()
+        _native.check(_native.lib().esp_reextend_batch_run(
+            self.buf.handle, self.dref.handle, ctypes.byref(self._b), ctypes.byref(self._ep), int(edit_dir),
+            ctypes.c_void_p(d["rec"].ptr), ctypes.c_void_p(d["out"].ptr), ctypes.c_void_p(d["out_off"].ptr),
+            ctypes.c_void_p(d["out_n"].ptr)), self.buf.handle)
+
+    def records(self) -> np.ndarray:
+        return self.d["rec"].to_numpy(np.uint8, REEXT_REC_DTYPE.itemsize * self.npairs).view(REEXT_REC_DTYPE)
+
+    def results(self) -> List[Reextension]:
+        rec = self.records()
+        oo = self.oo
+        out = self.d["out"].to_numpy(np.uint32, int(oo[-1]))
+        on = self.d["out_n"].to_numpy(np.uint32, self.npairs)
+        return [Reextension(int(r["status"]), int(r["pos"]), int(r["score"]), int(r["wstart"]), int(r["wlen"]),
+                            int(r["aligned"]), int(r["new_aligned"]),
+                            cigar_from_words(out[int(oo[k]):int(oo[k]) + int(on[k])])) for k, r in enumerate(rec)]
+
+    def free(self) -> None:
+        for a in self.d.values():
+            a.free()
+
+
 def reextend_batch(dref: DeviceReference, queries: Sequence[bytes], contigs: Sequence[int],
                    positions: Sequence[int], cigars: Sequence[Sequence[CigarOp]],
                    extend_params: Optional[ExtendParams], edit_dir: int) -> List[Reextension]:
     """reextend_with(query, placement, dir) per placement with one dir for the batch."""
-    buf = dref.buf
     n = len(queries)
     if n == 0:
         return []
@@ -250,34 +327,13 @@ def reextend_batch(dref: DeviceReference, queries: Sequence[bytes], contigs: Seq
     cof = np.zeros(n, dtype=np.uint64)
     if n:
         cof[1:] = np.cumsum(cn, dtype=np.uint64)[:-1]
-    caps = 2 * rl.astype(np.uint64) + 5
-    oo = np.zeros(n + 1, dtype=np.uint64)
-    oo[1:] = np.cumsum(caps)
-    d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
-         (("reads", rb), ("read_off", ro), ("read_len", rl), ("contig", np.asarray(contigs, dtype=np.uint32)),
-          ("pos", np.asarray(positions, dtype=np.uint32)),
-          ("cigar", np.array(cw or [0], dtype=np.uint32)), ("cig_off", cof), ("cig_n", cn), ("out_off", oo))}
-    d["rec"] = DeviceArray(buf, REEXT_REC_DTYPE.itemsize * n)
-    d["out"] = DeviceArray(buf, 4 * int(oo[-1]))
-    d["out_n"] = DeviceArray(buf, 4 * n)
+    db = DeviceReextendBatch(dref, rb, ro, rl, contigs, positions, np.array(cw or [0], dtype=np.uint32), cof, cn)
     try:
-        b = _native.CReextendBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr, d["contig"].ptr,
-                                   d["pos"].ptr, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr)
-        ep = (extend_params or ExtendParams()).to_c()
-        _native.check(_native.lib().esp_reextend_batch_run(
-            buf.handle, dref.handle, ctypes.byref(b), ctypes.byref(ep), int(edit_dir),
-            ctypes.c_void_p(d["rec"].ptr), ctypes.c_void_p(d["out"].ptr), ctypes.c_void_p(d["out_off"].ptr),
-            ctypes.c_void_p(d["out_n"].ptr)), buf.handle)
-        _native.check(_native.lib().esp_sync(buf.handle), buf.handle)
-        rec = d["rec"].to_numpy(np.uint8, REEXT_REC_DTYPE.itemsize * n).view(REEXT_REC_DTYPE)
-        out = d["out"].to_numpy(np.uint32, int(oo[-1]))
-        on = d["out_n"].to_numpy(np.uint32, n)
-        return [Reextension(int(r["status"]), int(r["pos"]), int(r["score"]), int(r["wstart"]), int(r["wlen"]),
-                            int(r["aligned"]), int(r["new_aligned"]),
-                            cigar_from_words(out[int(oo[k]):int(oo[k]) + int(on[k])])) for k, r in enumerate(rec)]
+        db.launch(extend_params, edit_dir)
+        db.sync()
+        return db.results()
     finally:
-        for a in d.values():
-            a.free()
+        db.free()
 
 
 IC_REC_DTYPE = np.dtype([("status", "<i4"), ("reason", "<i4"), ("genomic_start", "<u4"), ("score", "<i4"),
@@ -296,47 +352,51 @@ class IntronParams:
     intron_max: int = 105_000
 
 
-def intron_chain_batch(dref: DeviceReference, jlib: Optional[JunctionLib], reads: Sequence[bytes],
-                       anchors: Sequence[Sequence[Tuple[int, int, int, int]]], k: int = 15,
-                       params: IntronParams = IntronParams(), extend_params: Optional[ExtendParams] = None):
-    """try_intron_chain_placement per read; anchors[r] = the read's (qpos, rpos, contig, minus) pool.
-    Returns (records, extensions, stitched CIGARs)."""
-    buf = dref.buf
-    n = len(reads)
-    if n == 0:
-        return np.zeros(0, IC_REC_DTYPE), [], []
-    rb, ro, rl = _concat(reads)
-    na = np.array([len(a) for a in anchors], dtype=np.uint64)
-    ao = np.zeros(n + 1, dtype=np.uint64)
-    ao[1:] = np.cumsum(na)
-    flat = np.array([x for a in anchors for an in a for x in an], dtype=np.uint32) if int(ao[-1]) else \
-        np.zeros(4, np.uint32)
-    caps = 2 * rl.astype(np.uint64) + 5
-    co = np.zeros(n + 1, dtype=np.uint64)
-    co[1:] = np.cumsum(caps)
-    so = np.zeros(n + 1, dtype=np.uint64)
-    so[1:] = np.cumsum(caps + 4)
-    d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
-         (("reads", rb), ("read_off", ro), ("read_len", rl), ("anchors", flat), ("anc_off", ao), ("cig_off", co),
-          ("stitch_off", so))}
-    d["ext"] = DeviceArray(buf, 16 * n)
-    d["cigar"] = DeviceArray(buf, 4 * int(co[-1]))
-    d["cig_n"] = DeviceArray(buf, 4 * n)
-    d["stitched"] = DeviceArray(buf, 4 * int(so[-1]))
-    d["rec"] = DeviceArray(buf, IC_REC_DTYPE.itemsize * n)
-    try:
-        cb = _native.CIntronChainBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr, d["anchors"].ptr,
-                                       d["anc_off"].ptr, int(rl.max()) if n else 1)
-        ip = _native.CIntronParams(params.min_frag_cov, params.refine_window, params.refine_pattern_len,
-                                   params.intron_max)
-        out = _native.CBatchOut(d["ext"].ptr, None, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr, None)
-        ep = (extend_params or ExtendParams()).to_c()
+class DeviceIntronChainBatch(ContextTimings):
+    """try_intron_chain_placement's reads resident in HBM
+    (esp_intron_chain_batch_run): reads (concatenated) and the flat
+    (qpos, rpos, contig, minus) anchor pools with their u64 offsets."""
+
+    def __init__(self, dref: DeviceReference, rb, ro, rl, flat, ao):
+        self.buf = buf = dref.buf
+        self.dref = dref
+        n = self.npairs = int(rl.shape[0])
+        caps = 2 * rl.astype(np.uint64) + 5
+        self.co = np.zeros(n + 1, dtype=np.uint64)
+        self.co[1:] = np.cumsum(caps)
+        self.so = np.zeros(n + 1, dtype=np.uint64)
+        self.so[1:] = np.cumsum(caps + 4)
+        self.d = d = {name: DeviceArray.from_numpy(buf, arr) for name, arr in
+                      (("reads", rb), ("read_off", ro), ("read_len", rl), ("anchors", flat), ("anc_off", ao),
+                       ("cig_off", self.co), ("stitch_off", self.so))}
+        d["ext"] = DeviceArray(buf, 16 * n)
+        d["cigar"] = DeviceArray(buf, 4 * int(self.co[-1]))
+        d["cig_n"] = DeviceArray(buf, 4 * n)
+        d["stitched"] = DeviceArray(buf, 4 * int(self.so[-1]))
+        d["rec"] = DeviceArray(buf, IC_REC_DTYPE.itemsize * n)
+        self._cb = _native.CIntronChainBatch(n, d["reads"].ptr, d["read_off"].ptr, d["read_len"].ptr,
+                                             d["anchors"].ptr, d["anc_off"].ptr, int(rl.max()) if n else 1)
+        self._out = _native.CBatchOut(d["ext"].ptr, None, d["cigar"].ptr, d["cig_off"].ptr, d["cig_n"].ptr,
+                                      None)
+
+    def launch(self, jlib: Optional[JunctionLib], k: int = 15, params: IntronParams = IntronParams(),
+               extend_params: Optional[ExtendParams] = None) -> None:
+        d = self.d
+        self._ip = _native.CIntronParams(params.min_frag_cov, params.refine_window, params.refine_pattern_len,
+                                         params.intron_max)
+        self._ep = (extend_params or ExtendParams()).to_c()
         _native.check(_native.lib().esp_intron_chain_batch_run(
-            buf.handle, dref.handle, jlib.handle if jlib else None, ctypes.byref(cb), k, ctypes.byref(ip),
-            ctypes.byref(ep), ctypes.byref(out), ctypes.c_void_p(d["stitched"].ptr),
-            ctypes.c_void_p(d["stitch_off"].ptr), ctypes.c_void_p(d["rec"].ptr)), buf.handle)
-        _native.check(_native.lib().esp_sync(buf.handle), buf.handle)
-        rec = d["rec"].to_numpy(np.uint8, IC_REC_DTYPE.itemsize * n).view(IC_REC_DTYPE)
+            self.buf.handle, self.dref.handle, jlib.handle if jlib else None, ctypes.byref(self._cb), k,
+            ctypes.byref(self._ip), ctypes.byref(self._ep), ctypes.byref(self._out),
+            ctypes.c_void_p(d["stitched"].ptr), ctypes.c_void_p(d["stitch_off"].ptr),
+            ctypes.c_void_p(d["rec"].ptr)), self.buf.handle)
+
+    def records(self) -> np.ndarray:
+        return self.d["rec"].to_numpy(np.uint8, IC_REC_DTYPE.itemsize * self.npairs).view(IC_REC_DTYPE)
+
+    def results(self):
+        d, n, co, so = self.d, self.npairs, self.co, self.so
+        rec = self.records()
         e = d["ext"].to_numpy(np.uint32, 4 * n).reshape(n, 4)
         cig = d["cigar"].to_numpy(np.uint32, int(co[-1]))
         cn = d["cig_n"].to_numpy(np.uint32, n)
@@ -349,6 +409,30 @@ def intron_chain_batch(dref: DeviceReference, jlib: Optional[JunctionLib], reads
                         if ran else None)
             stitched.append(cigar_from_words(st[int(so[c]):int(so[c]) + int(rec[c]["n_stitched"])]))
         return rec, exts, stitched
-    finally:
-        for a in d.values():
+
+    def free(self) -> None:
+        for a in self.d.values():
             a.free()
+
+
+def intron_chain_batch(dref: DeviceReference, jlib: Optional[JunctionLib], reads: Sequence[bytes],
+                       anchors: Sequence[Sequence[Tuple[int, int, int, int]]], k: int = 15,
+                       params: IntronParams = IntronParams(), extend_params: Optional[ExtendParams] = None):
+    """try_intron_chain_placement per read; anchors[r] = the read's (qpos, rpos, contig, minus) pool.
+    Returns (records, extensions, stitched CIGARs)."""
+    n = len(reads)
+    if n == 0:
+        return np.zeros(0, IC_REC_DTYPE), [], []
+    rb, ro, rl = _concat(reads)
+    na = np.array([len(a) for a in anchors], dtype=np.uint64)
+    ao = np.zeros(n + 1, dtype=np.uint64)
+    ao[1:] = np.cumsum(na)
+    flat = np.array([x for a in anchors for an in a for x in an], dtype=np.uint32) if int(ao[-1]) else \
+        np.zeros(4, np.uint32)
+    db = DeviceIntronChainBatch(dref, rb, ro, rl, flat, ao)
+    try:
+        db.launch(jlib, k, params, extend_params)
+        db.sync()
+        return db.results()
+    finally:
+        db.free()

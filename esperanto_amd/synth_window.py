@@ -151,3 +151,72 @@ def placement_walk_workload(npairs: int, ref_len: int = 64_000_000, read_len: in
     pick = rng.random(npairs) < known_fraction
     junctions = sorted(set(zip([0] * int(pick.sum()), s[pick].tolist(), e[pick].tolist())))
     return ref, pl, junctions
+
+
+def junction_workload(nreads: int, ref_len: int = 64_000_000, read_len: int = 150, k: int = 15,
+                      stride: int = 8, seed: int = 0xE5A00007, known_fraction: float = 0.5,
+                      sub_rate: float = 0.01, clip: int = 12):
+    """Inputs for align_spliced over chains, try_intron_chain_placement and
+    reextend_with on one uniform-ACGT contig of ref_len bases.
+
+    Per read an intron [s, e) of log-uniform length 60..50,000 with GT at
+    [s, s+2) and AG at [e-2, e); split u ~ U{40..110}; read = contig[s-u, s)
+    ++ contig[e, e+m-u) with sub_rate substitutions.  Its chain anchors are
+    the exact k-mers at qpos 0, stride, 2*stride, ... lying wholly on one exon
+    with no substitution in them, in (rpos, qpos) order; its intron-chain pool
+    is the same anchors as (qpos, rpos, contig 0, plus).  known_fraction of
+    the introns go into the junction library (plus strand).  Re-extension
+    placements: a second read = contig[p, p+m) (substituted) at pos p + clip
+    with CIGAR clip S, m - 2 clip M, clip S.  Returns a dict of numpy arrays.
+    """
+    rng = np.random.default_rng(seed)
+    m = read_len
+    ref = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, ref_len)].copy()
+    u = rng.integers(40, 111, nreads)
+    intron = np.exp(rng.uniform(np.log(60), np.log(50_000), nreads)).astype(np.int64)
+    s = rng.integers(2000, ref_len - intron - 2000)
+    e = s + intron
+    ref[s], ref[s + 1] = ord("G"), ord("T")
+    ref[e - 2], ref[e - 1] = ord("A"), ord("G")
+    p2 = rng.integers(2000, ref_len - 2000 - m, nreads)
+    t = np.arange(m, dtype=np.int64)[None, :]
+    r1 = np.empty((nreads, m), dtype=np.uint8)
+    r2 = np.empty((nreads, m), dtype=np.uint8)
+    for lo in range(0, nreads, 65536):
+        hi = min(nreads, lo + 65536)
+        uu = u[lo:hi, None]
+        r1[lo:hi] = ref[np.where(t < uu, s[lo:hi, None] - uu + t, e[lo:hi, None] + t - uu)]
+        r2[lo:hi] = ref[p2[lo:hi, None] + t]
+    hit = rng.random(r1.shape) < sub_rate
+    r1[hit] = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, int(hit.sum()))]
+    _substitute(rng, r2, sub_rate)
+    # anchors: k-mers on one exon without a substituted base (a substitution
+    # may redraw the same base; such k-mers are dropped all the same)
+    q = np.arange(0, m - k + 1, stride, dtype=np.int64)[None, :]
+    bad = np.concatenate([np.zeros((nreads, 1), np.int64), np.cumsum(hit, axis=1)], axis=1)
+    clean = (bad[:, q[0] + k] - bad[:, q[0]]) == 0
+    one_exon = (q + k <= u[:, None]) | (q >= u[:, None])
+    keep = clean & one_exon
+    rpos = np.where(q < u[:, None], s[:, None] - u[:, None] + q, e[:, None] + q - u[:, None])
+    na = keep.sum(axis=1).astype(np.uint64)
+    ao = np.zeros(nreads + 1, dtype=np.uint64)
+    ao[1:] = np.cumsum(na)
+    qq = np.broadcast_to(q, keep.shape)[keep].astype(np.uint32)
+    rr = rpos[keep].astype(np.uint32)
+    chain_flat = np.stack([qq, rr], axis=1).reshape(-1)
+    zeros = np.zeros_like(qq)
+    pool_flat = np.stack([qq, rr, zeros, zeros], axis=1).reshape(-1)
+    known = rng.random(nreads) < known_fraction
+    quads = np.stack([np.zeros(int(known.sum()), np.uint32), s[known].astype(np.uint32),
+                      e[known].astype(np.uint32), np.zeros(int(known.sum()), np.uint32)], axis=1)
+    rl = np.full(nreads, m, dtype=np.uint32)
+    ro = np.arange(nreads, dtype=np.uint64) * m
+    words = np.array([(clip << 4) | 4, ((m - 2 * clip) << 4) | 0, (clip << 4) | 4], dtype=np.uint32)
+    return {"reference": Reference.from_sequences([ref.tobytes()]),
+            "reads": r1.reshape(-1), "read_off": ro, "read_len": rl,
+            "contig": np.zeros(nreads, np.uint32), "minus": np.zeros(nreads, np.uint32),
+            "chain_anchors": chain_flat, "chain_off": ao, "pool_anchors": pool_flat, "pool_off": ao.copy(),
+            "quads": quads, "s": s, "e": e, "u": u,
+            "rx_reads": r2.reshape(-1), "rx_pos": (p2 + clip).astype(np.uint32),
+            "rx_cigar": np.tile(words, nreads), "rx_cig_off": np.arange(nreads, dtype=np.uint64) * 3,
+            "rx_cig_n": np.full(nreads, 3, dtype=np.uint32), "k": k}

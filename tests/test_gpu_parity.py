@@ -141,6 +141,75 @@ def test_band_kernel_interior_sweep(buf):
     compare_batch(buf, group, 0, wide, label="band_interior_int32")
 
 
+def _left_clamped_cases():
+    """Bands with 0 <= offset < half (clamped on the left only): reads placed
+    at the window's first columns, on the band edge and inside it, with
+    substitutions and indels, random reads (zero rows), lower case, N."""
+    from seqgen import Rng, rand_seq
+    rng = Rng(0x1EF7)
+    out = []
+    for half in (16, 30, 32, 48, 64):
+        for off in sorted({0, 1, half // 2, half - 1}):
+            for m in (1, 20, 64, 65, 150, 200):
+                for place in (0, 3, off, off + half // 2):
+                    n = off + m + half + int(rng.below(40))
+                    w = bytearray(rand_seq(rng, n))
+                    read = bytearray(w[place:place + m]) if place + m <= n else bytearray(rand_seq(rng, m))
+                    for _ in range(int(rng.below(4))):
+                        q = int(rng.below(max(m, 1)))
+                        read[q] = b"ACGT"[int(rng.below(4))]
+                    if m > 30 and rng.below(3) == 0:  # an indel of 1-3 bases
+                        q, d = int(rng.below(m - 10)) + 5, int(rng.below(3)) + 1
+                        read = read[:q] + read[q + d:] + bytearray(rand_seq(rng, d))
+                    if rng.below(6) == 0:
+                        read = bytearray(rand_seq(rng, m))
+                    if rng.below(8) == 0:
+                        read[int(rng.below(m))] = ord("N")
+                    if rng.below(8) == 0:
+                        read = bytearray(bytes(read).lower())
+                    out.append((bytes(read), bytes(w), int(rng.below(3)), off, half))
+    return out
+
+
+def test_left_clamped_bands_take_the_band_kernel(buf):
+    """Left-clamped bands (0 <= offset < half) run in the band kernel: the
+    columns left of the window are 'N' columns holding H = 0 and the left
+    touch is the first maximum at or before the first real cell.  Routing
+    checked, records, CIGARs and TouchRec bit-exact against the oracle; in
+    extend_hint mode (hint = offset, half 30) the kept/escalated path too;
+    with a positive mismatch score they keep the strip kernel."""
+    from esperanto_amd.batch import DeviceBatch, PairBatch
+    cases = _left_clamped_cases()
+    span = {16: 320, 30: 320, 32: 320, 48: 352, 64: 384}
+    for d, group in _group_by_dir(cases).items():
+        hb = PairBatch.from_pairs([(c[0], c[1], c[3], c[4]) for c in group])
+        db = DeviceBatch(hb, buf)
+        db.launch_gotoh(ExtendParams(edit_dir=EditDir(d)), 0)
+        db.sync()
+        st = db.last_plan_stats()
+        want = {h: sum(1 for c in group if c[4] == h and 0 < len(c[0]) <= span[h] - 2 * h - 1
+                       and c[3] + len(c[0]) + h <= len(c[1])) for h in (16, 30, 32, 48, 64)}
+        say(f"left-clamped dir={d} routing {st}")
+        assert st["band_pairs"] == want, (st, want)
+        db.free()
+    compare_batch(buf, cases, 0, label="left_clamped")
+    compare_batch(buf, cases, 0, ExtendParams(match_score=1, mismatch=-3, gap_open=5, gap_ext=2),
+                  label="left_clamped_custom")
+    compare_batch(buf, cases, 0, ExtendParams(gap_open=40, gap_ext=3), label="left_clamped_unfolded")
+    hint = [(r, w, d, off, 30) for r, w, d, off, half in cases if half == 30 or off < 30]
+    compare_batch(buf, hint, 2, label="left_clamped_hint")
+    pos = ExtendParams(match_score=2, mismatch=1, gap_open=4, gap_ext=2)
+    group = [c for c in cases if c[2] == 0]
+    hb = PairBatch.from_pairs([(c[0], c[1], c[3], c[4]) for c in group])
+    db = DeviceBatch(hb, buf)
+    db.launch_gotoh(pos, 0)
+    db.sync()
+    st = db.last_plan_stats()
+    assert sum(st["band_pairs"].values()) == 0, st
+    db.free()
+    compare_batch(buf, group, 0, pos, label="left_clamped_positive_mismatch")
+
+
 def test_long_windows(buf):
     _, lf = sweeps.full_sweep(include_long=True)
     _, lb = sweeps.band_sweep(include_long=True)
