@@ -188,6 +188,37 @@ def bench_window(args, buf, params):
     return out
 
 
+def bench_plain(args, buf, params, c1=None):
+    """The plain one-launch-per-call form (esp_band, one wavefront per pair)
+    on the first C1 pairs: calls/s with each call's copies and sync, then each
+    call's HIP-event kernel time (a second pass, so that the read-back stays
+    out of the wall-clock rate), GCUPS and the fraction of one wave's issue
+    ceiling."""
+    from esperanto_amd.batch import band_cells as one_band_cells, workload_c1
+    from esperanto_amd.extend import band as plain_band
+    if c1 is None:
+        c1 = workload_c1(10_000)
+    ncalls = min(args.plain_calls, c1.npairs)
+    pairs = [c1.pair(k) for k in range(ncalls)]
+    plain_band(pairs[0][0], pairs[0][1], params, (32, 32), buf)
+    t0 = time.perf_counter()
+    for r, w in pairs:
+        plain_band(r, w, params, (32, 32), buf)
+    pps = ncalls / (time.perf_counter() - t0)
+    kms = []
+    for r, w in pairs:
+        plain_band(r, w, params, (32, 32), buf)
+        kms.append(buf.last_kernel_ms())
+    cells = sum(one_band_cells(len(r), len(w), 32, 32) for r, w in pairs)
+    ks = float(np.sum(kms)) / 1e3
+    tops = cells * GOTOH_OPS_PER_CELL / ks / 1e12
+    kern = {"kernel": "k_gotoh_wave", "calls": ncalls, "us_per_call": round(1e6 * ks / ncalls, 2),
+            "gcups": round(cells / ks / 1e9, 4), "roofline_frac_chip": round(tops / VALU_PEAK_TOPS, 7),
+            "roofline_frac_one_wave": round(tops / ONE_WAVE_PEAK_TOPS, 4)}
+    log(f"plain form: {pps:.0f} pairs/s, kernel {kern['us_per_call']} us/call, {kern['gcups']} GCUPS")
+    return {"pairs_per_s": round(pps), "kernel": kern}
+
+
 def bench_junction(args, buf, params):
     """align_spliced over chains (segment_chain, refine_junction_read per seam,
     the guards, then the spliced pipeline), try_intron_chain_placement (MEMs,
@@ -588,6 +619,9 @@ def main() -> int:
     ap.add_argument("--junction-check", type=int, default=2000,
                     help="reads of that leg checked against the oracle and timed on the CPU")
     ap.add_argument("--only-junction", action="store_true", help="run only the junction leg")
+    ap.add_argument("--only-plain", action="store_true",
+                    help="run only the plain one-launch-per-call form on the 10k C1 pairs")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph replay of the Gotoh batch")
     ap.add_argument("--only-myers", action="store_true", help="run only the batched Myers leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
@@ -625,6 +659,10 @@ def main() -> int:
         return 0
     if args.only_walk:
         print(json.dumps(bench_walk(args, buf, params)), flush=True)
+        buf.close()
+        return 0
+    if args.only_plain:
+        print(json.dumps(bench_plain(args, buf, params)), flush=True)
         buf.close()
         return 0
     if args.only_junction:
@@ -671,11 +709,13 @@ def main() -> int:
         f"band {band_ms:.2f} {per_half}, strip {strip_ms:.3f}), {gcups:.1f} GCUPS")
     pairs_per_s = hb.npairs / wall
     # The same batch captured once as a HIP graph and replayed.
-    g3 = db.capture_gotoh(params, 0)
-    graph_wall = _graph_steps(g3, db, args.steps, args.warmup)
-    g3.check()
-    g3.destroy()
-    log(f"gotoh batched, graph replay: {graph_wall*1e3:.2f} ms/step, {cells / graph_wall / 1e9:.1f} GCUPS")
+    graph_wall = float("nan")
+    if not args.no_graph:
+        g3 = db.capture_gotoh(params, 0)
+        graph_wall = _graph_steps(g3, db, args.steps, args.warmup)
+        g3.check()
+        g3.destroy()
+        log(f"gotoh batched, graph replay: {graph_wall*1e3:.2f} ms/step, {cells / graph_wall / 1e9:.1f} GCUPS")
     if args.copy_bytes:
         _copy_calibration(buf, db.d["wins"], args.copy_bytes)
     db.free()
@@ -697,27 +737,8 @@ def main() -> int:
     c1_graph_pps = c1.npairs / w1g
     log(f"c1 graph replay: {c1_graph_pps:.0f} pairs/s ({w1g * 1e3:.3f} ms/step)")
     d1.free()
-    ncalls = min(args.plain_calls, c1.npairs)
-    pairs = [c1.pair(k) for k in range(ncalls)]
-    plain_band(pairs[0][0], pairs[0][1], params, (32, 32), buf)
-    t0 = time.perf_counter()
-    for r, w in pairs:
-        plain_band(r, w, params, (32, 32), buf)
-    plain_pps = ncalls / (time.perf_counter() - t0)
-    # The plain form's kernel alone: each call's HIP-event kernel time (a
-    # second pass, so that the read-back stays out of the wall-clock rate).
-    kms_plain = []
-    for r, w in pairs:
-        plain_band(r, w, params, (32, 32), buf)
-        kms_plain.append(buf.last_kernel_ms())
-    from esperanto_amd.batch import band_cells as one_band_cells
-    plain_cells = sum(one_band_cells(len(r), len(w), 32, 32) for r, w in pairs)
-    plain_kernel_s = float(np.sum(kms_plain)) / 1e3
-    plain_kernel_gcups = plain_cells / plain_kernel_s / 1e9
-    plain_tops = plain_cells * GOTOH_OPS_PER_CELL / plain_kernel_s / 1e12
-    log(f"c1: batched {c1_batched_pps:.0f} pairs/s, plain {plain_pps:.0f} pairs/s, plain kernel "
-        f"{1e6 * plain_kernel_s / ncalls:.1f} us/call, {plain_kernel_gcups:.3f} GCUPS")
-
+    pl = bench_plain(args, buf, params, c1)
+    plain_pps = pl["pairs_per_s"]
     myers_pps, k2, myers_tops, myers_plain_pps = bench_myers(args, buf)
 
     # Splice + mate-rescue window pipelines on a 64 Mb reference.
@@ -759,18 +780,13 @@ def main() -> int:
                    "band_kernel_ms": round(band_ms, 3), "band_kernel_ms_per_half": per_half,
                    "kernel_gcups": round(kernel_gcups, 2),
                    "autotune": {str(h): v[0] for h, v in tune.items()} if tune else None,
-                   "graph_ms_per_step": round(graph_wall * 1e3, 3),
-                   "graph_gcups": round(cells / graph_wall / 1e9, 2),
+                   "graph_ms_per_step": None if args.no_graph else round(graph_wall * 1e3, 3),
+                   "graph_gcups": None if args.no_graph else round(cells / graph_wall / 1e9, 2),
                    "c1_batched_pairs_per_s": round(c1_batched_pps),
                    "c1_graph_pairs_per_s": round(c1_graph_pps),
                    "c1_plain_pairs_per_s": round(plain_pps),
                    "batched_over_plain": round(c1_batched_pps / plain_pps, 1),
-                   "c1_plain_kernel": {
-                       "kernel": "k_gotoh_wave", "calls": ncalls,
-                       "us_per_call": round(1e6 * plain_kernel_s / ncalls, 2),
-                       "gcups": round(plain_kernel_gcups, 4),
-                       "roofline_frac_chip": round(plain_tops / VALU_PEAK_TOPS, 7),
-                       "roofline_frac_one_wave": round(plain_tops / ONE_WAVE_PEAK_TOPS, 4)},
+                   "c1_plain_kernel": pl["kernel"],
                    "myers_pairs_per_s": round(myers_pps),
                    "myers_kernel_ms": round(k2, 3),
                    "myers_valu_tops": round(myers_tops, 2),

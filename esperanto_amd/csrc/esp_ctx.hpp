@@ -79,10 +79,10 @@ struct CaptureBounds {
 // and swaps it into its context while it records.
 struct ScratchSet {
     DevBuf counter, rows, trace, btrace, ftrace, plan, order, escbuf, censusbuf;
-    DevBuf wquery, warena, wdesc, wmyers, eng, jscratch, jarena, mlist;
+    DevBuf wquery, warena, wdesc, wmyers, eng, jscratch, jarena, mlist, scantmp;
     void release() {
         for (DevBuf *b : {&counter, &rows, &trace, &btrace, &ftrace, &plan, &order, &escbuf, &censusbuf, &wquery,
-                          &warena, &wdesc, &wmyers, &eng, &jscratch, &jarena, &mlist})
+                          &warena, &wdesc, &wmyers, &eng, &jscratch, &jarena, &mlist, &scantmp})
             b->release();
     }
 };
@@ -124,6 +124,8 @@ struct esp_ctx {
     esph::DevBuf jscratch, jarena;
     // EA-Myers batch: pairs longer than the caller's bound, for the 8-word launch
     esph::DevBuf mlist;
+    // tile sums of the prefix scans that lay out the junction-side arenas
+    esph::DevBuf scantmp;
     hipEvent_t ev_plan = nullptr;
     int band_blocks[espk::PLAN_STRIP] = {};
     esph::HostBuf hstage, hout, hplan;
@@ -222,6 +224,14 @@ inline int check_lengths(esp_ctx *c, uint64_t m, uint64_t n) {
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// espk::launch_scan2 on the context's scan scratch.
+inline int scan2(esp_ctx *c, const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off,
+                 uint64_t *len_off) {
+    HIP_TRY(c, c->scantmp.ensure(espk::scan2_tmp_bytes(n)));
+    HIP_TRY(c, espk::launch_scan2(cnt, len, n, cnt_off, len_off, (uint64_t *)c->scantmp.p, c->stream));
+    return ESP_OK;
+}
+
 // Window stride of the mate-rescue pipeline for reads of at most maxm
 // bases: its windows span 2 * 750 + m bases at most.
 inline uint64_t mate_window_stride(uint64_t maxm) { return align_up(1500 + 3 * maxm, 16); }
@@ -249,6 +259,7 @@ inline void swap_scratch(esp_ctx *c, ScratchSet &s) {
     std::swap(c->jscratch, s.jscratch);
     std::swap(c->jarena, s.jarena);
     std::swap(c->mlist, s.mlist);
+    std::swap(c->scantmp, s.scantmp);
 }
 
 // Launch timing: mark() before the first launch and after every launch of

@@ -455,6 +455,7 @@ int esp_destroy(esp_ctx *c) {
     c->jscratch.release();
     c->jarena.release();
     c->mlist.release();
+    c->scantmp.release();
     c->escbuf.release();
     c->hstage.release();
     c->hout.release();

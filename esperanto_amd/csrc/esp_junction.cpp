@@ -159,7 +159,8 @@ int esp_align_spliced_batch(esp_ctx *c, const esp_ref *r, const esp_junctions *j
     HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_chain_prep(a, c->stream));
-    HIP_TRY(c, espk::launch_scan2(a.nex, a.plen, n, exon_off, pseudo_off, c->stream));
+    rc = scan2(c, a.nex, a.plen, n, exon_off, pseudo_off);
+    if (rc) return rc;
     tim_mark(c, 5, 0);
     uint64_t nex = 0, pbytes = 0;
     rc = read_totals(c, exon_off + n, pseudo_off + n, nex, pbytes);
@@ -230,7 +231,8 @@ int esp_reextend_batch_run(esp_ctx *c, const esp_ref *r, const esp_reextend_batc
     HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_reext_prep(a, c->stream));
-    HIP_TRY(c, espk::launch_scan2(a.wlen, nullptr, n, win_off, nullptr, c->stream));
+    rc = scan2(c, a.wlen, nullptr, n, win_off, nullptr);
+    if (rc) return rc;
     uint64_t wbytes = 0, unused = 0;
     rc = read_totals(c, win_off + n, nullptr, wbytes, unused);
     if (rc) return rc;
@@ -362,7 +364,8 @@ int esp_intron_chain_batch_run(esp_ctx *c, const esp_ref *r, const esp_junctions
     };
     // Texts of the current jobs: offsets, one read-back of the total, decode.
     auto gather = [&]() -> int {
-        HIP_TRY(c, espk::launch_scan2(a.t_len, nullptr, nj, t_off, nullptr, c->stream));
+        int r1 = scan2(c, a.t_len, nullptr, nj, t_off, nullptr);
+        if (r1) return r1;
         uint64_t tbytes = 0, u = 0;
         int r2 = read_totals(c, t_off + nj, nullptr, tbytes, u);
         if (r2) return r2;
@@ -390,7 +393,8 @@ int esp_intron_chain_batch_run(esp_ctx *c, const esp_ref *r, const esp_junctions
     if (rc) return rc;
     tim_mark(c, 3, 0);
     HIP_TRY(c, espk::launch_ic(a, 3, c->stream));
-    HIP_TRY(c, espk::launch_scan2(nullptr, a.plen, n, nullptr, pseudo_off, c->stream));
+    rc = scan2(c, nullptr, a.plen, n, nullptr, pseudo_off);
+    if (rc) return rc;
     uint64_t pbytes = 0, u2 = 0;
     rc = read_totals(c, pseudo_off + n, nullptr, pbytes, u2);
     if (rc) return rc;

@@ -16,6 +16,8 @@
 // (at most 25 boundary shifts of 8 base comparisons; a chain's anchors), so
 // the cost is a few memory round trips per item; window decodes run a wave
 // per window as in the window pipelines.
+#include <algorithm>
+
 #include "esp_kernels.hpp"
 #include "esp_refdev.hpp"
 
@@ -773,41 +775,99 @@ __global__ __launch_bounds__(JBLOCK) void k_ic_finish(IcArgs a) {
     a.rec[k] = r;
 }
 
-// Exclusive scans of up to two arrays in one block: each thread sums a
-// contiguous chunk, the block scans the chunk sums, each thread writes its
-// chunk's prefixes; thread 0 of the last chunk writes the totals at [n].
-__global__ __launch_bounds__(1024) void k_scan2(const uint32_t *cnt, const uint64_t *len, uint64_t n,
-                                                uint64_t *cnt_off, uint64_t *len_off) {
-    __shared__ uint64_t sc[1024], sl[1024];
+// Exclusive scans of up to two arrays (u32 counts, u64 lengths) in three
+// launches: per-tile sums, one block scanning the tile sums (and writing the
+// totals at [n]), then each tile's own scan from its offset.  A tile is
+// SCAN_TILE elements, SCAN_PER consecutive ones per thread.
+constexpr uint32_t SCAN_THREADS = 256, SCAN_PER = 16, SCAN_TILE = SCAN_THREADS * SCAN_PER;
+
+// Block-wide exclusive scan of one u64 per thread (all SCAN_THREADS threads).
+__device__ uint64_t block_excl_scan(uint64_t v, uint64_t *sh, uint64_t &total) {
     const uint32_t t = threadIdx.x;
-    const uint64_t chunk = (n + 1023) / 1024;
-    const uint64_t b = min((uint64_t)t * chunk, n), e = min(b + chunk, n);
-    uint64_t c = 0, l = 0;
-    for (uint64_t i = b; i < e; i++) {
-        if (cnt) c += cnt[i];
-        if (len) l += len[i];
-    }
-    sc[t] = c;
-    sl[t] = l;
+    sh[t] = v;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint64_t vc = t >= off ? sc[t - off] : 0ull, vl = t >= off ? sl[t - off] : 0ull;
+    for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1) {
+        const uint64_t x = t >= off ? sh[t - off] : 0ull;
         __syncthreads();
-        sc[t] += vc;
-        sl[t] += vl;
+        sh[t] += x;
         __syncthreads();
     }
-    c = sc[t] - c;
-    l = sl[t] - l;
-    for (uint64_t i = b; i < e; i++) {
-        if (cnt_off) cnt_off[i] = c;
-        if (len_off) len_off[i] = l;
-        if (cnt) c += cnt[i];
-        if (len) l += len[i];
+    total = sh[SCAN_THREADS - 1];
+    const uint64_t r = sh[t] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const uint32_t *cnt, const uint64_t *len, uint64_t n,
+                                                            uint64_t *tmp) {
+    __shared__ uint64_t sh[SCAN_THREADS];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t c = 0, l = 0;
+    for (uint32_t j = 0; j < SCAN_PER; j++) {  // coalesced: consecutive threads, consecutive elements
+        const uint64_t i = base + (uint64_t)j * SCAN_THREADS + threadIdx.x;
+        if (i < n) {
+            if (cnt) c += cnt[i];
+            if (len) l += len[i];
+        }
     }
-    if (t == 1023) {
-        if (cnt_off) cnt_off[n] = sc[1023];
-        if (len_off) len_off[n] = sl[1023];
+    uint64_t tc, tl;
+    (void)block_excl_scan(c, sh, tc);
+    (void)block_excl_scan(l, sh, tl);
+    if (threadIdx.x == 0) {
+        tmp[2 * blockIdx.x] = tc;
+        tmp[2 * blockIdx.x + 1] = tl;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint64_t ntiles, uint64_t n, uint64_t *tmp,
+                                                          uint64_t *cnt_off, uint64_t *len_off) {
+    __shared__ uint64_t sh[SCAN_THREADS];
+    uint64_t carry_c = 0, carry_l = 0;
+    for (uint64_t b = 0; b < ntiles; b += SCAN_THREADS) {
+        const uint64_t i = b + threadIdx.x;
+        const uint64_t c = i < ntiles ? tmp[2 * i] : 0ull, l = i < ntiles ? tmp[2 * i + 1] : 0ull;
+        uint64_t tc, tl;
+        const uint64_t ec = block_excl_scan(c, sh, tc), el = block_excl_scan(l, sh, tl);
+        if (i < ntiles) {
+            tmp[2 * i] = carry_c + ec;
+            tmp[2 * i + 1] = carry_l + el;
+        }
+        carry_c += tc;
+        carry_l += tl;
+    }
+    if (threadIdx.x == 0) {
+        if (cnt_off) cnt_off[n] = carry_c;
+        if (len_off) len_off[n] = carry_l;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t *cnt, const uint64_t *len, uint64_t n,
+                                                            const uint64_t *tmp, uint64_t *cnt_off,
+                                                            uint64_t *len_off) {
+    __shared__ uint64_t sh[SCAN_THREADS];
+    const uint64_t b = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
+    uint64_t vc[SCAN_PER], vl[SCAN_PER];
+    uint64_t c = 0, l = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_PER; j++) {
+        const uint64_t i = b + j;
+        vc[j] = (cnt && i < n) ? cnt[i] : 0ull;
+        vl[j] = (len && i < n) ? len[i] : 0ull;
+        c += vc[j];
+        l += vl[j];
+    }
+    uint64_t tc, tl;
+    c = block_excl_scan(c, sh, tc) + tmp[2 * blockIdx.x];
+    l = block_excl_scan(l, sh, tl) + tmp[2 * blockIdx.x + 1];
+#pragma unroll
+    for (uint32_t j = 0; j < SCAN_PER; j++) {
+        const uint64_t i = b + j;
+        if (i < n) {
+            if (cnt_off) cnt_off[i] = c;
+            if (len_off) len_off[i] = l;
+        }
+        c += vc[j];
+        l += vl[j];
     }
 }
 
@@ -860,9 +920,15 @@ hipError_t launch_ic(const IcArgs &a, int stage, hipStream_t s) {
     return hipGetLastError();
 }
 
+size_t scan2_tmp_bytes(size_t n) { return 16 * std::max<size_t>((n + SCAN_TILE - 1) / SCAN_TILE, 1); }
+
 hipError_t launch_scan2(const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off, uint64_t *len_off,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, cnt, len, (uint64_t)n, cnt_off, len_off);
+                        uint64_t *tmp, hipStream_t s) {
+    const uint64_t ntiles = std::max<uint64_t>((n + SCAN_TILE - 1) / SCAN_TILE, 1);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0, s, cnt, len, (uint64_t)n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, ntiles, (uint64_t)n, tmp, cnt_off, len_off);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0, s, cnt, len, (uint64_t)n, tmp,
+                       cnt_off, len_off);
     return hipGetLastError();
 }
 

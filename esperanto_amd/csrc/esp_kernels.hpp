@@ -562,10 +562,11 @@ struct IcArgs {
 };
 hipError_t launch_ic(const IcArgs &a, int stage, hipStream_t s);
 
-// Exclusive prefix sums of up to two per-item arrays (either may be NULL)
-// into n + 1 entries each; the last entry is the total.
-hipError_t launch_scan2(const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off,
-                        uint64_t *len_off, hipStream_t s);
+// Exclusive prefix sums (totals at [n]) of u32 counts and u64 lengths, either
+// may be null; tmp holds scan2_tmp_bytes(n) of device scratch.
+size_t scan2_tmp_bytes(size_t n);
+hipError_t launch_scan2(const uint32_t *cnt, const uint64_t *len, size_t n, uint64_t *cnt_off, uint64_t *len_off,
+                        uint64_t *tmp, hipStream_t s);
 
 hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
 hipError_t launch_winmyers_prep(const WinMyersArgs &a, hipStream_t s);
