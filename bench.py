@@ -120,7 +120,7 @@ def bench_window(args, buf, params):
             for kind, half, ms in st:
                 key = {0: "plan", 1: f"band{half}", 2: "strip", 3: "myers", 5: "prep",
                        6: "select_finish", 7: "escalated_full", 11: "escalated_unpruned",
-                       12: "escalation_sort", 13: "escalated_pruned_b"}.get(kind, str(kind))
+                       12: "escalation_sort", 13: "escalated_wave"}.get(kind, str(kind))
                 stages.setdefault(key, []).append(ms)
         nsteps = len(tims)
         stages = {k2: round(float(np.sum(v)) / nsteps, 3) for k2, v in stages.items()}
@@ -136,7 +136,7 @@ def bench_window(args, buf, params):
         # timed full-matrix stages (sort, pruned passes, unpruned pass) is the
         # stage's GCUPS and roofline fraction.
         esc_ms = sum(stages.get(k, 0.0) for k in ("escalation_sort", "escalated_full",
-                                                    "escalated_pruned_b", "escalated_unpruned"))
+                                                    "escalated_wave", "escalated_unpruned"))
         if esc_ms and ps["escalated"]:
             from esperanto_amd import _native
             buf.set_option(_native.ESP_OPT_FULL_CENSUS, 1)
@@ -250,7 +250,7 @@ def bench_junction(args, buf, params):
     )
     out = {"reads": n, "reference_bases": int(args.window_ref_mb * 1_000_000)}
     kinds = {0: "plan", 1: "band30", 2: "strip", 3: "myers", 5: "prep", 6: "select_finish",
-             7: "escalated_full", 11: "escalated_unpruned", 12: "escalation_sort", 13: "escalated_pruned_b"}
+             7: "escalated_full", 11: "escalated_unpruned", 12: "escalation_sort", 13: "escalated_wave"}
     for name, db, launch in legs:
         wall, kms, tims = timed_steps(lambda: launch(db), db, max(args.steps, 3), 1)
         stages = {}

@@ -205,7 +205,8 @@ class ContextTimings:
     def last_plan_stats(self) -> dict:
         """Routing of the last batch: pairs per band half, strip pairs, escalations,
         and of those the pairs the pruned full-matrix passes finished, the pairs
-        the unpruned kernel ran and the pairs the second pruned pass took; with
+        the unpruned kernel ran and the pairs the pruned pass could not prove
+        (rerun one wavefront per pair); with
         the census option on, the full-matrix passes' computed cells, issued
         lane-cells and the cells of the full matrices they stand for."""
         st = _native.CPlanStats()
@@ -225,7 +226,8 @@ class ContextTimings:
         rerun of escalated extend_hint attempts (the first pruned pass when
         pruning is on), 8 placement walks, 9 fast lane, 10 a whole graph launch,
         11 the unpruned pass over what the pruned passes could not prove, 12 the
-        sort of the escalation list, 13 the second pruned pass."""
+        sort of the escalation list, 13 the one-wavefront-per-pair rerun of what
+        the pruned pass could not prove."""
         arr = (_native.CTiming * 16)()
         n = ctypes.c_size_t()
         _native.check(_native.lib().esp_last_timings(self.buf.handle, arr, 16, ctypes.byref(n)),

@@ -97,6 +97,7 @@ struct esp_ctx {
     std::string err;
     esph::DevBuf counter, rows, trace, btrace, ftrace, stage, stage_out, plan, order;
     int full_blocks[512] = {};  // occupancy of the full-matrix kernel per row tile
+    int wave_blocks = 0;        // occupancy of the one-pair-per-wave list kernel
     // Trace scratch caps per launch (grids shrink to fit); a server lowers them
     // so that several contexts share one device-memory budget.
     size_t trace_budget = esph::kTraceBudget;
@@ -110,9 +111,8 @@ struct esp_ctx {
     int census = 0;
     esph::DevBuf censusbuf;
     bool last_census = false;
-    // stripe half-width and head pad of the two pruned passes
+    // stripe half-width and head pad of the pruned pass
     int prune_half_a = espk::ESC_HALF_A, prune_pad_a = espk::ESC_PAD_A;
-    int prune_half_b = espk::ESC_HALF_B, prune_pad_b = espk::ESC_PAD_B;
     // pruned reruns: EscDev, per-pair band results, the sorted and fallback lists
     esph::DevBuf escbuf;
     // window pipelines: query arena, window / pseudo-reference arena,

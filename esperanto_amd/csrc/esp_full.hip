@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256) void k_esc_scatter(EscSortArgs a) {
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < ESC_NB; b += blockDim.x)
         if (h[b]) h[b] = atomicAdd(&a.esc->cursor[b], h[b]);
-    if (threadIdx.x == 0) fb_base = fb_count ? atomicAdd(&a.esc->fb2range[1], fb_count) : 0u;
+    if (threadIdx.x == 0) fb_base = fb_count ? atomicAdd(&a.esc->fbrange[1], fb_count) : 0u;
     __syncthreads();
     for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
         const uint32_t p = a.list[t];
@@ -612,7 +612,7 @@ __global__ __launch_bounds__(256) void k_esc_scatter(EscSortArgs a) {
         if (k < ESC_NB)
             a.plist[atomicAdd(&h[k], 1u)] = p;
         else
-            a.fb2list[atomicAdd(&fb_base, 1u)] = p;
+            a.fblist[atomicAdd(&fb_base, 1u)] = p;
     }
 }
 

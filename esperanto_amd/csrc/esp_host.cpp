@@ -324,11 +324,17 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         if (!c->full_blocks[rows & 511]) c->full_blocks[rows & 511] = espk::gotoh_full_blocks_per_cu(rows);
         const uint64_t resident = (uint64_t)c->ncu * c->full_blocks[rows & 511];
         const uint64_t cap = std::max<uint64_t>(1, c->full_trace_budget / (maxw * 64 * 4));
-        // 64 pairs a wave; pass B (one pair a wave) may fill the chip on fewer pairs
+        // 64 pairs a wave
         const uint64_t blocks = std::max<uint64_t>(1, std::min({resident, (bound + 63) / 64, cap}));
-        const uint64_t blocks_b = std::max<uint64_t>(1, std::min({resident, bound, cap}));
         const bool pruned = prune && rows <= espk::FULL_PRUNE_MAX_ROWS;
-        HIP_TRY(c, c->ftrace.ensure(maxw * 64 * 4 * (pruned ? std::max(blocks, blocks_b) : blocks)));
+        // the second pass: one wave per pair the pruned pass could not prove,
+        // each block with the full-matrix trace of the longest pair
+        const uint64_t wave_w = esc_maxm * ((esc_maxn + 7) / 8);
+        if (!c->wave_blocks) c->wave_blocks = espk::gotoh_wave_blocks_per_cu();
+        const uint64_t blocks_w = std::max<uint64_t>(
+            1, std::min<uint64_t>({(uint64_t)c->ncu * c->wave_blocks, bound,
+                                   std::max<uint64_t>(1, c->full_trace_budget / (std::max<uint64_t>(wave_w, 1) * 4))}));
+        HIP_TRY(c, c->ftrace.ensure(std::max(maxw * 64 * 4 * blocks, pruned ? wave_w * 4 * blocks_w : 0)));
         f.trace = (uint32_t *)c->ftrace.p;
         f.maxw = (uint32_t)maxw;
         if (c->census && !c->cap) {
@@ -348,7 +354,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             sa.pad = c->prune_pad_a;
             sa.esc = esc;
             sa.plist = plist;
-            sa.fb2list = fb2list;
+            sa.fblist = fblist;
             sa.win_len = b->win_len;
             sa.census = f.census;
             HIP_TRY(c, espk::launch_esc_sort(sa, (uint32_t)bound, c->stream));
@@ -367,16 +373,21 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             g.census_first = 0;  // the sort counted the matrices
             HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
             tim_mark(c, 7, rows);
-            g.order = fblist;
-            g.range = esc->fbrange;
-            g.counter = &esc->fbcounter;
-            g.prune_half = c->prune_half_b;
-            g.prune_pad = c->prune_pad_b;
-            g.fb_list = fb2list;
-            g.fb_range = esc->fb2range;
-            g.unit = 1;
-            g.census_first = 0;
-            HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks_b, c->stream));
+            // What the pass could not prove, and the pairs the sort found no
+            // pass could help: one wave per pair over the whole matrix, up to
+            // the list cap (windows over the wave kernel's limit, and the
+            // rest of a long list, go on to the unpruned lane kernel).
+            espk::GotohArgs w = f;
+            w.order = fblist;
+            w.range = esc->fbrange;
+            w.fb_list = fb2list;
+            w.fb_range = esc->fb2range;
+            w.trace = (uint32_t *)c->ftrace.p;
+            w.maxw = (uint32_t)std::max<uint64_t>(wave_w, 1);
+            // two rounds of the grid: a longer list runs faster in the lane
+            // kernel, whose one-wave latency is a whole window sweep
+            w.list_cap = (uint32_t)std::min<uint64_t>(2 * blocks_w, 0xFFFFFFFFull);
+            HIP_TRY(c, espk::launch_gotoh_wave_list(w, (int)blocks_w, c->stream));
             tim_mark(c, 13, rows);
             f.order = fb2list;
             f.range = esc->fb2range;
@@ -557,18 +568,13 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
         if (value == 0) {
             c->prune_half_a = espk::ESC_HALF_A;
             c->prune_pad_a = espk::ESC_PAD_A;
-            c->prune_half_b = espk::ESC_HALF_B;
-            c->prune_pad_b = espk::ESC_PAD_B;
             return ESP_OK;
         }
-        const int f[4] = {(int)(value & 255), (int)((value >> 8) & 255), (int)((value >> 16) & 255),
-                          (int)((value >> 24) & 255)};
-        if (value < 0 || value > 0xFFFFFFFFll || !f[0] || !f[1] || !f[2] || !f[3])
-            return fail(c, ESP_E_INVAL, "pruning shape fields must be 1..255");
+        const int f[2] = {(int)(value & 255), (int)((value >> 8) & 255)};
+        if (value < 0 || value > 0xFFFFll || !f[0] || !f[1])
+            return fail(c, ESP_E_INVAL, "pruning shape: half | pad << 8, each 1..255");
         c->prune_half_a = f[0];
         c->prune_pad_a = f[1];
-        c->prune_half_b = f[2];
-        c->prune_pad_b = f[3];
         return ESP_OK;
     }
     case ESP_OPT_TRACE_BUDGET:

@@ -73,6 +73,9 @@ struct GotohArgs {
     // Pairs per work unit of the full-matrix kernel: 64 (0 means 64) or 1
     // (one pair a wave, for short lists whose pairs share no stripe).
     uint32_t unit;
+    // One-wave-per-pair list kernel: entries from list_cap on are passed to
+    // fb_list instead (a long list runs faster in the lane kernel).
+    uint32_t list_cap;
     // Cell census of the full-matrix kernel (nullptr: off).  census[0] +=
     // cells computed inside the pairs' matrices (rows 1..m, columns 1..n),
     // census[1] += lane-cells issued (64 lanes x 8-row groups x columns,
@@ -196,6 +199,11 @@ __host__ __device__ inline uint64_t wave_trace_words(uint64_t m, uint64_t n, int
     return m * ((w + 7) / 8);
 }
 hipError_t launch_gotoh_wave(const GotohArgs &a, int blocks, hipStream_t s);
+// The same kernel over a device list of full-matrix reruns (order, range),
+// one wave per pair; maxw trace words per block; windows over WAVE_NMAX and
+// traces over maxw go to fb_list / fb_range.
+hipError_t launch_gotoh_wave_list(const GotohArgs &a, int blocks, hipStream_t s);
+int gotoh_wave_blocks_per_cu();
 
 hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s);
 
@@ -231,19 +239,18 @@ int gotoh_full_blocks_per_cu(int rows);
 // (head-row class, diagonal of the band attempt's best cell), so that the 64
 // pairs of a wave share one stripe; pairs the pruned pass cannot help go to
 // the fallback list at once.  Every count lives in EscDev (zeroed before).
-// The first pruned pass takes the sorted list with (ESC_HALF_A, ESC_PAD_A),
-// the second its failures one pair a wave with (ESC_HALF_B, ESC_PAD_B): a
+// The pruned pass takes the sorted list with (ESC_HALF_A, ESC_PAD_A): a
 // stripe row costs the read's span of columns, a head row the whole window.
+// Its failures go to the one-pair-per-wave kernel (launch_gotoh_wave_list).
 constexpr int ESC_HALF_A = 32, ESC_PAD_A = 16;
-constexpr int ESC_HALF_B = 64, ESC_PAD_B = 16;
 constexpr int ESC_NDIAG = 1024;    // diagonal buckets per class (two diagonals each)
 constexpr int ESC_NCLASS = 3;      // head groups <= 3, <= 6, <= 12
 constexpr int ESC_NB = ESC_NCLASS * ESC_NDIAG;
 struct EscDev {
     uint32_t prange[2];   // {0, pairs in the sorted list}
-    uint32_t fbrange[2];  // {0, failures of the first pruned pass}
+    uint32_t fbrange[2];  // {0, pairs the sort or the pruned pass left for the one-wave rerun}
     uint32_t fb2range[2]; // {0, pairs for the unpruned kernel}
-    uint32_t pcounter, fbcounter, fb2counter;  // work-queue heads of the three passes
+    uint32_t pcounter, fbcounter, fb2counter;  // work-queue heads (fbcounter unused)
     uint32_t hist[ESC_NB];
     uint32_t cursor[ESC_NB];
 };
@@ -256,7 +263,7 @@ struct EscSortArgs {
     int32_t pad;            // head pad of the first pruned pass
     EscDev *esc;
     uint32_t *plist;
-    uint32_t *fb2list;  // pairs no pruned pass can help go straight to the unpruned kernel
+    uint32_t *fblist;   // pairs the pruned pass cannot help go straight to its failure list
     // census (nullptr: off): census[2] += m x n of every listed pair
     const uint32_t *win_len;
     unsigned long long *census;

@@ -382,7 +382,58 @@ __global__ __launch_bounds__(64) void k_gotoh_wave(GotohArgs a) {
     wave_one<EDIT>(a, p, m, n, s_win, s_bh, s_be, s_tr, a.trace + (size_t)blockIdx.x * a.maxw);
 }
 
+// extend_hint's full-matrix reruns of the pairs the pruned pass could not
+// prove (the list order[range[0] ..], its count at range[1] on the device),
+// one wave per pair: block b takes entries b, b + gridDim.x, ... below the
+// count.  There is no work queue and every loop bound is uniform (the count
+// and each pair's lengths are read once into scalars).  Entries from
+// list_cap on, pairs whose window is longer than WAVE_NMAX and pairs whose
+// trace fits neither LDS nor the block's maxw words are appended to fb_list
+// (the unpruned lane kernel's list) through fb_range[1].  With a.census set: census[0] += m * n, census[1] +=
+// the lane-cells the steps issue.
+template <bool EDIT>
+__global__ __launch_bounds__(64) void k_gotoh_wave_list(GotohArgs a) {
+    __shared__ uint8_t s_win[WAVE_NMAX];
+    __shared__ int32_t s_bh[WAVE_NMAX + 1];
+    __shared__ int32_t s_be[WAVE_NMAX + 1];
+    __shared__ uint32_t s_tr[WAVE_TR_LDS];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t first = __builtin_amdgcn_readfirstlane(a.range[0]);
+    const uint32_t count = __builtin_amdgcn_readfirstlane(a.range[1]);
+    for (uint32_t u = blockIdx.x; u < count; u += gridDim.x) {
+        const uint32_t p = __builtin_amdgcn_readfirstlane(a.order[first + u]);
+        const uint32_t m = __builtin_amdgcn_readfirstlane(a.read_len[p]);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(a.win_len[p]);
+        const uint64_t need = wave_trace_words(m, n, ESP_MODE_FULL, 0);
+        if (u >= a.list_cap || n > (uint32_t)WAVE_NMAX || (need > WAVE_TR_LDS && need > a.maxw)) {
+            if (lane == 0) a.fb_list[atomicAdd(&a.fb_range[1], 1u)] = p;
+            continue;
+        }
+        wave_one<EDIT>(a, p, m, n, s_win, s_bh, s_be, s_tr, a.trace + (size_t)blockIdx.x * a.maxw);
+        if (a.census && lane == 0) {
+            const uint64_t blocks = (m + 63) / 64;
+            atomicAdd(&a.census[0], (unsigned long long)m * n);
+            atomicAdd(&a.census[1], 64ull * (blocks * n + (m - blocks)));
+        }
+        __syncthreads();  // the next pair reuses the LDS
+    }
+}
+
 }  // namespace
+
+hipError_t launch_gotoh_wave_list(const GotohArgs &a, int blocks, hipStream_t s) {
+    if (a.P.edit)
+        hipLaunchKernelGGL(k_gotoh_wave_list<true>, dim3(blocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_gotoh_wave_list<false>, dim3(blocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+int gotoh_wave_blocks_per_cu() {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_wave_list<false>, 64, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
 
 hipError_t launch_gotoh_wave(const GotohArgs &a, int blocks, hipStream_t s) {
     if (a.P.edit)

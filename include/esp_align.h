@@ -227,10 +227,10 @@ int esp_last_kernel_ms(esp_ctx *ctx, double *ms);
 /* Per-launch device times of the last batch call: kind (0 planner, 1 band
  * kernel, 2 strip kernel, 3 Myers kernel, 5 window gather / preparation,
  * 6 selection / stitching, 7 extend_hint full-matrix rerun of escalated band
- * attempts (the first pruned pass when pruning is on), 8 placement walk
- * kernel, 9 fast lane kernel, 10 a whole graph launch, 11 the unpruned
- * full-matrix pass over what the pruned passes could not prove, 12 the sort
- * of the escalation list, 13 the second pruned pass), half (band kernel) and
+ * attempts (the pruned pass when pruning is on), 8 placement walk kernel,
+ * 9 fast lane kernel, 10 a whole graph launch, 11 the unpruned full-matrix
+ * kernel, 12 the sort of the escalation list, 13 the one-wavefront-per-pair
+ * rerun of what the pruned pass could not prove), half (band kernel) and
  * ms. */
 typedef struct {
     int32_t kind;
@@ -246,11 +246,12 @@ int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
  * (the full-matrix kernels get twice that); grids shrink to fit.
  * ESP_OPT_FULL_PRUNE (1, the default, or 0): the full-matrix reruns first
  * take a pruned pass that computes only the cells that can still reach the
- * band attempt's score, proves per pair that no skipped cell could, and
- * leaves every pair it cannot prove to the unpruned kernel; results are the
- * same either way.  ESP_OPT_FULL_PRUNE_SHAPE sets the stripe half-width and
- * head pad (rows) of the two pruned passes: value = halfA | padA << 8 |
- * halfB << 16 | padB << 24, each 1..255 (0 restores 32, 16, 64, 16).
+ * band attempt's score and proves per pair that no skipped cell could; the
+ * pairs it cannot prove rerun one wavefront per pair over the whole matrix
+ * (windows up to 4096 bases; longer ones in the unpruned lane kernel);
+ * results are the same either way.  ESP_OPT_FULL_PRUNE_SHAPE sets the
+ * pruned pass's stripe half-width and head pad (rows): value = half |
+ * pad << 8, each 1..255 (0 restores 32, 16).
  * ESP_OPT_PLAIN_KERNEL picks the kernel of the single-pair calls (esp_band,
  * esp_extend, esp_extend_hint): 0 (default) one wavefront per pair, its 64
  * lanes sweeping 64 rows at once, for windows up to 4096 bases (longer ones
@@ -278,7 +279,8 @@ typedef struct {
     uint32_t escalated;
     uint32_t full_pruned;
     uint32_t full_unpruned;
-    uint32_t full_second;  /* escalated pairs the second pruned pass took */
+    uint32_t full_second;  /* escalated pairs the pruned pass could not prove
+                              (rerun one wavefront per pair) */
     /* With ESP_OPT_FULL_CENSUS on (else 0), over every full-matrix pass of
      * the batch: cells computed inside the pairs' matrices, lane-cells issued
      * (64 lanes x 8-row groups x columns, padding included), and the cells
