@@ -97,7 +97,6 @@ struct esp_ctx {
     std::string err;
     esph::DevBuf counter, rows, trace, btrace, ftrace, stage, stage_out, plan, order;
     int full_blocks[512] = {};  // occupancy of the full-matrix kernel per row tile
-    int wave_blocks = 0;        // occupancy of the one-pair-per-wave list kernel
     // Trace scratch caps per launch (grids shrink to fit); a server lowers them
     // so that several contexts share one device-memory budget.
     size_t trace_budget = esph::kTraceBudget;

@@ -293,7 +293,44 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             esc_maxn = hp->esc_maxn;
         }
     }
-    if (strip_count) {
+    // A strip class of at most ncu * 64 pairs with a read and a window, all
+    // windows within the wave kernel's limit (the eager call knows both from
+    // the plan summary), runs one wave per pair: a lane-per-pair launch
+    // takes as long as one lane's whole DP, which a few heavy pairs set.
+    const espk::PlanSummary *hs = capb ? nullptr : (const espk::PlanSummary *)c->hplan.p;
+    const bool wave_strip = hs && strip_count && c->plain_kernel == 0 &&
+                            hs->strip_work <= (uint64_t)c->ncu * 64 && hs->strip_maxn <= (uint32_t)espk::WAVE_NMAX;
+    if (wave_strip) {
+        // LDS for the class's longest window, the trace in global scratch
+        const uint32_t ln = (uint32_t)align_up(std::max<uint32_t>(hs->strip_maxn, 1), 16);
+        // every pair's wave trace fits its strip-layout trace (m * ceil(n/8)
+        // words against m * 4 * ceil(n/32)), so nothing overflows; the lane
+        // kernel below still drains whatever would (64 lanes, an empty list
+        // exits at once)
+        const uint64_t tw = std::max<uint64_t>(strip_maxw, 1);
+        const uint64_t blocks_w = std::max<uint64_t>(
+            1, std::min<uint64_t>({(uint64_t)c->ncu * espk::gotoh_wave_blocks_per_cu(ln, 0), strip_count,
+                                   c->trace_budget / (tw * 4)}));
+        HIP_TRY(c, c->trace.ensure(tw * 4 * std::max<uint64_t>(blocks_w, 64)));
+        HIP_TRY(c, c->mlist.ensure(strip_count * 4 + 64));
+        espk::GotohArgs w = a;
+        w.range = &dp->sum.range[espk::PLAN_STRIP][0];
+        w.trace = (uint32_t *)c->trace.p;
+        w.maxw = (uint32_t)tw;
+        w.list_cap = 0xFFFFFFFFu;
+        w.wave_lds_n = ln;
+        w.wave_lds_tr = 0;
+        w.fb_list = (uint32_t *)c->mlist.p;
+        w.fb_range = dp->wave_ovf;
+        HIP_TRY(c, espk::launch_gotoh_wave_list(w, (int)blocks_w, c->stream));
+        a.order = (const uint32_t *)c->mlist.p;
+        a.range = dp->wave_ovf;
+        a.counter = &dp->wave_ovf_counter;
+        rc = strip_launch(c, a, 64, strip_maxm, strip_maxw, c->trace_budget);
+        if (rc) return rc;
+        a.order = pa.order;
+        tim_mark(c, 2, 0);
+    } else if (strip_count) {
         a.range = &dp->sum.range[espk::PLAN_STRIP][0];
         a.counter = &dp->counters[espk::PLAN_STRIP];
         // a graph's strip grid is sized for every pair but holds at most a
@@ -330,9 +367,10 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         // the second pass: one wave per pair the pruned pass could not prove,
         // each block with the full-matrix trace of the longest pair
         const uint64_t wave_w = esc_maxm * ((esc_maxn + 7) / 8);
-        if (!c->wave_blocks) c->wave_blocks = espk::gotoh_wave_blocks_per_cu();
+        const uint32_t wave_ln = (uint32_t)align_up(std::min<uint64_t>(std::max<uint64_t>(esc_maxn, 1),
+                                                                       (uint64_t)espk::WAVE_NMAX), 16);
         const uint64_t blocks_w = std::max<uint64_t>(
-            1, std::min<uint64_t>({(uint64_t)c->ncu * c->wave_blocks, bound,
+            1, std::min<uint64_t>({(uint64_t)c->ncu * espk::gotoh_wave_blocks_per_cu(wave_ln, 0), bound,
                                    std::max<uint64_t>(1, c->full_trace_budget / (std::max<uint64_t>(wave_w, 1) * 4))}));
         HIP_TRY(c, c->ftrace.ensure(std::max(maxw * 64 * 4 * blocks, pruned ? wave_w * 4 * blocks_w : 0)));
         f.trace = (uint32_t *)c->ftrace.p;
@@ -387,6 +425,8 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             // two rounds of the grid: a longer list runs faster in the lane
             // kernel, whose one-wave latency is a whole window sweep
             w.list_cap = (uint32_t)std::min<uint64_t>(2 * blocks_w, 0xFFFFFFFFull);
+            w.wave_lds_n = wave_ln;
+            w.wave_lds_tr = 0;
             HIP_TRY(c, espk::launch_gotoh_wave_list(w, (int)blocks_w, c->stream));
             tim_mark(c, 13, rows);
             f.order = fb2list;

@@ -431,9 +431,11 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t *counter, uint32_t b, bool 
 
 __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
     __shared__ uint32_t h[PLAN_NCLASS * PLAN_NBIN];
+    __shared__ uint32_t work;
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x) h[t] = 0;
+    if (threadIdx.x == 0) work = 0;
     __syncthreads();
-    uint32_t smax = 0, emax = 0, enmax = 0;
+    uint32_t smax = 0, emax = 0, enmax = 0, snmax = 0, swork = 0;
     uint64_t wmax = 0, ewmax = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     // The loop bound is uniform per wave, so agg_inc sees whole waves.
@@ -450,6 +452,10 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
             if (cls == PLAN_STRIP) {
                 smax = max(smax, m);
                 wmax = max(wmax, sw);
+                if (m >= 1 && a.win_len[p] >= 1) {
+                    swork++;
+                    snmax = max(snmax, a.win_len[p]);
+                }
             } else if (a.hint_mode) {
                 // a band attempt that escalates reruns on the full matrix
                 emax = max(emax, m);
@@ -459,10 +465,13 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
         }
         (void)agg_inc(h, b, active);
     }
+    if (swork) atomicAdd(&work, swork);
     __syncthreads();
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x)
         if (h[t]) atomicAdd(&a.plan->hist[t], h[t]);
+    if (threadIdx.x == 0 && work) atomicAdd(&a.plan->sum.strip_work, work);
     atomicMax(&a.plan->sum.strip_maxm, smax);
+    atomicMax(&a.plan->sum.strip_maxn, snmax);
     atomicMax(&a.plan->sum.strip_maxw, (unsigned long long)wmax);
     if (a.hint_mode) {
         atomicMax(&a.plan->sum.esc_maxm, emax);

@@ -74,8 +74,10 @@ struct GotohArgs {
     // (one pair a wave, for short lists whose pairs share no stripe).
     uint32_t unit;
     // One-wave-per-pair list kernel: entries from list_cap on are passed to
-    // fb_list instead (a long list runs faster in the lane kernel).
+    // fb_list instead (a long list runs faster in the lane kernel); its LDS
+    // holds windows up to wave_lds_n bases and wave_lds_tr trace words.
     uint32_t list_cap;
+    uint32_t wave_lds_n, wave_lds_tr;
     // Cell census of the full-matrix kernel (nullptr: off).  census[0] +=
     // cells computed inside the pairs' matrices (rows 1..m, columns 1..n),
     // census[1] += lane-cells issued (64 lanes x 8-row groups x columns,
@@ -142,10 +144,15 @@ struct PlanSummary {
     uint32_t esc_maxm;              // extend_hint band pairs: largest read,
     uint32_t esc_maxn;              // largest window and
     unsigned long long esc_maxw;    // largest full-matrix strip trace (words)
+    uint32_t strip_work;            // strip-class pairs with a read and a window
+    uint32_t strip_maxn;            // their largest window
 };
 struct PlanDev {
     PlanSummary sum;  // first: the host reads back only this part
     uint32_t esc_range[2];  // {start in order (= npairs), escalated count}
+    // strip class run one wave per pair: {0, pairs passed on to the lane
+    // kernel}, and that kernel's queue head
+    uint32_t wave_ovf[2], wave_ovf_counter;
     uint32_t counters[PLAN_NCLASS + 2];
     uint32_t hist[PLAN_NCLASS * PLAN_NBIN];
     uint32_t cursor[PLAN_NCLASS * PLAN_NBIN];
@@ -202,8 +209,12 @@ hipError_t launch_gotoh_wave(const GotohArgs &a, int blocks, hipStream_t s);
 // The same kernel over a device list of full-matrix reruns (order, range),
 // one wave per pair; maxw trace words per block; windows over WAVE_NMAX and
 // traces over maxw go to fb_list / fb_range.
+// LDS of the list kernel: windows up to ln bases (a multiple of 16, at most
+// WAVE_NMAX) and tr trace words kept in LDS (more go to the block's global
+// trace); blocks per CU at that size.
 hipError_t launch_gotoh_wave_list(const GotohArgs &a, int blocks, hipStream_t s);
-int gotoh_wave_blocks_per_cu();
+size_t wave_list_lds(uint32_t ln, uint32_t tr);
+int gotoh_wave_blocks_per_cu(uint32_t ln, uint32_t tr);
 
 hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s);
 

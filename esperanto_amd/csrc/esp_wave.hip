@@ -276,7 +276,7 @@ __device__ bool wave_impure(const uint8_t *s, int32_t n) {
 // edge or scored below 85% of perfect) in ESP_MODE_HINT.
 template <bool EDIT>
 __device__ void wave_one(const GotohArgs &a, uint32_t p, uint32_t mu, uint32_t nu, uint8_t *s_win, int32_t *s_bh,
-                         int32_t *s_be, uint32_t *s_tr, uint32_t *g_tr) {
+                         int32_t *s_be, uint32_t *s_tr, uint32_t tr_lds, uint32_t *g_tr) {
     const uint32_t lane = threadIdx.x & 63u;
     const int32_t m = (int32_t)mu, n = (int32_t)nu;
     PairView pv;
@@ -322,11 +322,11 @@ __device__ void wave_one(const GotohArgs &a, uint32_t p, uint32_t mu, uint32_t n
     // the trace in LDS when it fits (each instance of the fill then stores it
     // with LDS or global instructions, never flat ones)
     auto fill = [&](uint32_t w) -> WaveFill {
-        if ((uint64_t)m * w <= WAVE_TR_LDS) return wave_fill<EDIT, true>(pv, a.P, wrap16, s_win, s_bh, s_be, s_tr, w);
+        if ((uint64_t)m * w <= tr_lds) return wave_fill<EDIT, true>(pv, a.P, wrap16, s_win, s_bh, s_be, s_tr, w);
         return wave_fill<EDIT, false>(pv, a.P, wrap16, s_win, s_bh, s_be, g_tr, w);
     };
     uint32_t wpr = wave_wpr(n, pv.half);
-    uint32_t *tr = (uint64_t)m * wpr <= WAVE_TR_LDS ? s_tr : g_tr;
+    uint32_t *tr = (uint64_t)m * wpr <= tr_lds ? s_tr : g_tr;
     WaveFill fr = fill(wpr);
     if (a.mode == ESP_MODE_HINT && band) {
         // extend_hint's escalation test in the reference's wrapping i32
@@ -336,7 +336,7 @@ __device__ void wave_one(const GotohArgs &a, uint32_t p, uint32_t mu, uint32_t n
             pv.off = 0;
             pv.half = n;
             wpr = wave_wpr(n, n);
-            tr = (uint64_t)m * wpr <= WAVE_TR_LDS ? s_tr : g_tr;
+            tr = (uint64_t)m * wpr <= tr_lds ? s_tr : g_tr;
             fr = fill(wpr);
             path = 1;
         }
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64) void k_gotoh_wave(GotohArgs a) {
     const uint32_t n = __builtin_amdgcn_readfirstlane(a.win_len[p]);
     const uint64_t need = wave_trace_words(m, n, a.mode, a.mode == ESP_MODE_BAND ? a.half[p] : 0);
     if (n > (uint32_t)WAVE_NMAX || (need > WAVE_TR_LDS && need > a.maxw)) return;
-    wave_one<EDIT>(a, p, m, n, s_win, s_bh, s_be, s_tr, a.trace + (size_t)blockIdx.x * a.maxw);
+    wave_one<EDIT>(a, p, m, n, s_win, s_bh, s_be, s_tr, WAVE_TR_LDS, a.trace + (size_t)blockIdx.x * a.maxw);
 }
 
 // extend_hint's full-matrix reruns of the pairs the pruned pass could not
@@ -391,12 +391,17 @@ __global__ __launch_bounds__(64) void k_gotoh_wave(GotohArgs a) {
 // trace fits neither LDS nor the block's maxw words are appended to fb_list
 // (the unpruned lane kernel's list) through fb_range[1].  With a.census set: census[0] += m * n, census[1] +=
 // the lane-cells the steps issue.
+// LDS is sized per launch (wave_list_lds): windows up to a.wave_lds_n bases
+// and a.wave_lds_tr trace words, so a batch of short windows keeps many
+// waves per CU.
 template <bool EDIT>
 __global__ __launch_bounds__(64) void k_gotoh_wave_list(GotohArgs a) {
-    __shared__ uint8_t s_win[WAVE_NMAX];
-    __shared__ int32_t s_bh[WAVE_NMAX + 1];
-    __shared__ int32_t s_be[WAVE_NMAX + 1];
-    __shared__ uint32_t s_tr[WAVE_TR_LDS];
+    extern __shared__ uint8_t wlds[];
+    const uint32_t ln = a.wave_lds_n;  // a multiple of 16
+    int32_t *s_bh = (int32_t *)wlds;
+    int32_t *s_be = s_bh + (ln + 1);
+    uint32_t *s_tr = (uint32_t *)(s_be + (ln + 1));
+    uint8_t *s_win = (uint8_t *)(s_tr + a.wave_lds_tr);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t first = __builtin_amdgcn_readfirstlane(a.range[0]);
     const uint32_t count = __builtin_amdgcn_readfirstlane(a.range[1]);
@@ -405,11 +410,11 @@ __global__ __launch_bounds__(64) void k_gotoh_wave_list(GotohArgs a) {
         const uint32_t m = __builtin_amdgcn_readfirstlane(a.read_len[p]);
         const uint32_t n = __builtin_amdgcn_readfirstlane(a.win_len[p]);
         const uint64_t need = wave_trace_words(m, n, ESP_MODE_FULL, 0);
-        if (u >= a.list_cap || n > (uint32_t)WAVE_NMAX || (need > WAVE_TR_LDS && need > a.maxw)) {
+        if (u >= a.list_cap || n > ln || (need > a.wave_lds_tr && need > a.maxw)) {
             if (lane == 0) a.fb_list[atomicAdd(&a.fb_range[1], 1u)] = p;
             continue;
         }
-        wave_one<EDIT>(a, p, m, n, s_win, s_bh, s_be, s_tr, a.trace + (size_t)blockIdx.x * a.maxw);
+        wave_one<EDIT>(a, p, m, n, s_win, s_bh, s_be, s_tr, a.wave_lds_tr, a.trace + (size_t)blockIdx.x * a.maxw);
         if (a.census && lane == 0) {
             const uint64_t blocks = (m + 63) / 64;
             atomicAdd(&a.census[0], (unsigned long long)m * n);
@@ -421,17 +426,21 @@ __global__ __launch_bounds__(64) void k_gotoh_wave_list(GotohArgs a) {
 
 }  // namespace
 
+size_t wave_list_lds(uint32_t ln, uint32_t tr) { return (size_t)8 * (ln + 1) + (size_t)4 * tr + ln; }
+
 hipError_t launch_gotoh_wave_list(const GotohArgs &a, int blocks, hipStream_t s) {
+    const size_t lds = wave_list_lds(a.wave_lds_n, a.wave_lds_tr);
     if (a.P.edit)
-        hipLaunchKernelGGL(k_gotoh_wave_list<true>, dim3(blocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(k_gotoh_wave_list<true>, dim3(blocks), dim3(64), lds, s, a);
     else
-        hipLaunchKernelGGL(k_gotoh_wave_list<false>, dim3(blocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(k_gotoh_wave_list<false>, dim3(blocks), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
-int gotoh_wave_blocks_per_cu() {
+int gotoh_wave_blocks_per_cu(uint32_t ln, uint32_t tr) {
     int nb = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_wave_list<false>, 64, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_wave_list<true>, 64,
+                                                                      wave_list_lds(ln, tr));
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
