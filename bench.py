@@ -410,76 +410,16 @@ def bench_walk(args, buf, params):
     return out
 
 
-def bench_serving(args, buf, params):
-    """Serving loop: a pool of mixed requests (log-uniform 1-4,096 pairs of 50-250-nt
-    reads; 10% mate-rescue requests with ~2 kb windows) over a 256 Mb reference
-    resident in HBM.  Closed loop first (the whole pool at once: the batched peak),
-    then open-loop Poisson arrivals at 25/50/90% of that peak for a fixed time,
-    latency = completion minus scheduled arrival."""
+def _serve_open_loop(srv, submit, pool, sizes, peak, seconds, rng):
+    """Open-loop Poisson arrivals at 25/50/90% of peak (pairs/s) for the given
+    time each; latency = completion minus scheduled arrival."""
     import queue
     import threading
-    from esperanto_amd.reference import DeviceReference
-    from esperanto_amd.server import Server, now_ns
-    from esperanto_amd.synth_window import serving_workload
-    log(f"serving workload: {args.serve_requests} requests on a {args.serve_ref_mb} Mb reference")
-    ref, pool = serving_workload(args.serve_requests, ref_len=args.serve_ref_mb * 1_000_000)
-    dref = DeviceReference(ref, buf)
-    srv = Server(dref, params, mem_budget=32 << 30, workers=args.serve_workers,
-                 max_batch_pairs=args.serve_batch_pairs)
-
-    def submit(r):
-        return srv.submit_arrays(r["kind"], r["reads"], r["read_len"], r["contig"],
-                                 r.get("win_start"), r.get("win_end"), r.get("hint"),
-                                 r.get("mate_pos"))
-
-    sizes = np.array([r["read_len"].shape[0] for r in pool])
-    # warm-up: one untimed pass of the whole pool grows every worker's scratch
-    # and pinned staging to the batch sizes the timed passes use
-    for t in [submit(r) for r in pool]:
-        t.wait()
-    t0 = now_ns()
-    tickets = [submit(r) for r in pool]
-    done = max(t.wait().done_ns for t in tickets)
-    peak = float(sizes.sum()) / ((done - t0) / 1e9)
-    log(f"serving closed loop: {len(pool)} requests, {sizes.sum()} pairs, {peak / 1e6:.2f} M pairs/s")
-    # bit-exactness of a sample against the oracle
-    verified = 0
-    if not args.no_cpu:
-        from oracle import oracle
-        from esperanto_amd.batch import PairBatch
-        from esperanto_amd.rescue import MateRequests
-        o = oracle.OracleRef(ref)
-        rng = np.random.default_rng(1)
-        for j in rng.choice(len(pool), min(24, len(pool)), replace=False):
-            r, t = pool[j], tickets[j]
-            n = r["read_len"].shape[0]
-            off = np.zeros(n + 1, dtype=np.int64)
-            off[1:] = np.cumsum(r["read_len"])
-            rec, cig, coff, cn = t.raw()
-            if r["kind"] == 0:
-                reads = [r["reads"][off[k]:off[k + 1]].tobytes() for k in range(n)]
-                wins = [oracle.decode(o, 0, int(r["win_start"][k]), int(r["win_end"][k])) for k in range(n)]
-                pb = PairBatch.from_pairs([(reads[k], wins[k], 32, 0) for k in range(n)])
-                e, _, ec, eo, en, _ = oracle.band_batch(pb, params, 2, nthreads=16)
-                got = rec.astype(np.int64)
-                got[:, 3] = rec[:, 3].view(np.int32)
-                ok = np.array_equal(got, e) and np.array_equal(cn, en)
-            else:
-                mq = MateRequests(r["reads"], off[:-1].astype(np.uint64), r["read_len"], r["contig"],
-                                  r["mate_pos"])
-                e, ec, eo, en = oracle.mate_batch(o, mq, params, nthreads=16)
-                ok = [x.tolist() for x in rec] == [x.tolist() for x in e] and np.array_equal(cn, en)
-            ok = ok and all(np.array_equal(cig[int(coff[k]):int(coff[k]) + int(cn[k])],
-                                           ec[int(coff[k]):int(coff[k]) + int(cn[k])]) for k in range(n))
-            if not ok:
-                raise RuntimeError(f"serving result of request {j} differs from the CPU path")
-            verified += 1
-    del tickets
+    from esperanto_amd.server import now_ns
     loads = []
-    rng = np.random.default_rng(5)
     for frac in (0.25, 0.5, 0.9):
         rate = frac * peak / float(sizes.mean())  # requests per second
-        nreq = max(1, int(rate * args.serve_seconds))
+        nreq = max(1, int(rate * seconds))
         gaps = rng.exponential(1.0 / rate, nreq)
         order = rng.integers(0, len(pool), nreq)
         q = queue.Queue()
@@ -518,14 +458,102 @@ def bench_serving(args, buf, params):
                       "p99_ms": round(float(np.percentile(lat_a, 99)), 3),
                       "max_ms": round(float(lat_a.max()), 3)})
         log(f"serving load {frac:.2f}: {loads[-1]}")
-    st = srv.stats()
-    srv.close()
+    return loads
+
+
+def bench_serving(args, buf, params, c3_pairs_per_s=None):
+    """Serving loop: a pool of mixed requests (log-uniform 1-4,096 pairs of 50-250-nt
+    reads; 10% mate-rescue requests with ~2 kb windows) over a 256 Mb reference
+    resident in HBM.  For each batch-size cap (pairs per batch: the default and a
+    small one that trades throughput for latency): closed loop first (the whole
+    pool at once: that configuration's peak), then open-loop Poisson arrivals at
+    25/50/90% of that peak for a fixed time."""
+    from esperanto_amd.reference import DeviceReference
+    from esperanto_amd.server import Server, now_ns
+    from esperanto_amd.synth_window import serving_workload
+    log(f"serving workload: {args.serve_requests} requests on a {args.serve_ref_mb} Mb reference")
+    ref, pool = serving_workload(args.serve_requests, ref_len=args.serve_ref_mb * 1_000_000)
+    dref = DeviceReference(ref, buf)
+    sizes = np.array([r["read_len"].shape[0] for r in pool])
+    out = {"requests_in_pool": len(pool), "pairs_in_pool": int(sizes.sum()),
+           "workers": args.serve_workers, "mem_budget_bytes": 32 << 30, "caps": []}
+    rng = np.random.default_rng(5)
+    for cap in (args.serve_batch_pairs, args.serve_small_batch_pairs):
+        srv = Server(dref, params, mem_budget=32 << 30, workers=args.serve_workers, max_batch_pairs=cap)
+
+        def submit(r):
+            return srv.submit_arrays(r["kind"], r["reads"], r["read_len"], r["contig"],
+                                     r.get("win_start"), r.get("win_end"), r.get("hint"),
+                                     r.get("mate_pos"))
+
+        # warm-up: one untimed pass of the whole pool grows every worker's
+        # scratch and pinned staging to the batch sizes the timed passes use
+        for t in [submit(r) for r in pool]:
+            t.wait()
+        t0 = now_ns()
+        tickets = [submit(r) for r in pool]
+        done = max(t.wait().done_ns for t in tickets)
+        peak = float(sizes.sum()) / ((done - t0) / 1e9)
+        log(f"serving cap {cap}: closed loop {len(pool)} requests, {sizes.sum()} pairs, {peak / 1e6:.2f} M pairs/s")
+        verified = 0
+        if not args.no_cpu and cap == args.serve_batch_pairs:
+            verified = _serve_verify(ref, pool, tickets, params)
+        del tickets
+        loads = _serve_open_loop(srv, submit, pool, sizes, peak, args.serve_seconds, rng)
+        st = srv.stats()
+        srv.close()
+        out["caps"].append({"max_batch_pairs": cap, "closed_loop_pairs_per_s": round(peak), "loads": loads,
+                            "verified_requests_bit_exact": verified,
+                            "max_batch_requests": st["max_batch_requests"], "batches": st["batches"],
+                            "peak_inflight_bytes": st["peak_inflight_bytes"]})
     dref.close()
-    return {"requests_in_pool": len(pool), "pairs_in_pool": int(sizes.sum()),
-            "workers": args.serve_workers, "mem_budget_bytes": 32 << 30,
-            "closed_loop_pairs_per_s": round(peak), "loads": loads,
-            "verified_requests_bit_exact": verified, "max_batch_requests": st["max_batch_requests"],
-            "batches": st["batches"], "peak_inflight_bytes": st["peak_inflight_bytes"]}
+    # the default cap's figures at the top level
+    out.update({k: out["caps"][0][k] for k in ("closed_loop_pairs_per_s", "loads", "verified_requests_bit_exact",
+                                               "max_batch_requests", "batches", "peak_inflight_bytes")})
+    if c3_pairs_per_s:
+        # Loads at fractions of the batched C3 rate overload this mix: its pairs
+        # are extend_hint calls (a band attempt, with full-matrix reruns for
+        # the escalated ones) and 10% carry ~2 kb rescue windows, far more work
+        # per pair than a C3 band pair, so even 25% of that rate exceeds the
+        # mix's closed-loop peak and the queue would grow without bound.
+        out["c3_batched_pairs_per_s"] = round(c3_pairs_per_s)
+        out["c3_25pct_over_mix_peak"] = round(0.25 * c3_pairs_per_s / out["closed_loop_pairs_per_s"], 2)
+    return out
+
+
+def _serve_verify(ref, pool, tickets, params):
+    """Bit-exactness of 24 sampled requests against the oracle."""
+    from oracle import oracle
+    from esperanto_amd.batch import PairBatch
+    from esperanto_amd.rescue import MateRequests
+    o = oracle.OracleRef(ref)
+    rng = np.random.default_rng(1)
+    verified = 0
+    for j in rng.choice(len(pool), min(24, len(pool)), replace=False):
+        r, t = pool[j], tickets[j]
+        n = r["read_len"].shape[0]
+        off = np.zeros(n + 1, dtype=np.int64)
+        off[1:] = np.cumsum(r["read_len"])
+        rec, cig, coff, cn = t.raw()
+        if r["kind"] == 0:
+            reads = [r["reads"][off[k]:off[k + 1]].tobytes() for k in range(n)]
+            wins = [oracle.decode(o, 0, int(r["win_start"][k]), int(r["win_end"][k])) for k in range(n)]
+            pb = PairBatch.from_pairs([(reads[k], wins[k], 32, 0) for k in range(n)])
+            e, _, ec, eo, en, _ = oracle.band_batch(pb, params, 2, nthreads=16)
+            got = rec.astype(np.int64)
+            got[:, 3] = rec[:, 3].view(np.int32)
+            ok = np.array_equal(got, e) and np.array_equal(cn, en)
+        else:
+            mq = MateRequests(r["reads"], off[:-1].astype(np.uint64), r["read_len"], r["contig"],
+                              r["mate_pos"])
+            e, ec, eo, en = oracle.mate_batch(o, mq, params, nthreads=16)
+            ok = [x.tolist() for x in rec] == [x.tolist() for x in e] and np.array_equal(cn, en)
+        ok = ok and all(np.array_equal(cig[int(coff[k]):int(coff[k]) + int(cn[k])],
+                                       ec[int(coff[k]):int(coff[k]) + int(cn[k])]) for k in range(n))
+        if not ok:
+            raise RuntimeError(f"serving result of request {j} differs from the CPU path")
+        verified += 1
+    return verified
 
 
 def _subset_splice(sp, k):
@@ -610,6 +638,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-batch-pairs", type=int, default=131072)
+    ap.add_argument("--serve-small-batch-pairs", type=int, default=8192,
+                    help="the smaller batch-size cap the serving leg also runs (latency over throughput)")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--walk-pairs", type=int, default=1_000_000,
                     help="read pairs of the placement-walk leg (two placements each; 0 skips it)")
@@ -753,7 +783,7 @@ def main() -> int:
         junction = bench_junction(args, buf, params)
     serve = None
     if args.serve_requests:
-        serve = bench_serving(args, buf, params)
+        serve = bench_serving(args, buf, params, pairs_per_s)
 
     cpu = None
     if not args.no_cpu:

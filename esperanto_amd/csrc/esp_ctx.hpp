@@ -104,7 +104,8 @@ struct esp_ctx {
     int full_kernel = 0;  // extend_hint reruns: 0 by batch size, 1 lane per pair
     int full_prune = 1;   // extend_hint reruns: pruned pass first (1) or the full matrix only (0)
     int plain_kernel = 0;  // single-pair calls: 0 one wave per pair (windows up to 4096), 1 one lane
-    bool last_pruned = false;  // the last batch ran the pruned pass (its counts are in escbuf)
+    bool last_pruned = false;  // the last batch ran the pruned reruns (their counts are in escbuf)
+    bool last_wave_direct = false;  // ... with every escalation sent to the one-wave rerun first
     // full-matrix cell census (ESP_OPT_FULL_CENSUS): on/off, its device
     // counters, and whether the last batch ran the full-matrix stage with it
     int census = 0;

@@ -225,6 +225,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     const int prune_s = max_subst(p);
     const bool prune = hint_band && c->full_prune && prune_s > 0;
     c->last_pruned = false;
+    c->last_wave_direct = false;
     c->last_census = false;
     espk::EscDev *esc = nullptr;
     uint32_t *plist = nullptr, *fblist = nullptr, *fb2list = nullptr;
@@ -372,7 +373,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         const uint64_t blocks_w = std::max<uint64_t>(
             1, std::min<uint64_t>({(uint64_t)c->ncu * espk::gotoh_wave_blocks_per_cu(wave_ln, 0), bound,
                                    std::max<uint64_t>(1, c->full_trace_budget / (std::max<uint64_t>(wave_w, 1) * 4))}));
-        HIP_TRY(c, c->ftrace.ensure(std::max(maxw * 64 * 4 * blocks, pruned ? wave_w * 4 * blocks_w : 0)));
+        HIP_TRY(c, c->ftrace.ensure(std::max(maxw * 64 * 4 * blocks, prune ? wave_w * 4 * blocks_w : 0)));
         f.trace = (uint32_t *)c->ftrace.p;
         f.maxw = (uint32_t)maxw;
         if (c->census && !c->cap) {
@@ -416,6 +417,7 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             // the list cap (windows over the wave kernel's limit, and the
             // rest of a long list, go on to the unpruned lane kernel).
             espk::GotohArgs w = f;
+            w.census_first = 0;  // the sort counted the matrices
             w.order = fblist;
             w.range = esc->fbrange;
             w.fb_list = fb2list;
@@ -433,6 +435,27 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             f.range = esc->fb2range;
             f.counter = &esc->fb2counter;
             f.census_first = 0;
+            HIP_TRY(c, espk::launch_gotoh_full(f, rows, false, (int)blocks, c->stream));
+            tim_mark(c, 11, rows);
+        } else if (prune) {
+            // Reads past the pruned pass's tiles: the escalations run one wave
+            // per pair first (up to the list cap), the rest in the unpruned
+            // lane kernel, whose one-wave latency is a whole window sweep.
+            c->last_pruned = true;
+            c->last_wave_direct = true;
+            espk::GotohArgs w = f;
+            w.fb_list = fb2list;
+            w.fb_range = esc->fb2range;
+            w.trace = (uint32_t *)c->ftrace.p;
+            w.maxw = (uint32_t)std::max<uint64_t>(wave_w, 1);
+            w.list_cap = (uint32_t)std::min<uint64_t>(2 * blocks_w, 0xFFFFFFFFull);
+            w.wave_lds_n = wave_ln;
+            w.wave_lds_tr = 0;
+            HIP_TRY(c, espk::launch_gotoh_wave_list(w, (int)blocks_w, c->stream));
+            tim_mark(c, 13, rows);
+            f.order = fb2list;
+            f.range = esc->fb2range;
+            f.counter = &esc->fb2counter;
             HIP_TRY(c, espk::launch_gotoh_full(f, rows, false, (int)blocks, c->stream));
             tim_mark(c, 11, rows);
         } else {
@@ -720,8 +743,9 @@ int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
         HIP_TRY(c, hipMemcpy(fa, ed->fbrange, sizeof(fa), hipMemcpyDeviceToHost));
         out->full_unpruned = fb[1];
         out->full_pruned = esc[1] - fb[1];
-        // pass B's failures are in fb2 beside the pairs no pass could help
-        out->full_second = fa[1];
+        // the pairs the one-wave rerun was given: the pruned pass's failures
+        // and the sort's hopeless pairs, or every escalation of a long read
+        out->full_second = c->last_wave_direct ? esc[1] : fa[1];
     }
     if (c->last_census && c->censusbuf.p) {
         unsigned long long cs[3];

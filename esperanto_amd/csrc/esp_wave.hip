@@ -419,6 +419,7 @@ __global__ __launch_bounds__(64) void k_gotoh_wave_list(GotohArgs a) {
             const uint64_t blocks = (m + 63) / 64;
             atomicAdd(&a.census[0], (unsigned long long)m * n);
             atomicAdd(&a.census[1], 64ull * (blocks * n + (m - blocks)));
+            if (a.census_first) atomicAdd(&a.census[2], (unsigned long long)m * n);
         }
         __syncthreads();  // the next pair reuses the LDS
     }

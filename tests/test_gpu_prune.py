@@ -12,7 +12,8 @@ like pseudo-references whose read sits off the hint diagonal) and must fall
 back (indels that walk the path off its stripe, tandem copies that tie at
 other diagonals, heavy substitution that leaves a low band score, N and
 lower-case bytes), at read lengths of every row tile (64 .. 192 pruned; a
-batch of 200-250-nt reads takes the 264-row tile, which has no pruned pass),
+batch of 200-250-nt reads takes the 264-row tile, which has no pruned pass,
+so its escalations go to the one-wave-per-pair rerun directly),
 with the default and a custom scoring (EditDir Ag included).
 """
 from __future__ import annotations
@@ -173,7 +174,10 @@ def test_pruned_full_matrix_equals_oracle_and_unpruned(buf, scoring):
         assert su["full_pruned"] == 0 and su["full_unpruned"] == su["escalated"]
         assert 12 not in ku
         if label == "long reads":
-            assert 12 not in kp and sp["full_pruned"] == 0, sp
+            # no pruned pass for the 264-row tile: every escalation goes to
+            # the one-wave-per-pair rerun first
+            assert 12 not in kp and 13 in kp and sp["full_second"] == sp["escalated"], sp
+            assert sp["full_pruned"] == sp["escalated"] - sp["full_unpruned"], sp
             continue
         assert 12 in kp and 13 in kp and 11 in kp
         if label != "adversarial":

@@ -1,7 +1,8 @@
 """Serving loop (esp_server_*) on the GPU.
 
-- a burst of mixed extension / mate-rescue requests of uneven sizes, batched
-  across requests by three workers, matches the oracle pair for pair
+- a burst of mixed extension / mate-rescue requests of uneven sizes on the
+  serving workload's 256 Mb reference, batched across requests by three
+  workers, matches the oracle pair for pair
   (extend_hint over the decoded windows; mate_window_placement);
 - every request resubmitted alone returns byte-identical records and CIGARs
   (the result of a pair does not depend on what shares its batch);
@@ -81,7 +82,9 @@ def test_server_mix_matches_oracle_and_alone(buf):
     from esperanto_amd.reference import DeviceReference
     from esperanto_amd.server import Server
     params = ExtendParams()
-    ref, reqs = serving_workload(90, ref_len=3_000_000, max_pairs=400, rescue_fraction=0.25)
+    # the serving workload's 256 Mb reference resident in HBM
+    say("serving workload on a 256 Mb reference")
+    ref, reqs = serving_workload(90, ref_len=256_000_000, max_pairs=400, rescue_fraction=0.25)
     o = oracle.OracleRef(ref)
     dref = DeviceReference(ref, buf)
     srv = Server(dref, params, mem_budget=16 << 30, workers=3, max_batch_pairs=3000)
