@@ -349,6 +349,7 @@ struct WalkArgs {
 };
 hipError_t launch_walk(const WalkArgs &a, int blocks, hipStream_t s);
 int walk_blocks_per_cu();
+int fast_lane_blocks_per_cu();
 
 // try_fast_lane over chains (anchors as (qpos, rpos) pairs).
 struct FastArgs {

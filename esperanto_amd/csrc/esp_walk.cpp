@@ -235,7 +235,8 @@ int esp_fast_lane_batch(esp_ctx *c, const esp_ref *r, const esp_fast_batch *b, c
     a.edit_dir = p->edit_dir;
     a.out = out;
     a.cigar = cigar;
-    const uint64_t blocks = std::min<uint64_t>((b->nchain + 255) / 256, (uint64_t)c->ncu * 8);
+    const uint64_t blocks = std::min<uint64_t>((b->nchain + 255) / 256,
+                                              (uint64_t)c->ncu * (uint64_t)espk::fast_lane_blocks_per_cu());
     HIP_TRY(c, ev_record(c, c->ev0));
     tim_begin(c);
     HIP_TRY(c, espk::launch_fast_lane(a, (int)std::max<uint64_t>(blocks, 1), c->stream));

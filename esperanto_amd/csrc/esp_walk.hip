@@ -682,8 +682,13 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, FastKadane &K, cons
 // (column scores from exact byte masks as in the walks; the running sum's
 // updates stay per column), then the zero-mismatch gate and the at most
 // three-op CIGAR.
+// One lane per chain, in rounds of 64 consecutive chains per wave: the
+// round's queries (consecutive in the batch) are copied into the wave's LDS
+// with coalesced loads first when they span at most STAGE_DW dwords, so each
+// query byte leaves HBM once instead of once per tile.
 __global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
     __shared__ uint32_t lut[256];
+    __shared__ __align__(16) uint32_t s_stage[4][STAGE_DW];
     {
         const uint32_t b = threadIdx.x;
         uint32_t v = 0;
@@ -692,106 +697,158 @@ __global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
     }
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < a.n; idx += stride) {
-        esp_fast_rec rec = {};
-        const uint32_t contig = a.contig[idx];
-        if (contig >= a.R.ncontig) {
-            rec.status = -1;
-            a.out[idx] = rec;
-            continue;
-        }
-        const uint32_t m = a.read_len[idx];
-        const uint32_t na = a.anc_n[idx];
-        const uint32_t *anc = a.anchors + 2 * a.anc_off[idx];
-        // conditions 2-4 (align.rs:2284-2305)
-        bool ok = na > 0;
-        int64_t diag = 0;
-        uint32_t qstart = 0, qend = 0, rs = 0;
-        if (ok) {
-            qstart = anc[0];
-            rs = anc[1];
-            diag = (int64_t)rs - (int64_t)qstart;
-            uint32_t prevq = qstart;
-            for (uint32_t j = 1; j < na && ok; j++) {
-                const uint32_t q = anc[2 * j], r = anc[2 * j + 1];
-                ok = (int64_t)r - (int64_t)q == diag && q - prevq <= a.k + a.w + 5u;
-                prevq = q;
-            }
-            qend = prevq + a.k;
-            ok = ok && qstart <= 2 && m - qend <= 2;
-        }
-        if (!ok) {
-            a.out[idx] = rec;
-            continue;
-        }
-        const uint32_t flank = m / 2 + 30;
-        const uint32_t wstart = rs > flank ? rs - flank : 0u;
-        const uint32_t *pk = (const uint32_t *)(a.R.packed + a.R.pk_off[contig]);
-        const uint32_t clen = a.R.len[contig];
-        const uint32_t *iv = a.R.iv + 2 * a.R.iv_off[contig];
-        const uint32_t niv = (uint32_t)(a.R.iv_off[contig + 1] - a.R.iv_off[contig]);
-        const uintptr_t qaddr = (uintptr_t)(a.reads + a.read_off[idx]);
-        const uint32_t *qw = (const uint32_t *)(qaddr & ~(uintptr_t)3);
-        const uint32_t qoff0 = (uint32_t)(qaddr & 3u);
-        const uint32_t qjmax = m ? (qoff0 + m - 1) >> 2 : 0u;
-        // dwords the contig's region holds: ceil(len / 4) + 16 bytes, rounded up to 16
-        const uint32_t pklast = ((((clen + 3u) >> 2) + 16u + 15u) & ~15u) / 4u - 1u;
-        const uint32_t exag = a.edit_dir == 1 ? 0xFFFFFFFFu : 0u, extc = a.edit_dir == 2 ? 0xFFFFFFFFu : 0u;
-        uint32_t ci = 0;
-        {
-            const int64_t p0 = diag > 0 ? diag : 0;
-            ci = niv ? iv_seek(iv, niv, (uint32_t)min(p0, (int64_t)0xFFFFFFFF)) : 0u;
-        }
-        FastKadane K;
-        for (uint32_t i0 = 0; i0 < m; i0 += FT) {
-            // one tile: its query and reference words load at once, then the
-            // FT columns run from registers
-            const uint32_t ncol = min(m - i0, FT);
-            uint32_t q[FT / 4 + 1], r[FT / 16 + 1];
-            const uint32_t j0 = (qoff0 + i0) >> 2;
-            if (j0 + FT / 4 <= qjmax) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t *const stage = s_stage[wave];
+    for (uint32_t first = blockIdx.x * blockDim.x + 64 * wave; first < a.n; first += stride) {
+        const uint32_t idx = first + lane;
+        const bool act = idx < a.n;
+        const uint32_t contig = act ? a.contig[idx] : 0u;
+        const bool incontig = act && contig < a.R.ncontig;
+        const uint32_t m = incontig ? a.read_len[idx] : 0u;
+        const uint64_t roff = incontig ? a.read_off[idx] : 0ull;
+        // the round's query span, uniform over the wave
+        uint64_t lo = m ? roff : ~0ull, hi = m ? roff + m : 0ull;
 #pragma unroll
-                for (uint32_t j = 0; j < FT / 16; j++) {
-                    const u32x4 v = *(const u32x4 *)(qw + j0 + 4 * j);
-                    q[4 * j] = v.x, q[4 * j + 1] = v.y, q[4 * j + 2] = v.z, q[4 * j + 3] = v.w;
+        for (int o = 32; o; o >>= 1) {
+            lo = min(lo, (uint64_t)__shfl_xor((unsigned long long)lo, o, 64));
+            hi = max(hi, (uint64_t)__shfl_xor((unsigned long long)hi, o, 64));
+        }
+        lo = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)lo) |
+             (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(lo >> 32)) << 32;
+        hi = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)hi) |
+             (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(hi >> 32)) << 32;
+        uintptr_t sbase = 0;
+        if (hi > lo) {
+            const uintptr_t g0 = (uintptr_t)(a.reads + lo) & ~(uintptr_t)3;
+            const uintptr_t g1 = ((uintptr_t)(a.reads + hi) + 3) & ~(uintptr_t)3;
+            const uint32_t ndw = (uint32_t)min((uint64_t)(g1 - g0) >> 2, (uint64_t)STAGE_DW + 1);
+            if (ndw <= STAGE_DW) {
+                // every dword copied lies in [g0, g1): the dwords holding the
+                // round's query bytes, which the global path reads too
+                const uint32_t *src = (const uint32_t *)g0;
+                for (uint32_t d = 4 * lane; d < ndw; d += 256) {
+                    if (d + 4 <= ndw) {
+                        const u32x4 v = *(const u32x4 *)(src + d);
+                        stage[d] = v.x, stage[d + 1] = v.y, stage[d + 2] = v.z, stage[d + 3] = v.w;
+                    } else {
+                        for (uint32_t k = d; k < ndw; k++) stage[k] = src[k];
+                    }
                 }
-                q[FT / 4] = qw[j0 + FT / 4];
-            } else {
-#pragma unroll
-                for (uint32_t j = 0; j <= FT / 4; j++) q[j] = qw[min(j0 + j, qjmax)];
+                sbase = g0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            const uint32_t qsh = (qoff0 + i0) & 3u;
-            const int64_t p0 = (int64_t)i0 + diag;
-            while (ci < niv && (int64_t)iv[2 * ci + 1] <= p0) ci++;
-            const bool inside = p0 >= 0 && p0 + FT <= (int64_t)clen;
-            const bool clear = ci >= niv || (int64_t)iv[2 * ci] >= p0 + FT;
-            const bool fast = inside && clear;
-            const uint32_t d0 = fast ? (uint32_t)p0 >> 4 : 0u;
+        }
+        do {
+            if (!act) break;
+            esp_fast_rec rec = {};
+            if (!incontig) {
+                rec.status = -1;
+                a.out[idx] = rec;
+                break;
+            }
+            const uint32_t na = a.anc_n[idx];
+            const uint32_t *anc = a.anchors + 2 * a.anc_off[idx];
+            // conditions 2-4 (align.rs:2284-2305)
+            bool ok = na > 0;
+            int64_t diag = 0;
+            uint32_t qstart = 0, qend = 0, rs = 0;
+            if (ok) {
+                qstart = anc[0];
+                rs = anc[1];
+                diag = (int64_t)rs - (int64_t)qstart;
+                uint32_t prevq = qstart;
+                for (uint32_t j = 1; j < na && ok; j++) {
+                    const uint32_t q = anc[2 * j], r = anc[2 * j + 1];
+                    ok = (int64_t)r - (int64_t)q == diag && q - prevq <= a.k + a.w + 5u;
+                    prevq = q;
+                }
+                qend = prevq + a.k;
+                ok = ok && qstart <= 2 && m - qend <= 2;
+            }
+            if (!ok) {
+                a.out[idx] = rec;
+                break;
+            }
+            const uint32_t flank = m / 2 + 30;
+            const uint32_t wstart = rs > flank ? rs - flank : 0u;
+            const uint32_t *pk = (const uint32_t *)(a.R.packed + a.R.pk_off[contig]);
+            const uint32_t clen = a.R.len[contig];
+            const uint32_t *iv = a.R.iv + 2 * a.R.iv_off[contig];
+            const uint32_t niv = (uint32_t)(a.R.iv_off[contig + 1] - a.R.iv_off[contig]);
+            const uintptr_t qaddr = (uintptr_t)(a.reads + roff);
+            const uint32_t *qw = (const uint32_t *)(qaddr & ~(uintptr_t)3);
+            // the lane's query dwords in the wave's LDS copy, when the round was copied
+            const uint32_t *sq = sbase ? stage + (uint32_t)(((uintptr_t)qw - sbase) >> 2) : nullptr;
+            const uint32_t qoff0 = (uint32_t)(qaddr & 3u);
+            const uint32_t qjmax = m ? (qoff0 + m - 1) >> 2 : 0u;
+            // dwords the contig's region holds: ceil(len / 4) + 16 bytes, rounded up to 16
+            const uint32_t pklast = ((((clen + 3u) >> 2) + 16u + 15u) & ~15u) / 4u - 1u;
+            const uint32_t exag = a.edit_dir == 1 ? 0xFFFFFFFFu : 0u, extc = a.edit_dir == 2 ? 0xFFFFFFFFu : 0u;
+            uint32_t ci = 0;
+            {
+                const int64_t p0 = diag > 0 ? diag : 0;
+                ci = niv ? iv_seek(iv, niv, (uint32_t)min(p0, (int64_t)0xFFFFFFFF)) : 0u;
+            }
+            FastKadane K;
+            for (uint32_t i0 = 0; i0 < m; i0 += FT) {
+                // one tile: its query and reference words load at once, then the
+                // FT columns run from registers
+                const uint32_t ncol = min(m - i0, FT);
+                uint32_t q[FT / 4 + 1], r[FT / 16 + 1];
+                const uint32_t j0 = (qoff0 + i0) >> 2;
+                if (sq) {
 #pragma unroll
-            for (uint32_t j = 0; j <= FT / 16; j++) r[j] = pk[min(d0 + j, pklast)];
-            if (ncol == FT)
-                fast_tile<true>(a, K, lut, q, r, qsh, fast, p0, i0, FT, pk, pklast, iv, niv, clen, exag, extc);
-            else
-                fast_tile<false>(a, K, lut, q, r, qsh, fast, p0, i0, ncol, pk, pklast, iv, niv, clen, exag, extc);
-        }
-        const int32_t best = K.best;
-        const uint32_t bs = K.bs, be = K.be, bmm = K.bmm;
-        if (best > 0 && bmm == 0) {
-            uint32_t *cg = a.cigar + 3 * (size_t)idx;
-            uint32_t n = 0;
-            if (bs > 0) cg[n++] = bs << 4 | OPS;
-            cg[n++] = (be - bs) << 4 | OPM;
-            if (m > be) cg[n++] = (m - be) << 4 | OPS;
-            rec.status = 1;
-            rec.score = best;
-            rec.wstart = wstart;
-            rec.ref_start = (uint32_t)((int64_t)bs + diag - (int64_t)wstart);
-            rec.read_start = bs;
-            rec.read_end = be;
-            rec.pos = wstart + rec.ref_start;
-            rec.n_ops = n;
-        }
-        a.out[idx] = rec;
+                    for (uint32_t j = 0; j <= FT / 4; j++) q[j] = sq[min(j0 + j, qjmax)];
+                } else if (j0 + FT / 4 <= qjmax) {
+#pragma unroll
+                    for (uint32_t j = 0; j < FT / 16; j++) {
+                        const u32x4 v = *(const u32x4 *)(qw + j0 + 4 * j);
+                        q[4 * j] = v.x, q[4 * j + 1] = v.y, q[4 * j + 2] = v.z, q[4 * j + 3] = v.w;
+                    }
+                    q[FT / 4] = qw[j0 + FT / 4];
+                } else {
+#pragma unroll
+                    for (uint32_t j = 0; j <= FT / 4; j++) q[j] = qw[min(j0 + j, qjmax)];
+                }
+                const uint32_t qsh = (qoff0 + i0) & 3u;
+                const int64_t p0 = (int64_t)i0 + diag;
+                while (ci < niv && (int64_t)iv[2 * ci + 1] <= p0) ci++;
+                const bool inside = p0 >= 0 && p0 + FT <= (int64_t)clen;
+                const bool clear = ci >= niv || (int64_t)iv[2 * ci] >= p0 + FT;
+                const bool fast = inside && clear;
+                const uint32_t d0 = fast ? (uint32_t)p0 >> 4 : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j <= FT / 16; j++) r[j] = pk[min(d0 + j, pklast)];
+                if (ncol == FT)
+                    fast_tile<true>(a, K, lut, q, r, qsh, fast, p0, i0, FT, pk, pklast, iv, niv, clen, exag, extc);
+                else
+                    fast_tile<false>(a, K, lut, q, r, qsh, fast, p0, i0, ncol, pk, pklast, iv, niv, clen, exag, extc);
+            }
+            const int32_t best = K.best;
+            const uint32_t bs = K.bs, be = K.be, bmm = K.bmm;
+            if (best > 0 && bmm == 0) {
+                uint32_t *cg = a.cigar + 3 * (size_t)idx;
+                uint32_t n = 0;
+                if (bs > 0) cg[n++] = bs << 4 | OPS;
+                cg[n++] = (be - bs) << 4 | OPM;
+                if (m > be) cg[n++] = (m - be) << 4 | OPS;
+                rec.status = 1;
+                rec.score = best;
+                rec.wstart = wstart;
+                rec.ref_start = (uint32_t)((int64_t)bs + diag - (int64_t)wstart);
+                rec.read_start = bs;
+                rec.read_end = be;
+                rec.pos = wstart + rec.ref_start;
+                rec.n_ops = n;
+            }
+            a.out[idx] = rec;
+        } while (0);
+        // the next round's copy overwrites this one only after every lane's
+        // reads of it (same-wave LDS operations complete in order)
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -800,6 +857,13 @@ __global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
 int walk_blocks_per_cu() {
     static int nb = 0;
     if (!nb && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk, 256, 0) != hipSuccess || nb < 1))
+        nb = 1;
+    return nb;
+}
+
+int fast_lane_blocks_per_cu() {
+    static int nb = 0;
+    if (!nb && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fast_lane, 256, 0) != hipSuccess || nb < 1))
         nb = 1;
     return nb;
 }
