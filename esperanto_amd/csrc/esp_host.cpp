@@ -344,10 +344,10 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
     //    escalated (its count is known only on the device; the grid drains
     //    the list and exits).  aux already holds 1 for these pairs.
     //    With pruning the list is first sorted by (head groups, band-maximum
-    //    diagonal).  Pass A takes the sorted list 64 pairs a wave; pass B takes
-    //    A's failures one pair a wave with a wider stripe and a deeper head;
-    //    the unpruned kernel drains B's failures and the pairs no pruned pass
-    //    can help (usually nothing: it then exits at once).
+    //    diagonal).  The pruned pass takes the sorted list 64 pairs a wave;
+    //    its failures and the pairs it cannot help rerun one wave per pair
+    //    over the whole matrix (up to a list cap); the unpruned lane kernel
+    //    drains the rest (usually nothing: it then exits at once).
     if (pa.hint_mode && esc_bound) {
         espk::GotohArgs f = a;
         f.mode = ESP_MODE_FULL;
