@@ -102,7 +102,7 @@ struct esp_ctx {
     size_t trace_budget = esph::kTraceBudget;
     size_t full_trace_budget = esph::kFullTraceBudget;
     int full_kernel = 0;  // extend_hint reruns: 0 by batch size, 1 lane per pair
-    int full_prune = 1;   // extend_hint reruns: pruned pass first (1) or the full matrix only (0)
+    int full_prune = 1;   // extend_hint reruns: pruned passes first (1; 2 without the register-window pass) or the full matrix only (0)
     int plain_kernel = 0;  // single-pair calls: 0 one wave per pair (windows up to 4096), 1 one lane
     bool last_pruned = false;  // the last batch ran the pruned reruns (their counts are in escbuf)
     bool last_wave_direct = false;  // ... with every escalation sent to the one-wave rerun first
@@ -127,6 +127,10 @@ struct esp_ctx {
     // tile sums of the prefix scans that lay out the junction-side arenas
     esph::DevBuf scantmp;
     hipEvent_t ev_plan = nullptr;
+    // A second stream (default priority) for the whole-tile pruned pass,
+    // which runs beside the register-window pass; fork and join events.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int band_blocks[espk::PLAN_STRIP] = {};
     esph::HostBuf hstage, hout, hplan;
     // Set while a graph records a call on this context (see CaptureBounds).

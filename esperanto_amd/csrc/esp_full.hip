@@ -467,6 +467,301 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
     }
 }
 
+// ---- Pruned pass over a register window ---------------------------------
+//
+// k_gotoh_full keeps every row of the tile in registers (one VGPR a row),
+// which holds it at two waves per SIMD.  A pair whose head is at most PR_HG
+// groups (the escalation sort's first class) only ever computes its head
+// groups and the stripe, which moves down at most one group a column, so
+// k_gotoh_pruned keeps just those: the head rows in H[8 * PR_HG] and the
+// stripe's groups in a window S[8 * PR_SG] whose base group sb (uniform)
+// follows the stripe.  When sb moves down one group, the window shifts by
+// one group (the leaving group's last row is kept as the next group's
+// diagonal) and the entering group starts from margins, as every group does
+// when it first enters the stripe in the whole-tile pass.  The computed set
+// is the whole-tile pass's (head groups, and the stripe groups from the
+// first non-head one, at most PR_SG of them), the stripe's half-width is
+// narrowed so its rows span at most PR_SG groups, and the exit checks are
+// the same: a pair that fails them goes to the fallback list untraced.  The
+// whole-tile pass runs the other head classes at the same time on a second
+// stream.
+constexpr int PR_HG = 3, PR_SG = 8, PR_CHUNK = 32;
+
+// One group of 8 rows (i0 + 1 .. i0 + 8) of a column: P[r] holds row i0 + r
+// + 1's packed (F + ext, H + oe) of the previous column and receives this
+// column's; Uc, Bc carry H + oe and E + ext of the cell above and pdiag the
+// row above's packed value of the previous column; rc points at row i0 + 1's
+// read code.  Folds the rows' keys into ckey and returns the trace word.
+template <bool FOLD>
+__device__ __forceinline__ uint32_t group8(uint32_t *P, const uint8_t *rc, uint32_t clut, int32_t i0, int32_t zr,
+                                           int32_t noev, int32_t oev, int32_t extv, int32_t &Uc, int32_t &Bc,
+                                           uint32_t &pdiag, int32_t &ckey) {
+    uint32_t acc = 0;
+    int32_t kpend = 0;
+    // the group's read codes load together (one LDS round trip a group)
+    uint32_t rcs[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) rcs[r] = rc[r * 64];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int32_t i = i0 + r + 1;
+        const uint32_t pk = P[r];
+        const uint32_t rc5 = rcs[r];
+        int32_t sc = __builtin_amdgcn_sbfe((int32_t)clut, rc5, 5);
+        if constexpr (!FOLD) sc += noev;
+        int32_t D, F, x3;
+        asm("v_add_u32_sdwa %0, sext(%1), %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:WORD_0 src1_sel:DWORD"
+            : "=v"(D) : "v"(pdiag), "v"(sc));
+        const int32_t E = max(max(Uc, Bc), zr);
+        asm("v_max_i32_sdwa %0, sext(%1), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:WORD_0 src1_sel:WORD_1"
+            : "=v"(F) : "v"(pk));
+        asm("v_sub_u32_sdwa %0, sext(%1), sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:WORD_0 src1_sel:WORD_1"
+            : "=v"(x3) : "v"(pk));
+        const int32_t H = max(max(D, E), F);
+        const uint32_t x0 = (uint32_t)(D - H), x1 = (uint32_t)(E - H);
+        const uint32_t x2 = (uint32_t)(Uc - Bc);
+        asm("v_alignbit_b32 %0, %0, %1, 31\n\t"
+            "v_alignbit_b32 %0, %0, %2, 31\n\t"
+            "v_alignbit_b32 %0, %0, %3, 31\n\t"
+            "v_alignbit_b32 %0, %0, %4, 31"
+            : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"((uint32_t)x3));
+        const int32_t Un = H + oev;
+        pdiag = pk;
+        uint32_t pn = (uint32_t)Un;
+        pack_hi(pn, F, extv);
+        P[r] = pn;
+        Uc = Un;
+        Bc = E + extv;
+        if (r & 1) {
+            int32_t kk;
+            asm("v_lshl_or_b32 %1, %2, 16, %4\n\t"
+                "v_max3_i32 %0, %0, %3, %1"
+                : "+v"(ckey), "=&v"(kk) : "v"(H), "v"(kpend), "s"(0xFFFF - i));
+        } else {
+            asm("v_lshl_or_b32 %0, %1, 16, %2" : "=v"(kpend) : "v"(H), "s"(0xFFFF - i));
+        }
+    }
+    return acc;
+}
+
+template <int MAXM, bool FOLD>
+__global__ __launch_bounds__(64, 3) void k_gotoh_pruned(GotohArgs a) {
+    static_assert(MAXM % 8 == 0 && MAXM <= 8 * 32, "rows come in groups of 8");
+    constexpr int WR = MAXM / 8;  // trace words per column (the whole-tile layout)
+    extern __shared__ uint8_t lds[];
+    // LDS: column LUTs (8 words), read codes x5 of rows 0..MAXM, window chunk
+    uint32_t *flut = (uint32_t *)lds;
+    const uint32_t lane = threadIdx.x;
+    uint8_t *rcode = lds + 32 + lane;                    // row i at rcode[i * 64]
+    uint8_t *chunk = lds + 32 + (MAXM + 1) * 64 + lane;  // column q at chunk[q * 64]
+    const GParams P = a.P;
+    const int32_t oe = P.oe, ext = P.ext, noe = -oe;
+    const int32_t zr = a.zero;
+    int32_t oev, extv, noev;
+    asm("v_mov_b32 %0, %1" : "=v"(oev) : "s"(oe));
+    asm("v_mov_b32 %0, %1" : "=v"(extv) : "s"(ext));
+    asm("v_mov_b32 %0, %1" : "=v"(noev) : "s"(noe));
+    const uint32_t first = a.range[0], total = a.range[1];
+    const uint32_t nunits = (total + 63) / 64;
+    if (lane < 8) flut[lane] = FOLD ? a.flutf[lane < 6 ? lane : 5] : a.flut[lane < 6 ? lane : 5];
+    for (int q = 0; q < PR_CHUNK; q++) chunk[q * 64] = 0;
+    __syncthreads();
+    uint32_t *tr = a.trace + (size_t)blockIdx.x * a.maxw * 64 + lane;
+    const uint32_t pinit = ((uint32_t)oe & 0xFFFFu) | ((uint32_t)NEG16 << 16);  // H = 0, F = NEG
+    for (;;) {
+        const uint32_t u = wave_pop(a.counter, lane, 1u);
+        if (u >= nunits) break;
+        const uint32_t ubegin = first + 64 * u;
+        const uint32_t ucount = min(64u, total - 64 * u);
+        const uint32_t p = a.order[ubegin + (lane < ucount ? lane : 0u)];
+        const bool active = lane < ucount && a.read_len[p] <= (uint32_t)MAXM &&
+                            (uint64_t)a.win_len[p] * WR <= (uint64_t)a.maxw;
+        PairView pv;
+        pv.read = a.reads + a.read_off[p];
+        pv.win = a.wins + a.win_off[p];
+        pv.m = active ? (int32_t)a.read_len[p] : 0;
+        pv.n = active ? (int32_t)a.win_len[p] : 0;
+        pv.off = 0;
+        pv.half = pv.n;
+        const int32_t m = pv.m, n = pv.n;
+        const int32_t S = a.prune_s;
+        const int32_t L = active ? a.esc_info[2 * (size_t)p] : 0;
+        const int32_t d = active ? a.esc_info[2 * (size_t)p + 1] : 0;
+        const int32_t tbase = L - S * m;
+        int32_t umaxm = m, umaxn = n;
+        int32_t nh = active ? esc_head_groups(m, L, S, a.prune_pad) : 0;
+        int32_t lo = active ? d : 0x7FFFFFFF, hi = active ? d : -0x7FFFFFFF;
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+            umaxm = max(umaxm, __shfl_xor(umaxm, sh));
+            umaxn = max(umaxn, __shfl_xor(umaxn, sh));
+            nh = max(nh, __shfl_xor(nh, sh));
+            lo = min(lo, __shfl_xor(lo, sh));
+            hi = max(hi, __shfl_xor(hi, sh));
+        }
+        // uniform after the reductions: scalar copies keep the column loop
+        // and the group branches scalar
+        const int32_t ngroups = __builtin_amdgcn_readfirstlane((umaxm + 7) >> 3);
+        umaxn = __builtin_amdgcn_readfirstlane(umaxn);
+        const int32_t nhead = __builtin_amdgcn_readfirstlane(min(nh, ngroups));
+        const int32_t dmin = __builtin_amdgcn_readfirstlane(lo);
+        const int32_t dmax = __builtin_amdgcn_readfirstlane(hi);
+        // the stripe's half-width: at most prune_half, and narrow enough that
+        // its 2 hw + 1 + (dmax - dmin) rows span at most PR_SG groups (a wave
+        // whose diagonals spread further, as when few pairs share each
+        // diagonal, computes a narrower stripe than its pairs need, and the
+        // pairs the exit checks cannot prove fail over)
+        const int64_t room = ((int64_t)8 * PR_SG - 9 - ((int64_t)dmax - (int64_t)dmin)) / 2;
+        const int32_t hw = (int32_t)max((int64_t)0, min((int64_t)a.prune_half, room));
+        // a head wider than the registers (the sort's class rules it out)
+        // fails over whole
+        bool fail = nhead > PR_HG;
+        // the stripe's groups at column jj: [sbase, stop] (empty when stop < sbase)
+        auto stripe = [&](int32_t jj, int32_t &sbase, int32_t &stop) {
+            const int32_t rlo = jj - dmax - hw, rhi = jj - dmin + hw;
+            const int32_t glo = rlo <= 1 ? 0 : (rlo - 1) >> 3;
+            const int32_t ghi = min(rhi < 1 ? -1 : (rhi - 1) >> 3, ngroups - 1);
+            sbase = max(glo, nhead);
+            stop = min(ghi, sbase + PR_SG - 1);
+        };
+        for (int32_t i = 1; i <= 8 * ngroups; i++)
+            rcode[(size_t)i * 64] = (uint8_t)(5 * (i <= m ? read_code(pv.read[i - 1]) : 5u));
+        uint32_t Hr[8 * PR_HG], Sr[8 * PR_SG];
+#pragma unroll
+        for (int r = 0; r < 8 * PR_HG; r++) Hr[r] = pinit;
+#pragma unroll
+        for (int r = 0; r < 8 * PR_SG; r++) Sr[r] = pinit;
+        // Columns run in stretches over which the window stays put and no
+        // chunk is staged, so the shift and the staging sit outside the
+        // per-column loop.
+        int32_t sbc, topc;  // the stripe of the next column to compute
+        stripe(1, sbc, topc);
+        int32_t sbw = sbc;  // the group Sr[0..8) holds
+        int32_t bestkey = 0xFFFF, bj = 0;
+        uint32_t ccells = 0, cgroups = 0;
+        int32_t j = fail ? umaxn + 1 : 1;
+        while (j <= umaxn) {
+            // the window follows the stripe down: sb grows by at most one
+            // group a column (the stripe's first row grows by one)
+            uint32_t drop = (uint32_t)oe & 0xFFFFu;
+            if (sbw < sbc) {
+                drop = Sr[7];
+#pragma unroll
+                for (int r = 0; r < 8 * (PR_SG - 1); r++) Sr[r] = Sr[r + 8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) Sr[8 * (PR_SG - 1) + r] = pinit;
+                sbw = sbc;
+            }
+            if (((j - 1) & (PR_CHUNK - 1)) == 0) {
+                const int32_t j0 = j - 1;
+                stage_win(pv.win + j0, max(0, min(PR_CHUNK, n - j0)), chunk);
+            }
+            bool more;
+            do {
+                const uint32_t clut = flut[chunk[((j - 1) & (PR_CHUNK - 1)) * 64]];
+                const int32_t sb = sbc, top = topc;
+                int32_t sbn, topn;
+                stripe(j + 1, sbn, topn);
+                int32_t mc = m, tbc = tbase;
+                asm volatile("" : "+v"(mc), "+v"(tbc));
+                // row 0: H(0, j) = 0, E(0, j) = NEG
+                int32_t Uc = oe, Bc = NEG + ext, ckey = 0;
+                uint32_t pdiag = (uint32_t)oe & 0xFFFFu;
+                uint32_t *tcol = tr + (size_t)(j - 1) * WR * 64;
+                // exit check of row i (H = the low half of its new packed
+                // value minus oe): a right or diagonal exit when `side`, a
+                // down or diagonal exit when `down`
+                auto check = [&](uint32_t pk, int32_t i, bool side, bool down) {
+                    const int32_t H = (int32_t)(int16_t)(pk & 0xFFFFu) - oe;
+                    fail |= ((side && j < n && i <= mc) || (down && j <= n && i < mc)) && H >= tbc + S * i;
+                };
+                // head groups: computed at every column; the last one's bottom
+                // row exits down unless the stripe continues it at j and j + 1
+                const bool below = sb == nhead && top >= sb && sbn == nhead && topn >= sbn;
+#pragma unroll
+                for (int g = 0; g < PR_HG; g++) {
+                    if (g < nhead) {
+                        tcol[g * 64] = group8<FOLD>(&Hr[8 * g], rcode + (size_t)(8 * g + 1) * 64, clut, 8 * g, zr,
+                                                    noev, oev, extv, Uc, Bc, pdiag, ckey);
+                        if (g == nhead - 1 && !below) check(Hr[8 * g + 7], 8 * g + 8, false, true);
+                    }
+                }
+                if (top >= sb) {
+                    if (sb != nhead) {
+                        // the group above is not computed here: margins, and
+                        // the diagonal from its last row if it was computed at
+                        // j - 1 (it has just left the window)
+                        Uc = oe;
+                        Bc = NEG + ext;
+                        pdiag = drop;
+                    }
+#pragma unroll
+                    for (int t = 0; t < PR_SG; t++) {
+                        if (sb + t <= top) {
+                            const int32_t g = sb + t;
+                            tcol[(size_t)g * 64] = group8<FOLD>(&Sr[8 * t], rcode + (size_t)(8 * g + 1) * 64, clut,
+                                                                8 * g, zr, noev, oev, extv, Uc, Bc, pdiag, ckey);
+                            // the top group leaving at j + 1: every row exits
+                            // right and diagonally; the bottom group: its last
+                            // row exits down
+                            const bool all = t == 0 && sbn > sb, last = g == top;
+                            if (all || last) {
+#pragma unroll
+                                for (int r = 0; r < 8; r++)
+                                    if (all || r == 7) check(Sr[8 * t + r], 8 * g + r + 1, all, last && r == 7);
+                            }
+                        }
+                    }
+                }
+                if (j <= n && ckey > bestkey) {
+                    bestkey = ckey;
+                    bj = j;
+                }
+                if (a.census) {
+                    const uint32_t mk = ((1u << nhead) - 1u) | (top >= sb ? (2u << top) - (1u << sb) : 0u);
+                    const uint32_t gm = (uint32_t)m >> 3;
+                    const uint32_t rows = 8u * (uint32_t)__builtin_popcount(mk & ((1u << gm) - 1u)) +
+                                          (((mk >> gm) & 1u) ? ((uint32_t)m & 7u) : 0u);
+                    ccells += j <= n ? rows : 0u;
+                    cgroups += (uint32_t)__builtin_popcount(mk);
+                }
+                drop = (uint32_t)oe & 0xFFFFu;
+                sbc = sbn;
+                topc = topn;
+                j++;
+                more = j <= umaxn && sbc == sbw && ((j - 1) & (PR_CHUNK - 1)) != 0;
+            } while (more);
+        }
+        if (a.census) {
+            if (active) atomicAdd(&a.census[0], (unsigned long long)ccells);
+            if (lane == 0) atomicAdd(&a.census[1], (unsigned long long)cgroups * 8ull * 64ull);
+        }
+        if (active && fail) {
+            a.fb_list[atomicAdd(&a.fb_range[1], 1u)] = p;
+        } else if (active) {
+            int32_t best = bestkey >> 16, bi = 0xFFFF - (bestkey & 0xFFFF);
+            if (best <= 0) {
+                best = 0;
+                bi = 0;
+                bj = 0;
+            }
+            const uint64_t c0 = a.cig_off[p];
+            const uint32_t cap = (uint32_t)(a.cig_off[p + 1] - c0);
+            uint32_t *cslot = a.cigar + c0;
+            TbResult tb = traceback_full<WR>(pv, P, rcode, tr, best, bi, bj, cslot, cap);
+            esp_ext e;
+            uint32_t nops;
+            finish_pair(pv, P, best, tb, cslot, cap, &e, &nops);
+            a.ext[p] = e;
+            a.cig_n[p] = nops;
+        }
+    }
+}
+
 template <int MAXM>
 size_t full_lds() { return 32 + (size_t)(MAXM + 1) * 64 + (size_t)FULL_CHUNK * 64; }
 
@@ -488,6 +783,27 @@ hipError_t launch_full_t(const GotohArgs &a, bool prune, int blocks, hipStream_t
         if (prune) return launch_full_p<MAXM, true>(a, blocks, s);
     if (prune) return hipErrorInvalidValue;
     return launch_full_p<MAXM, false>(a, blocks, s);
+}
+
+template <int MAXM>
+size_t pruned_lds() { return 32 + (size_t)(MAXM + 1) * 64 + (size_t)PR_CHUNK * 64; }
+
+template <int MAXM>
+hipError_t launch_pruned_t(const GotohArgs &a, int blocks, hipStream_t s) {
+    if (a.flut_folded)
+        hipLaunchKernelGGL((k_gotoh_pruned<MAXM, true>), dim3(blocks), dim3(64), pruned_lds<MAXM>(), s, a);
+    else
+        hipLaunchKernelGGL((k_gotoh_pruned<MAXM, false>), dim3(blocks), dim3(64), pruned_lds<MAXM>(), s, a);
+    return hipGetLastError();
+}
+
+template <int MAXM>
+int pruned_blocks_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gotoh_pruned<MAXM, true>, 64, pruned_lds<MAXM>()) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 
 template <int MAXM>
@@ -515,6 +831,25 @@ hipError_t launch_gotoh_full(const GotohArgs &a, int rows, bool prune, int block
     }
 }
 
+hipError_t launch_gotoh_pruned(const GotohArgs &a, int rows, int blocks, hipStream_t s) {
+    switch (rows) {
+    case 64: return launch_pruned_t<64>(a, blocks, s);
+    case 128: return launch_pruned_t<128>(a, blocks, s);
+    case 160: return launch_pruned_t<160>(a, blocks, s);
+    case 192: return launch_pruned_t<192>(a, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int gotoh_pruned_blocks_per_cu(int rows) {
+    switch (rows) {
+    case 64: return pruned_blocks_t<64>();
+    case 128: return pruned_blocks_t<128>();
+    case 160: return pruned_blocks_t<160>();
+    default: return pruned_blocks_t<192>();
+    }
+}
+
 namespace {
 
 // Escalation sort key: head-group class * ESC_NDIAG + diagonal bucket, or
@@ -528,7 +863,7 @@ __device__ __forceinline__ uint32_t esc_key(const EscSortArgs &a, uint32_t p) {
     const int32_t nh = esc_head_groups(m, L, a.s, a.pad);
     if (nh > 12 || 8 * nh >= m) return ESC_NB;
     const int32_t cls = nh <= 3 ? 0 : (nh <= 6 ? 1 : 2);
-    const int32_t db = min(max((d + 256) >> 1, 0), ESC_NDIAG - 1);
+    const int32_t db = min(max((d + 256) >> 2, 0), ESC_NDIAG - 1);
     return (uint32_t)(cls * ESC_NDIAG + db);
 }
 
@@ -583,7 +918,19 @@ __global__ __launch_bounds__(1024) void k_esc_scan(EscSortArgs a) {
     a.esc->cursor[3 * t] = base;
     a.esc->cursor[3 * t + 1] = base + h0;
     a.esc->cursor[3 * t + 2] = base + h0 + h1;
-    if (t == 1023) a.esc->prange[1] = part[t];
+    // the first class (heads of at most 3 groups) ends where bucket
+    // ESC_NDIAG begins
+    __shared__ uint32_t n0;
+    static_assert(ESC_NDIAG % 3 == 1, "bucket ESC_NDIAG is a thread's second");
+    if (t == ESC_NDIAG / 3) n0 = base + h0;
+    __syncthreads();
+    if (t == 1023) {
+        a.esc->prange[1] = part[t];
+        a.esc->crange0[0] = 0;
+        a.esc->crange0[1] = n0;
+        a.esc->crange1[0] = n0;
+        a.esc->crange1[1] = part[t] - n0;
+    }
 }
 
 // The same chunks as k_esc_hist: count per bucket in LDS, reserve each

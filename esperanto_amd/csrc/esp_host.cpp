@@ -373,7 +373,25 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
         const uint64_t blocks_w = std::max<uint64_t>(
             1, std::min<uint64_t>({(uint64_t)c->ncu * espk::gotoh_wave_blocks_per_cu(wave_ln, 0), bound,
                                    std::max<uint64_t>(1, c->full_trace_budget / (std::max<uint64_t>(wave_w, 1) * 4))}));
-        HIP_TRY(c, c->ftrace.ensure(std::max(maxw * 64 * 4 * blocks, prune ? wave_w * 4 * blocks_w : 0)));
+        // The register-window pass takes the sort's first head class (more
+        // waves per SIMD than the whole-tile pass), the whole-tile pass the
+        // other classes at the same time on the second stream: each pass's
+        // waves sweep whole windows, so one after the other would cost the
+        // second pass's one-wave latency on top of the first's.  Their trace
+        // slabs are disjoint: the whole-tile pass's follow the first pass's.
+        // Short windows keep the whole-tile pass alone: there the stripe is
+        // most of each matrix, and the window's bookkeeping costs more than
+        // the extra waves per SIMD return.
+        const bool windowed = pruned && c->full_prune == 1 && esc_maxn >= espk::PRUNE_WINDOW_MIN_N;
+        uint64_t blocks0 = 0, blocks1 = blocks;
+        if (windowed) {
+            blocks1 = std::max<uint64_t>(1, std::min<uint64_t>(blocks, std::max<uint64_t>(1, cap / 3)));
+            blocks0 = std::max<uint64_t>(
+                1, std::min({(uint64_t)c->ncu * espk::gotoh_pruned_blocks_per_cu(rows), (bound + 63) / 64,
+                             cap > blocks1 ? cap - blocks1 : (uint64_t)1}));
+        }
+        HIP_TRY(c, c->ftrace.ensure(std::max({maxw * 64 * 4 * (blocks0 + (windowed ? blocks1 : blocks)),
+                                              prune ? wave_w * 4 * blocks_w : 0})));
         f.trace = (uint32_t *)c->ftrace.p;
         f.maxw = (uint32_t)maxw;
         if (c->census && !c->cap) {
@@ -410,7 +428,21 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
             g.fb_list = fblist;
             g.fb_range = esc->fbrange;
             g.census_first = 0;  // the sort counted the matrices
-            HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
+            if (windowed) {
+                espk::GotohArgs g0 = g;
+                g0.range = esc->crange0;
+                g0.counter = &esc->c0counter;
+                g.range = esc->crange1;
+                g.trace = (uint32_t *)c->ftrace.p + blocks0 * maxw * 64;
+                HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
+                HIP_TRY(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+                HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks1, c->stream2));
+                HIP_TRY(c, hipEventRecord(c->ev_join, c->stream2));
+                HIP_TRY(c, espk::launch_gotoh_pruned(g0, rows, (int)blocks0, c->stream));
+                HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+            } else {
+                HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
+            }
             tim_mark(c, 7, rows);
             // What the pass could not prove, and the pairs the sort found no
             // pass could help: one wave per pair over the whole matrix, up to
@@ -492,7 +524,10 @@ int esp_create(int device, esp_ctx **out) {
     esph::noalloc_setup(device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return ESP_E_DEVICE;
     }
@@ -519,6 +554,10 @@ int esp_destroy(esp_ctx *c) {
     c->btrace.release();
     c->ftrace.release();
     if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     c->stage.release();
     c->stage_out.release();
     c->wquery.release();
@@ -624,7 +663,7 @@ int esp_ctx_set_option(esp_ctx *c, int option, int64_t value) {
         c->plain_kernel = (int)value;
         return ESP_OK;
     case ESP_OPT_FULL_PRUNE:
-        if (value < 0 || value > 1) return fail(c, ESP_E_INVAL, "full-matrix pruning not in {0,1}");
+        if (value < 0 || value > 2) return fail(c, ESP_E_INVAL, "full-matrix pruning not in {0,1,2}");
         c->full_prune = (int)value;
         return ESP_OK;
     case ESP_OPT_FULL_PRUNE_SHAPE: {

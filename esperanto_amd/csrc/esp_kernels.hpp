@@ -244,6 +244,13 @@ __host__ __device__ inline uint64_t band_trace_words(uint64_t m, uint64_t half) 
 int full_rows(uint32_t maxm);
 constexpr int FULL_PRUNE_MAX_ROWS = 192;  // largest row tile with a pruned pass
 hipError_t launch_gotoh_full(const GotohArgs &a, int rows, bool prune, int blocks, hipStream_t s);
+// The pruned pass over a register window (head and stripe groups only) for
+// the escalation sort's first head class; rows 64..192.
+hipError_t launch_gotoh_pruned(const GotohArgs &a, int rows, int blocks, hipStream_t s);
+// The register-window pass runs when the batch's longest escalated window
+// has at least this many bases.
+constexpr uint64_t PRUNE_WINDOW_MIN_N = 512;
+int gotoh_pruned_blocks_per_cu(int rows);
 int gotoh_full_blocks_per_cu(int rows);
 
 // Ordering of the escalation list for the pruned pass: a counting sort by
@@ -254,11 +261,14 @@ int gotoh_full_blocks_per_cu(int rows);
 // stripe row costs the read's span of columns, a head row the whole window.
 // Its failures go to the one-pair-per-wave kernel (launch_gotoh_wave_list).
 constexpr int ESC_HALF_A = 32, ESC_PAD_A = 16;
-constexpr int ESC_NDIAG = 1024;    // diagonal buckets per class (two diagonals each)
+constexpr int ESC_NDIAG = 1024;    // diagonal buckets per class (four diagonals each)
 constexpr int ESC_NCLASS = 3;      // head groups <= 3, <= 6, <= 12
 constexpr int ESC_NB = ESC_NCLASS * ESC_NDIAG;
 struct EscDev {
     uint32_t prange[2];   // {0, pairs in the sorted list}
+    uint32_t crange0[2];  // {0, pairs of the first head class}: the register-window pass
+    uint32_t crange1[2];  // {first class's count, the other classes' pairs}: the whole-tile pass
+    uint32_t c0counter;   // work-queue head of the register-window pass
     uint32_t fbrange[2];  // {0, pairs the sort or the pruned pass left for the one-wave rerun}
     uint32_t fb2range[2]; // {0, pairs for the unpruned kernel}
     uint32_t pcounter, fbcounter, fb2counter;  // work-queue heads (fbcounter unused)

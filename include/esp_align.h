@@ -244,9 +244,13 @@ int esp_last_timings(esp_ctx *ctx, esp_timing *out, size_t cap, size_t *n);
  * the lane-per-pair kernel in this build (2, wave per pair, is refused).
  * ESP_OPT_TRACE_BUDGET caps the traceback scratch of one launch in bytes
  * (the full-matrix kernels get twice that); grids shrink to fit.
- * ESP_OPT_FULL_PRUNE (1, the default, or 0): the full-matrix reruns first
+ * ESP_OPT_FULL_PRUNE (1, the default, 2 or 0): the full-matrix reruns first
  * take a pruned pass that computes only the cells that can still reach the
- * band attempt's score and proves per pair that no skipped cell could; the
+ * band attempt's score and proves per pair that no skipped cell could (1:
+ * when the longest escalated window has 512 bases or more, pairs with a
+ * head of at most 3 row groups in a pass that holds only the head and the
+ * stripe in registers, beside the whole-tile pass for the rest; 2: all in
+ * the whole-tile pass); the
  * pairs it cannot prove rerun one wavefront per pair over the whole matrix
  * (windows up to 4096 bases; longer ones in the unpruned lane kernel);
  * results are the same either way.  ESP_OPT_FULL_PRUNE_SHAPE sets the

@@ -56,12 +56,14 @@ def _subst(rng, seq, rate):
     return bytes(s)
 
 
-def _mate_like(rng, n):
-    """150-nt reads with 1% substitutions in 1,950-nt windows, hint = the planted start."""
+def _mate_like(rng, n, starts=None):
+    """150-nt reads with 1% substitutions in 1,950-nt windows, hint = the
+    planted start (drawn from `starts` when given: many pairs a diagonal, as
+    in a large batch)."""
     out = []
     for _ in range(n):
         w = _rand(rng, 1950)
-        p = int(rng.integers(0, 1950 - 150))
+        p = int(rng.integers(0, 1950 - 150)) if starts is None else int(rng.choice(starts))
         out.append((_subst(rng, w[p:p + 150], 0.01), w, p, 0))
     return out
 
@@ -124,6 +126,8 @@ def _long_reads(rng, n):
 
 
 def _run(buf, part, params, prune):
+    """prune: 0 off, 1 the register-window pass for small heads beside the
+    whole-tile pass for the rest, 2 the whole-tile pass for all."""
     from esperanto_amd import _native
     from esperanto_amd.batch import DeviceBatch
     buf.set_option(_native.ESP_OPT_FULL_PRUNE, int(prune))
@@ -156,20 +160,26 @@ def test_pruned_full_matrix_equals_oracle_and_unpruned(buf, scoring):
     params = ExtendParams() if scoring == "default" else ExtendParams(
         match_score=1, mismatch=-3, gap_open=5, gap_ext=1, edit_dir=EditDir.Ag)
     rng = np.random.default_rng(91 if scoring == "default" else 92)
+    starts = [int(x) for x in rng.integers(0, 1950 - 150, 12)]
     for label, pairs in (("mate-like", _mate_like(rng, 1500)),
+                         ("mate-like, shared diagonals", _mate_like(rng, 1500, starts)),
                          ("spliced-like", _spliced_like(rng, 1500)),
                          ("adversarial", _adversarial(rng, 1800)),
                          ("long reads", _long_reads(rng, 400))):
         part = PairBatch.from_pairs(pairs)
         say(f"{scoring} {label}: {part.npairs} pairs, pruning on")
-        gp, sp, kp = _run(buf, part, params, True)
+        gp, sp, kp = _run(buf, part, params, 1)
         say(f"{scoring} {label}: routing {sp}")
+        say(f"{scoring} {label}: whole-tile pruned pass only")
+        gw, sw, kw = _run(buf, part, params, 2)
+        say(f"{scoring} {label}: routing {sw}")
         say(f"{scoring} {label}: pruning off")
-        gu, su, ku = _run(buf, part, params, False)
+        gu, su, ku = _run(buf, part, params, 0)
         _same(gp, gu, f"{label} pruned vs unpruned")
+        _same(gw, gu, f"{label} whole-tile pruned vs unpruned")
         o = oracle.band_batch(part, params, 2, nthreads=8)
         _same(gp, o, f"{label} pruned vs oracle")
-        assert sp["escalated"] == su["escalated"] > 0
+        assert sp["escalated"] == su["escalated"] == sw["escalated"] > 0
         assert sp["full_pruned"] + sp["full_unpruned"] == sp["escalated"]
         assert su["full_pruned"] == 0 and su["full_unpruned"] == su["escalated"]
         assert 12 not in ku
@@ -181,8 +191,13 @@ def test_pruned_full_matrix_equals_oracle_and_unpruned(buf, scoring):
             continue
         assert 12 in kp and 13 in kp and 11 in kp
         if label != "adversarial":
-            # the pass proves nearly every escalation of a clean read
+            # the passes prove nearly every escalation of a clean read, or
+            # hand it to the one-wave-per-pair rerun (the register-window
+            # pass narrows the stripe of a wave whose diagonals spread far)
             assert sp["full_pruned"] >= 0.9 * sp["escalated"], sp
+            assert sw["full_pruned"] >= 0.9 * sw["escalated"], sw
+            # the whole-tile pass itself proves nearly all of them
+            assert sw["full_second"] <= 0.1 * sw["escalated"], sw
         else:
             assert sp["full_second"] > 0, sp  # the one-wave-per-pair rerun was exercised
 
@@ -204,15 +219,23 @@ def test_full_matrix_census(buf):
     for label, pairs in (("mate-like", mate), ("adversarial", _adversarial(rng, 600))):
         part = PairBatch.from_pairs(pairs)
         say(f"census {label}: {part.npairs} pairs")
-        ref_p, _, _ = _run(buf, part, params, True)
+        ref_p, _, _ = _run(buf, part, params, 1)
         buf.set_option(_native.ESP_OPT_FULL_CENSUS, 1)
         try:
-            gp, sp, _ = _run(buf, part, params, True)
-            gu, su, _ = _run(buf, part, params, False)
+            # the whole-tile pass's count (in a small batch the register-
+            # window pass's waves span diagonals far apart, its narrowed
+            # stripes leave more pairs to the one-wave rerun, and the census
+            # counts the rerun's full matrices too)
+            gp, sp, _ = _run(buf, part, params, 2)
+            g1, s1, _ = _run(buf, part, params, 1)
+            gu, su, _ = _run(buf, part, params, 0)
         finally:
             buf.set_option(_native.ESP_OPT_FULL_CENSUS, 0)
         _same(gp, ref_p, f"{label} census on vs off")
+        _same(g1, ref_p, f"{label} census on vs off, register-window pass")
         _same(gu, ref_p, f"{label} census unpruned vs pruned")
+        assert 0 < s1["full_cells_computed"] <= s1["full_lane_cells_issued"]
+        assert s1["full_cells_matrix"] == su["full_cells_matrix"]
         esc = np.nonzero(gp[5] == 1)[0]
         mn = int(sum(int(part.read_len[k]) * int(part.win_len[k]) for k in esc))
         say(f"census {label}: pruned {sp['full_cells_computed']} of {su['full_cells_matrix']} cells, "
