@@ -74,7 +74,8 @@ class CPlanStats(ctypes.Structure):
                 ("escalated", ctypes.c_uint32), ("full_pruned", ctypes.c_uint32),
                 ("full_unpruned", ctypes.c_uint32), ("full_second", ctypes.c_uint32),
                 ("full_cells_computed", ctypes.c_uint64), ("full_lane_cells_issued", ctypes.c_uint64),
-                ("full_cells_matrix", ctypes.c_uint64)]
+                ("full_cells_matrix", ctypes.c_uint64), ("full_small_head", ctypes.c_uint32),
+                ("full_large_head", ctypes.c_uint32)]
 
 
 class CChainBatch(ctypes.Structure):

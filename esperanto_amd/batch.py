@@ -208,7 +208,8 @@ class ContextTimings:
         the unpruned kernel ran and the pairs the pruned pass could not prove
         (rerun one wavefront per pair); with
         the census option on, the full-matrix passes' computed cells, issued
-        lane-cells and the cells of the full matrices they stand for."""
+        lane-cells and the cells of the full matrices they stand for; and the
+        pruned passes' pairs with small heads (at most 3 row groups) and the rest."""
         st = _native.CPlanStats()
         _native.check(_native.lib().esp_last_plan_stats(self.buf.handle, ctypes.byref(st)),
                       self.buf.handle)
@@ -218,7 +219,8 @@ class ContextTimings:
                 "full_second": st.full_second,
                 "full_cells_computed": st.full_cells_computed,
                 "full_lane_cells_issued": st.full_lane_cells_issued,
-                "full_cells_matrix": st.full_cells_matrix}
+                "full_cells_matrix": st.full_cells_matrix,
+                "full_small_head": st.full_small_head, "full_large_head": st.full_large_head}
 
     def last_timings(self):
         """[(kind, half, ms)] of the last batch call: 0 planner, 1 band, 2 strip,

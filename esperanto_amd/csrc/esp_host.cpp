@@ -785,6 +785,13 @@ int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
         // the pairs the one-wave rerun was given: the pruned pass's failures
         // and the sort's hopeless pairs, or every escalation of a long read
         out->full_second = c->last_wave_direct ? esc[1] : fa[1];
+        if (!c->last_wave_direct) {
+            uint32_t c0[2], c1[2];
+            HIP_TRY(c, hipMemcpy(c0, ed->crange0, sizeof(c0), hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(c1, ed->crange1, sizeof(c1), hipMemcpyDeviceToHost));
+            out->full_small_head = c0[1];
+            out->full_large_head = c1[1];
+        }
     }
     if (c->last_census && c->censusbuf.p) {
         unsigned long long cs[3];

@@ -292,6 +292,10 @@ typedef struct {
     uint64_t full_cells_computed;
     uint64_t full_lane_cells_issued;
     uint64_t full_cells_matrix;
+    /* Of the escalations the pruned passes took: pairs with a head of at
+     * most 3 row groups and the rest (the escalation sort's classes). */
+    uint32_t full_small_head;
+    uint32_t full_large_head;
 } esp_plan_stats;
 int esp_last_plan_stats(esp_ctx *ctx, esp_plan_stats *out);
 
