@@ -398,6 +398,123 @@ __device__ __forceinline__ bool mem_after(const uint32_t *x, const uint32_t *y) 
     return x[4] > y[4];
 }
 
+// First N interval of contig ctg ending after p (its index, or the end of
+// the contig's intervals).
+__device__ __forceinline__ uint64_t iv_after(const RefView &R, uint32_t ctg, uint32_t p) {
+    uint64_t lo = R.iv_off[ctg], hi = R.iv_off[ctg + 1];
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (R.iv[2 * mid + 1] <= p) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The extension loops of extend_mems (intron_chain.rs:137-152), 16 bases a
+// step.  A step loads the read's 16 bytes as five dwords and the contig's 16
+// bases as two packed dwords, spells the bases as ASCII through an LDS table
+// (packed byte -> four ASCII bytes) and compares four bytes at a time.  The
+// contig's N intervals bound the extension once per side (an N base never
+// equals a read base), so the bases compared are A, C, G or T and base_eq is
+// the read byte's upper case equal to them.
+constexpr uint32_t MEM_STEP = 16;
+
+struct MemSeq {
+    const uint32_t *rw;  // the read's bytes as the dwords holding them
+    uint32_t o0;         // byte offset of the read's first byte in rw[0]
+    uint32_t rjmax;      // last dword of rw holding a read byte
+    const uint32_t *pk;  // the contig's packed bytes as dwords
+    uint32_t pklast;     // last dword of the contig's region (its length / 4 + 16 bytes)
+    const uint32_t *lut; // LDS: packed byte -> its four bases in ASCII
+};
+
+// Read bytes [q, q + 16) (dwords past the read's last repeat it).
+__device__ __forceinline__ void mem_read16(const MemSeq &M, uint32_t q, uint32_t (&w)[4]) {
+    const uint32_t b = M.o0 + q, j0 = b >> 2;
+    uint32_t d[5];
+#pragma unroll
+    for (uint32_t u = 0; u < 5; u++) d[u] = M.rw[min(j0 + u, M.rjmax)];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], b & 3u);
+}
+
+// Contig bases [p, p + 16) in ASCII.
+__device__ __forceinline__ void mem_ref16(const MemSeq &M, uint32_t p, uint32_t (&w)[4]) {
+    const uint32_t pb = p >> 2;  // packed byte of base p
+    const uint32_t D0 = M.pk[min(pb >> 2, M.pklast)], D1 = M.pk[min((pb >> 2) + 1, M.pklast)];
+    const uint32_t sh = pb & 3u;
+    const uint32_t B = __builtin_amdgcn_alignbyte(D1, D0, sh);  // packed bytes pb .. pb + 3
+    const uint32_t B4 = (D1 >> (8 * sh)) & 0xFFu;                // packed byte pb + 4
+    uint32_t a[5];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) a[k] = M.lut[(B >> (8 * k)) & 0xFFu];
+    a[4] = M.lut[B4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) w[k] = __builtin_amdgcn_alignbyte(a[k + 1], a[k], p & 3u);
+}
+
+// Right: while q < rl, r < win_hi and read[q] equals base r.
+__device__ void mem_extend_right(const RefView &R, const MemSeq &M, uint32_t ctg, uint32_t rl, uint32_t win_hi,
+                                 uint32_t &q, uint32_t &r) {
+    uint32_t stop = win_hi;
+    const uint64_t t = iv_after(R, ctg, r);
+    if (t < R.iv_off[ctg + 1]) stop = min(stop, max(R.iv[2 * t], r));  // an N base at or after r
+    while (q < rl && r < stop) {
+        const uint32_t n = min(min(rl - q, stop - r), MEM_STEP);
+        uint32_t rb[4], cb[4];
+        mem_read16(M, q, rb);
+        mem_ref16(M, r, cb);
+        uint32_t k = n;
+        bool found = false;
+#pragma unroll
+        for (uint32_t w = 0; w < 4; w++) {
+            const uint32_t x = (rb[w] & 0xDFDFDFDFu) ^ cb[w];
+            if (!found && x) {
+                k = min(k, 4 * w + ((uint32_t)__builtin_ctz(x) >> 3));
+                found = true;
+            }
+        }
+        q += k;
+        r += k;
+        if (k < MEM_STEP) break;
+    }
+}
+
+// Left: while q > 0, r > win_lo and read[q - 1] equals base r - 1.
+__device__ void mem_extend_left(const RefView &R, const MemSeq &M, uint32_t ctg, uint32_t win_lo, uint32_t &q,
+                                uint32_t &r) {
+    if (q == 0 || r <= win_lo) return;
+    const uint64_t a0 = R.iv_off[ctg];
+    const uint64_t t = iv_after(R, ctg, r - 1);
+    if (t < R.iv_off[ctg + 1] && R.iv[2 * t] <= r - 1) return;  // base r - 1 is N
+    // bases from the end of the last interval before r - 1 on are not N
+    const uint32_t floor_r = max(win_lo, t > a0 ? R.iv[2 * (t - 1) + 1] : 0u);
+    while (q > 0 && r > floor_r) {
+        // the L bytes before q against the L bases before r, compared from
+        // the top byte down (bytes of the window from L on are masked)
+        const uint32_t L = min(min(q, r), MEM_STEP), n = min(L, r - floor_r);
+        uint32_t rb[4], cb[4];
+        mem_read16(M, q - L, rb);
+        mem_ref16(M, r - L, cb);
+        uint32_t k = n;
+        bool found = false;
+#pragma unroll
+        for (int w = 3; w >= 0; w--) {
+            const uint32_t lo = 4u * (uint32_t)w;
+            const uint32_t mk = L >= lo + 4 ? 0xFFFFFFFFu : (L <= lo ? 0u : (1u << (8 * (L - lo))) - 1u);
+            const uint32_t x = ((rb[w] & 0xDFDFDFDFu) ^ cb[w]) & mk;
+            if (!found && x) {
+                const uint32_t hb = lo + ((31u - (uint32_t)__builtin_clz(x)) >> 3);
+                k = min(k, L - 1 - hb);
+                found = true;
+            }
+        }
+        q -= k;
+        r -= k;
+        if (k < MEM_STEP) break;
+    }
+}
+
 // Frag fields: contig, strand, q_lo, q_hi, r_lo, r_hi, first member, members.
 __device__ uint32_t ic_cov(const uint32_t *f, const uint32_t *mems) {
     uint32_t total = 0, end = 0;
@@ -416,6 +533,15 @@ __device__ uint32_t ic_cov(const uint32_t *f, const uint32_t *mems) {
 // (best_frag_pair :233-277) and the two refinement searches' geometry
 // (refine_donor_end :285-309, refine_acceptor_start :318-342).
 __global__ __launch_bounds__(JBLOCK) void k_ic_frags(IcArgs a) {
+    __shared__ uint32_t lut[256];
+    static_assert(JBLOCK >= 256, "one table entry a thread");
+    if (threadIdx.x < 256) {
+        const uint32_t b = threadIdx.x;
+        uint32_t v = 0;
+        for (int t = 0; t < 4; t++) v |= ((ACGT_ASCII >> (8 * ((b >> (6 - 2 * t)) & 3u))) & 0xFFu) << (8 * t);
+        lut[b] = v;
+    }
+    __syncthreads();
     const uint32_t k = blockIdx.x * JBLOCK + threadIdx.x;
     if (k >= a.n) return;
     const uint64_t a0 = a.anc_off[k], na = a.anc_off[k + 1] - a0;
@@ -451,19 +577,19 @@ __global__ __launch_bounds__(JBLOCK) void k_ic_frags(IcArgs a) {
         const uint32_t win_lo = rpos > rl ? rpos - rl : 0u;
         const uint32_t win_hi = (uint32_t)min((uint64_t)rpos + K + rl, (uint64_t)clen);
         if (win_hi <= win_lo) continue;
+        MemSeq M;
+        M.rw = (const uint32_t *)((uintptr_t)read & ~(uintptr_t)3);
+        M.o0 = (uint32_t)((uintptr_t)read & 3u);
+        M.rjmax = (M.o0 + rl - 1) >> 2;
+        M.pk = (const uint32_t *)(a.R.packed + a.R.pk_off[ctg]);
+        M.pklast = ((((clen + 3u) >> 2) + 16u + 15u) & ~15u) / 4u - 1u;
+        M.lut = lut;
         uint32_t q_lo = qpos, r_lo = rpos;
         // q_lo <= rl, r_lo <= win_hi: an anchor past the read or the contig
         // does not extend (the reference would index out of its slices)
-        while (q_lo > 0 && r_lo > win_lo && q_lo <= rl && r_lo <= win_hi &&
-               ic_base_eq(read[q_lo - 1], base_ascii(a.R, ctg, r_lo - 1))) {
-            q_lo--;
-            r_lo--;
-        }
+        if (q_lo <= rl && r_lo <= win_hi) mem_extend_left(a.R, M, ctg, win_lo, q_lo, r_lo);
         uint32_t q_hi = qpos + K, r_hi = rpos + K;
-        while (q_hi < rl && r_hi < win_hi && ic_base_eq(read[q_hi], base_ascii(a.R, ctg, r_hi))) {
-            q_hi++;
-            r_hi++;
-        }
+        mem_extend_right(a.R, M, ctg, rl, win_hi, q_hi, r_hi);
         if (q_hi > q_lo) {
             uint32_t x[6] = {ctg, an[3] ? 1u : 0u, q_lo, q_hi, r_lo, r_hi};
             // stable insertion into the sorted prefix

@@ -109,12 +109,15 @@ def refine_queries(seed: int, chains, truth, n: int = 600):
     return out
 
 
-def intron_chain_workload(seed: int, nreads: int = 300, read_len: int = 150):
+def intron_chain_workload(seed: int, nreads: int = 300, read_len: int = 150, n_runs: bool = False):
     """Reads over planted two-exon loci with their anchor pools (qpos, rpos,
     contig, minus): exact k-mers on both exons at a stride of 3..7, a few
     noise anchors elsewhere (and on the other strand), plus unspliced reads
-    (one dominant fragment) and reads with a short exon side.  Returns
-    (reference, reads, anchor pools, library quads)."""
+    (one dominant fragment) and reads with a short exon side.  With n_runs,
+    runs of N are then written into the genome next to and inside the exons
+    (the anchors stay as planted, so some seed extensions stop at an N or
+    start beside one) and a fifth of the reads get lower-case stretches.
+    Returns (reference, reads, anchor pools, library quads)."""
     rng = np.random.default_rng(seed)
     span = 9000
     genome = bytearray(np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, span * nreads + 4000)].tobytes())
@@ -154,8 +157,19 @@ def intron_chain_workload(seed: int, nreads: int = 300, read_len: int = 150):
             pool.append((q, int(rng.integers(0, len(genome) - K)), 0, int(rng.integers(0, 2))))
         order = rng.permutation(len(pool))
         pools.append([pool[i] for i in order])
+        if n_runs and rng.random() < 0.2:
+            lb = bytearray(rb)
+            a0 = int(rng.integers(0, read_len - 20))
+            lb[a0:a0 + 20] = bytes(lb[a0:a0 + 20]).lower()
+            rb = bytes(lb)
         reads.append(rb)
         if rng.random() < 0.3:
             quads.append((0, s, e, int(minus)))
+        if n_runs:
+            for _ in range(int(rng.integers(1, 4))):
+                at = int(rng.choice([s - u - 30, s - u, s - 40, e, e + read_len - u - 30]))
+                at = max(0, min(len(genome) - 40, at + int(rng.integers(-8, 9))))
+                ln = int(rng.integers(1, 25))
+                genome[at:at + ln] = b"N" * ln
     ref = Reference.from_sequences([bytes(genome)])
     return ref, reads, pools, quads

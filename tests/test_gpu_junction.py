@@ -241,11 +241,14 @@ IC_FIELDS = ("status", "reason", "genomic_start", "score", "left_start", "jstart
 SIG = ("GtAg", "GcAg", "AtAc", "NonCanonical")
 
 
-@pytest.mark.parametrize("seed", [3, 4])
-def test_intron_chain_batch_matches_oracle(buf, seed):
+@pytest.mark.parametrize("seed,n_runs", [(3, False), (4, False), (5, True)])
+def test_intron_chain_batch_matches_oracle(buf, seed, n_runs):
+    """Every record field, extension and stitched CIGAR against the oracle;
+    with n_runs, N runs in the genome beside and inside the exons and
+    lower-case read stretches (the seed extensions stop at N and fold case)."""
     from esperanto_amd.junction import JunctionLib, intron_chain_batch
     from esperanto_amd.reference import DeviceReference
-    ref, reads, pools, quads = splicegen.intron_chain_workload(seed, 600)
+    ref, reads, pools, quads = splicegen.intron_chain_workload(seed, 600, n_runs=n_runs)
     o = oracle.OracleRef(ref)
     jl = oracle.JLib(quads)
     dref = DeviceReference(ref, buf)
@@ -264,7 +267,7 @@ def test_intron_chain_batch_matches_oracle(buf, seed):
             if w["status"] == 1:
                 assert exts[k] == w["ext"], k
                 placed += 1
-        assert placed > 300, placed
+        assert placed > (250 if n_runs else 300), placed
     finally:
         dlib.close()
         dref.close()
