@@ -188,6 +188,29 @@ def bench_window(args, buf, params):
     return out
 
 
+def bench_strip(args, buf, params):
+    """Band halves the band kernel has no instance for (20, 40, 56): the C3
+    shape (reads U{50..250}, window = read + 2 x band, 1% substitutions with
+    A->G weighted 3x, 0.2% indels of 1-5), every pair through the int32 strip
+    kernel.  GCUPS over the strip launch and its fraction of the int32 peak
+    at 18 ops a cell."""
+    from esperanto_amd.batch import DeviceBatch, synth_pairs
+    hb = synth_pairs(0xE5A00006, args.strip_pairs, 50, 250, [20, 40, 56], sub_rate=0.01, ag_weight=3.0,
+                     indel_rate=0.002, indel_max=5)
+    cells = band_cells(hb)
+    db = DeviceBatch(hb, buf)
+    wall, kms, tims = timed_steps(lambda: db.launch_gotoh(params, 0), db, max(args.steps, 3), 1)
+    st = db.last_plan_stats()
+    db.free()
+    strip_ms = float(np.mean([sum(t[2] for t in s if t[0] == 2) for s in tims]))
+    tops = cells * GOTOH_OPS_PER_CELL / (strip_ms / 1e3) / 1e12
+    out = {"pairs": hb.npairs, "halves": [20, 40, 56], "cells": cells, "strip_pairs": st["strip_pairs"],
+           "ms_per_step": round(wall * 1e3, 3), "strip_kernel_ms": round(strip_ms, 3),
+           "gcups": round(cells / (strip_ms / 1e3) / 1e9, 1), "roofline_frac": round(tops / VALU_PEAK_TOPS, 4)}
+    log(f"strip kernel (halves 20/40/56): {out}")
+    return out
+
+
 def bench_plain(args, buf, params, c1=None):
     """The plain one-launch-per-call form (esp_band, one wavefront per pair)
     on the first C1 pairs: calls/s with each call's copies and sync, then each
@@ -649,6 +672,9 @@ def main() -> int:
     ap.add_argument("--junction-check", type=int, default=2000,
                     help="reads of that leg checked against the oracle and timed on the CPU")
     ap.add_argument("--only-junction", action="store_true", help="run only the junction leg")
+    ap.add_argument("--strip-pairs", type=int, default=200_000,
+                    help="pairs of the strip-kernel leg (band halves without a band-kernel instance; 0 skips it)")
+    ap.add_argument("--only-strip", action="store_true", help="run only the strip-kernel leg")
     ap.add_argument("--only-plain", action="store_true",
                     help="run only the plain one-launch-per-call form on the 10k C1 pairs")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph replay of the Gotoh batch")
@@ -689,6 +715,10 @@ def main() -> int:
         return 0
     if args.only_walk:
         print(json.dumps(bench_walk(args, buf, params)), flush=True)
+        buf.close()
+        return 0
+    if args.only_strip:
+        print(json.dumps(bench_strip(args, buf, params)), flush=True)
         buf.close()
         return 0
     if args.only_plain:
@@ -781,6 +811,9 @@ def main() -> int:
     junction = None
     if args.junction_reads:
         junction = bench_junction(args, buf, params)
+    strip = None
+    if args.strip_pairs:
+        strip = bench_strip(args, buf, params)
     serve = None
     if args.serve_requests:
         serve = bench_serving(args, buf, params, pairs_per_s)
@@ -826,6 +859,7 @@ def main() -> int:
                    "splice_rescue": win,
                    "placement_walks": walks,
                    "junction": junction,
+                   "strip_kernel": strip,
                    "serving": serve},
         "roofline": {"bound": "valu", "achieved": round(achieved_tops, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
