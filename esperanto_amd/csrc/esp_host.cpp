@@ -1,6 +1,7 @@
 // esp_host.cpp -- host runtime behind the C ABI (include/esp_align.h).
 //
-// A context owns one HIP stream (default priority), grow-only device scratch
+// A context owns one HIP stream (default priority; a second one, forked and
+// joined inside a call, carries one of the pruned passes), grow-only device scratch
 // and pinned staging for the single-pair calls.  Batched calls are
 // asynchronous on the context stream; single-pair calls stage the pair,
 // launch one kernel and copy the result back (the plain form).

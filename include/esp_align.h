@@ -78,7 +78,9 @@ typedef struct {
 } esp_touch;
 
 /* Context: one HIP stream plus device scratch; replaces ExtendBuffer.  Not
- * thread-safe; use one context per host thread. */
+ * thread-safe; use one context per host thread.  (extend_hint's pruned
+ * full-matrix reruns fork onto a second stream of the context and join back
+ * before the call's next stage; callers see only the context stream.) */
 typedef struct esp_ctx esp_ctx;
 
 int esp_create(int device, esp_ctx **out);
