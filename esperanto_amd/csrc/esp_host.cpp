@@ -437,10 +437,18 @@ int esph::gotoh_batch_core(esp_ctx *c, const esp_batch *b, const esp_params *p, 
                 g.trace = (uint32_t *)c->ftrace.p + blocks0 * maxw * 64;
                 HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
                 HIP_TRY(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-                HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks1, c->stream2));
-                HIP_TRY(c, hipEventRecord(c->ev_join, c->stream2));
-                HIP_TRY(c, espk::launch_gotoh_pruned(g0, rows, (int)blocks0, c->stream));
-                HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+                // the join is queued whatever fails in between, so that no
+                // work of this call is left on the second stream
+                const hipError_t e1 = espk::launch_gotoh_full(g, rows, true, (int)blocks1, c->stream2);
+                const hipError_t e2 = hipEventRecord(c->ev_join, c->stream2);
+                const hipError_t e3 =
+                    e1 == hipSuccess ? espk::launch_gotoh_pruned(g0, rows, (int)blocks0, c->stream) : hipSuccess;
+                const hipError_t e4 = e2 == hipSuccess ? hipStreamWaitEvent(c->stream, c->ev_join, 0) : hipSuccess;
+                if (e2 != hipSuccess) (void)hipStreamSynchronize(c->stream2);
+                HIP_TRY(c, e1);
+                HIP_TRY(c, e2);
+                HIP_TRY(c, e3);
+                HIP_TRY(c, e4);
             } else {
                 HIP_TRY(c, espk::launch_gotoh_full(g, rows, true, (int)blocks, c->stream));
             }
