@@ -267,6 +267,7 @@ __global__ __launch_bounds__(64) void k_gotoh_full(GotohArgs a) {
             }
             // uniform after the reductions: scalar copies keep the group
             // branches scalar
+            if (lo > hi) lo = hi = 0;  // no active lane: no column runs
             nhead = __builtin_amdgcn_readfirstlane(min(nh, ngroups));
             dmin = __builtin_amdgcn_readfirstlane(lo);
             dmax = __builtin_amdgcn_readfirstlane(hi);
@@ -607,6 +608,7 @@ __global__ __launch_bounds__(64, 3) void k_gotoh_pruned(GotohArgs a) {
         // and the group branches scalar
         const int32_t ngroups = __builtin_amdgcn_readfirstlane((umaxm + 7) >> 3);
         umaxn = __builtin_amdgcn_readfirstlane(umaxn);
+        if (lo > hi) lo = hi = 0;  // no active lane: no column runs
         const int32_t nhead = __builtin_amdgcn_readfirstlane(min(nh, ngroups));
         const int32_t dmin = __builtin_amdgcn_readfirstlane(lo);
         const int32_t dmax = __builtin_amdgcn_readfirstlane(hi);
