@@ -114,6 +114,18 @@ def _adversarial(rng, n):
     return out
 
 
+def _short_reads(rng, n, lo, hi):
+    """Reads of lo..hi nt with 1% substitutions in 900-nt windows, hint = the
+    planted start: the 64- and 128-row tiles of the pruned passes."""
+    out = []
+    for _ in range(n):
+        m = int(rng.integers(lo, hi + 1))
+        w = _rand(rng, 900)
+        p = int(rng.integers(0, 900 - m))
+        out.append((_subst(rng, w[p:p + m], 0.01), w, p, 0))
+    return out
+
+
 def _long_reads(rng, n):
     """200-250-nt reads: the 264-row tile, which has no pruned pass."""
     out = []
@@ -164,6 +176,8 @@ def test_pruned_full_matrix_equals_oracle_and_unpruned(buf, scoring):
     for label, pairs in (("mate-like", _mate_like(rng, 1500)),
                          ("mate-like, shared diagonals", _mate_like(rng, 1500, starts)),
                          ("spliced-like", _spliced_like(rng, 1500)),
+                         ("short reads, 64-row tile", _short_reads(rng, 1200, 40, 64)),
+                         ("short reads, 128-row tile", _short_reads(rng, 1200, 70, 128)),
                          ("adversarial", _adversarial(rng, 1800)),
                          ("long reads", _long_reads(rng, 400))):
         part = PairBatch.from_pairs(pairs)
