@@ -112,8 +112,19 @@ def bench_window(args, buf, params):
     dref = DeviceReference(ref, buf)
     out = {"pairs": n, "reference_bases": int(ref.lens.sum()),
            "reference_device_bytes": dref.device_bytes()}
+    from esperanto_amd import _native
+    from esperanto_amd.batch import tune_prune_shape
     for name, db in (("spliced", DeviceSpliceBatch(sp, dref, buf)),
                      ("mate_rescue", DeviceMateBatch(mt, dref, buf))):
+        # the pruned passes' stripe and head shape, tuned on the pipeline itself
+        # (untimed; results do not depend on it)
+        tune = None
+        if not args.no_tune:
+            def run_once():
+                db.launch(params)
+                db.sync()
+            tune = tune_prune_shape(buf, run_once, db.last_timings, reps=1)
+            log(f"{name}: pruned-pass shape tuned: {tune}")
         w, k, tims = timed_steps(lambda: db.launch(params), db, max(args.steps, 3), 1)
         stages = {}
         for st in tims:
@@ -128,6 +139,7 @@ def bench_window(args, buf, params):
                      "device_ms": round(k, 3), "stages_ms": stages}
         log(f"{name}: {n / w / 1e6:.2f} M pairs/s, {w * 1e3:.2f} ms/step, stages {stages}")
         ps = db.last_plan_stats()
+        out[name]["prune_shape_tune"] = tune
         out[name]["escalated_to_full"] = ps["escalated"]
         out[name]["escalated_pruned"] = ps["full_pruned"]
         out[name]["band_attempts"] = ps["band_pairs"][30]
@@ -138,7 +150,6 @@ def bench_window(args, buf, params):
         esc_ms = sum(stages.get(k, 0.0) for k in ("escalation_sort", "escalated_full",
                                                     "escalated_wave", "escalated_unpruned"))
         if esc_ms and ps["escalated"]:
-            from esperanto_amd import _native
             buf.set_option(_native.ESP_OPT_FULL_CENSUS, 1)
             db.launch(params)
             db.sync()
@@ -168,6 +179,7 @@ def bench_window(args, buf, params):
             rec = db.raw_results()[0]
             out[name]["placed_fraction"] = round(float((rec["placed"] == 1).mean()), 4)
         db.free()
+        buf.set_option(_native.ESP_OPT_FULL_PRUNE_SHAPE, 0)  # the next leg starts from the default
     if not args.no_cpu:
         from oracle import oracle
         threads = min(16, os.cpu_count() or 1)
@@ -681,7 +693,8 @@ def main() -> int:
     ap.add_argument("--only-myers", action="store_true", help="run only the batched Myers leg")
     ap.add_argument("--only-gotoh", action="store_true",
                     help="time only the batched Gotoh workload (profiling runs)")
-    ap.add_argument("--no-tune", action="store_true", help="keep the band grids at the occupancy limit")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the band grids at the occupancy limit and the pruned passes at their default shape")
     ap.add_argument("--band-grid", type=str, default="",
                     help="blocks per CU for halves 16,30,32,48,64 (an earlier autotune's choice; skips tuning)")
     ap.add_argument("--copy-bytes", type=int, default=0,

@@ -35,7 +35,7 @@ EXPORTS = [
     "esp_junctions_upload", "esp_junctions_free",
     "esp_walk_placements", "esp_intron_length_penalty", "esp_fast_lane_batch",
     "esp_engine_myers_batch", "esp_engine_verify_batch", "esp_gotoh_graph_create", "esp_graph_launch",
-    "esp_graph_check", "esp_graph_destroy", "esp_autotune_band", "esp_myers_infix",
+    "esp_graph_check", "esp_graph_destroy", "esp_autotune_band", "esp_autotune_prune", "esp_myers_infix",
     "esp_mate_rescue_graph_create", "esp_spliced_graph_create",
 ]
 
@@ -117,6 +117,12 @@ class CTuneReport(ctypes.Structure):
     _fields_ = [("half", ctypes.c_int32 * 5), ("occupancy", ctypes.c_int32 * 5),
                 ("blocks_per_cu", ctypes.c_int32 * 5), ("ms_default", ctypes.c_double * 5),
                 ("ms_best", ctypes.c_double * 5), ("candidates", ctypes.c_int32)]
+
+
+class CPruneTuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int32), ("half", ctypes.c_int32 * 16), ("pad", ctypes.c_int32 * 16),
+                ("ms", ctypes.c_double * 16), ("chosen_half", ctypes.c_int32), ("chosen_pad", ctypes.c_int32),
+                ("ms_default", ctypes.c_double), ("ms_best", ctypes.c_double)]
 
 
 class CMateBatch(ctypes.Structure):
@@ -238,6 +244,8 @@ def lib():
     L.esp_myers_infix.argtypes = [vp, u8p, sz, u8p, sz, P(ctypes.c_int32)]
     L.esp_autotune_band.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), P(CBatchOut), ctypes.c_int,
                                     P(CTuneReport)]
+    L.esp_autotune_prune.argtypes = [vp, P(CBatch), P(CHostMeta), P(CParams), P(CBatchOut), ctypes.c_int,
+                                     P(CPruneTuneReport)]
     L.esp_graph_check.argtypes = [vp, vp]
     L.esp_graph_destroy.argtypes = [vp, vp]
     L.esp_engine_myers_batch.argtypes = [vp, P(CBatch), P(CHostMeta), ctypes.c_int, vp]

@@ -148,6 +148,36 @@ def workload_c3(npairs: int = 2_000_000) -> PairBatch:
                        ag_weight=3.0, indel_rate=0.002, indel_max=5)
 
 
+# Stripe half-width and head pad (rows) of the pruned full-matrix passes that
+# the tuners try, the default first (esp_autotune_prune tries the same list).
+PRUNE_SHAPES = ((32, 16), (24, 16), (48, 16), (24, 12), (32, 12), (24, 8), (32, 8), (32, 24))
+
+
+def tune_prune_shape(buf: ExtendBuffer, launch, timings, reps: int = 2) -> dict:
+    """esp_autotune_prune for a whole pipeline (mate rescue, spliced DP, ...):
+    runs launch() (which must wait for the device) 1 + reps times at each shape
+    of PRUNE_SHAPES, sums the full-matrix stages (escalation sort, pruned
+    passes, one-wave and unpruned reruns) of timings() (the last call's
+    [(kind, half, ms)]), and keeps the fastest shape in the context when it
+    beats the default by more than 2%.  Results do not depend on the shape.
+    Returns {"chosen": (half, pad), "ms_default", "ms_best", "candidates"}."""
+    cands = []
+    for half, pad in PRUNE_SHAPES:
+        buf.set_option(_native.ESP_OPT_FULL_PRUNE_SHAPE, half | pad << 8)
+        total = 0.0
+        for r in range(reps + 1):
+            launch()
+            if r:
+                total += sum(ms for kind, _, ms in timings() if kind in (7, 11, 12, 13))
+        cands.append((half, pad, round(total / reps, 3)))
+    best = min(range(len(cands)), key=lambda k: cands[k][2])
+    if not cands[0][2] or cands[best][2] > 0.98 * cands[0][2]:
+        best = 0
+    half, pad, _ = cands[best]
+    buf.set_option(_native.ESP_OPT_FULL_PRUNE_SHAPE, half | pad << 8)
+    return {"chosen": (half, pad), "ms_default": cands[0][2], "ms_best": cands[best][2], "candidates": cands}
+
+
 class DeviceArray:
     """Device allocation owned through the library (esp_device_alloc)."""
 
@@ -289,6 +319,22 @@ class DeviceBatch(ContextTimings):
                       self.buf.handle)
         return {int(r.half[k]): (int(r.blocks_per_cu[k]), int(r.occupancy[k]), round(float(r.ms_default[k]), 3),
                                  round(float(r.ms_best[k]), 3)) for k in range(5)}
+
+    def autotune_prune(self, params: Optional[ExtendParams] = None, reps: int = 2) -> dict:
+        """Tunes the pruned full-matrix passes' stripe half-width and head pad on
+        this batch in extend_hint mode (esp_autotune_prune); the context keeps
+        the choice.  Returns {"chosen": (half, pad), "ms_default", "ms_best",
+        "candidates": [(half, pad, ms), ...]}."""
+        p = (params or ExtendParams()).to_c()
+        r = _native.CPruneTuneReport()
+        _native.check(_native.lib().esp_autotune_prune(self.buf.handle, ctypes.byref(self._cb),
+                                                       ctypes.byref(self._meta), ctypes.byref(p),
+                                                       ctypes.byref(self._out), reps, ctypes.byref(r)),
+                      self.buf.handle)
+        return {"chosen": (int(r.chosen_half), int(r.chosen_pad)), "ms_default": round(float(r.ms_default), 3),
+                "ms_best": round(float(r.ms_best), 3),
+                "candidates": [(int(r.half[k]), int(r.pad[k]), round(float(r.ms[k]), 3))
+                               for k in range(int(r.candidates))]}
 
     def capture_gotoh(self, params: Optional[ExtendParams] = None, mode: int = MODE_BAND) -> "GotohGraph":
         """launch_gotoh recorded once as a HIP graph; GotohGraph.launch() replays it."""

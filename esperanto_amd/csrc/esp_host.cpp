@@ -769,6 +769,59 @@ int esp_autotune_band(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta,
     return ESP_OK;
 }
 
+int esp_autotune_prune(esp_ctx *c, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
+                       esp_batch_out *o, int reps, esp_prune_tune_report *rep) {
+    if (!c) return ESP_E_INVAL;
+    if (!b || !o || !rep || b->npairs == 0) return fail(c, ESP_E_INVAL, "probe batch, output and report are required");
+    if (reps < 1 || reps > 64) return fail(c, ESP_E_INVAL, "reps must be 1..64");
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::memset(rep, 0, sizeof *rep);
+    // the default first, then narrower and wider stripes and heads (the
+    // pipeline tuner, esperanto_amd.batch.tune_prune_shape, tries the same)
+    static const int kShapes[][2] = {{espk::ESC_HALF_A, espk::ESC_PAD_A}, {24, 16}, {48, 16}, {24, 12},
+                                     {32, 12}, {24, 8},  {32, 8},  {32, 24}};
+    const int nshape = (int)(sizeof kShapes / sizeof kShapes[0]);
+    const int keep_half = c->prune_half_a, keep_pad = c->prune_pad_a;
+    bool escalated = false;
+    for (int s = 0; s < nshape; s++) {
+        c->prune_half_a = kShapes[s][0];
+        c->prune_pad_a = kShapes[s][1];
+        double sum = 0.0;
+        for (int r = 0; r <= reps; r++) {  // the first run warms up
+            int rc = esp_gotoh_batch(c, b, meta, p, ESP_MODE_HINT, o);
+            if (rc) {
+                c->prune_half_a = keep_half;
+                c->prune_pad_a = keep_pad;
+                return rc;
+            }
+            esp_timing tim[esp_ctx::kMaxEv];
+            size_t nt = 0;
+            rc = esp_last_timings(c, tim, esp_ctx::kMaxEv, &nt);
+            if (rc) return rc;
+            if (r == 0) continue;
+            for (size_t e = 0; e < nt; e++)
+                if (tim[e].kind == 7 || tim[e].kind == 11 || tim[e].kind == 12 || tim[e].kind == 13) {
+                    sum += tim[e].ms;
+                    escalated = true;
+                }
+        }
+        rep->half[s] = kShapes[s][0];
+        rep->pad[s] = kShapes[s][1];
+        rep->ms[s] = sum / reps;
+        rep->candidates++;
+    }
+    rep->ms_default = rep->ms[0];
+    int best = 0;
+    for (int s = 1; s < nshape; s++)
+        if (rep->ms[s] < rep->ms[best]) best = s;
+    // a choice within 2% of the default's time is taken as noise
+    if (!escalated || rep->ms[best] > 0.98 * rep->ms_default) best = 0;
+    rep->chosen_half = c->prune_half_a = kShapes[best][0];
+    rep->chosen_pad = c->prune_pad_a = kShapes[best][1];
+    rep->ms_best = rep->ms[best];
+    return ESP_OK;
+}
+
 int esp_last_plan_stats(esp_ctx *c, esp_plan_stats *out) {
     if (!c || !out) return ESP_E_INVAL;
     std::memset(out, 0, sizeof(*out));

@@ -200,6 +200,26 @@ typedef struct {
 int esp_autotune_band(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
                       esp_batch_out *out, int reps, esp_tune_report *report);
 
+/* Autotuner of the pruned full-matrix passes' tile shape: runs the probe
+ * batch in extend_hint mode (1 + reps times per candidate) at each stripe
+ * half-width and head pad of the candidate list (the default 32/16 first),
+ * times the full-matrix stages (escalation sort, pruned passes, one-wave and
+ * unpruned reruns) with HIP events, and keeps the fastest shape in the
+ * context (as ESP_OPT_FULL_PRUNE_SHAPE would set it) when it beats the
+ * default by more than 2%.  Results do not depend on the shape.  A probe with
+ * no escalation keeps the default.  No counterpart in the reference: it picks
+ * launch shapes, not results. */
+typedef struct {
+    int32_t candidates;
+    int32_t half[16];  /* stripe half-width of each candidate (rows) */
+    int32_t pad[16];   /* head pad of each candidate (rows) */
+    double ms[16];     /* the candidate's full-matrix stages, ms per run */
+    int32_t chosen_half, chosen_pad;
+    double ms_default, ms_best;
+} esp_prune_tune_report;
+int esp_autotune_prune(esp_ctx *ctx, const esp_batch *b, const esp_host_meta *meta, const esp_params *p,
+                       esp_batch_out *out, int reps, esp_prune_tune_report *report);
+
 /* One esp_gotoh_batch call captured as a HIP graph, in any mode (extend_hint's
  * escalations included: the escalation list, its sort and the pruned and
  * unpruned full-matrix passes all run from device counters).  The graph

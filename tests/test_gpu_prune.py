@@ -265,3 +265,30 @@ def test_full_matrix_census(buf):
             assert sp["full_cells_computed"] < mn
         else:
             assert su["full_cells_matrix"] <= mn
+
+
+def test_prune_autotune_changes_shapes_not_results(buf):
+    """esp_autotune_prune times every stripe/head shape of its list on the
+    probe (extend_hint mode) and keeps one; the records, CIGARs and extend_hint
+    paths after tuning equal those before it, and the oracle's."""
+    from esperanto_amd import _native
+    from esperanto_amd.batch import DeviceBatch, PairBatch
+    params = ExtendParams()
+    rng = np.random.default_rng(95)
+    part = PairBatch.from_pairs(_mate_like(rng, 1200))
+    before, st, _ = _run(buf, part, params, 1)
+    db = DeviceBatch(part, buf)
+    try:
+        rep = db.autotune_prune(params, reps=1)
+        say(f"prune autotune: {rep}")
+        assert len(rep["candidates"]) == 8 and rep["candidates"][0][:2] == (32, 16)
+        assert rep["chosen"] in [c[:2] for c in rep["candidates"]]
+        assert all(c[2] > 0 for c in rep["candidates"]) and rep["ms_best"] <= rep["ms_default"]
+        db.launch_gotoh(params, 2)
+        db.sync()
+        after = db.gotoh_results()
+    finally:
+        db.free()
+        buf.set_option(_native.ESP_OPT_FULL_PRUNE_SHAPE, 0)
+    _same(after, before, "tuned vs default shape")
+    _same(after, oracle.band_batch(part, params, 2, nthreads=8), "tuned vs oracle")
