@@ -282,6 +282,45 @@ constexpr uint32_t WT = 64;
 // Dwords of LDS per wave for its round's queries: 64 queries of up to 150
 // bytes, and three blocks of four waves still fit a CU's 160 KB.
 constexpr uint32_t STAGE_DW = 2432;
+// u32x4 loads per lane that cover STAGE_DW dwords over a wave.
+constexpr uint32_t STAGE_IT = (STAGE_DW + 255) / 256;
+static_assert(STAGE_DW % 4 == 0, "the copy writes whole u32x4 groups");
+typedef const __attribute__((address_space(1))) uint32_t *gdw;
+typedef const __attribute__((address_space(1))) u32x4 *gdw4;
+
+// Copies the ndw (<= STAGE_DW) dwords from g0 into the wave's LDS stage, B
+// u32x4 loads per lane in flight at a time: a lane issues a batch's loads
+// before it writes any of them, so the copy waits for memory once per batch
+// rather than once per 256 dwords.  Each load lies in [g0, g0 + 4 ndw): the
+// lane whose group runs past ndw loads its dwords with the index clamped to
+// the last one (the extra LDS dwords, still inside the stage, are never read).
+template <uint32_t B>
+__device__ __forceinline__ void stage_copy(uint32_t *stage, uintptr_t g0, uint32_t ndw, uint32_t lane) {
+    const gdw src = (gdw)g0;
+#pragma unroll
+    for (uint32_t i0 = 0; i0 < STAGE_IT; i0 += B) {
+        u32x4 v[B];
+#pragma unroll
+        for (uint32_t i = 0; i < B; i++) {
+            const uint32_t d = 4 * lane + 256 * (i0 + i);
+            if (d + 4 <= ndw) {
+                v[i] = *(gdw4)(src + d);
+            } else if (d < ndw) {
+                v[i].x = src[d];
+                v[i].y = src[min(d + 1, ndw - 1)];
+                v[i].z = src[min(d + 2, ndw - 1)];
+                v[i].w = src[min(d + 3, ndw - 1)];
+            }
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < B; i++) {
+            const uint32_t d = 4 * lane + 256 * (i0 + i);
+            if (d < ndw) *(u32x4 *)(stage + d) = v[i];
+        }
+    }
+}
+// Loads in flight per lane in each kernel's copy.
+constexpr uint32_t WALK_STAGE_B = STAGE_IT, FAST_STAGE_B = STAGE_IT;
 // sbase: the (wave-uniform) address the wave's LDS copy of its round's
 // queries starts at, 0 when the round's queries were not copied.
 __device__ __forceinline__ void tile_load(const Lane &L, const uint32_t *stage, uintptr_t sbase,
@@ -431,6 +470,9 @@ __device__ __forceinline__ void seg_score(const WalkArgs &a, Lane &L, const uint
     uint32_t run = fresh ? 0u : L.run;
 #pragma unroll
     for (uint32_t t = 0; t < NG; t++) {
+        // a segment shorter than SEG ends its M op, so the run it leaves is
+        // never read again: groups wholly past ncol change nothing that is
+        if (4 * t >= ncol) break;
         const uint32_t cm = first_bytes((int32_t)ncol - 4 * (int32_t)t) & HI;
         uint32_t q4 = L.qlen ? align4(q[t + 1], q[t], qsh) : N4;
         uint32_t r4 = align4(A[t + 1], A[t], rsh);
@@ -536,15 +578,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (ndw <= STAGE_DW) {
                 // every dword copied lies in [g0, g1): the dwords holding the
                 // round's query bytes, which the global path reads too
-                const uint32_t *src = (const uint32_t *)g0;
-                for (uint32_t d = 4 * lane; d < ndw; d += 256) {
-                    if (d + 4 <= ndw) {
-                        const u32x4 v = *(const u32x4 *)(src + d);
-                        stage[d] = v.x, stage[d + 1] = v.y, stage[d + 2] = v.z, stage[d + 3] = v.w;
-                    } else {
-                        for (uint32_t k = d; k < ndw; k++) stage[k] = src[k];
-                    }
-                }
+                stage_copy<WALK_STAGE_B>(stage, g0, ndw, lane);
                 sbase = g0;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -595,14 +629,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 // ------------------------------------------------------------ fast lane
 
-// Columns per fast-lane tile.
-constexpr uint32_t FT = 128;
+// Columns per fast-lane tile.  The tile is unrolled over its columns in two
+// forms (whole and partial): 64 columns keep the kernel's code near 32 KB,
+// inside the instruction cache two CUs share, where 128 columns ran slower
+// for their code size alone.
+constexpr uint32_t FT = 64;
 
 // Kadane state of try_fast_lane (align.rs:2346-2366).
 struct FastKadane {
     int32_t best = 0, cur = 0;
     uint32_t bs = 0, be = 0, cs = 0, cmm = 0, bmm = 0;
 };
+
+// Sixteen ASCII reference bases from x0 on, base by base: N inside an N
+// interval, before the contig start or past its end.  Out of line, so that
+// the tiles unrolled over their columns hold one call each rather than a
+// copy of the interval search per column.
+struct Ref16 {
+    uint32_t w[4];
+};
+__device__ __attribute__((noinline)) Ref16 ref16_slow(int64_t x0, const uint32_t *pk, uint32_t pklast,
+                                                      const uint32_t *iv, uint32_t niv, uint32_t clen) {
+    Ref16 out;
+    for (int t = 0; t < 4; t++) {
+        uint32_t v = 0;
+        for (int c = 0; c < 4; c++) {
+            const int64_t x = x0 + 4 * t + c;
+            uint32_t base = 'N';
+            if (x >= 0 && x < (int64_t)clen)
+                base = ref_base(pk[min((uint32_t)x >> 4, pklast)], iv, niv, clen, (uint32_t)x);
+            v |= base << (8 * c);
+        }
+        out.w[t] = v;
+    }
+    return out;
+}
 
 // One tile of FT columns from registers: query words q (aligned, shifted by
 // qsh), reference words r (dwords from (p0 >> 4), used when fast), else the
@@ -617,6 +678,8 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, FastKadane &K, cons
     const uint32_t bo = fast ? ((uint32_t)p0 >> 2) & 3u : 0u, rsh = fast ? (uint32_t)p0 & 3u : 0u;
 #pragma unroll
     for (uint32_t g = 0; g < FT / 16; g++) {
+        // columns from ncol on leave the Kadane state as it is
+        if (!FULL && 16 * g >= ncol) break;
         // reference: four ASCII words of this 16-column group
         uint32_t r4s[4];
         if (fast) {
@@ -628,18 +691,9 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, FastKadane &K, cons
 #pragma unroll
             for (int t = 0; t < 4; t++) r4s[t] = align4(A[t + 1], A[t], rsh);
         } else {
+            const Ref16 w = ref16_slow(p0 + 16 * g, pk, pklast, iv, niv, clen);
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                uint32_t v = 0;
-                for (int c = 0; c < 4; c++) {
-                    const int64_t x = p0 + 16 * g + 4 * t + c;
-                    uint32_t base = 'N';
-                    if (x >= 0 && x < (int64_t)clen)
-                        base = ref_base(pk[min((uint32_t)x >> 4, pklast)], iv, niv, clen, (uint32_t)x);
-                    v |= base << (8 * c);
-                }
-                r4s[t] = v;
-            }
+            for (int t = 0; t < 4; t++) r4s[t] = w.w[t];
         }
 #pragma unroll
         for (uint32_t t = 0; t < 4; t++) {
@@ -686,7 +740,7 @@ __device__ __forceinline__ void fast_tile(const FastArgs &a, FastKadane &K, cons
 // round's queries (consecutive in the batch) are copied into the wave's LDS
 // with coalesced loads first when they span at most STAGE_DW dwords, so each
 // query byte leaves HBM once instead of once per tile.
-__global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fast_lane(FastArgs a) {
     __shared__ uint32_t lut[256];
     __shared__ __align__(16) uint32_t s_stage[4][STAGE_DW];
     {
@@ -726,15 +780,7 @@ __global__ __launch_bounds__(256) void k_fast_lane(FastArgs a) {
             if (ndw <= STAGE_DW) {
                 // every dword copied lies in [g0, g1): the dwords holding the
                 // round's query bytes, which the global path reads too
-                const uint32_t *src = (const uint32_t *)g0;
-                for (uint32_t d = 4 * lane; d < ndw; d += 256) {
-                    if (d + 4 <= ndw) {
-                        const u32x4 v = *(const u32x4 *)(src + d);
-                        stage[d] = v.x, stage[d + 1] = v.y, stage[d + 2] = v.z, stage[d + 3] = v.w;
-                    } else {
-                        for (uint32_t k = d; k < ndw; k++) stage[k] = src[k];
-                    }
-                }
+                stage_copy<FAST_STAGE_B>(stage, g0, ndw, lane);
                 sbase = g0;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
