@@ -429,19 +429,53 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t *counter, uint32_t b, bool 
     return mine;
 }
 
-__global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
+// Planner blocks: PLAN_BT threads each, every block over one contiguous
+// chunk of the pairs.  Global atomics are the planner's cost (they serialize
+// on their address), so each block reduces in LDS first and then adds once
+// per bin it holds (the histogram) or reserves its share of each bin with one
+// add (the scatter): a few hundred global adds per block instead of one per
+// pair, and one per statistic per block instead of one per thread.
+constexpr int PLAN_BT = 1024;
+
+__device__ __forceinline__ void plan_chunk(uint32_t npairs, uint32_t &p0, uint32_t &p1) {
+    const uint32_t per = (npairs + gridDim.x - 1) / gridDim.x;
+    p0 = min(blockIdx.x * per, npairs);
+    p1 = min(p0 + per, npairs);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        const unsigned long long u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(PLAN_BT) void k_plan_hist(PlanArgs a) {
     __shared__ uint32_t h[PLAN_NCLASS * PLAN_NBIN];
-    __shared__ uint32_t work;
+    __shared__ uint32_t work, bsmax, bemax, benmax, bsnmax;
+    __shared__ unsigned long long bwmax, bewmax;
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x) h[t] = 0;
-    if (threadIdx.x == 0) work = 0;
+    if (threadIdx.x == 0) {
+        work = bsmax = bemax = benmax = bsnmax = 0;
+        bwmax = bewmax = 0;
+    }
     __syncthreads();
     uint32_t smax = 0, emax = 0, enmax = 0, snmax = 0, swork = 0;
-    uint64_t wmax = 0, ewmax = 0;
-    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long wmax = 0, ewmax = 0;
+    uint32_t p0, p1;
+    plan_chunk(a.npairs, p0, p1);
     // The loop bound is uniform per wave, so agg_inc sees whole waves.
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.npairs; base += stride) {
+    for (uint32_t base = p0; base < p1; base += blockDim.x) {
         const uint32_t p = base + threadIdx.x;
-        const bool active = p < a.npairs;
+        const bool active = p < p1;
         uint32_t cls = PLAN_STRIP, b = 0;
         if (active) {
             b = plan_bin(a, p, cls);
@@ -451,7 +485,7 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
                 atomicAdd(&a.plan->sum.bad, 1u);
             if (cls == PLAN_STRIP) {
                 smax = max(smax, m);
-                wmax = max(wmax, sw);
+                wmax = max(wmax, (unsigned long long)sw);
                 if (m >= 1 && a.win_len[p] >= 1) {
                     swork++;
                     snmax = max(snmax, a.win_len[p]);
@@ -460,23 +494,41 @@ __global__ __launch_bounds__(256) void k_plan_hist(PlanArgs a) {
                 // a band attempt that escalates reruns on the full matrix
                 emax = max(emax, m);
                 enmax = max(enmax, a.win_len[p]);
-                ewmax = max(ewmax, strip_trace_words(m, a.win_len[p]));
+                ewmax = max(ewmax, (unsigned long long)strip_trace_words(m, a.win_len[p]));
             }
         }
         (void)agg_inc(h, b, active);
     }
-    if (swork) atomicAdd(&work, swork);
+    // block reductions: one LDS update per wave, one global update per block
+    smax = wave_max_u32(smax);
+    snmax = wave_max_u32(snmax);
+    emax = wave_max_u32(emax);
+    enmax = wave_max_u32(enmax);
+    wmax = wave_max_u64(wmax);
+    ewmax = wave_max_u64(ewmax);
+    for (int o = 32; o; o >>= 1) swork += (uint32_t)__shfl_xor((int)swork, o, 64);
+    if ((threadIdx.x & 63u) == 0) {
+        if (swork) atomicAdd(&work, swork);
+        atomicMax(&bsmax, smax);
+        atomicMax(&bsnmax, snmax);
+        atomicMax(&bemax, emax);
+        atomicMax(&benmax, enmax);
+        atomicMax(&bwmax, wmax);
+        atomicMax(&bewmax, ewmax);
+    }
     __syncthreads();
     for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x)
         if (h[t]) atomicAdd(&a.plan->hist[t], h[t]);
-    if (threadIdx.x == 0 && work) atomicAdd(&a.plan->sum.strip_work, work);
-    atomicMax(&a.plan->sum.strip_maxm, smax);
-    atomicMax(&a.plan->sum.strip_maxn, snmax);
-    atomicMax(&a.plan->sum.strip_maxw, (unsigned long long)wmax);
-    if (a.hint_mode) {
-        atomicMax(&a.plan->sum.esc_maxm, emax);
-        atomicMax(&a.plan->sum.esc_maxn, enmax);
-        atomicMax(&a.plan->sum.esc_maxw, (unsigned long long)ewmax);
+    if (threadIdx.x == 0) {
+        if (work) atomicAdd(&a.plan->sum.strip_work, work);
+        atomicMax(&a.plan->sum.strip_maxm, bsmax);
+        atomicMax(&a.plan->sum.strip_maxn, bsnmax);
+        atomicMax(&a.plan->sum.strip_maxw, bwmax);
+        if (a.hint_mode) {
+            atomicMax(&a.plan->sum.esc_maxm, bemax);
+            atomicMax(&a.plan->sum.esc_maxn, benmax);
+            atomicMax(&a.plan->sum.esc_maxw, bewmax);
+        }
     }
 }
 
@@ -521,14 +573,32 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs a) {
     if (t == 0) a.plan->esc_range[0] = a.npairs;  // escalation list follows the order
 }
 
-__global__ __launch_bounds__(256) void k_plan_scatter(PlanArgs a) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.npairs; base += stride) {
+// Each block counts its chunk per bin in LDS, reserves that many slots of
+// each bin with one global add, and hands them out with LDS adds.  Pairs of
+// one bin share class and read length; their order inside the bin is free.
+__global__ __launch_bounds__(PLAN_BT) void k_plan_scatter(PlanArgs a) {
+    __shared__ uint32_t lh[PLAN_NCLASS * PLAN_NBIN];
+    for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x) lh[t] = 0;
+    __syncthreads();
+    uint32_t p0, p1;
+    plan_chunk(a.npairs, p0, p1);
+    for (uint32_t base = p0; base < p1; base += blockDim.x) {
         const uint32_t p = base + threadIdx.x;
-        const bool active = p < a.npairs;
+        const bool active = p < p1;
         uint32_t cls = PLAN_STRIP, b = 0;
         if (active) b = plan_bin(a, p, cls);
-        const uint32_t pos = agg_inc(a.plan->cursor, b, active);
+        (void)agg_inc(lh, b, active);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < PLAN_NCLASS * PLAN_NBIN; t += blockDim.x)
+        if (lh[t]) lh[t] = atomicAdd(&a.plan->cursor[t], lh[t]);
+    __syncthreads();
+    for (uint32_t base = p0; base < p1; base += blockDim.x) {
+        const uint32_t p = base + threadIdx.x;
+        const bool active = p < p1;
+        uint32_t cls = PLAN_STRIP, b = 0;
+        if (active) b = plan_bin(a, p, cls);
+        const uint32_t pos = agg_inc(lh, b, active);
         if (active) a.order[pos] = p;
     }
 }
@@ -609,10 +679,13 @@ hipError_t launch_gotoh_strip(const GotohArgs &a, int blocks, hipStream_t s) {
 
 hipError_t launch_plan(const PlanArgs &a, int ncu, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_plan, dim3(8), dim3(256), 0, s, a.plan);
-    const int blocks = (int)std::min<uint64_t>((uint64_t)ncu * 4, ((uint64_t)a.npairs + 255) / 256 + 1);
-    hipLaunchKernelGGL(k_plan_hist, dim3(blocks), dim3(256), 0, s, a);
+    // half a block per CU: few enough blocks that the per-block global adds
+    // stay cheap, with 16 waves each to keep the loads in flight
+    const int blocks = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)std::max(ncu / 2, 1), ((uint64_t)a.npairs + PLAN_BT - 1) / PLAN_BT));
+    hipLaunchKernelGGL(k_plan_hist, dim3(blocks), dim3(PLAN_BT), 0, s, a);
     hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, a);
-    hipLaunchKernelGGL(k_plan_scatter, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_plan_scatter, dim3(blocks), dim3(PLAN_BT), 0, s, a);
     return hipGetLastError();
 }
 
