@@ -438,9 +438,9 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t *counter, uint32_t b, bool 
 constexpr int PLAN_BT = 1024;
 
 __device__ __forceinline__ void plan_chunk(uint32_t npairs, uint32_t &p0, uint32_t &p1) {
-    const uint32_t per = (npairs + gridDim.x - 1) / gridDim.x;
-    p0 = min(blockIdx.x * per, npairs);
-    p1 = min(p0 + per, npairs);
+    const uint64_t per = ((uint64_t)npairs + gridDim.x - 1) / gridDim.x;
+    p0 = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)npairs);
+    p1 = (uint32_t)min((uint64_t)p0 + per, (uint64_t)npairs);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
